@@ -1,0 +1,268 @@
+"""Python face of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads ``libpsi_oracle.so`` (the C restatement in ``psi_oracle.c``) with
+ctypes and offers numpy-level wrappers with the reference's names.  Only
+tests, the smoke check and the benchmark's CPU-baseline leg import this
+module; the product package never does.
+
+Every wrapper cites the reference routine it restates (paths relative to
+the pyOM2 source tree).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(HERE, "libpsi_oracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        I, D, V = C.c_int, C.c_double, None
+        L.orc_make_coeff.argtypes = [I, I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_make_coeff.restype = V
+        L.orc_apply_op.argtypes = [I, I, _dp, _dp, _dp]
+        L.orc_apply_op.restype = V
+        L.orc_absmax.argtypes = [I, I, _dp]
+        L.orc_absmax.restype = D
+        L.orc_dot.argtypes = [I, I, _dp, _dp]
+        L.orc_dot.restype = D
+        L.orc_setcyclic.argtypes = [I, I, I, _dp]
+        L.orc_setcyclic.restype = V
+        L.orc_make_inv.argtypes = [I, I, _dp, I, _ip, _ip, _ip, _dp]
+        L.orc_make_inv.restype = V
+        L.orc_congrad.argtypes = [I, I, I, _dp, _dp, _dp, _dp, D, I, C.POINTER(I), C.POINTER(I),
+                                  C.POINTER(D), C.c_void_p, I]
+        L.orc_congrad.restype = I
+        L.orc_line_integral.argtypes = [I, I, I, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp,
+                                        C.POINTER(I)]
+        L.orc_line_integral.restype = D
+        L.orc_psi_velocity.argtypes = [I, I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_psi_velocity.restype = V
+        L.orc_isleperim.argtypes = [I, I, I, _ip, _ip, C.c_void_p, I]
+        L.orc_isleperim.restype = I
+        L.orc_trace_island.argtypes = [I, I, I, _ip, I, _ip, _ip, _ip, _ip]
+        L.orc_trace_island.restype = I
+        L.orc_dense_solve.argtypes = [I, _dp, _dp]
+        L.orc_dense_solve.restype = I
+        _lib = L
+    return _lib
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Islands:
+    """Boundary lists of all land masses (streamfct_init.f90:87-93)."""
+    nisle: int
+    offsets: np.ndarray   # int32, nisle+1
+    bi: np.ndarray        # int32 boundary point i (Fortran index)
+    bj: np.ndarray
+    di: np.ndarray        # int32 line direction
+    dj: np.ndarray
+    allmap: np.ndarray    # int32 (ny+4, nx+4) land-mass labels / perimeters
+
+    def isle(self, k):
+        s, e = self.offsets[k], self.offsets[k + 1]
+        return self.bi[s:e], self.bj[s:e], self.di[s:e], self.dj[s:e]
+
+
+def make_coeff(b, hur=None, hvr=None):
+    """make_coeff_streamfunction (solve_streamfct.f90:123-148) -> cf[9, ny+4, nx+4]
+    with plane (ii+1)+3*(jj+1)."""
+    cf = np.zeros((9,) + b.shape)
+    lib().orc_make_coeff(b.nx, b.ny, _c(b.hur if hur is None else hur),
+                         _c(b.hvr if hvr is None else hvr), _c(b.dxt), _c(b.dxu),
+                         _c(b.dyt), _c(b.dyu), _c(b.cost), _c(b.cosu), cf)
+    return cf
+
+
+def apply_op(nx, ny, cf, p):
+    """apply_op (solve_pressure.f90:254-274)."""
+    res = np.zeros_like(p)
+    lib().orc_apply_op(nx, ny, _c(cf), _c(p), res)
+    return res
+
+
+def dot_sfc(nx, ny, a, b):
+    """dot_sfc (solve_streamfct.f90:348-362)."""
+    return lib().orc_dot(nx, ny, _c(a), _c(b))
+
+
+def absmax_sfc(nx, ny, a):
+    """absmax_sfc (solve_streamfct.f90:331-345)."""
+    return lib().orc_absmax(nx, ny, _c(a))
+
+
+def setcyclic(nx, ny, cyclic_x, a):
+    """setcyclic_xy (parallel_none.f90:188-211), in place."""
+    assert a.flags.c_contiguous and a.dtype == np.float64
+    lib().orc_setcyclic(nx, ny, int(cyclic_x), a)
+    return a
+
+
+def make_inv(nx, ny, cf, isl: Islands):
+    """make_inv_sfc (solve_streamfct.f90:381-422)."""
+    Z = np.zeros(cf.shape[1:])
+    lib().orc_make_inv(nx, ny, _c(cf), isl.nisle, _c(isl.offsets, np.int32),
+                       _c(isl.bi, np.int32), _c(isl.bj, np.int32), Z)
+    return Z
+
+
+@dataclasses.dataclass
+class CGResult:
+    sol: np.ndarray
+    iterations: int
+    converged: bool
+    est_error: float
+    status: int
+    trace: np.ndarray | None = None
+
+
+def congrad(nx, ny, cyclic_x, cf, Z, forc, sol, eps=1e-12, max_iter=1000, trace_cap=0):
+    """congrad_streamfunction (solve_streamfct.f90:153-327); sol is copied."""
+    s = _c(sol).copy()
+    it, cv, est = C.c_int(), C.c_int(), C.c_double()
+    tr = np.zeros((max(trace_cap, 1), 4))
+    st = lib().orc_congrad(nx, ny, int(cyclic_x), _c(cf), _c(Z), _c(forc), s, float(eps),
+                           int(max_iter), C.byref(it), C.byref(cv), C.byref(est),
+                           tr.ctypes.data if trace_cap else None, int(trace_cap))
+    return CGResult(s, it.value, bool(cv.value), est.value, st,
+                    tr[:min(trace_cap, it.value)] if trace_cap else None)
+
+
+def line_integral(b, isl: Islands, k, uloc, vloc):
+    """line_integral (streamfct_init.f90:311-377) for island k (0-based)."""
+    bi, bj, di, dj = isl.isle(k)
+    bad = C.c_int()
+    v = lib().orc_line_integral(b.nx, b.ny, len(bi), _c(bi, np.int32), _c(bj, np.int32),
+                                _c(di, np.int32), _c(dj, np.int32), _c(b.dxu), _c(b.dyu),
+                                _c(b.cost), _c(uloc), _c(vloc), C.byref(bad))
+    if bad.value:
+        raise ValueError("missing line_dir in line integral")
+    return v
+
+
+def psi_velocity(b, psi):
+    """Velocity (fpx, fpy) of a streamfunction field for the island integrals
+    (streamfct_init.f90:294-300; solve_streamfct.f90:68-74)."""
+    fpx = np.zeros(b.shape)
+    fpy = np.zeros(b.shape)
+    lib().orc_psi_velocity(b.nx, b.ny, _c(psi), _c(b.maskU), _c(b.maskV), _c(b.hur),
+                           _c(b.hvr), _c(b.dxt), _c(b.dyt), _c(b.cosu), fpx, fpy)
+    return fpx, fpy
+
+
+def isleperim(kmt, cyclic):
+    """isleperim (island_module.f90:7-127) on an (jmt, imt) int map.
+    Returns (map, nisle)."""
+    jmt, imt = kmt.shape
+    m = np.zeros((jmt, imt), np.int32)
+    n = lib().orc_isleperim(imt, jmt, int(cyclic), _c(kmt, np.int32), m, None, 0)
+    if n < 0:
+        raise MemoryError("isleperim")
+    return m, n
+
+
+def trace_islands(b) -> Islands:
+    """The island part of streamfunction_init (streamfct_init.f90:35-267):
+    label land masses, then walk each one's perimeter."""
+    kmt = np.where(b.kbot > 0, 5, 0).astype(np.int32)
+    kmt[:2, :] = 0
+    kmt[-2:, :] = 0
+    kmt[:, :2] = 0
+    kmt[:, -2:] = 0
+    kmt[2:-2, 2:-2] = np.where(b.kbot[2:-2, 2:-2] > 0, 5, 0)
+    nx = b.nx
+    if b.cyclic_x:
+        kmt[:, nx + 2:nx + 4] = kmt[:, 2:4]
+        kmt[:, 0:2] = kmt[:, nx:nx + 2]
+    allmap, nisle = isleperim(kmt, b.cyclic_x)
+    if b.cyclic_x:
+        allmap[:, nx + 2:nx + 4] = allmap[:, 2:4]
+        allmap[:, 0:2] = allmap[:, nx:nx + 2]
+    offs = [0]
+    BI, BJ, DI, DJ = [], [], [], []
+    for isle in range(1, nisle + 1):
+        k1 = np.where(allmap == isle, 0, 1).astype(np.int32)
+        m1, _ = isleperim(k1, b.cyclic_x)
+        cap = 8 * (b.nx + 4) * (b.ny + 4) + 16
+        bi = np.zeros(cap, np.int32); bj = np.zeros(cap, np.int32)
+        di = np.zeros(cap, np.int32); dj = np.zeros(cap, np.int32)
+        n = lib().orc_trace_island(b.nx, b.ny, int(b.cyclic_x), m1, cap, bi, bj, di, dj)
+        if n < 0:
+            raise RuntimeError(f"island {isle}: perimeter walk failed ({n})")
+        BI.append(bi[:n]); BJ.append(bj[:n]); DI.append(di[:n]); DJ.append(dj[:n])
+        offs.append(offs[-1] + n)
+    cat = (lambda xs: np.concatenate(xs).astype(np.int32) if xs else np.zeros(0, np.int32))
+    return Islands(nisle, np.array(offs, np.int32), cat(BI), cat(BJ), cat(DI), cat(DJ), allmap)
+
+
+def dense_solve(A, b):
+    """DGESV restated (LU, partial pivoting); returns (x, info)."""
+    A = _c(A).copy()
+    x = _c(b).copy()
+    info = lib().orc_dense_solve(A.shape[0], A, x)
+    return x, info
+
+
+@dataclasses.dataclass
+class InitResult:
+    islands: Islands
+    psin: np.ndarray       # (nisle, ny+4, nx+4)
+    line_psin: np.ndarray  # (nisle, nisle), [n, isle]
+    cf: np.ndarray
+    Z: np.ndarray
+    iterations: list
+
+
+def streamfunction_init(b, eps=1e-12, max_iter=1000):
+    """streamfunction_init (streamfct_init.f90:7-304)."""
+    isl = trace_islands(b)
+    cf = make_coeff(b)
+    Z = make_inv(b.nx, b.ny, cf, isl)
+    forc = np.zeros(b.shape)
+    psin = np.zeros((isl.nisle,) + b.shape)
+    its = []
+    for k in range(isl.nisle):
+        p = np.zeros(b.shape)
+        bi, bj, _, _ = isl.isle(k)
+        ok = (bi >= -1) & (bi <= b.nx + 2) & (bj >= -1) & (bj <= b.ny + 2)
+        p[bj[ok] + 1, bi[ok] + 1] = 1.0
+        setcyclic(b.nx, b.ny, b.cyclic_x, p)
+        r = congrad(b.nx, b.ny, b.cyclic_x, cf, Z, forc, p, eps, max_iter)
+        psin[k] = setcyclic(b.nx, b.ny, b.cyclic_x, r.sol)
+        its.append(r.iterations)
+    line_psin = np.zeros((isl.nisle, isl.nisle))
+    for isle in range(isl.nisle):
+        fpx, fpy = psi_velocity(b, psin[isle])
+        for n in range(isl.nisle):
+            line_psin[n, isle] = line_integral(b, isl, n, fpx, fpy)
+    return InitResult(isl, psin, line_psin, cf, Z, its)
+
+
+def true_relres(b, cf, Z, forc, sol):
+    """||forc - A sol||_2 / ||forc||_2 over active points (Z != 0)."""
+    r = forc - apply_op(b.nx, b.ny, cf, sol)
+    m = Z != 0
+    return float(np.sqrt((r[m] ** 2).sum()) / max(np.sqrt((forc[m] ** 2).sum()), 1e-300))

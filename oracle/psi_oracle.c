@@ -1,0 +1,489 @@
+/*
+ * psi_oracle.c -- CPU oracle for the barotropic streamfunction solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain serial C, the
+ * reference algorithm of pyOM2's streamfunction path so that tests, the
+ * smoke check and the CPU-baseline leg of the benchmark can check the HIP
+ * product against it.  Nothing in the product (pyom2_amd/, libpsisolve.so)
+ * links, loads or calls this code.
+ *
+ * Each routine cites the reference file:line it restates (paths relative to
+ * the pyOM2 source tree).  Arithmetic is FP64, evaluated in the reference's
+ * operation order; build with -ffp-contract=off so no FMA is introduced and
+ * results track the Fortran build bit for bit on x86-64.
+ *
+ * Array convention: a 2-D field a(i,j), i in [-1,nx+2], j in [-1,ny+2]
+ * (halo onx=2, column-major with i fastest) is stored at
+ * a[(j+1)*(nx+4) + (i+1)].  1-D metric arrays along x (dxt, dxu) have
+ * nx+4 entries, along y (dyt, dyu, cost, cosu) ny+4 entries, with the same
+ * +1 offset.  Integer maps use the same layout.
+ *
+ * Island boundary lists are flattened: island k (0-based; the reference's
+ * isle = k+1) owns entries off[k] .. off[k+1]-1 of bi/bj (point, Fortran
+ * index) and di/dj (line direction).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IX(i, j, nx) ((size_t)((j) + 1) * (size_t)((nx) + 4) + (size_t)((i) + 1))
+
+/* Fortran default-real literals widened to real*8 (the reference writes
+ * 1.e-10 in a real*8 expression, solve_streamfct.f90:262). */
+static double f32lit(float v) { return (double)v; }
+
+/* ---------------------------------------------------------------------- */
+/* make_coeff_streamfunction  (for_src/external/solve_streamfct.f90:123-148)
+ * cf has 9 planes; plane (ii,jj), ii,jj in {-1,0,1}, is at
+ * cf + ((ii+1) + 3*(jj+1)) * (nx+4)*(ny+4).  Only the interior is set; the
+ * rest stays zero (cf=0 at :134). */
+void orc_make_coeff(int nx, int ny, const double *hur, const double *hvr,
+                    const double *dxt, const double *dxu, const double *dyt,
+                    const double *dyu, const double *cost, const double *cosu,
+                    double *cf)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    memset(cf, 0, 9 * plane * sizeof(double));
+    double *cC = cf + 4 * plane, *cE = cf + 5 * plane, *cW = cf + 3 * plane;
+    double *cN = cf + 7 * plane, *cS = cf + 1 * plane;
+    for (int j = 1; j <= ny; ++j) {
+        for (int i = 1; i <= nx; ++i) {
+            size_t p = IX(i, j, nx);
+            double cu2 = cosu[j + 1] * cosu[j + 1];
+            double t;
+            t = hvr[IX(i + 1, j, nx)] / dxu[i + 1] / dxt[i + 2] / cu2;
+            cC[p] = cC[p] - t;
+            cE[p] = cE[p] + t;
+            t = hvr[IX(i, j, nx)] / dxu[i + 1] / dxt[i + 1] / cu2;
+            cC[p] = cC[p] - t;
+            cW[p] = cW[p] + t;
+            t = hur[IX(i, j + 1, nx)] / dyu[j + 1] / dyt[j + 2] * cost[j + 2] / cosu[j + 1];
+            cC[p] = cC[p] - t;
+            cN[p] = cN[p] + t;
+            t = hur[IX(i, j, nx)] / dyu[j + 1] / dyt[j + 1] * cost[j + 1] / cosu[j + 1];
+            cC[p] = cC[p] - t;
+            cS[p] = cS[p] + t;
+        }
+    }
+}
+
+/* apply_op  (for_src/external/solve_pressure.f90:254-274): res = 0, then the
+ * nine planes are accumulated with jj outermost, ii next, over the interior. */
+void orc_apply_op(int nx, int ny, const double *cf, const double *p1, double *res)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    memset(res, 0, plane * sizeof(double));
+    for (int jj = -1; jj <= 1; ++jj)
+        for (int ii = -1; ii <= 1; ++ii) {
+            const double *c = cf + (size_t)((ii + 1) + 3 * (jj + 1)) * plane;
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i)
+                    res[IX(i, j, nx)] = res[IX(i, j, nx)] + c[IX(i, j, nx)] * p1[IX(i + ii, j + jj, nx)];
+        }
+}
+
+/* absmax_sfc (solve_streamfct.f90:331-345), serial global_max is a no-op
+ * (for_src/parallel/parallel_none.f90:108-114). */
+double orc_absmax(int nx, int ny, const double *p1)
+{
+    double s2 = 0.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            double a = fabs(p1[IX(i, j, nx)]);
+            s2 = (a > s2) ? a : s2;   /* Fortran max(a, s2) */
+        }
+    return s2;
+}
+
+/* dot_sfc (solve_streamfct.f90:348-362), j outer / i inner. */
+double orc_dot(int nx, int ny, const double *p1, const double *p2)
+{
+    double s2 = 0.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            s2 = s2 + p1[IX(i, j, nx)] * p2[IX(i, j, nx)];
+    return s2;
+}
+
+/* setcyclic_xy (parallel_none.f90:188-211), x direction only (cyclic y is
+ * rejected by streamfunction_init, streamfct_init.f90:27-30). */
+void orc_setcyclic(int nx, int ny, int cyclic_x, double *p1)
+{
+    if (!cyclic_x) return;
+    for (int i = 1; i <= 2; ++i)
+        for (int j = -1; j <= ny + 2; ++j) {
+            p1[IX(nx + i, j, nx)] = p1[IX(i, j, nx)];
+            p1[IX(1 - i, j, nx)] = p1[IX(nx - i + 1, j, nx)];
+        }
+}
+
+/* make_inv_sfc (solve_streamfct.f90:381-422). */
+void orc_make_inv(int nx, int ny, const double *cf, int nisle, const int *off,
+                  const int *bi, const int *bj, double *Z)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const double *cC = cf + 4 * plane;
+    memset(Z, 0, plane * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) Z[IX(i, j, nx)] = cC[IX(i, j, nx)];
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            size_t p = IX(i, j, nx);
+            Z[p] = (Z[p] != 0.0) ? 1.0 / Z[p] : 0.0;
+        }
+    for (int k = 0; k < nisle; ++k)
+        for (int n = off[k]; n < off[k + 1]; ++n) {
+            int i = bi[n], j = bj[n];
+            if (i >= -1 && i <= nx + 2 && j >= -1 && j <= ny + 2) Z[IX(i, j, nx)] = 0.0;
+        }
+}
+
+/* inv_op_sfc (solve_streamfct.f90:364-378). */
+static void inv_op(int nx, int ny, const double *Z, const double *res, double *Zres)
+{
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) Zres[IX(i, j, nx)] = Z[IX(i, j, nx)] * res[IX(i, j, nx)];
+}
+
+/* congrad_streamfunction (solve_streamfct.f90:153-327), minus the SAVE'd
+ * coefficient cache: the caller passes cf and Z.  Returns 0 on the normal
+ * end (label 101), 1 on the error end (label 99, max iterations or
+ * divergence), 2 if the error end saw a NaN estimate (where the reference
+ * calls panic_snap and halt_stop).  The iteration trace, if non-NULL, gets
+ * (beta, s_dot_As, alpha, smax) per iteration, up to trace_cap entries. */
+int orc_congrad(int nx, int ny, int cyclic_x, const double *cf, const double *Z,
+                const double *forc, double *sol, double congr_epsilon,
+                int congr_max_iterations, int *iterations, int *converged,
+                double *est_error, double *trace, int trace_cap)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    double *res = calloc(plane, sizeof(double));
+    double *Zres = calloc(plane, sizeof(double));
+    double *ss = calloc(plane, sizeof(double));
+    double *As = calloc(plane, sizeof(double));
+    double estimated_error = 0.0, betakm1, betak, betak_min = 0.0, betaquot;
+    double s_dot_As, smax, alpha, step, step1 = 0.0, convergence_rate, zresmax;
+    int n, normal;
+
+    orc_setcyclic(nx, ny, cyclic_x, sol);                       /* :191-192 */
+    orc_apply_op(nx, ny, cf, sol, res);                         /* :196 */
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) res[IX(i, j, nx)] = forc[IX(i, j, nx)] - res[IX(i, j, nx)];
+
+    n = 0;
+    inv_op(nx, ny, Z, res, Zres);                               /* :207 */
+    zresmax = orc_absmax(nx, ny, Zres);
+    if (100.0 * zresmax < congr_epsilon) {                      /* :210 */
+        estimated_error = 100.0 * zresmax;
+        normal = 1;
+        goto done;
+    }
+    betakm1 = 1.0;
+    memset(ss, 0, plane * sizeof(double));
+    normal = 0;
+    for (n = 1; n <= congr_max_iterations; ++n) {               /* :223 */
+        inv_op(nx, ny, Z, res, Zres);
+        betak = orc_dot(nx, ny, Zres, res);
+        if (n == 1) {
+            betak_min = fabs(betak);
+        } else if (n > 2) {
+            betak_min = (betak_min < fabs(betak)) ? betak_min : fabs(betak);
+            if (fabs(betak) > 100.0 * betak_min) goto done;     /* diverging, :236-238 */
+        }
+        betaquot = betak / betakm1;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i)
+                ss[IX(i, j, nx)] = Zres[IX(i, j, nx)] + betaquot * ss[IX(i, j, nx)];
+        orc_setcyclic(nx, ny, cyclic_x, ss);
+        orc_apply_op(nx, ny, cf, ss, As);
+        s_dot_As = orc_dot(nx, ny, ss, As);
+        if (fabs(s_dot_As) < fabs(betak) * f32lit(1.e-10f)) {  /* :262 */
+            smax = orc_absmax(nx, ny, ss);
+            estimated_error = 100.0 * smax;
+            normal = 1;
+            goto done;
+        }
+        alpha = betak / s_dot_As;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                size_t p = IX(i, j, nx);
+                sol[p] = sol[p] + alpha * ss[p];
+                res[p] = res[p] - alpha * As[p];
+            }
+        smax = orc_absmax(nx, ny, ss);
+        if (trace && n <= trace_cap) {
+            double *t = trace + 4 * (size_t)(n - 1);
+            t[0] = betak; t[1] = s_dot_As; t[2] = alpha; t[3] = smax;
+        }
+        step = fabs(alpha) * smax;                              /* :288 */
+        if (n == 1) {
+            step1 = step;
+            estimated_error = step;
+            if (step < congr_epsilon) { normal = 1; goto done; }
+        } else if (step < congr_epsilon) {
+            convergence_rate = exp(log(step / step1) / (double)(n - 1));
+            estimated_error = step * convergence_rate / (1.0 - convergence_rate);
+            if (estimated_error < congr_epsilon) { normal = 1; goto done; }
+        }
+        betakm1 = betak;
+    }
+    /* max iterations: the do-variable is congr_max_iterations+1 here (:303) */
+done:
+    *iterations = n;
+    *converged = normal;
+    *est_error = estimated_error;
+    free(res); free(Zres); free(ss); free(As);
+    if (normal) return 0;
+    return (estimated_error != estimated_error) ? 2 : 1;
+}
+
+/* line_integral (for_src/external/streamfct_init.f90:311-377) for one
+ * island, given its boundary points and directions. */
+double orc_line_integral(int nx, int ny, int nb, const int *bi, const int *bj,
+                         const int *di, const int *dj, const double *dxu,
+                         const double *dyu, const double *cost, const double *uloc,
+                         const double *vloc, int *bad_dir)
+{
+    double line = 0.0;
+    int is = 0, ie = nx, js = 0, je = ny;   /* is_pe=js_pe=1 shifted to 0 (:325-326) */
+    *bad_dir = 0;
+    for (int n = 0; n < nb; ++n) {
+        int i = bi[n], j = bj[n];
+        if (!(i >= is && i <= ie && j >= js && j <= je)) continue;
+        int a = di[n], b = dj[n];
+        if (a == 1 && b == 0)
+            line = line + vloc[IX(i, j, nx)] * dyu[j + 1] + uloc[IX(i, j + 1, nx)] * dxu[i + 1] * cost[j + 2];
+        else if (a == -1 && b == 0)
+            line = line - vloc[IX(i + 1, j, nx)] * dyu[j + 1] - uloc[IX(i, j, nx)] * dxu[i + 1] * cost[j + 1];
+        else if (a == 0 && b == 1)
+            line = line + vloc[IX(i, j, nx)] * dyu[j + 1] - uloc[IX(i, j, nx)] * dxu[i + 1] * cost[j + 1];
+        else if (a == 0 && b == -1)
+            line = line + uloc[IX(i, j + 1, nx)] * dxu[i + 1] * cost[j + 2] - vloc[IX(i + 1, j, nx)] * dyu[j + 1];
+        else { *bad_dir = 1; return line; }
+    }
+    return line;
+}
+
+/* Velocity of a streamfunction field as used for the island integrals
+ * (streamfct_init.f90:294-300 and solve_streamfct.f90:68-74): fields are
+ * zero, then set for i,j in [0, n+2]. */
+void orc_psi_velocity(int nx, int ny, const double *psi, const double *maskU,
+                      const double *maskV, const double *hur, const double *hvr,
+                      const double *dxt, const double *dyt, const double *cosu,
+                      double *fpx, double *fpy)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    memset(fpx, 0, plane * sizeof(double));
+    memset(fpy, 0, plane * sizeof(double));
+    for (int j = 0; j <= ny + 2; ++j)
+        for (int i = 0; i <= nx + 2; ++i) {
+            size_t p = IX(i, j, nx);
+            fpx[p] = -(maskU[p] * (psi[p] - psi[IX(i, j - 1, nx)]) / dyt[j + 1] * hur[p]);
+            fpy[p] = maskV[p] * (psi[p] - psi[IX(i - 1, j, nx)]) / (cosu[j + 1] * dxt[i + 1]) * hvr[p];
+        }
+}
+
+/* ---------------------------------------------------------------------- */
+/* isleperim / expand (for_src/external/island_module.f90:7-273): flood fill
+ * labelling 8-connected land masses of a (nx+4)x(ny+4) map and their ocean
+ * perimeter cells.  kmt>0 is ocean.  Output map: land mass k -> k (1-based),
+ * its perimeter -> -k, other ocean 0.  The reference's fixed queue/perimeter
+ * limits (maxq, maxipp, mnisle) are not reproduced: this queue grows.
+ * Returns the number of land masses, or -1 on allocation failure.
+ * nippts (optional, capacity cap) receives perimeter counts per mass. */
+typedef struct { int *i, *j; size_t front, back, cap; } queue_t;
+
+static int qpush(queue_t *q, int i, int j)
+{
+    if (q->back == q->cap) {
+        if (q->front > 0) {
+            size_t live = q->back - q->front;
+            memmove(q->i, q->i + q->front, live * sizeof(int));
+            memmove(q->j, q->j + q->front, live * sizeof(int));
+            q->front = 0; q->back = live;
+        }
+        if (q->back == q->cap) {
+            size_t nc = q->cap * 2;
+            int *ni = realloc(q->i, nc * sizeof(int)), *nj;
+            if (!ni) return -1;
+            q->i = ni;
+            nj = realloc(q->j, nc * sizeof(int));
+            if (!nj) return -1;
+            q->j = nj; q->cap = nc;
+        }
+    }
+    q->i[q->back] = i; q->j[q->back] = j; q->back++;
+    return 0;
+}
+
+int orc_isleperim(int imt, int jmt, int cyclic, const int *kmt, int *map,
+                  int *nippts, int nippts_cap)
+{
+    /* 1-based (I,J) in [1,imt]x[1,jmt] map to map[(J-1)*imt + (I-1)] */
+#define M(I, J) map[(size_t)((J) - 1) * (size_t)imt + (size_t)((I) - 1)]
+    const int OFF = -1, LAND = 1, OCEAN = 0;
+    for (int I = 1; I <= imt; ++I)
+        for (int J = 1; J <= jmt; ++J)
+            M(I, J) = (kmt[(size_t)(J - 1) * imt + (I - 1)] > 0) ? OCEAN : LAND;
+    queue_t q = {malloc(1024 * sizeof(int)), malloc(1024 * sizeof(int)), 0, 0, 1024};
+    if (!q.i || !q.j) return -1;
+    int label = 2;
+    int iwest = cyclic ? 2 : 1, ieast = cyclic ? imt - 1 : imt;
+    int counts_len = 0;
+    for (int J = jmt; J >= 1; --J)
+        for (int I = iwest; I <= ieast; ++I) {
+            if (M(I, J) != LAND) continue;
+            int nperim = 0;
+            q.front = q.back = 0;
+            if (qpush(&q, I, J)) return -1;
+            while (q.front < q.back) {          /* expand, island_module.f90:187-271 */
+                int i = q.i[q.front], j = q.j[q.front];
+                q.front++;
+                if (i == OFF || j == OFF) continue;
+                int v = M(i, j);
+                if (v == label || v == -label) continue;
+                if (v == LAND) {
+                    M(i, j) = label;
+                    int jn = (j < jmt) ? j + 1 : OFF;
+                    int js = (j > 1) ? j - 1 : OFF;
+                    int ie, iw;
+                    if (cyclic) {
+                        ie = (i < imt - 1) ? i + 1 : (i + 1) - imt + 2;
+                        iw = (i > 2) ? i - 1 : (i - 1) + imt - 2;
+                    } else {
+                        ie = (i < imt) ? i + 1 : OFF;
+                        iw = (i > 1) ? i - 1 : OFF;
+                    }
+                    if (qpush(&q, i, jn) || qpush(&q, ie, jn) || qpush(&q, ie, j) ||
+                        qpush(&q, ie, js) || qpush(&q, i, js) || qpush(&q, iw, js) ||
+                        qpush(&q, iw, j) || qpush(&q, iw, jn)) return -1;
+                } else if (v == OCEAN || v < 0) {
+                    if (v < 0) continue;         /* perimeter of another mass */
+                    M(i, j) = -label;
+                    nperim++;
+                }
+                /* else: labelled for another mass (reported by the reference) */
+            }
+            if (nippts && counts_len < nippts_cap) nippts[counts_len] = nperim;
+            counts_len++;
+            label++;
+        }
+    int nisle = label - 2;
+    /* relabel (island_module.f90:100-106) */
+    for (int I = iwest; I <= ieast; ++I)
+        for (int J = 1; J <= jmt; ++J) {
+            int v = M(I, J);
+            if (v != 0) M(I, J) = v - ((v > 0) ? 1 : -1);
+        }
+    if (cyclic)
+        for (int J = 1; J <= jmt; ++J) {
+            M(1, J) = M(imt - 1, J);
+            M(imt, J) = M(2, J);
+        }
+    free(q.i); free(q.j);
+    return nisle;
+#undef M
+}
+
+/* Perimeter walk of one island (streamfct_init.f90:103-258).  `map` is the
+ * single-island map (1 land, -1 perimeter, 0 ocean) in the halo layout.
+ * Writes up to cap points; returns the number of points, -1 if no start
+ * point, -2 if the walk lost track, -3 if cap was exceeded. */
+int orc_trace_island(int nx, int ny, int cyclic_x, const int *map, int cap,
+                     int *bi, int *bj, int *di, int *dj)
+{
+#define MP(i, j) map[IX((i), (j), nx)]
+    int n = 0, ij0 = 0, ij1 = 0, d0 = 0, d1 = 0, found = 0;
+    for (int pass = 0; pass < 2 && !found; ++pass) {
+        int ibeg = pass == 0 ? nx / 2 + 1 : nx / 2;
+        int iend = pass == 0 ? nx : 1;
+        int istep = pass == 0 ? 1 : -1;
+        for (int i = ibeg; (istep > 0 ? i <= iend : i >= iend) && !found; i += istep)
+            for (int j = 0; j <= ny; ++j) {
+                if (MP(i, j) == 1 && MP(i, j + 1) == -1) {
+                    ij0 = i; ij1 = j; d0 = 1; d1 = 0;
+                    if (cap < 1) return -3;
+                    bi[0] = ij0 - 1; bj[0] = ij1;
+                    found = 1; break;
+                }
+                if (MP(i, j) == -1 && MP(i, j + 1) == 1) {
+                    ij0 = i - 1; ij1 = j; d0 = -1; d1 = 0;
+                    if (cap < 1) return -3;
+                    bi[0] = ij0 + 1; bj[0] = ij1;
+                    found = 1; break;
+                }
+            }
+    }
+    if (!found) return -1;
+    di[0] = d0; dj[0] = d1;
+    n = 2;
+    if (cap < 2) return -3;
+    bi[1] = ij0; bj[1] = ij1;
+    int cont = 1;
+    while (cont) {
+        int p0 = 0, p1 = 0, r0 = 0, r1 = 0;
+        if (d0 == 0 && d1 == 1)  { p0 = ij0;     p1 = ij1 + 1; r0 = ij0 + 1; r1 = ij1 + 1; }
+        if (d0 == -1 && d1 == 0) { p0 = ij0;     p1 = ij1;     r0 = ij0;     r1 = ij1 + 1; }
+        if (d0 == 0 && d1 == -1) { p0 = ij0 + 1; p1 = ij1;     r0 = ij0;     r1 = ij1; }
+        if (d0 == 1 && d1 == 0)  { p0 = ij0 + 1; p1 = ij1 + 1; r0 = ij0 + 1; r1 = ij1; }
+        if (p0 < -1 || p0 > nx + 2 || p1 < -1 || p1 > ny + 2 ||
+            r0 < -1 || r0 > nx + 2 || r1 < -1 || r1 > ny + 2) return -2;
+        int a = MP(p0, p1), b = MP(r0, r1);
+        if (a == -1 && b == 1) {
+            /* forward */
+        } else if (a == -1 && b == -1) {
+            int t = d0; d0 = d1; d1 = -t;            /* turn right */
+        } else if (a == 1 && (b == 1 || b == -1)) {
+            int t = d0; d0 = -d1; d1 = t;            /* turn left */
+        } else {
+            return -2;
+        }
+        di[n - 1] = d0; dj[n - 1] = d1;
+        ij0 += d0; ij1 += d1;
+        if (bi[0] == ij0 && bj[0] == ij1) cont = 0;
+        if (cyclic_x && d0 == 1 && d1 == 0 && ij0 > nx) ij0 -= nx;
+        if (cyclic_x && d0 == -1 && d1 == 0 && ij0 < 1) ij0 += nx;
+        if (bi[0] == ij0 && bj[0] == ij1) cont = 0;
+        if (cont) {
+            n++;
+            if (n > cap) return -3;
+            bi[n - 1] = ij0; bj[n - 1] = ij1;
+        }
+    }
+    return n;
+#undef MP
+}
+
+/* ---------------------------------------------------------------------- */
+/* DGESV restated (LAPACK, not vendored in the reference; called at
+ * solve_streamfct.f90:81): LU with partial pivoting (row of largest
+ * |a|, first such row on ties), then forward and back substitution.
+ * A is row-major n x n, overwritten; b overwritten with x.  Returns 0, or
+ * k>0 if U(k,k) is exactly zero (LAPACK's INFO convention). */
+int orc_dense_solve(int n, double *A, double *b)
+{
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        double amax = fabs(A[k * n + k]);
+        for (int r = k + 1; r < n; ++r)
+            if (fabs(A[r * n + k]) > amax) { amax = fabs(A[r * n + k]); piv = r; }
+        if (A[piv * n + k] == 0.0) return k + 1;
+        if (piv != k) {
+            for (int c = 0; c < n; ++c) { double t = A[k * n + c]; A[k * n + c] = A[piv * n + c]; A[piv * n + c] = t; }
+            double t = b[k]; b[k] = b[piv]; b[piv] = t;
+        }
+        for (int r = k + 1; r < n; ++r) {
+            double l = A[r * n + k] / A[k * n + k];
+            A[r * n + k] = l;
+            for (int c = k + 1; c < n; ++c) A[r * n + c] -= l * A[k * n + c];
+            b[r] -= l * b[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        double s = b[k];
+        for (int c = k + 1; c < n; ++c) s -= A[k * n + c] * b[c];
+        b[k] = s / A[k * n + k];
+    }
+    return 0;
+}

@@ -1,0 +1,83 @@
+"""Run the reference's own Fortran routines through oracle/_ref/ref_harness
+(TEST INFRASTRUCTURE ONLY; built by ``make -C oracle ref`` from the pyOM2
+sources where they lie).  Used to write golden fixtures and to pin the C
+oracle.  The harness binary exists only where the reference sources were
+present at build time."""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+REF_SRC = "/root/reference/for_src"
+
+
+def available() -> bool:
+    return os.path.exists(HARNESS)
+
+
+def build() -> bool:
+    if not os.path.isdir(REF_SRC):
+        return available()
+    subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    return available()
+
+
+def run(b, forc, sol, eps=1e-12, max_iter=1000, uloc=None, vloc=None):
+    """Run streamfunction_init + congrad_streamfunction + apply_op +
+    line_integral of the reference on basin b.  Returns a dict."""
+    nx, ny = b.nx, b.ny
+    uloc = np.zeros(b.shape) if uloc is None else uloc
+    vloc = np.zeros(b.shape) if vloc is None else vloc
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nx, ny, int(b.cyclic_x), max_iter], np.int32).tofile(f)
+            np.array([eps], np.float64).tofile(f)
+            for a in (b.dxt, b.dxu, b.dyt, b.dyu, b.cost, b.cosu):
+                np.ascontiguousarray(a, np.float64).tofile(f)
+            np.ascontiguousarray(b.kbot, np.int32).tofile(f)
+            for a in (b.hur, b.hvr, b.maskU, b.maskV, forc, sol, uloc, vloc):
+                np.ascontiguousarray(a, np.float64).tofile(f)
+        p = subprocess.run([HARNESS, fin, fout], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=3600)
+        if not os.path.exists(fout):
+            raise RuntimeError("reference harness failed:\n" + p.stdout.decode()[-2000:])
+        raw = open(fout, "rb").read()
+    off = 0
+
+    def ints(n):
+        nonlocal off
+        v = np.frombuffer(raw, np.int32, n, off)
+        off += 4 * n
+        return v.copy()
+
+    def dbl(n):
+        nonlocal off
+        v = np.frombuffer(raw, np.float64, n, off)
+        off += 8 * n
+        return v.copy()
+
+    nisle = int(ints(1)[0])
+    nrb = ints(nisle)
+    bnd = []
+    for k in range(nisle):
+        n = int(nrb[k])
+        bi, bj = ints(n), ints(n)
+        di, dj = ints(n), ints(n)
+        bnd.append((bi, bj, di, dj))
+    iters, conv = ints(2)
+    plane = (ny + 4) * (nx + 4)
+    line_psin = dbl(nisle * nisle).reshape(nisle, nisle).T   # Fortran (n, isle)
+    psin = dbl(plane * nisle).reshape(nisle, ny + 4, nx + 4)
+    sol_out = dbl(plane).reshape(ny + 4, nx + 4)
+    res = dbl(plane).reshape(ny + 4, nx + 4)
+    lines = dbl(nisle)
+    return dict(nisle=nisle, nr_boundary=nrb, boundary=bnd, iterations=int(iters),
+                converged=bool(conv), line_psin=line_psin, psin=psin, sol=sol_out,
+                res=res, lines=lines, log=p.stdout.decode(errors="replace"))
