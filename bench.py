@@ -1,0 +1,176 @@
+"""Benchmark: ms per 2048^2 streamfunction solve to a 1e-10 relative FP64
+residual on one MI355X.
+
+Workload: the 2048^2 spherical 0.05 deg basin with two seeded islands
+(pyom2_amd.basins.spherical_basin, seed 0), a sequence of slowly drifting
+forcings (AR(1), rho = 0.99) solved one after another with the reference's
+warm start dpsi(taup1) = 2 dpsi(tau) - dpsi(taum1).  A step is one solve.
+Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line with the headline path's ms per solve, the other
+paths beside it, the fused sweep kernel's roofline figure and the CPU
+oracle timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per point per fused PCG iteration: x, r (as u = Z r) and p
+# read and written (6 words), hur/hvr-derived coefficients read (2 words),
+# a 1-byte Dirichlet/land mask
+BYTES_PER_PT = {"plain": 6 * 8 + 2 * 8 + 1, "mixed": 6 * 4 + 2 * 4 + 1}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--modes", default="mixed,plain", help="first one is the headline")
+    ap.add_argument("--relres", type=float, default=1e-10)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--check-every", type=int, default=0)
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from pyom2_amd import basins
+    from pyom2_amd.streamfunction import SolveOptions, StreamfunctionSolver, islands_trace
+
+    dev = torch.device("cuda:0")
+    t = time.time()
+    b = basins.spherical_basin(seed=0, n=args.n, n_islands=2)
+    isl = islands_trace(b.kbot)
+    log(f"[bench] basin {b.nx}x{b.ny} nisle={isl.nisle} built in {time.time() - t:.1f}s")
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=dev)
+    nsteps = args.warmup + args.steps
+    forcings = [torch.as_tensor(f, device=dev) for f in
+                basins.ar1_forcings(b, nsteps, rho=0.99, seed=1, kmax=32)]
+    torch.cuda.synchronize()
+    log(f"[bench] {nsteps} forcings resident on the GPU")
+    modes = args.modes.split(",")
+    results = {}
+    for mode in modes:
+        opts = SolveOptions(mode=mode, criterion="relres", relres_target=args.relres,
+                            max_iterations=200000, check_every=args.check_every)
+        dpsi = torch.zeros((3,) + b.shape, dtype=torch.float64, device=dev)
+        taum1, tau, taup1 = 0, 1, 2
+        per = []
+        for k in range(nsteps):
+            if k == args.warmup:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            ts = time.perf_counter()
+            dpsi[taup1] = 2 * dpsi[tau] - dpsi[taum1]
+            its, conv, info = s.congrad_streamfunction(forcings[k], dpsi[taup1], opts)
+            torch.cuda.synchronize()
+            te = time.perf_counter()
+            if not conv:
+                raise SystemExit(f"{mode} solve {k} did not converge: {info}")
+            if k >= args.warmup:
+                per.append(dict(ms=(te - ts) * 1e3, its=its, relres=info["relres_true"],
+                                sweep_ms=info["sweep_ms"], launches=info["sweep_launches"],
+                                outer=info["outer_iterations"]))
+            if args.verbose:
+                log(f"[bench] {mode} step {k}: {(te - ts) * 1e3:.2f} ms its={its} "
+                    f"relres={info['relres_true']:.2e} outer={info['outer_iterations']}")
+            taum1, tau, taup1 = tau, taup1, taum1
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+        its = np.array([p["its"] for p in per])
+        sweep_ms = sum(p["sweep_ms"] for p in per)
+        launches = sum(p["launches"] for p in per)
+        npts = b.nx * b.ny
+        # average duration of one fused sweep launch and the algorithmic bytes
+        # it moves (early-exit launches after convergence move none)
+        avg_launch_us = 1e3 * sweep_ms / max(launches, 1)
+        bytes_per_launch = BYTES_PER_PT[mode if mode in BYTES_PER_PT else "mixed"] * npts * \
+            its.sum() / max(launches, 1)
+        achieved = bytes_per_launch / (avg_launch_us * 1e-6) / 1e9
+        results[mode] = dict(ms_per_solve=1e3 * total / args.steps, iterations_mean=float(its.mean()),
+                             relres_max=max(p["relres"] for p in per),
+                             outer_mean=float(np.mean([p["outer"] for p in per])),
+                             sweep_avg_launch_us=avg_launch_us, sweep_launches=launches,
+                             achieved_GBs=achieved, frac=achieved / HBM_PEAK_GBS,
+                             total_s=total)
+        log(f"[bench] {mode}: {results[mode]}")
+
+    head = modes[0]
+    r = results[head]
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(b, isl, args.cpu_iters, results.get("plain", r)["iterations_mean"])
+    line = {
+        "metric": "ms per 2048^2 psi solve to 1e-10 rel. FP64 residual",
+        "value": round(r["ms_per_solve"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_solve"], 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-refined" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"spherical basin {b.nx}x{b.ny}, 0.05 deg, 2 islands, "
+                               f"AR(1) rho=0.99 forcings, warm-started solves, relres<{args.relres:g}",
+                   "path": head, "nisle": isl.nisle},
+        "paths": {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+                  for m, d in results.items()},
+        "roofline": {"bound": "hbm", "kernel": f"k_cg_sweep<{'float' if head != 'plain' else 'double'}>",
+                     "achieved": round(r["achieved_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["frac"], 4), "traffic": None,
+                     "bytes_per_pt_iter": BYTES_PER_PT.get(head, 33),
+                     "avg_launch_us": round(r["sweep_avg_launch_us"], 3)},
+        "cpu_baseline": cpu,
+    }
+    if "plain" in results and head != "plain":
+        line["speedup_vs_plain_fp64"] = round(results["plain"]["ms_per_solve"] / r["ms_per_solve"], 3)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(b, isl, iters, plain_its):
+    """The CPU oracle (serial C restatement of congrad_streamfunction, one
+    core) on a bounded sample: `iters` PCG iterations of the same 2048^2
+    solve, scaled to the plain path's mean iteration count per solve."""
+    from oracle import oracle as O
+    cf = O.make_coeff(b)
+    Z = O.make_inv(b.nx, b.ny, cf, O.Islands(isl.nisle, isl.offsets, isl.bi, isl.bj, isl.di,
+                                             isl.dj, None))
+    t = time.perf_counter()
+    O.congrad(b.nx, b.ny, False, cf, Z, b.forc, np.zeros(b.shape), 0.0, iters)
+    dt = time.perf_counter() - t
+    per_it = dt / iters
+    return {"value": round(per_it * plain_its * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"{iters} PCG iterations of the {b.nx}x{b.ny} solve in {dt:.1f} s "
+                      f"({per_it * 1e3:.1f} ms/iteration, C restatement -O2, 1 thread), "
+                      f"scaled to {plain_its:.0f} iterations per solve"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+/*
+ * psisolve.h -- C ABI of the MI355X-native barotropic streamfunction solver.
+ *
+ * Drop-in replacement for pyOM2's streamfunction path.  Every entry point
+ * names the reference routine it replaces (paths relative to the pyOM2
+ * source tree).  Conventions:
+ *
+ *  - 2-D fields are caller-owned DEVICE arrays of double (or the stated
+ *    type), C-contiguous with shape (ny+4, nx+4): the element a(i,j) of
+ *    pyOM2's Fortran array a(-1:nx+2,-1:ny+2) (halo onx=2) lives at
+ *    a[(j+1)*(nx+4) + (i+1)].  1-D metric arrays are HOST arrays with
+ *    nx+4 (x) or ny+4 (y) entries in the same offset convention.
+ *  - Every function returns a psi_status; the process never exits.  A
+ *    context's last error text is available from psi_ctx_last_error().
+ *  - `stream` arguments are hipStream_t values passed as void* (NULL is
+ *    the null stream).  A context is bound to one stream and one device;
+ *    contexts are independent of each other (the reference's SAVE'd
+ *    coefficient cache becomes per-context state) but one context must not
+ *    be used from two threads at once.
+ *  - psi_congrad and psi_solve_streamfunction allocate no device memory:
+ *    all scratch is allocated by psi_ctx_create.
+ */
+#ifndef PSISOLVE_H
+#define PSISOLVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum psi_status {
+    PSI_OK = 0,
+    PSI_NOT_CONVERGED = 1,          /* max iterations reached (converged=.false.) */
+    PSI_DIVERGED = 2,               /* divergence test fired (converged=.false.) */
+    PSI_NAN = 3,                    /* NaN estimate: the reference would halt */
+    PSI_SINGULAR_ISLAND_SYSTEM = 4, /* DGESV info != 0: the reference would halt */
+    PSI_BAD_ARG = 5,
+    PSI_HIP_ERROR = 6,
+    PSI_TRACE_FAILED = 7,           /* island perimeter walk lost track / no start */
+    PSI_NO_LEARNED_MODEL = 8
+} psi_status;
+
+typedef enum psi_mode {
+    PSI_MODE_PLAIN = 0,    /* FP64 Jacobi-PCG */
+    PSI_MODE_MIXED = 1,    /* FP32 inner Jacobi-PCG under FP64 iterative refinement */
+    PSI_MODE_LEARNED = 2   /* flexible CG with the learned preconditioner (FP32 inner) */
+} psi_mode;
+
+typedef enum psi_criterion {
+    /* reference semantics: absolute bound congr_epsilon on the extrapolated
+     * max-norm step (solve_streamfct.f90:288-297) */
+    PSI_CRIT_REFERENCE = 0,
+    /* ||forc - A sol||_2 / ||forc||_2 over active points below relres_target,
+     * confirmed on the FP64 true residual */
+    PSI_CRIT_RELRES = 1
+} psi_criterion;
+
+typedef struct psi_grid {
+    int32_t nx, ny;        /* interior points */
+    int32_t cyclic_x;      /* enable_cyclic_x (cyclic y is rejected, as in
+                              streamfct_init.f90:27-30) */
+    const double *dxt, *dxu;               /* host, nx+4 */
+    const double *dyt, *dyu, *cost, *cosu; /* host, ny+4 */
+} psi_grid;
+
+/* Device fields describing the basin.  maskU/maskV (surface level) are only
+ * needed by the island integrals and may be NULL when nisle <= 1. */
+typedef struct psi_fields {
+    const double *hur, *hvr;      /* device (ny+4, nx+4) */
+    const double *maskU, *maskV;  /* device (ny+4, nx+4) or NULL */
+} psi_fields;
+
+/* Island boundary lists (streamfct_init.f90:87-93, 258): island k (0-based,
+ * the reference's isle = k+1) owns entries offsets[k]..offsets[k+1]-1. */
+typedef struct psi_islands {
+    int32_t nisle;
+    const int32_t *offsets;        /* host, nisle+1 */
+    const int32_t *bi, *bj;        /* host, boundary point (Fortran i, j) */
+    const int32_t *di, *dj;        /* host, line direction */
+} psi_islands;
+
+typedef struct psi_opts {
+    int32_t mode;              /* psi_mode */
+    int32_t criterion;         /* psi_criterion */
+    double congr_epsilon;      /* reference: 1e-12 */
+    int32_t max_iterations;    /* congr_max_iterations (per solve, inner iterations) */
+    double relres_target;      /* PSI_CRIT_RELRES target, e.g. 1e-10 */
+    int32_t check_every;       /* iterations between host polls of the device
+                                  convergence flag; <= 0 picks a default */
+    double inner_reduction;    /* mixed/learned: relative residual reduction per
+                                  FP32 inner solve; <= 0 picks a default */
+    int32_t max_outer;         /* mixed/learned: FP64 refinement steps; <= 0 default */
+    int32_t reserved[5];
+} psi_opts;
+
+typedef struct psi_info {
+    int32_t iterations;        /* congr_itts (total inner iterations for mixed) */
+    int32_t converged;         /* logical converged */
+    int32_t status;            /* psi_status of the solve */
+    int32_t outer_iterations;  /* FP64 refinement steps (0 for plain) */
+    double est_error;          /* reference estimated_error */
+    double relres_recursive;   /* recursively updated relative residual */
+    double relres_true;        /* FP64 ||forc - A sol|| / ||forc|| (active points) */
+    double elapsed_ms;         /* device time of the solve (HIP events) */
+    double sweep_ms;           /* device time spanned by the fused PCG sweeps */
+    int32_t sweep_launches;    /* sweep kernels launched (incl. ones that found
+                                  the solve finished and returned at once) */
+    int32_t setup_launches;    /* setup + residual kernels launched */
+} psi_info;
+
+typedef struct psi_ctx psi_ctx;
+
+/* ---- island setup (host) ----------------------------------------------
+ * Replaces the island part of streamfunction_init (streamfct_init.f90:35-267)
+ * with isleperim/expand (island_module.f90:7-273): labels the 8-connected
+ * land masses of kbot (host int32 (ny+4, nx+4), 0 = land) and walks each
+ * perimeter.  Output buffers are caller-owned host arrays: offsets needs
+ * nisle+1 entries (cap_isles), the point/direction arrays cap_points
+ * entries.  When a cap is too small, returns PSI_BAD_ARG with *nisle and
+ * *npoints set to the sizes needed.  No fixed limits (the reference's
+ * maxipp/mnisle/maxq) apply. */
+int psi_islands_trace(int32_t nx, int32_t ny, int32_t cyclic_x, const int32_t *kbot,
+                      int32_t *nisle, int32_t *offsets, int32_t cap_isles,
+                      int32_t *bi, int32_t *bj, int32_t *di, int32_t *dj,
+                      int32_t cap_points, int32_t *npoints);
+
+/* ---- context ------------------------------------------------------------
+ * Builds the cached operator (make_coeff_streamfunction,
+ * solve_streamfct.f90:123-148, cached at :172-180) and the Jacobi inverse
+ * with its Dirichlet set (make_inv_sfc, :381-422), and allocates all
+ * device scratch.  `islands` may be NULL (no Dirichlet points). */
+int psi_ctx_create(psi_ctx **out, const psi_grid *grid, const psi_fields *fields,
+                   const psi_islands *islands, void *stream);
+/* Re-derive the cached coefficients from new hur/hvr (the reference's cache
+ * never invalidates; this is the explicit invalidation). */
+int psi_ctx_update_coeffs(psi_ctx *ctx, const double *hur, const double *hvr);
+int psi_ctx_set_stream(psi_ctx *ctx, void *stream);
+void psi_ctx_destroy(psi_ctx *ctx);
+const char *psi_ctx_last_error(const psi_ctx *ctx);
+int psi_ctx_nisle(const psi_ctx *ctx);
+/* Bytes of device memory the context holds. */
+int64_t psi_ctx_device_bytes(const psi_ctx *ctx);
+
+/* ---- the solve ----------------------------------------------------------
+ * congrad_streamfunction(is_,ie_,js_,je_, forc, iterations, sol, converged)
+ * (solve_streamfct.f90:153-327): solves A sol = forc with the Jacobi
+ * preconditioner; sol is the first guess on entry (its cyclic halo is set
+ * first, :191-192) and the solution on exit.  info receives iterations,
+ * converged and the estimates.  Returns PSI_OK, PSI_NOT_CONVERGED,
+ * PSI_DIVERGED or PSI_NAN (the last three with info->converged = 0). */
+int psi_congrad(psi_ctx *ctx, const psi_opts *opts, const double *forc, double *sol,
+                psi_info *info);
+
+/* ---- building blocks, exposed for the drop-in and for parity tests ----- */
+/* apply_op (solve_pressure.f90:254-274) with the cached operator: res = A p
+ * on the interior, res = 0 on the halo. */
+int psi_apply_op(psi_ctx *ctx, const double *p, double *res);
+/* make_coeff_streamfunction: the 9 coefficient planes cf(:,:,ii+2,jj+2),
+ * plane (ii+1)+3*(jj+1), each (ny+4, nx+4), into cf (device). */
+int psi_make_coeff(psi_ctx *ctx, double *cf);
+/* make_inv_sfc: Z (device (ny+4, nx+4)). */
+int psi_make_inv(psi_ctx *ctx, double *Z);
+/* dot_sfc (solve_streamfct.f90:348-362) and absmax_sfc (:331-345) over the
+ * interior; results to host. */
+int psi_dot(psi_ctx *ctx, const double *a, const double *b, double *out);
+int psi_absmax(psi_ctx *ctx, const double *a, double *out);
+/* setcyclic_xy (parallel_none.f90:188-211), in place; no-op unless cyclic_x. */
+int psi_setcyclic(psi_ctx *ctx, double *a);
+/* line_integral (streamfct_init.f90:311-377) around island isle (1-based). */
+int psi_line_integral(psi_ctx *ctx, int32_t isle, const double *uloc, const double *vloc,
+                      double *line);
+/* Barotropic velocity of a streamfunction for the island integrals
+ * (streamfct_init.f90:294-300, solve_streamfct.f90:68-74). */
+int psi_psi_velocity(psi_ctx *ctx, const double *psi, double *fpx, double *fpy);
+/* FP64 true relative residual ||forc - A sol||_2/||forc||_2 over active
+ * points (the figure psi_info.relres_true reports). */
+int psi_true_relres(psi_ctx *ctx, const double *forc, const double *sol, double *relres);
+
+/* ---- island machinery ---------------------------------------------------
+ * streamfunction_init's solver part (streamfct_init.f90:269-303): for every
+ * island, psin = 1 on its perimeter, congrad with zero forcing; then
+ * line_psin(n, isle) = line integral around n of the velocity of psin(isle).
+ * psin: device, nisle planes of (ny+4, nx+4); line_psin: host nisle*nisle,
+ * row-major [n][isle]; iterations: host nisle entries (may be NULL). */
+int psi_streamfunction_init(psi_ctx *ctx, const psi_opts *opts, double *psin,
+                            double *line_psin, int32_t *iterations);
+
+/* The 2-D body of solve_streamfunction (solve_streamfct.f90:39-94) after the
+ * depth integration: given fpx, fpy (depth-integrated forcing times hur/hvr,
+ * device), it forms forc = curl (:49-53), warm-starts
+ * dpsi(taup1) = 2 dpsi(tau) - dpsi(taum1) (:56), solves (:57), sets the
+ * cyclic halo (:58-59), solves the island system for dpsin(2:nisle, tau)
+ * (:61-88) and steps psi(taup1) with Adams-Bashforth (:90-94).
+ * psi, dpsi: device arrays of 3 time levels, each (ny+4, nx+4), level
+ * index 0..2 (the reference's 1..3).  dpsin: host nisle x 3, row-major
+ * [isle][level].  psin: device nisle planes; line_psin: host [n][isle].
+ * forc_out (device, may be NULL) receives the assembled forcing. */
+int psi_solve_streamfunction(psi_ctx *ctx, const psi_opts *opts, const double *fpx,
+                             const double *fpy, double *psi, double *dpsi,
+                             double *dpsin, const double *psin, const double *line_psin,
+                             int32_t taum1, int32_t tau, int32_t taup1, double dt_mom,
+                             double AB_eps, double *forc_out, psi_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSISOLVE_H */

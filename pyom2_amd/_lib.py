@@ -1,0 +1,121 @@
+"""ctypes binding of libpsisolve.so (declarations in include/psisolve.h).
+
+The library is loaded from the package directory and nowhere else; if it is
+missing the import fails loudly -- there is no CPU fallback."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsisolve.so")
+
+PSI_OK = 0
+PSI_NOT_CONVERGED = 1
+PSI_DIVERGED = 2
+PSI_NAN = 3
+PSI_SINGULAR_ISLAND_SYSTEM = 4
+PSI_BAD_ARG = 5
+PSI_HIP_ERROR = 6
+PSI_TRACE_FAILED = 7
+PSI_NO_LEARNED_MODEL = 8
+STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "DIVERGED", 3: "NAN", 4: "SINGULAR_ISLAND_SYSTEM",
+                5: "BAD_ARG", 6: "HIP_ERROR", 7: "TRACE_FAILED", 8: "NO_LEARNED_MODEL"}
+
+PSI_MODE_PLAIN = 0
+PSI_MODE_MIXED = 1
+PSI_MODE_LEARNED = 2
+PSI_CRIT_REFERENCE = 0
+PSI_CRIT_RELRES = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class PsiGrid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("cyclic_x", C.c_int32),
+                ("dxt", _dp), ("dxu", _dp), ("dyt", _dp), ("dyu", _dp), ("cost", _dp),
+                ("cosu", _dp)]
+
+
+class PsiFields(C.Structure):
+    _fields_ = [("hur", C.c_void_p), ("hvr", C.c_void_p), ("maskU", C.c_void_p),
+                ("maskV", C.c_void_p)]
+
+
+class PsiIslands(C.Structure):
+    _fields_ = [("nisle", C.c_int32), ("offsets", _ip), ("bi", _ip), ("bj", _ip), ("di", _ip),
+                ("dj", _ip)]
+
+
+class PsiOpts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("criterion", C.c_int32), ("congr_epsilon", C.c_double),
+                ("max_iterations", C.c_int32), ("relres_target", C.c_double),
+                ("check_every", C.c_int32), ("inner_reduction", C.c_double),
+                ("max_outer", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class PsiInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("status", C.c_int32),
+                ("outer_iterations", C.c_int32), ("est_error", C.c_double),
+                ("relres_recursive", C.c_double), ("relres_true", C.c_double),
+                ("elapsed_ms", C.c_double), ("sweep_ms", C.c_double),
+                ("sweep_launches", C.c_int32), ("setup_launches", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# (name, restype, argtypes) of every entry point in include/psisolve.h
+SIGNATURES = [
+    ("psi_islands_trace", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip, C.c_int32, _ip,
+                                    _ip, _ip, _ip, C.c_int32, _ip]),
+    ("psi_ctx_create", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PsiGrid), C.POINTER(PsiFields),
+                                 C.POINTER(PsiIslands), C.c_void_p]),
+    ("psi_ctx_update_coeffs", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psi_ctx_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_ctx_destroy", None, [C.c_void_p]),
+    ("psi_ctx_last_error", C.c_char_p, [C.c_void_p]),
+    ("psi_ctx_nisle", C.c_int, [C.c_void_p]),
+    ("psi_ctx_device_bytes", C.c_int64, [C.c_void_p]),
+    ("psi_congrad", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.c_void_p, C.c_void_p,
+                              C.POINTER(PsiInfo)]),
+    ("psi_apply_op", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psi_make_coeff", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_make_inv", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    ("psi_absmax", C.c_int, [C.c_void_p, C.c_void_p, _dp]),
+    ("psi_setcyclic", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_line_integral", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, _dp]),
+    ("psi_psi_velocity", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psi_true_relres", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    ("psi_streamfunction_init", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.c_void_p, _dp, _ip]),
+    ("psi_solve_streamfunction", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, _dp, C.c_void_p, _dp, C.c_int32,
+                                           C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                           C.c_void_p, C.POINTER(PsiInfo)]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libpsisolve.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m pyom2_amd.build` "
+                              "(the HIP extension is required; there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class PsiError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code

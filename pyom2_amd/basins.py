@@ -278,15 +278,15 @@ def beta_plane_basin(seed: int = 0, nx: int = 128, ny: int = 128, dx: float = 20
     grid = make_grid(nx, ny, dx, dx, cyclic_x=cyclic_x)
     xt = np.arange(nx) * dx
     yt = np.arange(ny) * dx
-    for _attempt in range(100):
+    # seamounts must not create land (the basin has no islands unless asked):
+    # redraw up to 20 times, then fill the shallows up to 200 m
+    for _attempt in range(20):
         ht = np.full((ny, nx), 4000.0)
         _gaussians(rng, ht, xt, yt, n_seamounts, 500.0, 2500.0, 50e3, 200e3)
-        ocean = ht >= 200.0
-        if ocean.all():
+        if (ht >= 200.0).all():
             break
-    else:
-        raise RuntimeError("could not draw seamounts without land")
-    ocean[:, :] = True
+    ht = np.maximum(ht, 200.0)
+    ocean = np.ones((ny, nx), bool)
     ocean[0, :] = False
     ocean[-1, :] = False
     if not cyclic_x:
@@ -296,7 +296,8 @@ def beta_plane_basin(seed: int = 0, nx: int = 128, ny: int = 128, dx: float = 20
     if n_islands:
         a_lo = max(2.0, min(nx, ny) / 24)
         a_hi = max(a_lo + 1, min(nx, ny) / 10)
-        placed = _ellipses(rng, ocean, n_islands, a_lo, a_hi)
+        placed = _ellipses(rng, ocean, n_islands, a_lo, a_hi,
+                           margin=max(2, min(6, min(nx, ny) // 10)))
     return _assemble("beta_plane", seed, nx, ny, grid, ht, ocean, rng, n_modes, kmax,
                      1e-10, cyclic_x, placed)
 

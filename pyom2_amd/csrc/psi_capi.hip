@@ -1,0 +1,821 @@
+// psi_capi.hip -- context management, PCG drivers and the C ABI declared in
+// include/psisolve.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/psisolve.h"
+#include "psi_device.h"
+
+// The kernels are compiled in this translation unit.
+#include "psi_kernels.hip"
+
+using namespace psi;
+
+struct psi_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Geom g{};
+    size_t plane = 0;
+    std::vector<double> h_dxt, h_dxu, h_dyt, h_dyu, h_cost, h_cosu;
+    double *dxt = nullptr, *dxu = nullptr, *dyt = nullptr, *dyu = nullptr, *cost = nullptr,
+           *cosu = nullptr;
+    const double *hur = nullptr, *hvr = nullptr, *maskU = nullptr, *maskV = nullptr;
+    // stored operator
+    double *E = nullptr, *N = nullptr, *rx = nullptr, *ry = nullptr, *cC = nullptr;
+    float *E32 = nullptr, *N32 = nullptr, *rx32 = nullptr, *ry32 = nullptr;
+    uint8_t *act = nullptr, *dirichlet = nullptr;
+    // islands
+    int nisle = 0;
+    std::vector<int32_t> offs;
+    int32_t *bi = nullptr, *bj = nullptr, *di = nullptr, *dj = nullptr;
+    // CG scratch
+    double *u64[2] = {nullptr, nullptr}, *p64[2] = {nullptr, nullptr};
+    float *u32[2] = {nullptr, nullptr}, *p32[2] = {nullptr, nullptr}, *e32 = nullptr;
+    double *r64 = nullptr, *tmp64 = nullptr, *tmp64b = nullptr;
+    double *part = nullptr, *result = nullptr;
+    unsigned* counter = nullptr;
+    CgState* st = nullptr;
+    CgState* h_st = nullptr;      // pinned: [0..1] poll slots, [2] init
+    double* h_res = nullptr;      // pinned scalars
+    hipEvent_t ev[2] = {nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
+    hipEvent_t ts0 = nullptr, ts1[2] = {nullptr, nullptr};   // sweep-span timing
+    int64_t bytes = 0;
+    std::vector<void*> allocs;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(psi_ctx* c, int code, const char* fmt, const char* what = nullptr) {
+    char buf[512];
+    snprintf(buf, sizeof buf, fmt, what ? what : "");
+    if (c) c->err = buf;
+    else g_create_err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) return fail((c), PSI_HIP_ERROR, "HIP error: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(psi_ctx* c, T** p, size_t n) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, n * sizeof(T) + 64);
+    if (e != hipSuccess) return fail(c, PSI_HIP_ERROR, "hipMalloc: %s", hipGetErrorString(e));
+    e = hipMemsetAsync(v, 0, n * sizeof(T) + 64, c->stream);
+    if (e != hipSuccess) return fail(c, PSI_HIP_ERROR, "hipMemset: %s", hipGetErrorString(e));
+    c->allocs.push_back(v);
+    c->bytes += (int64_t)(n * sizeof(T) + 64);
+    *p = (T*)v;
+    return PSI_OK;
+}
+
+inline unsigned nblk(size_t n, unsigned t = 256) { return (unsigned)((n + t - 1) / t); }
+
+int check_launch(psi_ctx* c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, PSI_HIP_ERROR, "kernel launch: %s", hipGetErrorString(e));
+    return PSI_OK;
+}
+
+int build_operator(psi_ctx* c) {
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_build_EN, dim3(nblk(c->plane)), dim3(256), 0, c->stream, g, c->hur,
+                       c->hvr, c->dxt, c->dxu, c->dyt, c->dyu, c->cost, c->cosu, c->E, c->N);
+    hipLaunchKernelGGL(k_build_active, dim3(nblk(c->plane)), dim3(256), 0, c->stream, g, c->E,
+                       c->N, c->rx, c->ry, c->dirichlet, c->act, c->cC);
+    hipLaunchKernelGGL(k_convert<float>, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->plane,
+                       c->E, c->E32);
+    hipLaunchKernelGGL(k_convert<float>, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->plane,
+                       c->N, c->N32);
+    return check_launch(c);
+}
+
+struct CgRun {
+    int status = ST_RUNNING;
+    int iterations = 0;
+    double est_error = 0, rr = 0, rr0 = 0, bb = 0;
+    double sweep_ms = 0;
+    long launches = 0;
+};
+
+// Setup + fused sweeps until the device flags completion.  Sweeps are
+// issued in batches of `batch`; the host keeps one batch queued ahead of
+// the one whose state it is inspecting, so the GPU never idles on a poll.
+template <class T, class TB>
+int cg_run(psi_ctx* c, const T* E, const T* N, const T* rx, const T* ry, const TB* b,
+           double bscale, T* x, bool zero_guess, T** u, T** p, const CgState& init, int batch,
+           CgRun* out) {
+    const Geom& g = c->g;
+    c->h_st[2] = init;
+    HIPCHK(c, hipMemcpyAsync(c->st, &c->h_st[2], sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    const dim3 grid(g.ntiles), blk(NT);
+    if (zero_guess)
+        hipLaunchKernelGGL((k_cg_setup<T, TB>), grid, blk, 0, c->stream, g, E, N, rx, ry, c->act, b,
+                           bscale, (const T*)nullptr, x, u[0], p[0], c->part, c->counter, c->st);
+    else
+        hipLaunchKernelGGL((k_cg_setup<T, TB>), grid, blk, 0, c->stream, g, E, N, rx, ry, c->act, b,
+                           bscale, (const T*)x, (T*)nullptr, u[0], p[0], c->part, c->counter, c->st);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    long issued = 0;
+    const long limit = (long)init.max_it + 2;
+    int slot = 0;
+    int queue[2], qn = 0;
+    long issued_at[2] = {0, 0};
+    bool timing_started = false;
+    auto issue = [&]() -> int {
+        if (!timing_started) {
+            HIPCHK(c, hipEventRecord(c->ts0, c->stream));
+            timing_started = true;
+        }
+        for (int k = 0; k < batch && issued < limit; ++k, ++issued) {
+            const int s = (int)(issued & 1);
+            hipLaunchKernelGGL((k_cg_sweep<T>), grid, blk, 0, c->stream, g, E, N, rx, ry, c->act, x,
+                               (const T*)u[s], (const T*)p[s], u[s ^ 1], p[s ^ 1], c->part,
+                               c->counter, c->st);
+        }
+        int r = check_launch(c);
+        if (r) return r;
+        HIPCHK(c, hipEventRecord(c->ts1[slot], c->stream));
+        issued_at[slot] = issued;
+        HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->st, sizeof(CgState), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+        queue[qn++] = slot;
+        slot ^= 1;
+        return PSI_OK;
+    };
+    // first poll right after setup (early exit), then batches
+    HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->st, sizeof(CgState), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+    issued_at[slot] = -1;   // no sweeps behind this poll
+    queue[qn++] = slot;
+    slot ^= 1;
+    CgState fin{};
+    for (;;) {
+        if (qn < 2 && issued < limit) {
+            rc = issue();
+            if (rc) return rc;
+        }
+        const int s = queue[0];
+        queue[0] = queue[1];
+        --qn;
+        HIPCHK(c, hipEventSynchronize(c->ev[s]));
+        if (c->h_st[s].done) {
+            fin = c->h_st[s];
+            if (issued_at[s] > 0) {
+                float ms = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&ms, c->ts0, c->ts1[s]));
+                out->sweep_ms = ms;
+                out->launches = issued_at[s];
+            }
+            break;
+        }
+        if (qn == 0 && issued >= limit) {   // cannot happen: max_it sets done
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipMemcpy(&fin, c->st, sizeof(CgState), hipMemcpyDeviceToHost));
+            break;
+        }
+    }
+    out->status = fin.status;
+    out->iterations = fin.n;
+    out->est_error = fin.est_error;
+    out->rr = fin.rr;
+    out->rr0 = fin.rr0;
+    out->bb = fin.bb;
+    return PSI_OK;
+}
+
+// FP64 residual of (b, x): returns |r|^2, |b|^2 (active points), max|r|, count.
+int residual(psi_ctx* c, const double* b, const double* x, double* r_out, double res[4]) {
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_residual, dim3(g.ntiles), dim3(NT), 0, c->stream, g, c->E, c->N, c->rx,
+                       c->ry, c->act, b, x, r_out, c->part, c->counter, c->result);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_res, c->result, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; ++k) res[k] = c->h_res[k];
+    return PSI_OK;
+}
+
+int status_of(int st) {
+    switch (st) {
+        case ST_CONVERGED: return PSI_OK;
+        case ST_MAXIT: return PSI_NOT_CONVERGED;
+        case ST_DIVERGED: return PSI_DIVERGED;
+        case ST_NAN: return PSI_NAN;
+        default: return PSI_NOT_CONVERGED;
+    }
+}
+
+CgState make_init(const psi_opts* o, int criterion, double rtarget, int max_it, bool rel_initial) {
+    CgState s{};
+    s.eps = o->congr_epsilon;
+    s.rtarget2 = rtarget * rtarget;
+    s.max_it = max_it;
+    s.criterion = criterion;
+    s.rel_to_initial = rel_initial ? 1 : 0;
+    return s;
+}
+
+int solve_plain(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
+    const int batch = o->check_every > 0 ? o->check_every : 32;
+    CgRun run;
+    const int crit = o->criterion == PSI_CRIT_RELRES ? CRIT_RELRES : CRIT_REFERENCE;
+    int total = 0, rc;
+    int restarts = 0;
+    double res[4] = {0, 0, 0, 0};
+    for (;;) {
+        CgState init = make_init(o, crit, o->relres_target, o->max_iterations - total, false);
+        if (init.max_it < 1) init.max_it = 1;
+        rc = cg_run<double, double>(c, c->E, c->N, c->rx, c->ry, forc, 1.0, sol, false, c->u64,
+                                    c->p64, init, batch, &run);
+        if (rc) return rc;
+        total += run.status == ST_MAXIT ? init.max_it : run.iterations;
+        info->sweep_ms += run.sweep_ms;
+        info->sweep_launches += (int32_t)run.launches;
+        info->setup_launches += 2;
+        rc = residual(c, forc, sol, nullptr, res);
+        if (rc) return rc;
+        if (crit != CRIT_RELRES || run.status != ST_CONVERGED) break;
+        const double bb = res[1] > 0 ? res[1] : run.rr0;
+        if (res[0] <= o->relres_target * o->relres_target * bb || restarts >= 3 ||
+            total >= o->max_iterations)
+            break;
+        ++restarts;   // recursive residual drifted from the true one: restart
+    }
+    info->iterations = run.status == ST_MAXIT ? o->max_iterations + 1 : total;
+    info->status = status_of(run.status);
+    info->converged = run.status == ST_CONVERGED;
+    info->outer_iterations = restarts;
+    info->est_error = run.est_error;
+    info->relres_recursive = run.bb > 0 ? sqrt(run.rr / run.bb) : 0.0;
+    info->relres_true = res[1] > 0 ? sqrt(res[0] / res[1]) : sqrt(res[0]);
+    return info->status;
+}
+
+int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
+    const int batch = o->check_every > 0 ? o->check_every : 32;
+    const double inner = o->inner_reduction > 0 ? o->inner_reduction : 1e-5;
+    const int max_outer = o->max_outer > 0 ? o->max_outer : 30;
+    const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
+    double res[4];
+    int rc = residual(c, forc, sol, c->r64, res);
+    if (rc) return rc;
+    const double bb = res[1] > 0 ? res[1] : res[0];
+    int total = 0, outer = 0, last = ST_CONVERGED;
+    double est = 0;
+    while (res[0] > target * target * bb) {
+        if (outer >= max_outer || total >= o->max_iterations) {
+            last = ST_MAXIT;
+            break;
+        }
+        const double rms = sqrt(res[0] / std::max(res[3], 1.0));
+        if (!(rms > 0) || rms != rms) {
+            last = ST_NAN;
+            break;
+        }
+        const double scale = 1.0 / rms;
+        CgState init = make_init(o, CRIT_RELRES, inner, o->max_iterations - total, true);
+        CgRun run;
+        rc = cg_run<float, double>(c, c->E32, c->N32, c->rx32, c->ry32, c->r64, scale, c->e32,
+                                   true, c->u32, c->p32, init, batch, &run);
+        if (rc) return rc;
+        total += run.status == ST_MAXIT ? init.max_it : run.iterations;
+        info->sweep_ms += run.sweep_ms;
+        info->sweep_launches += (int32_t)run.launches;
+        info->setup_launches += 3;
+        est = run.est_error;
+        if (run.status == ST_NAN || run.status == ST_DIVERGED) {
+            last = run.status;
+            break;
+        }
+        hipLaunchKernelGGL((k_add_correction<float>), dim3(nblk(c->plane)), dim3(256), 0, c->stream,
+                           c->g, sol, c->e32, rms);
+        rc = check_launch(c);
+        if (rc) return rc;
+        rc = residual(c, forc, sol, c->r64, res);
+        if (rc) return rc;
+        ++outer;
+    }
+    info->iterations = total;
+    info->outer_iterations = outer;
+    info->status = status_of(last);
+    info->converged = last == ST_CONVERGED;
+    info->est_error = est;
+    info->relres_true = bb > 0 ? sqrt(res[0] / bb) : 0.0;
+    info->relres_recursive = info->relres_true;
+    return info->status;
+}
+
+bool valid_field_ctx(psi_ctx* c) { return c && c->E; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* psi_ctx_last_error(const psi_ctx* c) {
+    return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+int psi_ctx_nisle(const psi_ctx* c) { return c ? c->nisle : 0; }
+int64_t psi_ctx_device_bytes(const psi_ctx* c) { return c ? c->bytes : 0; }
+
+void psi_ctx_destroy(psi_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    else (void)hipDeviceSynchronize();
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->h_st) (void)hipHostFree(c->h_st);
+    if (c->h_res) (void)hipHostFree(c->h_res);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->t0) (void)hipEventDestroy(c->t0);
+    if (c->t1) (void)hipEventDestroy(c->t1);
+    if (c->ts0) (void)hipEventDestroy(c->ts0);
+    for (auto& e : c->ts1)
+        if (e) (void)hipEventDestroy(e);
+    delete c;
+}
+
+int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
+                   const psi_islands* isl, void* stream) {
+    if (!out || !grid || !f || !f->hur || !f->hvr) return fail(nullptr, PSI_BAD_ARG, "null argument%s");
+    *out = nullptr;
+    if (grid->nx < 2 || grid->ny < 2 || !grid->dxt || !grid->dxu || !grid->dyt || !grid->dyu ||
+        !grid->cost || !grid->cosu)
+        return fail(nullptr, PSI_BAD_ARG, "bad grid%s");
+    psi_ctx* c = new psi_ctx();
+    c->stream = (hipStream_t)stream;
+    (void)hipGetDevice(&c->device);
+    Geom& g = c->g;
+    g.nx = grid->nx;
+    g.ny = grid->ny;
+    g.pitch = grid->nx + 4;
+    g.cyclic = grid->cyclic_x ? 1 : 0;
+    g.tiles_x = (g.nx + TX - 1) / TX;
+    g.tiles_y = (g.ny + TY - 1) / TY;
+    g.ntiles = g.tiles_x * g.tiles_y;
+    c->plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const size_t nxh = g.nx + 4, nyh = g.ny + 4;
+    c->h_dxt.assign(grid->dxt, grid->dxt + nxh);
+    c->h_dxu.assign(grid->dxu, grid->dxu + nxh);
+    c->h_dyt.assign(grid->dyt, grid->dyt + nyh);
+    c->h_dyu.assign(grid->dyu, grid->dyu + nyh);
+    c->h_cost.assign(grid->cost, grid->cost + nyh);
+    c->h_cosu.assign(grid->cosu, grid->cosu + nyh);
+    c->hur = f->hur;
+    c->hvr = f->hvr;
+    c->maskU = f->maskU;
+    c->maskV = f->maskV;
+    int rc = PSI_OK;
+#define TRY(x)                 \
+    do {                       \
+        rc = (x);              \
+        if (rc) goto failed;   \
+    } while (0)
+    {
+        TRY(dalloc(c, &c->dxt, nxh));
+        TRY(dalloc(c, &c->dxu, nxh));
+        TRY(dalloc(c, &c->dyt, nyh));
+        TRY(dalloc(c, &c->dyu, nyh));
+        TRY(dalloc(c, &c->cost, nyh));
+        TRY(dalloc(c, &c->cosu, nyh));
+        TRY(dalloc(c, &c->rx, nxh));
+        TRY(dalloc(c, &c->ry, nyh));
+        TRY(dalloc(c, &c->rx32, nxh));
+        TRY(dalloc(c, &c->ry32, nyh));
+        TRY(dalloc(c, &c->E, c->plane));
+        TRY(dalloc(c, &c->N, c->plane));
+        TRY(dalloc(c, &c->cC, c->plane));
+        TRY(dalloc(c, &c->E32, c->plane));
+        TRY(dalloc(c, &c->N32, c->plane));
+        TRY(dalloc(c, &c->act, c->plane));
+        TRY(dalloc(c, &c->dirichlet, c->plane));
+        for (int k = 0; k < 2; ++k) {
+            TRY(dalloc(c, &c->u64[k], c->plane));
+            TRY(dalloc(c, &c->p64[k], c->plane));
+            TRY(dalloc(c, &c->u32[k], c->plane));
+            TRY(dalloc(c, &c->p32[k], c->plane));
+        }
+        TRY(dalloc(c, &c->e32, c->plane));
+        TRY(dalloc(c, &c->r64, c->plane));
+        TRY(dalloc(c, &c->tmp64, c->plane));
+        TRY(dalloc(c, &c->tmp64b, c->plane));
+        TRY(dalloc(c, &c->part, (size_t)NRED * (g.ntiles + 1024)));
+        TRY(dalloc(c, &c->result, 8));
+        TRY(dalloc(c, &c->counter, 4));
+        TRY(dalloc(c, &c->st, 1));
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(c, PSI_HIP_ERROR, "allocation failed%s");
+            goto failed;
+        }
+        if (hipHostMalloc((void**)&c->h_st, 4 * sizeof(CgState), 0) != hipSuccess ||
+            hipHostMalloc((void**)&c->h_res, 8 * sizeof(double), 0) != hipSuccess) {
+            rc = fail(c, PSI_HIP_ERROR, "hipHostMalloc failed%s");
+            goto failed;
+        }
+        memset(c->h_st, 0, 4 * sizeof(CgState));
+        for (auto& e : c->ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                rc = fail(c, PSI_HIP_ERROR, "event create failed%s");
+                goto failed;
+            }
+        if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess ||
+            hipEventCreate(&c->ts0) != hipSuccess || hipEventCreate(&c->ts1[0]) != hipSuccess ||
+            hipEventCreate(&c->ts1[1]) != hipSuccess) {
+            rc = fail(c, PSI_HIP_ERROR, "event create failed%s");
+            goto failed;
+        }
+        // 1-D metric ratios: W(i,j) = E(i-1,j) * dxu(i-1)/dxu(i);
+        // S(i,j) = N(i,j-1) * dyu(j-1) cosu(j-1) / (dyu(j) cosu(j)).
+        std::vector<double> rx(nxh, 0.0), ry(nyh, 0.0);
+        std::vector<float> rxf(nxh), ryf(nyh);
+        for (size_t k = 1; k < nxh; ++k) rx[k] = c->h_dxu[k - 1] / c->h_dxu[k];
+        for (size_t k = 1; k < nyh; ++k)
+            ry[k] = c->h_dyu[k - 1] * c->h_cosu[k - 1] / (c->h_dyu[k] * c->h_cosu[k]);
+        for (size_t k = 0; k < nxh; ++k) rxf[k] = (float)rx[k];
+        for (size_t k = 0; k < nyh; ++k) ryf[k] = (float)ry[k];
+        if (hipMemcpy(c->dxt, grid->dxt, nxh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->dxu, grid->dxu, nxh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->dyt, grid->dyt, nyh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->dyu, grid->dyu, nyh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->cost, grid->cost, nyh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->cosu, grid->cosu, nyh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->rx, rx.data(), nxh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->ry, ry.data(), nyh * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->rx32, rxf.data(), nxh * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->ry32, ryf.data(), nyh * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(c, PSI_HIP_ERROR, "metric upload failed%s");
+            goto failed;
+        }
+        // islands
+        if (isl && isl->nisle > 0) {
+            if (!isl->offsets || !isl->bi || !isl->bj || !isl->di || !isl->dj) {
+                rc = fail(c, PSI_BAD_ARG, "island lists incomplete%s");
+                goto failed;
+            }
+            c->nisle = isl->nisle;
+            c->offs.assign(isl->offsets, isl->offsets + isl->nisle + 1);
+            const int np = c->offs.back();
+            if (np > 0) {
+                TRY(dalloc(c, &c->bi, np));
+                TRY(dalloc(c, &c->bj, np));
+                TRY(dalloc(c, &c->di, np));
+                TRY(dalloc(c, &c->dj, np));
+                if (hipMemcpy(c->bi, isl->bi, np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(c->bj, isl->bj, np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(c->di, isl->di, np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(c->dj, isl->dj, np * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                    rc = fail(c, PSI_HIP_ERROR, "island upload failed%s");
+                    goto failed;
+                }
+                hipLaunchKernelGGL(k_mark_points, dim3(nblk(np)), dim3(256), 0, c->stream, g, np,
+                                   c->bi, c->bj, c->dirichlet);
+                TRY(check_launch(c));
+            }
+        }
+        TRY(build_operator(c));
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(c, PSI_HIP_ERROR, "setup failed%s");
+            goto failed;
+        }
+        *out = c;
+        return PSI_OK;
+    }
+failed:
+    g_create_err = c->err;
+    psi_ctx_destroy(c);
+    return rc;
+#undef TRY
+}
+
+int psi_ctx_update_coeffs(psi_ctx* c, const double* hur, const double* hvr) {
+    if (!valid_field_ctx(c) || !hur || !hvr) return PSI_BAD_ARG;
+    c->hur = hur;
+    c->hvr = hvr;
+    int rc = build_operator(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+int psi_ctx_set_stream(psi_ctx* c, void* stream) {
+    if (!c) return PSI_BAD_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->stream = (hipStream_t)stream;
+    return PSI_OK;
+}
+
+int psi_congrad(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
+    if (!valid_field_ctx(c) || !o || !forc || !sol || !info) return PSI_BAD_ARG;
+    if (o->max_iterations < 1) return fail(c, PSI_BAD_ARG, "max_iterations < 1%s");
+    memset(info, 0, sizeof *info);
+    HIPCHK(c, hipEventRecord(c->t0, c->stream));
+    if (c->g.cyclic)
+        hipLaunchKernelGGL(k_setcyclic, dim3(nblk(c->g.ny + 4)), dim3(256), 0, c->stream, c->g, sol);
+    int rc;
+    switch (o->mode) {
+        case PSI_MODE_PLAIN: rc = solve_plain(c, o, forc, sol, info); break;
+        case PSI_MODE_MIXED: rc = solve_mixed(c, o, forc, sol, info); break;
+        case PSI_MODE_LEARNED: return fail(c, PSI_NO_LEARNED_MODEL, "no learned model loaded%s");
+        default: return fail(c, PSI_BAD_ARG, "unknown mode%s");
+    }
+    if (rc == PSI_HIP_ERROR || rc == PSI_BAD_ARG) return rc;
+    HIPCHK(c, hipEventRecord(c->t1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->t1));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    info->elapsed_ms = ms;
+    return info->status;
+}
+
+int psi_apply_op(psi_ctx* c, const double* p, double* res) {
+    if (!valid_field_ctx(c) || !p || !res) return PSI_BAD_ARG;
+    hipLaunchKernelGGL(k_zero_halo, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->g, res);
+    hipLaunchKernelGGL(k_apply_op, dim3(c->g.ntiles), dim3(NT), 0, c->stream, c->g, c->E, c->N,
+                       c->rx, c->ry, p, res);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+int psi_make_coeff(psi_ctx* c, double* cf) {
+    if (!valid_field_ctx(c) || !cf) return PSI_BAD_ARG;
+    hipLaunchKernelGGL(k_make_coeff_ref, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->g, c->hur,
+                       c->hvr, c->dxt, c->dxu, c->dyt, c->dyu, c->cost, c->cosu, cf);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+int psi_make_inv(psi_ctx* c, double* Z) {
+    if (!valid_field_ctx(c) || !Z) return PSI_BAD_ARG;
+    hipLaunchKernelGGL(k_make_inv, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->g, c->cC,
+                       c->act, Z);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+static int dot_absmax(psi_ctx* c, const double* a, const double* b, double* out2) {
+    const int blocks = 1024;
+    hipLaunchKernelGGL(k_dot_absmax, dim3(blocks), dim3(NT), 0, c->stream, c->g, a, b, c->part,
+                       c->counter, c->result);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_res, c->result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out2[0] = c->h_res[0];
+    out2[1] = c->h_res[1];
+    return PSI_OK;
+}
+
+int psi_dot(psi_ctx* c, const double* a, const double* b, double* out) {
+    if (!valid_field_ctx(c) || !a || !b || !out) return PSI_BAD_ARG;
+    double r[2];
+    int rc = dot_absmax(c, a, b, r);
+    if (rc) return rc;
+    *out = r[0];
+    return PSI_OK;
+}
+
+int psi_absmax(psi_ctx* c, const double* a, double* out) {
+    if (!valid_field_ctx(c) || !a || !out) return PSI_BAD_ARG;
+    double r[2];
+    int rc = dot_absmax(c, a, nullptr, r);
+    if (rc) return rc;
+    *out = r[1];
+    return PSI_OK;
+}
+
+int psi_setcyclic(psi_ctx* c, double* a) {
+    if (!valid_field_ctx(c) || !a) return PSI_BAD_ARG;
+    if (!c->g.cyclic) return PSI_OK;
+    hipLaunchKernelGGL(k_setcyclic, dim3(nblk(c->g.ny + 4)), dim3(256), 0, c->stream, c->g, a);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+static int line_integral_async(psi_ctx* c, int isle, const double* uloc, const double* vloc,
+                               double* dev_out2) {
+    const int k = isle - 1;
+    const int s = c->offs[k], n = c->offs[k + 1] - s;
+    hipLaunchKernelGGL(k_line_integral, dim3(1), dim3(NT), 0, c->stream, c->g, n, c->bi + s,
+                       c->bj + s, c->di + s, c->dj + s, c->dxu, c->dyu, c->cost, uloc, vloc,
+                       dev_out2);
+    return check_launch(c);
+}
+
+int psi_line_integral(psi_ctx* c, int32_t isle, const double* uloc, const double* vloc,
+                      double* line) {
+    if (!valid_field_ctx(c) || !uloc || !vloc || !line || isle < 1 || isle > c->nisle)
+        return PSI_BAD_ARG;
+    int rc = line_integral_async(c, isle, uloc, vloc, c->result);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_res, c->result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->h_res[1] != 0.0) return fail(c, PSI_BAD_ARG, "missing line_dir in line integral%s");
+    *line = c->h_res[0];
+    return PSI_OK;
+}
+
+int psi_psi_velocity(psi_ctx* c, const double* psi_f, double* fpx, double* fpy) {
+    if (!valid_field_ctx(c) || !psi_f || !fpx || !fpy) return PSI_BAD_ARG;
+    if (!c->maskU || !c->maskV) return fail(c, PSI_BAD_ARG, "maskU/maskV not given%s");
+    hipLaunchKernelGGL(k_psi_velocity, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->g, psi_f,
+                       c->maskU, c->maskV, c->hur, c->hvr, c->dxt, c->dyt, c->cosu, fpx, fpy);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+int psi_true_relres(psi_ctx* c, const double* forc, const double* sol, double* relres) {
+    if (!valid_field_ctx(c) || !forc || !sol || !relres) return PSI_BAD_ARG;
+    double res[4];
+    int rc = residual(c, forc, sol, nullptr, res);
+    if (rc) return rc;
+    *relres = res[1] > 0 ? sqrt(res[0] / res[1]) : sqrt(res[0]);
+    return PSI_OK;
+}
+
+int psi_streamfunction_init(psi_ctx* c, const psi_opts* o, double* psin, double* line_psin,
+                            int32_t* iterations) {
+    if (!valid_field_ctx(c) || !o || !psin || !line_psin) return PSI_BAD_ARG;
+    if (c->nisle > 0 && (!c->maskU || !c->maskV))
+        return fail(c, PSI_BAD_ARG, "maskU/maskV needed for line_psin%s");
+    const int n = c->nisle;
+    double* zero = c->tmp64;
+    HIPCHK(c, hipMemsetAsync(zero, 0, c->plane * 8, c->stream));
+    for (int k = 0; k < n; ++k) {
+        double* pk = psin + (size_t)k * c->plane;
+        HIPCHK(c, hipMemsetAsync(pk, 0, c->plane * 8, c->stream));
+        const int s = c->offs[k], np = c->offs[k + 1] - s;
+        // psin = 1 on the island's perimeter (streamfct_init.f90:274-279)
+        std::vector<int32_t> hbi(np), hbj(np);
+        HIPCHK(c, hipMemcpy(hbi.data(), c->bi + s, np * 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(hbj.data(), c->bj + s, np * 4, hipMemcpyDeviceToHost));
+        std::vector<double> host(c->plane, 0.0);
+        for (int q = 0; q < np; ++q) {
+            const int i = hbi[q], j = hbj[q];
+            if (i >= -1 && i <= c->g.nx + 2 && j >= -1 && j <= c->g.ny + 2)
+                host[(size_t)(j + 1) * c->g.pitch + (i + 1)] = 1.0;
+        }
+        HIPCHK(c, hipMemcpyAsync(pk, host.data(), c->plane * 8, hipMemcpyHostToDevice, c->stream));
+        psi_info info;
+        const int rc = psi_congrad(c, o, zero, pk, &info);
+        if (rc == PSI_HIP_ERROR || rc == PSI_BAD_ARG) return rc;
+        if (c->g.cyclic)
+            hipLaunchKernelGGL(k_setcyclic, dim3(nblk(c->g.ny + 4)), dim3(256), 0, c->stream, c->g,
+                               pk);
+        if (iterations) iterations[k] = info.iterations;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    // line_psin(n, isle) (streamfct_init.f90:292-303)
+    for (int isle = 0; isle < n; ++isle) {
+        hipLaunchKernelGGL(k_psi_velocity, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->g,
+                           psin + (size_t)isle * c->plane, c->maskU, c->maskV, c->hur, c->hvr,
+                           c->dxt, c->dyt, c->cosu, c->tmp64, c->tmp64b);
+        for (int m = 0; m < n; ++m) {
+            int rc = line_integral_async(c, m + 1, c->tmp64, c->tmp64b, c->result + 2 + 0);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->h_res, c->result + 2, 2 * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            line_psin[(size_t)m * n + isle] = c->h_res[0];
+        }
+    }
+    return PSI_OK;
+}
+
+int psi_solve_streamfunction(psi_ctx* c, const psi_opts* o, const double* fpx, const double* fpy,
+                             double* psi_f, double* dpsi, double* dpsin, const double* psin,
+                             const double* line_psin, int32_t taum1, int32_t tau, int32_t taup1,
+                             double dt_mom, double AB_eps, double* forc_out, psi_info* info) {
+    if (!valid_field_ctx(c) || !o || !fpx || !fpy || !psi_f || !dpsi || !info) return PSI_BAD_ARG;
+    if (taum1 < 0 || taum1 > 2 || tau < 0 || tau > 2 || taup1 < 0 || taup1 > 2)
+        return fail(c, PSI_BAD_ARG, "time level out of range%s");
+    const int n = c->nisle;
+    if (n > 1 && (!dpsin || !psin || !line_psin || !c->maskU || !c->maskV))
+        return fail(c, PSI_BAD_ARG, "island arrays missing%s");
+    const size_t P = c->plane;
+    double* forc = forc_out ? forc_out : c->tmp64;
+    hipLaunchKernelGGL(k_curl_forcing, dim3(nblk(P)), dim3(256), 0, c->stream, c->g, fpx, fpy,
+                       c->dxu, c->dyu, c->cost, c->cosu, forc);
+    hipLaunchKernelGGL(k_warm_start, dim3(nblk(P)), dim3(256), 0, c->stream, P, dpsi + tau * P,
+                       dpsi + taum1 * P, dpsi + taup1 * P);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    rc = psi_congrad(c, o, forc, dpsi + taup1 * P, info);
+    if (rc == PSI_HIP_ERROR || rc == PSI_BAD_ARG || rc == PSI_NAN) return rc;
+    const int solve_rc = rc;
+    if (c->g.cyclic)
+        hipLaunchKernelGGL(k_setcyclic, dim3(nblk(c->g.ny + 4)), dim3(256), 0, c->stream, c->g,
+                           dpsi + taup1 * P);
+    std::vector<double> coef((size_t)std::max(n, 1), 0.0);
+    const double w1 = 1.5 + AB_eps, w0 = 0.5 + AB_eps;
+    if (n > 1) {
+        // line_forc(k) - line_psi0(k) for k = 2..nisle (solve_streamfct.f90:63-77)
+        std::vector<double> rhs((size_t)n, 0.0);
+        for (int k = 2; k <= n; ++k) {
+            rc = line_integral_async(c, k, fpx, fpy, c->result);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->h_res, c->result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            rhs[k - 1] = c->h_res[0];
+        }
+        hipLaunchKernelGGL(k_psi_velocity, dim3(nblk(P)), dim3(256), 0, c->stream, c->g,
+                           dpsi + taup1 * P, c->maskU, c->maskV, c->hur, c->hvr, c->dxt, c->dyt,
+                           c->cosu, c->tmp64, c->tmp64b);
+        for (int k = 2; k <= n; ++k) {
+            rc = line_integral_async(c, k, c->tmp64, c->tmp64b, c->result);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->h_res, c->result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            rhs[k - 1] = rhs[k - 1] - c->h_res[0];
+        }
+        // DGESV on line_psin(2:nisle, 2:nisle) (solve_streamfct.f90:79-87):
+        // LU with partial pivoting, FP64, on the host (at most a few unknowns).
+        const int m = n - 1;
+        std::vector<double> A((size_t)m * m), x(m);
+        for (int r = 0; r < m; ++r) {
+            for (int q = 0; q < m; ++q) A[(size_t)r * m + q] = line_psin[(size_t)(r + 1) * n + (q + 1)];
+            x[r] = rhs[r + 1];
+        }
+        for (int k = 0; k < m; ++k) {
+            int piv = k;
+            double amax = fabs(A[(size_t)k * m + k]);
+            for (int r = k + 1; r < m; ++r)
+                if (fabs(A[(size_t)r * m + k]) > amax) {
+                    amax = fabs(A[(size_t)r * m + k]);
+                    piv = r;
+                }
+            if (A[(size_t)piv * m + k] == 0.0)
+                return fail(c, PSI_SINGULAR_ISLAND_SYSTEM, "island system singular%s");
+            if (piv != k) {
+                for (int q = 0; q < m; ++q) std::swap(A[(size_t)k * m + q], A[(size_t)piv * m + q]);
+                std::swap(x[k], x[piv]);
+            }
+            for (int r = k + 1; r < m; ++r) {
+                const double l = A[(size_t)r * m + k] / A[(size_t)k * m + k];
+                A[(size_t)r * m + k] = l;
+                for (int q = k + 1; q < m; ++q) A[(size_t)r * m + q] -= l * A[(size_t)k * m + q];
+                x[r] -= l * x[k];
+            }
+        }
+        for (int k = m - 1; k >= 0; --k) {
+            double s = x[k];
+            for (int q = k + 1; q < m; ++q) s -= A[(size_t)k * m + q] * x[q];
+            x[k] = s / A[(size_t)k * m + k];
+        }
+        for (int k = 2; k <= n; ++k) dpsin[(size_t)(k - 1) * 3 + tau] = x[k - 2];
+        for (int k = 2; k <= n; ++k)
+            coef[k - 1] = dt_mom * (w1 * dpsin[(size_t)(k - 1) * 3 + tau] -
+                                    w0 * dpsin[(size_t)(k - 1) * 3 + taum1]);
+    }
+    double* dcoef = c->result + 4;   // up to 4 islands via scratch; more via tmp buffer
+    const double* psin_arg = psin ? psin : c->tmp64;
+    if (n > 1) {
+        if (n > 4) dcoef = c->tmp64b;   // velocity scratch is free again here
+        HIPCHK(c, hipMemcpyAsync(dcoef, coef.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    hipLaunchKernelGGL(k_ab_update, dim3(nblk(P)), dim3(256), 0, c->stream, P, psi_f + tau * P,
+                       dpsi + taup1 * P, dpsi + tau * P, dt_mom, w1, w0, psin_arg, dcoef,
+                       n > 1 ? n : 0, psi_f + taup1 * P);
+    rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return solve_rc;
+}
+
+}  // extern "C"
