@@ -113,8 +113,10 @@ def test_streamfunction_init_matches_reference(case):
     psin = N(psin)
     for k in range(int(d["nisle"])):
         assert relerr(psin[k], d["psin"][k]) <= 1e-9, k
-    np.testing.assert_allclose(line_psin, d["line_psin"], rtol=1e-8,
-                               atol=1e-8 * np.abs(d["line_psin"]).max())
+    if int(d["nisle"]) > 1:
+        # with one land mass psin is 1 everywhere and line_psin is round-off
+        np.testing.assert_allclose(line_psin, d["line_psin"], rtol=1e-8,
+                                   atol=1e-8 * np.abs(np.diag(d["line_psin"])).max())
 
 
 def test_relres_criterion_and_mixed_precision():
