@@ -14,7 +14,7 @@ constexpr int HX = TX + 4;
 constexpr int HY = TY + 4;
 constexpr int NT = 256;            // threads per workgroup (4 wave64)
 constexpr int NW = NT / 64;
-constexpr int NRED = 5;            // reduction slots per sweep
+constexpr int NRED = 6;            // reduction slots per sweep
 
 struct Geom {
     int nx, ny, pitch, cyclic;     // pitch = nx + 4 (elements)

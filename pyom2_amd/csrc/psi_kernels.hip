@@ -6,9 +6,11 @@
 //     p_k     = u_k + beta_k p_{k-1}                    (radius 2)
 //     u_{k+1} = u_k - alpha_k Z (A p_k)                 (radius 1)
 //     x_{k+1} = x_k + alpha_k p_k,  w = A u_{k+1}       (radius 0)
-// together with the reductions (r, Z r), (Z r, A Z r), max|p|, |r|^2 that
-// give alpha_{k+1}, beta_{k+1} by the one-reduction (Chronopoulos-Gear)
-// form of the reference recurrence (solve_streamfct.f90:223-299).  The last
+// together with the reductions (r, Z r), max|p|, |r|^2 and the three inner
+// products of u, w = A u, p and s = A p that give the next (p, A p) exactly,
+// so one reduction point per iteration yields alpha_{k+1}, beta_{k+1} of the
+// reference recurrence (solve_streamfct.f90:223-299) also for the
+// non-symmetric operator of spherical grids.  The last
 // workgroup to finish folds the partials in a fixed order and runs the
 // reference's divergence / degenerate / convergence tests on the device.
 #include <hip/hip_runtime.h>
@@ -184,8 +186,14 @@ __device__ void cg_after_setup(CgState* st, double gam, double del, double zmax,
 
 // After sweep n: step/extrapolation test of iteration n (:283-297), then
 // the prologue of iteration n+1: beta, divergence test (:232-240),
-// degenerate test (:262-266) and alpha (:270).
-__device__ void cg_after_sweep(CgState* st, double gam, double del, double pmax, double rr) {
+// degenerate test (:262-266) and alpha (:270).  With u = Z r_{n+1},
+// w = A u, p = p_n and s = A p_n, the reference's s_dot_As of iteration
+// n+1 is (p_{n+1}, A p_{n+1}) = d1 + beta d2 + beta^2 d3 with
+// d1 = (u, w), d2 = (u, s) + (p, w), d3 = (p, s): exact for the
+// non-symmetric operator of spherical grids, where the symmetric
+// shortcut d1 - beta gamma / alpha would change the iteration.
+__device__ void cg_after_sweep(CgState* st, double gam, double d1, double d2, double d3,
+                               double pmax, double rr) {
     const int n = st->n + 1;
     const double alpha = st->alpha;
     st->rr = rr;
@@ -221,7 +229,7 @@ __device__ void cg_after_sweep(CgState* st, double gam, double del, double pmax,
         st->done = 1;
         return;
     }
-    if (gam != gam || del != del || rr != rr) {
+    if (gam != gam || d1 != d1 || rr != rr) {
         st->n = n;
         st->status = ST_NAN;
         st->done = 1;
@@ -238,7 +246,7 @@ __device__ void cg_after_sweep(CgState* st, double gam, double del, double pmax,
             return;
         }
     }
-    const double pAp = del - beta * gam / alpha;
+    const double pAp = d1 + beta * d2 + beta * beta * d3;
     st->beta = beta;
     st->gamma = gam;
     st->n = n;
@@ -309,7 +317,7 @@ __global__ __launch_bounds__(NT) void k_cg_setup(
     }
     // keep b at radius 0 for |b|^2 and r0 for |r|^2: compute them before
     // overwriting (radius-0 points are a subset of radius-1 points).
-    double acc[NRED] = {0, 0, 0, 0, 0};
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < (W1 * H1 + NT - 1) / NT; ++k) {
@@ -415,8 +423,8 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
         }
     }
     __syncthreads();
-    // radius 0: x update, w = A u_{k+1}, reductions
-    double acc[NRED] = {0, 0, 0, 0, 0};
+    // radius 0: x update, w = A u_{k+1}, s = A p_k, reductions
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < (TX * TY) / NT; ++k) {
         const int l = threadIdx.x + k * NT;
         const int gg = 2 + l / TX, f = 2 + l % TX;
@@ -425,26 +433,30 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
             const int idx = gg * HX + f;
             T cc;
             const T w = stencil5(sU, sE, sN, idx, sRX[f], sRY[gg], cc);
+            const T sv = stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
             const T un = sU[idx], pn = sP[idx];
             const size_t o = (size_t)r * g.pitch + c;
             x[o] = x[o] + alpha * pn;
             u_out[o] = un;
             p_out[o] = pn;
-            const double ud = (double)un, ccd = (double)cc;
+            const double ud = (double)un, pd = (double)pn, ccd = (double)cc;
+            const double wd = (double)w, sd = (double)sv;
             acc[0] += ud * ud * ccd;
-            acc[1] += ud * (double)w;
-            acc[2] = fmax(acc[2], fabs((double)pn));
+            acc[1] += ud * wd;
+            acc[2] = fmax(acc[2], fabs(pd));
             if (sA[idx]) {
                 const double rv = ud * ccd;
                 acc[3] += rv * rv;
             }
+            acc[4] += ud * sd + pd * wd;
+            acc[5] += pd * sd;
         }
     }
     block_reduce<NRED>(acc, 1u << 2, sRed);
     if (publish_partials<NRED>(acc, part, counter, blockIdx.x, gridDim.x)) {
         fold_partials<NRED>(acc, 1u << 2, part, gridDim.x, sRed);
         if (threadIdx.x == 0) {
-            cg_after_sweep(st, acc[0], acc[1], acc[2], acc[3]);
+            cg_after_sweep(st, acc[0], acc[1], acc[4], acc[5], acc[2], acc[3]);
             *counter = 0u;
         }
     }
@@ -482,7 +494,7 @@ __global__ __launch_bounds__(NT) void k_residual(
     }
     load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
     __syncthreads();
-    double acc[NRED] = {0, 0, 0, 0, 0};
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < (TX * TY) / NT; ++k) {
         const int l = threadIdx.x + k * NT;
         const int gg = 2 + l / TX, f = 2 + l % TX;
