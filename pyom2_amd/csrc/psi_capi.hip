@@ -40,8 +40,10 @@ struct psi_ctx {
     float *u32[2] = {nullptr, nullptr}, *p32[2] = {nullptr, nullptr}, *e32 = nullptr;
     double *r64 = nullptr, *tmp64 = nullptr, *tmp64b = nullptr;
     double *part = nullptr, *result = nullptr;
+    double* part2 = nullptr;      // solver partials, 2 x NRED x G
+    int G = 0;                    // resident workgroups of the solver kernels
     unsigned* counter = nullptr;
-    CgState* st = nullptr;
+    CgState* st = nullptr;        // 2 states (double-buffered by launch)
     CgState* h_st = nullptr;      // pinned: [0..1] poll slots, [2] init
     double* h_res = nullptr;      // pinned scalars
     hipEvent_t ev[2] = {nullptr, nullptr}, t0 = nullptr, t1 = nullptr;
@@ -104,100 +106,92 @@ int build_operator(psi_ctx* c) {
 }
 
 struct CgRun {
-    int status = ST_RUNNING;
-    int iterations = 0;
-    double est_error = 0, rr = 0, rr0 = 0, bb = 0;
+    CgState fin{};
     double sweep_ms = 0;
     long launches = 0;
 };
 
-// Setup + fused sweeps until the device flags completion.  Sweeps are
-// issued in batches of `batch`; the host keeps one batch queued ahead of
-// the one whose state it is inspecting, so the GPU never idles on a poll.
+template <class T>
+struct CgPath {
+    const T *E, *N, *rx, *ry;
+    T *u[2], *p[2];
+};
+
+// Launch bookkeeping of one solve: launch L reads state/partials (L-1) & 1
+// and writes L & 1 (the setup is launch 0 and reads the host-written state 0).
+struct Launches {
+    long L = 0;
+    CgState* st_in(psi_ctx* c) const { return c->st + ((L - 1) & 1); }
+    CgState* st_out(psi_ctx* c) const { return c->st + (L & 1); }
+    double* part_in(psi_ctx* c) const { return c->part2 + ((L - 1) & 1) * (size_t)NRED * c->G; }
+    double* part_out(psi_ctx* c) const { return c->part2 + (L & 1) * (size_t)NRED * c->G; }
+};
+
 template <class T, class TB>
-int cg_run(psi_ctx* c, const T* E, const T* N, const T* rx, const T* ry, const TB* b,
-           double bscale, T* x, bool zero_guess, T** u, T** p, const CgState& init, int batch,
-           CgRun* out) {
-    const Geom& g = c->g;
+int cg_setup(psi_ctx* c, const CgPath<T>& P, const TB* b, double bscale, T* x, bool zero_guess,
+             const CgState& init, Launches& Ln) {
     c->h_st[2] = init;
     HIPCHK(c, hipMemcpyAsync(c->st, &c->h_st[2], sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-    const dim3 grid(g.ntiles), blk(NT);
-    if (zero_guess)
-        hipLaunchKernelGGL((k_cg_setup<T, TB>), grid, blk, 0, c->stream, g, E, N, rx, ry, c->act, b,
-                           bscale, (const T*)nullptr, x, u[0], p[0], c->part, c->counter, c->st);
-    else
-        hipLaunchKernelGGL((k_cg_setup<T, TB>), grid, blk, 0, c->stream, g, E, N, rx, ry, c->act, b,
-                           bscale, (const T*)x, (T*)nullptr, u[0], p[0], c->part, c->counter, c->st);
-    int rc = check_launch(c);
-    if (rc) return rc;
-    long issued = 0;
-    const long limit = (long)init.max_it + 2;
-    int slot = 0;
-    int queue[2], qn = 0;
+    Ln.L = 0;
+    hipLaunchKernelGGL((k_cg_setup<T, TB>), dim3(c->G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
+                       P.rx, P.ry, c->act, b, bscale, zero_guess ? (const T*)nullptr : (const T*)x,
+                       zero_guess ? x : (T*)nullptr, P.u[0], P.p[0], Ln.part_out(c));
+    Ln.L = 1;
+    return check_launch(c);
+}
+
+// Issue fused sweeps in batches until a poll of the device state shows the
+// solve (or this leg of it) finished.  The host keeps one batch queued ahead
+// of the state it inspects, so the GPU never idles on a poll; sweeps issued
+// after completion see the finished state, forward it and return at once.
+template <class T>
+int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, long max_launch,
+               CgRun* out) {
+    int slot = 0, queue[2], qn = 0;
     long issued_at[2] = {0, 0};
-    bool timing_started = false;
+    const long L0 = Ln.L;
+    HIPCHK(c, hipEventRecord(c->ts0, c->stream));
     auto issue = [&]() -> int {
-        if (!timing_started) {
-            HIPCHK(c, hipEventRecord(c->ts0, c->stream));
-            timing_started = true;
-        }
-        for (int k = 0; k < batch && issued < limit; ++k, ++issued) {
-            const int s = (int)(issued & 1);
-            hipLaunchKernelGGL((k_cg_sweep<T>), grid, blk, 0, c->stream, g, E, N, rx, ry, c->act, x,
-                               (const T*)u[s], (const T*)p[s], u[s ^ 1], p[s ^ 1], c->part,
-                               c->counter, c->st);
-        }
+        for (int k = 0; k < batch && Ln.L - L0 < max_launch; ++k, ++Ln.L)
+            hipLaunchKernelGGL((k_cg_sweep<T>), dim3(c->G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
+                               P.rx, P.ry, c->act, x, P.u[0], P.u[1], P.p[0], P.p[1],
+                               Ln.st_in(c), Ln.st_out(c), Ln.part_in(c), Ln.part_out(c));
         int r = check_launch(c);
         if (r) return r;
         HIPCHK(c, hipEventRecord(c->ts1[slot], c->stream));
-        issued_at[slot] = issued;
-        HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->st, sizeof(CgState), hipMemcpyDeviceToHost,
-                                 c->stream));
+        issued_at[slot] = Ln.L - L0;
+        // the latest state is the one the last launch wrote
+        HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->st + ((Ln.L - 1) & 1), sizeof(CgState),
+                                 hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
         queue[qn++] = slot;
         slot ^= 1;
         return PSI_OK;
     };
-    // first poll right after setup (early exit), then batches
-    HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->st, sizeof(CgState), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
-    issued_at[slot] = -1;   // no sweeps behind this poll
-    queue[qn++] = slot;
-    slot ^= 1;
-    CgState fin{};
     for (;;) {
-        if (qn < 2 && issued < limit) {
-            rc = issue();
+        if (qn < 2 && Ln.L - L0 < max_launch) {
+            int rc = issue();
             if (rc) return rc;
         }
+        if (qn == 0) return fail(c, PSI_HIP_ERROR, "solver launch budget exhausted%s");
         const int s = queue[0];
         queue[0] = queue[1];
         --qn;
         HIPCHK(c, hipEventSynchronize(c->ev[s]));
         if (c->h_st[s].done) {
-            fin = c->h_st[s];
-            if (issued_at[s] > 0) {
-                float ms = 0.f;
-                HIPCHK(c, hipEventElapsedTime(&ms, c->ts0, c->ts1[s]));
-                out->sweep_ms = ms;
-                out->launches = issued_at[s];
+            out->fin = c->h_st[s];
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ts0, c->ts1[s]));
+            out->sweep_ms += ms;
+            out->launches += issued_at[s];
+            // drain the batch queued behind this poll so the next leg starts clean
+            if (qn) {
+                HIPCHK(c, hipEventSynchronize(c->ev[queue[0]]));
+                out->launches += issued_at[queue[0]] - issued_at[s];
             }
-            break;
-        }
-        if (qn == 0 && issued >= limit) {   // cannot happen: max_it sets done
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            HIPCHK(c, hipMemcpy(&fin, c->st, sizeof(CgState), hipMemcpyDeviceToHost));
-            break;
+            return PSI_OK;
         }
     }
-    out->status = fin.status;
-    out->iterations = fin.n;
-    out->est_error = fin.est_error;
-    out->rr = fin.rr;
-    out->rr0 = fin.rr0;
-    out->bb = fin.bb;
-    return PSI_OK;
 }
 
 // FP64 residual of (b, x): returns |r|^2, |b|^2 (active points), max|r|, count.
@@ -214,6 +208,17 @@ int residual(psi_ctx* c, const double* b, const double* x, double* r_out, double
     return PSI_OK;
 }
 
+template <class T>
+int cg_refresh(psi_ctx* c, const CgPath<T>& P, double rscale, double rr_thresh, int new_max_it,
+               Launches& Ln) {
+    hipLaunchKernelGGL((k_cg_refresh<T>), dim3(c->G), dim3(NT), 0, c->stream, c->g, P.E, P.N, P.rx,
+                       P.ry, c->act, c->r64, rscale, P.u[0], P.u[1], (const T*)P.p[0],
+                       (const T*)P.p[1], c->result, rr_thresh, new_max_it, Ln.st_in(c),
+                       Ln.st_out(c), Ln.part_out(c));
+    ++Ln.L;
+    return check_launch(c);
+}
+
 int status_of(int st) {
     switch (st) {
         case ST_CONVERGED: return PSI_OK;
@@ -224,104 +229,120 @@ int status_of(int st) {
     }
 }
 
-CgState make_init(const psi_opts* o, int criterion, double rtarget, int max_it, bool rel_initial) {
+CgState make_init(const psi_opts* o, int criterion, double rtarget, int max_it) {
     CgState s{};
     s.eps = o->congr_epsilon;
     s.rtarget2 = rtarget * rtarget;
     s.max_it = max_it;
     s.criterion = criterion;
-    s.rel_to_initial = rel_initial ? 1 : 0;
+    s.phase = PHASE_SETUP;
     return s;
 }
 
-int solve_plain(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
+// One Jacobi-PCG solve in precision T on the correction x (FP64: x is the
+// solution itself; FP32: x is a correction flushed into sol).  With the
+// relres criterion the recursive residual is confirmed on the FP64 true
+// residual; a miss replaces the recursive residual by the true one and
+// continues (no restart).  refresh_every > 0 also replaces it every that
+// many iterations (needed in FP32, where the two drift apart).
+template <class T>
+int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* forc, double* sol,
+              T* xw, double bscale_in, bool fp64, int refresh_every, psi_info* info) {
     const int batch = o->check_every > 0 ? o->check_every : 32;
-    CgRun run;
-    const int crit = o->criterion == PSI_CRIT_RELRES ? CRIT_RELRES : CRIT_REFERENCE;
-    int total = 0, rc;
-    int restarts = 0;
+    const bool relres = fp64 ? (o->criterion == PSI_CRIT_RELRES) : true;
+    const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
+    const int abs_max = o->max_iterations;
     double res[4] = {0, 0, 0, 0};
-    for (;;) {
-        CgState init = make_init(o, crit, o->relres_target, o->max_iterations - total, false);
-        if (init.max_it < 1) init.max_it = 1;
-        rc = cg_run<double, double>(c, c->E, c->N, c->rx, c->ry, forc, 1.0, sol, false, c->u64,
-                                    c->p64, init, batch, &run);
+    double scale = 1.0, bb_forc = 0.0;
+    Launches Ln;
+    CgRun run;
+    int rc;
+    if (!fp64) {
+        rc = residual(c, forc, sol, c->r64, res);
         if (rc) return rc;
-        total += run.status == ST_MAXIT ? init.max_it : run.iterations;
-        info->sweep_ms += run.sweep_ms;
-        info->sweep_launches += (int32_t)run.launches;
-        info->setup_launches += 2;
+        info->setup_launches += 1;
+        bb_forc = res[1] > 0 ? res[1] : res[0];
+        if (res[0] <= target * target * bb_forc) {
+            info->iterations = 0;
+            info->status = PSI_OK;
+            info->converged = 1;
+            info->relres_true = bb_forc > 0 ? sqrt(res[0] / bb_forc) : 0.0;
+            return PSI_OK;
+        }
+        const double rms = sqrt(res[0] / std::max(res[3], 1.0));
+        if (!(rms > 0) || rms != rms) return fail(c, PSI_NAN, "NaN residual%s");
+        scale = 1.0 / rms;
+    }
+    const int first_max = refresh_every > 0 ? std::min(refresh_every, abs_max) : abs_max;
+    CgState init = make_init(o, relres ? CRIT_RELRES : CRIT_REFERENCE, target, first_max);
+    if (!fp64) init.bb_fixed = bb_forc * scale * scale;
+    if (fp64)
+        rc = cg_setup<T, double>(c, P, forc, bscale_in, xw, false, init, Ln);
+    else
+        rc = cg_setup<T, double>(c, P, c->r64, scale, xw, true, init, Ln);
+    if (rc) return rc;
+    info->setup_launches += 1;
+    int refreshes = 0;
+    for (;;) {
+        rc = cg_iterate<T>(c, P, xw, Ln, batch, (long)abs_max + 4, &run);
+        if (rc) return rc;
+        const CgState& f = run.fin;
+        const bool leg_end = f.status == ST_MAXIT && f.n < abs_max;
+        if (!relres || !(f.status == ST_CONVERGED || leg_end)) break;
+        // confirm on the FP64 true residual
+        if (!fp64) {
+            hipLaunchKernelGGL((k_add_correction<T>), dim3(nblk(c->plane)), dim3(256), 0, c->stream,
+                               c->g, sol, xw, 1.0 / scale);
+            info->setup_launches += 1;
+        }
+        rc = residual(c, forc, sol, c->r64, res);
+        if (rc) return rc;
+        info->setup_launches += 1;
+        if (bb_forc == 0.0) bb_forc = res[1] > 0 ? res[1] : f.rr0;
+        const double thresh = target * target * bb_forc;
+        if (res[0] <= thresh) {
+            run.fin.status = ST_CONVERGED;
+            break;
+        }
+        if (f.n >= abs_max) {
+            run.fin.status = ST_MAXIT;
+            break;
+        }
+        const int next = refresh_every > 0 ? std::min(f.n + refresh_every, abs_max) : abs_max;
+        rc = cg_refresh<T>(c, P, fp64 ? 1.0 : scale, thresh, next, Ln);
+        if (rc) return rc;
+        info->setup_launches += 1;
+        ++refreshes;
+    }
+    if (!fp64 && !relres) return fail(c, PSI_BAD_ARG, "internal%s");
+    const CgState& f = run.fin;
+    info->iterations = f.status == ST_MAXIT ? abs_max + 1 : f.n;
+    info->status = status_of(f.status);
+    info->converged = f.status == ST_CONVERGED;
+    info->outer_iterations = refreshes;
+    info->est_error = f.est_error;
+    info->sweep_ms += run.sweep_ms;
+    info->sweep_launches += (int32_t)run.launches;
+    info->relres_recursive = f.bb > 0 ? sqrt(f.rr / f.bb) : 0.0;
+    if (!relres || res[1] == 0.0) {
         rc = residual(c, forc, sol, nullptr, res);
         if (rc) return rc;
-        if (crit != CRIT_RELRES || run.status != ST_CONVERGED) break;
-        const double bb = res[1] > 0 ? res[1] : run.rr0;
-        if (res[0] <= o->relres_target * o->relres_target * bb || restarts >= 3 ||
-            total >= o->max_iterations)
-            break;
-        ++restarts;   // recursive residual drifted from the true one: restart
+        info->setup_launches += 1;
     }
-    info->iterations = run.status == ST_MAXIT ? o->max_iterations + 1 : total;
-    info->status = status_of(run.status);
-    info->converged = run.status == ST_CONVERGED;
-    info->outer_iterations = restarts;
-    info->est_error = run.est_error;
-    info->relres_recursive = run.bb > 0 ? sqrt(run.rr / run.bb) : 0.0;
-    info->relres_true = res[1] > 0 ? sqrt(res[0] / res[1]) : sqrt(res[0]);
+    const double bbn = bb_forc > 0 ? bb_forc : res[1];
+    info->relres_true = bbn > 0 ? sqrt(res[0] / bbn) : sqrt(res[0]);
     return info->status;
 }
 
+int solve_plain(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
+    CgPath<double> P{c->E, c->N, c->rx, c->ry, {c->u64[0], c->u64[1]}, {c->p64[0], c->p64[1]}};
+    return solve_pcg<double>(c, o, P, forc, sol, sol, 1.0, true, 0, info);
+}
+
 int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
-    const int batch = o->check_every > 0 ? o->check_every : 32;
-    const double inner = o->inner_reduction > 0 ? o->inner_reduction : 1e-5;
-    const int max_outer = o->max_outer > 0 ? o->max_outer : 30;
-    const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
-    double res[4];
-    int rc = residual(c, forc, sol, c->r64, res);
-    if (rc) return rc;
-    const double bb = res[1] > 0 ? res[1] : res[0];
-    int total = 0, outer = 0, last = ST_CONVERGED;
-    double est = 0;
-    while (res[0] > target * target * bb) {
-        if (outer >= max_outer || total >= o->max_iterations) {
-            last = ST_MAXIT;
-            break;
-        }
-        const double rms = sqrt(res[0] / std::max(res[3], 1.0));
-        if (!(rms > 0) || rms != rms) {
-            last = ST_NAN;
-            break;
-        }
-        const double scale = 1.0 / rms;
-        CgState init = make_init(o, CRIT_RELRES, inner, o->max_iterations - total, true);
-        CgRun run;
-        rc = cg_run<float, double>(c, c->E32, c->N32, c->rx32, c->ry32, c->r64, scale, c->e32,
-                                   true, c->u32, c->p32, init, batch, &run);
-        if (rc) return rc;
-        total += run.status == ST_MAXIT ? init.max_it : run.iterations;
-        info->sweep_ms += run.sweep_ms;
-        info->sweep_launches += (int32_t)run.launches;
-        info->setup_launches += 3;
-        est = run.est_error;
-        if (run.status == ST_NAN || run.status == ST_DIVERGED) {
-            last = run.status;
-            break;
-        }
-        hipLaunchKernelGGL((k_add_correction<float>), dim3(nblk(c->plane)), dim3(256), 0, c->stream,
-                           c->g, sol, c->e32, rms);
-        rc = check_launch(c);
-        if (rc) return rc;
-        rc = residual(c, forc, sol, c->r64, res);
-        if (rc) return rc;
-        ++outer;
-    }
-    info->iterations = total;
-    info->outer_iterations = outer;
-    info->status = status_of(last);
-    info->converged = last == ST_CONVERGED;
-    info->est_error = est;
-    info->relres_true = bb > 0 ? sqrt(res[0] / bb) : 0.0;
-    info->relres_recursive = info->relres_true;
-    return info->status;
+    CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]}};
+    const int every = o->reserved[0] > 0 ? o->reserved[0] : 400;
+    return solve_pcg<float>(c, o, P, forc, sol, c->e32, 1.0, false, every, info);
 }
 
 bool valid_field_ctx(psi_ctx* c) { return c && c->E; }
@@ -422,9 +443,26 @@ int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
         TRY(dalloc(c, &c->tmp64, c->plane));
         TRY(dalloc(c, &c->tmp64b, c->plane));
         TRY(dalloc(c, &c->part, (size_t)NRED * (g.ntiles + 1024)));
+        {
+            // resident grid: CUs x workgroups per CU of the FP64 sweep (the
+            // FP32 kernels fit at least as many), a multiple of 8 (XCDs)
+            int ncu = 0, occ = 0;
+            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+                    hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg_sweep<double>, NT, 0) !=
+                    hipSuccess) {
+                rc = fail(c, PSI_HIP_ERROR, "device query failed%s");
+                goto failed;
+            }
+            int G = std::max(1, ncu * std::max(occ, 1));
+            if (G > g.ntiles) G = g.ntiles;
+            if (G >= 8) G &= ~7;
+            c->G = G;
+        }
+        TRY(dalloc(c, &c->part2, (size_t)2 * NRED * c->G));
         TRY(dalloc(c, &c->result, 8));
         TRY(dalloc(c, &c->counter, 4));
-        TRY(dalloc(c, &c->st, 1));
+        TRY(dalloc(c, &c->st, 2));
         if (hipStreamSynchronize(c->stream) != hipSuccess) {
             rc = fail(c, PSI_HIP_ERROR, "allocation failed%s");
             goto failed;

@@ -23,16 +23,20 @@ struct Geom {
 
 // Solver state living in device memory; updated by the last workgroup of
 // each sweep, read by every workgroup of the next one.
+// Solver state in device memory, double-buffered by launch: launch L reads
+// state (L-1) & 1 and the partials (L-1) & 1 of the previous launch, and
+// workgroup 0 writes state L & 1.
 struct CgState {
     double alpha, beta, gamma;     // next sweep's step, direction update, current (r, Z r)
     double betak_min, step1, est_error;
     double rr, rr0, bb, zmax0, pmax;
-    double eps, rtarget2;
+    double eps, rtarget2, bb_fixed;
     int n, status, done, max_it;
-    int criterion, degenerate, rel_to_initial, pad;
+    int criterion, degenerate, phase, pad;
 };
 
 enum : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXIT = 2, ST_DIVERGED = 3, ST_NAN = 4 };
+enum : int { PHASE_SETUP = 0, PHASE_SWEEP = 1, PHASE_REFRESH = 2 };
 enum : int { CRIT_REFERENCE = 0, CRIT_RELRES = 1 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on

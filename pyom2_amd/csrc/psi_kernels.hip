@@ -146,320 +146,488 @@ __device__ __forceinline__ void load_metrics_1d(const Geom& g, int c0, int r0, c
 }
 
 // ---------------------------------------------------------------------------
-// CG state transitions (run by one thread of the last workgroup)
+// Work schedule.  The solver kernels run on a fixed grid of G resident
+// workgroups; every workgroup sweeps a static set of tiles and publishes one
+// partial per reduction slot.  Each XCD (workgroup b runs on XCD b % 8) owns
+// a contiguous range of tiles, so neighbouring tiles, which share halo rows,
+// are swept through the same L2.
 // ---------------------------------------------------------------------------
-
-// After the setup sweep: the reference's entry test (solve_streamfct.f90:
-// 206-213) and the prologue of iteration 1 (:227-266 with ss = 0).
-__device__ void cg_after_setup(CgState* st, double gam, double del, double zmax, double rr,
-                               double bb) {
-    st->rr0 = rr;
-    st->rr = rr;
-    if (st->rel_to_initial || bb == 0.0) bb = rr;
-    st->bb = bb;
-    st->zmax0 = zmax;
-    st->n = 0;
-    st->degenerate = 0;
-    st->est_error = 100.0 * zmax;
-    bool conv = (st->criterion == CRIT_REFERENCE) ? (100.0 * zmax < st->eps)
-                                                   : (rr <= st->rtarget2 * bb);
-    if (conv) {
-        st->status = ST_CONVERGED;
-        st->done = 1;
+__device__ __forceinline__ void tile_schedule(const Geom& g, int& first, int& stride, int& end) {
+    const int b = blockIdx.x, G = gridDim.x;
+    if (G < 8 || (G & 7)) {
+        first = b;
+        stride = G;
+        end = g.ntiles;
         return;
     }
-    if (gam != gam || del != del) {
-        st->status = ST_NAN;
-        st->done = 1;
-        return;
-    }
-    st->betak_min = fabs(gam);
-    st->beta = 0.0;
-    st->gamma = gam;
-    if (fabs(del) < fabs(gam) * (double)1.e-10f) {
-        st->degenerate = 1;
-        st->alpha = 0.0;
-        return;
-    }
-    st->alpha = gam / del;
+    const int x = b & 7, k = b >> 3;
+    const int q = g.ntiles >> 3, rem = g.ntiles & 7;
+    const int start = x * q + (x < rem ? x : rem);
+    first = start + k;
+    stride = G >> 3;
+    end = start + q + (x < rem ? 1 : 0);
 }
 
-// After sweep n: step/extrapolation test of iteration n (:283-297), then
-// the prologue of iteration n+1: beta, divergence test (:232-240),
-// degenerate test (:262-266) and alpha (:270).  With u = Z r_{n+1},
-// w = A u, p = p_n and s = A p_n, the reference's s_dot_As of iteration
-// n+1 is (p_{n+1}, A p_{n+1}) = d1 + beta d2 + beta^2 d3 with
-// d1 = (u, w), d2 = (u, s) + (p, w), d3 = (p, s): exact for the
-// non-symmetric operator of spherical grids, where the symmetric
-// shortcut d1 - beta gamma / alpha would change the iteration.
-__device__ void cg_after_sweep(CgState* st, double gam, double d1, double d2, double d3,
-                               double pmax, double rr) {
-    const int n = st->n + 1;
-    const double alpha = st->alpha;
-    st->rr = rr;
-    st->pmax = pmax;
-    if (st->degenerate) {
-        st->est_error = 100.0 * pmax;
-        st->n = n;
-        st->status = ST_CONVERGED;
-        st->done = 1;
+// Fold the previous kernel's partials (slot-major, G per slot) in a fixed
+// order.  Every workgroup does this and gets the bit-identical result, so
+// no grid-wide synchronisation (and no cross-XCD cache maintenance) is
+// needed inside a kernel.  Result valid in thread 0.
+template <int NR>
+__device__ __forceinline__ void fold_prev(const double* __restrict__ part, int G, unsigned maxmask,
+                                          double (&v)[NR], double* sh) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) v[k] = 0.0;
+    for (int b0 = threadIdx.x; b0 < G; b0 += 4 * NT) {
+        double x[4][NR];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int b = b0 + q * NT;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) x[q][k] = (b < G) ? part[(size_t)k * G + b] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+                v[k] = ((maxmask >> k) & 1u) ? fmax(v[k], x[q][k]) : v[k] + x[q][k];
+    }
+    block_reduce<NR>(v, maxmask, sh);
+}
+
+template <int NR>
+__device__ __forceinline__ void write_partials(const double (&v)[NR], double* __restrict__ part) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) part[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CG state transitions, evaluated identically by every workgroup from the
+// folded partials.  Slots: setup {gamma, d1, max|u|, |r|^2, |b|^2, -};
+// sweep and refresh {gamma, d1, max|p|, |r|^2, d2, d3}.
+// ---------------------------------------------------------------------------
+
+// Prologue of iteration m = n+1 given gamma = (r, Z r) and the pieces of
+// (p_{n+1}, A p_{n+1}) = d1 + beta d2 + beta^2 d3: beta, the divergence test
+// (solve_streamfct.f90:232-240), the degenerate test (:262-266), alpha (:270).
+__device__ void cg_prologue(CgState& s, double gam, double d1, double d2, double d3) {
+    const int m = s.n + 1;
+    const double beta = gam / s.gamma;
+    if (m > 2) {
+        s.betak_min = fmin(s.betak_min, fabs(gam));
+        if (fabs(gam) > 100.0 * s.betak_min) {
+            s.n = m;
+            s.status = ST_DIVERGED;
+            s.done = 1;
+            return;
+        }
+    }
+    const double pAp = d1 + beta * d2 + beta * beta * d3;
+    s.beta = beta;
+    s.gamma = gam;
+    if (fabs(pAp) < fabs(gam) * (double)1.e-10f) {
+        s.degenerate = 1;
+        s.alpha = 0.0;
+        return;
+    }
+    s.alpha = gam / pAp;
+}
+
+// After the setup sweep: the reference's entry test (:206-213) and the
+// prologue of iteration 1 (ss = 0, so p_1 = Z r_0 and (p, A p) = d1).
+__device__ void cg_from_setup(CgState& s, const double* red) {
+    const double gam = red[0], d1 = red[1], zmax = red[2], rr = red[3];
+    double bb = s.bb_fixed > 0 ? s.bb_fixed : red[4];
+    s.rr0 = rr;
+    s.rr = rr;
+    if (bb == 0.0) bb = rr;
+    s.bb = bb;
+    s.zmax0 = zmax;
+    s.n = 0;
+    s.degenerate = 0;
+    s.est_error = 100.0 * zmax;
+    const bool conv = (s.criterion == CRIT_REFERENCE) ? (100.0 * zmax < s.eps)
+                                                       : (rr <= s.rtarget2 * bb);
+    if (conv) {
+        s.status = ST_CONVERGED;
+        s.done = 1;
+        return;
+    }
+    if (gam != gam || d1 != d1) {
+        s.status = ST_NAN;
+        s.done = 1;
+        return;
+    }
+    s.betak_min = fabs(gam);
+    s.beta = 0.0;
+    s.gamma = gam;
+    if (fabs(d1) < fabs(gam) * (double)1.e-10f) {
+        s.degenerate = 1;
+        s.alpha = 0.0;
+        return;
+    }
+    s.alpha = gam / d1;
+}
+
+// After sweep n: the step / extrapolation test of iteration n (:283-297),
+// then the prologue of iteration n+1.
+__device__ void cg_from_sweep(CgState& s, const double* red) {
+    const double gam = red[0], d1 = red[1], pmax = red[2], rr = red[3], d2 = red[4], d3 = red[5];
+    const int n = s.n + 1;
+    const double alpha = s.alpha;
+    s.rr = rr;
+    s.pmax = pmax;
+    if (s.degenerate) {
+        s.est_error = 100.0 * pmax;
+        s.n = n;
+        s.status = ST_CONVERGED;
+        s.done = 1;
         return;
     }
     const double step = fabs(alpha) * pmax;
     bool conv = false;
     if (n == 1) {
-        st->step1 = step;
-        st->est_error = step;
-        conv = step < st->eps;
-    } else if (step < st->eps) {
-        const double rate = exp(log(step / st->step1) / (double)(n - 1));
-        st->est_error = step * rate / (1.0 - rate);
-        conv = st->est_error < st->eps;
+        s.step1 = step;
+        s.est_error = step;
+        conv = step < s.eps;
+    } else if (step < s.eps) {
+        const double rate = exp(log(step / s.step1) / (double)(n - 1));
+        s.est_error = step * rate / (1.0 - rate);
+        conv = s.est_error < s.eps;
     }
-    if (st->criterion == CRIT_RELRES) conv = (rr <= st->rtarget2 * st->bb);
+    if (s.criterion == CRIT_RELRES) conv = (rr <= s.rtarget2 * s.bb);
+    s.n = n;
     if (conv) {
-        st->n = n;
-        st->status = ST_CONVERGED;
-        st->done = 1;
+        s.status = ST_CONVERGED;
+        s.done = 1;
         return;
     }
-    if (n >= st->max_it) {
-        st->n = st->max_it + 1;
-        st->status = ST_MAXIT;
-        st->done = 1;
+    if (n >= s.max_it) {
+        s.status = ST_MAXIT;
+        s.done = 1;
         return;
     }
     if (gam != gam || d1 != d1 || rr != rr) {
-        st->n = n;
-        st->status = ST_NAN;
-        st->done = 1;
+        s.status = ST_NAN;
+        s.done = 1;
         return;
     }
-    const int m = n + 1;
-    const double beta = gam / st->gamma;
-    if (m > 2) {
-        st->betak_min = fmin(st->betak_min, fabs(gam));
-        if (fabs(gam) > 100.0 * st->betak_min) {
-            st->n = m;
-            st->status = ST_DIVERGED;
-            st->done = 1;
-            return;
+    cg_prologue(s, gam, d1, d2, d3);
+}
+
+// After a residual replacement: the recursively updated residual was
+// replaced by the true one; continue with the prologue of iteration n+1.
+__device__ void cg_from_refresh(CgState& s, const double* red) {
+    const double gam = red[0], d1 = red[1], rr = red[3], d2 = red[4], d3 = red[5];
+    s.rr = rr;
+    if (gam != gam || d1 != d1 || rr != rr) {
+        s.status = ST_NAN;
+        s.done = 1;
+        return;
+    }
+    cg_prologue(s, gam, d1, d2, d3);
+}
+
+// Every workgroup: fold the previous partials, advance the state, publish it
+// (workgroup 0) and return it in LDS.
+__device__ __forceinline__ void advance_state(const CgState* __restrict__ st_in,
+                                              CgState* __restrict__ st_out,
+                                              const double* __restrict__ part_in, int next_phase,
+                                              CgState& sst, double* sRed) {
+    const int done_in = st_in->done;
+    double red[NRED];
+    if (!done_in) fold_prev<NRED>(part_in, gridDim.x, 1u << 2, red, sRed);
+    if (threadIdx.x == 0) {
+        CgState s = *st_in;
+        if (!done_in) {
+            if (s.phase == PHASE_SETUP) cg_from_setup(s, red);
+            else if (s.phase == PHASE_REFRESH) cg_from_refresh(s, red);
+            else cg_from_sweep(s, red);
+            s.phase = next_phase;
         }
+        sst = s;
+        if (blockIdx.x == 0) *st_out = s;
     }
-    const double pAp = d1 + beta * d2 + beta * beta * d3;
-    st->beta = beta;
-    st->gamma = gam;
-    st->n = n;
-    if (fabs(pAp) < fabs(gam) * (double)1.e-10f) {
-        st->degenerate = 1;
-        st->alpha = 0.0;
-        return;
-    }
-    st->alpha = gam / pAp;
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
-// Setup sweep: r0 = b - A x0, u0 = Z r0, w0 = A u0.
-// TB is the type of the right-hand side (double for the FP64 path; the FP32
-// inner solve reads the FP64 residual and scales it on the fly).  x == null
-// means a zero first guess; then x_out (if set) is zeroed on the interior.
+// Setup sweep: r0 = b - A x0, u0 = Z r0, w0 = A u0; partials for
+// cg_from_setup.  TB is the type of the right-hand side (the FP32 inner
+// solve reads the FP64 residual and scales it on the fly).  x == null means
+// a zero first guess; then x_zero_out (if set) is zeroed on the interior.
 // ---------------------------------------------------------------------------
 template <class T, class TB>
 __global__ __launch_bounds__(NT) void k_cg_setup(
     Geom g, const T* __restrict__ E, const T* __restrict__ N, const T* __restrict__ rx,
     const T* __restrict__ ry, const uint8_t* __restrict__ act, const TB* __restrict__ b,
     double bscale, const T* __restrict__ x, T* __restrict__ x_zero_out, T* __restrict__ u_out,
-    T* __restrict__ p_out, double* __restrict__ part, unsigned* __restrict__ counter,
-    CgState* __restrict__ st) {
+    T* __restrict__ p_out, double* __restrict__ part_out) {
     __shared__ T sX[HX * HY], sU[HX * HY], sE[HX * HY], sN[HX * HY];
     __shared__ uint8_t sA[HX * HY];
     __shared__ T sRX[HX], sRY[HY];
     __shared__ double sRed[NRED * NW];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
-    const int c0 = tx * TX, r0 = ty * TY;
-    for (int l = threadIdx.x; l < HX * HY; l += NT) {
-        const int gg = l / HX, f = l - gg * HX;
-        const int r = r0 + gg, c = map_col(c0 + f, g);
-        T xv = 0, bv = 0, e = 0, n = 0;
-        uint8_t a = 0;
-        if (in_array(r, c, g)) {
-            const size_t o = (size_t)r * g.pitch + c;
-            xv = x ? x[o] : T(0);
-            bv = (T)((double)b[o] * bscale);
-            e = E[o];
-            n = N[o];
-            a = act[o];
-        }
-        sX[l] = xv;
-        sU[l] = bv;   // holds b until overwritten by u0 on radius-1 points
-        sE[l] = e;
-        sN[l] = n;
-        sA[l] = a;
-    }
-    load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
-    __syncthreads();
     constexpr int W1 = HX - 2, H1 = HY - 2;
-    // radius 1: u0 = Z (b - A x)
-    T uloc[(W1 * H1 + NT - 1) / NT];
-#pragma unroll
-    for (int k = 0; k < (W1 * H1 + NT - 1) / NT; ++k) {
-        const int l = threadIdx.x + k * NT;
-        uloc[k] = 0;
-        if (l < W1 * H1) {
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
+    int t0, ts, te;
+    tile_schedule(g, t0, ts, te);
+    for (int tile = t0; tile < te; tile += ts) {
+        const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
+        for (int l = threadIdx.x; l < HX * HY; l += NT) {
+            const int gg = l / HX, f = l - gg * HX;
+            const int r = r0 + gg, c = map_col(c0 + f, g);
+            T xv = 0, bv = 0, e = 0, n = 0;
+            uint8_t a = 0;
+            if (in_array(r, c, g)) {
+                const size_t o = (size_t)r * g.pitch + c;
+                xv = x ? x[o] : T(0);
+                bv = (T)((double)b[o] * bscale);
+                e = E[o];
+                n = N[o];
+                a = act[o];
+            }
+            sX[l] = xv;
+            sU[l] = bv;
+            sE[l] = e;
+            sN[l] = n;
+            sA[l] = a;
+        }
+        load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
+        __syncthreads();
+        // radius 1: u0 = Z (b - A x), in place over b (each point reads only
+        // its own b); |b|^2 on the tile's own active points first
+        for (int l = threadIdx.x; l < W1 * H1; l += NT) {
             const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
             const int idx = gg * HX + f;
             T cc;
             const T ax = stencil5(sX, sE, sN, idx, sRX[f], sRY[gg], cc);
-            const T res = sU[idx] - ax;
-            uloc[k] = sA[idx] ? (T(1) / cc) * res : T(0);
+            const T bv = sU[idx];
+            const bool own = gg >= 2 && gg < HY - 2 && f >= 2 && f < HX - 2 &&
+                             (r0 + gg) <= g.ny + 1 && (c0 + f) <= g.nx + 1;
+            if (own && sA[idx]) acc[4] += (double)bv * (double)bv;
+            sU[idx] = sA[idx] ? (T(1) / cc) * (bv - ax) : T(0);
         }
-    }
-    // keep b at radius 0 for |b|^2 and r0 for |r|^2: compute them before
-    // overwriting (radius-0 points are a subset of radius-1 points).
-    double acc[NRED] = {0, 0, 0, 0, 0, 0};
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < (W1 * H1 + NT - 1) / NT; ++k) {
-        const int l = threadIdx.x + k * NT;
-        if (l < W1 * H1) {
-            const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
-            const int idx = gg * HX + f;
-            const bool inner = gg >= 2 && gg < HY - 2 && f >= 2 && f < HX - 2 &&
-                               (r0 + gg) <= g.ny + 1 && (c0 + f) <= g.nx + 1;
-            if (inner && sA[idx]) {
-                const double bv = (double)sU[idx];
-                acc[4] += bv * bv;
-            }
-            sU[idx] = uloc[k];
-        }
-    }
-    __syncthreads();
-    for (int k = 0; k < (TX * TY) / NT; ++k) {
-        const int l = threadIdx.x + k * NT;
-        const int gg = 2 + l / TX, f = 2 + l % TX;
-        const int r = r0 + gg, c = c0 + f;
-        if (r <= g.ny + 1 && c <= g.nx + 1) {
-            const int idx = gg * HX + f;
-            T cc;
-            const T w = stencil5(sU, sE, sN, idx, sRX[f], sRY[gg], cc);
-            const T u0 = sU[idx];
-            const size_t o = (size_t)r * g.pitch + c;
-            u_out[o] = u0;
-            p_out[o] = T(0);
-            if (x_zero_out) x_zero_out[o] = T(0);
-            const double ud = (double)u0, ccd = (double)cc;
-            acc[0] += ud * ud * ccd;
-            acc[1] += ud * (double)w;
-            acc[2] = fmax(acc[2], fabs(ud));
-            if (sA[idx]) {
-                const double rv = ud * ccd;
-                acc[3] += rv * rv;
+        __syncthreads();
+        for (int k = 0; k < (TX * TY) / NT; ++k) {
+            const int l = threadIdx.x + k * NT;
+            const int gg = 2 + l / TX, f = 2 + l % TX;
+            const int r = r0 + gg, c = c0 + f;
+            if (r <= g.ny + 1 && c <= g.nx + 1) {
+                const int idx = gg * HX + f;
+                T cc;
+                const T w = stencil5(sU, sE, sN, idx, sRX[f], sRY[gg], cc);
+                const T u0 = sU[idx];
+                const size_t o = (size_t)r * g.pitch + c;
+                u_out[o] = u0;
+                p_out[o] = T(0);
+                if (x_zero_out) x_zero_out[o] = T(0);
+                const double ud = (double)u0, ccd = (double)cc;
+                acc[0] += ud * ud * ccd;
+                acc[1] += ud * (double)w;
+                acc[2] = fmax(acc[2], fabs(ud));
+                if (sA[idx]) {
+                    const double rv = ud * ccd;
+                    acc[3] += rv * rv;
+                }
             }
         }
+        __syncthreads();
     }
     block_reduce<NRED>(acc, 1u << 2, sRed);
-    if (publish_partials<NRED>(acc, part, counter, blockIdx.x, gridDim.x)) {
-        fold_partials<NRED>(acc, 1u << 2, part, gridDim.x, sRed);
-        if (threadIdx.x == 0) {
-            st->done = 0;
-            st->status = ST_RUNNING;
-            cg_after_setup(st, acc[0], acc[1], acc[2], acc[3], acc[4]);
-            *counter = 0u;
-        }
-    }
+    write_partials<NRED>(acc, part_out);
 }
 
 // ---------------------------------------------------------------------------
-// One fused PCG iteration.
+// One fused PCG iteration.  u/p are double-buffered by iteration parity:
+// after n iterations u_n and p_{n-1} live in buffer n & 1.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(NT) void k_cg_sweep(
     Geom g, const T* __restrict__ E, const T* __restrict__ N, const T* __restrict__ rx,
-    const T* __restrict__ ry, const uint8_t* __restrict__ act, T* __restrict__ x,
-    const T* __restrict__ u_in, const T* __restrict__ p_in, T* __restrict__ u_out,
-    T* __restrict__ p_out, double* __restrict__ part, unsigned* __restrict__ counter,
-    CgState* __restrict__ st) {
-    if (st->done) return;
-    const T alpha = (T)st->alpha, beta = (T)st->beta;
+    const T* __restrict__ ry, const uint8_t* __restrict__ act, T* __restrict__ x, T* ub0, T* ub1,
+    T* pb0, T* pb1, const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
+    const double* __restrict__ part_in, double* __restrict__ part_out) {
     __shared__ T sU[HX * HY], sP[HX * HY], sE[HX * HY], sN[HX * HY];
     __shared__ uint8_t sA[HX * HY];
     __shared__ T sRX[HX], sRY[HY];
     __shared__ double sRed[NRED * NW];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
-    const int c0 = tx * TX, r0 = ty * TY;
-    // frame load fused with p_k = u_k + beta p_{k-1} (radius 2)
-    for (int l = threadIdx.x; l < HX * HY; l += NT) {
-        const int gg = l / HX, f = l - gg * HX;
-        const int r = r0 + gg, c = map_col(c0 + f, g);
-        T uu = 0, pp = 0, e = 0, n = 0;
-        uint8_t a = 0;
-        if (in_array(r, c, g)) {
-            const size_t o = (size_t)r * g.pitch + c;
-            uu = u_in[o];
-            pp = p_in[o];
-            e = E[o];
-            n = N[o];
-            a = act[o];
-        }
-        sU[l] = uu;
-        sP[l] = uu + beta * pp;
-        sE[l] = e;
-        sN[l] = n;
-        sA[l] = a;
-    }
-    load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
-    __syncthreads();
-    // radius 1: u_{k+1} = u_k - alpha Z (A p_k)
+    __shared__ CgState sst;
+    advance_state(st_in, st_out, part_in, PHASE_SWEEP, sst, sRed);
+    if (sst.done) return;
+    const T alpha = (T)sst.alpha, beta = (T)sst.beta;
+    const int odd = sst.n & 1;
+    const T* __restrict__ u_in = odd ? ub1 : ub0;
+    const T* __restrict__ p_in = odd ? pb1 : pb0;
+    T* __restrict__ u_out = odd ? ub0 : ub1;
+    T* __restrict__ p_out = odd ? pb0 : pb1;
     constexpr int W1 = HX - 2, H1 = HY - 2;
-    for (int l = threadIdx.x; l < W1 * H1; l += NT) {
-        const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
-        const int idx = gg * HX + f;
-        if (sA[idx]) {
-            T cc;
-            const T s = stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
-            sU[idx] = sU[idx] - alpha * ((T(1) / cc) * s);
-        }
-    }
-    __syncthreads();
-    // radius 0: x update, w = A u_{k+1}, s = A p_k, reductions
     double acc[NRED] = {0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < (TX * TY) / NT; ++k) {
-        const int l = threadIdx.x + k * NT;
-        const int gg = 2 + l / TX, f = 2 + l % TX;
-        const int r = r0 + gg, c = c0 + f;
-        if (r <= g.ny + 1 && c <= g.nx + 1) {
-            const int idx = gg * HX + f;
-            T cc;
-            const T w = stencil5(sU, sE, sN, idx, sRX[f], sRY[gg], cc);
-            const T sv = stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
-            const T un = sU[idx], pn = sP[idx];
-            const size_t o = (size_t)r * g.pitch + c;
-            x[o] = x[o] + alpha * pn;
-            u_out[o] = un;
-            p_out[o] = pn;
-            const double ud = (double)un, pd = (double)pn, ccd = (double)cc;
-            const double wd = (double)w, sd = (double)sv;
-            acc[0] += ud * ud * ccd;
-            acc[1] += ud * wd;
-            acc[2] = fmax(acc[2], fabs(pd));
-            if (sA[idx]) {
-                const double rv = ud * ccd;
-                acc[3] += rv * rv;
+    int t0, ts, te;
+    tile_schedule(g, t0, ts, te);
+    for (int tile = t0; tile < te; tile += ts) {
+        const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
+        // frame load fused with p_k = u_k + beta p_{k-1} (radius 2)
+        for (int l = threadIdx.x; l < HX * HY; l += NT) {
+            const int gg = l / HX, f = l - gg * HX;
+            const int r = r0 + gg, c = map_col(c0 + f, g);
+            T uu = 0, pp = 0, e = 0, n = 0;
+            uint8_t a = 0;
+            if (in_array(r, c, g)) {
+                const size_t o = (size_t)r * g.pitch + c;
+                uu = u_in[o];
+                pp = p_in[o];
+                e = E[o];
+                n = N[o];
+                a = act[o];
             }
-            acc[4] += ud * sd + pd * wd;
-            acc[5] += pd * sd;
+            sU[l] = uu;
+            sP[l] = uu + beta * pp;
+            sE[l] = e;
+            sN[l] = n;
+            sA[l] = a;
         }
+        load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
+        __syncthreads();
+        // radius 1: u_{k+1} = u_k - alpha Z (A p_k)
+        for (int l = threadIdx.x; l < W1 * H1; l += NT) {
+            const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
+            const int idx = gg * HX + f;
+            if (sA[idx]) {
+                T cc;
+                const T s = stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
+                sU[idx] = sU[idx] - alpha * ((T(1) / cc) * s);
+            }
+        }
+        __syncthreads();
+        // radius 0: x update, w = A u_{k+1}, s = A p_k, reductions
+        for (int k = 0; k < (TX * TY) / NT; ++k) {
+            const int l = threadIdx.x + k * NT;
+            const int gg = 2 + l / TX, f = 2 + l % TX;
+            const int r = r0 + gg, c = c0 + f;
+            if (r <= g.ny + 1 && c <= g.nx + 1) {
+                const int idx = gg * HX + f;
+                T cc;
+                const T w = stencil5(sU, sE, sN, idx, sRX[f], sRY[gg], cc);
+                const T sv = stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
+                const T un = sU[idx], pn = sP[idx];
+                const size_t o = (size_t)r * g.pitch + c;
+                x[o] = x[o] + alpha * pn;
+                u_out[o] = un;
+                p_out[o] = pn;
+                const double ud = (double)un, pd = (double)pn, ccd = (double)cc;
+                const double wd = (double)w, sd = (double)sv;
+                acc[0] += ud * ud * ccd;
+                acc[1] += ud * wd;
+                acc[2] = fmax(acc[2], fabs(pd));
+                if (sA[idx]) {
+                    const double rv = ud * ccd;
+                    acc[3] += rv * rv;
+                }
+                acc[4] += ud * sd + pd * wd;
+                acc[5] += pd * sd;
+            }
+        }
+        __syncthreads();
     }
     block_reduce<NRED>(acc, 1u << 2, sRed);
-    if (publish_partials<NRED>(acc, part, counter, blockIdx.x, gridDim.x)) {
-        fold_partials<NRED>(acc, 1u << 2, part, gridDim.x, sRed);
-        if (threadIdx.x == 0) {
-            cg_after_sweep(st, acc[0], acc[1], acc[4], acc[5], acc[2], acc[3]);
-            *counter = 0u;
-        }
+    write_partials<NRED>(acc, part_out);
+}
+
+// ---------------------------------------------------------------------------
+// Residual replacement: u_n = Z r_true * rscale (r_true from k_residual,
+// FP64), p_{n-1} kept; partials for cg_from_refresh.  If the true residual
+// already meets the target (rr_true <= rr_thresh) the solve is finished.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(NT) void k_cg_refresh(
+    Geom g, const T* __restrict__ E, const T* __restrict__ N, const T* __restrict__ rx,
+    const T* __restrict__ ry, const uint8_t* __restrict__ act, const double* __restrict__ r_true,
+    double rscale, T* ub0, T* ub1, const T* pb0, const T* pb1, const double* __restrict__ true_res,
+    double rr_thresh, int new_max_it, const CgState* __restrict__ st_in,
+    CgState* __restrict__ st_out, double* __restrict__ part_out) {
+    __shared__ T sU[HX * HY], sP[HX * HY], sE[HX * HY], sN[HX * HY];
+    __shared__ uint8_t sA[HX * HY];
+    __shared__ T sRX[HX], sRY[HY];
+    __shared__ double sRed[NRED * NW];
+    const bool finished = true_res[0] <= rr_thresh;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        CgState s = *st_in;
+        s.phase = PHASE_REFRESH;
+        s.max_it = new_max_it;
+        s.degenerate = 0;
+        s.done = finished ? 1 : 0;
+        s.status = finished ? ST_CONVERGED : ST_RUNNING;
+        *st_out = s;
     }
+    if (finished) return;
+    const int odd = st_in->n & 1;
+    T* __restrict__ u_out = odd ? ub1 : ub0;
+    const T* __restrict__ p_in = odd ? pb1 : pb0;
+    constexpr int W1 = HX - 2, H1 = HY - 2;
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
+    int t0, ts, te;
+    tile_schedule(g, t0, ts, te);
+    for (int tile = t0; tile < te; tile += ts) {
+        const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
+        for (int l = threadIdx.x; l < HX * HY; l += NT) {
+            const int gg = l / HX, f = l - gg * HX;
+            const int r = r0 + gg, c = map_col(c0 + f, g);
+            T rv = 0, pp = 0, e = 0, n = 0;
+            uint8_t a = 0;
+            if (in_array(r, c, g)) {
+                const size_t o = (size_t)r * g.pitch + c;
+                rv = (T)(r_true[o] * rscale);
+                pp = p_in[o];
+                e = E[o];
+                n = N[o];
+                a = act[o];
+            }
+            sU[l] = rv;
+            sP[l] = pp;
+            sE[l] = e;
+            sN[l] = n;
+            sA[l] = a;
+        }
+        load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
+        __syncthreads();
+        // radius 1: u = Z r
+        for (int l = threadIdx.x; l < W1 * H1; l += NT) {
+            const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
+            const int idx = gg * HX + f;
+            T cc;
+            (void)stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
+            sU[idx] = sA[idx] ? (T(1) / cc) * sU[idx] : T(0);
+        }
+        __syncthreads();
+        for (int k = 0; k < (TX * TY) / NT; ++k) {
+            const int l = threadIdx.x + k * NT;
+            const int gg = 2 + l / TX, f = 2 + l % TX;
+            const int r = r0 + gg, c = c0 + f;
+            if (r <= g.ny + 1 && c <= g.nx + 1) {
+                const int idx = gg * HX + f;
+                T cc;
+                const T w = stencil5(sU, sE, sN, idx, sRX[f], sRY[gg], cc);
+                const T sv = stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
+                const T un = sU[idx], pn = sP[idx];
+                u_out[(size_t)r * g.pitch + c] = un;
+                const double ud = (double)un, pd = (double)pn, ccd = (double)cc;
+                const double wd = (double)w, sd = (double)sv;
+                acc[0] += ud * ud * ccd;
+                acc[1] += ud * wd;
+                acc[2] = fmax(acc[2], fabs(pd));
+                if (sA[idx]) {
+                    const double rr = ud * ccd;
+                    acc[3] += rr * rr;
+                }
+                acc[4] += ud * sd + pd * wd;
+                acc[5] += pd * sd;
+            }
+        }
+        __syncthreads();
+    }
+    block_reduce<NRED>(acc, 1u << 2, sRed);
+    write_partials<NRED>(acc, part_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -525,14 +693,18 @@ __global__ __launch_bounds__(NT) void k_residual(
     }
 }
 
-// x += s * e on the interior (FP64 accumulate of an inner correction).
+// x += s * e on the interior, then e = 0 (FP64 accumulate of an inner
+// FP32 correction).
 template <class T>
-__global__ void k_add_correction(Geom g, double* __restrict__ x, const T* __restrict__ e, double s) {
+__global__ void k_add_correction(Geom g, double* __restrict__ x, T* __restrict__ e, double s) {
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= plane) return;
     const int r = (int)(idx / g.pitch), c = (int)(idx % g.pitch);
-    if (r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) x[idx] = x[idx] + s * (double)e[idx];
+    if (r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
+        x[idx] = x[idx] + s * (double)e[idx];
+        e[idx] = T(0);
+    }
 }
 
 // apply_op with the stored operator: res = A p on the interior, 0 elsewhere.
@@ -723,27 +895,5 @@ __global__ void k_ab_update(size_t n, const double* __restrict__ psi_tau,
     for (int k = 1; k < nisle; ++k) v = v + coef[k] * psin[(size_t)k * n + i];
     psi_taup1[i] = v;
 }
-
-// ---------------------------------------------------------------------------
-// Explicit instantiations used by the host code.
-// ---------------------------------------------------------------------------
-template __global__ void k_convert<float>(size_t, const double*, float*);
-template __global__ void k_cg_setup<double, double>(Geom, const double*, const double*,
-                                                    const double*, const double*, const uint8_t*,
-                                                    const double*, double, const double*, double*,
-                                                    double*, double*, double*, unsigned*, CgState*);
-template __global__ void k_cg_setup<float, double>(Geom, const float*, const float*, const float*,
-                                                   const float*, const uint8_t*, const double*,
-                                                   double, const float*, float*, float*, float*,
-                                                   double*, unsigned*, CgState*);
-template __global__ void k_cg_sweep<double>(Geom, const double*, const double*, const double*,
-                                            const double*, const uint8_t*, double*, const double*,
-                                            const double*, double*, double*, double*, unsigned*,
-                                            CgState*);
-template __global__ void k_cg_sweep<float>(Geom, const float*, const float*, const float*,
-                                           const float*, const uint8_t*, float*, const float*,
-                                           const float*, float*, float*, double*, unsigned*,
-                                           CgState*);
-template __global__ void k_add_correction<float>(Geom, double*, const float*, double);
 
 }  // namespace psi
