@@ -247,7 +247,8 @@ CgState make_init(const psi_opts* o, int criterion, double rtarget, int max_it) 
 // many iterations (needed in FP32, where the two drift apart).
 template <class T>
 int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* forc, double* sol,
-              T* xw, double bscale_in, bool fp64, int refresh_every, psi_info* info) {
+              T* xw, double bscale_in, bool fp64, int refresh_every, double leg_reduction,
+              psi_info* info) {
     const int batch = o->check_every > 0 ? o->check_every : 32;
     const bool relres = fp64 ? (o->criterion == PSI_CRIT_RELRES) : true;
     const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
@@ -276,6 +277,7 @@ int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* f
     const int first_max = refresh_every > 0 ? std::min(refresh_every, abs_max) : abs_max;
     CgState init = make_init(o, relres ? CRIT_RELRES : CRIT_REFERENCE, target, first_max);
     if (!fp64) init.bb_fixed = bb_forc * scale * scale;
+    init.red2 = leg_reduction > 0 ? leg_reduction * leg_reduction : 0.0;
     if (fp64)
         rc = cg_setup<T, double>(c, P, forc, bscale_in, xw, false, init, Ln);
     else
@@ -287,7 +289,7 @@ int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* f
         rc = cg_iterate<T>(c, P, xw, Ln, batch, (long)abs_max + 4, &run);
         if (rc) return rc;
         const CgState& f = run.fin;
-        const bool leg_end = f.status == ST_MAXIT && f.n < abs_max;
+        const bool leg_end = (f.status == ST_MAXIT && f.n < abs_max) || f.status == ST_LEG;
         if (!relres || !(f.status == ST_CONVERGED || leg_end)) break;
         // confirm on the FP64 true residual
         if (!fp64) {
@@ -336,13 +338,15 @@ int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* f
 
 int solve_plain(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
     CgPath<double> P{c->E, c->N, c->rx, c->ry, {c->u64[0], c->u64[1]}, {c->p64[0], c->p64[1]}};
-    return solve_pcg<double>(c, o, P, forc, sol, sol, 1.0, true, 0, info);
+    return solve_pcg<double>(c, o, P, forc, sol, sol, 1.0, true, 0, 0.0, info);
 }
 
 int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
     CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]}};
-    const int every = o->reserved[0] > 0 ? o->reserved[0] : 400;
-    return solve_pcg<float>(c, o, P, forc, sol, c->e32, 1.0, false, every, info);
+    // FP32 legs end after each `inner_reduction` drop of the residual (the
+    // recursive FP32 residual is then replaced by the FP64 true one)
+    const double red = o->inner_reduction > 0 ? o->inner_reduction : 0.05;
+    return solve_pcg<float>(c, o, P, forc, sol, c->e32, 1.0, false, o->reserved[0], red, info);
 }
 
 bool valid_field_ctx(psi_ctx* c) { return c && c->E; }

@@ -31,11 +31,13 @@ struct CgState {
     double betak_min, step1, est_error;
     double rr, rr0, bb, zmax0, pmax;
     double eps, rtarget2, bb_fixed;
+    double rr_ref, red2;           // residual replacement: end the leg once
+                                   // |r|^2 <= red2 * rr_ref (0: never)
     int n, status, done, max_it;
     int criterion, degenerate, phase, pad;
 };
 
-enum : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXIT = 2, ST_DIVERGED = 3, ST_NAN = 4 };
+enum : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXIT = 2, ST_DIVERGED = 3, ST_NAN = 4, ST_LEG = 5 };
 enum : int { PHASE_SETUP = 0, PHASE_SWEEP = 1, PHASE_REFRESH = 2 };
 enum : int { CRIT_REFERENCE = 0, CRIT_RELRES = 1 };
 

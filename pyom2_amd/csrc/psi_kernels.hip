@@ -241,6 +241,7 @@ __device__ void cg_from_setup(CgState& s, const double* red) {
     double bb = s.bb_fixed > 0 ? s.bb_fixed : red[4];
     s.rr0 = rr;
     s.rr = rr;
+    s.rr_ref = rr;
     if (bb == 0.0) bb = rr;
     s.bb = bb;
     s.zmax0 = zmax;
@@ -308,6 +309,11 @@ __device__ void cg_from_sweep(CgState& s, const double* red) {
         s.done = 1;
         return;
     }
+    if (s.red2 > 0 && rr <= s.red2 * s.rr_ref) {   // time to replace the residual
+        s.status = ST_LEG;
+        s.done = 1;
+        return;
+    }
     if (gam != gam || d1 != d1 || rr != rr) {
         s.status = ST_NAN;
         s.done = 1;
@@ -321,11 +327,15 @@ __device__ void cg_from_sweep(CgState& s, const double* red) {
 __device__ void cg_from_refresh(CgState& s, const double* red) {
     const double gam = red[0], d1 = red[1], rr = red[3], d2 = red[4], d3 = red[5];
     s.rr = rr;
+    s.rr_ref = rr;
     if (gam != gam || d1 != d1 || rr != rr) {
         s.status = ST_NAN;
         s.done = 1;
         return;
     }
+    // the true residual may sit above the drifted recursive one: rebase the
+    // divergence test on it
+    s.betak_min = fabs(gam);
     cg_prologue(s, gam, d1, d2, d3);
 }
 
