@@ -450,8 +450,57 @@ __global__ __launch_bounds__(NT) void k_cg_setup(
 
 // ---------------------------------------------------------------------------
 // One fused PCG iteration.  u/p are double-buffered by iteration parity:
-// after n iterations u_n and p_{n-1} live in buffer n & 1.
+// after n iterations u_n and p_{n-1} live in buffer n & 1.  The frame of the
+// next tile is fetched into registers while the current one is computed;
+// the first frame is in flight while the previous partials are folded.
 // ---------------------------------------------------------------------------
+template <class T>
+struct FrameRegs {
+    static constexpr int NL = (HX * HY + NT - 1) / NT;
+    T u[NL], p[NL], e[NL], n[NL];
+    uint8_t a[NL];
+
+    __device__ __forceinline__ void fetch(const Geom& g, int tile, const T* __restrict__ u_in,
+                                          const T* __restrict__ p_in, const T* __restrict__ E,
+                                          const T* __restrict__ N,
+                                          const uint8_t* __restrict__ act) {
+        const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int l = threadIdx.x + q * NT;
+            const int gg = l / HX, f = l - gg * HX;
+            const int r = r0 + gg, c = map_col(c0 + f, g);
+            u[q] = 0;
+            p[q] = 0;
+            e[q] = 0;
+            n[q] = 0;
+            a[q] = 0;
+            if (l < HX * HY && in_array(r, c, g)) {
+                const size_t o = (size_t)r * g.pitch + c;
+                u[q] = u_in[o];
+                p[q] = p_in[o];
+                e[q] = E[o];
+                n[q] = N[o];
+                a[q] = act[o];
+            }
+        }
+    }
+    // frame -> LDS, fused with p_k = u_k + beta p_{k-1}
+    __device__ __forceinline__ void store(T beta, T* sU, T* sP, T* sE, T* sN, uint8_t* sA) const {
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int l = threadIdx.x + q * NT;
+            if (l < HX * HY) {
+                sU[l] = u[q];
+                sP[l] = u[q] + beta * p[q];
+                sE[l] = e[q];
+                sN[l] = n[q];
+                sA[l] = a[q];
+            }
+        }
+    }
+};
+
 template <class T>
 __global__ __launch_bounds__(NT) void k_cg_sweep(
     Geom g, const T* __restrict__ E, const T* __restrict__ N, const T* __restrict__ rx,
@@ -463,42 +512,33 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
     __shared__ T sRX[HX], sRY[HY];
     __shared__ double sRed[NRED * NW];
     __shared__ CgState sst;
-    advance_state(st_in, st_out, part_in, PHASE_SWEEP, sst, sRed);
-    if (sst.done) return;
-    const T alpha = (T)sst.alpha, beta = (T)sst.beta;
-    const int odd = sst.n & 1;
+    // iteration count after this launch's state update (setup -> 0,
+    // sweep -> n+1, refresh -> n): selects the u/p buffers before the fold
+    const int phase_in = st_in->phase, n_in = st_in->n;
+    const int n_pred = st_in->done ? n_in
+                                   : (phase_in == PHASE_SETUP ? 0
+                                                              : (phase_in == PHASE_SWEEP ? n_in + 1
+                                                                                         : n_in));
+    const int odd = n_pred & 1;
     const T* __restrict__ u_in = odd ? ub1 : ub0;
     const T* __restrict__ p_in = odd ? pb1 : pb0;
     T* __restrict__ u_out = odd ? ub0 : ub1;
     T* __restrict__ p_out = odd ? pb0 : pb1;
-    constexpr int W1 = HX - 2, H1 = HY - 2;
-    double acc[NRED] = {0, 0, 0, 0, 0, 0};
     int t0, ts, te;
     tile_schedule(g, t0, ts, te);
+    FrameRegs<T> fr;
+    if (!st_in->done && t0 < te) fr.fetch(g, t0, u_in, p_in, E, N, act);
+    advance_state(st_in, st_out, part_in, PHASE_SWEEP, sst, sRed);
+    if (sst.done) return;
+    const T alpha = (T)sst.alpha, beta = (T)sst.beta;
+    constexpr int W1 = HX - 2, H1 = HY - 2;
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
     for (int tile = t0; tile < te; tile += ts) {
         const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
-        // frame load fused with p_k = u_k + beta p_{k-1} (radius 2)
-        for (int l = threadIdx.x; l < HX * HY; l += NT) {
-            const int gg = l / HX, f = l - gg * HX;
-            const int r = r0 + gg, c = map_col(c0 + f, g);
-            T uu = 0, pp = 0, e = 0, n = 0;
-            uint8_t a = 0;
-            if (in_array(r, c, g)) {
-                const size_t o = (size_t)r * g.pitch + c;
-                uu = u_in[o];
-                pp = p_in[o];
-                e = E[o];
-                n = N[o];
-                a = act[o];
-            }
-            sU[l] = uu;
-            sP[l] = uu + beta * pp;
-            sE[l] = e;
-            sN[l] = n;
-            sA[l] = a;
-        }
+        fr.store(beta, sU, sP, sE, sN, sA);
         load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
         __syncthreads();
+        if (tile + ts < te) fr.fetch(g, tile + ts, u_in, p_in, E, N, act);
         // radius 1: u_{k+1} = u_k - alpha Z (A p_k)
         for (int l = threadIdx.x; l < W1 * H1; l += NT) {
             const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
