@@ -8,7 +8,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpsisolve.so")
+LIB_PATH = os.environ.get("PSI_LIBRARY", os.path.join(HERE, "libpsisolve.so"))
 
 PSI_OK = 0
 PSI_NOT_CONVERGED = 1

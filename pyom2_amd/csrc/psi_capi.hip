@@ -4,6 +4,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -41,7 +42,7 @@ struct psi_ctx {
     double *r64 = nullptr, *tmp64 = nullptr, *tmp64b = nullptr;
     double *part = nullptr, *result = nullptr;
     double* part2 = nullptr;      // solver partials, 2 x NRED x G
-    int G = 0;                    // resident workgroups of the solver kernels
+    int G = 0, G32 = 0;           // resident workgroups of the FP64 / FP32 solver kernels
     unsigned* counter = nullptr;
     CgState* st = nullptr;        // 2 states (double-buffered by launch)
     CgState* h_st = nullptr;      // pinned: [0..1] poll slots, [2] init
@@ -115,16 +116,18 @@ template <class T>
 struct CgPath {
     const T *E, *N, *rx, *ry;
     T *u[2], *p[2];
+    int G;
 };
 
 // Launch bookkeeping of one solve: launch L reads state/partials (L-1) & 1
 // and writes L & 1 (the setup is launch 0 and reads the host-written state 0).
 struct Launches {
     long L = 0;
+    int G = 0;
     CgState* st_in(psi_ctx* c) const { return c->st + ((L - 1) & 1); }
     CgState* st_out(psi_ctx* c) const { return c->st + (L & 1); }
-    double* part_in(psi_ctx* c) const { return c->part2 + ((L - 1) & 1) * (size_t)NRED * c->G; }
-    double* part_out(psi_ctx* c) const { return c->part2 + (L & 1) * (size_t)NRED * c->G; }
+    double* part_in(psi_ctx* c) const { return c->part2 + ((L - 1) & 1) * (size_t)NRED * G; }
+    double* part_out(psi_ctx* c) const { return c->part2 + (L & 1) * (size_t)NRED * G; }
 };
 
 template <class T, class TB>
@@ -133,7 +136,8 @@ int cg_setup(psi_ctx* c, const CgPath<T>& P, const TB* b, double bscale, T* x, b
     c->h_st[2] = init;
     HIPCHK(c, hipMemcpyAsync(c->st, &c->h_st[2], sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     Ln.L = 0;
-    hipLaunchKernelGGL((k_cg_setup<T, TB>), dim3(c->G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
+    Ln.G = P.G;
+    hipLaunchKernelGGL((k_cg_setup<T, TB>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
                        P.rx, P.ry, c->act, b, bscale, zero_guess ? (const T*)nullptr : (const T*)x,
                        zero_guess ? x : (T*)nullptr, P.u[0], P.p[0], Ln.part_out(c));
     Ln.L = 1;
@@ -153,7 +157,7 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
     HIPCHK(c, hipEventRecord(c->ts0, c->stream));
     auto issue = [&]() -> int {
         for (int k = 0; k < batch && Ln.L - L0 < max_launch; ++k, ++Ln.L)
-            hipLaunchKernelGGL((k_cg_sweep<T>), dim3(c->G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
+            hipLaunchKernelGGL((k_cg_sweep<T>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
                                P.rx, P.ry, c->act, x, P.u[0], P.u[1], P.p[0], P.p[1],
                                Ln.st_in(c), Ln.st_out(c), Ln.part_in(c), Ln.part_out(c));
         int r = check_launch(c);
@@ -211,7 +215,7 @@ int residual(psi_ctx* c, const double* b, const double* x, double* r_out, double
 template <class T>
 int cg_refresh(psi_ctx* c, const CgPath<T>& P, double rscale, double rr_thresh, int new_max_it,
                Launches& Ln) {
-    hipLaunchKernelGGL((k_cg_refresh<T>), dim3(c->G), dim3(NT), 0, c->stream, c->g, P.E, P.N, P.rx,
+    hipLaunchKernelGGL((k_cg_refresh<T>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E, P.N, P.rx,
                        P.ry, c->act, c->r64, rscale, P.u[0], P.u[1], (const T*)P.p[0],
                        (const T*)P.p[1], c->result, rr_thresh, new_max_it, Ln.st_in(c),
                        Ln.st_out(c), Ln.part_out(c));
@@ -337,12 +341,13 @@ int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* f
 }
 
 int solve_plain(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
-    CgPath<double> P{c->E, c->N, c->rx, c->ry, {c->u64[0], c->u64[1]}, {c->p64[0], c->p64[1]}};
+    CgPath<double> P{c->E, c->N, c->rx, c->ry, {c->u64[0], c->u64[1]}, {c->p64[0], c->p64[1]}, c->G};
     return solve_pcg<double>(c, o, P, forc, sol, sol, 1.0, true, 0, 0.0, info);
 }
 
 int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
-    CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]}};
+    CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]},
+                    c->G32};
     // FP32 legs end after each `inner_reduction` drop of the residual (the
     // recursive FP32 residual is then replaced by the FP64 true one)
     const double red = o->inner_reduction > 0 ? o->inner_reduction : 0.05;
@@ -450,20 +455,28 @@ int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
         {
             // resident grid: CUs x workgroups per CU of the FP64 sweep (the
             // FP32 kernels fit at least as many), a multiple of 8 (XCDs)
-            int ncu = 0, occ = 0;
+            int ncu = 0, occ64 = 0, occ32 = 0;
             if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
                     hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg_sweep<double>, NT, 0) !=
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ64, k_cg_sweep<double>, NT, 0) !=
+                    hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ32, k_cg_sweep<float>, NT, 0) !=
                     hipSuccess) {
                 rc = fail(c, PSI_HIP_ERROR, "device query failed%s");
                 goto failed;
             }
-            int G = std::max(1, ncu * std::max(occ, 1));
-            if (G > g.ntiles) G = g.ntiles;
-            if (G >= 8) G &= ~7;
-            c->G = G;
+            auto grid_for = [&](int occ) {
+                const char* ov = getenv("PSI_GRID_PER_CU");
+                if (ov && atoi(ov) > 0) occ = std::min(occ, atoi(ov));
+                int G = std::max(1, ncu * std::max(occ, 1));
+                if (G > g.ntiles) G = g.ntiles;
+                if (G >= 8) G &= ~7;
+                return G;
+            };
+            c->G = grid_for(occ64);
+            c->G32 = grid_for(occ32);
         }
-        TRY(dalloc(c, &c->part2, (size_t)2 * NRED * c->G));
+        TRY(dalloc(c, &c->part2, (size_t)2 * NRED * std::max(c->G, c->G32)));
         TRY(dalloc(c, &c->result, 8));
         TRY(dalloc(c, &c->counter, 4));
         TRY(dalloc(c, &c->st, 2));

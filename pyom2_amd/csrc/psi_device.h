@@ -8,8 +8,11 @@
 namespace psi {
 
 // Output tile per workgroup and its LDS frame (2-point halo each side).
+#ifndef PSI_TILE_Y
+#define PSI_TILE_Y 16
+#endif
 constexpr int TX = 64;
-constexpr int TY = 16;
+constexpr int TY = PSI_TILE_Y;
 constexpr int HX = TX + 4;
 constexpr int HY = TY + 4;
 constexpr int NT = 256;            // threads per workgroup (4 wave64)

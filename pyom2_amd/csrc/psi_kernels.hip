@@ -454,6 +454,9 @@ __global__ __launch_bounds__(NT) void k_cg_setup(
 // next tile is fetched into registers while the current one is computed;
 // the first frame is in flight while the previous partials are folded.
 // ---------------------------------------------------------------------------
+#ifndef PSI_PREFETCH
+#define PSI_PREFETCH 1
+#endif
 template <class T>
 struct FrameRegs {
     static constexpr int NL = (HX * HY + NT - 1) / NT;
@@ -535,10 +538,15 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
     double acc[NRED] = {0, 0, 0, 0, 0, 0};
     for (int tile = t0; tile < te; tile += ts) {
         const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
+#if !PSI_PREFETCH
+        if (tile != t0) fr.fetch(g, tile, u_in, p_in, E, N, act);
+#endif
         fr.store(beta, sU, sP, sE, sN, sA);
         load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
         __syncthreads();
+#if PSI_PREFETCH
         if (tile + ts < te) fr.fetch(g, tile + ts, u_in, p_in, E, N, act);
+#endif
         // radius 1: u_{k+1} = u_k - alpha Z (A p_k)
         for (int l = threadIdx.x; l < W1 * H1; l += NT) {
             const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
