@@ -202,6 +202,39 @@ int psi_solve_streamfunction(psi_ctx *ctx, const psi_opts *opts, const double *f
                              int32_t taum1, int32_t tau, int32_t taup1, double dt_mom,
                              double AB_eps, double *forc_out, psi_info *info);
 
+/* ---- flexible CG with the learned preconditioner ---------------------------
+ * No counterpart in the reference.  The host applies the network of psinn.h
+ * to the current residual between psi_fcg_input_scale and psi_fcg_step.
+ * FP32 vectors, FP64 area-weighted inner products (S = W A is symmetric),
+ * FP64 residual replacement in psi_fcg_flush. */
+/* Z (fp32 Jacobi inverse, 0 off active points) and the inverse depth at psi
+ * points, both device (ny+4, nx+4): the network's coefficient inputs. */
+int psi_fcg_prepare(psi_ctx *ctx, float *Z32, float *hinv32);
+/* r = forc - A sol in FP64, scaled to FP32; restart the directions.
+ * *relres = ||r|| / ||forc|| (host sync). */
+int psi_fcg_begin(psi_ctx *ctx, const double *forc, const double *sol, double *relres);
+/* Device scalar 1/rms(Z r) for the network input; returns the FP32 residual
+ * and the scalar's device addresses (no host sync). */
+int psi_fcg_input_scale(psi_ctx *ctx, const float *Z32, const float **r32, const double **scale);
+/* One flexible-CG step with preconditioned residual z (fp32, network layout
+ * H x W, solver grid in its top-left corner); no host sync. */
+int psi_fcg_step(psi_ctx *ctx, const float *znet, int H, int W);
+/* Recursively updated relative residual (host sync). */
+int psi_fcg_read(psi_ctx *ctx, double *relres_recursive);
+/* sol += correction; r = forc - A sol in FP64 (residual replacement);
+ * *relres = true relative residual; directions restart. */
+int psi_fcg_flush(psi_ctx *ctx, const double *forc, double *sol, double *relres);
+
+/* Training support of the learned preconditioner (no reference counterpart):
+ * A-norm cosine loss L = mean_b 1 - (z.Wr)^2 / ((z.Sz)(e.Wr)) of the network
+ * output z (fp32, B x H x W network layout) for residuals R (B FP64 planes,
+ * solver layout) with ER_b = e_b . W r_b; writes dL/dz to gz and L to *loss
+ * (device).  work: 2B device doubles. */
+int psi_nn_anorm_loss(psi_ctx *ctx, const float *znet, int H, int W, int B, const double *R,
+                      const double *ER, double *work, float *gz, double *loss);
+/* Area-weighted interior product a . W b of two device fields (host result). */
+int psi_nn_weighted_dot(psi_ctx *ctx, const double *a, const double *b, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,37 @@ SIGNATURES = [
                                            C.c_void_p, C.c_void_p, _dp, C.c_void_p, _dp, C.c_int32,
                                            C.c_int32, C.c_int32, C.c_double, C.c_double,
                                            C.c_void_p, C.POINTER(PsiInfo)]),
+    ("psi_fcg_prepare", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psi_fcg_begin", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    ("psi_fcg_input_scale", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_void_p)]),
+    ("psi_fcg_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    ("psi_fcg_read", C.c_int, [C.c_void_p, _dp]),
+    ("psi_fcg_flush", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    # learned-preconditioner network kernels (include/psinn.h)
+    ("psinn_mfma_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psinn_conv3x3", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                C.c_void_p, C.c_int, C.c_void_p]),
+    ("psinn_conv3x3_wgrad", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    ("psinn_pack_weights", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p, C.c_void_p]),
+    ("psinn_avgpool2", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p]),
+    ("psinn_sumpool2", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_void_p, C.c_void_p]),
+    ("psinn_grad_combine", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                     C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psinn_expand16", C.c_int, [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]),
+    ("psinn_adam", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_float,
+                             C.c_float, C.c_float, C.c_float, C.c_int, C.c_float, C.c_void_p]),
+    ("psinn_net_input", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                  C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None

@@ -40,6 +40,9 @@ struct psi_ctx {
     double *u64[2] = {nullptr, nullptr}, *p64[2] = {nullptr, nullptr};
     float *u32[2] = {nullptr, nullptr}, *p32[2] = {nullptr, nullptr}, *e32 = nullptr;
     double *r64 = nullptr, *tmp64 = nullptr, *tmp64b = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *pf32 = nullptr;   // flexible CG
+    double* fcg = nullptr;        // flexible-CG device scalars
+    int fcg_restart = 1;
     double *part = nullptr, *result = nullptr;
     double* part2 = nullptr;      // solver partials, 2 x NRED x G
     int G = 0, G32 = 0;           // resident workgroups of the FP64 / FP32 solver kernels
@@ -53,6 +56,240 @@ struct psi_ctx {
     std::vector<void*> allocs;
     std::string err;
 };
+
+
+// ---------------------------------------------------------------------------
+// Flexible CG with a learned preconditioner (no reference counterpart).
+// FP32 vectors on the solver grid, FP64 atomically accumulated inner
+// products with the area weights W = dxu cosu dyu (S = W A is symmetric), FP64
+// residual replacement in psi_fcg_flush.  Device scalars (c->fcg):
+// 0 (z, W q_old)  1 (p, W q)  2 (p, W q)_old  3 (r, W p)  4 |r|^2 active
+// 5 input scale  6 |Z r|^2 active  7 active count  8 scale (r32 = r64*scale)
+// 9 |forc|^2 active.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double area_w(const Geom& g, const double* dxu, const double* cosu,
+                                         const double* dyu, int r, int c) {
+    return dxu[c] * cosu[r] * dyu[r];
+}
+
+__global__ void k_fcg_prepare(Geom g, const double* __restrict__ cC, const uint8_t* __restrict__ act,
+                              const double* __restrict__ hur, const double* __restrict__ hvr,
+                              float* __restrict__ Z32, float* __restrict__ hinv32) {
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= plane) return;
+    const int r = (int)(idx / g.pitch), c = (int)(idx % g.pitch);
+    float z = 0.f, h = 0.f;
+    if (r >= 1 && r <= g.ny + 2 && c >= 1 && c <= g.nx + 2) {
+        if (act[idx]) z = (float)(1.0 / cC[idx]);
+        // inverse depth at the psi point: mean of the four faces around it
+        h = (float)(0.25 * (hur[idx] + hur[idx + g.pitch] + hvr[idx] + hvr[idx + 1]));
+    }
+    Z32[idx] = z;
+    hinv32[idx] = h;
+}
+
+__global__ void k_fcg_begin(Geom g, const double* __restrict__ r64, const uint8_t* __restrict__ act,
+                            const double* __restrict__ sc, float* __restrict__ r32,
+                            float* __restrict__ e32, float* __restrict__ p32) {
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= plane) return;
+    r32[idx] = act[idx] ? (float)(r64[idx] * sc[8]) : 0.f;
+    e32[idx] = 0.f;
+    p32[idx] = 0.f;
+}
+
+__global__ void k_fcg_zr2(Geom g, const float* __restrict__ r32, const float* __restrict__ Z32,
+                          double* __restrict__ sc) {
+    __shared__ double sh[2 * NW];
+    double acc[2] = {0.0, 0.0};
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
+        const double v = (double)Z32[idx] * (double)r32[idx];
+        acc[0] += v * v;
+        acc[1] += Z32[idx] != 0.f ? 1.0 : 0.0;
+    }
+    block_reduce<2>(acc, 0u, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(&sc[6], acc[0]);
+        atomicAdd(&sc[7], acc[1]);
+    }
+}
+
+__global__ void k_fcg_scalar(double* __restrict__ sc, int what) {
+    if (what == 0) {          // finish the input scale, reset its accumulators
+        sc[5] = sc[6] > 0 ? 1.0 / sqrt(sc[6] / fmax(sc[7], 1.0)) : 1.0;
+        sc[6] = 0.0;
+        sc[7] = 0.0;
+    } else {                  // start of a step
+        sc[2] = sc[1];
+        sc[0] = 0.0;
+        sc[1] = 0.0;
+        sc[3] = 0.0;
+        sc[4] = 0.0;
+    }
+}
+
+// z (network layout H x W) -> zs on the solver grid; (zs, W q_old)
+__global__ void k_fcg_scatter(Geom g, const float* __restrict__ znet, int H, int W,
+                              const uint8_t* __restrict__ act, const float* __restrict__ q32,
+                              const double* __restrict__ dxu, const double* __restrict__ cosu,
+                              const double* __restrict__ dyu, float* __restrict__ zs,
+                              double* __restrict__ sc) {
+    __shared__ double sh[NW];
+    double acc[1] = {0.0};
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
+        const int r = (int)(idx / g.pitch), c = (int)(idx % g.pitch);
+        float z = 0.f;
+        if (act[idx] && r < H && c < W) z = znet[(size_t)r * W + c];
+        zs[idx] = z;
+        acc[0] += (double)z * (double)q32[idx] * area_w(g, dxu, cosu, dyu, r, c);
+    }
+    block_reduce<1>(acc, 0u, sh);
+    if (threadIdx.x == 0) atomicAdd(&sc[0], acc[0]);
+}
+
+__global__ void k_fcg_dir(Geom g, const float* __restrict__ zs, float* __restrict__ p32,
+                          const double* __restrict__ sc, int restart) {
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= plane) return;
+    const float beta = restart ? 0.f : (float)(sc[0] / sc[2]);
+    p32[idx] = zs[idx] - beta * p32[idx];
+}
+
+// q = A p (FP32 operator), (p, W q), (r, W p)
+__global__ void k_fcg_apply(Geom g, const float* __restrict__ E, const float* __restrict__ N,
+                            const float* __restrict__ rx, const float* __restrict__ ry,
+                            const float* __restrict__ p, const float* __restrict__ r32,
+                            const double* __restrict__ dxu, const double* __restrict__ cosu,
+                            const double* __restrict__ dyu, float* __restrict__ q,
+                            double* __restrict__ sc) {
+    __shared__ double sh[2 * NW];
+    double acc[2] = {0.0, 0.0};
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const int P = g.pitch;
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
+        const int r = (int)(idx / P), c = (int)(idx % P);
+        float v = 0.f;
+        if (r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
+            const int cl = map_col(c - 1, g), cr = map_col(c + 1, g);
+            const size_t il = (size_t)r * P + cl, ir = (size_t)r * P + cr;
+            const float e = E[idx], w = E[il] * rx[c], n = N[idx], s = N[idx - P] * ry[r];
+            const float cc = (((-e) - w) - n) - s;
+            v = s * p[idx - P] + w * p[il] + cc * p[idx] + e * p[ir] + n * p[idx + P];
+            const double wa = area_w(g, dxu, cosu, dyu, r, c);
+            acc[0] += (double)p[idx] * (double)v * wa;
+            acc[1] += (double)r32[idx] * (double)p[idx] * wa;
+        }
+        q[idx] = v;
+    }
+    block_reduce<2>(acc, 0u, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(&sc[1], acc[0]);
+        atomicAdd(&sc[3], acc[1]);
+    }
+}
+
+__global__ void k_fcg_update(Geom g, const float* __restrict__ p, const float* __restrict__ q,
+                             const uint8_t* __restrict__ act, float* __restrict__ e32,
+                             float* __restrict__ r32, double* __restrict__ sc) {
+    __shared__ double sh[NW];
+    double acc[1] = {0.0};
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const float alpha = (float)(sc[3] / sc[1]);
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
+        if (!act[idx]) continue;
+        e32[idx] += alpha * p[idx];
+        const float rv = r32[idx] - alpha * q[idx];
+        r32[idx] = rv;
+        acc[0] += (double)rv * (double)rv;
+    }
+    block_reduce<1>(acc, 0u, sh);
+    if (threadIdx.x == 0) atomicAdd(&sc[4], acc[0]);
+}
+
+}  // namespace
+
+namespace {
+// A-norm cosine loss of the learned preconditioner.  For sample b with
+// residual r (FP64, solver layout), true error e (through ER = e . W r) and
+// network output z (fp32, network layout H x W, masked to active points):
+//   a = z . W r,  q = z . S z (S = W A),  L = 1 - a^2 / (q ER)
+//   dL/dz = -(2a/(q ER)) W r + (2a^2/(q^2 ER)) S z
+__global__ void k_nn_loss1(Geom g, const float* __restrict__ znet, int H, int W,
+                           const double* __restrict__ R, const uint8_t* __restrict__ act,
+                           const float* __restrict__ E, const float* __restrict__ N,
+                           const float* __restrict__ rx, const float* __restrict__ ry,
+                           const double* __restrict__ dxu, const double* __restrict__ cosu,
+                           const double* __restrict__ dyu, float* __restrict__ Sz,
+                           double* __restrict__ work) {
+    __shared__ double sh[2 * NW];
+    const int b = blockIdx.y;
+    const float* z = znet + (size_t)b * H * W;
+    const double* r = R + (size_t)b * g.pitch * (g.ny + 4);
+    float* out = Sz + (size_t)b * H * W;
+    double acc[2] = {0.0, 0.0};
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const int P = g.pitch;
+    auto zm = [&](size_t idx) -> float {
+        const int rr = (int)(idx / P), cc = (int)(idx % P);
+        return act[idx] ? z[(size_t)rr * W + cc] : 0.f;
+    };
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
+        const int rr = (int)(idx / P), cc = (int)(idx % P);
+        if (!act[idx]) continue;
+        const float e = E[idx], w = E[idx - 1] * rx[cc], n = N[idx], s = N[idx - P] * ry[rr];
+        const float c0 = (((-e) - w) - n) - s;
+        const float zc = zm(idx);
+        const float az = s * zm(idx - P) + w * zm(idx - 1) + c0 * zc + e * zm(idx + 1) + n * zm(idx + P);
+        const double wa = dxu[cc] * cosu[rr] * dyu[rr];
+        const double szv = wa * (double)az;
+        out[(size_t)rr * W + cc] = (float)szv;
+        acc[0] += (double)zc * wa * r[idx];
+        acc[1] += (double)zc * szv;
+    }
+    block_reduce<2>(acc, 0u, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(&work[2 * b], acc[0]);
+        atomicAdd(&work[2 * b + 1], acc[1]);
+    }
+}
+
+__global__ void k_nn_loss2(Geom g, int H, int W, int B, const double* __restrict__ R,
+                           const uint8_t* __restrict__ act, const double* __restrict__ ER,
+                           const double* __restrict__ dxu, const double* __restrict__ cosu,
+                           const double* __restrict__ dyu, const double* __restrict__ work,
+                           float* __restrict__ Sz_to_grad, double* __restrict__ loss) {
+    const int b = blockIdx.y;
+    const double a = work[2 * b], q = work[2 * b + 1], er = ER[b];
+    const double ca = -2.0 * a / (q * er) / B, cs = 2.0 * a * a / (q * q * er) / B;
+    const double* r = R + (size_t)b * g.pitch * (g.ny + 4);
+    float* gz = Sz_to_grad + (size_t)b * H * W;
+    const long n = (long)H * W;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long)gridDim.x * blockDim.x) {
+        const int y = (int)(p / W), x = (int)(p % W);
+        float gv = 0.f;
+        if (y < g.ny + 4 && x < g.pitch) {
+            const size_t idx = (size_t)y * g.pitch + x;
+            if (act[idx]) {
+                const double wa = dxu[x] * cosu[y] * dyu[y];
+                gv = (float)(ca * wa * r[idx] + cs * (double)gz[p]);
+            }
+        }
+        gz[p] = gv;
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        double L = 0.0;
+        for (int k = 0; k < B; ++k) L += 1.0 - work[2 * k] * work[2 * k] / (work[2 * k + 1] * ER[k]);
+        *loss = L / B;
+    }
+}
+}  // namespace
 
 namespace {
 
@@ -449,6 +686,11 @@ int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
         }
         TRY(dalloc(c, &c->e32, c->plane));
         TRY(dalloc(c, &c->r64, c->plane));
+        TRY(dalloc(c, &c->r32, c->plane));
+        TRY(dalloc(c, &c->z32, c->plane));
+        TRY(dalloc(c, &c->q32, c->plane));
+        TRY(dalloc(c, &c->pf32, c->plane));
+        TRY(dalloc(c, &c->fcg, 16));
         TRY(dalloc(c, &c->tmp64, c->plane));
         TRY(dalloc(c, &c->tmp64b, c->plane));
         TRY(dalloc(c, &c->part, (size_t)NRED * (g.ntiles + 1024)));
@@ -871,6 +1113,122 @@ int psi_solve_streamfunction(psi_ctx* c, const psi_opts* o, const double* fpx, c
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return solve_rc;
+}
+
+// ---- flexible CG with the learned preconditioner --------------------------
+int psi_fcg_prepare(psi_ctx* c, float* Z32, float* hinv32) {
+    if (!valid_field_ctx(c) || !Z32 || !hinv32) return PSI_BAD_ARG;
+    hipLaunchKernelGGL(k_fcg_prepare, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->g, c->cC,
+                       c->act, c->hur, c->hvr, Z32, hinv32);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+static int fcg_residual(psi_ctx* c, const double* forc, const double* sol, bool set_scale,
+                        double* relres) {
+    double res[4];
+    int rc = residual(c, forc, sol, c->r64, res);
+    if (rc) return rc;
+    double h[10];
+    HIPCHK(c, hipMemcpy(h, c->fcg, 10 * sizeof(double), hipMemcpyDeviceToHost));
+    if (set_scale) {
+        const double rms = sqrt(res[0] / std::max(res[3], 1.0));
+        h[8] = rms > 0 ? 1.0 / rms : 1.0;
+        h[9] = res[1] > 0 ? res[1] : res[0];
+        HIPCHK(c, hipMemcpy(c->fcg + 8, h + 8, 2 * sizeof(double), hipMemcpyHostToDevice));
+    }
+    *relres = h[9] > 0 ? sqrt(res[0] / h[9]) : 0.0;
+    hipLaunchKernelGGL(k_fcg_begin, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->g, c->r64,
+                       c->act, c->fcg, c->r32, c->e32, c->pf32);
+    c->fcg_restart = 1;
+    return check_launch(c);
+}
+
+int psi_fcg_begin(psi_ctx* c, const double* forc, const double* sol, double* relres) {
+    if (!valid_field_ctx(c) || !forc || !sol || !relres) return PSI_BAD_ARG;
+    HIPCHK(c, hipMemsetAsync(c->fcg, 0, 16 * sizeof(double), c->stream));
+    return fcg_residual(c, forc, sol, true, relres);
+}
+
+int psi_fcg_input_scale(psi_ctx* c, const float* Z32, const float** r32, const double** scale) {
+    if (!valid_field_ctx(c) || !Z32) return PSI_BAD_ARG;
+    hipLaunchKernelGGL(k_fcg_zr2, dim3(1024), dim3(NT), 0, c->stream, c->g, c->r32, Z32, c->fcg);
+    hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(1), 0, c->stream, c->fcg, 0);
+    if (r32) *r32 = c->r32;
+    if (scale) *scale = c->fcg + 5;
+    return check_launch(c);
+}
+
+int psi_fcg_step(psi_ctx* c, const float* znet, int H, int W) {
+    if (!valid_field_ctx(c) || !znet) return PSI_BAD_ARG;
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(1), 0, c->stream, c->fcg, 1);
+    hipLaunchKernelGGL(k_fcg_scatter, dim3(1024), dim3(NT), 0, c->stream, g, znet, H, W, c->act,
+                       c->q32, c->dxu, c->cosu, c->dyu, c->z32, c->fcg);
+    hipLaunchKernelGGL(k_fcg_dir, dim3(nblk(c->plane)), dim3(256), 0, c->stream, g, c->z32, c->pf32,
+                       c->fcg, c->fcg_restart);
+    hipLaunchKernelGGL(k_fcg_apply, dim3(1024), dim3(NT), 0, c->stream, g, c->E32, c->N32, c->rx32,
+                       c->ry32, c->pf32, c->r32, c->dxu, c->cosu, c->dyu, c->q32, c->fcg);
+    hipLaunchKernelGGL(k_fcg_update, dim3(1024), dim3(NT), 0, c->stream, g, c->pf32, c->q32, c->act,
+                       c->e32, c->r32, c->fcg);
+    c->fcg_restart = 0;
+    return check_launch(c);
+}
+
+int psi_fcg_read(psi_ctx* c, double* relres_recursive) {
+    if (!valid_field_ctx(c) || !relres_recursive) return PSI_BAD_ARG;
+    HIPCHK(c, hipMemcpyAsync(c->h_res, c->fcg + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_res + 1, c->fcg + 8, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double bbs = c->h_res[2] * c->h_res[1] * c->h_res[1];
+    *relres_recursive = bbs > 0 ? sqrt(c->h_res[0] / bbs) : 0.0;
+    return PSI_OK;
+}
+
+int psi_fcg_flush(psi_ctx* c, const double* forc, double* sol, double* relres) {
+    if (!valid_field_ctx(c) || !forc || !sol || !relres) return PSI_BAD_ARG;
+    double sc[10];
+    HIPCHK(c, hipMemcpy(sc, c->fcg, 10 * sizeof(double), hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL((k_add_correction<float>), dim3(nblk(c->plane)), dim3(256), 0, c->stream,
+                       c->g, sol, c->e32, 1.0 / sc[8]);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    return fcg_residual(c, forc, sol, false, relres);
+}
+
+// A-norm cosine training loss (see k_nn_loss1); work: 2*B doubles, loss: 1
+// double, both device.  gz (B x H x W fp32) receives dL/dz.
+int psi_nn_anorm_loss(psi_ctx* c, const float* znet, int H, int W, int B, const double* R,
+                      const double* ER, double* work, float* gz, double* loss) {
+    if (!valid_field_ctx(c) || !znet || !R || !ER || !work || !gz || !loss || B < 1 ||
+        H < c->g.ny + 4 || W < c->g.pitch)
+        return PSI_BAD_ARG;
+    HIPCHK(c, hipMemsetAsync(work, 0, 2 * B * sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_nn_loss1, dim3(256, B), dim3(NT), 0, c->stream, c->g, znet, H, W, R, c->act,
+                       c->E32, c->N32, c->rx32, c->ry32, c->dxu, c->cosu, c->dyu, gz, work);
+    hipLaunchKernelGGL(k_nn_loss2, dim3(256, B), dim3(NT), 0, c->stream, c->g, H, W, B, R, c->act, ER,
+                       c->dxu, c->cosu, c->dyu, work, gz, loss);
+    return check_launch(c);
+}
+
+// Products with the area weights for training data: ER_b = e_b . W r_b.
+int psi_nn_weighted_dot(psi_ctx* c, const double* a, const double* b, double* out) {
+    if (!valid_field_ctx(c) || !a || !b || !out) return PSI_BAD_ARG;
+    std::vector<double> ha(c->plane), hb(c->plane);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(ha.data(), a, c->plane * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hb.data(), b, c->plane * 8, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (int r = 2; r <= c->g.ny + 1; ++r)
+        for (int col = 2; col <= c->g.nx + 1; ++col) {
+            const size_t i = (size_t)r * c->g.pitch + col;
+            s += ha[i] * hb[i] * c->h_dxu[col] * c->h_cosu[r] * c->h_dyu[r];
+        }
+    *out = s;
+    return PSI_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,428 @@
+// psi_nn.hip -- kernels of the learned preconditioner: a 4-level
+// encoder-decoder convolutional net (channels 16/32/64/128, 3x3 convolutions
+// as implicit GEMM on BF16 MFMA with FP32 accumulation), its backward pass,
+// Adam, the A-norm training loss and the flexible-CG helpers that use it.
+//
+// Activations are NHWC bfloat16 ([batch][H][W][C], C a multiple of 16).
+// MFMA v_mfma_f32_16x16x16_bf16 (wave64): lane l supplies A[l%16][4(l/16)..+3]
+// and B[4(l/16)..+3][l%16] and receives D[4(l/16)+i][l%16], i = 0..3.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/psinn.h"
+
+namespace psinn {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Lane-layout probe: D = A (16x16) x B (16x16), bf16 in, fp32 out, row-major.
+__global__ void k_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
+    const int l = threadIdx.x;
+    short4_t a, b;
+    for (int j = 0; j < 4; ++j) {
+        a[j] = (short)A[(l & 15) * 16 + 4 * (l >> 4) + j];
+        b[j] = (short)B[(4 * (l >> 4) + j) * 16 + (l & 15)];
+    }
+    float4_t d = {0, 0, 0, 0};
+    d = mfma16(a, b, d);
+    for (int i = 0; i < 4; ++i) D[(4 * (l >> 4) + i) * 16 + (l & 15)] = d[i];
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, stride 1, zero padding, implicit GEMM on MFMA.
+// Input channels are the concatenation [in1 (C1) | in2 (C2)]; in1 may be
+// read through a nearest x2 upsampling (up1: in1 has H/2 x W/2).  Each wave
+// computes 16 consecutive pixels of one row for all 16*NB output channels.
+// Weights are pre-packed per (tap, kb, nb) as 64 lanes x 4 bf16 (B fragment
+// order).  Epilogue: + bias, optional ReLU; output bf16 or fp32 with
+// cout_store channels (<= 16*NB).
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in1, int C1, int up1,
+                                                 const uint16_t* __restrict__ in2, int C2, int H,
+                                                 int W, const uint16_t* __restrict__ wpack,
+                                                 const float* __restrict__ bias, int relu,
+                                                 uint16_t* __restrict__ out16,
+                                                 float* __restrict__ out32, int cout_store) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 16;
+    const int y = blockIdx.y * 4 + wave;
+    const int bimg = blockIdx.z;
+    if (y >= H) return;
+    const int KB = (C1 + C2) >> 4;
+    const int H1 = up1 ? (H >> 1) : H, W1 = up1 ? (W >> 1) : W;
+    const uint16_t* img1 = in1 + (size_t)bimg * H1 * W1 * C1;
+    const uint16_t* img2 = in2 ? in2 + (size_t)bimg * H * W * C2 : nullptr;
+    float4_t acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = (float4_t){0, 0, 0, 0};
+    const int px_l = x0 + (lane & 15);
+    const int cq = 4 * (lane >> 4);
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int yy = y + dy, xx = px_l + dx;
+        const bool inb = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        for (int kb = 0; kb < KB; ++kb) {
+            const int c = kb * 16 + cq;
+            short4_t a = {0, 0, 0, 0};
+            if (inb) {
+                const uint16_t* src;
+                if (c < C1) {
+                    const int ys = up1 ? (yy >> 1) : yy, xs = up1 ? (xx >> 1) : xx;
+                    src = img1 + ((size_t)ys * W1 + xs) * C1 + c;
+                } else {
+                    src = img2 + ((size_t)yy * W + xx) * C2 + (c - C1);
+                }
+                a = *(const short4_t*)src;
+            }
+            const uint16_t* wp = wpack + ((size_t)(tap * KB + kb) * NB) * 256 + lane * 4;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const short4_t b = *(const short4_t*)(wp + nb * 256);
+                acc[nb] = mfma16(a, b, acc[nb]);
+            }
+        }
+    }
+    const int co_l = lane & 15;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        const int co = nb * 16 + co_l;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int px = x0 + cq + i;
+            if (px >= W || co >= cout_store) continue;
+            float v = acc[nb][i] + bv;
+            if (relu) v = fmaxf(v, 0.f);
+            const size_t o = (((size_t)bimg * H + y) * W + px) * cout_store + co;
+            if (out32) out32[o] = v;
+            else out16[o] = f2bf(v);
+        }
+    }
+}
+
+// Weight gradient of k_conv3x3: dW[tap][ci][co] += sum_p in[p + tap][ci] * g[p][co]
+// (MFMA with M = ci, N = co, K = pixels), dbias[co] += sum_p g[p][co].
+// One wave per (tap, kb, nb) tile and pixel chunk; fp32 atomics.
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(
+    const uint16_t* __restrict__ in1, int C1, int up1, const uint16_t* __restrict__ in2, int C2,
+    int B, int H, int W, const uint16_t* __restrict__ g, int COUT, int chunk,
+    float* __restrict__ dW, float* __restrict__ dbias) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int KB = (C1 + C2) >> 4, NB = COUT >> 4;
+    const int tile = blockIdx.x * 4 + wave;
+    if (tile >= 9 * KB * NB) return;
+    const int tap = tile / (KB * NB), kb = (tile / NB) % KB, nb = tile % NB;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int H1 = up1 ? (H >> 1) : H, W1 = up1 ? (W >> 1) : W;
+    const long npix = (long)B * H * W;
+    const long p0 = (long)blockIdx.y * chunk;
+    const long p1 = p0 + chunk < npix ? p0 + chunk : npix;
+    const int ci = kb * 16 + (lane & 15);
+    const int co = nb * 16 + (lane & 15);
+    float4_t acc = {0, 0, 0, 0};
+    float bsum = 0.f;
+    for (long pb = p0; pb < p1; pb += 16) {
+        short4_t a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long p = pb + 4 * (lane >> 4) + j;
+            uint16_t av = 0, gv = 0;
+            if (p < p1) {
+                const int bimg = (int)(p / ((long)H * W));
+                const int rem = (int)(p % ((long)H * W));
+                const int y = rem / W, x = rem % W;
+                gv = g[p * COUT + co];
+                const int yy = y + dy, xx = x + dx;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    if (ci < C1) {
+                        const int ys = up1 ? (yy >> 1) : yy, xs = up1 ? (xx >> 1) : xx;
+                        av = in1[(((size_t)bimg * H1 + ys) * W1 + xs) * C1 + ci];
+                    } else {
+                        av = in2[(((size_t)bimg * H + yy) * W + xx) * C2 + (ci - C1)];
+                    }
+                }
+            }
+            a[j] = (short)av;
+            b[j] = (short)gv;
+            if (tap == 0 && kb == 0) bsum += bf2f(gv);
+        }
+        acc = mfma16(a, b, acc);
+    }
+    // D[m = ci][n = co]: lane holds ci rows 4(l/16)+i, co column l%16
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int cim = kb * 16 + 4 * (lane >> 4) + i;
+        atomicAdd(&dW[((size_t)tap * (C1 + C2) + cim) * COUT + co], acc[i]);
+    }
+    if (tap == 0 && kb == 0 && dbias) {
+        // lanes l, l+16, l+32, l+48 hold partial sums of the same co
+        bsum += __shfl_xor(bsum, 16, 64);
+        bsum += __shfl_xor(bsum, 32, 64);
+        if (lane < 16) atomicAdd(&dbias[co], bsum);
+    }
+}
+
+// Pack fp32 master weights w[tap][ci][co] (CIN x COUT, tap = 3 ky + kx) into
+// B fragments.  transposed = 0: forward convolution (K = ci, N = co).
+// transposed = 1: data-gradient convolution (K = co, N = ci, taps flipped):
+// dIn[p][ci] = sum_tap sum_co dOut[p - d(tap)][co] w[tap][ci][co]
+//            = sum_tap' sum_co dOut[p + d(tap')][co] w[8 - tap'][ci][co].
+// K16/N16: number of 16-blocks along K and N (zero padded).
+__global__ void k_pack_weights(const float* __restrict__ w, int CIN, int COUT, int transposed,
+                               int K16, int N16, uint16_t* __restrict__ pack) {
+    const long total = 9L * K16 * N16 * 256;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int j = (int)(idx & 3), lane = (int)((idx >> 2) & 63);
+    long t = idx >> 8;
+    const int nb = (int)(t % N16);
+    t /= N16;
+    const int kb = (int)(t % K16);
+    const int tap = (int)(t / K16);
+    const int k = kb * 16 + 4 * (lane >> 4) + j, n = nb * 16 + (lane & 15);
+    float v = 0.f;
+    if (!transposed) {
+        if (k < CIN && n < COUT) v = w[((size_t)tap * CIN + k) * COUT + n];
+    } else {
+        if (k < COUT && n < CIN) v = w[((size_t)(8 - tap) * CIN + n) * COUT + k];
+    }
+    pack[idx] = f2bf(v);
+}
+
+// 2x2 average pool, bf16 NHWC.
+__global__ void k_avgpool2(const uint16_t* __restrict__ in, int B, int H, int W, int C,
+                           uint16_t* __restrict__ out) {
+    const long n = (long)B * (H / 2) * (W / 2) * C;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int c = (int)(idx % C);
+    long t = idx / C;
+    const int x = (int)(t % (W / 2));
+    t /= (W / 2);
+    const int y = (int)(t % (H / 2));
+    const int b = (int)(t / (H / 2));
+    const uint16_t* p = in + (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
+    const float s = bf2f(p[0]) + bf2f(p[C]) + bf2f(p[(size_t)W * C]) + bf2f(p[(size_t)W * C + C]);
+    out[idx] = f2bf(0.25f * s);
+}
+
+// Sum of the 4 fine children (gradient of a nearest x2 upsampling) for the
+// channel slice [c0, c0+C) of a fine fp32 tensor with CF channels.
+__global__ void k_sumpool2(const float* __restrict__ fine, int B, int H, int W, int CF, int c0,
+                           int C, float* __restrict__ coarse) {
+    const long n = (long)B * (H / 2) * (W / 2) * C;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int c = (int)(idx % C);
+    long t = idx / C;
+    const int x = (int)(t % (W / 2));
+    t /= (W / 2);
+    const int y = (int)(t % (H / 2));
+    const int b = (int)(t / (H / 2));
+    const float* p = fine + (((size_t)b * H + 2 * y) * W + 2 * x) * CF + c0 + c;
+    coarse[idx] = p[0] + p[CF] + p[(size_t)W * CF] + p[(size_t)W * CF + CF];
+}
+
+// out_bf16[p][c] = (act[p][c] > 0) * (A[p][cA0 + c] + 0.25 * Bc[p/2][c])
+// A: fp32 with CA channels (may be null); Bc: coarse fp32 with C channels
+// (gradient of a 2x2 average pool, may be null); act: bf16 with C channels.
+__global__ void k_grad_combine(const uint16_t* __restrict__ act, int B, int H, int W, int C,
+                               const float* __restrict__ A, int CA, int cA0,
+                               const float* __restrict__ Bc, uint16_t* __restrict__ out) {
+    const long n = (long)B * H * W * C;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int c = (int)(idx % C);
+    const long p = idx / C;
+    float v = 0.f;
+    if (A) v += A[p * CA + cA0 + c];
+    if (Bc) {
+        const int x = (int)(p % W);
+        const long t = p / W;
+        const int y = (int)(t % H);
+        const int b = (int)(t / H);
+        v += 0.25f * Bc[(((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+    }
+    if (!(bf2f(act[idx]) > 0.f)) v = 0.f;
+    out[idx] = f2bf(v);
+}
+
+// fp32 single channel [B][H][W] -> bf16 16-channel (channel 0 = value).
+__global__ void k_expand16(const float* __restrict__ g, long n, uint16_t* __restrict__ out) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * 16) return;
+    out[idx] = (idx & 15) ? (uint16_t)0 : f2bf(g[idx >> 4]);
+}
+
+// Adam with bias correction on fp32 parameters.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, float gscale) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Grid <-> network layout.  The solver grid is (ny+4) x (nx+4) with pitch
+// nx+4; the network grid is H x W (multiples of 8, >= the solver grid) with
+// the solver grid in its top-left corner and zeros elsewhere.
+// ---------------------------------------------------------------------------
+
+// Network input (bf16, 16 channels, 3 used): Z r * s_in (s_in from *scale,
+// the reciprocal rms of Z r over active points), the normalised inverse
+// depth, the active mask.  r and Z in solver layout.
+template <class T>
+__global__ void k_net_input(const T* __restrict__ r, const float* __restrict__ Z,
+                            const float* __restrict__ hinv, int sh, int sw, int H, int W,
+                            const double* __restrict__ scale, uint16_t* __restrict__ out) {
+    const long n = (long)H * W;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int y = (int)(p / W), x = (int)(p % W);
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    if (y < sh && x < sw) {
+        const size_t o = (size_t)y * sw + x;
+        const float z = Z[o];
+        c0 = (float)((double)z * (double)r[o] * scale[0]);
+        c1 = hinv[o];
+        c2 = z != 0.f ? 1.f : 0.f;
+    }
+    uint16_t* q = out + p * 16;
+    q[0] = f2bf(c0);
+    q[1] = f2bf(c1);
+    q[2] = f2bf(c2);
+#pragma unroll
+    for (int k = 3; k < 16; ++k) q[k] = 0;
+}
+
+}  // namespace psinn
+
+using namespace psinn;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
+static int launch_status() { return hipGetLastError() == hipSuccess ? 0 : 6; }
+
+extern "C" {
+
+int psinn_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream) {
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, A, B, D);
+    return launch_status();
+}
+
+int psinn_conv3x3(const uint16_t* in1, int C1, int up1, const uint16_t* in2, int C2, int B, int H,
+                  int W, const uint16_t* wpack, int cout_pad, const float* bias, int relu,
+                  uint16_t* out16, float* out32, int cout_store, void* stream) {
+    if ((C1 & 15) || (C2 & 15) || (cout_pad & 15) || C1 <= 0 || B <= 0 || H <= 0 || W <= 0 ||
+        (up1 && ((H & 1) || (W & 1))) || (!out16 && !out32) || cout_store > cout_pad)
+        return 5;
+    const dim3 grid((W + 15) / 16, (H + 3) / 4, B), blk(256);
+    hipStream_t s = (hipStream_t)stream;
+    switch (cout_pad / 16) {
+        case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+        case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+        case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+        case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+        case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+        case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+        case 12: hipLaunchKernelGGL(k_conv3x3<12>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+        default: return 5;
+    }
+    return launch_status();
+}
+
+int psinn_conv3x3_wgrad(const uint16_t* in1, int C1, int up1, const uint16_t* in2, int C2, int B,
+                        int H, int W, const uint16_t* g, int cout_pad, float* dW, float* dbias,
+                        void* stream) {
+    if ((C1 & 15) || (C2 & 15) || (cout_pad & 15)) return 5;
+    const int KB = (C1 + C2) / 16, NB = cout_pad / 16;
+    const int tiles = 9 * KB * NB;
+    const long npix = (long)B * H * W;
+    const int chunk = 1024;
+    const dim3 grid((tiles + 3) / 4, (unsigned)((npix + chunk - 1) / chunk)), blk(256);
+    hipLaunchKernelGGL(k_conv3x3_wgrad, grid, blk, 0, (hipStream_t)stream, in1, C1, up1, in2, C2, B,
+                       H, W, g, cout_pad, chunk, dW, dbias);
+    return launch_status();
+}
+
+int psinn_pack_weights(const float* w, int cin, int cout, int transposed, int k16, int n16,
+                       uint16_t* pack, void* stream) {
+    const long total = 9L * k16 * n16 * 256;
+    hipLaunchKernelGGL(k_pack_weights, dim3(nb256(total)), dim3(256), 0, (hipStream_t)stream, w,
+                       cin, cout, transposed, k16, n16, pack);
+    return launch_status();
+}
+
+int psinn_avgpool2(const uint16_t* in, int B, int H, int W, int C, uint16_t* out, void* stream) {
+    const long n = (long)B * (H / 2) * (W / 2) * C;
+    hipLaunchKernelGGL(k_avgpool2, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream, in, B, H, W,
+                       C, out);
+    return launch_status();
+}
+
+int psinn_sumpool2(const float* fine, int B, int H, int W, int CF, int c0, int C, float* coarse,
+                   void* stream) {
+    const long n = (long)B * (H / 2) * (W / 2) * C;
+    hipLaunchKernelGGL(k_sumpool2, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream, fine, B, H, W,
+                       CF, c0, C, coarse);
+    return launch_status();
+}
+
+int psinn_grad_combine(const uint16_t* act, int B, int H, int W, int C, const float* A, int CA,
+                       int cA0, const float* Bc, uint16_t* out, void* stream) {
+    const long n = (long)B * H * W * C;
+    hipLaunchKernelGGL(k_grad_combine, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream, act, B, H,
+                       W, C, A, CA, cA0, Bc, out);
+    return launch_status();
+}
+
+int psinn_expand16(const float* g, long n, uint16_t* out, void* stream) {
+    hipLaunchKernelGGL(k_expand16, dim3(nb256(n * 16)), dim3(256), 0, (hipStream_t)stream, g, n, out);
+    return launch_status();
+}
+
+int psinn_adam(float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2,
+               float eps, int step, float gscale, void* stream) {
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, n, lr,
+                       b1, b2, eps, c1, c2, gscale);
+    return launch_status();
+}
+
+int psinn_net_input(const void* r, int r_is_f64, const float* Z, const float* hinv, int sh, int sw,
+                    int H, int W, const double* scale, uint16_t* out, void* stream) {
+    const long n = (long)H * W;
+    if (r_is_f64)
+        hipLaunchKernelGGL(k_net_input<double>, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream,
+                           (const double*)r, Z, hinv, sh, sw, H, W, scale, out);
+    else
+        hipLaunchKernelGGL(k_net_input<float>, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream,
+                           (const float*)r, Z, hinv, sh, sw, H, W, scale, out);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,476 @@
+"""Learned preconditioner for the streamfunction solve (no counterpart in
+pyOM2): a 4-level encoder-decoder convolutional network with skip
+connections (channels 16/32/64/128, 3x3 convolutions as implicit GEMM on
+BF16 MFMA with FP32 accumulation) that maps (residual, inverse depth, mask)
+to an error estimate, used inside flexible CG.  Forward, backward, Adam and
+the A-norm training loss run in the library's own HIP kernels
+(include/psinn.h, include/psisolve.h psi_nn_*); torch only allocates memory.
+
+Network (H, W multiples of 8; activations NHWC bf16):
+
+    x (3 used of 16) -E0-> a0 16 -pool-E1-> a1 32 -pool-E2-> a2 64 -pool-E3-> a3 128
+    D2: [up a3 | a2] -> d2 64;  D1: [up d2 | a1] -> d1 32;  D0: [up d1 | a0] -> d0 16
+    OUT: d0 -> z (1 channel, fp32, no ReLU)
+
+Training objective: for a residual r with true error e = A^-1 r, the
+preconditioned direction z should be A-parallel to e.  With the symmetric
+area-weighted operator S = W A the loss is 1 - (z.Wr)^2 / ((z.Sz)(e.Wr)),
+the fraction of the energy-norm error that a line search along z leaves --
+exactly what one flexible-CG step achieves.  It is invariant to the scale of
+z, as flexible CG is.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .streamfunction import SolveOptions, StreamfunctionSolver
+
+BF16 = torch.bfloat16
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with status {rc}")
+
+
+@dataclasses.dataclass
+class Layer:
+    name: str
+    c1: int           # channels of input 1
+    up1: bool         # input 1 read through a x2 nearest upsampling
+    c2: int           # channels of input 2 (skip), 0 if none
+    cout: int         # output channels (real)
+    relu: bool = True
+
+    @property
+    def cin(self):
+        return self.c1 + self.c2
+
+    @property
+    def cout_pad(self):
+        return (self.cout + 15) // 16 * 16
+
+
+LAYERS = [
+    Layer("E0", 16, False, 0, 16),
+    Layer("E1", 16, False, 0, 32),
+    Layer("E2", 32, False, 0, 64),
+    Layer("E3", 64, False, 0, 128),
+    Layer("D2", 128, True, 64, 64),
+    Layer("D1", 64, True, 32, 32),
+    Layer("D0", 32, True, 16, 16),
+    Layer("OUT", 16, False, 0, 1, relu=False),
+]
+N_INPUTS = 3   # residual, inverse depth, mask
+
+
+class PrecondNet:
+    """Parameters, activations and the forward/backward passes."""
+
+    def __init__(self, device="cuda", seed=0, stream=None):
+        self.L = _lib.lib()
+        self.dev = torch.device(device)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        g = torch.Generator().manual_seed(seed)
+        self.w, self.b, self.gw, self.gb, self.mw, self.vw, self.mb, self.vb = ({} for _ in range(8))
+        self.pack_f, self.pack_t = {}, {}
+        for ly in LAYERS:
+            fan_in = 9 * (N_INPUTS if ly.name == "E0" else ly.cin)
+            std = math.sqrt(2.0 / fan_in) if ly.relu else 0.1 * math.sqrt(1.0 / fan_in)
+            w = torch.randn(9, ly.cin, ly.cout_pad, generator=g) * std
+            if ly.name == "E0":
+                w[:, N_INPUTS:, :] = 0.0
+            w[:, :, ly.cout:] = 0.0
+            self.w[ly.name] = w.to(self.dev).contiguous()
+            self.b[ly.name] = torch.zeros(ly.cout_pad, device=self.dev)
+            for d in (self.gw, self.mw, self.vw):
+                d[ly.name] = torch.zeros_like(self.w[ly.name])
+            for d in (self.gb, self.mb, self.vb):
+                d[ly.name] = torch.zeros_like(self.b[ly.name])
+            self.pack_f[ly.name] = torch.empty(9 * (ly.cin // 16) * (ly.cout_pad // 16) * 256,
+                                               dtype=torch.int16, device=self.dev)
+            self.pack_t[ly.name] = torch.empty(9 * (ly.cout_pad // 16) * (ly.cin // 16) * 256,
+                                               dtype=torch.int16, device=self.dev)
+        self.step_count = 0
+        self.repack()
+        self._acts = {}
+
+    # -- parameters --------------------------------------------------------
+    @property
+    def s(self):
+        return C.c_void_p(self.stream.cuda_stream)
+
+    def repack(self):
+        for ly in LAYERS:
+            k16, n16 = ly.cin // 16, ly.cout_pad // 16
+            _chk(self.L.psinn_pack_weights(_p(self.w[ly.name]), ly.cin, ly.cout_pad, 0, k16, n16,
+                                           _p(self.pack_f[ly.name]), self.s), "pack")
+            _chk(self.L.psinn_pack_weights(_p(self.w[ly.name]), ly.cin, ly.cout_pad, 1, n16, k16,
+                                           _p(self.pack_t[ly.name]), self.s), "pack_t")
+
+    def state_dict(self):
+        d = {}
+        for ly in LAYERS:
+            d[f"{ly.name}.w"] = self.w[ly.name].cpu().numpy()
+            d[f"{ly.name}.b"] = self.b[ly.name].cpu().numpy()
+        return d
+
+    def load_state_dict(self, d):
+        for ly in LAYERS:
+            self.w[ly.name].copy_(torch.as_tensor(np.asarray(d[f"{ly.name}.w"]), device=self.dev))
+            self.b[ly.name].copy_(torch.as_tensor(np.asarray(d[f"{ly.name}.b"]), device=self.dev))
+        self.repack()
+
+    def save(self, path):
+        np.savez(path, **self.state_dict())
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        net = cls(device=device)
+        with np.load(path, allow_pickle=False) as z:
+            net.load_state_dict({k: z[k] for k in z.files})
+        return net
+
+    # -- passes --------------------------------------------------------------
+    def _buf(self, key, shape, dtype):
+        t = self._acts.get(key)
+        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype:
+            t = torch.empty(shape, dtype=dtype, device=self.dev)
+            self._acts[key] = t
+        return t
+
+    def _conv(self, ly, in1, in2, B, H, W, out16=None, out32=None, transposed=False):
+        if not transposed:
+            pack, c1, up1, c2, cpad, store = (self.pack_f[ly.name], ly.c1, int(ly.up1), ly.c2,
+                                              ly.cout_pad, ly.cout if out32 is not None else ly.cout_pad)
+            bias, relu = self.b[ly.name], int(ly.relu)
+        else:
+            pack, c1, up1, c2, cpad, store = (self.pack_t[ly.name], ly.cout_pad, 0, 0, ly.cin, ly.cin)
+            bias, relu = None, 0
+        _chk(self.L.psinn_conv3x3(_p(in1), c1, up1, _p(in2), c2, B, H, W, _p(pack), cpad, _p(bias),
+                                  relu, _p(out16), _p(out32), store, self.s), f"conv {ly.name}")
+
+    def forward(self, x16, B, H, W):
+        """x16: bf16 (B, H, W, 16).  Returns z, fp32 (B, H, W)."""
+        assert H % 8 == 0 and W % 8 == 0
+        L = self.L
+        hs = [(H >> k, W >> k) for k in range(4)]
+        A = {}
+        A["x"] = x16
+        A["a0"] = self._buf("a0", (B, H, W, 16), BF16)
+        self._conv(LAYERS[0], x16, None, B, H, W, out16=A["a0"])
+        prev = "a0"
+        for lvl, (ly, c) in enumerate(zip(LAYERS[1:4], (16, 32, 64)), start=1):
+            h, w = hs[lvl]
+            pin = self._buf(f"p{lvl - 1}", (B, h, w, c), BF16)
+            _chk(L.psinn_avgpool2(_p(A[prev]), B, hs[lvl - 1][0], hs[lvl - 1][1], c, _p(pin), self.s),
+                 "pool")
+            A[f"p{lvl - 1}"] = pin
+            out = self._buf(f"a{lvl}", (B, h, w, ly.cout), BF16)
+            self._conv(ly, pin, None, B, h, w, out16=out)
+            A[f"a{lvl}"] = out
+            prev = f"a{lvl}"
+        up, skip_names = "a3", ("a2", "a1", "a0")
+        for lvl, (ly, sk) in zip((2, 1, 0), zip(LAYERS[4:7], skip_names)):
+            h, w = hs[lvl]
+            out = self._buf(f"d{lvl}", (B, h, w, ly.cout), BF16)
+            self._conv(ly, A[up], A[sk], B, h, w, out16=out)
+            A[f"d{lvl}"] = out
+            up = f"d{lvl}"
+        z = self._buf("z", (B, H, W), torch.float32)
+        self._conv(LAYERS[7], A["d0"], None, B, H, W, out32=z)
+        A["z"] = z
+        self._fwd = (A, B, H, W)
+        return z
+
+    def zero_grad(self):
+        for ly in LAYERS:
+            self.gw[ly.name].zero_()
+            self.gb[ly.name].zero_()
+
+    def backward(self, gz):
+        """Accumulate parameter gradients for dL/dz = gz (fp32, B x H x W)."""
+        A, B, H, W = self._fwd
+        L = self.L
+        hs = [(H >> k, W >> k) for k in range(4)]
+        ly = {l.name: l for l in LAYERS}
+
+        def wgrad(layer, in1, in2, g, h, w):
+            _chk(L.psinn_conv3x3_wgrad(_p(in1), layer.c1, int(layer.up1), _p(in2), layer.c2, B, h, w,
+                                       _p(g), layer.cout_pad, _p(self.gw[layer.name]),
+                                       _p(self.gb[layer.name]), self.s), f"wgrad {layer.name}")
+
+        def data_grad(layer, g, h, w):
+            out = self._buf(f"gin_{layer.name}", (B, h, w, layer.cin), torch.float32)
+            self._conv(layer, g, None, B, h, w, out32=out, transposed=True)
+            return out
+
+        def combine(act, h, w, c, Asrc, CA, cA0, Bc, key):
+            out = self._buf(key, (B, h, w, c), BF16)
+            _chk(L.psinn_grad_combine(_p(act), B, h, w, c, _p(Asrc), CA, cA0, _p(Bc), _p(out),
+                                      self.s), "combine")
+            return out
+
+        def sumpool(fine, h, w, CF, c0, c, key):
+            out = self._buf(key, (B, h // 2, w // 2, c), torch.float32)
+            _chk(L.psinn_sumpool2(_p(fine), B, h, w, CF, c0, c, _p(out), self.s), "sumpool")
+            return out
+
+        # output layer: z = conv(d0)
+        g16 = self._buf("gz16", (B, H, W, 16), BF16)
+        _chk(L.psinn_expand16(_p(gz), B * H * W, _p(g16), self.s), "expand")
+        wgrad(ly["OUT"], A["d0"], None, g16, H, W)
+        gin = data_grad(ly["OUT"], g16, H, W)                       # d/d d0, fp32 16
+        g_pre = combine(A["d0"], H, W, 16, gin, 16, 0, None, "g_d0")
+        # decoder: D0 (L0), D1 (L1), D2 (L2)
+        skips = {}
+        for lvl, name, up_in, c_up, c_skip, sk in ((0, "D0", "d1", 32, 16, "a0"),
+                                                   (1, "D1", "d2", 64, 32, "a1"),
+                                                   (2, "D2", "a3", 128, 64, "a2")):
+            h, w = hs[lvl]
+            wgrad(ly[name], A[up_in], A[sk], g_pre, h, w)
+            gin = data_grad(ly[name], g_pre, h, w)                  # fp32 (c_up + c_skip)
+            skips[sk] = (gin, c_up + c_skip, c_up)
+            gup = sumpool(gin, h, w, c_up + c_skip, 0, c_up, f"gup_{name}")
+            hc, wc = hs[lvl + 1]
+            g_pre = combine(A[up_in], hc, wc, c_up, gup, c_up, 0, None, f"g_{up_in}")
+        # encoder: E3 (L3) .. E1 (L1), then E0
+        for lvl, name in ((3, "E3"), (2, "E2"), (1, "E1")):
+            h, w = hs[lvl]
+            wgrad(ly[name], A[f"p{lvl - 1}"], None, g_pre, h, w)
+            gp = data_grad(ly[name], g_pre, h, w)                   # d/d p_{lvl-1}, fp32
+            hf, wf = hs[lvl - 1]
+            sk = f"a{lvl - 1}"
+            gin, CA, cA0 = skips[sk]
+            g_pre = combine(A[sk], hf, wf, ly[name].c1, gin, CA, cA0, gp, f"g_{sk}")
+        wgrad(ly["E0"], A["x"], None, g_pre, H, W)
+
+    def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, gscale=1.0):
+        self.step_count += 1
+        for ly in LAYERS:
+            for w, g, m, v in ((self.w, self.gw, self.mw, self.vw), (self.b, self.gb, self.mb, self.vb)):
+                _chk(self.L.psinn_adam(_p(w[ly.name]), _p(g[ly.name]), _p(m[ly.name]), _p(v[ly.name]),
+                                       w[ly.name].numel(), lr, b1, b2, eps, self.step_count, gscale,
+                                       self.s), "adam")
+        self.repack()
+
+
+# ---------------------------------------------------------------------------
+# Inputs, training data and training
+# ---------------------------------------------------------------------------
+def net_shape(solver: StreamfunctionSolver):
+    H = (solver.ny + 4 + 7) // 8 * 8
+    W = (solver.nx + 4 + 7) // 8 * 8
+    return H, W
+
+
+class SolverInputs:
+    """Per-basin network coefficient inputs (Z, inverse depth) and buffers."""
+
+    def __init__(self, solver: StreamfunctionSolver):
+        self.solver = solver
+        self.H, self.W = net_shape(solver)
+        self.Z32 = torch.zeros(solver.shape, dtype=torch.float32, device=solver.device)
+        self.hinv = torch.zeros(solver.shape, dtype=torch.float32, device=solver.device)
+        _chk(solver.L.psi_fcg_prepare(solver.ctx, _p(self.Z32), _p(self.hinv)), "prepare")
+        m = self.hinv.max()
+        if float(m) > 0:
+            self.hinv /= m
+        self.scale = torch.zeros(1, dtype=torch.float64, device=solver.device)
+
+    def build(self, r, x16, b=0):
+        """Network input for residual r (FP64 or FP32 solver-layout tensor)
+        into sample b of x16, normalised by 1/rms(Z r) over active points."""
+        s = self.solver
+        zr = (self.Z32.double() * r.double())
+        m = self.Z32 != 0
+        rms = torch.sqrt((zr[m] ** 2).mean()) if bool(m.any()) else torch.ones((), device=s.device)
+        self.scale.copy_(1.0 / torch.clamp(rms, min=1e-300).reshape(1))
+        _chk(s.L.psinn_net_input(_p(r), int(r.dtype == torch.float64), _p(self.Z32), _p(self.hinv),
+                                 s.ny + 4, s.nx + 4, self.H, self.W, _p(self.scale),
+                                 C.c_void_p(x16[b].data_ptr()), C.c_void_p(s.stream.cuda_stream)),
+             "net_input")
+
+
+def smooth_error(rng, solver: StreamfunctionSolver, n_modes=24, kmax=None, noise=0.02):
+    """A random error field with a red spectrum on the solver grid (FP64)."""
+    ny, nx = solver.ny, solver.nx
+    kmax = kmax or max(2, min(nx, ny) // 4)
+    jj = np.arange(ny + 4)[:, None]
+    ii = np.arange(nx + 4)[None, :]
+    e = np.zeros(solver.shape)
+    for _ in range(n_modes):
+        kx, ky = rng.integers(1, kmax + 1, size=2)
+        a = rng.normal() / (kx * kx + ky * ky)
+        px, py = rng.uniform(0, 2 * math.pi, 2)
+        e += a * np.sin(math.pi * ky * jj / (ny + 4) + py) * np.sin(math.pi * kx * ii / (nx + 4) + px)
+    e *= np.abs(e).max() ** -1 if np.abs(e).max() > 0 else 1.0
+    e += noise * rng.normal(size=e.shape)
+    return e
+
+
+def harvest_pairs(solver: StreamfunctionSolver, forc, rng, n_synthetic=24,
+                  cg_iters=(5, 20, 60, 150, 400, 1000, 2500)):
+    """(r, e) pairs for one basin: errors of plain FP64 CG iterates
+    (x* - x_k, r_k = forc - A x_k, k in cg_iters) and synthetic red-spectrum
+    errors e with r = A e.  Returns FP64 device tensors R, E (n, ny+4, nx+4)."""
+    dev = solver.device
+    Z = solver.make_inv_sfc()
+    act = (Z != 0).double()
+    xs = solver.zeros()
+    solver.congrad_streamfunction(forc, xs, SolveOptions(criterion="relres", relres_target=1e-13,
+                                                         max_iterations=200000))
+    Rs, Es = [], []
+    for k in cg_iters:
+        x = solver.zeros()
+        solver.congrad_streamfunction(forc, x, SolveOptions(criterion="relres",
+                                                            relres_target=1e-30, max_iterations=k))
+        e = (xs - x) * act
+        if float(e.abs().max()) == 0:
+            continue
+        Rs.append(solver.apply_op(e) * act)
+        Es.append(e)
+    for _ in range(n_synthetic):
+        e = torch.as_tensor(smooth_error(rng, solver), device=dev) * act
+        Rs.append(solver.apply_op(e) * act)
+        Es.append(e)
+    return torch.stack(Rs), torch.stack(Es)
+
+
+@dataclasses.dataclass
+class TrainSet:
+    solver: StreamfunctionSolver
+    inputs: SolverInputs
+    R: torch.Tensor
+    ER: torch.Tensor
+    X: torch.Tensor          # bf16 network inputs (n, H, W, 16)
+
+
+def make_trainset(solver, forc, rng, **kw):
+    R, E = harvest_pairs(solver, forc, rng, **kw)
+    inp = SolverInputs(solver)
+    n = R.shape[0]
+    X = torch.zeros((n, inp.H, inp.W, 16), dtype=BF16, device=solver.device)
+    ER = torch.zeros(n, dtype=torch.float64, device=solver.device)
+    for k in range(n):
+        inp.build(R[k].contiguous(), X, k)
+        out = C.c_double()
+        _chk(solver.L.psi_nn_weighted_dot(solver.ctx, _p(E[k].contiguous()), _p(R[k].contiguous()),
+                                          C.byref(out)), "weighted dot")
+        ER[k] = out.value
+    return TrainSet(solver, inp, R, ER, X)
+
+
+def loss_and_grad(net: PrecondNet, ts: TrainSet, idx):
+    s = ts.solver
+    B = len(idx)
+    X = ts.X[idx].contiguous()
+    R = ts.R[idx].contiguous()
+    ER = ts.ER[idx].contiguous()
+    z = net.forward(X, B, ts.inputs.H, ts.inputs.W)
+    gz = torch.empty_like(z)
+    work = torch.empty(2 * B, dtype=torch.float64, device=s.device)
+    loss = torch.empty(1, dtype=torch.float64, device=s.device)
+    _chk(s.L.psi_nn_anorm_loss(s.ctx, _p(z), ts.inputs.H, ts.inputs.W, B, _p(R), _p(ER), _p(work),
+                               _p(gz), _p(loss)), "loss")
+    return loss, gz
+
+
+def train(net: PrecondNet, sets, steps=500, batch=4, lr=1e-3, seed=0, log_every=50, log=print):
+    """Adam on the A-norm loss, one basin per step (round robin)."""
+    g = torch.Generator().manual_seed(seed)
+    hist = []
+    t0 = time.time()
+    for it in range(steps):
+        ts = sets[it % len(sets)]
+        n = ts.R.shape[0]
+        idx = torch.randint(0, n, (min(batch, n),), generator=g).tolist()
+        net.zero_grad()
+        loss, gz = loss_and_grad(net, ts, idx)
+        net.backward(gz)
+        net.adam_step(lr=lr)
+        if it % log_every == 0 or it == steps - 1:
+            v = float(loss)
+            hist.append((it, v))
+            if log:
+                log(f"[train] step {it} loss {v:.4f} ({time.time() - t0:.1f}s)")
+    return hist
+
+
+# ---------------------------------------------------------------------------
+# Flexible CG with the network as preconditioner
+# ---------------------------------------------------------------------------
+class LearnedSolver:
+    """Flexible CG (FP32 inner, FP64 residual replacement) preconditioned by
+    the network, with Jacobi-PCG (the mixed path) as fallback."""
+
+    def __init__(self, solver: StreamfunctionSolver, net: PrecondNet):
+        self.solver = solver
+        self.net = net
+        self.inputs = SolverInputs(solver)
+        self.x16 = torch.zeros((1, self.inputs.H, self.inputs.W, 16), dtype=BF16,
+                               device=solver.device)
+        self.scale_ptr = C.c_void_p()
+        self.r32_ptr = C.c_void_p()
+
+    def solve(self, forc, sol, relres_target=1e-10, max_iterations=5000, poll_every=8,
+              leg_reduction=0.05, budget_iterations=None, fallback_opts=None):
+        """Returns (iterations, converged, info).  If `budget_iterations` of
+        flexible CG have not converged, the rest of the solve is handed to
+        Jacobi-PCG (the mixed path) from the current iterate."""
+        s, L = self.solver, self.solver.L
+        ctx = s.ctx
+        st = C.c_void_p(s.stream.cuda_stream)
+        rr = C.c_double()
+        _chk(L.psi_fcg_begin(ctx, _p(forc), _p(sol), C.byref(rr)), "fcg_begin")
+        true_rel = rr.value
+        its, flushes = 0, 0
+        t0 = time.perf_counter()
+        H, W = self.inputs.H, self.inputs.W
+        last_true = true_rel
+        budget = budget_iterations if budget_iterations is not None else max_iterations
+        while true_rel > relres_target and its < budget:
+            for _ in range(poll_every):
+                _chk(L.psi_fcg_input_scale(ctx, _p(self.inputs.Z32), C.byref(self.r32_ptr),
+                                           C.byref(self.scale_ptr)), "fcg_scale")
+                _chk(L.psinn_net_input(self.r32_ptr, 0, _p(self.inputs.Z32), _p(self.inputs.hinv),
+                                       s.ny + 4, s.nx + 4, H, W, self.scale_ptr, _p(self.x16), st),
+                     "net_input")
+                z = self.net.forward(self.x16, 1, H, W)
+                _chk(L.psi_fcg_step(ctx, _p(z), H, W), "fcg_step")
+                its += 1
+            _chk(L.psi_fcg_read(ctx, C.byref(rr)), "fcg_read")
+            rec = rr.value
+            if not math.isfinite(rec):
+                break
+            if rec <= relres_target or rec <= leg_reduction * last_true:
+                _chk(L.psi_fcg_flush(ctx, _p(forc), _p(sol), C.byref(rr)), "fcg_flush")
+                true_rel = rr.value
+                last_true = true_rel
+                flushes += 1
+        if its and true_rel > relres_target:
+            _chk(L.psi_fcg_flush(ctx, _p(forc), _p(sol), C.byref(rr)), "fcg_flush")
+            true_rel = rr.value
+        torch.cuda.synchronize(s.device)
+        info = {"fcg_iterations": its, "flushes": flushes, "fcg_relres": true_rel,
+                "fcg_ms": 1e3 * (time.perf_counter() - t0), "fallback_iterations": 0}
+        converged = true_rel <= relres_target
+        if not converged and fallback_opts is not None:
+            f_its, conv, finfo = s.congrad_streamfunction(forc, sol, fallback_opts)
+            info["fallback_iterations"] = f_its
+            converged = conv
+            true_rel = finfo["relres_true"]
+        info["relres_true"] = true_rel
+        return its + info["fallback_iterations"], converged, info
