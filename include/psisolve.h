@@ -224,6 +224,8 @@ int psi_fcg_read(psi_ctx *ctx, double *relres_recursive);
 /* sol += correction; r = forc - A sol in FP64 (residual replacement);
  * *relres = true relative residual; directions restart. */
 int psi_fcg_flush(psi_ctx *ctx, const double *forc, double *sol, double *relres);
+/* The ten flexible-CG device scalars (diagnostics; host sync). */
+int psi_fcg_scalars(psi_ctx *ctx, double *out10);
 
 /* Training support of the learned preconditioner (no reference counterpart):
  * A-norm cosine loss L = mean_b 1 - (z.Wr)^2 / ((z.Sz)(e.Wr)) of the network

@@ -101,6 +101,7 @@ SIGNATURES = [
     ("psi_fcg_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
     ("psi_fcg_read", C.c_int, [C.c_void_p, _dp]),
     ("psi_fcg_flush", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    ("psi_fcg_scalars", C.c_int, [C.c_void_p, _dp]),
     ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),

@@ -69,6 +69,17 @@ struct psi_ctx {
 // ---------------------------------------------------------------------------
 namespace {
 
+// The scalars are accumulated with device-scope atomics, which complete
+// beyond the per-XCD L2 caches; every other access to them must therefore
+// be a device-scope atomic load or store as well, never a plain access that
+// could sit in (or read from) one XCD's L2.
+__device__ __forceinline__ double ld_dev(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ double area_w(const Geom& g, const double* dxu, const double* cosu,
                                          const double* dyu, int r, int c) {
     return dxu[c] * cosu[r] * dyu[r];
@@ -92,12 +103,12 @@ __global__ void k_fcg_prepare(Geom g, const double* __restrict__ cC, const uint8
 }
 
 __global__ void k_fcg_begin(Geom g, const double* __restrict__ r64, const uint8_t* __restrict__ act,
-                            const double* __restrict__ sc, float* __restrict__ r32,
+                            const double* sc, float* __restrict__ r32,
                             float* __restrict__ e32, float* __restrict__ p32) {
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= plane) return;
-    r32[idx] = act[idx] ? (float)(r64[idx] * sc[8]) : 0.f;
+    r32[idx] = act[idx] ? (float)(r64[idx] * ld_dev(sc + 8)) : 0.f;
     e32[idx] = 0.f;
     p32[idx] = 0.f;
 }
@@ -119,17 +130,18 @@ __global__ void k_fcg_zr2(Geom g, const float* __restrict__ r32, const float* __
     }
 }
 
-__global__ void k_fcg_scalar(double* __restrict__ sc, int what) {
+__global__ void k_fcg_scalar(double* sc, int what) {
     if (what == 0) {          // finish the input scale, reset its accumulators
-        sc[5] = sc[6] > 0 ? 1.0 / sqrt(sc[6] / fmax(sc[7], 1.0)) : 1.0;
-        sc[6] = 0.0;
-        sc[7] = 0.0;
+        const double s6 = ld_dev(sc + 6), s7 = ld_dev(sc + 7);
+        st_dev(sc + 5, s6 > 0 ? 1.0 / sqrt(s6 / fmax(s7, 1.0)) : 1.0);
+        st_dev(sc + 6, 0.0);
+        st_dev(sc + 7, 0.0);
     } else {                  // start of a step
-        sc[2] = sc[1];
-        sc[0] = 0.0;
-        sc[1] = 0.0;
-        sc[3] = 0.0;
-        sc[4] = 0.0;
+        st_dev(sc + 2, ld_dev(sc + 1));
+        st_dev(sc + 0, 0.0);
+        st_dev(sc + 1, 0.0);
+        st_dev(sc + 3, 0.0);
+        st_dev(sc + 4, 0.0);
     }
 }
 
@@ -154,11 +166,11 @@ __global__ void k_fcg_scatter(Geom g, const float* __restrict__ znet, int H, int
 }
 
 __global__ void k_fcg_dir(Geom g, const float* __restrict__ zs, float* __restrict__ p32,
-                          const double* __restrict__ sc, int restart) {
+                          const double* sc, int restart) {
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= plane) return;
-    const float beta = restart ? 0.f : (float)(sc[0] / sc[2]);
+    const float beta = restart ? 0.f : (float)(ld_dev(sc + 0) / ld_dev(sc + 2));
     p32[idx] = zs[idx] - beta * p32[idx];
 }
 
@@ -197,11 +209,11 @@ __global__ void k_fcg_apply(Geom g, const float* __restrict__ E, const float* __
 
 __global__ void k_fcg_update(Geom g, const float* __restrict__ p, const float* __restrict__ q,
                              const uint8_t* __restrict__ act, float* __restrict__ e32,
-                             float* __restrict__ r32, double* __restrict__ sc) {
+                             float* __restrict__ r32, double* sc) {
     __shared__ double sh[NW];
     double acc[1] = {0.0};
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
-    const float alpha = (float)(sc[3] / sc[1]);
+    const float alpha = (float)(ld_dev(sc + 3) / ld_dev(sc + 1));
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
         if (!act[idx]) continue;
         e32[idx] += alpha * p[idx];
@@ -263,10 +275,10 @@ __global__ void k_nn_loss1(Geom g, const float* __restrict__ znet, int H, int W,
 __global__ void k_nn_loss2(Geom g, int H, int W, int B, const double* __restrict__ R,
                            const uint8_t* __restrict__ act, const double* __restrict__ ER,
                            const double* __restrict__ dxu, const double* __restrict__ cosu,
-                           const double* __restrict__ dyu, const double* __restrict__ work,
+                           const double* __restrict__ dyu, const double* work,
                            float* __restrict__ Sz_to_grad, double* __restrict__ loss) {
     const int b = blockIdx.y;
-    const double a = work[2 * b], q = work[2 * b + 1], er = ER[b];
+    const double a = ld_dev(work + 2 * b), q = ld_dev(work + 2 * b + 1), er = ER[b];
     const double ca = -2.0 * a / (q * er) / B, cs = 2.0 * a * a / (q * q * er) / B;
     const double* r = R + (size_t)b * g.pitch * (g.ny + 4);
     float* gz = Sz_to_grad + (size_t)b * H * W;
@@ -285,7 +297,10 @@ __global__ void k_nn_loss2(Geom g, int H, int W, int B, const double* __restrict
     }
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
         double L = 0.0;
-        for (int k = 0; k < B; ++k) L += 1.0 - work[2 * k] * work[2 * k] / (work[2 * k + 1] * ER[k]);
+        for (int k = 0; k < B; ++k) {
+            const double ak = ld_dev(work + 2 * k), qk = ld_dev(work + 2 * k + 1);
+            L += 1.0 - ak * ak / (qk * ER[k]);
+        }
         *loss = L / B;
     }
 }
@@ -1228,6 +1243,13 @@ int psi_nn_weighted_dot(psi_ctx* c, const double* a, const double* b, double* ou
             s += ha[i] * hb[i] * c->h_dxu[col] * c->h_cosu[r] * c->h_dyu[r];
         }
     *out = s;
+    return PSI_OK;
+}
+
+int psi_fcg_scalars(psi_ctx* c, double* out10) {
+    if (!valid_field_ctx(c) || !out10) return PSI_BAD_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out10, c->fcg, 10 * sizeof(double), hipMemcpyDeviceToHost));
     return PSI_OK;
 }
 

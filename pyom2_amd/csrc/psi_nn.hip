@@ -305,7 +305,10 @@ __global__ void k_net_input(const T* __restrict__ r, const float* __restrict__ Z
     if (y < sh && x < sw) {
         const size_t o = (size_t)y * sw + x;
         const float z = Z[o];
-        c0 = (float)((double)z * (double)r[o] * scale[0]);
+        // scale is produced by an atomically updated device scalar: read it
+        // with a device-scope load (not through this XCD's L2)
+        c0 = (float)((double)z * (double)r[o] *
+                     __hip_atomic_load(scale, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         c1 = hinv[o];
         c2 = z != 0.f ? 1.f : 0.f;
     }
