@@ -66,7 +66,30 @@ class PsiInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-# (name, restype, argtypes) of every entry point in include/psisolve.h
+class PsiBatchMember(C.Structure):
+    _fields_ = [("grid", PsiGrid), ("hur", C.c_void_p), ("hvr", C.c_void_p),
+                ("islands", PsiIslands)]
+
+
+class PsiBatchTune(C.Structure):
+    _fields_ = [("tile_x", C.c_int32), ("tile_y", C.c_int32), ("tiles_per_item", C.c_int32),
+                ("groups_per_cu", C.c_int32), ("graph_launches", C.c_int32),
+                ("ns_per_point", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PsiBatchStats(C.Structure):
+    _fields_ = [("elapsed_ms", C.c_double), ("launches", C.c_int32),
+                ("graph_replays", C.c_int32), ("polls", C.c_int32), ("reserved", C.c_int32),
+                ("active_points", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+# (name, restype, argtypes) of every entry point in include/*.h
 SIGNATURES = [
     ("psi_islands_trace", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip, C.c_int32, _ip,
                                     _ip, _ip, _ip, C.c_int32, _ip]),
@@ -105,6 +128,21 @@ SIGNATURES = [
     ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
+    # batched ensembles (include/psibatch.h)
+    ("psi_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(PsiBatchMember),
+                                   C.c_void_p]),
+    ("psi_batch_destroy", None, [C.c_void_p]),
+    ("psi_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("psi_batch_device_bytes", C.c_int64, [C.c_void_p]),
+    ("psi_batch_active_points", C.c_int64, [C.c_void_p]),
+    ("psi_batch_max_groups_per_cu", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("psi_batch_get_tune", C.c_int, [C.c_void_p, C.POINTER(PsiBatchTune)]),
+    ("psi_batch_set_tune", C.c_int, [C.c_void_p, C.POINTER(PsiBatchTune)]),
+    ("psi_batch_autotune", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(PsiBatchTune), _dp,
+                                     C.c_int32, _ip]),
+    ("psi_batch_congrad", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_void_p), C.POINTER(PsiInfo), _dp,
+                                    C.POINTER(PsiBatchStats)]),
     # learned-preconditioner network kernels (include/psinn.h)
     ("psinn_mfma_probe", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psinn_conv3x3", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,

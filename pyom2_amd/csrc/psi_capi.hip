@@ -1254,3 +1254,8 @@ int psi_fcg_scalars(psi_ctx* c, double* out10) {
 }
 
 }  // extern "C"
+
+// Batched ensembles (include/psibatch.h) share this translation unit's
+// kernels and helpers.
+#include "../../include/psibatch.h"
+#include "psi_batch.hip"

@@ -74,16 +74,17 @@ __device__ __forceinline__ bool in_array(int r, int c, const Geom& g) {
 // C = -E - W - N - S in the reference's order (solve_streamfct.f90:137-145).
 // Terms are summed S, W, C, E, N: the order apply_op
 // (solve_pressure.f90:264-273) visits the non-zero planes.
-template <class T>
+// S is the frame's row stride.
+template <class T, int S = HX>
 __device__ __forceinline__ T stencil5(const T* s, const T* E, const T* N, int idx, T rxv, T ryv,
                                       T& cc) {
-    const T e = E[idx], w = E[idx - 1] * rxv, n = N[idx], so = N[idx - HX] * ryv;
+    const T e = E[idx], w = E[idx - 1] * rxv, n = N[idx], so = N[idx - S] * ryv;
     cc = (((-e) - w) - n) - so;
-    T r = so * s[idx - HX];
+    T r = so * s[idx - S];
     r = r + w * s[idx - 1];
     r = r + cc * s[idx];
     r = r + e * s[idx + 1];
-    r = r + n * s[idx + HX];
+    r = r + n * s[idx + S];
     return r;
 }
 

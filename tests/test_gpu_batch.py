@@ -1,0 +1,211 @@
+"""GPU tests of batched ensembles (include/psibatch.h, pyom2_amd.ensemble).
+
+Every member of a batch is one congrad_streamfunction solve
+(solve_streamfct.f90:153-327) run to a relative FP64 residual.  Checks:
+  * the FP64 true residual of every returned solution, recomputed on the CPU
+    with the oracle's apply_op over the oracle's active set, meets the
+    target (1e-10, +2% for the different summation order);
+  * solutions agree with a sparse direct solve of the same system
+    (scipy.sparse LU; |x - x*| / |x*| <= 1e-7: relres 1e-10 times the
+    condition numbers of these small basins) and with the single-basin
+    mixed-precision solver (same bound);
+  * every tile shape / work split gives a converged answer;
+  * warm starts, iteration caps and argument errors behave as documented.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+from oracle import oracle as O
+from pyom2_amd import basins
+from pyom2_amd._lib import PSI_BAD_ARG, PSI_NOT_CONVERGED, PSI_OK, PsiError
+from pyom2_amd.streamfunction import SolveOptions, StreamfunctionSolver
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+TARGET = 1e-10
+
+# (seed, nx, ny, islands, cyclic_x): uneven sizes and aspect ratios, 0-3
+# islands, one periodic channel
+SPECS = [(1, 24, 20, 0, False), (2, 40, 33, 1, False), (3, 64, 64, 2, False),
+         (4, 17, 50, 0, False), (5, 96, 48, 3, False), (6, 72, 90, 1, False),
+         (7, 48, 32, 0, True), (8, 130, 70, 2, False), (9, 33, 129, 1, False),
+         (10, 64, 16, 0, False), (11, 100, 100, 3, False)]
+
+
+def T(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    from pyom2_amd.ensemble import EnsembleSolver
+    print("\n[batch ensemble of", len(SPECS), "members]", flush=True)
+    bs = [basins.beta_plane_basin(seed=s, nx=nx, ny=ny, n_islands=ni, cyclic_x=cy)
+          for s, nx, ny, ni, cy in SPECS]
+    ens = EnsembleSolver(bs, device=DEV)
+    yield bs, ens
+    ens.close()
+
+
+def cyclic_halo(b, t):
+    """Solution with its cyclic halo set (solve_streamfct.f90:58-59), as the
+    oracle's apply_op reads the halo."""
+    return O.setcyclic(b.nx, b.ny, b.cyclic_x, np.ascontiguousarray(t.cpu().numpy()))
+
+
+def oracle_system(b):
+    isl = O.trace_islands(b)
+    cf = O.make_coeff(b)
+    Z = O.make_inv(b.nx, b.ny, cf, isl)
+    return cf, Z
+
+
+def sparse_solution(b, cf, Z):
+    """Direct solve of A x = forc on the active points, x = 0 elsewhere
+    (cyclic members wrap their x neighbours)."""
+    ny, nx = b.ny, b.nx
+    act = Z != 0
+    idx = -np.ones(act.shape, np.int64)
+    idx[act] = np.arange(act.sum())
+    rows, cols, vals = [], [], []
+    planes = {(0, 0): 4, (1, 0): 5, (-1, 0): 3, (0, 1): 7, (0, -1): 1}
+    rr, cc = np.nonzero(act)
+    for (di, dj), k in planes.items():
+        r2 = rr + dj
+        c2 = cc + di
+        if b.cyclic_x:
+            c2 = (c2 - 2) % nx + 2
+        ok = (r2 >= 0) & (r2 < ny + 4) & (c2 >= 0) & (c2 < nx + 4)
+        j = np.full(rr.shape, -1)
+        j[ok] = idx[r2[ok], c2[ok]]
+        keep = j >= 0
+        rows.append(idx[rr[keep], cc[keep]])
+        cols.append(j[keep])
+        vals.append(cf[k][rr[keep], cc[keep]])
+    A = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(act.sum(),) * 2)
+    x = np.zeros(act.shape)
+    x[act] = spla.spsolve(A, b.forc[act])
+    return x
+
+
+@pytest.fixture(scope="module")
+def solved(ensemble):
+    bs, ens = ensemble
+    sols = ens.zeros()
+    res = ens.solve([T(b.forc) for b in bs], sols, relres_target=TARGET, max_iterations=20000)
+    torch.cuda.synchronize()
+    return bs, ens, res, [cyclic_halo(b, s) for b, s in zip(bs, sols)]
+
+
+def test_batch_true_residual_on_cpu(solved):
+    bs, ens, res, sols = solved
+    assert res.status == PSI_OK
+    assert res.stats["active_points"] > 0
+    for k, b in enumerate(bs):
+        info = res.infos[k]
+        cf, Z = oracle_system(b)
+        rel = O.true_relres(b, cf, Z, b.forc, sols[k])
+        print(f"member {k} {b.nx}x{b.ny} islands={b.n_islands_planned}: its={info['iterations']} "
+              f"legs={info['outer_iterations']} relres={rel:.3e} lat={res.latency_ms[k]:.2f} ms",
+              flush=True)
+        assert info["status"] == PSI_OK and info["converged"] == 1
+        assert rel <= 1.02 * TARGET
+        assert info["relres_true"] <= TARGET
+        assert abs(info["relres_true"] - rel) <= 0.02 * TARGET
+        assert res.latency_ms[k] > 0
+
+
+def test_batch_matches_sparse_direct_solve(solved):
+    bs, ens, res, sols = solved
+    for k, b in enumerate(bs):
+        cf, Z = oracle_system(b)
+        x = sparse_solution(b, cf, Z)
+        act = Z != 0
+        e = relerr(sols[k][act], x[act])
+        print(f"member {k}: |x - x_direct|/|x_direct| = {e:.2e}", flush=True)
+        assert e <= 1e-7
+
+
+def test_batch_matches_single_basin_mixed_solver(solved):
+    bs, ens, res, sols = solved
+    for k, b in enumerate(bs):
+        s = StreamfunctionSolver.from_basin(b, device=DEV)
+        x = s.zeros()
+        its, conv, info = s.congrad_streamfunction(T(b.forc), x, SolveOptions(
+            mode="mixed", criterion="relres", relres_target=TARGET, max_iterations=20000))
+        s.close()
+        assert conv
+        assert relerr(sols[k][2:-2, 2:-2], x.cpu().numpy()[2:-2, 2:-2]) <= 1e-7
+        # same recurrence, different reduction grouping: iteration counts close
+        assert abs(res.infos[k]["iterations"] - its) <= 0.1 * its + 10, (k, its)
+
+
+@pytest.mark.parametrize("shape", [(64, 16), (64, 8), (32, 16), (32, 8)])
+@pytest.mark.parametrize("per_item", [1, 3])
+def test_every_schedule_converges(ensemble, shape, per_item):
+    bs, ens = ensemble
+    print(f"\n[schedule {shape} x{per_item}]", flush=True)
+    ens.set_tune(tile_x=shape[0], tile_y=shape[1], tiles_per_item=per_item, graph_launches=8)
+    sols = ens.zeros()
+    res = ens.solve([T(b.forc) for b in bs], sols, relres_target=TARGET, max_iterations=20000)
+    torch.cuda.synchronize()
+    assert res.status == PSI_OK
+    for k in (0, 4, 6, 8, 10):
+        cf, Z = oracle_system(bs[k])
+        x = cyclic_halo(bs[k], sols[k])
+        assert O.true_relres(bs[k], cf, Z, bs[k].forc, x) <= 1.02 * TARGET
+    ens.set_tune()
+
+
+def test_autotune_installs_a_candidate(ensemble):
+    bs, ens = ensemble
+    best, table = ens.autotune(launches=8)
+    assert len(table) >= 16
+    assert best["ns_per_point"] > 0
+    assert min(r["ns_per_point"] for r in table) == pytest.approx(best["ns_per_point"])
+    t = ens.tune
+    assert (t["tile_x"], t["tile_y"], t["tiles_per_item"]) == (
+        best["tile_x"], best["tile_y"], best["tiles_per_item"])
+    sols = ens.zeros()
+    res = ens.solve([T(b.forc) for b in bs], sols, relres_target=TARGET, max_iterations=20000)
+    assert res.status == PSI_OK
+    ens.set_tune()
+
+
+def test_warm_start_and_repeat(ensemble):
+    bs, ens = ensemble
+    forcs = [T(b.forc) for b in bs]
+    sols = ens.zeros()
+    r1 = ens.solve(forcs, sols, relres_target=TARGET, max_iterations=20000)
+    # a converged first guess: done at the entry test, 0 iterations
+    r2 = ens.solve(forcs, sols, relres_target=TARGET, max_iterations=20000)
+    assert r2.status == PSI_OK
+    assert (r2.iterations == 0).all()
+    # a slightly changed forcing from the previous solution: fewer iterations
+    forcs3 = [f * 1.001 + 1e-4 * f.roll(1, 0) for f in forcs]
+    r3 = ens.solve(forcs3, sols, relres_target=TARGET, max_iterations=20000)
+    assert r3.status == PSI_OK
+    assert r3.iterations.sum() < r1.iterations.sum()
+    assert (r3.relres_true <= TARGET).all()
+
+
+def test_iteration_cap_and_bad_arguments(ensemble):
+    bs, ens = ensemble
+    forcs = [T(b.forc) for b in bs]
+    sols = ens.zeros()
+    res = ens.solve(forcs, sols, relres_target=TARGET, max_iterations=5)
+    assert res.status == PSI_NOT_CONVERGED
+    assert all(i["iterations"] == 6 and i["converged"] == 0 for i in res.infos)
+    with pytest.raises(PsiError) as e:
+        ens.solve(forcs, ens.zeros(), SolveOptions(mode="plain", criterion="relres"))
+    assert e.value.code == PSI_BAD_ARG
+    with pytest.raises(ValueError):
+        ens.solve(forcs[:-1], sols)

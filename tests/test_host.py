@@ -1,5 +1,5 @@
 """CPU-side checks of the product: the C ABI library loads and exports every
-entry point include/psisolve.h declares, the host island tracer reproduces
+entry point include/*.h declares, the host island tracer reproduces
 the reference's boundary lists, and the host-side state logic."""
 import os
 import re
