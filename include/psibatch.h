@@ -39,6 +39,7 @@ typedef struct psi_batch_member {
 typedef struct psi_batch_tune {
     int32_t tile_x, tile_y;       /* output tile per workgroup: 64x16, 64x8, 32x16, 32x8 */
     int32_t tiles_per_item;       /* tiles of one member per work item (1..64) */
+    int32_t items_per_grab;       /* work items a workgroup takes per grab (1..1024) */
     int32_t groups_per_cu;        /* resident workgroups per CU (1..occupancy) */
     int32_t graph_launches;       /* launches per captured graph (even, 2..256) */
     double ns_per_point;          /* autotuner: ns per active point per launch */
@@ -56,7 +57,7 @@ typedef struct psi_batch_stats {
 /* Builds every member's operator (make_coeff_streamfunction, :123-148, and
  * the Jacobi inverse with its Dirichlet set) into packed device arrays and
  * allocates all scratch.  The default schedule is 64x16 tiles, 4 tiles per
- * item, full occupancy, 16 launches per graph. */
+ * item, 2 items per grab, full occupancy, 16 launches per graph. */
 int psi_batch_create(psi_batch **out, int32_t count, const psi_batch_member *members,
                      void *stream);
 void psi_batch_destroy(psi_batch *batch);
@@ -69,12 +70,14 @@ int psi_batch_max_groups_per_cu(const psi_batch *batch, int32_t tile_x, int32_t 
 int psi_batch_get_tune(const psi_batch *batch, psi_batch_tune *tune);
 int psi_batch_set_tune(psi_batch *batch, const psi_batch_tune *tune);
 
-/* Autotuner: times `launches` batched sweeps (after 6 untimed ones) for
- * every tile shape x tiles_per_item in {1,2,4,8} x {full, half} occupancy on
- * the members' own operators, and installs the fastest schedule.  Caller
- * data is untouched.  table (optional, table_cap rows of 5 doubles: tile_x,
- * tile_y, tiles_per_item, groups_per_cu, ns_per_point) receives every
- * candidate; *table_n the number of candidates. */
+/* Autotuner: times `launches` batched launches (after 4 untimed ones) in
+ * which every member runs one full FP32 sweep, for every tile shape x
+ * tiles_per_item in {2,4,8} x items_per_grab in {1,4} x {full, half}
+ * occupancy on the members' own operators, and installs the fastest
+ * schedule.  Caller data is untouched.  table (optional, table_cap rows of
+ * 6 doubles: tile_x, tile_y, tiles_per_item, items_per_grab, groups_per_cu,
+ * ns_per_point) receives every candidate; *table_n the number of
+ * candidates. */
 int psi_batch_autotune(psi_batch *batch, int32_t launches, psi_batch_tune *best,
                        double *table, int32_t table_cap, int32_t *table_n);
 

@@ -73,8 +73,8 @@ class PsiBatchMember(C.Structure):
 
 class PsiBatchTune(C.Structure):
     _fields_ = [("tile_x", C.c_int32), ("tile_y", C.c_int32), ("tiles_per_item", C.c_int32),
-                ("groups_per_cu", C.c_int32), ("graph_launches", C.c_int32),
-                ("ns_per_point", C.c_double)]
+                ("items_per_grab", C.c_int32), ("groups_per_cu", C.c_int32),
+                ("graph_launches", C.c_int32), ("ns_per_point", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

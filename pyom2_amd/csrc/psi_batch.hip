@@ -54,6 +54,8 @@ struct BParams {
     double eps;                  // congr_epsilon (estimate only)
     int max_it;                  // inner iterations per member
     int K;                       // tiles per work item
+    int grab;                    // work items per grab from the launch's counter
+    int frozen;                  // autotuner: every member sweeps every launch
 };
 
 struct BState {
@@ -102,6 +104,14 @@ __device__ inline void b_after_cg(BState& s) {
 // folded into red) to the work of this launch.
 __device__ inline void b_transition(BState& s, const double* red, const BParams& P) {
     CgState& c = s.c;
+    if (P.frozen) {
+        // timing runs: a full sweep per member per launch, state unchanged
+        s.phase = BP_SWEEP;
+        c.alpha = 0.0;
+        c.beta = 0.0;
+        c.n = 0;
+        return;
+    }
     switch (s.phase) {
         case BP_START:
             s.phase = BP_RESID0;
@@ -350,47 +360,106 @@ __device__ __forceinline__ void work_start(const BMember& m, const BArrays& A, i
     }
 }
 
-// One fused FP32 PCG iteration on the item's tiles (the single-solve sweep
-// with per-member geometry): p = u + beta p, u -= alpha Z A p (radius 1),
-// e += alpha p, w = A u, s = A p and the six partials.
+// Pointers of one FP32 sweep: after n iterations u_n and p_{n-1} live in
+// buffer n & 1.
+struct SweepPtrs {
+    const float *u_in, *p_in, *E, *N;
+    const uint8_t* act;
+    float *u_out, *p_out, *x;
+};
+
+__device__ __forceinline__ SweepPtrs sweep_ptrs(const BMember& m, const BArrays& A, int n) {
+    const bool odd = n & 1;
+    SweepPtrs q;
+    q.u_in = (odd ? A.u1 : A.u0) + m.off;
+    q.p_in = (odd ? A.p1 : A.p0) + m.off;
+    q.E = A.E32 + m.off;
+    q.N = A.N32 + m.off;
+    q.act = A.act + m.off;
+    q.u_out = (odd ? A.u0 : A.u1) + m.off;
+    q.p_out = (odd ? A.p0 : A.p1) + m.off;
+    q.x = A.e32 + m.off;
+    return q;
+}
+
+// One tile's sweep inputs in registers: u, p, E, N and the mask over the
+// (BX+4) x (BY+4) frame, and the correction e at the tile's own points.
+// The next tile is fetched while the current one is computed.
 template <int BX, int BY>
-__device__ __forceinline__ void work_sweep(const BMember& m, const BArrays& A, int tb, int te,
-                                           int n, float alpha, float beta, double (&acc)[NRED],
-                                           Frames<BX, BY>& F) {
-    constexpr int HXb = BX + 4, HYb = BY + 4, FR = HXb * HYb, W1 = HXb - 2, H1 = HYb - 2;
-    const Geom& g = m.g;
-    const float* __restrict__ E = A.E32 + m.off;
-    const float* __restrict__ N = A.N32 + m.off;
-    const uint8_t* __restrict__ act = A.act + m.off;
-    const int odd = n & 1;
-    const float* __restrict__ u_in = (odd ? A.u1 : A.u0) + m.off;
-    const float* __restrict__ p_in = (odd ? A.p1 : A.p0) + m.off;
-    float* __restrict__ u_out = (odd ? A.u0 : A.u1) + m.off;
-    float* __restrict__ p_out = (odd ? A.p0 : A.p1) + m.off;
-    float* __restrict__ x = A.e32 + m.off;
-    for (int t = tb; t < te; ++t) {
-        const int c0 = (t % g.tiles_x) * BX, r0 = (t / g.tiles_x) * BY;
-        for (int l = threadIdx.x; l < FR; l += NT) {
+struct FrameRegsB {
+    static constexpr int HXb = BX + 4, HYb = BY + 4, FR = HXb * HYb;
+    static constexpr int NL = (FR + NT - 1) / NT, PER = (BX * BY) / NT;
+    float u[NL], p[NL], e[NL], n[NL];
+    uint8_t a[NL];
+    float xo[PER];
+
+    __device__ __forceinline__ void fetch(const Geom& g, int tile, const SweepPtrs& q) {
+        const int c0 = (tile % g.tiles_x) * BX, r0 = (tile / g.tiles_x) * BY;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int l = threadIdx.x + k * NT;
             const int gg = l / HXb, f = l - gg * HXb;
             const int r = r0 + gg, c = map_col(c0 + f, g);
-            float u = 0.f, pp = 0.f, e = 0.f, nn = 0.f;
-            uint8_t a = 0;
-            if (in_array(r, c, g)) {
-                const size_t o = (size_t)r * g.pitch + c;
-                u = u_in[o];
-                pp = p_in[o];
-                e = E[o];
-                nn = N[o];
-                a = act[o];
+            u[k] = 0.f;
+            p[k] = 0.f;
+            e[k] = 0.f;
+            n[k] = 0.f;
+            a[k] = 0;
+            if (l < FR && in_array(r, c, g)) {
+                const unsigned o = (unsigned)(r * g.pitch + c);
+                u[k] = q.u_in[o];
+                p[k] = q.p_in[o];
+                e[k] = q.E[o];
+                n[k] = q.N[o];
+                a[k] = q.act[o];
             }
-            F.U[l] = u;
-            F.P[l] = u + beta * pp;
-            F.E[l] = e;
-            F.N[l] = nn;
-            F.A[l] = a;
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int l = threadIdx.x + k * NT;
+            const int r = r0 + 2 + l / BX, c = c0 + 2 + l % BX;
+            xo[k] = (r <= g.ny + 1 && c <= g.nx + 1) ? q.x[(unsigned)(r * g.pitch + c)] : 0.f;
+        }
+    }
+    // frame -> LDS, fused with p_k = u_k + beta p_{k-1}
+    __device__ __forceinline__ void store(float beta, Frames<BX, BY>& F) const {
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int l = threadIdx.x + k * NT;
+            if (l < FR) {
+                F.U[l] = u[k];
+                F.P[l] = u[k] + beta * p[k];
+                F.E[l] = e[k];
+                F.N[l] = n[k];
+                F.A[l] = a[k];
+            }
+        }
+    }
+};
+
+// Fused FP32 PCG iterations over tiles [T0, T1) of one member (the
+// single-solve sweep with per-member geometry): p = u + beta p,
+// u -= alpha Z A p (radius 1), e += alpha p, w = A u, s = A p and the six
+// partials, written per work item (every K tiles).  fr holds tile T0.
+template <int BX, int BY>
+__device__ __forceinline__ void work_sweep_run(const BMember& m, const SweepPtrs& q, int T0,
+                                               int T1, int K, float alpha, float beta,
+                                               const BArrays& A, FrameRegsB<BX, BY>& fr,
+                                               Frames<BX, BY>& F, double* sRed,
+                                               double* __restrict__ part_out) {
+    constexpr int HXb = BX + 4, HYb = BY + 4, W1 = HXb - 2, H1 = HYb - 2;
+    constexpr int PER = (BX * BY) / NT;
+    const Geom& g = m.g;
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
+    for (int t = T0; t < T1; ++t) {
+        const int c0 = (t % g.tiles_x) * BX, r0 = (t / g.tiles_x) * BY;
+        fr.store(beta, F);
+        float xo[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) xo[k] = fr.xo[k];
         load_metrics_b<BX, BY>(m, A, c0, r0, F);
         __syncthreads();
+        if (t + 1 < T1) fr.fetch(g, t + 1, q);
         for (int l = threadIdx.x; l < W1 * H1; l += NT) {
             const int gg = 1 + l / W1, f = 1 + (l - (gg - 1) * W1);
             const int idx = gg * HXb + f;
@@ -402,7 +471,7 @@ __device__ __forceinline__ void work_sweep(const BMember& m, const BArrays& A, i
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < (BX * BY) / NT; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const int l = threadIdx.x + k * NT;
             const int gg = 2 + l / BX, f = 2 + l % BX;
             const int r = r0 + gg, c = c0 + f;
@@ -412,10 +481,10 @@ __device__ __forceinline__ void work_sweep(const BMember& m, const BArrays& A, i
                 const float w = stencil5<float, HXb>(F.U, F.E, F.N, idx, F.RX[f], F.RY[gg], cc);
                 const float sv = stencil5<float, HXb>(F.P, F.E, F.N, idx, F.RX[f], F.RY[gg], cc);
                 const float un = F.U[idx], pn = F.P[idx];
-                const size_t o = (size_t)r * g.pitch + c;
-                x[o] = x[o] + alpha * pn;
-                u_out[o] = un;
-                p_out[o] = pn;
+                const unsigned o = (unsigned)(r * g.pitch + c);
+                q.x[o] = xo[k] + alpha * pn;
+                q.u_out[o] = un;
+                q.p_out[o] = pn;
                 const double ud = (double)un, pd = (double)pn, ccd = (double)cc;
                 const double wd = (double)w, sd = (double)sv;
                 acc[0] += ud * ud * ccd;
@@ -430,31 +499,61 @@ __device__ __forceinline__ void work_sweep(const BMember& m, const BArrays& A, i
             }
         }
         __syncthreads();
+        if ((t + 1) % K == 0 || t + 1 == T1) {    // end of a work item
+            block_reduce<NRED>(acc, 1u << 2, sRed);
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int k = 0; k < NRED; ++k)
+                    part_out[(size_t)(m.seg_base + t / K) * NRED + k] = acc[k];
+            }
+#pragma unroll
+            for (int k = 0; k < NRED; ++k) acc[k] = 0.0;
+        }
     }
 }
 
 // ---- the batched step --------------------------------------------------------
 // Launch L reads states/partials of parity L & 1 and writes (L + 1) & 1.
-// Item p of the list goes to workgroup p % G; with G a multiple of 8 each
-// XCD walks a contiguous range of the list, so the chunks of one member that
-// share halo rows meet in the same L2.
+// Workgroups grab chunks of `grab` consecutive work items from the launch's
+// counter (ctr[L & 1]; workgroup 0 clears ctr[(L + 1) & 1], last used by
+// the previous launch, for the next one), so members in the slower
+// residual phases do not leave one workgroup behind the rest.  Within a
+// chunk a workgroup folds and advances a member's state once per run of
+// that member's items.
 template <int BX, int BY>
 __global__ __launch_bounds__(NT) void k_batch_step(
     const BMember* __restrict__ mem, const int2* __restrict__ items,
     const int* __restrict__ nitems_p, BArrays A, BParams P, const BState* __restrict__ st_in,
     BState* __restrict__ st_out, const double* __restrict__ part_in,
-    double* __restrict__ part_out, BFinal* __restrict__ fin) {
+    double* __restrict__ part_out, BFinal* __restrict__ fin, int* ctr, int par) {
     __shared__ Frames<BX, BY> F;
     __shared__ double sRed[NRED * NW];
     __shared__ BState sst;
+    __shared__ int s_chunk;
     const int nitems = *nitems_p;
-    const int G = gridDim.x;
-    const bool by_xcd = G >= 8 && (G & 7) == 0 && nitems >= 8;
-    for (int p = blockIdx.x; p < nitems; p += G) {
-        const int2 item = items[by_xcd ? xcd_tile(p, nitems) : p];
-        const int s = item.x, seg = item.y;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(ctr + (par ^ 1), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    FrameRegsB<BX, BY> fr;
+    for (;;) {
+    if (threadIdx.x == 0)
+        s_chunk = __hip_atomic_fetch_add(ctr + par, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int chunk = s_chunk;
+    if ((long long)chunk * P.grab >= nitems) break;
+    const int i_end = min(nitems, (chunk + 1) * P.grab);
+    for (int it = chunk * P.grab; it < i_end;) {
+        const int2 item = items[it];
+        const int s = item.x, seg0 = item.y;
         const BMember m = mem[s];
-        const int ph_in = st_in[s].phase;
+        const int run = min(i_end - it, m.nseg - seg0);   // this range's items of member s
+        const int T0 = seg0 * P.K, T1 = min(m.g.ntiles, (seg0 + run) * P.K);
+        const int ph_in = st_in[s].phase, n_in = st_in[s].c.n;
+        // the first tile of a sweep is fetched before the fold, with the
+        // iteration parity the transition will give if the member keeps sweeping
+        const bool may_sweep = ph_in == BP_SWEEP || ph_in == BP_SETUP || ph_in == BP_REFRESH;
+        const int n_pred = P.frozen ? 0 : (ph_in == BP_SWEEP ? n_in + 1
+                                                             : (ph_in == BP_SETUP ? 0 : n_in));
+        if (may_sweep) fr.fetch(m.g, T0, sweep_ptrs(m, A, n_pred));
         double red[NRED] = {0, 0, 0, 0, 0, 0};
         if (needs_fold(ph_in)) fold_member(part_in + (size_t)m.seg_base * NRED, m.nseg, red, sRed);
         if (threadIdx.x == 0) {
@@ -462,7 +561,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(
             if (st.phase != BP_DONE) {
                 b_transition(st, red, P);
                 st.launches += 1;
-                if (st.phase == BP_DONE && seg == 0) {
+                if (st.phase == BP_DONE && seg0 == 0) {
                     BFinal f;
                     f.rrt = st.rrt;
                     f.bbt = st.bbt;
@@ -480,39 +579,48 @@ __global__ __launch_bounds__(NT) void k_batch_step(
                 }
             }
             sst = st;
-            if (seg == 0) st_out[s] = st;
+            if (seg0 == 0) st_out[s] = st;
         }
         __syncthreads();
         const int ph = sst.phase;
-        if (ph != BP_DONE) {
-            const int tb = seg * P.K, te = min(m.g.ntiles, tb + P.K);
-            double acc[NRED] = {0, 0, 0, 0, 0, 0};
-            switch (ph) {
-                case BP_RESID0:
-                case BP_RESID: work_resid<BX, BY>(m, A, tb, te, acc); break;
-                case BP_SETUP:
-                    work_start<BX, BY>(m, A, tb, te, false, 0, (float)sst.scale, acc, F);
-                    break;
-                case BP_REFRESH:
-                    work_start<BX, BY>(m, A, tb, te, true, sst.c.n, (float)sst.scale, acc, F);
-                    break;
-                case BP_SWEEP:
-                    work_sweep<BX, BY>(m, A, tb, te, sst.c.n, (float)sst.c.alpha,
-                                       (float)sst.c.beta, acc, F);
-                    break;
-                case BP_CORR: work_corr<BX, BY>(m, A, tb, te, 1.0 / sst.scale); break;
-                default: break;
-            }
-            if (ph != BP_CORR) {
-                block_reduce<NRED>(acc, 1u << 2, sRed);
-                if (threadIdx.x == 0) {
-#pragma unroll
-                    for (int k = 0; k < NRED; ++k)
-                        part_out[(size_t)(m.seg_base + seg) * NRED + k] = acc[k];
+        if (ph == BP_SWEEP) {
+            const int n = sst.c.n;
+            const SweepPtrs q = sweep_ptrs(m, A, n);
+            if (!may_sweep || n != n_pred) fr.fetch(m.g, T0, q);
+            work_sweep_run<BX, BY>(m, q, T0, T1, P.K, (float)sst.c.alpha, (float)sst.c.beta, A,
+                                   fr, F, sRed, part_out);
+        } else if (ph != BP_DONE) {
+            for (int j = 0; j < run; ++j) {
+                const int seg = seg0 + j;
+                const int tb = seg * P.K, te = min(m.g.ntiles, tb + P.K);
+                double acc[NRED] = {0, 0, 0, 0, 0, 0};
+                switch (ph) {
+                    case BP_RESID0:
+                    case BP_RESID: work_resid<BX, BY>(m, A, tb, te, acc); break;
+                    case BP_SETUP:
+                        work_start<BX, BY>(m, A, tb, te, false, 0, (float)sst.scale, acc, F);
+                        break;
+                    case BP_REFRESH:
+                        work_start<BX, BY>(m, A, tb, te, true, sst.c.n, (float)sst.scale, acc,
+                                           F);
+                        break;
+                    case BP_CORR: work_corr<BX, BY>(m, A, tb, te, 1.0 / sst.scale); break;
+                    default: break;
                 }
+                if (ph != BP_CORR) {
+                    block_reduce<NRED>(acc, 1u << 2, sRed);
+                    if (threadIdx.x == 0) {
+#pragma unroll
+                        for (int k = 0; k < NRED; ++k)
+                            part_out[(size_t)(m.seg_base + seg) * NRED + k] = acc[k];
+                    }
+                }
+                __syncthreads();
             }
         }
         __syncthreads();
+        it += run;
+    }
     }
 }
 
@@ -595,6 +703,7 @@ struct psi_batch {
     size_t items_cap = 0, part_cap = 0;
     int2* d_items = nullptr;
     int* d_nitems = nullptr;
+    int* d_ctr = nullptr;         // 2 work counters (by launch parity)
     int2* h_items[2] = {nullptr, nullptr};   // pinned, alternating
     int* h_nitems[2] = {nullptr, nullptr};   // pinned, alternating
     double* part = nullptr;       // 2 x part_cap
@@ -767,6 +876,7 @@ int valid_tune(const psi_batch* b, const psi_batch_tune* t) {
     const int s = shape_index(t->tile_x, t->tile_y);
     if (s < 0) return -1;
     if (t->tiles_per_item < 1 || t->tiles_per_item > 64) return -1;
+    if (t->items_per_grab < 1 || t->items_per_grab > 1024) return -1;
     if (t->groups_per_cu < 1 || t->groups_per_cu > std::max(b->occ[s], 1)) return -1;
     if (t->graph_launches < 2 || t->graph_launches > 256 || (t->graph_launches & 1)) return -1;
     return s;
@@ -838,6 +948,7 @@ int upload_schedule(psi_batch* b, const double* const* forc, double* const* sol)
     BHIP(b, hipMemcpyAsync(b->st, b->h_st, sizeof(BState) * b->count, hipMemcpyHostToDevice,
                            b->stream));
     BHIP(b, hipMemsetAsync(b->fin, 0, sizeof(BFinal) * b->count, b->stream));
+    BHIP(b, hipMemsetAsync(b->d_ctr, 0, 2 * sizeof(int), b->stream));
     return PSI_OK;
 }
 
@@ -871,8 +982,10 @@ int launch_step(psi_batch* b, int par, const BArrays& A, const BParams& P) {
     const int2* items = b->d_items;
     const int* nitems = b->d_nitems;
     BFinal* fin = b->fin;
-    void* args[] = {(void*)&mem, (void*)&items, (void*)&nitems, (void*)&A, (void*)&P,
-                    (void*)&si,  (void*)&so,    (void*)&pi,     (void*)&po, (void*)&fin};
+    int* ctr = b->d_ctr;
+    void* args[] = {(void*)&mem, (void*)&items, (void*)&nitems, (void*)&A,   (void*)&P,
+                    (void*)&si,  (void*)&so,    (void*)&pi,     (void*)&po,  (void*)&fin,
+                    (void*)&ctr, (void*)&par};
     BHIP(b, hipLaunchKernel(kernel_for(b->shape), dim3(b->G), dim3(NT), args, 0, b->stream));
     return PSI_OK;
 }
@@ -886,6 +999,8 @@ BParams params_of(const psi_batch* b, const psi_opts* o) {
     P.eps = o->congr_epsilon;
     P.max_it = o->max_iterations;
     P.K = b->tune.tiles_per_item;
+    P.grab = b->tune.items_per_grab;
+    P.frozen = 0;
     return P;
 }
 
@@ -980,6 +1095,7 @@ int psi_batch_create(psi_batch** out, int32_t count, const psi_batch_member* mem
         b->part_cap = max_tiles * NRED;
         BTRY(balloc(b, &b->d_items, b->items_cap));
         BTRY(balloc(b, &b->d_nitems, 1));
+        BTRY(balloc(b, &b->d_ctr, 2));
         BTRY(balloc(b, &b->part, 2 * b->part_cap));
         BTRY(balloc(b, &b->st, 2 * (size_t)count));
         BTRY(balloc(b, &b->fin, (size_t)count));
@@ -1038,6 +1154,7 @@ int psi_batch_create(psi_batch** out, int32_t count, const psi_batch_member* mem
         t.tile_x = 64;
         t.tile_y = 16;
         t.tiles_per_item = 4;
+        t.items_per_grab = 2;
         t.groups_per_cu = std::max(b->occ[0], 1);
         t.graph_launches = 16;
         BTRY(apply_tune(b, &t));
@@ -1077,7 +1194,7 @@ int psi_batch_max_groups_per_cu(const psi_batch* b, int32_t tile_x, int32_t tile
 // Times `launches` batched sweeps per candidate schedule on the members'
 // operators (right-hand side of ones, zero first guess, internal buffers)
 // and keeps the fastest.  Candidates: every tile shape x tiles per item in
-// {1, 2, 4, 8} x {full, half} occupancy.
+// {2, 4, 8} x items per grab in {1, 4} x {full, half} occupancy.
 int psi_batch_autotune(psi_batch* b, int32_t launches, psi_batch_tune* best_out,
                        double* table, int32_t table_cap, int32_t* table_n) {
     if (!b) return bfail(nullptr, PSI_BAD_ARG, "null batch%s");
@@ -1109,22 +1226,24 @@ int psi_batch_autotune(psi_batch* b, int32_t launches, psi_batch_tune* best_out,
     for (int s = 0; s < kNumShapes && rc == PSI_OK; ++s) {
         const int occ = std::max(b->occ[s], 1);
         const int gpcs[2] = {occ, std::max(occ / 2, 1)};
-        for (int K : {1, 2, 4, 8}) {
-            for (int gi = 0; gi < (gpcs[1] == gpcs[0] ? 1 : 2); ++gi) {
+        for (int K : {2, 4, 8}) {
+            for (int gi = 0; gi < (gpcs[1] == gpcs[0] ? 2 : 4); ++gi) {
                 psi_batch_tune t{};
                 t.tile_x = kShapes[s].bx;
                 t.tile_y = kShapes[s].by;
                 t.tiles_per_item = K;
-                t.groups_per_cu = gpcs[gi];
+                t.items_per_grab = (gi & 1) ? 4 : 1;
+                t.groups_per_cu = gpcs[gi >> 1];
                 t.graph_launches = keep.graph_launches;
                 if ((rc = apply_tune(b, &t))) break;
                 hipLaunchKernelGGL(k_batch_tune_fill, dim3(nblk(b->total)), dim3(256), 0,
                                    b->stream, b->total, (const uint8_t*)b->act, tb64, tx64);
                 if ((rc = upload_schedule(b, fb.data(), fx.data()))) break;
-                const BParams P = params_of(b, &o);
-                // START -> RESID0 -> SETUP -> 3 warm-up sweeps, then the timed ones
+                BParams P = params_of(b, &o);
+                P.frozen = 1;
+                // every member sweeps in every launch: 4 warm-up launches, then the timed ones
                 int par = 0;
-                for (int w = 0; w < 6 && rc == PSI_OK; ++w, par ^= 1) rc = launch_step(b, par, A, P);
+                for (int w = 0; w < 4 && rc == PSI_OK; ++w, par ^= 1) rc = launch_step(b, par, A, P);
                 if (rc) break;
                 float ms = 0.f;
                 if (hipEventRecord(b->tb, b->stream) != hipSuccess) rc = PSI_HIP_ERROR;
@@ -1137,12 +1256,13 @@ int psi_batch_autotune(psi_batch* b, int32_t launches, psi_batch_tune* best_out,
                 if (rc) break;
                 const double ns = 1e6 * ms / launches / std::max<int64_t>(b->active_points, 1);
                 if (table && rows < table_cap) {
-                    double* row = table + (size_t)rows * 5;
+                    double* row = table + (size_t)rows * 6;
                     row[0] = t.tile_x;
                     row[1] = t.tile_y;
                     row[2] = t.tiles_per_item;
-                    row[3] = t.groups_per_cu;
-                    row[4] = ns;
+                    row[3] = t.items_per_grab;
+                    row[4] = t.groups_per_cu;
+                    row[5] = ns;
                 }
                 ++rows;
                 if (ns < best_ns) {

@@ -106,25 +106,26 @@ class EnsembleSolver:
         self._check(self.L.psi_batch_get_tune(self.batch, C.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, tile_x=64, tile_y=16, tiles_per_item=4, groups_per_cu=None,
-                 graph_launches=16):
+    def set_tune(self, tile_x=64, tile_y=16, tiles_per_item=4, items_per_grab=2,
+                 groups_per_cu=None, graph_launches=16):
         if groups_per_cu is None:
             groups_per_cu = self.L.psi_batch_max_groups_per_cu(self.batch, tile_x, tile_y)
-        t = PsiBatchTune(tile_x, tile_y, tiles_per_item, groups_per_cu, graph_launches, 0.0)
+        t = PsiBatchTune(tile_x, tile_y, tiles_per_item, items_per_grab, groups_per_cu,
+                         graph_launches, 0.0)
         self._check(self.L.psi_batch_set_tune(self.batch, C.byref(t)))
 
     def autotune(self, launches: int = 24):
         """Time every candidate schedule on the members' operators and keep
         the fastest.  Returns (best, table of all candidates)."""
-        cap = 64
-        table = np.zeros((cap, 5), np.float64)
+        cap = 128
+        table = np.zeros((cap, 6), np.float64)
         n = C.c_int32()
         best = PsiBatchTune()
         torch.cuda.current_stream(self.device).synchronize()
         self._check(self.L.psi_batch_autotune(self.batch, launches, C.byref(best), _f64p(table),
                                               cap, C.byref(n)))
         rows = [dict(tile_x=int(r[0]), tile_y=int(r[1]), tiles_per_item=int(r[2]),
-                     groups_per_cu=int(r[3]), ns_per_point=float(r[4]))
+                     items_per_grab=int(r[3]), groups_per_cu=int(r[4]), ns_per_point=float(r[5]))
                 for r in table[:min(n.value, cap)]]
         return best.as_dict(), rows
 

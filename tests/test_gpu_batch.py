@@ -149,11 +149,12 @@ def test_batch_matches_single_basin_mixed_solver(solved):
 
 
 @pytest.mark.parametrize("shape", [(64, 16), (64, 8), (32, 16), (32, 8)])
-@pytest.mark.parametrize("per_item", [1, 3])
-def test_every_schedule_converges(ensemble, shape, per_item):
+@pytest.mark.parametrize("per_item,grab", [(1, 1), (3, 5)])
+def test_every_schedule_converges(ensemble, shape, per_item, grab):
     bs, ens = ensemble
-    print(f"\n[schedule {shape} x{per_item}]", flush=True)
-    ens.set_tune(tile_x=shape[0], tile_y=shape[1], tiles_per_item=per_item, graph_launches=8)
+    print(f"\n[schedule {shape} x{per_item} grab {grab}]", flush=True)
+    ens.set_tune(tile_x=shape[0], tile_y=shape[1], tiles_per_item=per_item,
+                 items_per_grab=grab, graph_launches=8)
     sols = ens.zeros()
     res = ens.solve([T(b.forc) for b in bs], sols, relres_target=TARGET, max_iterations=20000)
     torch.cuda.synchronize()
