@@ -7,6 +7,12 @@ forcings (AR(1), rho = 0.99) solved one after another with the reference's
 warm start dpsi(taup1) = 2 dpsi(tau) - dpsi(taum1).  A step is one solve.
 Inputs are resident in HBM before the timed region starts.
 
+Paths: "learned" is the learned-preconditioner solver with its measured
+policy (flexible CG with the trained network or Jacobi-PCG, whichever needs
+fewer ms per residual decade on a probe from the same first guess; probes at
+the first solve and every --reprobe-every solves); "mixed" is FP32-inner /
+FP64-refined Jacobi-PCG; "plain" is FP64 Jacobi-PCG.
+
 Prints one JSON line with the headline path's ms per solve, the other
 paths beside it, the fused sweep kernel's roofline figure and the CPU
 oracle timed on a bounded sample on this host.
@@ -37,11 +43,18 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--modes", default="mixed,plain", help="first one is the headline")
+    ap.add_argument("--modes", default="learned,mixed,plain", help="first one is the headline")
+    ap.add_argument("--weights", default="", help="learned preconditioner weights (.npz)")
+    ap.add_argument("--reprobe-every", type=int, default=50)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (3 times): a "
+                         "kernel of known traffic that calibrates HBM byte counters")
     ap.add_argument("--relres", type=float, default=1e-10)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--check-every", type=int, default=0)
+    ap.add_argument("--inner-reduction", type=float, default=0.0,
+                    help="mixed path: residual reduction per FP32 leg (0: library default)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--workload", default="solve", choices=["solve", "ensemble"],
                     help="solve: the 2048^2 two-island solve (headline); ensemble: 512 "
@@ -72,6 +85,14 @@ def main():
     from pyom2_amd.streamfunction import SolveOptions, StreamfunctionSolver, islands_trace
 
     dev = torch.device("cuda:0")
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: 3 x {args.copy_mib} MiB")
+        del src, dst
     t = time.time()
     b = basins.spherical_basin(seed=0, n=args.n, n_islands=2)
     isl = islands_trace(b.kbot)
@@ -85,18 +106,33 @@ def main():
     modes = args.modes.split(",")
     results = {}
     for mode in modes:
-        opts = SolveOptions(mode=mode, criterion="relres", relres_target=args.relres,
-                            max_iterations=200000, check_every=args.check_every)
+        opts = SolveOptions(mode="mixed" if mode == "learned" else mode, criterion="relres",
+                            relres_target=args.relres, max_iterations=200000,
+                            check_every=args.check_every, inner_reduction=args.inner_reduction)
+        ls = None
+        if mode == "learned":
+            from pyom2_amd import learned
+            net = learned.PrecondNet.load(args.weights or learned.DEFAULT_WEIGHTS, device=dev)
+            ls = learned.LearnedSolver(s, net, reprobe_every=args.reprobe_every)
         dpsi = torch.zeros((3,) + b.shape, dtype=torch.float64, device=dev)
         taum1, tau, taup1 = 0, 1, 2
-        per = []
+        per, probes, choices = [], [], []
         for k in range(nsteps):
             if k == args.warmup:
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
             ts = time.perf_counter()
             dpsi[taup1] = 2 * dpsi[tau] - dpsi[taum1]
-            its, conv, info = s.congrad_streamfunction(forcings[k], dpsi[taup1], opts)
+            if ls is not None:
+                its, conv, linfo = ls.solve(forcings[k], dpsi[taup1], relres_target=args.relres,
+                                            fallback_opts=opts)
+                if linfo["probe"]:
+                    probes.append(dict(step=k, **linfo["probe"]))
+                choices.append(linfo["choice"])
+                info = dict(relres_true=linfo["relres_true"], sweep_ms=0.0, sweep_launches=0,
+                            outer_iterations=linfo["flushes"])
+            else:
+                its, conv, info = s.congrad_streamfunction(forcings[k], dpsi[taup1], opts)
             torch.cuda.synchronize()
             te = time.perf_counter()
             if not conv:
@@ -127,10 +163,20 @@ def main():
                              sweep_avg_launch_us=avg_launch_us, sweep_launches=launches,
                              achieved_GBs=achieved, frac=achieved / HBM_PEAK_GBS,
                              total_s=total)
+        if ls is not None:
+            timed = choices[args.warmup:]
+            results[mode].update(choice=max(set(timed), key=timed.count), probes=probes,
+                                 reprobe_every=args.reprobe_every)
+            for key in ("sweep_avg_launch_us", "sweep_launches", "achieved_GBs", "frac"):
+                results[mode].pop(key)
         log(f"[bench] {mode}: {results[mode]}")
 
     head = modes[0]
     r = results[head]
+    # the roofline figure is the fused sweep kernel of the Jacobi path that
+    # carries the solve (the learned policy's fallback is the mixed path)
+    kpath = head if head != "learned" else ("mixed" if "mixed" in results else None)
+    rk = results.get(kpath, {}) if kpath else {}
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(b, isl, args.cpu_iters, results.get("plain", r)["iterations_mean"])
@@ -145,23 +191,38 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32-inner/f64-refined" if head == "mixed" else "f64",
+        "dtype": "f64" if head == "plain" else "f32-inner/f64-refined",
         "data": "synthetic",
         "config": {"workload": f"spherical basin {b.nx}x{b.ny}, 0.05 deg, 2 islands, "
                                f"AR(1) rho=0.99 forcings, warm-started solves, relres<{args.relres:g}",
                    "path": head, "nisle": isl.nisle},
-        "paths": {m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
-                  for m, d in results.items()},
-        "roofline": {"bound": "hbm", "kernel": f"k_cg_sweep<{'float' if head != 'plain' else 'double'}>",
-                     "achieved": round(r["achieved_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["frac"], 4), "traffic": None,
-                     "bytes_per_pt_iter": BYTES_PER_PT.get(head, 33),
-                     "avg_launch_us": round(r["sweep_avg_launch_us"], 3)},
+        "paths": {m: {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
+                      for k, v in d.items()} for m, d in results.items()},
+        "roofline": ({"bound": "hbm",
+                      "kernel": f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>",
+                      "path": kpath, "achieved": round(rk["achieved_GBs"], 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rk["frac"], 4),
+                      "traffic": None, "bytes_per_pt_iter": BYTES_PER_PT[kpath],
+                      "avg_launch_us": round(rk["sweep_avg_launch_us"], 3)} if rk else None),
         "cpu_baseline": cpu,
     }
     if "plain" in results and head != "plain":
         line["speedup_vs_plain_fp64"] = round(results["plain"]["ms_per_solve"] / r["ms_per_solve"], 3)
-    print(json.dumps(line), flush=True)
+    if head == "learned" and "mixed" in results:
+        line["learned_vs_jacobi_mixed"] = round(results["mixed"]["ms_per_solve"] /
+                                                r["ms_per_solve"], 3)
+    print(json.dumps(finite(line)), flush=True)
+
+
+def finite(v):
+    """JSON-safe copy: non-finite floats become null."""
+    if isinstance(v, dict):
+        return {k: finite(x) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [finite(x) for x in v]
+    if isinstance(v, float) and not np.isfinite(v):
+        return None
+    return v
 
 
 def cpu_baseline(b, isl, iters, plain_its):
@@ -271,7 +332,7 @@ def ensemble_main(args):
         "autotune": tune_rows,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(finite(line)), flush=True)
 
 
 def ensemble_cpu_baseline(bs, its_mean, iters):

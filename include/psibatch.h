@@ -85,7 +85,7 @@ int psi_batch_autotune(psi_batch *batch, int32_t launches, psi_batch_tune *best,
  * to ||forc - A sol||_2 <= relres_target ||forc||_2 over active points
  * (FP64 true residual).  opts: mode PSI_MODE_MIXED and criterion
  * PSI_CRIT_RELRES are required; max_iterations bounds each member's inner
- * iterations; inner_reduction sets the FP32 leg length.  forc, sol: host
+ * iterations; inner_reduction sets the FP32 leg length (<= 0: 0.05).  forc, sol: host
  * arrays of count device pointers.  infos (optional, count entries):
  * iterations, converged, status, outer_iterations (FP64 residual checks),
  * relres_true, elapsed_ms = the member's completion time from the start of

@@ -89,7 +89,7 @@ typedef struct psi_opts {
     int32_t check_every;       /* iterations between host polls of the device
                                   convergence flag; <= 0 picks a default */
     double inner_reduction;    /* mixed/learned: relative residual reduction per
-                                  FP32 inner solve; <= 0 picks a default */
+                                  FP32 inner solve; <= 0 picks the default 0.25 */
     int32_t max_outer;         /* mixed/learned: FP64 refinement steps; <= 0 default */
     int32_t reserved[5];
 } psi_opts;

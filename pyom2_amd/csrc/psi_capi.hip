@@ -546,7 +546,17 @@ int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* f
         if (rc) return rc;
         const CgState& f = run.fin;
         const bool leg_end = (f.status == ST_MAXIT && f.n < abs_max) || f.status == ST_LEG;
-        if (!relres || !(f.status == ST_CONVERGED || leg_end)) break;
+        if (!relres || !(f.status == ST_CONVERGED || leg_end)) {
+            // iteration budget spent or divergence: sol keeps the current
+            // iterate, as the reference's does (a NaN correction is dropped)
+            if (!fp64 && (f.status == ST_MAXIT || f.status == ST_DIVERGED)) {
+                hipLaunchKernelGGL((k_add_correction<T>), dim3(nblk(c->plane)), dim3(256), 0,
+                                   c->stream, c->g, sol, xw, 1.0 / scale);
+                info->setup_launches += 1;
+                res[1] = 0.0;   // final residual below
+            }
+            break;
+        }
         // confirm on the FP64 true residual
         if (!fp64) {
             hipLaunchKernelGGL((k_add_correction<T>), dim3(nblk(c->plane)), dim3(256), 0, c->stream,
@@ -601,8 +611,11 @@ int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, 
     CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]},
                     c->G32};
     // FP32 legs end after each `inner_reduction` drop of the residual (the
-    // recursive FP32 residual is then replaced by the FP64 true one)
-    const double red = o->inner_reduction > 0 ? o->inner_reduction : 0.05;
+    // recursive FP32 residual is then replaced by the FP64 true one).  The
+    // default 0.25 was measured on the 2048^2 two-island basin: legs of 0.2-0.5
+    // need ~11.9k iterations to 1e-10, 0.05 needs 13.6k, 0.01 needs 30.6k (the
+    // FP32 recursion drifts from the true residual over long legs).
+    const double red = o->inner_reduction > 0 ? o->inner_reduction : 0.25;
     return solve_pcg<float>(c, o, P, forc, sol, c->e32, 1.0, false, o->reserved[0], red, info);
 }
 

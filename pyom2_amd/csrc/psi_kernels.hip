@@ -457,16 +457,39 @@ __global__ __launch_bounds__(NT) void k_cg_setup(
 #ifndef PSI_PREFETCH
 #define PSI_PREFETCH 1
 #endif
+// PSI_PREFETCH_X: the tile's own x values travel with the prefetched frame
+// (else x is read where it is updated).  PSI_NT_STORE: the sweep's outputs
+// are written with non-temporal stores.
+#ifndef PSI_PREFETCH_X
+#define PSI_PREFETCH_X 1
+#endif
+#ifndef PSI_NT_STORE
+#define PSI_NT_STORE 0
+#endif
+template <class T>
+__device__ __forceinline__ void put(T* p, T v) {
+#if PSI_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 template <class T>
 struct FrameRegs {
     static constexpr int NL = (HX * HY + NT - 1) / NT;
+    static constexpr int PER = (TX * TY) / NT;
     T u[NL], p[NL], e[NL], n[NL];
     uint8_t a[NL];
+#if PSI_PREFETCH_X
+    T xo[PER];
+#endif
 
     __device__ __forceinline__ void fetch(const Geom& g, int tile, const T* __restrict__ u_in,
                                           const T* __restrict__ p_in, const T* __restrict__ E,
                                           const T* __restrict__ N,
-                                          const uint8_t* __restrict__ act) {
+                                          const uint8_t* __restrict__ act,
+                                          const T* __restrict__ x) {
         const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
 #pragma unroll
         for (int q = 0; q < NL; ++q) {
@@ -487,6 +510,16 @@ struct FrameRegs {
                 a[q] = act[o];
             }
         }
+#if PSI_PREFETCH_X
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int l = threadIdx.x + k * NT;
+            const int r = r0 + 2 + l / TX, c = c0 + 2 + l % TX;
+            xo[k] = (r <= g.ny + 1 && c <= g.nx + 1) ? x[(size_t)r * g.pitch + c] : T(0);
+        }
+#else
+        (void)x;
+#endif
     }
     // frame -> LDS, fused with p_k = u_k + beta p_{k-1}
     __device__ __forceinline__ void store(T beta, T* sU, T* sP, T* sE, T* sN, uint8_t* sA) const {
@@ -530,7 +563,7 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
     int t0, ts, te;
     tile_schedule(g, t0, ts, te);
     FrameRegs<T> fr;
-    if (!st_in->done && t0 < te) fr.fetch(g, t0, u_in, p_in, E, N, act);
+    if (!st_in->done && t0 < te) fr.fetch(g, t0, u_in, p_in, E, N, act, x);
     advance_state(st_in, st_out, part_in, PHASE_SWEEP, sst, sRed);
     if (sst.done) return;
     const T alpha = (T)sst.alpha, beta = (T)sst.beta;
@@ -539,13 +572,18 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
     for (int tile = t0; tile < te; tile += ts) {
         const int c0 = (tile % g.tiles_x) * TX, r0 = (tile / g.tiles_x) * TY;
 #if !PSI_PREFETCH
-        if (tile != t0) fr.fetch(g, tile, u_in, p_in, E, N, act);
+        if (tile != t0) fr.fetch(g, tile, u_in, p_in, E, N, act, x);
 #endif
         fr.store(beta, sU, sP, sE, sN, sA);
+#if PSI_PREFETCH_X
+        T xo[FrameRegs<T>::PER];
+#pragma unroll
+        for (int k = 0; k < FrameRegs<T>::PER; ++k) xo[k] = fr.xo[k];
+#endif
         load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
         __syncthreads();
 #if PSI_PREFETCH
-        if (tile + ts < te) fr.fetch(g, tile + ts, u_in, p_in, E, N, act);
+        if (tile + ts < te) fr.fetch(g, tile + ts, u_in, p_in, E, N, act, x);
 #endif
         // radius 1: u_{k+1} = u_k - alpha Z (A p_k)
         for (int l = threadIdx.x; l < W1 * H1; l += NT) {
@@ -559,6 +597,7 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
         }
         __syncthreads();
         // radius 0: x update, w = A u_{k+1}, s = A p_k, reductions
+#pragma unroll
         for (int k = 0; k < (TX * TY) / NT; ++k) {
             const int l = threadIdx.x + k * NT;
             const int gg = 2 + l / TX, f = 2 + l % TX;
@@ -570,9 +609,13 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
                 const T sv = stencil5(sP, sE, sN, idx, sRX[f], sRY[gg], cc);
                 const T un = sU[idx], pn = sP[idx];
                 const size_t o = (size_t)r * g.pitch + c;
-                x[o] = x[o] + alpha * pn;
-                u_out[o] = un;
-                p_out[o] = pn;
+#if PSI_PREFETCH_X
+                put(x + o, xo[k] + alpha * pn);
+#else
+                put(x + o, x[o] + alpha * pn);
+#endif
+                put(u_out + o, un);
+                put(p_out + o, pn);
                 const double ud = (double)un, pd = (double)pn, ccd = (double)cc;
                 const double wd = (double)w, sd = (double)sv;
                 acc[0] += ud * ud * ccd;

@@ -412,11 +412,25 @@ def train(net: PrecondNet, sets, steps=500, batch=4, lr=1e-3, seed=0, log_every=
 # ---------------------------------------------------------------------------
 # Flexible CG with the network as preconditioner
 # ---------------------------------------------------------------------------
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                               "precond_unet.npz")
+
+
 class LearnedSolver:
     """Flexible CG (FP32 inner, FP64 residual replacement) preconditioned by
-    the network, with Jacobi-PCG (the mixed path) as fallback."""
+    the network, with Jacobi-PCG (the mixed path) as fallback.
 
-    def __init__(self, solver: StreamfunctionSolver, net: PrecondNet):
+    policy="auto": the first solve, and every `reprobe_every`-th after it,
+    probes both methods from the same first guess -- `probe` flexible-CG
+    iterations and `probe_jacobi` Jacobi-PCG iterations -- and measures each
+    in milliseconds per decade of FP64 true relative residual.  The faster
+    one finishes that solve (from its probe's iterate) and serves the solves
+    until the next probe.  policy="learned" runs flexible CG only (the
+    `fallback_opts` Jacobi-PCG solve takes over if it has not converged after
+    `max_iterations`); policy="jacobi" runs Jacobi-PCG only."""
+
+    def __init__(self, solver: StreamfunctionSolver, net: PrecondNet, reprobe_every=50,
+                 probe=32, probe_jacobi=512):
         self.solver = solver
         self.net = net
         self.inputs = SolverInputs(solver)
@@ -424,25 +438,28 @@ class LearnedSolver:
                                device=solver.device)
         self.scale_ptr = C.c_void_p()
         self.r32_ptr = C.c_void_p()
+        self.reprobe_every = reprobe_every
+        self.probe = probe
+        self.probe_jacobi = probe_jacobi
+        self.choice = None
+        self.solves = 0
+        self.last_probe = None
 
-    def solve(self, forc, sol, relres_target=1e-10, max_iterations=5000, poll_every=8,
-              leg_reduction=0.05, budget_iterations=None, fallback_opts=None):
-        """Returns (iterations, converged, info).  If `budget_iterations` of
-        flexible CG have not converged, the rest of the solve is handed to
-        Jacobi-PCG (the mixed path) from the current iterate."""
+    def _fcg(self, forc, sol, relres_target, max_its, poll_every, leg_reduction):
+        """Flexible CG from the first guess in sol for at most max_its
+        iterations; returns (iterations, FP64 true relres, flushes, relres at
+        the start)."""
         s, L = self.solver, self.solver.L
         ctx = s.ctx
         st = C.c_void_p(s.stream.cuda_stream)
         rr = C.c_double()
         _chk(L.psi_fcg_begin(ctx, _p(forc), _p(sol), C.byref(rr)), "fcg_begin")
-        true_rel = rr.value
+        true_rel = start = rr.value
         its, flushes = 0, 0
-        t0 = time.perf_counter()
         H, W = self.inputs.H, self.inputs.W
         last_true = true_rel
-        budget = budget_iterations if budget_iterations is not None else max_iterations
-        while true_rel > relres_target and its < budget:
-            for _ in range(poll_every):
+        while true_rel > relres_target and its < max_its:
+            for _ in range(min(poll_every, max_its - its)):
                 _chk(L.psi_fcg_input_scale(ctx, _p(self.inputs.Z32), C.byref(self.r32_ptr),
                                            C.byref(self.scale_ptr)), "fcg_scale")
                 _chk(L.psinn_net_input(self.r32_ptr, 0, _p(self.inputs.Z32), _p(self.inputs.hinv),
@@ -463,14 +480,73 @@ class LearnedSolver:
         if its and true_rel > relres_target:
             _chk(L.psi_fcg_flush(ctx, _p(forc), _p(sol), C.byref(rr)), "fcg_flush")
             true_rel = rr.value
-        torch.cuda.synchronize(s.device)
-        info = {"fcg_iterations": its, "flushes": flushes, "fcg_relres": true_rel,
-                "fcg_ms": 1e3 * (time.perf_counter() - t0), "fallback_iterations": 0}
-        converged = true_rel <= relres_target
-        if not converged and fallback_opts is not None:
-            f_its, conv, finfo = s.congrad_streamfunction(forc, sol, fallback_opts)
-            info["fallback_iterations"] = f_its
-            converged = conv
-            true_rel = finfo["relres_true"]
+        return its, true_rel, flushes, start
+
+    def _jacobi(self, forc, sol, opts):
+        its, conv, info = self.solver.congrad_streamfunction(forc, sol, opts)
+        return its, conv, info
+
+    @staticmethod
+    def _ms_per_decade(ms, rel0, rel1):
+        dec = math.log10(rel0 / rel1) if rel1 > 0 and rel0 > 0 else 0.0
+        return ms / dec if dec > 1e-3 else float("inf")
+
+    def _timed(self, fn):
+        torch.cuda.synchronize(self.solver.device)
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize(self.solver.device)
+        return out, 1e3 * (time.perf_counter() - t)
+
+    def solve(self, forc, sol, relres_target=1e-10, max_iterations=5000, poll_every=8,
+              leg_reduction=0.05, policy="auto", fallback_opts=None, budget_iterations=None):
+        """Returns (iterations, converged, info)."""
+        s = self.solver
+        jac = fallback_opts or SolveOptions(mode="mixed", criterion="relres",
+                                            relres_target=relres_target,
+                                            max_iterations=max(max_iterations, 400000))
+        info = {"choice": None, "fcg_iterations": 0, "fallback_iterations": 0, "flushes": 0,
+                "probe": None}
+        fcg_its = jac_its = 0
+        if policy == "auto" and (self.choice is None or self.solves % self.reprobe_every == 0):
+            # both probes from the same first guess
+            x0 = sol.clone()
+            (j_its, _, jinfo), j_ms = self._timed(lambda: self._jacobi(forc, sol, SolveOptions(
+                mode="mixed", criterion="relres", relres_target=relres_target,
+                max_iterations=self.probe_jacobi)))
+            x_jac = sol.clone()
+            sol.copy_(x0)
+            (f_its, f_rel, f_fl, f_rel0), f_ms = self._timed(lambda: self._fcg(
+                forc, sol, relres_target, self.probe, poll_every, leg_reduction))
+            j_rate = self._ms_per_decade(j_ms, f_rel0, jinfo["relres_true"])
+            f_rate = self._ms_per_decade(f_ms, f_rel0, f_rel)
+            self.choice = "learned" if f_rate < j_rate else "jacobi"
+            self.last_probe = {"jacobi_ms_per_decade": j_rate, "learned_ms_per_decade": f_rate,
+                               "jacobi_iterations": j_its, "learned_iterations": f_its,
+                               "relres_start": f_rel0, "jacobi_relres": jinfo["relres_true"],
+                               "learned_relres": f_rel}
+            info["probe"] = self.last_probe
+            if self.choice == "jacobi":
+                sol.copy_(x_jac)           # keep the Jacobi probe's progress
+                jac_its += j_its
+            else:
+                fcg_its += f_its
+                info["flushes"] += f_fl
+        choice = {"auto": self.choice, "learned": "learned", "jacobi": "jacobi"}[policy]
+        info["choice"] = choice
+        true_rel = None
+        if choice == "learned":
+            budget = budget_iterations if budget_iterations is not None else max_iterations
+            its, true_rel, fl, _ = self._fcg(forc, sol, relres_target, budget, poll_every,
+                                             leg_reduction)
+            fcg_its += its
+            info["flushes"] += fl
+        if choice == "jacobi" or true_rel is None or true_rel > relres_target:
+            its, conv, jinfo = self._jacobi(forc, sol, jac)
+            jac_its += its
+            true_rel = jinfo["relres_true"]
+        self.solves += 1
+        info["fcg_iterations"] = fcg_its
+        info["fallback_iterations"] = jac_its
         info["relres_true"] = true_rel
-        return its + info["fallback_iterations"], converged, info
+        return fcg_its + jac_its, true_rel <= relres_target, info

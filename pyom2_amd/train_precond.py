@@ -23,8 +23,7 @@ import torch
 from . import basins, learned
 from .streamfunction import SolveOptions, StreamfunctionSolver, islands_trace
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "precond_unet.npz")
+DEFAULT_WEIGHTS = learned.DEFAULT_WEIGHTS
 
 
 def log(*a):
@@ -111,7 +110,8 @@ def evaluate(net, n, seed=11, relres=1e-10, modes=("learned", "mixed", "plain"))
         t = time.perf_counter()
         if mode == "learned":
             ls = learned.LearnedSolver(s, net)
-            its, conv, info = ls.solve(forc, sol, relres_target=relres, max_iterations=20000)
+            its, conv, info = ls.solve(forc, sol, relres_target=relres, max_iterations=20000,
+                                       policy="learned")
         else:
             its, conv, info = s.congrad_streamfunction(forc, sol, SolveOptions(
                 mode=mode, criterion="relres", relres_target=relres, max_iterations=400000))

@@ -265,11 +265,28 @@ def test_training_reduces_loss_and_fcg_converges():
     assert l1 < 0.8 * l0
     ls = learned.LearnedSolver(s, net)
     sol = s.zeros()
-    its, conv, info = ls.solve(forc, sol, relres_target=1e-10, max_iterations=3000)
+    its, conv, info = ls.solve(forc, sol, relres_target=1e-10, max_iterations=3000,
+                               policy="learned")
     print(f"  learned FCG: {its} iterations, info={info}")
-    assert conv
+    assert conv and info["fallback_iterations"] == 0
     cf = O.make_coeff(b)
     isl = islands_trace(b.kbot)
     Z = O.make_inv(b.nx, b.ny, cf, O.Islands(isl.nisle, isl.offsets, isl.bi, isl.bj, isl.di,
                                              isl.dj, None))
     assert O.true_relres(b, cf, Z, b.forc, sol.cpu().numpy()) <= 1.01e-10
+    # what the trained network gains over Jacobi-PCG on the same solve
+    xj = s.zeros()
+    jits, jconv, _ = s.congrad_streamfunction(forc, xj, mode="mixed", criterion="relres",
+                                              relres_target=1e-10, max_iterations=20000)
+    print(f"  Jacobi-PCG (mixed): {jits} iterations; learned FCG: {its}")
+    assert jconv and its < jits
+    # the measured policy probes both from one first guess, picks the one
+    # with fewer ms per residual decade and converges
+    sol2 = s.zeros()
+    its2, conv2, info2 = ls.solve(forc, sol2, relres_target=1e-10, policy="auto")
+    print(f"  auto policy: {info2['choice']} {its2} iterations, probe={info2['probe']}")
+    assert conv2 and info2["choice"] in ("learned", "jacobi")
+    pr = info2["probe"]
+    assert info2["choice"] == ("learned" if pr["learned_ms_per_decade"] <
+                               pr["jacobi_ms_per_decade"] else "jacobi")
+    assert O.true_relres(b, cf, Z, b.forc, sol2.cpu().numpy()) <= 1.01e-10

@@ -156,6 +156,26 @@ def test_forced_exits():
     assert conv and relerr(N(sol), d["sol"]) <= 1e-9
 
 
+def test_mixed_iteration_cap_keeps_the_iterate():
+    """A mixed solve stopped by max_iterations leaves its current iterate in
+    sol (as the reference's congrad does) and reports that iterate's FP64
+    residual."""
+    b, d = fixtures.load("islands_64")
+    s = solver_for(b, d)
+    isl = O.Islands(int(d["nisle"]), d["offsets"], d["bi"], d["bj"], d["di"], d["dj"], None)
+    cf = O.make_coeff(b)
+    Z = O.make_inv(b.nx, b.ny, cf, isl)
+    for cap in (7, 60):
+        sol = s.zeros()
+        its, conv, info = s.congrad_streamfunction(T(b.forc), sol, mode="mixed",
+                                                   criterion="relres", relres_target=1e-10,
+                                                   max_iterations=cap)
+        assert (its, conv, info["status"]) == (cap + 1, False, 1)
+        rel = O.true_relres(b, cf, Z, b.forc, N(sol))
+        assert 0 < rel < 1.0
+        assert abs(info["relres_true"] - rel) <= 1e-6 * rel
+
+
 def _oracle_step(b, init, fpx, fpy, psi, dpsi, dpsin, taum1, tau, taup1, dt, ab):
     """solve_streamfunction's 2-D body (solve_streamfct.f90:49-94) on the oracle."""
     nx, ny = b.nx, b.ny
