@@ -34,7 +34,14 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per point per fused PCG iteration: x, r (as u = Z r) and p
 # read and written (6 words), hur/hvr-derived coefficients read (2 words),
 # a 1-byte Dirichlet/land mask
-BYTES_PER_PT = {"plain": 6 * 8 + 2 * 8 + 1, "mixed": 6 * 4 + 2 * 4 + 1}
+BYTES_PER_PT = {"plain": 6 * 8 + 2 * 8 + 1, "mixed": 6 * 4 + 2 * 4 + 1,
+                # polynomial-preconditioned FP32 sweep: x, r, z = M r and p
+                # read and written, E and N, the mask
+                "poly": 8 * 4 + 2 * 4 + 1}
+# HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
+# profiles/k_cg_sweep_spherical_2048.md
+TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6}
 
 
 def parse():
@@ -53,6 +60,8 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--check-every", type=int, default=0)
+    ap.add_argument("--poly-degree", type=int, default=3)
+    ap.add_argument("--poly-lower", type=float, default=0.0)
     ap.add_argument("--inner-reduction", type=float, default=0.0,
                     help="mixed path: residual reduction per FP32 leg (0: library default)")
     ap.add_argument("--verbose", action="store_true")
@@ -108,7 +117,8 @@ def main():
     for mode in modes:
         opts = SolveOptions(mode="mixed" if mode == "learned" else mode, criterion="relres",
                             relres_target=args.relres, max_iterations=200000,
-                            check_every=args.check_every, inner_reduction=args.inner_reduction)
+                            check_every=args.check_every, inner_reduction=args.inner_reduction,
+                            poly_degree=args.poly_degree, poly_lower=args.poly_lower)
         ls = None
         if mode == "learned":
             from pyom2_amd import learned
@@ -202,7 +212,9 @@ def main():
                       "kernel": f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>",
                       "path": kpath, "achieved": round(rk["achieved_GBs"], 1),
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rk["frac"], 4),
-                      "traffic": None, "bytes_per_pt_iter": BYTES_PER_PT[kpath],
+                      "traffic": TRAFFIC_2048.get(kpath) if b.nx == b.ny == 2048 else None,
+                      "traffic_unit": "bytes per launch",
+                      "bytes_per_pt_iter": BYTES_PER_PT[kpath],
                       "avg_launch_us": round(rk["sweep_avg_launch_us"], 3)} if rk else None),
         "cpu_baseline": cpu,
     }

@@ -44,7 +44,9 @@ typedef enum psi_status {
 typedef enum psi_mode {
     PSI_MODE_PLAIN = 0,    /* FP64 Jacobi-PCG */
     PSI_MODE_MIXED = 1,    /* FP32 inner Jacobi-PCG under FP64 iterative refinement */
-    PSI_MODE_LEARNED = 2   /* flexible CG with the learned preconditioner (FP32 inner) */
+    PSI_MODE_LEARNED = 2,  /* flexible CG with the learned preconditioner (FP32 inner) */
+    PSI_MODE_POLY = 3      /* as MIXED, preconditioned by a Chebyshev polynomial of degree
+                              poly_degree in Z A (Z the Jacobi inverse) times Z */
 } psi_mode;
 
 typedef enum psi_criterion {
@@ -91,7 +93,11 @@ typedef struct psi_opts {
     double inner_reduction;    /* mixed/learned: relative residual reduction per
                                   FP32 inner solve; <= 0 picks the default 0.25 */
     int32_t max_outer;         /* mixed/learned: FP64 refinement steps; <= 0 default */
-    int32_t reserved[5];
+    int32_t poly_degree;       /* PSI_MODE_POLY: 1..4; <= 0 picks 3 */
+    double poly_lower;         /* PSI_MODE_POLY: the polynomial's interval is
+                                  [2 poly_lower, 2]; <= 0 picks 0.02 */
+    int32_t leg_max_iterations;/* mixed/poly: inner iterations per FP32 leg; <= 0: no cap */
+    int32_t reserved[3];
 } psi_opts;
 
 typedef struct psi_info {
@@ -226,6 +232,13 @@ int psi_fcg_read(psi_ctx *ctx, double *relres_recursive);
 int psi_fcg_flush(psi_ctx *ctx, const double *forc, double *sol, double *relres);
 /* The ten flexible-CG device scalars (diagnostics; host sync). */
 int psi_fcg_scalars(psi_ctx *ctx, double *out10);
+
+/* Scalars of the PSI_MODE_POLY preconditioner (no reference counterpart):
+ * the Chebyshev iteration for Z A e = Z r on [2 lower, 2] of the given degree
+ * (1..4), applied as d_0 = Z r * out[0]; d_j = out[2j-1] d_{j-1} +
+ * out[2j] res_{j-1}; res_j = res_{j-1} - Z A d_j; z = sum_j d_j.  out holds
+ * 1 + 2 degree floats. */
+int psi_poly_coefficients(int32_t degree, double lower, float *out);
 
 /* Training support of the learned preconditioner (no reference counterpart):
  * A-norm cosine loss L = mean_b 1 - (z.Wr)^2 / ((z.Sz)(e.Wr)) of the network

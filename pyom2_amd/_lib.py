@@ -25,6 +25,7 @@ STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "DIVERGED", 3: "NAN", 4: "SINGUL
 PSI_MODE_PLAIN = 0
 PSI_MODE_MIXED = 1
 PSI_MODE_LEARNED = 2
+PSI_MODE_POLY = 3
 PSI_CRIT_REFERENCE = 0
 PSI_CRIT_RELRES = 1
 
@@ -52,7 +53,8 @@ class PsiOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("criterion", C.c_int32), ("congr_epsilon", C.c_double),
                 ("max_iterations", C.c_int32), ("relres_target", C.c_double),
                 ("check_every", C.c_int32), ("inner_reduction", C.c_double),
-                ("max_outer", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("max_outer", C.c_int32), ("poly_degree", C.c_int32), ("poly_lower", C.c_double),
+                ("leg_max_iterations", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class PsiInfo(C.Structure):
@@ -125,6 +127,7 @@ SIGNATURES = [
     ("psi_fcg_read", C.c_int, [C.c_void_p, _dp]),
     ("psi_fcg_flush", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     ("psi_fcg_scalars", C.c_int, [C.c_void_p, _dp]),
+    ("psi_poly_coefficients", C.c_int, [C.c_int32, C.c_double, C.POINTER(C.c_float)]),
     ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),

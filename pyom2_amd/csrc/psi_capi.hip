@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,7 @@
 
 // The kernels are compiled in this translation unit.
 #include "psi_kernels.hip"
+#include "psi_poly.hip"
 
 using namespace psi;
 
@@ -39,6 +41,8 @@ struct psi_ctx {
     // CG scratch
     double *u64[2] = {nullptr, nullptr}, *p64[2] = {nullptr, nullptr};
     float *u32[2] = {nullptr, nullptr}, *p32[2] = {nullptr, nullptr}, *e32 = nullptr;
+    float* rp32[2] = {nullptr, nullptr};   // residuals of the polynomial-preconditioned path
+    int Gpoly[PMAXK + 1] = {0};            // resident grid of k_poly_pcg<K> (0: not yet queried)
     double *r64 = nullptr, *tmp64 = nullptr, *tmp64b = nullptr;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *pf32 = nullptr;   // flexible CG
     double* fcg = nullptr;        // flexible-CG device scalars
@@ -369,7 +373,52 @@ struct CgPath {
     const T *E, *N, *rx, *ry;
     T *u[2], *p[2];
     int G;
+    int poly = 0;                  // > 0: Chebyshev-preconditioned FP32 kernels of degree poly
+    ChebCoef ch{};
+    T* rb[2] = {nullptr, nullptr}; // their residual buffers (u holds z = M r)
 };
+
+// Launch of the polynomial-preconditioned kernel of mode MODE and the
+// path's degree.
+template <int MODE>
+int launch_poly(psi_ctx* c, const CgPath<float>& P, float* x, const double* r64, double rscale,
+                const double* true_res, double thresh, int new_max_it, const CgState* si,
+                CgState* so, const double* pi, double* po) {
+#define PSI_POLY_LAUNCH(K)                                                                    \
+    hipLaunchKernelGGL((k_poly_pcg<K, MODE>), dim3(P.G), dim3(NT), 0, c->stream,            \
+                       poly_geom(c->g, K), P.ch,                                              \
+                       P.E, P.N, P.rx, P.ry, c->act, x, P.u[0], P.u[1], P.p[0], P.p[1],       \
+                       P.rb[0], P.rb[1], r64, rscale, true_res, thresh, new_max_it, si, so,  \
+                       pi, po)
+    switch (P.poly) {
+        case 1: PSI_POLY_LAUNCH(1); break;
+        case 2: PSI_POLY_LAUNCH(2); break;
+        case 3: PSI_POLY_LAUNCH(3); break;
+        case 4: PSI_POLY_LAUNCH(4); break;
+        default: return PSI_BAD_ARG;
+    }
+#undef PSI_POLY_LAUNCH
+    return PSI_OK;
+}
+
+template <class T>
+int launch_poly_checked(psi_ctx* c, int mode, const CgPath<T>& P, T* x, const double* r64,
+                        double rscale, const double* true_res, double thresh, int new_max_it,
+                        const CgState* si, CgState* so, const double* pi, double* po) {
+    if constexpr (std::is_same<T, float>::value) {
+        int rc = mode == PM_SETUP
+                     ? launch_poly<PM_SETUP>(c, P, x, r64, rscale, true_res, thresh, new_max_it,
+                                             si, so, pi, po)
+                 : mode == PM_SWEEP
+                     ? launch_poly<PM_SWEEP>(c, P, x, r64, rscale, true_res, thresh, new_max_it,
+                                             si, so, pi, po)
+                     : launch_poly<PM_REFRESH>(c, P, x, r64, rscale, true_res, thresh,
+                                               new_max_it, si, so, pi, po);
+        return rc;
+    } else {
+        return PSI_BAD_ARG;
+    }
+}
 
 // Launch bookkeeping of one solve: launch L reads state/partials (L-1) & 1
 // and writes L & 1 (the setup is launch 0 and reads the host-written state 0).
@@ -389,9 +438,22 @@ int cg_setup(psi_ctx* c, const CgPath<T>& P, const TB* b, double bscale, T* x, b
     HIPCHK(c, hipMemcpyAsync(c->st, &c->h_st[2], sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     Ln.L = 0;
     Ln.G = P.G;
-    hipLaunchKernelGGL((k_cg_setup<T, TB>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
-                       P.rx, P.ry, c->act, b, bscale, zero_guess ? (const T*)nullptr : (const T*)x,
-                       zero_guess ? x : (T*)nullptr, P.u[0], P.p[0], Ln.part_out(c));
+    if (P.poly > 0) {
+        // the polynomial path starts from a zero first guess on the residual b
+        if constexpr (std::is_same<TB, double>::value) {
+            if (!zero_guess) return fail(c, PSI_BAD_ARG, "internal: poly setup%s");
+            if (launch_poly_checked<T>(c, PM_SETUP, P, x, b, bscale, nullptr, 0.0, 0, nullptr,
+                                       nullptr, nullptr, Ln.part_out(c)))
+                return fail(c, PSI_BAD_ARG, "polynomial degree%s");
+        } else {
+            return fail(c, PSI_BAD_ARG, "internal: poly setup%s");
+        }
+    } else {
+        hipLaunchKernelGGL((k_cg_setup<T, TB>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E,
+                           P.N, P.rx, P.ry, c->act, b, bscale,
+                           zero_guess ? (const T*)nullptr : (const T*)x,
+                           zero_guess ? x : (T*)nullptr, P.u[0], P.p[0], Ln.part_out(c));
+    }
     Ln.L = 1;
     return check_launch(c);
 }
@@ -408,10 +470,17 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
     const long L0 = Ln.L;
     HIPCHK(c, hipEventRecord(c->ts0, c->stream));
     auto issue = [&]() -> int {
-        for (int k = 0; k < batch && Ln.L - L0 < max_launch; ++k, ++Ln.L)
-            hipLaunchKernelGGL((k_cg_sweep<T>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
-                               P.rx, P.ry, c->act, x, P.u[0], P.u[1], P.p[0], P.p[1],
-                               Ln.st_in(c), Ln.st_out(c), Ln.part_in(c), Ln.part_out(c));
+        for (int k = 0; k < batch && Ln.L - L0 < max_launch; ++k, ++Ln.L) {
+            if (P.poly > 0)
+                (void)launch_poly_checked<T>(c, PM_SWEEP, P, x, nullptr, 0.0, nullptr, 0.0, 0,
+                                             Ln.st_in(c), Ln.st_out(c), Ln.part_in(c),
+                                             Ln.part_out(c));
+            else
+                hipLaunchKernelGGL((k_cg_sweep<T>), dim3(P.G), dim3(NT), 0, c->stream, c->g,
+                                   P.E, P.N, P.rx, P.ry, c->act, x, P.u[0], P.u[1], P.p[0],
+                                   P.p[1], Ln.st_in(c), Ln.st_out(c), Ln.part_in(c),
+                                   Ln.part_out(c));
+        }
         int r = check_launch(c);
         if (r) return r;
         HIPCHK(c, hipEventRecord(c->ts1[slot], c->stream));
@@ -467,10 +536,17 @@ int residual(psi_ctx* c, const double* b, const double* x, double* r_out, double
 template <class T>
 int cg_refresh(psi_ctx* c, const CgPath<T>& P, double rscale, double rr_thresh, int new_max_it,
                Launches& Ln) {
-    hipLaunchKernelGGL((k_cg_refresh<T>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E, P.N, P.rx,
-                       P.ry, c->act, c->r64, rscale, P.u[0], P.u[1], (const T*)P.p[0],
-                       (const T*)P.p[1], c->result, rr_thresh, new_max_it, Ln.st_in(c),
-                       Ln.st_out(c), Ln.part_out(c));
+    if (P.poly > 0) {
+        if (launch_poly_checked<T>(c, PM_REFRESH, P, (T*)nullptr, c->r64, rscale, c->result,
+                                   rr_thresh, new_max_it, Ln.st_in(c), Ln.st_out(c), nullptr,
+                                   Ln.part_out(c)))
+            return fail(c, PSI_BAD_ARG, "polynomial degree%s");
+    } else {
+        hipLaunchKernelGGL((k_cg_refresh<T>), dim3(P.G), dim3(NT), 0, c->stream, c->g, P.E, P.N,
+                           P.rx, P.ry, c->act, c->r64, rscale, P.u[0], P.u[1], (const T*)P.p[0],
+                           (const T*)P.p[1], c->result, rr_thresh, new_max_it, Ln.st_in(c),
+                           Ln.st_out(c), Ln.part_out(c));
+    }
     ++Ln.L;
     return check_launch(c);
 }
@@ -607,6 +683,64 @@ int solve_plain(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, 
     return solve_pcg<double>(c, o, P, forc, sol, sol, 1.0, true, 0, 0.0, info);
 }
 
+// Chebyshev iteration scalars for ZA on [a, b] = [lower * 2, 2] (degree k).
+ChebCoef make_cheb(int k, double lower) {
+    ChebCoef ch{};
+    const double b = 2.0, a = lower * b;
+    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    ch.inv_theta = (float)(1.0 / theta);
+    for (int j = 0; j < k && j < PMAXK; ++j) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        ch.c1[j] = (float)(rn * rho);
+        ch.c2[j] = (float)(2.0 * rn / delta);
+        rho = rn;
+    }
+    return ch;
+}
+
+// Resident grid of k_poly_pcg<K, PM_SWEEP> (queried once per degree).
+int poly_grid(psi_ctx* c, int k) {
+    if (c->Gpoly[k]) return c->Gpoly[k];
+    int ncu = 0, occ = 0;
+    const void* fn = nullptr;
+    switch (k) {
+        case 1: fn = reinterpret_cast<const void*>(&k_poly_pcg<1, PM_SWEEP>); break;
+        case 2: fn = reinterpret_cast<const void*>(&k_poly_pcg<2, PM_SWEEP>); break;
+        case 3: fn = reinterpret_cast<const void*>(&k_poly_pcg<3, PM_SWEEP>); break;
+        default: fn = reinterpret_cast<const void*>(&k_poly_pcg<4, PM_SWEEP>); break;
+    }
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, NT, 0) != hipSuccess)
+        return 0;
+    const char* ov = getenv("PSI_GRID_PER_CU");
+    if (ov && atoi(ov) > 0) occ = std::min(occ, atoi(ov));
+    int G = std::max(1, ncu * std::max(occ, 1));
+    G = std::min(G, std::max(c->G, c->G32));   // the partials buffer holds max(G, G32)
+    G = std::min(G, poly_geom(c->g, k).ntiles);
+    if (G >= 8) G &= ~7;
+    c->Gpoly[k] = G;
+    return G;
+}
+
+int solve_poly(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
+    const int k = o->poly_degree > 0 ? o->poly_degree : 3;
+    if (k > 4) return fail(c, PSI_BAD_ARG, "poly_degree must be 1..4%s");
+    const double lower = o->poly_lower > 0 ? o->poly_lower : 0.02;
+    if (!(lower < 1.0)) return fail(c, PSI_BAD_ARG, "poly_lower must be in (0, 1)%s");
+    CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]},
+                    poly_grid(c, k)};
+    if (P.G <= 0) return fail(c, PSI_HIP_ERROR, "occupancy query failed%s");
+    P.poly = k;
+    P.ch = make_cheb(k, lower);
+    P.rb[0] = c->rp32[0];
+    P.rb[1] = c->rp32[1];
+    const double red = o->inner_reduction > 0 ? o->inner_reduction : 0.25;
+    return solve_pcg<float>(c, o, P, forc, sol, c->e32, 1.0, false, o->leg_max_iterations, red,
+                            info);
+}
+
 int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
     CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]},
                     c->G32};
@@ -616,7 +750,8 @@ int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, 
     // need ~11.9k iterations to 1e-10, 0.05 needs 13.6k, 0.01 needs 30.6k (the
     // FP32 recursion drifts from the true residual over long legs).
     const double red = o->inner_reduction > 0 ? o->inner_reduction : 0.25;
-    return solve_pcg<float>(c, o, P, forc, sol, c->e32, 1.0, false, o->reserved[0], red, info);
+    return solve_pcg<float>(c, o, P, forc, sol, c->e32, 1.0, false, o->leg_max_iterations, red,
+                            info);
 }
 
 bool valid_field_ctx(psi_ctx* c) { return c && c->E; }
@@ -713,6 +848,8 @@ int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
             TRY(dalloc(c, &c->p32[k], c->plane));
         }
         TRY(dalloc(c, &c->e32, c->plane));
+        TRY(dalloc(c, &c->rp32[0], c->plane));
+        TRY(dalloc(c, &c->rp32[1], c->plane));
         TRY(dalloc(c, &c->r64, c->plane));
         TRY(dalloc(c, &c->r32, c->plane));
         TRY(dalloc(c, &c->z32, c->plane));
@@ -862,6 +999,7 @@ int psi_congrad(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, 
     switch (o->mode) {
         case PSI_MODE_PLAIN: rc = solve_plain(c, o, forc, sol, info); break;
         case PSI_MODE_MIXED: rc = solve_mixed(c, o, forc, sol, info); break;
+        case PSI_MODE_POLY: rc = solve_poly(c, o, forc, sol, info); break;
         case PSI_MODE_LEARNED: return fail(c, PSI_NO_LEARNED_MODEL, "no learned model loaded%s");
         default: return fail(c, PSI_BAD_ARG, "unknown mode%s");
     }
@@ -1256,6 +1394,17 @@ int psi_nn_weighted_dot(psi_ctx* c, const double* a, const double* b, double* ou
             s += ha[i] * hb[i] * c->h_dxu[col] * c->h_cosu[r] * c->h_dyu[r];
         }
     *out = s;
+    return PSI_OK;
+}
+
+int psi_poly_coefficients(int32_t degree, double lower, float* out) {
+    if (degree < 1 || degree > 4 || !(lower > 0 && lower < 1) || !out) return PSI_BAD_ARG;
+    const ChebCoef ch = make_cheb(degree, lower);
+    out[0] = ch.inv_theta;
+    for (int j = 0; j < degree; ++j) {
+        out[1 + 2 * j] = ch.c1[j];
+        out[2 + 2 * j] = ch.c2[j];
+    }
     return PSI_OK;
 }
 

@@ -1,0 +1,296 @@
+// psi_poly.hip -- FP32 PCG sweeps with a Chebyshev polynomial preconditioner
+// (included into psi_capi.hip after psi_kernels.hip).
+//
+// No counterpart in the reference, which preconditions congrad_streamfunction
+// with Jacobi only (make_inv_sfc / inv_op_sfc, solve_streamfct.f90:364-422).
+// Here the preconditioner is M = q(ZA) Z, Z the same Jacobi inverse (1/C on
+// active points, 0 on land and island perimeters) and q the degree-K
+// polynomial of the Chebyshev iteration for ZA e = Z r on the interval
+// [a, b] (b = 2 bounds the spectrum of ZA: it is similar to a symmetric
+// matrix, W A being symmetric, and Gershgorin puts it in [0, 2]).  M is
+// linear and fixed, so plain CG applies.  PCG iterations drop about
+// (K+1)-fold while the stencil applications per solve stay the same.  The
+// sweep is bound by its fixed cost per launch rather than by bandwidth, so
+// the K extra applications per iteration, done in registers on a frame whose
+// halo grows to 2+K, cost far less than the launches they save.
+//
+// Per iteration the sweep keeps the single-reduction recurrence of the
+// Jacobi sweep: p = z + beta p; r -= alpha A p; z = M r; x += alpha p;
+// w = A z, s = A p; partials (r,z), (z,w), max|p|, |r|^2, (z,s)+(p,w), (p,s).
+// Stored vectors (FP32): x, r, z, p (r and z are no longer proportional
+// point by point, so both are kept).
+
+namespace psi {
+
+constexpr int PMAXK = 6;
+
+// Chebyshev iteration scalars: d_0 = Zr / theta; d_j = c1[j-1] d_{j-1} +
+// c2[j-1] res_{j-1}; res_j = res_{j-1} - ZA d_j; z = sum_j d_j.
+struct ChebCoef {
+    float inv_theta;
+    float c1[PMAXK], c2[PMAXK];
+};
+
+enum : int { PM_SETUP = 0, PM_SWEEP = 1, PM_REFRESH = 2 };
+
+// Frame of one tile: 64 columns (one per lane) x 4 * RPT rows, wave w of the
+// workgroup holding rows [w RPT, (w+1) RPT) of every column in registers.
+// The own points are the frame minus a margin R = K + 2.
+template <int K, int RPT>
+struct PolyShape {
+    static constexpr int R = K + 2, FW = 64, FH = 4 * RPT;
+    static constexpr int TXo = FW - 2 * R, TYo = FH - 2 * R;
+};
+
+// Rows per thread for degree K: the own tile is (64 - 2R) x (4 RPT - 2R).
+template <int K>
+struct PolyRows {
+    static constexpr int value = K <= 2 ? 6 : 7;
+};
+
+// Neighbouring columns of the frame are neighbouring lanes.  Lanes 0 and 63
+// (frame edges, never inside a ring that is computed) get their own value.
+__device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
+
+// Rows above and below a wave's band, exchanged through LDS:
+// xch[buf][vec][0 = first row | 1 = last row][wave][lane].
+struct Exchange {
+    float x[2][2][2][NW][64];
+};
+
+// The 5-point operator on a strip held in registers, summed S, W, C, E, N
+// (the order of stencil5); up / dn are the rows above and below the strip.
+template <int RPT>
+__device__ __forceinline__ void apply_strip(const float (&v)[RPT], float up, float dn,
+                                            const float (&cS)[RPT], const float (&cW)[RPT],
+                                            const float (&cE)[RPT], const float (&cN)[RPT],
+                                            float (&out)[RPT]) {
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const float vu = i ? v[i - 1] : up, vd = (i < RPT - 1) ? v[i + 1] : dn;
+        const float vl = from_left(v[i]), vr = from_right(v[i]);
+        const float cc = (((-cE[i]) - cW[i]) - cN[i]) - cS[i];
+        float a = cS[i] * vu;
+        a = a + cW[i] * vl;
+        a = a + cc * v[i];
+        a = a + cE[i] * vr;
+        a = a + cN[i] * vd;
+        out[i] = a;
+    }
+}
+
+// Publishes the first and last rows of up to two strips and returns the
+// rows adjacent to this wave's band (one barrier; buffers alternate, so a
+// buffer is rewritten only after the barrier that follows its last read).
+template <int RPT, int NV>
+__device__ __forceinline__ void exchange(Exchange& X, int& buf, const float (&a)[RPT],
+                                         const float (&b)[RPT], float* up, float* dn) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    X.x[buf][0][0][w][lane] = a[0];
+    X.x[buf][0][1][w][lane] = a[RPT - 1];
+    if (NV > 1) {
+        X.x[buf][1][0][w][lane] = b[0];
+        X.x[buf][1][1][w][lane] = b[RPT - 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        up[v] = w > 0 ? X.x[buf][v][1][w - 1][lane] : 0.f;
+        dn[v] = w < NW - 1 ? X.x[buf][v][0][w + 1][lane] : 0.f;
+    }
+    buf ^= 1;
+}
+
+// One launch of the polynomial-preconditioned FP32 PCG on tiles of gp (the
+// (64 - 2R) x (4 RPT - 2R) tiling).  MODE:
+//  PM_SETUP   r = r64 * rscale; p = 0; e = 0; z = M r (buffers 0); partials for
+//             cg_from_setup (slot 2 = max|z|, slot 4 unused: bb is fixed).
+//  PM_SWEEP   one fused iteration; the state advances as in k_cg_sweep.
+//  PM_REFRESH r = r64 * rscale, z = M r next to the kept p (buffers n & 1);
+//             partials for cg_from_refresh; state handling as k_cg_refresh.
+template <int K, int MODE>
+__global__ __launch_bounds__(NT) void k_poly_pcg(
+    Geom gp, ChebCoef ch, const float* __restrict__ E, const float* __restrict__ N,
+    const float* __restrict__ rx, const float* __restrict__ ry, const uint8_t* __restrict__ act,
+    float* __restrict__ x, float* zb0, float* zb1, float* pb0, float* pb1, float* rb0,
+    float* rb1, const double* __restrict__ r64, double rscale,
+    const double* __restrict__ true_res, double rr_thresh, int new_max_it,
+    const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
+    const double* __restrict__ part_in, double* __restrict__ part_out) {
+    constexpr int RPT = PolyRows<K>::value;
+    using PS = PolyShape<K, RPT>;
+    constexpr int R = PS::R, FW = PS::FW, FH = PS::FH, TXo = PS::TXo, TYo = PS::TYo;
+    __shared__ Exchange X;
+    __shared__ double sRed[NRED * NW];
+    __shared__ CgState sst;
+    const Geom& g = gp;
+    float alpha = 0.f, beta = 0.f;
+    int n = 0;
+    if (MODE == PM_SWEEP) {
+        advance_state(st_in, st_out, part_in, PHASE_SWEEP, sst, sRed);
+        if (sst.done) return;
+        alpha = (float)sst.alpha;
+        beta = (float)sst.beta;
+        n = sst.n;
+    } else if (MODE == PM_REFRESH) {
+        const bool finished = true_res[0] <= rr_thresh;
+        if (threadIdx.x == 0 && blockIdx.x == 0) {
+            CgState s = *st_in;
+            s.phase = PHASE_REFRESH;
+            s.max_it = new_max_it;
+            s.degenerate = 0;
+            s.done = finished ? 1 : 0;
+            s.status = finished ? ST_CONVERGED : ST_RUNNING;
+            *st_out = s;
+        }
+        if (finished) return;
+        n = st_in->n;
+    }
+    // buffers: after n iterations z_n, r_n and p_{n-1} live in buffer n & 1
+    const int odd = n & 1;
+    const float* __restrict__ z_in = odd ? zb1 : zb0;
+    const float* __restrict__ p_in = odd ? pb1 : pb0;
+    const float* __restrict__ r_in = odd ? rb1 : rb0;
+    float* __restrict__ z_out = (MODE == PM_SWEEP) ? (odd ? zb0 : zb1) : (odd ? zb1 : zb0);
+    float* __restrict__ p_out = (MODE == PM_SWEEP) ? (odd ? pb0 : pb1) : (odd ? pb1 : pb0);
+    float* __restrict__ r_out = (MODE == PM_SWEEP) ? (odd ? rb0 : rb1) : (odd ? rb1 : rb0);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
+    int xbuf = 0;
+    int t0, ts, te;
+    tile_schedule(g, t0, ts, te);
+    for (int tile = t0; tile < te; tile += ts) {
+        const int c0 = (tile % g.tiles_x) * TXo, r0 = (tile / g.tiles_x) * TYo;
+        // frame row 0 / column 0 sit R - 2 rows / columns before the tile's
+        // first own point (array row / column 2 + r0 / c0)
+        const int fr0 = r0 + 2 - R, fc0 = c0 + 2 - R;
+        const int col = map_col(fc0 + lane, g);
+        const bool col_in = col >= 0 && col < g.nx + 4;
+        const float rxv = col_in ? rx[col] : 0.f;
+        // ---- load the band: vectors, coefficients, own x -------------------
+        float P[RPT], Rr[RPT], cS[RPT], cW[RPT], cE[RPT], cN[RPT], zi[RPT], xo[RPT];
+        unsigned amask = 0u;
+        float nprev;
+        {
+            const int r = fr0 + w * RPT - 1;
+            nprev = (col_in && r >= 0 && r < g.ny + 4) ? N[(size_t)r * g.pitch + col] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int gg = w * RPT + i, r = fr0 + gg;
+            float pv = 0.f, rv = 0.f, e = 0.f, nn = 0.f, ryv = 0.f, xv = 0.f;
+            uint8_t a = 0;
+            if (col_in && r >= 0 && r < g.ny + 4) {
+                const size_t o = (size_t)r * g.pitch + col;
+                if (MODE == PM_SWEEP) {
+                    pv = z_in[o] + beta * p_in[o];
+                    rv = r_in[o];
+                } else {
+                    if (MODE == PM_REFRESH) pv = p_in[o];
+                    rv = (float)(r64[o] * rscale);
+                }
+                e = E[o];
+                nn = N[o];
+                a = act[o];
+                ryv = ry[r];
+                if (MODE == PM_SWEEP && gg >= R && gg < FH - R && lane >= R && lane < FW - R)
+                    xv = x[o];
+            }
+            P[i] = pv;
+            Rr[i] = rv;
+            cE[i] = e;
+            cN[i] = nn;
+            cS[i] = nprev * ryv;
+            nprev = nn;
+            xo[i] = xv;
+            amask |= (a ? 1u : 0u) << i;
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            cW[i] = from_left(cE[i]) * rxv;
+            const float cc = (((-cE[i]) - cW[i]) - cN[i]) - cS[i];
+            zi[i] = ((amask >> i) & 1u) ? 1.f / cc : 0.f;
+        }
+        float up[2], dn[2], tmp[RPT];
+        if (MODE == PM_SWEEP) {
+            // r_{k+1} = r_k - alpha A p_k (valid on radius K+1)
+            exchange<RPT, 1>(X, xbuf, P, P, up, dn);
+            apply_strip<RPT>(P, up[0], dn[0], cS, cW, cE, cN, tmp);
+#pragma unroll
+            for (int i = 0; i < RPT; ++i)
+                if ((amask >> i) & 1u) Rr[i] = Rr[i] - alpha * tmp[i];
+        }
+        // ---- z = q(ZA) Z r: Chebyshev iteration, valid on radius 1 --------
+        float D[RPT], S[RPT], Y[RPT];
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const float v = zi[i] * Rr[i];
+            S[i] = v;
+            D[i] = v * ch.inv_theta;
+            Y[i] = D[i];
+        }
+#pragma unroll
+        for (int j = 1; j <= K; ++j) {
+            exchange<RPT, 1>(X, xbuf, D, D, up, dn);
+            apply_strip<RPT>(D, up[0], dn[0], cS, cW, cE, cN, tmp);
+            const float c1 = ch.c1[j - 1], c2 = ch.c2[j - 1];
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                S[i] = S[i] - zi[i] * tmp[i];
+                D[i] = c1 * D[i] + c2 * S[i];
+                Y[i] = Y[i] + D[i];
+            }
+        }
+        // ---- own points: w = A z, s = A p, outputs and partials -----------
+        float Wv[RPT], Sv[RPT];
+        exchange<RPT, 2>(X, xbuf, Y, P, up, dn);
+        apply_strip<RPT>(Y, up[0], dn[0], cS, cW, cE, cN, Wv);
+        apply_strip<RPT>(P, up[1], dn[1], cS, cW, cE, cN, Sv);
+        const int c = fc0 + lane;
+        const bool own_col = lane >= R && lane < FW - R && c <= g.nx + 1;
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int gg = w * RPT + i, r = fr0 + gg;
+            if (own_col && gg >= R && gg < FH - R && r <= g.ny + 1) {
+                const size_t o = (size_t)r * g.pitch + c;
+                const float zn = Y[i], pn = P[i], rn = Rr[i];
+                if (MODE == PM_SWEEP) {
+                    x[o] = xo[i] + alpha * pn;
+                    p_out[o] = pn;
+                }
+                if (MODE == PM_SETUP) {
+                    x[o] = 0.f;
+                    p_out[o] = 0.f;
+                }
+                z_out[o] = zn;
+                r_out[o] = rn;
+                const double zd = (double)zn, pd = (double)pn, rd = (double)rn;
+                const double wd = (double)Wv[i], sd = (double)Sv[i];
+                if ((amask >> i) & 1u) {
+                    acc[0] += rd * zd;
+                    acc[3] += rd * rd;
+                }
+                acc[1] += zd * wd;
+                acc[2] = fmax(acc[2], MODE == PM_SETUP ? fabs(zd) : fabs(pd));
+                acc[4] += zd * sd + pd * wd;
+                acc[5] += pd * sd;
+            }
+        }
+    }
+    block_reduce<NRED>(acc, 1u << 2, sRed);
+    write_partials<NRED>(acc, part_out);
+}
+
+// Tiling of the polynomial kernels of degree K over the grid of g.
+inline Geom poly_geom(const Geom& g, int K) {
+    Geom gp = g;
+    const int R = K + 2, rpt = K <= 2 ? 6 : 7;
+    const int txo = 64 - 2 * R, tyo = 4 * rpt - 2 * R;
+    gp.tiles_x = (g.nx + txo - 1) / txo;
+    gp.tiles_y = (g.ny + tyo - 1) / tyo;
+    gp.ntiles = gp.tiles_x * gp.tiles_y;
+    return gp;
+}
+
+}  // namespace psi

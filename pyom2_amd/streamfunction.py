@@ -34,7 +34,7 @@ from ._lib import PsiError, PsiFields, PsiGrid, PsiInfo, PsiIslands, PsiOpts
 AB_EPS_REFERENCE = float(np.float32(0.1))
 
 MODES = {"plain": _lib.PSI_MODE_PLAIN, "mixed": _lib.PSI_MODE_MIXED,
-         "learned": _lib.PSI_MODE_LEARNED}
+         "learned": _lib.PSI_MODE_LEARNED, "poly": _lib.PSI_MODE_POLY}
 CRITERIA = {"reference": _lib.PSI_CRIT_REFERENCE, "relres": _lib.PSI_CRIT_RELRES}
 
 
@@ -105,6 +105,8 @@ class SolveOptions:
     check_every: int = 0
     inner_reduction: float = 0.0
     max_outer: int = 0
+    poly_degree: int = 0
+    poly_lower: float = 0.0
 
     def as_c(self):
         o = PsiOpts()
@@ -116,6 +118,8 @@ class SolveOptions:
         o.check_every = self.check_every
         o.inner_reduction = self.inner_reduction
         o.max_outer = self.max_outer
+        o.poly_degree = self.poly_degree
+        o.poly_lower = self.poly_lower
         return o
 
 

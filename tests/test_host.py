@@ -89,3 +89,41 @@ def test_basin_generators():
     ens = basins.ensemble_basins(seed=0, count=6)
     for e in ens:
         assert 16 <= e.nx <= 1024 and 16 <= e.ny <= 1024
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+@pytest.mark.parametrize("lower", [0.02, 0.1])
+def test_poly_preconditioner_is_the_chebyshev_polynomial(degree, lower):
+    """The scalars of the polynomial preconditioner, run through their
+    recurrence on a scalar eigenvalue l of Z A, give q(l) whose residual
+    polynomial p(l) = 1 - l q(l) is the scaled Chebyshev polynomial
+    T_{k+1}((b + a - 2l) / (b - a)) / T_{k+1}((b + a) / (b - a)) on [0, b],
+    a = 2 lower, b = 2: p(0) = 1, |p| <= 1 / T_{k+1}(theta) on [a, b] with
+    equality at the k+2 extrema, and 0 < l q(l) < 2 on (0, b]."""
+    import ctypes as C
+    L = _lib.lib()
+    out = (C.c_float * (1 + 2 * degree))()
+    assert L.psi_poly_coefficients(degree, lower, out) == 0
+    out = np.array(out, np.float64)
+    a, b = 2 * lower, 2.0
+
+    def q(lam):
+        v = 1.0                      # Z r for an eigenvector of Z A
+        d = v * out[0]
+        z, res = d, v - lam * d
+        for j in range(1, degree + 1):
+            d = out[2 * j - 1] * d + out[2 * j] * res
+            z += d
+            res = res - lam * d
+        return z
+
+    lam = np.linspace(0.0, b, 4001)[1:]
+    p = 1 - lam * np.array([q(x) for x in lam])
+    theta = (b + a) / (b - a)
+    cheb = np.polynomial.chebyshev.Chebyshev.basis(degree + 1)
+    expect = cheb((b + a - 2 * lam) / (b - a)) / cheb(theta)
+    np.testing.assert_allclose(p, expect, atol=2e-5)
+    inside = lam >= a
+    assert np.abs(p[inside]).max() <= 1 / cheb(theta) * (1 + 1e-4)
+    assert ((1 - p) > 0).all() and ((1 - p) < 2).all()
+    assert L.psi_poly_coefficients(5, lower, out.ctypes.data_as(C.POINTER(C.c_float))) == 5
