@@ -8,9 +8,11 @@ warm start dpsi(taup1) = 2 dpsi(tau) - dpsi(taum1).  A step is one solve.
 Inputs are resident in HBM before the timed region starts.
 
 Paths: "learned" is the learned-preconditioner solver with its measured
-policy (flexible CG with the trained network or Jacobi-PCG, whichever needs
-fewer ms per residual decade on a probe from the same first guess; probes at
-the first solve and every --reprobe-every solves); "mixed" is FP32-inner /
+policy (flexible CG with the trained network or the fastest classical path
+measured here, whichever needs fewer ms per residual decade on a probe from
+the same first guess; probes at the first solve and every --reprobe-every
+solves); "poly" is FP32-inner / FP64-refined PCG with a Chebyshev polynomial
+of the Jacobi-preconditioned operator (degree 4); "mixed" is FP32-inner /
 FP64-refined Jacobi-PCG; "plain" is FP64 Jacobi-PCG.
 
 Prints one JSON line with the headline path's ms per solve, the other
@@ -50,7 +52,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--modes", default="learned,mixed,plain", help="first one is the headline")
+    ap.add_argument("--modes", default="learned,poly,mixed,plain",
+                    help="first one is the headline")
     ap.add_argument("--weights", default="", help="learned preconditioner weights (.npz)")
     ap.add_argument("--reprobe-every", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -60,7 +63,7 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--check-every", type=int, default=0)
-    ap.add_argument("--poly-degree", type=int, default=3)
+    ap.add_argument("--poly-degree", type=int, default=0, help="0: library default")
     ap.add_argument("--poly-lower", type=float, default=0.0)
     ap.add_argument("--inner-reduction", type=float, default=0.0,
                     help="mixed path: residual reduction per FP32 leg (0: library default)")
@@ -115,7 +118,10 @@ def main():
     modes = args.modes.split(",")
     results = {}
     for mode in modes:
-        opts = SolveOptions(mode="mixed" if mode == "learned" else mode, criterion="relres",
+        # the learned path falls back to the fastest classical preconditioner
+        # among the paths measured here
+        fallback = "poly" if "poly" in modes else "mixed"
+        opts = SolveOptions(mode=fallback if mode == "learned" else mode, criterion="relres",
                             relres_target=args.relres, max_iterations=200000,
                             check_every=args.check_every, inner_reduction=args.inner_reduction,
                             poly_degree=args.poly_degree, poly_lower=args.poly_lower)
@@ -175,8 +181,9 @@ def main():
                              total_s=total)
         if ls is not None:
             timed = choices[args.warmup:]
-            results[mode].update(choice=max(set(timed), key=timed.count), probes=probes,
-                                 reprobe_every=args.reprobe_every)
+            ch = max(set(timed), key=timed.count)
+            results[mode].update(choice=ch if ch == "learned" else f"{ch} fallback ({fallback})",
+                                 probes=probes, reprobe_every=args.reprobe_every)
             for key in ("sweep_avg_launch_us", "sweep_launches", "achieved_GBs", "frac"):
                 results[mode].pop(key)
         log(f"[bench] {mode}: {results[mode]}")
@@ -185,7 +192,8 @@ def main():
     r = results[head]
     # the roofline figure is the fused sweep kernel of the Jacobi path that
     # carries the solve (the learned policy's fallback is the mixed path)
-    kpath = head if head != "learned" else ("mixed" if "mixed" in results else None)
+    kpath = head if head != "learned" else next((m for m in ("poly", "mixed") if m in results),
+                                                   None)
     rk = results.get(kpath, {}) if kpath else {}
     cpu = None
     if not args.skip_cpu:
@@ -209,10 +217,12 @@ def main():
         "paths": {m: {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
                       for k, v in d.items()} for m, d in results.items()},
         "roofline": ({"bound": "hbm",
-                      "kernel": f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>",
+                      "kernel": ("k_poly_pcg<4, sweep>" if kpath == "poly" else
+                                 f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>"),
                       "path": kpath, "achieved": round(rk["achieved_GBs"], 1),
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rk["frac"], 4),
-                      "traffic": TRAFFIC_2048.get(kpath) if b.nx == b.ny == 2048 else None,
+                      "traffic": (TRAFFIC_2048.get(kpath) if b.nx == b.ny == 2048 else
+                                  None),
                       "traffic_unit": "bytes per launch",
                       "bytes_per_pt_iter": BYTES_PER_PT[kpath],
                       "avg_launch_us": round(rk["sweep_avg_launch_us"], 3)} if rk else None),
@@ -223,6 +233,8 @@ def main():
     if head == "learned" and "mixed" in results:
         line["learned_vs_jacobi_mixed"] = round(results["mixed"]["ms_per_solve"] /
                                                 r["ms_per_solve"], 3)
+    if head == "learned":
+        line["config"]["path"] = f"learned policy (chose {r.get('choice')})"
     print(json.dumps(finite(line)), flush=True)
 
 

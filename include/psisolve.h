@@ -93,9 +93,9 @@ typedef struct psi_opts {
     double inner_reduction;    /* mixed/learned: relative residual reduction per
                                   FP32 inner solve; <= 0 picks the default 0.25 */
     int32_t max_outer;         /* mixed/learned: FP64 refinement steps; <= 0 default */
-    int32_t poly_degree;       /* PSI_MODE_POLY: 1..4; <= 0 picks 3 */
+    int32_t poly_degree;       /* PSI_MODE_POLY: 1..6; <= 0 picks 4 */
     double poly_lower;         /* PSI_MODE_POLY: the polynomial's interval is
-                                  [2 poly_lower, 2]; <= 0 picks 0.02 */
+                                  [2 poly_lower, 2]; <= 0 picks 0.005 */
     int32_t leg_max_iterations;/* mixed/poly: inner iterations per FP32 leg; <= 0: no cap */
     int32_t reserved[3];
 } psi_opts;
@@ -235,7 +235,7 @@ int psi_fcg_scalars(psi_ctx *ctx, double *out10);
 
 /* Scalars of the PSI_MODE_POLY preconditioner (no reference counterpart):
  * the Chebyshev iteration for Z A e = Z r on [2 lower, 2] of the given degree
- * (1..4), applied as d_0 = Z r * out[0]; d_j = out[2j-1] d_{j-1} +
+ * (1..6), applied as d_0 = Z r * out[0]; d_j = out[2j-1] d_{j-1} +
  * out[2j] res_{j-1}; res_j = res_{j-1} - Z A d_j; z = sum_j d_j.  out holds
  * 1 + 2 degree floats. */
 int psi_poly_coefficients(int32_t degree, double lower, float *out);

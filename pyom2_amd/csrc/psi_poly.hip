@@ -45,7 +45,7 @@ struct PolyShape {
 // Rows per thread for degree K: the own tile is (64 - 2R) x (4 RPT - 2R).
 template <int K>
 struct PolyRows {
-    static constexpr int value = K <= 2 ? 6 : 7;
+    static constexpr int value = K <= 2 ? 6 : (K <= 4 ? 7 : 8);
 };
 
 // Neighbouring columns of the frame are neighbouring lanes.  Lanes 0 and 63
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(NT) void k_poly_pcg(
 // Tiling of the polynomial kernels of degree K over the grid of g.
 inline Geom poly_geom(const Geom& g, int K) {
     Geom gp = g;
-    const int R = K + 2, rpt = K <= 2 ? 6 : 7;
+    const int R = K + 2, rpt = K <= 2 ? 6 : (K <= 4 ? 7 : 8);
     const int txo = 64 - 2 * R, tyo = 4 * rpt - 2 * R;
     gp.tiles_x = (g.nx + txo - 1) / txo;
     gp.tiles_y = (g.ny + tyo - 1) / tyo;

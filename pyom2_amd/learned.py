@@ -422,10 +422,11 @@ class LearnedSolver:
 
     policy="auto": the first solve, and every `reprobe_every`-th after it,
     probes both methods from the same first guess -- `probe` flexible-CG
-    iterations and `probe_jacobi` Jacobi-PCG iterations -- and measures each
-    in milliseconds per decade of FP64 true relative residual.  The faster
-    one finishes that solve (from its probe's iterate) and serves the solves
-    until the next probe.  policy="learned" runs flexible CG only (the
+    iterations and `probe_jacobi` iterations of the fallback solver
+    (`fallback_opts`: Jacobi-PCG, mixed precision, unless given) -- and
+    measures each in milliseconds per decade of FP64 true relative residual.
+    The faster one finishes that solve (from its probe's iterate) and serves
+    the solves until the next probe.  policy="learned" runs flexible CG only (the
     `fallback_opts` Jacobi-PCG solve takes over if it has not converged after
     `max_iterations`); policy="jacobi" runs Jacobi-PCG only."""
 
@@ -511,9 +512,8 @@ class LearnedSolver:
         if policy == "auto" and (self.choice is None or self.solves % self.reprobe_every == 0):
             # both probes from the same first guess
             x0 = sol.clone()
-            (j_its, _, jinfo), j_ms = self._timed(lambda: self._jacobi(forc, sol, SolveOptions(
-                mode="mixed", criterion="relres", relres_target=relres_target,
-                max_iterations=self.probe_jacobi)))
+            probe_opts = dataclasses.replace(jac, max_iterations=self.probe_jacobi)
+            (j_its, _, jinfo), j_ms = self._timed(lambda: self._jacobi(forc, sol, probe_opts))
             x_jac = sol.clone()
             sol.copy_(x0)
             (f_its, f_rel, f_fl, f_rel0), f_ms = self._timed(lambda: self._fcg(
@@ -534,6 +534,7 @@ class LearnedSolver:
                 info["flushes"] += f_fl
         choice = {"auto": self.choice, "learned": "learned", "jacobi": "jacobi"}[policy]
         info["choice"] = choice
+        info["fallback_mode"] = jac.mode
         true_rel = None
         if choice == "learned":
             budget = budget_iterations if budget_iterations is not None else max_iterations

@@ -91,8 +91,8 @@ def test_basin_generators():
         assert 16 <= e.nx <= 1024 and 16 <= e.ny <= 1024
 
 
-@pytest.mark.parametrize("degree", [1, 2, 3, 4])
-@pytest.mark.parametrize("lower", [0.02, 0.1])
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 6])
+@pytest.mark.parametrize("lower", [0.005, 0.02, 0.1])
 def test_poly_preconditioner_is_the_chebyshev_polynomial(degree, lower):
     """The scalars of the polynomial preconditioner, run through their
     recurrence on a scalar eigenvalue l of Z A, give q(l) whose residual
@@ -126,4 +126,4 @@ def test_poly_preconditioner_is_the_chebyshev_polynomial(degree, lower):
     inside = lam >= a
     assert np.abs(p[inside]).max() <= 1 / cheb(theta) * (1 + 1e-4)
     assert ((1 - p) > 0).all() and ((1 - p) < 2).all()
-    assert L.psi_poly_coefficients(5, lower, out.ctypes.data_as(C.POINTER(C.c_float))) == 5
+    assert L.psi_poly_coefficients(7, lower, out.ctypes.data_as(C.POINTER(C.c_float))) == 5
