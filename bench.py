@@ -42,8 +42,8 @@ BYTES_PER_PT = {"plain": 6 * 8 + 2 * 8 + 1, "mixed": 6 * 4 + 2 * 4 + 1,
                 "poly": 8 * 4 + 2 * 4 + 1}
 # HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
-# profiles/k_cg_sweep_spherical_2048.md
-TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6}
+# profiles/k_cg_sweep_spherical_2048.md, profiles/k_poly_pcg_spherical_2048.md
+TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6, "poly": 175.5e6}
 
 
 def parse():
