@@ -73,6 +73,9 @@ def parse():
                          "batched basins of 64^2-512^2")
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--ensemble-mode", default="poly", choices=["poly", "mixed"],
+                    help="ensemble: FP32 legs preconditioned by the Chebyshev polynomial "
+                         "(poly) or by Jacobi alone (mixed)")
     ap.add_argument("--tune", default="", help="fixed schedule tile_x,tile_y,tiles_per_item,"
                                                 "items_per_grab[,groups_per_cu] instead of the "
                                                 "autotuner")
@@ -106,7 +109,9 @@ def main():
         log(f"[bench] calibration copy: 3 x {args.copy_mib} MiB")
         del src, dst
     t = time.time()
-    b = basins.spherical_basin(seed=0, n=args.n, n_islands=2)
+    # the 2048^2 basin's domain (102.4 deg of latitude about the equator) at
+    # any resolution: spacing 0.05 deg * 2048 / n
+    b = basins.spherical_basin(seed=0, n=args.n, n_islands=2, dlon=0.05 * 2048 / args.n)
     isl = islands_trace(b.kbot)
     log(f"[bench] basin {b.nx}x{b.ny} nisle={isl.nisle} built in {time.time() - t:.1f}s")
     s = StreamfunctionSolver.from_basin(b, islands=isl, device=dev)
@@ -211,7 +216,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "plain" else "f32-inner/f64-refined",
         "data": "synthetic",
-        "config": {"workload": f"spherical basin {b.nx}x{b.ny}, 0.05 deg, 2 islands, "
+        "config": {"workload": f"spherical basin {b.nx}x{b.ny}, {0.05 * 2048 / args.n:g} deg, "
+                               f"2 islands, "
                                f"AR(1) rho=0.99 forcings, warm-started solves, relres<{args.relres:g}",
                    "path": head, "nisle": isl.nisle},
         "paths": {m: {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
@@ -297,8 +303,9 @@ def ensemble_main(args):
         best, tune_rows = ens.autotune(launches=24)
         log(f"[ensemble] autotune {time.time() - t:.1f}s -> {best}")
     forcs = [torch.as_tensor(b.forc, device=dev) for b in bs]
-    opts = SolveOptions(mode="mixed", criterion="relres", relres_target=args.relres,
-                        max_iterations=200000)
+    opts = SolveOptions(mode=args.ensemble_mode, criterion="relres", relres_target=args.relres,
+                        max_iterations=200000, poly_degree=args.poly_degree,
+                        poly_lower=args.poly_lower)
     sols = ens.zeros()
     lat, its, per_step = [], [], []
     for k in range(args.warmup + args.steps):
@@ -324,7 +331,7 @@ def ensemble_main(args):
     # algorithmic bytes: FP32 inner sweep bytes per point-iteration over the
     # members' interior points
     point_its = float((its * npts[None, :]).sum())
-    achieved = BYTES_PER_PT["mixed"] * point_its / total / 1e9
+    achieved = BYTES_PER_PT[args.ensemble_mode] * point_its / total / 1e9
     cpu = None
     if not args.skip_cpu:
         cpu = ensemble_cpu_baseline(bs, its.mean(axis=0), args.cpu_iters)
@@ -344,15 +351,18 @@ def ensemble_main(args):
         "data": "synthetic",
         "config": {"workload": f"{len(bs)} beta-plane basins, N log-uniform in [64^2, 512^2], "
                                f"aspect 0.5-2, 0-3 islands, zero first guess, relres<{args.relres:g}",
+                   "path": args.ensemble_mode,
                    "points": int(npts.sum()), "active_points": ens.active_points,
                    "tune": ens.tune},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "max": round(float(lat.max()), 3)},
         "iterations": {"mean": round(float(its.mean()), 1), "max": int(its.max())},
-        "roofline": {"bound": "hbm", "kernel": "k_batch_step", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_pt_iter": BYTES_PER_PT["mixed"]},
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_batch_poly" if args.ensemble_mode == "poly" else "k_batch_step",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_pt_iter": BYTES_PER_PT[args.ensemble_mode]},
         "autotune": tune_rows,
         "cpu_baseline": cpu,
     }

@@ -46,6 +46,7 @@ struct BArrays {
     double* r64;
     float* e32;
     float *u0, *u1, *p0, *p1;
+    float *r0, *r1;              // residuals of the polynomial path (u holds z = M r)
 };
 
 struct BParams {
@@ -56,6 +57,7 @@ struct BParams {
     int K;                       // tiles per work item
     int grab;                    // work items per grab from the launch's counter
     int frozen;                  // autotuner: every member sweeps every launch
+    ChebCoef ch;                 // polynomial path: Chebyshev scalars
 };
 
 struct BState {
@@ -201,10 +203,10 @@ __device__ __forceinline__ void work_resid(const BMember& m, const BArrays& A, i
     for (int t = tb; t < te; ++t) {
         const int c0 = (t % g.tiles_x) * BX, r0 = (t / g.tiles_x) * BY;
 #pragma unroll
-        for (int k = 0; k < (BX * BY) / NT; ++k) {
+        for (int k = 0; k < (BX * BY + NT - 1) / NT; ++k) {
             const int l = threadIdx.x + k * NT;
             const int r = r0 + 2 + l / BX, c = c0 + 2 + l % BX;
-            if (r <= g.ny + 1 && c <= g.nx + 1) {
+            if (l < BX * BY && r <= g.ny + 1 && c <= g.nx + 1) {
                 const size_t o = (size_t)r * g.pitch + c;
                 const size_t row = (size_t)r * g.pitch;
                 const int cm = map_col(c - 1, g), cp = map_col(c + 1, g);
@@ -241,10 +243,10 @@ __device__ __forceinline__ void work_corr(const BMember& m, const BArrays& A, in
     for (int t = tb; t < te; ++t) {
         const int c0 = (t % g.tiles_x) * BX, r0 = (t / g.tiles_x) * BY;
 #pragma unroll
-        for (int k = 0; k < (BX * BY) / NT; ++k) {
+        for (int k = 0; k < (BX * BY + NT - 1) / NT; ++k) {
             const int l = threadIdx.x + k * NT;
             const int r = r0 + 2 + l / BX, c = c0 + 2 + l % BX;
-            if (r <= g.ny + 1 && c <= g.nx + 1) {
+            if (l < BX * BY && r <= g.ny + 1 && c <= g.nx + 1) {
                 const size_t o = (size_t)r * g.pitch + c;
                 x[o] = x[o] + (double)e32[o] * inv_scale;
                 e32[o] = 0.f;
@@ -512,6 +514,45 @@ __device__ __forceinline__ void work_sweep_run(const BMember& m, const SweepPtrs
     }
 }
 
+// Fold member s's partials of the previous launch, advance its state (every
+// item of the member computes the same), publish it (the item holding the
+// member's first segment) and leave it in sst.
+__device__ __forceinline__ void member_step(const BMember& m, int s, int seg0, int ph_in,
+                                            const BParams& P,
+                                            const BState* __restrict__ st_in,
+                                            BState* __restrict__ st_out,
+                                            const double* __restrict__ part_in,
+                                            BFinal* __restrict__ fin, BState& sst, double* sRed) {
+    double red[NRED] = {0, 0, 0, 0, 0, 0};
+    if (needs_fold(ph_in)) fold_member(part_in + (size_t)m.seg_base * NRED, m.nseg, red, sRed);
+    if (threadIdx.x == 0) {
+        BState st = st_in[s];
+        if (st.phase != BP_DONE) {
+            b_transition(st, red, P);
+            st.launches += 1;
+            if (st.phase == BP_DONE && seg0 == 0) {
+                BFinal f;
+                f.rrt = st.rrt;
+                f.bbt = st.bbt;
+                f.rr_rec = st.c.rr;
+                f.bb_rec = st.c.bb;
+                f.est_error = st.c.est_error;
+                f.t_done = wall_clock64();
+                f.done = 1;
+                f.status = st.c.status;
+                f.iterations = st.c.status == ST_MAXIT ? P.max_it + 1 : st.c.n;
+                f.legs = st.legs;
+                f.launches = st.launches;
+                f.pad = 0;
+                fin[s] = f;
+            }
+        }
+        sst = st;
+        if (seg0 == 0) st_out[s] = st;
+    }
+    __syncthreads();
+}
+
 // ---- the batched step --------------------------------------------------------
 // Launch L reads states/partials of parity L & 1 and writes (L + 1) & 1.
 // Workgroups grab chunks of `grab` consecutive work items from the launch's
@@ -554,34 +595,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(
         const int n_pred = P.frozen ? 0 : (ph_in == BP_SWEEP ? n_in + 1
                                                              : (ph_in == BP_SETUP ? 0 : n_in));
         if (may_sweep) fr.fetch(m.g, T0, sweep_ptrs(m, A, n_pred));
-        double red[NRED] = {0, 0, 0, 0, 0, 0};
-        if (needs_fold(ph_in)) fold_member(part_in + (size_t)m.seg_base * NRED, m.nseg, red, sRed);
-        if (threadIdx.x == 0) {
-            BState st = st_in[s];
-            if (st.phase != BP_DONE) {
-                b_transition(st, red, P);
-                st.launches += 1;
-                if (st.phase == BP_DONE && seg0 == 0) {
-                    BFinal f;
-                    f.rrt = st.rrt;
-                    f.bbt = st.bbt;
-                    f.rr_rec = st.c.rr;
-                    f.bb_rec = st.c.bb;
-                    f.est_error = st.c.est_error;
-                    f.t_done = wall_clock64();
-                    f.done = 1;
-                    f.status = st.c.status;
-                    f.iterations = st.c.status == ST_MAXIT ? P.max_it + 1 : st.c.n;
-                    f.legs = st.legs;
-                    f.launches = st.launches;
-                    f.pad = 0;
-                    fin[s] = f;
-                }
-            }
-            sst = st;
-            if (seg0 == 0) st_out[s] = st;
-        }
-        __syncthreads();
+        member_step(m, s, seg0, ph_in, P, st_in, st_out, part_in, fin, sst, sRed);
         const int ph = sst.phase;
         if (ph == BP_SWEEP) {
             const int n = sst.c.n;
@@ -621,6 +635,107 @@ __global__ __launch_bounds__(NT) void k_batch_step(
         __syncthreads();
         it += run;
     }
+    }
+}
+
+// Polynomial-preconditioned FP32 legs of one member over tiles [T0, T1) of
+// its poly tiling (partials per work item of K tiles).
+template <int KP, int MODE>
+__device__ __forceinline__ void poly_run(const BMember& m, const BArrays& A, const BParams& P,
+                                         const BState& st, int T0, int T1, Exchange& X, int& xbuf,
+                                         double* sRed, double* __restrict__ part_out) {
+    PolyPtrs q;
+    q.E = A.E32 + m.off;
+    q.N = A.N32 + m.off;
+    q.rx = A.rx32 + m.offx;
+    q.ry = A.ry32 + m.offy;
+    q.act = A.act + m.off;
+    q.x = A.e32 + m.off;
+    q.r64 = A.r64 + m.off;
+    q.rscale = st.scale;
+    const int n = MODE == PM_SETUP ? 0 : st.c.n;
+    poly_buffers<MODE>(q, n, A.u0 + m.off, A.u1 + m.off, A.p0 + m.off, A.p1 + m.off,
+                       A.r0 + m.off, A.r1 + m.off);
+    const float alpha = (float)st.c.alpha, beta = (float)st.c.beta;
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
+    for (int t = T0; t < T1; ++t) {
+        poly_tile<KP, MODE>(m.g, t, q, alpha, beta, P.ch, X, xbuf, acc);
+        if ((t + 1) % P.K == 0 || t + 1 == T1) {    // end of a work item
+            block_reduce<NRED>(acc, 1u << 2, sRed);
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int k = 0; k < NRED; ++k)
+                    part_out[(size_t)(m.seg_base + t / P.K) * NRED + k] = acc[k];
+            }
+#pragma unroll
+            for (int k = 0; k < NRED; ++k) acc[k] = 0.0;
+        }
+    }
+}
+
+// The batched step with the degree-KP polynomial preconditioner in the FP32
+// legs (the member's tiles are those of the poly tiling, (64 - 2R) x
+// (4 RPT - 2R)); work distribution and state machine as k_batch_step.
+template <int KP>
+__global__ __launch_bounds__(NT) void k_batch_poly(
+    const BMember* __restrict__ mem, const int2* __restrict__ items,
+    const int* __restrict__ nitems_p, BArrays A, BParams P, const BState* __restrict__ st_in,
+    BState* __restrict__ st_out, const double* __restrict__ part_in,
+    double* __restrict__ part_out, BFinal* __restrict__ fin, int* ctr, int par) {
+    constexpr int RPT = PolyRows<KP>::value;
+    constexpr int TXo = PolyShape<KP, RPT>::TXo, TYo = PolyShape<KP, RPT>::TYo;
+    __shared__ Exchange X;
+    __shared__ double sRed[NRED * NW];
+    __shared__ BState sst;
+    __shared__ int s_chunk;
+    const int nitems = *nitems_p;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(ctr + (par ^ 1), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int xbuf = 0;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_chunk =
+                __hip_atomic_fetch_add(ctr + par, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int chunk = s_chunk;
+        if ((long long)chunk * P.grab >= nitems) break;
+        const int i_end = min(nitems, (chunk + 1) * P.grab);
+        for (int it = chunk * P.grab; it < i_end;) {
+            const int2 item = items[it];
+            const int s = item.x, seg0 = item.y;
+            const BMember m = mem[s];
+            const int run = min(i_end - it, m.nseg - seg0);
+            const int T0 = seg0 * P.K, T1 = min(m.g.ntiles, (seg0 + run) * P.K);
+            member_step(m, s, seg0, st_in[s].phase, P, st_in, st_out, part_in, fin, sst, sRed);
+            const int ph = sst.phase;
+            if (ph == BP_SWEEP) {
+                poly_run<KP, PM_SWEEP>(m, A, P, sst, T0, T1, X, xbuf, sRed, part_out);
+            } else if (ph == BP_SETUP) {
+                poly_run<KP, PM_SETUP>(m, A, P, sst, T0, T1, X, xbuf, sRed, part_out);
+            } else if (ph == BP_REFRESH) {
+                poly_run<KP, PM_REFRESH>(m, A, P, sst, T0, T1, X, xbuf, sRed, part_out);
+            } else if (ph != BP_DONE) {
+                for (int j = 0; j < run; ++j) {
+                    const int seg = seg0 + j;
+                    const int tb = seg * P.K, te = min(m.g.ntiles, tb + P.K);
+                    if (ph == BP_CORR) {
+                        work_corr<TXo, TYo>(m, A, tb, te, 1.0 / sst.scale);
+                    } else {
+                        double acc[NRED] = {0, 0, 0, 0, 0, 0};
+                        work_resid<TXo, TYo>(m, A, tb, te, acc);
+                        block_reduce<NRED>(acc, 1u << 2, sRed);
+                        if (threadIdx.x == 0) {
+#pragma unroll
+                            for (int k = 0; k < NRED; ++k)
+                                part_out[(size_t)(m.seg_base + seg) * NRED + k] = acc[k];
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            __syncthreads();
+            it += run;
+        }
     }
 }
 
@@ -671,6 +786,21 @@ const void* kernel_for(int shape) {
     }
 }
 
+// Polynomial degrees of the batched path and their kernels / own tiles.
+constexpr int kPolyMin = 2, kPolyMax = 4;
+const void* poly_kernel_for(int k) {
+    switch (k) {
+        case 2: return reinterpret_cast<const void*>(&k_batch_poly<2>);
+        case 3: return reinterpret_cast<const void*>(&k_batch_poly<3>);
+        default: return reinterpret_cast<const void*>(&k_batch_poly<4>);
+    }
+}
+void poly_tile_dims(int k, int* bx, int* by) {
+    const int R = k + 2, rpt = PolyRowsHost(k);
+    *bx = 64 - 2 * R;
+    *by = 4 * rpt - 2 * R;
+}
+
 }  // namespace
 
 struct psi_batch {
@@ -691,7 +821,11 @@ struct psi_batch {
     double *E64 = nullptr, *N64 = nullptr, *rx64 = nullptr, *ry64 = nullptr, *r64 = nullptr;
     float *E32 = nullptr, *N32 = nullptr, *rx32 = nullptr, *ry32 = nullptr, *e32 = nullptr;
     float *u[2] = {nullptr, nullptr}, *p[2] = {nullptr, nullptr};
+    float* rb[2] = {nullptr, nullptr};   // residuals of the polynomial path
     uint8_t* act = nullptr;
+    bool metrics_positive = true;        // the polynomial path's precondition
+    int poly_k = 0;                      // > 0: descriptors laid out for k_batch_poly<poly_k>
+    int occ_poly[kPolyMax + 1] = {0};
     int64_t active_points = 0;
     // schedule
     psi_batch_tune tune{};
@@ -883,10 +1017,8 @@ int valid_tune(const psi_batch* b, const psi_batch_tune* t) {
 }
 
 // Descriptors, partial-record layout and the full item list for a schedule.
-int apply_tune(psi_batch* b, const psi_batch_tune* t) {
-    const int s = valid_tune(b, t);
-    if (s < 0) return bfail(b, PSI_BAD_ARG, "invalid batch schedule%s");
-    const int bx = kShapes[s].bx, by = kShapes[s].by, K = t->tiles_per_item;
+// Descriptors and partial-record layout for tiles bx x by, K tiles per item.
+int apply_layout(psi_batch* b, int bx, int by, int K) {
     b->h_mem.resize(b->count);
     int seg_base = 0;
     size_t nitems = 0;
@@ -912,11 +1044,38 @@ int apply_tune(psi_batch* b, const psi_batch_tune* t) {
     if (nitems > b->items_cap || (size_t)seg_base * NRED > b->part_cap)
         return bfail(b, PSI_BAD_ARG, "schedule exceeds the preallocated work list%s");
     b->total_seg = seg_base;
+    return PSI_OK;
+}
+
+int grid_of(const psi_batch* b, int groups_per_cu) {
+    int G = b->ncu * groups_per_cu;
+    if (G >= 8) G &= ~7;
+    return std::max(G, 1);
+}
+
+// Installs a Jacobi-path schedule.
+int apply_tune(psi_batch* b, const psi_batch_tune* t) {
+    const int s = valid_tune(b, t);
+    if (s < 0) return bfail(b, PSI_BAD_ARG, "invalid batch schedule%s");
+    const int rc = apply_layout(b, kShapes[s].bx, kShapes[s].by, t->tiles_per_item);
+    if (rc) return rc;
     b->tune = *t;
     b->shape = s;
-    int G = b->ncu * t->groups_per_cu;
-    if (G >= 8) G &= ~7;
-    b->G = std::max(G, 1);
+    b->poly_k = 0;
+    b->G = grid_of(b, t->groups_per_cu);
+    return PSI_OK;
+}
+
+// Lays the descriptors out for the polynomial kernel of degree k, keeping
+// the schedule's tiles per item and grab size (occupancy capped by the
+// kernel's).
+int apply_poly(psi_batch* b, int k) {
+    int bx, by;
+    poly_tile_dims(k, &bx, &by);
+    const int rc = apply_layout(b, bx, by, b->tune.tiles_per_item);
+    if (rc) return rc;
+    b->poly_k = k;
+    b->G = grid_of(b, std::max(1, std::min(b->tune.groups_per_cu, b->occ_poly[k])));
     return PSI_OK;
 }
 
@@ -969,6 +1128,8 @@ BArrays arrays_of(const psi_batch* b) {
     A.u1 = b->u[1];
     A.p0 = b->p[0];
     A.p1 = b->p[1];
+    A.r0 = b->rb[0];
+    A.r1 = b->rb[1];
     return A;
 }
 
@@ -986,7 +1147,8 @@ int launch_step(psi_batch* b, int par, const BArrays& A, const BParams& P) {
     void* args[] = {(void*)&mem, (void*)&items, (void*)&nitems, (void*)&A,   (void*)&P,
                     (void*)&si,  (void*)&so,    (void*)&pi,     (void*)&po,  (void*)&fin,
                     (void*)&ctr, (void*)&par};
-    BHIP(b, hipLaunchKernel(kernel_for(b->shape), dim3(b->G), dim3(NT), args, 0, b->stream));
+    const void* fn = b->poly_k ? poly_kernel_for(b->poly_k) : kernel_for(b->shape);
+    BHIP(b, hipLaunchKernel(fn, dim3(b->G), dim3(NT), args, 0, b->stream));
     return PSI_OK;
 }
 
@@ -1075,6 +1237,13 @@ int psi_batch_create(psi_batch** out, int32_t count, const psi_batch_member* mem
             offy += h.ny + 4;
             // most work items any schedule can make: the smallest tile, 1 per item
             max_tiles += (size_t)((h.nx + 31) / 32) * ((h.ny + 7) / 8);
+            // the polynomial path needs positive metrics (diagonal dominance)
+            const psi_grid& gr = members[k].grid;
+            for (int q = 1; q < h.nx + 3; ++q)
+                if (!(gr.dxt[q] > 0 && gr.dxu[q] > 0)) b->metrics_positive = false;
+            for (int q = 1; q < h.ny + 3; ++q)
+                if (!(gr.dyt[q] > 0 && gr.dyu[q] > 0 && gr.cost[q] > 0 && gr.cosu[q] > 0))
+                    b->metrics_positive = false;
         }
         b->total = off;
         b->totx = offx;
@@ -1088,6 +1257,7 @@ int psi_batch_create(psi_batch** out, int32_t count, const psi_batch_member* mem
         for (int q = 0; q < 2; ++q) {
             BTRY(balloc(b, &b->u[q], b->total));
             BTRY(balloc(b, &b->p[q], b->total));
+            BTRY(balloc(b, &b->rb[q], b->total));
         }
         BTRY(balloc(b, &b->act, b->total));
         BTRY(balloc(b, &b->rx64, (size_t)offx));
@@ -1134,6 +1304,12 @@ int psi_batch_create(psi_batch** out, int32_t count, const psi_batch_member* mem
         for (int s = 0; s < kNumShapes; ++s)
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occ[s], kernel_for(s), NT, 0) !=
                 hipSuccess) {
+                rc = bfail(b, PSI_HIP_ERROR, "occupancy query failed%s");
+                goto failed;
+            }
+        for (int k = kPolyMin; k <= kPolyMax; ++k)
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->occ_poly[k], poly_kernel_for(k),
+                                                             NT, 0) != hipSuccess) {
                 rc = bfail(b, PSI_HIP_ERROR, "occupancy query failed%s");
                 goto failed;
             }
@@ -1294,19 +1470,33 @@ int psi_batch_congrad(psi_batch* b, const psi_opts* o, const double* const* forc
                       double* const* sol, psi_info* infos, double* latency_ms,
                       psi_batch_stats* stats) {
     if (!b || !o || !forc || !sol) return bfail(b, PSI_BAD_ARG, "null argument%s");
-    if (o->mode != PSI_MODE_MIXED || o->criterion != PSI_CRIT_RELRES)
+    if ((o->mode != PSI_MODE_MIXED && o->mode != PSI_MODE_POLY) ||
+        o->criterion != PSI_CRIT_RELRES)
         return bfail(b, PSI_BAD_ARG,
-                     "batched solves run the mixed-precision path with the relres criterion%s");
+                     "batched solves run the mixed or poly path with the relres criterion%s");
     if (o->max_iterations < 1) return bfail(b, PSI_BAD_ARG, "max_iterations < 1%s");
+    int poly = 0;
+    double lower = 0.0;
+    if (o->mode == PSI_MODE_POLY) {
+        poly = o->poly_degree > 0 ? o->poly_degree : 4;
+        lower = o->poly_lower > 0 ? o->poly_lower : 0.005;
+        if (poly < kPolyMin || poly > kPolyMax || !(lower < 1.0))
+            return bfail(b, PSI_BAD_ARG, "batched poly path: degree 2..4, poly_lower in (0,1)%s");
+        if (!b->metrics_positive)
+            return bfail(b, PSI_BAD_ARG, "poly mode needs positive grid metrics%s");
+    }
     for (int k = 0; k < b->count; ++k)
         if (!forc[k] || !sol[k]) return bfail(b, PSI_BAD_ARG, "null member field%s");
     if (b->own_stream) {
         BHIP(b, hipEventRecord(b->ev_in, nullptr));
         BHIP(b, hipStreamWaitEvent(b->stream, b->ev_in, 0));
     }
-    int rc = upload_schedule(b, forc, sol);
+    int rc = poly ? apply_poly(b, poly) : apply_tune(b, &b->tune);
     if (rc) return rc;
-    const BParams P = params_of(b, o);
+    rc = upload_schedule(b, forc, sol);
+    if (rc) return rc;
+    BParams P = params_of(b, o);
+    if (poly) P.ch = make_cheb(poly, lower);
     const BArrays A = arrays_of(b);
     const int KL = b->tune.graph_launches;
     // capture KL launches (parities alternate; KL is even, so every replay

@@ -736,6 +736,16 @@ int solve_poly(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, p
     if (k > PMAXK) return fail(c, PSI_BAD_ARG, "poly_degree must be 1..6%s");
     const double lower = o->poly_lower > 0 ? o->poly_lower : 0.005;
     if (!(lower < 1.0)) return fail(c, PSI_BAD_ARG, "poly_lower must be in (0, 1)%s");
+    // The polynomial is built for a spectrum of Z A in [0, 2], which holds when
+    // every off-diagonal coefficient is >= 0 (weak diagonal dominance): hur,
+    // hvr >= 0 and positive metrics.  Refuse grids whose metrics are not.
+    const int nxh = c->g.nx + 4, nyh = c->g.ny + 4;
+    for (int q = 1; q < nxh - 1; ++q)
+        if (!(c->h_dxu[q] > 0 && c->h_dxt[q] > 0))
+            return fail(c, PSI_BAD_ARG, "poly mode needs positive dxt/dxu%s");
+    for (int q = 1; q < nyh - 1; ++q)
+        if (!(c->h_dyu[q] > 0 && c->h_dyt[q] > 0 && c->h_cosu[q] > 0 && c->h_cost[q] > 0))
+            return fail(c, PSI_BAD_ARG, "poly mode needs positive dyt/dyu/cost/cosu%s");
     CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]},
                     poly_grid(c, k)};
     if (P.G <= 0) return fail(c, PSI_HIP_ERROR, "occupancy query failed%s");

@@ -47,6 +47,7 @@ template <int K>
 struct PolyRows {
     static constexpr int value = K <= 2 ? 6 : (K <= 4 ? 7 : 8);
 };
+inline int PolyRowsHost(int k) { return k <= 2 ? 6 : (k <= 4 ? 7 : 8); }
 
 // Neighbouring columns of the frame are neighbouring lanes.  Lanes 0 and 63
 // (frame edges, never inside a ring that is computed) get their own value.
@@ -102,13 +103,166 @@ __device__ __forceinline__ void exchange(Exchange& X, int& buf, const float (&a)
     buf ^= 1;
 }
 
-// One launch of the polynomial-preconditioned FP32 PCG on tiles of gp (the
-// (64 - 2R) x (4 RPT - 2R) tiling).  MODE:
-//  PM_SETUP   r = r64 * rscale; p = 0; e = 0; z = M r (buffers 0); partials for
+// Vectors of one polynomial-preconditioned launch over one grid.
+struct PolyPtrs {
+    const float *E, *N, *rx, *ry;
+    const uint8_t* act;
+    float* x;
+    const float *z_in, *p_in, *r_in;
+    float *z_out, *p_out, *r_out;
+    const double* r64;
+    double rscale;
+};
+
+// after n iterations z_n, r_n and p_{n-1} live in buffer n & 1; a sweep
+// writes buffer (n+1) & 1, setup (n = 0) and refresh write buffer n & 1
+template <int MODE>
+__device__ __forceinline__ void poly_buffers(PolyPtrs& q, int n, float* zb0, float* zb1,
+                                             float* pb0, float* pb1, float* rb0, float* rb1) {
+    const int odd = n & 1;
+    q.z_in = odd ? zb1 : zb0;
+    q.p_in = odd ? pb1 : pb0;
+    q.r_in = odd ? rb1 : rb0;
+    q.z_out = (MODE == PM_SWEEP) ? (odd ? zb0 : zb1) : (odd ? zb1 : zb0);
+    q.p_out = (MODE == PM_SWEEP) ? (odd ? pb0 : pb1) : (odd ? pb1 : pb0);
+    q.r_out = (MODE == PM_SWEEP) ? (odd ? rb0 : rb1) : (odd ? rb1 : rb0);
+}
+
+// One tile of g (tiled (64 - 2R) x (4 RPT - 2R)).  MODE:
+//  PM_SETUP   r = r64 * rscale; p = 0; x = 0; z = M r; partials for
 //             cg_from_setup (slot 2 = max|z|, slot 4 unused: bb is fixed).
-//  PM_SWEEP   one fused iteration; the state advances as in k_cg_sweep.
-//  PM_REFRESH r = r64 * rscale, z = M r next to the kept p (buffers n & 1);
-//             partials for cg_from_refresh; state handling as k_cg_refresh.
+//  PM_SWEEP   one fused iteration with alpha, beta.
+//  PM_REFRESH r = r64 * rscale, z = M r next to the kept p; partials for
+//             cg_from_refresh.
+template <int K, int MODE>
+__device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtrs& q, float alpha,
+                                          float beta, const ChebCoef& ch, Exchange& X, int& xbuf,
+                                          double (&acc)[NRED]) {
+    constexpr int RPT = PolyRows<K>::value;
+    using PS = PolyShape<K, RPT>;
+    constexpr int R = PS::R, FW = PS::FW, FH = PS::FH, TXo = PS::TXo, TYo = PS::TYo;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c0 = (tile % g.tiles_x) * TXo, r0 = (tile / g.tiles_x) * TYo;
+    // frame row 0 / column 0 sit R - 2 rows / columns before the tile's
+    // first own point (array row / column 2 + r0 / c0)
+    const int fr0 = r0 + 2 - R, fc0 = c0 + 2 - R;
+    const int col = map_col(fc0 + lane, g);
+    const bool col_in = col >= 0 && col < g.nx + 4;
+    const float rxv = col_in ? q.rx[col] : 0.f;
+    // ---- load the band: vectors, coefficients, own x -----------------------
+    float P[RPT], Rr[RPT], cS[RPT], cW[RPT], cE[RPT], cN[RPT], zi[RPT], xo[RPT];
+    unsigned amask = 0u;
+    float nprev;
+    {
+        const int r = fr0 + w * RPT - 1;
+        nprev = (col_in && r >= 0 && r < g.ny + 4) ? q.N[(size_t)r * g.pitch + col] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int gg = w * RPT + i, r = fr0 + gg;
+        float pv = 0.f, rv = 0.f, e = 0.f, nn = 0.f, ryv = 0.f, xv = 0.f;
+        uint8_t a = 0;
+        if (col_in && r >= 0 && r < g.ny + 4) {
+            const size_t o = (size_t)r * g.pitch + col;
+            if (MODE == PM_SWEEP) {
+                pv = q.z_in[o] + beta * q.p_in[o];
+                rv = q.r_in[o];
+            } else {
+                if (MODE == PM_REFRESH) pv = q.p_in[o];
+                rv = (float)(q.r64[o] * q.rscale);
+            }
+            e = q.E[o];
+            nn = q.N[o];
+            a = q.act[o];
+            ryv = q.ry[r];
+            if (MODE == PM_SWEEP && gg >= R && gg < FH - R && lane >= R && lane < FW - R)
+                xv = q.x[o];
+        }
+        P[i] = pv;
+        Rr[i] = rv;
+        cE[i] = e;
+        cN[i] = nn;
+        cS[i] = nprev * ryv;
+        nprev = nn;
+        xo[i] = xv;
+        amask |= (a ? 1u : 0u) << i;
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        cW[i] = from_left(cE[i]) * rxv;
+        const float cc = (((-cE[i]) - cW[i]) - cN[i]) - cS[i];
+        zi[i] = ((amask >> i) & 1u) ? 1.f / cc : 0.f;
+    }
+    float up[2], dn[2], tmp[RPT];
+    if (MODE == PM_SWEEP) {
+        // r_{k+1} = r_k - alpha A p_k (valid on radius K+1)
+        exchange<RPT, 1>(X, xbuf, P, P, up, dn);
+        apply_strip<RPT>(P, up[0], dn[0], cS, cW, cE, cN, tmp);
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+            if ((amask >> i) & 1u) Rr[i] = Rr[i] - alpha * tmp[i];
+    }
+    // ---- z = q(ZA) Z r: Chebyshev iteration, valid on radius 1 ------------
+    float D[RPT], S[RPT], Y[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const float v = zi[i] * Rr[i];
+        S[i] = v;
+        D[i] = v * ch.inv_theta;
+        Y[i] = D[i];
+    }
+#pragma unroll
+    for (int j = 1; j <= K; ++j) {
+        exchange<RPT, 1>(X, xbuf, D, D, up, dn);
+        apply_strip<RPT>(D, up[0], dn[0], cS, cW, cE, cN, tmp);
+        const float c1 = ch.c1[j - 1], c2 = ch.c2[j - 1];
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            S[i] = S[i] - zi[i] * tmp[i];
+            D[i] = c1 * D[i] + c2 * S[i];
+            Y[i] = Y[i] + D[i];
+        }
+    }
+    // ---- own points: w = A z, s = A p, outputs and partials ---------------
+    float Wv[RPT], Sv[RPT];
+    exchange<RPT, 2>(X, xbuf, Y, P, up, dn);
+    apply_strip<RPT>(Y, up[0], dn[0], cS, cW, cE, cN, Wv);
+    apply_strip<RPT>(P, up[1], dn[1], cS, cW, cE, cN, Sv);
+    const int c = fc0 + lane;
+    const bool own_col = lane >= R && lane < FW - R && c <= g.nx + 1;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int gg = w * RPT + i, r = fr0 + gg;
+        if (own_col && gg >= R && gg < FH - R && r <= g.ny + 1) {
+            const size_t o = (size_t)r * g.pitch + c;
+            const float zn = Y[i], pn = P[i], rn = Rr[i];
+            if (MODE == PM_SWEEP) {
+                q.x[o] = xo[i] + alpha * pn;
+                q.p_out[o] = pn;
+            }
+            if (MODE == PM_SETUP) {
+                q.x[o] = 0.f;
+                q.p_out[o] = 0.f;
+            }
+            q.z_out[o] = zn;
+            q.r_out[o] = rn;
+            const double zd = (double)zn, pd = (double)pn, rd = (double)rn;
+            const double wd = (double)Wv[i], sd = (double)Sv[i];
+            if ((amask >> i) & 1u) {
+                acc[0] += rd * zd;
+                acc[3] += rd * rd;
+            }
+            acc[1] += zd * wd;
+            acc[2] = fmax(acc[2], MODE == PM_SETUP ? fabs(zd) : fabs(pd));
+            acc[4] += zd * sd + pd * wd;
+            acc[5] += pd * sd;
+        }
+    }
+}
+
+// One launch of the polynomial-preconditioned FP32 PCG over the tiles of gp
+// (the poly tiling of the grid); MODE as in poly_tile.  The sweep advances
+// the state as k_cg_sweep does; the refresh handles it as k_cg_refresh.
 template <int K, int MODE>
 __global__ __launch_bounds__(NT) void k_poly_pcg(
     Geom gp, ChebCoef ch, const float* __restrict__ E, const float* __restrict__ N,
@@ -118,13 +272,9 @@ __global__ __launch_bounds__(NT) void k_poly_pcg(
     const double* __restrict__ true_res, double rr_thresh, int new_max_it,
     const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
     const double* __restrict__ part_in, double* __restrict__ part_out) {
-    constexpr int RPT = PolyRows<K>::value;
-    using PS = PolyShape<K, RPT>;
-    constexpr int R = PS::R, FW = PS::FW, FH = PS::FH, TXo = PS::TXo, TYo = PS::TYo;
     __shared__ Exchange X;
     __shared__ double sRed[NRED * NW];
     __shared__ CgState sst;
-    const Geom& g = gp;
     float alpha = 0.f, beta = 0.f;
     int n = 0;
     if (MODE == PM_SWEEP) {
@@ -147,137 +297,22 @@ __global__ __launch_bounds__(NT) void k_poly_pcg(
         if (finished) return;
         n = st_in->n;
     }
-    // buffers: after n iterations z_n, r_n and p_{n-1} live in buffer n & 1
-    const int odd = n & 1;
-    const float* __restrict__ z_in = odd ? zb1 : zb0;
-    const float* __restrict__ p_in = odd ? pb1 : pb0;
-    const float* __restrict__ r_in = odd ? rb1 : rb0;
-    float* __restrict__ z_out = (MODE == PM_SWEEP) ? (odd ? zb0 : zb1) : (odd ? zb1 : zb0);
-    float* __restrict__ p_out = (MODE == PM_SWEEP) ? (odd ? pb0 : pb1) : (odd ? pb1 : pb0);
-    float* __restrict__ r_out = (MODE == PM_SWEEP) ? (odd ? rb0 : rb1) : (odd ? rb1 : rb0);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    PolyPtrs q;
+    q.E = E;
+    q.N = N;
+    q.rx = rx;
+    q.ry = ry;
+    q.act = act;
+    q.x = x;
+    q.r64 = r64;
+    q.rscale = rscale;
+    poly_buffers<MODE>(q, n, zb0, zb1, pb0, pb1, rb0, rb1);
     double acc[NRED] = {0, 0, 0, 0, 0, 0};
     int xbuf = 0;
     int t0, ts, te;
-    tile_schedule(g, t0, ts, te);
-    for (int tile = t0; tile < te; tile += ts) {
-        const int c0 = (tile % g.tiles_x) * TXo, r0 = (tile / g.tiles_x) * TYo;
-        // frame row 0 / column 0 sit R - 2 rows / columns before the tile's
-        // first own point (array row / column 2 + r0 / c0)
-        const int fr0 = r0 + 2 - R, fc0 = c0 + 2 - R;
-        const int col = map_col(fc0 + lane, g);
-        const bool col_in = col >= 0 && col < g.nx + 4;
-        const float rxv = col_in ? rx[col] : 0.f;
-        // ---- load the band: vectors, coefficients, own x -------------------
-        float P[RPT], Rr[RPT], cS[RPT], cW[RPT], cE[RPT], cN[RPT], zi[RPT], xo[RPT];
-        unsigned amask = 0u;
-        float nprev;
-        {
-            const int r = fr0 + w * RPT - 1;
-            nprev = (col_in && r >= 0 && r < g.ny + 4) ? N[(size_t)r * g.pitch + col] : 0.f;
-        }
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int gg = w * RPT + i, r = fr0 + gg;
-            float pv = 0.f, rv = 0.f, e = 0.f, nn = 0.f, ryv = 0.f, xv = 0.f;
-            uint8_t a = 0;
-            if (col_in && r >= 0 && r < g.ny + 4) {
-                const size_t o = (size_t)r * g.pitch + col;
-                if (MODE == PM_SWEEP) {
-                    pv = z_in[o] + beta * p_in[o];
-                    rv = r_in[o];
-                } else {
-                    if (MODE == PM_REFRESH) pv = p_in[o];
-                    rv = (float)(r64[o] * rscale);
-                }
-                e = E[o];
-                nn = N[o];
-                a = act[o];
-                ryv = ry[r];
-                if (MODE == PM_SWEEP && gg >= R && gg < FH - R && lane >= R && lane < FW - R)
-                    xv = x[o];
-            }
-            P[i] = pv;
-            Rr[i] = rv;
-            cE[i] = e;
-            cN[i] = nn;
-            cS[i] = nprev * ryv;
-            nprev = nn;
-            xo[i] = xv;
-            amask |= (a ? 1u : 0u) << i;
-        }
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            cW[i] = from_left(cE[i]) * rxv;
-            const float cc = (((-cE[i]) - cW[i]) - cN[i]) - cS[i];
-            zi[i] = ((amask >> i) & 1u) ? 1.f / cc : 0.f;
-        }
-        float up[2], dn[2], tmp[RPT];
-        if (MODE == PM_SWEEP) {
-            // r_{k+1} = r_k - alpha A p_k (valid on radius K+1)
-            exchange<RPT, 1>(X, xbuf, P, P, up, dn);
-            apply_strip<RPT>(P, up[0], dn[0], cS, cW, cE, cN, tmp);
-#pragma unroll
-            for (int i = 0; i < RPT; ++i)
-                if ((amask >> i) & 1u) Rr[i] = Rr[i] - alpha * tmp[i];
-        }
-        // ---- z = q(ZA) Z r: Chebyshev iteration, valid on radius 1 --------
-        float D[RPT], S[RPT], Y[RPT];
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const float v = zi[i] * Rr[i];
-            S[i] = v;
-            D[i] = v * ch.inv_theta;
-            Y[i] = D[i];
-        }
-#pragma unroll
-        for (int j = 1; j <= K; ++j) {
-            exchange<RPT, 1>(X, xbuf, D, D, up, dn);
-            apply_strip<RPT>(D, up[0], dn[0], cS, cW, cE, cN, tmp);
-            const float c1 = ch.c1[j - 1], c2 = ch.c2[j - 1];
-#pragma unroll
-            for (int i = 0; i < RPT; ++i) {
-                S[i] = S[i] - zi[i] * tmp[i];
-                D[i] = c1 * D[i] + c2 * S[i];
-                Y[i] = Y[i] + D[i];
-            }
-        }
-        // ---- own points: w = A z, s = A p, outputs and partials -----------
-        float Wv[RPT], Sv[RPT];
-        exchange<RPT, 2>(X, xbuf, Y, P, up, dn);
-        apply_strip<RPT>(Y, up[0], dn[0], cS, cW, cE, cN, Wv);
-        apply_strip<RPT>(P, up[1], dn[1], cS, cW, cE, cN, Sv);
-        const int c = fc0 + lane;
-        const bool own_col = lane >= R && lane < FW - R && c <= g.nx + 1;
-#pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int gg = w * RPT + i, r = fr0 + gg;
-            if (own_col && gg >= R && gg < FH - R && r <= g.ny + 1) {
-                const size_t o = (size_t)r * g.pitch + c;
-                const float zn = Y[i], pn = P[i], rn = Rr[i];
-                if (MODE == PM_SWEEP) {
-                    x[o] = xo[i] + alpha * pn;
-                    p_out[o] = pn;
-                }
-                if (MODE == PM_SETUP) {
-                    x[o] = 0.f;
-                    p_out[o] = 0.f;
-                }
-                z_out[o] = zn;
-                r_out[o] = rn;
-                const double zd = (double)zn, pd = (double)pn, rd = (double)rn;
-                const double wd = (double)Wv[i], sd = (double)Sv[i];
-                if ((amask >> i) & 1u) {
-                    acc[0] += rd * zd;
-                    acc[3] += rd * rd;
-                }
-                acc[1] += zd * wd;
-                acc[2] = fmax(acc[2], MODE == PM_SETUP ? fabs(zd) : fabs(pd));
-                acc[4] += zd * sd + pd * wd;
-                acc[5] += pd * sd;
-            }
-        }
-    }
+    tile_schedule(gp, t0, ts, te);
+    for (int tile = t0; tile < te; tile += ts)
+        poly_tile<K, MODE>(gp, tile, q, alpha, beta, ch, X, xbuf, acc);
     block_reduce<NRED>(acc, 1u << 2, sRed);
     write_partials<NRED>(acc, part_out);
 }
@@ -285,7 +320,7 @@ __global__ __launch_bounds__(NT) void k_poly_pcg(
 // Tiling of the polynomial kernels of degree K over the grid of g.
 inline Geom poly_geom(const Geom& g, int K) {
     Geom gp = g;
-    const int R = K + 2, rpt = K <= 2 ? 6 : (K <= 4 ? 7 : 8);
+    const int R = K + 2, rpt = PolyRowsHost(K);
     const int txo = 64 - 2 * R, tyo = 4 * rpt - 2 * R;
     gp.tiles_x = (g.nx + txo - 1) / txo;
     gp.tiles_y = (g.ny + tyo - 1) / tyo;

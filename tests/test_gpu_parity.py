@@ -168,6 +168,20 @@ def test_polynomial_preconditioner(name):
         assert its * (k + 1) <= 1.5 * its_m
 
 
+def test_polynomial_preconditioner_refuses_invalid_metrics():
+    """A grid reaching past the poles (cos(lat) < 0) breaks the diagonal
+    dominance the Chebyshev interval [0, 2] relies on: the poly mode refuses
+    it instead of diverging."""
+    b = basins.spherical_basin(seed=1, n=64, n_islands=0, dlon=3.5)
+    assert (b.cosu[2:-2] < 0).any()
+    s = StreamfunctionSolver.from_basin(b, device=DEV)
+    with pytest.raises(PsiError) as e:
+        s.congrad_streamfunction(T(b.forc), s.zeros(), mode="poly", criterion="relres",
+                                 max_iterations=100)
+    assert e.value.code == 5
+    s.close()
+
+
 def test_forced_exits():
     b, d = fixtures.load("islands_64")
     s = solver_for(b, d)
