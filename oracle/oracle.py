@@ -62,6 +62,11 @@ def lib():
         L.orc_trace_island.restype = I
         L.orc_dense_solve.argtypes = [I, _dp, _dp]
         L.orc_dense_solve.restype = I
+        L.orc_congrad_omp.argtypes = [I, I, I, _dp, _dp, _dp, _dp, D, I, C.POINTER(I),
+                                      C.POINTER(I), C.POINTER(D)]
+        L.orc_congrad_omp.restype = I
+        L.orc_threads.argtypes = []
+        L.orc_threads.restype = I
         _lib = L
     return _lib
 
@@ -149,6 +154,21 @@ def congrad(nx, ny, cyclic_x, cf, Z, forc, sol, eps=1e-12, max_iter=1000, trace_
                            tr.ctypes.data if trace_cap else None, int(trace_cap))
     return CGResult(s, it.value, bool(cv.value), est.value, st,
                     tr[:min(trace_cap, it.value)] if trace_cap else None)
+
+
+def congrad_threads(nx, ny, cyclic_x, cf, Z, forc, sol, eps=1e-12, max_iter=1000):
+    """The same algorithm with the grid loops on OpenMP threads (timing only;
+    its reductions group sums differently from the serial oracle)."""
+    s = _c(sol).copy()
+    it, cv, est = C.c_int(), C.c_int(), C.c_double()
+    st = lib().orc_congrad_omp(nx, ny, int(cyclic_x), _c(cf), _c(Z), _c(forc), s, float(eps),
+                               int(max_iter), C.byref(it), C.byref(cv), C.byref(est))
+    return CGResult(s, it.value, bool(cv.value), est.value, st)
+
+
+def threads():
+    """OpenMP threads the threaded variant uses."""
+    return lib().orc_threads()
 
 
 def line_integral(b, isl: Islands, k, uloc, vloc):

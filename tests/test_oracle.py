@@ -237,3 +237,21 @@ def test_dense_solve_matches_lapack():
         np.testing.assert_allclose(x, np.linalg.solve(A, rhs), rtol=1e-12)
     _, info = O.dense_solve(np.array([[1.0, 2.0], [2.0, 4.0]]), np.array([1.0, 1.0]))
     assert info == 2
+
+
+@pytest.mark.parametrize("name", ["c0_beta_plane_128", "spherical_96", "cyclic_40x32"])
+def test_threaded_oracle_matches_serial(name):
+    """The OpenMP variant (the benchmark's all-cores CPU baseline) runs the
+    same algorithm: the same iterations up to reduction-order rounding and
+    the reference's solution to 1e-9."""
+    b, d = fixtures.load(name)
+    isl = O.Islands(int(d["nisle"]), d["offsets"], d["bi"], d["bj"], d["di"], d["dj"], None)
+    cf = O.make_coeff(b)
+    Z = O.make_inv(b.nx, b.ny, cf, isl)
+    r = O.congrad_threads(b.nx, b.ny, b.cyclic_x, cf, Z, b.forc, d["sol0"], float(d["eps"]),
+                          int(d["max_iter"]))
+    assert r.converged == bool(d["converged"])
+    assert abs(r.iterations - int(d["iterations"])) <= 2 + 0.01 * int(d["iterations"])
+    err = np.abs(r.sol - d["sol"]).max() / np.abs(d["sol"]).max()
+    assert err <= 1e-9
+    assert O.threads() >= 1
