@@ -61,7 +61,10 @@ def parse():
                          "kernel of known traffic that calibrates HBM byte counters")
     ap.add_argument("--relres", type=float, default=1e-10)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--cpu-iters", type=int, default=100,
+                    help="CPU baseline: iterations per oracle run (repeated until the budget)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: seconds of serial oracle work (half that threaded)")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--poly-degree", type=int, default=0, help="0: library default")
     ap.add_argument("--poly-lower", type=float, default=0.0)
@@ -187,7 +190,7 @@ def main():
         if ls is not None:
             timed = choices[args.warmup:]
             ch = max(set(timed), key=timed.count)
-            results[mode].update(choice=ch if ch == "learned" else f"{ch} fallback ({fallback})",
+            results[mode].update(choice=ch if ch == "learned" else f"fallback: {fallback}",
                                  probes=probes, reprobe_every=args.reprobe_every)
             for key in ("sweep_avg_launch_us", "sweep_launches", "achieved_GBs", "frac"):
                 results[mode].pop(key)
@@ -203,8 +206,9 @@ def main():
     cpu = cpu_all = None
     if not args.skip_cpu:
         its_ref = results.get("plain", r)["iterations_mean"]
-        cpu = cpu_baseline(b, isl, args.cpu_iters, its_ref)
-        cpu_all = cpu_baseline(b, isl, 4 * args.cpu_iters, its_ref, threaded=True)
+        cpu = cpu_baseline(b, isl, args.cpu_iters, its_ref, budget_s=args.cpu_seconds)
+        cpu_all = cpu_baseline(b, isl, args.cpu_iters, its_ref, threaded=True,
+                               budget_s=args.cpu_seconds / 2)
     line = {
         "metric": "ms per 2048\u00b2 psi solve to 1e-10 rel. FP64 residual: learned vs Jacobi PCG",
         "value": round(r["ms_per_solve"], 3),
@@ -258,7 +262,7 @@ def finite(v):
     return v
 
 
-def cpu_baseline(b, isl, iters, plain_its, threaded=False):
+def cpu_baseline(b, isl, iters, plain_its, threaded=False, budget_s=10.0):
     """The CPU oracle (C restatement of congrad_streamfunction; serial, the
     reference's own configuration, or with its grid loops on all OpenMP
     threads) on a bounded sample: `iters` PCG iterations of the same solve,
@@ -267,18 +271,30 @@ def cpu_baseline(b, isl, iters, plain_its, threaded=False):
     cf = O.make_coeff(b)
     Z = O.make_inv(b.nx, b.ny, cf, O.Islands(isl.nisle, isl.offsets, isl.bi, isl.bj, isl.di,
                                              isl.dj, None))
-    run = O.congrad_threads if threaded else O.congrad
     cores = O.threads() if threaded else 1
-    t = time.perf_counter()
-    run(b.nx, b.ny, False, cf, Z, b.forc, np.zeros(b.shape), 0.0, iters)
-    dt = time.perf_counter() - t
-    per_it = dt / iters
+    n, dt = timed_oracle(b, cf, Z, iters, budget_s, threaded)
+    per_it = dt / n
     how = f"OpenMP, {cores} threads" if threaded else "1 thread"
     return {"value": round(per_it * plain_its * 1e3, 1), "unit": "ms", "cores": cores,
             "kind": "port",
-            "sample": f"{iters} PCG iterations of the {b.nx}x{b.ny} solve in {dt:.1f} s "
-                      f"({per_it * 1e3:.1f} ms/iteration, C restatement -O2, {how}), "
+            "sample": f"{n} PCG iterations of the {b.nx}x{b.ny} solve in {dt:.1f} s "
+                      f"({per_it * 1e3:.2f} ms/iteration, C restatement -O2, {how}), "
                       f"scaled to {plain_its:.0f} iterations per solve"}
+
+
+def timed_oracle(b, cf, Z, iters, budget_s, threaded):
+    """Runs the oracle's congrad from a zero first guess, `iters` iterations
+    at a time, until `budget_s` seconds have passed; returns (iterations
+    run, seconds)."""
+    from oracle import oracle as O
+    run = O.congrad_threads if threaded else O.congrad
+    n, t0 = 0, time.perf_counter()
+    while True:
+        r = run(b.nx, b.ny, b.cyclic_x, cf, Z, b.forc, np.zeros(b.shape), 0.0, iters)
+        n += r.iterations
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            return n, dt
 
 
 def ensemble_main(args):
@@ -352,8 +368,9 @@ def ensemble_main(args):
                                                      relres_target=args.relres,
                                                      max_iterations=200000))
             its_jacobi = rj.iterations
-        cpu = ensemble_cpu_baseline(bs, its_jacobi, args.cpu_iters)
-        cpu_all = ensemble_cpu_baseline(bs, its_jacobi, 4 * args.cpu_iters, threaded=True)
+        cpu = ensemble_cpu_baseline(bs, its_jacobi, args.cpu_iters, budget_s=args.cpu_seconds)
+        cpu_all = ensemble_cpu_baseline(bs, its_jacobi, args.cpu_iters, threaded=True,
+                                        budget_s=args.cpu_seconds / 2)
     solves_per_s = len(bs) * args.steps / total
     line = {
         "metric": "solves/s, 512-member batched psi ensemble (64^2-512^2) to 1e-10 rel. FP64 residual",
@@ -389,7 +406,7 @@ def ensemble_main(args):
     print(json.dumps(finite(line)), flush=True)
 
 
-def ensemble_cpu_baseline(bs, its_mean, iters, threaded=False):
+def ensemble_cpu_baseline(bs, its_mean, iters, threaded=False, budget_s=10.0):
     """The CPU oracle on a bounded sample: `iters` Jacobi-PCG iterations on
     the largest member, as seconds per point-iteration, scaled to the
     ensemble's point-iterations of the batched Jacobi path (its iteration
@@ -400,17 +417,14 @@ def ensemble_cpu_baseline(bs, its_mean, iters, threaded=False):
     isl = O.trace_islands(big)
     cf = O.make_coeff(big)
     Z = O.make_inv(big.nx, big.ny, cf, isl)
-    run = O.congrad_threads if threaded else O.congrad
     cores = O.threads() if threaded else 1
-    t = time.perf_counter()
-    run(big.nx, big.ny, big.cyclic_x, cf, Z, big.forc, np.zeros(big.shape), 0.0, iters)
-    dt = time.perf_counter() - t
-    per_pt_it = dt / iters / (big.nx * big.ny)
+    n, dt = timed_oracle(big, cf, Z, iters, budget_s, threaded)
+    per_pt_it = dt / n / (big.nx * big.ny)
     total = per_pt_it * float(sum(n * (b.nx * b.ny) for n, b in zip(its_mean, bs)))
     how = f"OpenMP, {cores} threads" if threaded else "1 thread"
     return {"value": round(len(bs) / total, 3), "unit": "solves/s", "cores": cores,
             "kind": "port",
-            "sample": f"{iters} PCG iterations of the largest member ({big.nx}x{big.ny}) in "
+            "sample": f"{n} PCG iterations of the largest member ({big.nx}x{big.ny}) in "
                       f"{dt:.2f} s ({per_pt_it * 1e9:.2f} ns per point-iteration, C "
                       f"restatement, {how}), scaled to the ensemble's Jacobi point-iterations"}
 
