@@ -43,11 +43,16 @@ struct PolyShape {
 };
 
 // Rows per thread for degree K: the own tile is (64 - 2R) x (4 RPT - 2R).
+#ifndef PSI_POLY_RPT
+#define PSI_POLY_RPT 0            // > 0: rows per thread for every degree
+#endif
 template <int K>
 struct PolyRows {
-    static constexpr int value = K <= 2 ? 6 : (K <= 4 ? 7 : 8);
+    static constexpr int value = PSI_POLY_RPT > 0 ? PSI_POLY_RPT : (K <= 2 ? 6 : (K <= 4 ? 7 : 8));
 };
-inline int PolyRowsHost(int k) { return k <= 2 ? 6 : (k <= 4 ? 7 : 8); }
+inline int PolyRowsHost(int k) {
+    return PSI_POLY_RPT > 0 ? PSI_POLY_RPT : (k <= 2 ? 6 : (k <= 4 ? 7 : 8));
+}
 
 // Neighbouring columns of the frame are neighbouring lanes.  Lanes 0 and 63
 // (frame edges, never inside a ring that is computed) get their own value.
