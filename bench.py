@@ -22,6 +22,7 @@ oracle timed on a bounded sample on this host.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -71,7 +72,10 @@ def parse():
     ap.add_argument("--inner-reduction", type=float, default=0.0,
                     help="mixed path: residual reduction per FP32 leg (0: library default)")
     ap.add_argument("--verbose", action="store_true")
-    ap.add_argument("--workload", default="solve", choices=["solve", "ensemble"],
+    ap.add_argument("--nz", type=int, default=40, help="levels of the step workload")
+    ap.add_argument("--step-mode", default="poly", choices=["poly", "mixed", "plain"],
+                    help="solve path of the step workload")
+    ap.add_argument("--workload", default="solve", choices=["solve", "ensemble", "step"],
                     help="solve: the 2048^2 two-island solve (headline); ensemble: 512 "
                          "batched basins of 64^2-512^2")
     ap.add_argument("--members", type=int, default=512)
@@ -97,6 +101,8 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         return ensemble_main(args)
+    if args.workload == "step":
+        return step_main(args)
     import torch
 
     from pyom2_amd import basins
@@ -427,6 +433,185 @@ def ensemble_cpu_baseline(bs, its_mean, iters, threaded=False, budget_s=10.0):
             "sample": f"{n} PCG iterations of the largest member ({big.nx}x{big.ny}) in "
                       f"{dt:.2f} s ({per_pt_it * 1e9:.2f} ns per point-iteration, C "
                       f"restatement, {how}), scaled to the ensemble's Jacobi point-iterations"}
+
+
+# algorithmic bytes per point and level of the 3-D kernels around the solve:
+# hydrostatic pressure (rho, maskT in; p_hydro out) + forcing (p_hydro; du, dv
+# read and written; du_mix, dv_mix, maskU, maskV), and the velocity update
+# (u, v, du, dv at tau; du, dv at taum1; du_mix, dv_mix, maskU, maskV in; u, v out)
+LEVEL_BYTES = {"forcing": 24 + 72, "velocity": 96}
+
+
+def step_main(args):
+    """The whole solve_streamfunction per step (solve_streamfct.f90:10-120):
+    hydrostatic pressure and forcing assembly over nz levels, the psi solve
+    to a 1e-10 relative FP64 residual with the warm start, the island system,
+    and the velocity update, on the 2048^2 spherical basin with two islands.
+    The tendencies drift from step to step (AR(1), rho = 0.99, outside the
+    timed calls); everything else is model state resident in HBM."""
+    import torch
+
+    from pyom2_amd import basins
+    from pyom2_amd.streamfunction import (BarotropicState, Levels, MomentumState, SolveOptions,
+                                          StreamfunctionSolver, islands_trace)
+
+    dev = torch.device("cuda:0")
+    nz = args.nz
+    t = time.time()
+    b0 = basins.spherical_basin(seed=0, n=args.n, n_islands=2, dlon=0.05 * 2048 / args.n)
+    dzt, dzw, kb = basins.column_levels(b0, nz)
+    isl = islands_trace(b0.kbot)
+    # calc_topo's masks and depths (numerics.f90:215-259) on the device
+    kb_d = torch.as_tensor(kb, device=dev)
+    lvl = torch.arange(1, nz + 1, device=dev)[:, None, None]
+    maskT = ((kb_d[None] != 0) & (kb_d[None] <= lvl)).double()
+    maskU = maskT.clone()
+    maskU[:, :, :-1] = torch.minimum(maskT[:, :, :-1], maskT[:, :, 1:])
+    maskV = maskT.clone()
+    maskV[:, :-1, :] = torch.minimum(maskT[:, :-1, :], maskT[:, 1:, :])
+    hu = torch.zeros(b0.shape, dtype=torch.float64, device=dev)
+    hv = torch.zeros_like(hu)
+    for k in range(nz):
+        hu = hu + maskU[k] * float(dzt[k])
+        hv = hv + maskV[k] * float(dzt[k])
+    hur = torch.where(hu != 0, 1.0 / torch.where(hu != 0, hu, 1.0), 0.0)
+    hvr = torch.where(hv != 0, 1.0 / torch.where(hv != 0, hv, 1.0), 0.0)
+    b = dataclasses.replace(b0, hur=hur.cpu().numpy(), hvr=hvr.cpu().numpy(),
+                            maskU=maskU[-1].cpu().numpy(), maskV=maskV[-1].cpu().numpy())
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=dev)
+    lev = Levels(dzt, dzw, maskT, maskU, maskV, dev)
+    del maskT, maskU, maskV, hu, hv
+    opts = SolveOptions(mode=args.step_mode, criterion="relres", relres_target=args.relres,
+                        max_iterations=200000, poly_degree=args.poly_degree,
+                        poly_lower=args.poly_lower)
+    psin, line_psin, _ = s.streamfunction_init(opts)
+    # synthetic model state of plausible magnitudes (seeded, on the device)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    sh3 = (nz,) + b.shape
+
+    def rnd(scale, mask, levels=None):
+        shape = sh3 if levels is None else (levels,) + sh3
+        return scale * torch.randn(shape, generator=g, dtype=torch.float64, device=dev) * mask
+
+    strat = torch.linspace(4.0, 0.0, nz, dtype=torch.float64, device=dev)[:, None, None]
+    rho = (1024.0 + strat + 0.05 * torch.randn((3,) + sh3, generator=g, dtype=torch.float64,
+                                                device=dev)) * lev.maskT
+    ms = MomentumState(rho, rnd(0.1, lev.maskU, 3), rnd(0.1, lev.maskV, 3),
+                       rnd(1e-6, lev.maskU, 3), rnd(1e-6, lev.maskV, 3), rnd(1e-7, lev.maskU),
+                       rnd(1e-7, lev.maskV), torch.zeros(sh3, dtype=torch.float64, device=dev))
+    st = BarotropicState.zeros(s, 3600.0, psin=psin.contiguous(), line_psin=line_psin)
+    torch.cuda.synchronize()
+    # rho, u, v, du, dv with 3 time levels; du_mix, dv_mix, p_hydro and 3 masks
+    gb = 21 * nz * b.shape[0] * b.shape[1] * 8 / 1e9
+    log(f"[step] basin {b.nx}x{b.ny} x {nz} levels, nisle={isl.nisle}, {gb:.1f} GB of 3-D "
+        f"state resident, set up in {time.time() - t:.1f}s")
+    c = float(np.sqrt(1 - 0.99 ** 2))
+    per_step, its, solve_ms = [], [], []
+    for k in range(args.warmup + args.steps):
+        # drift the tendencies at the coming tau level (outside the timing)
+        ms.du[st.tau] = 0.99 * ms.du[st.taum1] + rnd(c * 1e-6, lev.maskU)
+        ms.dv[st.tau] = 0.99 * ms.dv[st.taum1] + rnd(c * 1e-6, lev.maskV)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        n_it, conv, info = s.solve_streamfunction_3d(st, lev, ms, opts)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        if not conv:
+            raise SystemExit(f"step {k}: solve did not converge ({info})")
+        if k >= args.warmup:
+            per_step.append(te - ts)
+            its.append(n_it)
+            solve_ms.append(info["elapsed_ms"])
+        log(f"[step] {k}: {(te - ts) * 1e3:.2f} ms, solve {info['elapsed_ms']:.2f} ms, "
+            f"{n_it} iterations, relres {info['relres_true']:.2e}")
+        st.time_goes_by()
+    # the 3-D kernels alone, timed with events on the solver's stream
+    reps = 10
+    fpx, fpy = s.zeros(), s.zeros()
+    u_out = torch.empty(sh3, dtype=torch.float64, device=dev)
+    v_out = torch.empty_like(u_out)
+
+    def timed(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s.stream):
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_f = timed(lambda: s.streamfunction_forcing(lev, ms.rho[st.tau], ms.du[st.tau],
+                                                 ms.dv[st.tau], ms.du_mix, ms.dv_mix, ms.p_hydro,
+                                                 fpx, fpy))
+    t_v = timed(lambda: s.velocity_update(lev, 3600.0, ms.u[st.tau], ms.v[st.tau], ms.du[st.tau],
+                                          ms.dv[st.tau], ms.du[st.taum1], ms.dv[st.taum1],
+                                          ms.du_mix, ms.dv_mix, st.psi[st.tau], u_out, v_out))
+    pts = b.shape[0] * b.shape[1]
+    kern = {}
+    for name, ms_k in (("forcing", t_f), ("velocity", t_v)):
+        gbs = LEVEL_BYTES[name] * pts * nz / (ms_k * 1e-3) / 1e9
+        kern[name] = {"ms": round(ms_k, 4), "achieved": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "bytes_per_pt_level": LEVEL_BYTES[name]}
+    total = sum(per_step)
+    cpu = None if args.skip_cpu else levels_cpu_baseline(nz, budget_s=args.cpu_seconds)
+    line = {
+        "metric": f"ms per {args.n}^2 x {nz}-level solve_streamfunction step (forcing, psi solve "
+                  f"to {args.relres:g} rel. FP64 residual, velocity update)",
+        "value": round(1e3 * total / args.steps, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64 (3-D parts); solve " + ("f64" if args.step_mode == "plain"
+                                              else "f32-inner/f64-refined"),
+        "data": "synthetic",
+        "config": {"workload": f"spherical basin {b.nx}x{b.ny}, {nz} levels, 2 islands, "
+                               f"drifting tendencies, warm start", "path": args.step_mode},
+        "solve_ms_mean": round(float(np.mean(solve_ms)), 3),
+        "iterations_mean": round(float(np.mean(its)), 1),
+        "levels_kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": "k_velocity_update",
+                     "achieved": kern["velocity"]["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kern["velocity"]["frac"], "traffic": None},
+        "cpu_baseline_levels": cpu,
+    }
+    print(json.dumps(finite(line)), flush=True)
+
+
+def levels_cpu_baseline(nz, budget_s=10.0, n=512):
+    """The oracle's forcing assembly and velocity update (1 core) on an n^2
+    basin with nz levels, repeated within the budget, as ns per point-level."""
+    from oracle import oracle as O
+    from pyom2_amd import basins
+    from pyom2_amd.streamfunction import AB_EPS_REFERENCE, GRAV_REFERENCE, RHO_0_REFERENCE
+    b0 = basins.spherical_basin(seed=0, n=n, n_islands=0, dlon=0.05 * 2048 / n)
+    b, col = basins.column_basin(b0, nz)
+    m = basins.momentum_fields(col, 0)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        O.streamfunction_forcing(b, col, m.rho[1], m.du[1], m.dv[1], m.du_mix, m.dv_mix,
+                                 GRAV_REFERENCE / RHO_0_REFERENCE)
+        O.velocity_update(b, col, 3600.0, AB_EPS_REFERENCE, m.u[1], m.v[1], m.du[1], m.dv[1], m.du[0],
+                          m.dv[0], m.du_mix, m.dv_mix, m.psi[2])
+        reps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    ns = dt / (b.shape[0] * b.shape[1] * nz) * 1e9
+    return {"value": round(ns, 3), "unit": "ns per point-level (forcing + velocity update)",
+            "cores": 1, "kind": "port",
+            "sample": f"{reps} x (forcing + velocity update) on {b.nx}x{b.ny}x{nz} in "
+                      f"{dt * reps:.1f} s, C restatement, 1 thread"}
 
 
 if __name__ == "__main__":

@@ -17,8 +17,8 @@
  *    contexts are independent of each other (the reference's SAVE'd
  *    coefficient cache becomes per-context state) but one context must not
  *    be used from two threads at once.
- *  - psi_congrad and psi_solve_streamfunction allocate no device memory:
- *    all scratch is allocated by psi_ctx_create.
+ *  - psi_congrad, psi_solve_streamfunction and the 3-D entry points
+ *    allocate no device memory: all scratch is allocated by psi_ctx_create.
  */
 #ifndef PSISOLVE_H
 #define PSISOLVE_H
@@ -207,6 +207,63 @@ int psi_solve_streamfunction(psi_ctx *ctx, const psi_opts *opts, const double *f
                              double *dpsin, const double *psin, const double *line_psin,
                              int32_t taum1, int32_t tau, int32_t taup1, double dt_mom,
                              double AB_eps, double *forc_out, psi_info *info);
+
+/* ---- the 3-D parts of solve_streamfunction --------------------------------
+ * A 3-D field a(i,j,k) is nz consecutive (ny+4, nx+4) planes (level k = 1,
+ * the bottom, first): the Fortran a(-1:nx+2, -1:ny+2, nz) in place.  A
+ * field with time levels, a(:,:,:,1:3), is 3 such blocks.  Fields are
+ * device arrays; dzt/dzw are host arrays like the other 1-D metrics.  nz is
+ * at most 128. */
+typedef struct psi_levels {
+    int32_t nz;
+    int32_t reserved;
+    const double *dzt, *dzw;               /* host, nz entries (main_module.f90:108) */
+    const double *maskT, *maskU, *maskV;   /* nz planes each (:96-98) */
+} psi_levels;
+
+/* Hydrostatic pressure, its gradient added to du/dv and the depth-integrated
+ * forcing (solve_streamfct.f90:22-48): p_hydro from rho (tau level), du/dv
+ * (tau level, updated in place on the interior), fpx = hur sum_k (du+du_mix)
+ * maskU dzt, fpy likewise, with their cyclic halo.  fxa = grav/rho_0
+ * (main_module.f90:18-19: pass 9.81 and 1024 as the reference's real*8
+ * parameters hold them). */
+int psi_streamfunction_forcing(psi_ctx *ctx, const psi_levels *lev, double grav, double rho_0,
+                               const double *rho, double *du, double *dv,
+                               const double *du_mix, const double *dv_mix, double *p_hydro,
+                               double *fpx, double *fpy);
+
+/* Barotropic and baroclinic velocity at taup1 (solve_streamfct.f90:95-118;
+ * the open-boundary call set_obc_streamfct, :101, is not part of it):
+ * Adams-Bashforth step of u, v from du, dv (tau, taum1) and du_mix, dv_mix,
+ * removal of the vertical mean, then the barotropic velocity of psi(taup1)
+ * added on the interior.  Outputs u_taup1, v_taup1 (nz planes each). */
+int psi_velocity_update(psi_ctx *ctx, const psi_levels *lev, double dt_mom, double AB_eps,
+                        const double *u_tau, const double *v_tau, const double *du_tau,
+                        const double *dv_tau, const double *du_taum1, const double *dv_taum1,
+                        const double *du_mix, const double *dv_mix, const double *psi_taup1,
+                        double *u_taup1, double *v_taup1);
+
+/* The model state solve_streamfunction reads and writes (main_module.f90:
+ * rho :127, u/du and v/dv :143-144, du_mix/dv_mix :148-149, p_hydro :154). */
+typedef struct psi_momentum {
+    const double *rho;             /* 3 time levels x nz planes */
+    double *u, *v;                 /* 3 x nz planes */
+    double *du, *dv;               /* 3 x nz planes */
+    const double *du_mix, *dv_mix; /* nz planes */
+    double *p_hydro;               /* nz planes */
+} psi_momentum;
+
+/* The whole solve_streamfunction (solve_streamfct.f90:10-120, serial, without
+ * open boundaries): psi_streamfunction_forcing, psi_solve_streamfunction and
+ * psi_velocity_update in sequence on the context's stream, with fpx/fpy in
+ * context scratch.  Returns the solve's status; on PSI_NAN (where the
+ * reference halts) the velocities are not updated. */
+int psi_solve_streamfunction_3d(psi_ctx *ctx, const psi_opts *opts, const psi_levels *lev,
+                                const psi_momentum *mom, double grav, double rho_0,
+                                double *psi, double *dpsi, double *dpsin, const double *psin,
+                                const double *line_psin, int32_t taum1, int32_t tau,
+                                int32_t taup1, double dt_mom, double AB_eps, double *forc_out,
+                                psi_info *info);
 
 /* ---- flexible CG with the learned preconditioner ---------------------------
  * No counterpart in the reference.  The host applies the network of psinn.h

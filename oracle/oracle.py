@@ -286,3 +286,103 @@ def true_relres(b, cf, Z, forc, sol):
     r = forc - apply_op(b.nx, b.ny, cf, sol)
     m = Z != 0
     return float(np.sqrt((r[m] ** 2).sum()) / max(np.sqrt((forc[m] ** 2).sum()), 1e-300))
+
+
+def _levels_lib():
+    L = lib()
+    if not getattr(L, "_levels_set", False):
+        I, D = C.c_int, C.c_double
+        L.orc_streamfunction_forcing.argtypes = [I, I, I, I, D] + [_dp] * 18
+        L.orc_streamfunction_forcing.restype = None
+        L.orc_velocity_update.argtypes = [I, I, I, D, D] + [_dp] * 19
+        L.orc_velocity_update.restype = None
+        L._levels_set = True
+    return L
+
+
+def streamfunction_forcing(b, col, rho, du, dv, du_mix, dv_mix, fxa):
+    """Hydrostatic pressure, pressure-gradient update of du/dv and the
+    depth-integrated forcing of solve_streamfunction (solve_streamfct.f90:22-48).
+    rho, du, dv: tau levels (nz, ny+4, nx+4); du, dv are returned updated
+    (copies).  Returns (p_hydro, du, dv, fpx, fpy)."""
+    du = _c(du).copy()
+    dv = _c(dv).copy()
+    p = np.zeros(col.shape)
+    fpx = np.zeros(b.shape)
+    fpy = np.zeros(b.shape)
+    _levels_lib().orc_streamfunction_forcing(
+        b.nx, b.ny, col.nz, int(b.cyclic_x), fxa, _c(b.dxu), _c(b.dyu), _c(b.cost),
+        _c(col.dzt), _c(col.dzw), _c(col.maskT), _c(col.maskU), _c(col.maskV), _c(b.hur),
+        _c(b.hvr), _c(rho), du, dv, _c(du_mix), _c(dv_mix), p, fpx, fpy)
+    return p, du, dv, fpx, fpy
+
+
+def velocity_update(b, col, dt_mom, AB_eps, u_tau, v_tau, du_tau, dv_tau, du_taum1, dv_taum1,
+                    du_mix, dv_mix, psi_taup1):
+    """u, v at taup1: Adams-Bashforth step, vertical-mean removal and the
+    barotropic mode of psi(taup1) (solve_streamfct.f90:95-118)."""
+    u = np.zeros(col.shape)
+    v = np.zeros(col.shape)
+    _levels_lib().orc_velocity_update(
+        b.nx, b.ny, col.nz, dt_mom, AB_eps, _c(b.dxt), _c(b.dyt), _c(b.cosu), _c(col.dzt),
+        _c(col.maskU), _c(col.maskV), _c(b.hur), _c(b.hvr), _c(u_tau), _c(v_tau), _c(du_tau),
+        _c(dv_tau), _c(du_taum1), _c(dv_taum1), _c(du_mix), _c(dv_mix), _c(psi_taup1), u, v)
+    return u, v
+
+
+def curl_forcing(b, fpx, fpy):
+    """forc = curl of (fpx, fpy) on the interior (solve_streamfct.f90:49-53)."""
+    nx, ny = b.nx, b.ny
+    forc = np.zeros(b.shape)
+    j = np.arange(1, ny + 1)[:, None]
+    i = np.arange(1, nx + 1)[None, :]
+    forc[j + 1, i + 1] = ((fpy[j + 1, i + 2] - fpy[j + 1, i + 1]) / (b.cosu[j + 1] * b.dxu[i + 1]) -
+                          (b.cost[j + 2] * fpx[j + 2, i + 1] - b.cost[j + 1] * fpx[j + 1, i + 1]) /
+                          (b.cosu[j + 1] * b.dyu[j + 1]))
+    return forc
+
+
+def solve_streamfunction_2d(b, init, fpx, fpy, psi, dpsi, dpsin, taum1, tau, taup1, dt, ab,
+                            eps=1e-12, max_iter=3000):
+    """solve_streamfunction's 2-D body (solve_streamfct.f90:49-94): curl, warm
+    start, congrad, island system, Adams-Bashforth step of psi.  psi, dpsi
+    (3, ny+4, nx+4) and dpsin (nisle, 3) are updated in place.  Returns the
+    congrad result."""
+    nx, ny = b.nx, b.ny
+    forc = curl_forcing(b, fpx, fpy)
+    dpsi[taup1] = 2 * dpsi[tau] - dpsi[taum1]
+    r = congrad(nx, ny, b.cyclic_x, init.cf, init.Z, forc, dpsi[taup1], eps, max_iter)
+    dpsi[taup1] = setcyclic(nx, ny, b.cyclic_x, r.sol)
+    n = init.islands.nisle
+    if n > 1:
+        lf = np.array([line_integral(b, init.islands, k, fpx, fpy) for k in range(n)])
+        u, v = psi_velocity(b, dpsi[taup1])
+        l0 = np.array([line_integral(b, init.islands, k, u, v) for k in range(n)])
+        x, info = dense_solve(init.line_psin[1:, 1:], (lf - l0)[1:])
+        if info != 0:
+            raise ValueError("island system singular")
+        dpsin[1:, tau] = x
+    psi[taup1] = psi[tau] + dt * ((1.5 + ab) * dpsi[taup1] - (0.5 + ab) * dpsi[tau])
+    for k in range(1, n):
+        psi[taup1] = psi[taup1] + dt * ((1.5 + ab) * dpsin[k, tau] -
+                                        (0.5 + ab) * dpsin[k, taum1]) * init.psin[k]
+    return r
+
+
+def solve_streamfunction(b, init, col, mom, dpsin, taum1, tau, taup1, dt, ab, grav, rho_0,
+                         eps=1e-12, max_iter=3000):
+    """The whole solve_streamfunction (solve_streamfct.f90:10-120, serial, no
+    open boundaries) on a basins.Momentum state, updated in place (p_hydro
+    is returned).  Returns (p_hydro, congrad result)."""
+    fxa = grav / rho_0
+    p, du, dv, fpx, fpy = streamfunction_forcing(b, col, mom.rho[tau], mom.du[tau], mom.dv[tau],
+                                                 mom.du_mix, mom.dv_mix, fxa)
+    mom.du[tau] = du
+    mom.dv[tau] = dv
+    r = solve_streamfunction_2d(b, init, fpx, fpy, mom.psi, mom.dpsi, dpsin, taum1, tau, taup1,
+                                dt, ab, eps, max_iter)
+    u, v = velocity_update(b, col, dt, ab, mom.u[tau], mom.v[tau], mom.du[tau], mom.dv[tau],
+                           mom.du[taum1], mom.dv[taum1], mom.du_mix, mom.dv_mix, mom.psi[taup1])
+    mom.u[taup1] = u
+    mom.v[taup1] = v
+    return p, r

@@ -487,3 +487,113 @@ int orc_dense_solve(int n, double *A, double *b)
     }
     return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* The 3-D parts of solve_streamfunction around the 2-D solve.
+ * A 3-D field a(i,j,k), k = 1..nz, is stored as nz consecutive 2-D planes
+ * (level k at offset (k-1)*(nx+4)*(ny+4)), byte-identical to the Fortran
+ * a(-1:nx+2, -1:ny+2, nz).  dzt, dzw have nz entries. */
+
+/* Hydrostatic pressure, the pressure gradient added to du/dv and the
+ * depth-integrated forcing (solve_streamfct.f90:22-48):
+ *   p_hydro(:,:,nz) = 0.5 rho(nz) fxa dzw(nz) maskT(nz)                    (:24)
+ *   p_hydro(:,:,k)  = maskT(k) (p_hydro(k+1) + 0.5 (rho(k+1)+rho(k)) fxa dzw(k))  (:25-27)
+ *   du -= (p(i+1,j)-p(i,j))/(dxu(i) cost(j)) maskU,
+ *   dv -= (p(i,j+1)-p(i,j))/dyu(j) maskV          on the interior         (:30-35)
+ *   fpx = sum_k (du+du_mix) maskU dzt(k) * hur, fpy likewise with hvr     (:37-42)
+ *   cyclic halo of fpx, fpy                                               (:43-46)
+ * fxa = grav/rho_0 (:23).  rho, du, dv are the tau levels. */
+void orc_streamfunction_forcing(int nx, int ny, int nz, int cyclic_x, double fxa,
+                                const double *dxu, const double *dyu, const double *cost,
+                                const double *dzt, const double *dzw, const double *maskT,
+                                const double *maskU, const double *maskV, const double *hur,
+                                const double *hvr, const double *rho, double *du, double *dv,
+                                const double *du_mix, const double *dv_mix, double *p_hydro,
+                                double *fpx, double *fpy)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const size_t top = (size_t)(nz - 1) * plane;
+    for (size_t q = 0; q < plane; ++q)
+        p_hydro[top + q] = 0.5 * rho[top + q] * fxa * dzw[nz - 1] * maskT[top + q];
+    for (int k = nz - 2; k >= 0; --k) {
+        const size_t o = (size_t)k * plane;
+        for (size_t q = 0; q < plane; ++q)
+            p_hydro[o + q] = maskT[o + q] * (p_hydro[o + plane + q] +
+                                             0.5 * (rho[o + plane + q] + rho[o + q]) * fxa * dzw[k]);
+    }
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int k = 0; k < nz; ++k) {
+                const size_t o = (size_t)k * plane, q = o + IX(i, j, nx);
+                du[q] = du[q] - (p_hydro[o + IX(i + 1, j, nx)] - p_hydro[q]) /
+                                    (dxu[i + 1] * cost[j + 1]) * maskU[q];
+                dv[q] = dv[q] - (p_hydro[o + IX(i, j + 1, nx)] - p_hydro[q]) / dyu[j + 1] *
+                                    maskV[q];
+            }
+    for (size_t q = 0; q < plane; ++q) { fpx[q] = 0.0; fpy[q] = 0.0; }
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        for (size_t q = 0; q < plane; ++q) {
+            fpx[q] = fpx[q] + (du[o + q] + du_mix[o + q]) * maskU[o + q] * dzt[k];
+            fpy[q] = fpy[q] + (dv[o + q] + dv_mix[o + q]) * maskV[o + q] * dzt[k];
+        }
+    }
+    for (size_t q = 0; q < plane; ++q) { fpx[q] = fpx[q] * hur[q]; fpy[q] = fpy[q] * hvr[q]; }
+    orc_setcyclic(nx, ny, cyclic_x, fpx);
+    orc_setcyclic(nx, ny, cyclic_x, fpy);
+}
+
+/* Velocity update after the solve (solve_streamfct.f90:95-118; the
+ * open-boundary call set_obc_streamfct at :101 is not part of it):
+ *   u(taup1) = u(tau) + dt_mom (du_mix + (1.5+AB_eps) du(tau)
+ *                               - (0.5+AB_eps) du(taum1)) maskU             (:96)
+ *   fpx = sum_k u(taup1) maskU dzt(k);  u(taup1) -= fpx maskU hur          (:104-112)
+ *   u(taup1) -= maskU (psi(i,j)-psi(i,j-1))/dyt(j) hur(i,j)   on the interior
+ *   v(taup1) += maskV (psi(i,j)-psi(i-1,j))/(cosu(j) dxt(i)) hvr(i,j)      (:114-119)
+ * psi is psi(:,:,taup1). */
+void orc_velocity_update(int nx, int ny, int nz, double dt_mom, double AB_eps,
+                         const double *dxt, const double *dyt, const double *cosu,
+                         const double *dzt, const double *maskU, const double *maskV,
+                         const double *hur, const double *hvr, const double *u_tau,
+                         const double *v_tau, const double *du_tau, const double *dv_tau,
+                         const double *du_taum1, const double *dv_taum1, const double *du_mix,
+                         const double *dv_mix, const double *psi, double *u_taup1,
+                         double *v_taup1)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const size_t n3 = (size_t)nz * plane;
+    for (size_t q = 0; q < n3; ++q) {
+        u_taup1[q] = u_tau[q] + dt_mom * (du_mix[q] + (1.5 + AB_eps) * du_tau[q] -
+                                          (0.5 + AB_eps) * du_taum1[q]) * maskU[q];
+        v_taup1[q] = v_tau[q] + dt_mom * (dv_mix[q] + (1.5 + AB_eps) * dv_tau[q] -
+                                          (0.5 + AB_eps) * dv_taum1[q]) * maskV[q];
+    }
+    double *fpx = calloc(plane, sizeof(double)), *fpy = calloc(plane, sizeof(double));
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        for (size_t q = 0; q < plane; ++q) {
+            fpx[q] = fpx[q] + u_taup1[o + q] * maskU[o + q] * dzt[k];
+            fpy[q] = fpy[q] + v_taup1[o + q] * maskV[o + q] * dzt[k];
+        }
+    }
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        for (size_t q = 0; q < plane; ++q) {
+            u_taup1[o + q] = u_taup1[o + q] - fpx[q] * maskU[o + q] * hur[q];
+            v_taup1[o + q] = v_taup1[o + q] - fpy[q] * maskV[o + q] * hvr[q];
+        }
+    }
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const size_t p = IX(i, j, nx);
+            const double dps_y = psi[p] - psi[IX(i, j - 1, nx)];
+            const double dps_x = psi[p] - psi[IX(i - 1, j, nx)];
+            for (int k = 0; k < nz; ++k) {
+                const size_t q = (size_t)k * plane + p;
+                u_taup1[q] = u_taup1[q] - maskU[q] * dps_y / dyt[j + 1] * hur[p];
+                v_taup1[q] = v_taup1[q] + maskV[q] * dps_x / (cosu[j + 1] * dxt[i + 1]) * hvr[p];
+            }
+        }
+    free(fpx);
+    free(fpy);
+}

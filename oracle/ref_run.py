@@ -81,3 +81,48 @@ def run(b, forc, sol, eps=1e-12, max_iter=1000, uloc=None, vloc=None):
     return dict(nisle=nisle, nr_boundary=nrb, boundary=bnd, iterations=int(iters),
                 converged=bool(conv), line_psin=line_psin, psin=psin, sol=sol_out,
                 res=res, lines=lines, log=p.stdout.decode(errors="replace"))
+
+
+STEP = os.path.join(HERE, "_ref", "ref_step")
+
+
+def step_available() -> bool:
+    return os.path.exists(STEP)
+
+
+def run_step(b, col, mom, eps=1e-12, max_iter=1000, dt_mom=3600.0):
+    """Run the reference's streamfunction_init + solve_streamfunction on basin
+    b with levels col and state mom (basins.Column / basins.Momentum, time
+    levels taum1=0, tau=1, taup1=2).  The basin must have no islands.
+    Returns a dict of the outputs."""
+    nx, ny, nz = b.nx, b.ny, col.nz
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nx, ny, nz, int(b.cyclic_x), max_iter], np.int32).tofile(f)
+            np.array([eps, dt_mom], np.float64).tofile(f)
+            for a in (b.dxt, b.dxu, b.dyt, b.dyu, b.cost, b.cosu, col.dzt, col.dzw):
+                f64(a).tofile(f)
+            np.ascontiguousarray(col.kbot, np.int32).tofile(f)
+            for a in (b.hur, b.hvr, col.maskT, col.maskU, col.maskV, mom.rho[1], mom.du[1],
+                      mom.du[0], mom.dv[1], mom.dv[0], mom.du_mix, mom.dv_mix, mom.u[1],
+                      mom.v[1], mom.psi[1], mom.dpsi[1], mom.dpsi[0]):
+                f64(a).tofile(f)
+        p = subprocess.run([STEP, fin, fout], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=3600)
+        if not os.path.exists(fout):
+            raise RuntimeError("reference step failed:\n" + p.stdout.decode()[-2000:])
+        raw = open(fout, "rb").read()
+    plane = (ny + 4) * (nx + 4)
+    nisle, itts = np.frombuffer(raw, np.int32, 2, 0)
+    off = 8
+    out = {}
+    for name, n in (("p_hydro", nz), ("du", nz), ("dv", nz), ("u", nz), ("v", nz),
+                    ("psi", 1), ("dpsi", 1)):
+        a = np.frombuffer(raw, np.float64, n * plane, off).copy()
+        off += 8 * n * plane
+        out[name] = a.reshape((n, ny + 4, nx + 4) if name not in ("psi", "dpsi")
+                              else (ny + 4, nx + 4))
+    out.update(nisle=int(nisle), iterations=int(itts), log=p.stdout.decode(errors="replace"))
+    return out

@@ -91,6 +91,18 @@ class PsiBatchStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class PsiLevels(C.Structure):
+    _fields_ = [("nz", C.c_int32), ("reserved", C.c_int32), ("dzt", C.c_void_p),
+                ("dzw", C.c_void_p), ("maskT", C.c_void_p), ("maskU", C.c_void_p),
+                ("maskV", C.c_void_p)]
+
+
+class PsiMomentum(C.Structure):
+    _fields_ = [("rho", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("du", C.c_void_p),
+                ("dv", C.c_void_p), ("du_mix", C.c_void_p), ("dv_mix", C.c_void_p),
+                ("p_hydro", C.c_void_p)]
+
+
 # (name, restype, argtypes) of every entry point in include/*.h
 SIGNATURES = [
     ("psi_islands_trace", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip, C.c_int32, _ip,
@@ -119,6 +131,15 @@ SIGNATURES = [
                                            C.c_void_p, C.c_void_p, _dp, C.c_void_p, _dp, C.c_int32,
                                            C.c_int32, C.c_int32, C.c_double, C.c_double,
                                            C.c_void_p, C.POINTER(PsiInfo)]),
+    ("psi_streamfunction_forcing", C.c_int, [C.c_void_p, C.POINTER(PsiLevels), C.c_double,
+                                             C.c_double] + [C.c_void_p] * 8),
+    ("psi_velocity_update", C.c_int, [C.c_void_p, C.POINTER(PsiLevels), C.c_double, C.c_double]
+     + [C.c_void_p] * 11),
+    ("psi_solve_streamfunction_3d", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.POINTER(PsiLevels),
+                                              C.POINTER(PsiMomentum), C.c_double, C.c_double,
+                                              C.c_void_p, C.c_void_p, _dp, C.c_void_p, _dp,
+                                              C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                              C.c_double, C.c_void_p, C.POINTER(PsiInfo)]),
     ("psi_fcg_prepare", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_fcg_begin", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     ("psi_fcg_input_scale", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),

@@ -364,3 +364,132 @@ def ensemble_basins(seed: int = 0, count: int = 512, lo: int = 64, hi: int = 512
         out.append(beta_plane_basin(seed=int(rng.integers(0, 2**31 - 1)), nx=nx, ny=ny,
                                     n_islands=nisl))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Vertical levels around the barotropic solve
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Column:
+    """Vertical grid and 3-D land masks of a basin (calc_grid / calc_topo,
+    numerics.f90:132-136, 215-259).  Level k = 1..nz (index k-1 here) counts
+    upwards: level nz is the surface.  3-D fields are C-ordered
+    ``(nz, ny+4, nx+4)``, byte-identical to Fortran ``a(-1:nx+2, -1:ny+2, nz)``."""
+    nz: int
+    dzt: np.ndarray           # (nz,)
+    dzw: np.ndarray           # (nz,)
+    kbot: np.ndarray          # int32 (ny+4, nx+4): bottom level, 0 on land
+    maskT: np.ndarray
+    maskU: np.ndarray
+    maskV: np.ndarray
+
+    @property
+    def shape(self):
+        return self.maskT.shape
+
+
+def vertical_grid(nz: int, depth: float = 5000.0, stretch: float = 1.08):
+    """Layer thicknesses growing geometrically with depth (thinnest at the
+    surface, level nz), summing to ``depth``; dzw from u_centered_grid as
+    calc_grid does (numerics.f90:132)."""
+    w = stretch ** np.arange(nz)[::-1]
+    dzt = depth * w / w.sum()
+    dzw, _, _ = u_centered_grid(dzt)
+    return dzt, dzw
+
+
+def _setcyclic(a: np.ndarray, nx: int):
+    """setcyclic_xy on the last axis (parallel_none.f90:188-211)."""
+    a[..., nx + 2:nx + 4] = a[..., 2:4]
+    a[..., 0:2] = a[..., nx:nx + 2]
+
+
+def column_levels(b: Basin, nz: int, depth: float | None = None):
+    """Layer thicknesses and bottom levels of basin b: each ocean column is
+    wet from the level whose centre lies above its depth ht up to the surface
+    (at least one level).  Returns (dzt, dzw, kbot) with kbot 0 on land."""
+    depth = float(b.ht.max()) if depth is None else depth
+    dzt, dzw = vertical_grid(nz, depth)
+    ztop = np.concatenate([np.cumsum(dzt[::-1])[::-1][1:], [0.0]])   # depth of each level's top
+    zc = ztop + 0.5 * dzt                                              # centre depth
+    wet_levels = np.zeros(b.shape, np.int64)
+    for z in zc:
+        wet_levels += z < b.ht
+    wet_levels = np.maximum(wet_levels, 1)
+    kbot = np.where(b.kbot != 0, nz - wet_levels + 1, 0).astype(np.int32)
+    return dzt, dzw, kbot
+
+
+def column_basin(b: Basin, nz: int, depth: float | None = None):
+    """Levels for basin b (column_levels); masks follow calc_topo
+    (numerics.f90:215-247) and the inverse depths hur = 1/sum_k maskU dzt,
+    hvr likewise (:253-259).  Returns the basin with hu/hv/hur/hvr and the
+    surface masks replaced by those of the levels (so the 2-D solve and the
+    3-D fields agree) and the Column."""
+    dzt, dzw, kbot = column_levels(b, nz, depth)
+    k = np.arange(1, nz + 1)[:, None, None]
+    maskT = ((kbot[None] != 0) & (kbot[None] <= k)).astype(np.float64)
+    if b.cyclic_x:
+        _setcyclic(maskT, b.nx)
+    maskU = maskT.copy()
+    maskU[:, :, :-1] = np.minimum(maskT[:, :, :-1], maskT[:, :, 1:])
+    maskV = maskT.copy()
+    maskV[:, :-1, :] = np.minimum(maskT[:, :-1, :], maskT[:, 1:, :])
+    if b.cyclic_x:
+        _setcyclic(maskU, b.nx)
+        _setcyclic(maskV, b.nx)
+    hu = np.zeros(b.shape)
+    hv = np.zeros(b.shape)
+    for kk in range(nz):
+        hu = hu + maskU[kk] * dzt[kk]
+        hv = hv + maskV[kk] * dzt[kk]
+    with np.errstate(divide="ignore"):
+        hur = np.where(hu != 0.0, 1.0 / np.where(hu != 0.0, hu, 1.0), 0.0)
+        hvr = np.where(hv != 0.0, 1.0 / np.where(hv != 0.0, hv, 1.0), 0.0)
+    b3 = dataclasses.replace(b, hu=hu, hv=hv, hur=hur, hvr=hvr, maskT=maskT[-1].copy(),
+                             maskU=maskU[-1].copy(), maskV=maskV[-1].copy())
+    return b3, Column(nz, dzt, dzw, kbot, maskT, maskU, maskV)
+
+
+@dataclasses.dataclass
+class Momentum:
+    """The model state solve_streamfunction reads and writes
+    (main_module.f90:147-158, 264-311): time levels are the leading axis
+    (0..2 for the reference's 1..3)."""
+    rho: np.ndarray        # (3, nz, ny+4, nx+4)
+    u: np.ndarray
+    v: np.ndarray
+    du: np.ndarray
+    dv: np.ndarray
+    du_mix: np.ndarray     # (nz, ny+4, nx+4)
+    dv_mix: np.ndarray
+    psi: np.ndarray        # (3, ny+4, nx+4)
+    dpsi: np.ndarray
+
+
+def momentum_fields(col: Column, seed: int = 0) -> Momentum:
+    """Seeded synthetic state of plausible magnitudes: density 1024-1028
+    kg/m^3 stratified with horizontal structure, velocities ~0.1 m/s,
+    tendencies ~1e-6 m/s^2, streamfunction ~1e6 m^3/s; all masked."""
+    rng = np.random.default_rng(seed)
+    nz, ny4, nx4 = col.shape
+    mT, mU, mV = col.maskT, col.maskU, col.maskV
+    strat = np.linspace(4.0, 0.0, nz)[:, None, None]
+    jj, ii = np.meshgrid(np.linspace(0, 1, ny4), np.linspace(0, 1, nx4), indexing="ij")
+    horiz = 0.5 * np.sin(3 * np.pi * ii) * np.cos(2 * np.pi * jj)
+    rho = np.empty((3, nz, ny4, nx4))
+    u = np.empty_like(rho)
+    v = np.empty_like(rho)
+    du = np.empty_like(rho)
+    dv = np.empty_like(rho)
+    for t in range(3):
+        rho[t] = (1024.0 + strat + horiz[None] + 0.05 * rng.normal(size=(nz, ny4, nx4))) * mT
+        u[t] = 0.1 * rng.normal(size=(nz, ny4, nx4)) * mU
+        v[t] = 0.1 * rng.normal(size=(nz, ny4, nx4)) * mV
+        du[t] = 1e-6 * rng.normal(size=(nz, ny4, nx4)) * mU
+        dv[t] = 1e-6 * rng.normal(size=(nz, ny4, nx4)) * mV
+    du_mix = 1e-7 * rng.normal(size=(nz, ny4, nx4)) * mU
+    dv_mix = 1e-7 * rng.normal(size=(nz, ny4, nx4)) * mV
+    psi = 1e6 * rng.normal(size=(3, ny4, nx4))
+    dpsi = 1e4 * rng.normal(size=(3, ny4, nx4))
+    return Momentum(rho, u, v, du, dv, du_mix, dv_mix, psi, dpsi)

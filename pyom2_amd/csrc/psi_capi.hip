@@ -18,6 +18,7 @@
 // The kernels are compiled in this translation unit.
 #include "psi_kernels.hip"
 #include "psi_poly.hip"
+#include "psi_levels.hip"
 
 using namespace psi;
 
@@ -44,6 +45,10 @@ struct psi_ctx {
     float* rp32[2] = {nullptr, nullptr};   // residuals of the polynomial-preconditioned path
     int Gpoly[PMAXK + 1] = {0};            // resident grid of k_poly_pcg<K> (0: not yet queried)
     double *r64 = nullptr, *tmp64 = nullptr, *tmp64b = nullptr;
+    // 3-D parts of solve_streamfunction: fpx/fpy of the whole routine and the
+    // layer thicknesses (dzt, dzw) as last uploaded
+    double *fpx3 = nullptr, *fpy3 = nullptr, *dz = nullptr;
+    std::vector<double> h_dz;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *pf32 = nullptr;   // flexible CG
     double* fcg = nullptr;        // flexible-CG device scalars
     int fcg_restart = 1;
@@ -875,6 +880,9 @@ int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
         TRY(dalloc(c, &c->fcg, 16));
         TRY(dalloc(c, &c->tmp64, c->plane));
         TRY(dalloc(c, &c->tmp64b, c->plane));
+        TRY(dalloc(c, &c->fpx3, c->plane));
+        TRY(dalloc(c, &c->fpy3, c->plane));
+        TRY(dalloc(c, &c->dz, 2 * (size_t)LV_MAX_NZ));
         TRY(dalloc(c, &c->part, (size_t)NRED * (g.ntiles + 1024)));
         {
             // resident grid: CUs x workgroups per CU of the FP64 sweep (the
@@ -1293,6 +1301,126 @@ int psi_solve_streamfunction(psi_ctx* c, const psi_opts* o, const double* fpx, c
                        dpsi + taup1 * P, dpsi + tau * P, dt_mom, w1, w0, psin_arg, dcoef,
                        n > 1 ? n : 0, psi_f + taup1 * P);
     rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return solve_rc;
+}
+
+// ---- the 3-D parts of solve_streamfunction ---------------------------------
+static int levels_check(psi_ctx* c, const psi_levels* L) {
+    if (!L || L->nz < 1 || L->nz > LV_MAX_NZ || !L->dzt || !L->dzw || !L->maskT || !L->maskU ||
+        !L->maskV)
+        return fail(c, PSI_BAD_ARG, "levels: nz must be 1..128 and every array given%s");
+    // dzt, dzw to the device when they changed (host copy of the last upload)
+    const size_t nz = (size_t)L->nz;
+    bool same = c->h_dz.size() == 2 * nz;
+    for (size_t k = 0; same && k < nz; ++k)
+        same = c->h_dz[k] == L->dzt[k] && c->h_dz[nz + k] == L->dzw[k];
+    if (!same) {
+        c->h_dz.assign(L->dzt, L->dzt + nz);
+        c->h_dz.insert(c->h_dz.end(), L->dzw, L->dzw + nz);
+        HIPCHK(c, hipMemcpyAsync(c->dz, c->h_dz.data(), 2 * nz * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));   // h_dz may change on the next call
+    }
+    return PSI_OK;
+}
+
+static int forcing_async(psi_ctx* c, const psi_levels* L, double grav, double rho_0,
+                         const double* rho, double* du, double* dv, const double* du_mix,
+                         const double* dv_mix, double* p_hydro, double* fpx, double* fpy) {
+    const double fxa = grav / rho_0;   // solve_streamfct.f90:23
+    const int nz = L->nz;
+    hipLaunchKernelGGL(k_hydrostatic, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->plane, nz,
+                       fxa, c->dz + nz, rho, L->maskT, p_hydro);
+    const dim3 grid((c->g.pitch + 63) / 64, (c->g.ny + 4 + LV_TY - 1) / LV_TY);
+    hipLaunchKernelGGL(k_streamfunction_forcing, grid, dim3(64, LV_TY), 0, c->stream, c->g, nz,
+                       c->dxu, c->dyu, c->cost, c->dz, L->maskU, L->maskV, c->hur, c->hvr,
+                       p_hydro, du, dv, du_mix, dv_mix, fpx, fpy);
+    if (c->g.cyclic) {
+        hipLaunchKernelGGL(k_setcyclic, dim3(nblk(c->g.ny + 4)), dim3(256), 0, c->stream, c->g,
+                           fpx);
+        hipLaunchKernelGGL(k_setcyclic, dim3(nblk(c->g.ny + 4)), dim3(256), 0, c->stream, c->g,
+                           fpy);
+    }
+    return check_launch(c);
+}
+
+static int velocity_async(psi_ctx* c, const psi_levels* L, double dt_mom, double AB_eps,
+                          const double* u_tau, const double* v_tau, const double* du_tau,
+                          const double* dv_tau, const double* du_taum1, const double* dv_taum1,
+                          const double* du_mix, const double* dv_mix, const double* psi_f,
+                          double* u_out, double* v_out) {
+    const double w1 = 1.5 + AB_eps, w0 = 0.5 + AB_eps;   // solve_streamfct.f90:96-97
+    const int nz = L->nz, lpt = velocity_lpt(nz);
+    const auto kern = velocity_kernel(lpt);
+    if (!kern) return fail(c, PSI_BAD_ARG, "levels: nz out of range%s");
+    const dim3 grid((c->g.pitch + 63) / 64, c->g.ny + 4);
+    const dim3 block(64, (nz + lpt - 1) / lpt);
+    hipLaunchKernelGGL(kern, grid, block, 0, c->stream, c->g, nz, dt_mom, w1, w0, c->dxt, c->dyt,
+                       c->cosu, c->dz, L->maskU, L->maskV, c->hur, c->hvr, u_tau, v_tau, du_tau,
+                       dv_tau, du_taum1, dv_taum1, du_mix, dv_mix, psi_f, u_out, v_out);
+    return check_launch(c);
+}
+
+int psi_streamfunction_forcing(psi_ctx* c, const psi_levels* L, double grav, double rho_0,
+                               const double* rho, double* du, double* dv, const double* du_mix,
+                               const double* dv_mix, double* p_hydro, double* fpx, double* fpy) {
+    if (!valid_field_ctx(c) || !rho || !du || !dv || !du_mix || !dv_mix || !p_hydro || !fpx ||
+        !fpy || !(rho_0 != 0.0))
+        return PSI_BAD_ARG;
+    int rc = levels_check(c, L);
+    if (rc) return rc;
+    rc = forcing_async(c, L, grav, rho_0, rho, du, dv, du_mix, dv_mix, p_hydro, fpx, fpy);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+int psi_velocity_update(psi_ctx* c, const psi_levels* L, double dt_mom, double AB_eps,
+                        const double* u_tau, const double* v_tau, const double* du_tau,
+                        const double* dv_tau, const double* du_taum1, const double* dv_taum1,
+                        const double* du_mix, const double* dv_mix, const double* psi_taup1,
+                        double* u_taup1, double* v_taup1) {
+    if (!valid_field_ctx(c) || !u_tau || !v_tau || !du_tau || !dv_tau || !du_taum1 ||
+        !dv_taum1 || !du_mix || !dv_mix || !psi_taup1 || !u_taup1 || !v_taup1)
+        return PSI_BAD_ARG;
+    int rc = levels_check(c, L);
+    if (rc) return rc;
+    rc = velocity_async(c, L, dt_mom, AB_eps, u_tau, v_tau, du_tau, dv_tau, du_taum1, dv_taum1,
+                        du_mix, dv_mix, psi_taup1, u_taup1, v_taup1);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+int psi_solve_streamfunction_3d(psi_ctx* c, const psi_opts* o, const psi_levels* L,
+                                const psi_momentum* m, double grav, double rho_0, double* psi_f,
+                                double* dpsi, double* dpsin, const double* psin,
+                                const double* line_psin, int32_t taum1, int32_t tau,
+                                int32_t taup1, double dt_mom, double AB_eps, double* forc_out,
+                                psi_info* info) {
+    if (!valid_field_ctx(c) || !o || !m || !m->rho || !m->u || !m->v || !m->du || !m->dv ||
+        !m->du_mix || !m->dv_mix || !m->p_hydro || !psi_f || !dpsi || !info || !(rho_0 != 0.0))
+        return PSI_BAD_ARG;
+    if (taum1 < 0 || taum1 > 2 || tau < 0 || tau > 2 || taup1 < 0 || taup1 > 2)
+        return fail(c, PSI_BAD_ARG, "time level out of range%s");
+    int rc = levels_check(c, L);
+    if (rc) return rc;
+    const size_t V = (size_t)L->nz * c->plane;   // one time level of a 3-D field
+    rc = forcing_async(c, L, grav, rho_0, m->rho + tau * V, m->du + tau * V, m->dv + tau * V,
+                       m->du_mix, m->dv_mix, m->p_hydro, c->fpx3, c->fpy3);
+    if (rc) return rc;
+    rc = psi_solve_streamfunction(c, o, c->fpx3, c->fpy3, psi_f, dpsi, dpsin, psin, line_psin,
+                                  taum1, tau, taup1, dt_mom, AB_eps, forc_out, info);
+    if (rc == PSI_HIP_ERROR || rc == PSI_BAD_ARG || rc == PSI_NAN ||
+        rc == PSI_SINGULAR_ISLAND_SYSTEM)
+        return rc;
+    const int solve_rc = rc;
+    rc = velocity_async(c, L, dt_mom, AB_eps, m->u + tau * V, m->v + tau * V, m->du + tau * V,
+                        m->dv + tau * V, m->du + taum1 * V, m->dv + taum1 * V, m->du_mix,
+                        m->dv_mix, psi_f + taup1 * c->plane, m->u + taup1 * V,
+                        m->v + taup1 * V);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return solve_rc;

@@ -15,6 +15,9 @@ the C ABI of libpsisolve.so (include/psisolve.h) on device tensors:
   ``streamfunction_init``, ``solve_streamfunction``
 * ``BarotropicState``          -- psi / dpsi time levels, dpsin, and
   ``time_goes_by`` (py_src/pyOM.py:199-207)
+* ``Levels``, ``MomentumState`` -- vertical grid / 3-D masks and the 3-D model
+  state for the whole routine (``streamfunction_forcing``,
+  ``velocity_update``, ``solve_streamfunction_3d``)
 
 Fields are float64 tensors of shape (ny+4, nx+4) on the GPU: element
 ``a(i, j)`` of pyOM2's Fortran array sits at ``[j+1, i+1]``.
@@ -28,10 +31,13 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PsiError, PsiFields, PsiGrid, PsiInfo, PsiIslands, PsiOpts
+from ._lib import (PsiError, PsiFields, PsiGrid, PsiInfo, PsiIslands, PsiLevels, PsiMomentum,
+                   PsiOpts)
 
-# Fortran default-real literal 0.1 widened to real*8 (main_module.f90:52).
+# Fortran default-real literals widened to real*8 (main_module.f90:18-19, 52).
 AB_EPS_REFERENCE = float(np.float32(0.1))
+GRAV_REFERENCE = float(np.float32(9.81))
+RHO_0_REFERENCE = 1024.0
 
 MODES = {"plain": _lib.PSI_MODE_PLAIN, "mixed": _lib.PSI_MODE_MIXED,
          "learned": _lib.PSI_MODE_LEARNED, "poly": _lib.PSI_MODE_POLY}
@@ -313,6 +319,133 @@ class StreamfunctionSolver:
         state.dpsin[:] = dpsin
         d = info.as_dict()
         return d["iterations"], bool(d["converged"]), d
+
+
+    # -- the 3-D parts of solve_streamfunction --------------------------------
+    def _levels(self, lev: "Levels"):
+        if lev.maskT.shape[-2:] != self.shape:
+            raise ValueError("levels do not match the basin")
+        return C.byref(lev.as_c())
+
+    def streamfunction_forcing(self, lev: "Levels", rho, du, dv, du_mix, dv_mix, p_hydro, fpx,
+                               fpy, grav=GRAV_REFERENCE, rho_0=RHO_0_REFERENCE):
+        """Hydrostatic pressure, its gradient into du/dv (in place) and the
+        depth-integrated forcing fpx, fpy (solve_streamfct.f90:22-48).  rho,
+        du, dv are the tau levels, (nz, ny+4, nx+4) each."""
+        sh3 = (lev.nz,) + self.shape
+        args = [_dev3(a, sh3) for a in (rho, du, dv, du_mix, dv_mix, p_hydro)]
+        self._sync_in()
+        self._check(self.L.psi_streamfunction_forcing(
+            self.ctx, self._levels(lev), grav, rho_0, *args, _dev(fpx, self.shape),
+            _dev(fpy, self.shape)))
+
+    def velocity_update(self, lev: "Levels", dt_mom, u_tau, v_tau, du_tau, dv_tau, du_taum1,
+                        dv_taum1, du_mix, dv_mix, psi_taup1, u_taup1, v_taup1,
+                        AB_eps=AB_EPS_REFERENCE):
+        """u, v at taup1 (solve_streamfct.f90:95-118 without open boundaries)."""
+        sh3 = (lev.nz,) + self.shape
+        a3 = [_dev3(a, sh3) for a in (u_tau, v_tau, du_tau, dv_tau, du_taum1, dv_taum1, du_mix,
+                                      dv_mix)]
+        self._sync_in()
+        self._check(self.L.psi_velocity_update(
+            self.ctx, self._levels(lev), dt_mom, AB_eps, *a3, _dev(psi_taup1, self.shape),
+            _dev3(u_taup1, sh3), _dev3(v_taup1, sh3)))
+
+    def solve_streamfunction_3d(self, state: "BarotropicState", lev: "Levels",
+                                mom: "MomentumState", opts: SolveOptions | None = None,
+                                grav=GRAV_REFERENCE, rho_0=RHO_0_REFERENCE, forc_out=None, **kw):
+        """The whole solve_streamfunction (solve_streamfct.f90:10-120, serial,
+        no open boundaries) on the model state: p_hydro, du/dv(tau) updated,
+        psi/dpsi(taup1), dpsin(tau), u/v(taup1).  Returns
+        (iterations, converged, info) like solve_streamfunction."""
+        opts = opts or SolveOptions(**kw)
+        if mom.nz != lev.nz:
+            raise ValueError("momentum state and levels differ in nz")
+        info = PsiInfo()
+        dpsin = np.ascontiguousarray(state.dpsin)
+        lp = np.ascontiguousarray(state.line_psin, np.float64) if self.nisle > 1 else np.zeros(1)
+        psin_ptr = C.c_void_p(state.psin.data_ptr()) if state.psin is not None else None
+        self._sync_in()
+        rc = self.L.psi_solve_streamfunction_3d(
+            self.ctx, C.byref(opts.as_c()), self._levels(lev), C.byref(mom.as_c(self.shape)),
+            grav, rho_0, C.c_void_p(state.psi.data_ptr()), C.c_void_p(state.dpsi.data_ptr()),
+            _f64p(dpsin), psin_ptr, _f64p(lp), state.taum1, state.tau, state.taup1,
+            state.dt_mom, state.AB_eps,
+            _dev(forc_out, self.shape) if forc_out is not None else None, C.byref(info))
+        self._check(rc, (_lib.PSI_OK, _lib.PSI_NOT_CONVERGED, _lib.PSI_DIVERGED))
+        state.dpsin[:] = dpsin
+        d = info.as_dict()
+        return d["iterations"], bool(d["converged"]), d
+
+
+def _dev3(t: torch.Tensor, shape):
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float64 CUDA tensor")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {tuple(t.shape)}")
+    return C.c_void_p(t.data_ptr())
+
+
+class Levels:
+    """Vertical grid and 3-D masks (main_module.f90:96-98, 108): dzt, dzw on
+    the host, maskT/U/V device tensors (nz, ny+4, nx+4)."""
+
+    def __init__(self, dzt, dzw, maskT, maskU, maskV, device="cuda"):
+        self.dzt = np.ascontiguousarray(dzt, np.float64)
+        self.dzw = np.ascontiguousarray(dzw, np.float64)
+        self.nz = int(self.dzt.shape[0])
+        def t(a):
+            if isinstance(a, torch.Tensor):
+                return a.to(device=device, dtype=torch.float64).contiguous()
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64,
+                                   device=device).contiguous()
+        self.maskT, self.maskU, self.maskV = t(maskT), t(maskU), t(maskV)
+        if self.dzw.shape[0] != self.nz or any(m.shape[0] != self.nz for m in
+                                               (self.maskT, self.maskU, self.maskV)):
+            raise ValueError("dzt, dzw and the masks need nz levels")
+
+    @classmethod
+    def from_column(cls, col, device="cuda"):
+        return cls(col.dzt, col.dzw, col.maskT, col.maskU, col.maskV, device)
+
+    def as_c(self):
+        return PsiLevels(self.nz, 0, self.dzt.ctypes.data, self.dzw.ctypes.data,
+                         self.maskT.data_ptr(), self.maskU.data_ptr(), self.maskV.data_ptr())
+
+
+@dataclasses.dataclass
+class MomentumState:
+    """The 3-D model state solve_streamfunction reads and writes
+    (main_module.f90:127, 143-149, 154) as device tensors: rho, u, v, du, dv
+    with 3 time levels (3, nz, ny+4, nx+4); du_mix, dv_mix, p_hydro (nz, ny+4, nx+4)."""
+    rho: torch.Tensor
+    u: torch.Tensor
+    v: torch.Tensor
+    du: torch.Tensor
+    dv: torch.Tensor
+    du_mix: torch.Tensor
+    dv_mix: torch.Tensor
+    p_hydro: torch.Tensor
+
+    @property
+    def nz(self):
+        return int(self.du_mix.shape[0])
+
+    @classmethod
+    def from_numpy(cls, m, device="cuda"):
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64,
+                                      device=device).contiguous()
+        return cls(t(m.rho), t(m.u), t(m.v), t(m.du), t(m.dv), t(m.du_mix), t(m.dv_mix),
+                   torch.zeros(m.du_mix.shape, dtype=torch.float64, device=device))
+
+    def as_c(self, shape):
+        sh3 = (self.nz,) + tuple(shape)
+        for a in (self.rho, self.u, self.v, self.du, self.dv):
+            _dev3(a, (3,) + sh3)
+        for a in (self.du_mix, self.dv_mix, self.p_hydro):
+            _dev3(a, sh3)
+        return PsiMomentum(*[a.data_ptr() for a in (self.rho, self.u, self.v, self.du, self.dv,
+                                                    self.du_mix, self.dv_mix, self.p_hydro)])
 
 
 @dataclasses.dataclass

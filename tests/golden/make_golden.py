@@ -10,6 +10,11 @@ island boundary lists and directions (streamfunction_init), the psin
 fields and line_psin matrix, the solution, iteration count and converged
 flag of congrad_streamfunction, apply_op of that solution, and the island
 line integrals of a seeded random (u, v) pair.
+
+The ``step_*`` fixtures come from oracle/_ref/ref_step, the reference's
+whole solve_streamfunction on a basin with vertical levels and no islands:
+inputs (levels, masks, the model state at tau/taum1) and outputs (p_hydro,
+du/dv after the pressure gradient, u/v/psi/dpsi at taup1, iterations).
 """
 from __future__ import annotations
 
@@ -66,6 +71,49 @@ CASES = {
 }
 
 
+def cyclic_bridge_basin():
+    """Zonally periodic 28x18 channel whose two walls are joined by a
+    meridional ridge, so the land is one mass (no island system)."""
+    nx, ny = 28, 18
+    ocean = np.ones((ny, nx), bool)
+    ocean[0, :] = ocean[-1, :] = False
+    ocean[:, 9] = False
+    ht = np.full((ny, nx), 3500.0)
+    ht[:, 15:22] = 1200.0
+    grid = basins.make_grid(nx, ny, 30e3, 25e3, cyclic_x=True)
+    rng = np.random.default_rng(21)
+    return basins._assemble("cyclic_bridge", 21, nx, ny, grid, ht, ocean, rng, 8, 3, 1e-10,
+                            cyclic_x=True)
+
+
+STEP_CASES = {
+    "step_24x20_nz5": lambda: (basins.beta_plane_basin(seed=1, nx=24, ny=20), 5, 0),
+    "step_cyclic_28x18_nz4": lambda: (cyclic_bridge_basin(), 4, 1),
+}
+
+
+def make_step(name):
+    b, nz, seed = STEP_CASES[name]()
+    b3, col = basins.column_basin(b, nz)
+    mom = basins.momentum_fields(col, seed)
+    eps, max_iter, dt = 1e-12, 2000, 3600.0
+    r = ref_run.run_step(b3, col, mom, eps, max_iter, dt)
+    out = dict(
+        nx=b3.nx, ny=b3.ny, nz=nz, cyclic_x=int(b3.cyclic_x), eps=eps, max_iter=max_iter,
+        dt_mom=dt, dxt=b3.dxt, dxu=b3.dxu, dyt=b3.dyt, dyu=b3.dyu, cost=b3.cost, cosu=b3.cosu,
+        kbot=b3.kbot, hur=b3.hur, hvr=b3.hvr, dzt=col.dzt, dzw=col.dzw, kbot_levels=col.kbot,
+        maskT=col.maskT, maskU=col.maskU, maskV=col.maskV,
+        rho=mom.rho[1], du_tau=mom.du[1], du_taum1=mom.du[0], dv_tau=mom.dv[1],
+        dv_taum1=mom.dv[0], du_mix=mom.du_mix, dv_mix=mom.dv_mix, u_tau=mom.u[1],
+        v_tau=mom.v[1], psi_tau=mom.psi[1], dpsi_tau=mom.dpsi[1], dpsi_taum1=mom.dpsi[0],
+        out_p_hydro=r["p_hydro"], out_du=r["du"], out_dv=r["dv"], out_u=r["u"], out_v=r["v"],
+        out_psi=r["psi"], out_dpsi=r["dpsi"], iterations=r["iterations"],
+    )
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: iterations={r['iterations']} -> {os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def make(name):
     b, kw = CASES[name]()
     rng = np.random.default_rng(1234)
@@ -96,5 +144,5 @@ def make(name):
 if __name__ == "__main__":
     if not ref_run.build():
         sys.exit("reference sources not available: cannot build oracle/_ref")
-    for n in (sys.argv[1:] or CASES):
-        make(n)
+    for n in (sys.argv[1:] or list(CASES) + list(STEP_CASES)):
+        (make_step if n in STEP_CASES else make)(n)

@@ -1,0 +1,222 @@
+"""GPU parity of the 3-D parts of solve_streamfunction (hydrostatic pressure,
+forcing assembly, velocity update; solve_streamfct.f90:22-48, 95-118) and of
+the whole routine, through the C ABI.
+
+Tolerances: the 3-D kernels keep the reference's operation order with
+floating-point contraction off, so p_hydro, du/dv, fpx/fpy and u/v (given
+psi(taup1)) are compared bit for bit with the oracle, which itself matches
+the reference's Fortran bit for bit.  psi/dpsi come out of the PCG solve and
+carry its tolerance: max|psi - psi_ref| / max|psi_ref| <= 1e-9."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pyom2_amd import basins
+from pyom2_amd._lib import PSI_BAD_ARG, PsiError
+from pyom2_amd.streamfunction import (AB_EPS_REFERENCE as AB, GRAV_REFERENCE as GRAV,
+                                      RHO_0_REFERENCE as RHO_0, BarotropicState, Levels,
+                                      MomentumState, SolveOptions, StreamfunctionSolver,
+                                      islands_trace)
+from tests import fixtures
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def T(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _case(nx, ny, nz, cyclic=False, seed=0, islands=0):
+    b0 = basins.beta_plane_basin(seed=seed, nx=nx, ny=ny, cyclic_x=cyclic, n_islands=islands)
+    b, col = basins.column_basin(b0, nz)
+    return b, col, basins.momentum_fields(col, seed + 100)
+
+
+@pytest.mark.parametrize("nz,cyclic", [(1, False), (3, True), (17, False), (40, True)])
+def test_forcing_matches_oracle_bitwise(nz, cyclic):
+    print(f"\n[forcing nz={nz} cyclic={cyclic}]", flush=True)
+    b, col, m = _case(70, 37, nz, cyclic, seed=nz)
+    s = StreamfunctionSolver.from_basin(b, device=DEV)
+    lev = Levels.from_column(col, DEV)
+    du, dv = T(m.du[1]), T(m.dv[1])
+    p = torch.zeros(col.shape, dtype=torch.float64, device=DEV)
+    fpx, fpy = s.zeros(), s.zeros()
+    s.streamfunction_forcing(lev, T(m.rho[1]), du, dv, T(m.du_mix), T(m.dv_mix), p, fpx, fpy)
+    p_o, du_o, dv_o, fpx_o, fpy_o = O.streamfunction_forcing(b, col, m.rho[1], m.du[1], m.dv[1],
+                                                             m.du_mix, m.dv_mix, GRAV / RHO_0)
+    for got, want, k in ((p, p_o, "p_hydro"), (du, du_o, "du"), (dv, dv_o, "dv"),
+                         (fpx, fpx_o, "fpx"), (fpy, fpy_o, "fpy")):
+        np.testing.assert_array_equal(N(got), want, err_msg=k)
+    s.close()
+
+
+# level counts at and around each levels-per-thread switch (2, 4, 8, 16)
+@pytest.mark.parametrize("nz", [1, 2, 3, 16, 17, 33, 64, 65, 100, 128])
+def test_velocity_matches_oracle_bitwise(nz):
+    print(f"\n[velocity nz={nz}]", flush=True)
+    b, col, m = _case(70, 23, nz, cyclic=nz % 2 == 1, seed=nz)
+    s = StreamfunctionSolver.from_basin(b, device=DEV)
+    lev = Levels.from_column(col, DEV)
+    u = torch.full(col.shape, np.nan, dtype=torch.float64, device=DEV)
+    v = torch.full(col.shape, np.nan, dtype=torch.float64, device=DEV)
+    s.velocity_update(lev, 1800.0, T(m.u[1]), T(m.v[1]), T(m.du[1]), T(m.dv[1]), T(m.du[0]),
+                      T(m.dv[0]), T(m.du_mix), T(m.dv_mix), T(m.psi[2]), u, v)
+    u_o, v_o = O.velocity_update(b, col, 1800.0, AB, m.u[1], m.v[1], m.du[1], m.dv[1], m.du[0],
+                                 m.dv[0], m.du_mix, m.dv_mix, m.psi[2])
+    np.testing.assert_array_equal(N(u), u_o)
+    np.testing.assert_array_equal(N(v), v_o)
+    s.close()
+
+
+def _run_3d(s, b, col, m, st, opts, steps=1):
+    lev = Levels.from_column(col, DEV)
+    ms = MomentumState.from_numpy(m, DEV)
+    out = []
+    for _ in range(steps):
+        its, conv, info = s.solve_streamfunction_3d(st, lev, ms, opts)
+        out.append((its, conv, info))
+    return ms, out
+
+
+@pytest.mark.parametrize("name", fixtures.STEP_NAMES)
+def test_whole_routine_against_reference_fixture(name):
+    """psi_solve_streamfunction_3d on the reference's own whole-routine
+    fixture: p_hydro, du, dv bit for bit; the solve to the PCG tolerance with
+    the reference's iteration count; u, v bit for bit given the solve's psi."""
+    print(f"\n[whole {name}]", flush=True)
+    b, col, m, d = fixtures.load_step(name)
+    isl = islands_trace(b.kbot, b.cyclic_x)
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=DEV)
+    opts = SolveOptions(congr_epsilon=float(d["eps"]), max_iterations=int(d["max_iter"]))
+    psin, line_psin, _ = s.streamfunction_init(opts)
+    st = BarotropicState.zeros(s, float(d["dt_mom"]), psin=psin.contiguous(),
+                               line_psin=line_psin)
+    st.psi.copy_(T(m.psi))
+    st.dpsi.copy_(T(m.dpsi))
+    ms, ((its, conv, info),) = _run_3d(s, b, col, m, st, opts)
+    assert conv
+    assert abs(its - int(d["iterations"])) <= 0.02 * int(d["iterations"]) + 3
+    np.testing.assert_array_equal(N(ms.p_hydro), d["out_p_hydro"])
+    np.testing.assert_array_equal(N(ms.du[1]), d["out_du"])
+    np.testing.assert_array_equal(N(ms.dv[1]), d["out_dv"])
+    assert relerr(N(st.dpsi[2]), d["out_dpsi"]) <= 1e-9
+    assert relerr(N(st.psi[2]), d["out_psi"]) <= 1e-9
+    u_o, v_o = O.velocity_update(b, col, float(d["dt_mom"]), AB, m.u[1], m.v[1], d["out_du"],
+                                 d["out_dv"], m.du[0], m.dv[0], m.du_mix, m.dv_mix,
+                                 N(st.psi[2]))
+    np.testing.assert_array_equal(N(ms.u[2]), u_o)
+    np.testing.assert_array_equal(N(ms.v[2]), v_o)
+    # and close to the reference's own u, v (which saw the reference's psi)
+    assert relerr(N(ms.u[2]), d["out_u"]) <= 1e-6
+    assert relerr(N(ms.v[2]), d["out_v"]) <= 1e-6
+    s.close()
+
+
+def test_whole_routine_with_islands_three_steps():
+    """Three time steps of the whole routine on a basin with two islands
+    against the oracle composition (the island system included)."""
+    print("\n[whole islands]", flush=True)
+    b, col, m = _case(60, 48, 6, seed=4, islands=2)
+    isl = islands_trace(b.kbot)
+    assert isl.nisle == 3
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=DEV)
+    opts = SolveOptions(congr_epsilon=1e-12, max_iterations=3000)
+    psin, line_psin, _ = s.streamfunction_init(opts)
+    init = O.streamfunction_init(b, 1e-12, 3000)
+    dt = 3600.0
+    st = BarotropicState.zeros(s, dt, psin=psin.contiguous(), line_psin=line_psin)
+    st.psi.copy_(T(m.psi))
+    st.dpsi.copy_(T(m.dpsi))
+    lev = Levels.from_column(col, DEV)
+    ms = MomentumState.from_numpy(m, DEV)
+    mo = basins.Momentum(*[a.copy() for a in (m.rho, m.u, m.v, m.du, m.dv, m.du_mix, m.dv_mix,
+                                              m.psi, m.dpsi)])
+    dpsin_o = np.zeros((3, 3))
+    for step in range(3):
+        tm1, t, tp1 = st.taum1, st.tau, st.taup1
+        its, conv, info = s.solve_streamfunction_3d(st, lev, ms, opts)
+        assert conv
+        p_o, r = O.solve_streamfunction(b, init, col, mo, dpsin_o, tm1, t, tp1, dt, AB, GRAV,
+                                        RHO_0, 1e-12, 3000)
+        np.testing.assert_array_equal(N(ms.p_hydro), p_o)
+        np.testing.assert_array_equal(N(ms.du[t]), mo.du[t])
+        assert relerr(N(st.dpsi[tp1]), mo.dpsi[tp1]) <= 1e-9
+        np.testing.assert_allclose(st.dpsin[1:, t], dpsin_o[1:, t], rtol=1e-7)
+        assert relerr(N(st.psi[tp1]), mo.psi[tp1]) <= 1e-8
+        assert relerr(N(ms.u[tp1]), mo.u[tp1]) <= 1e-6
+        # continue both from the GPU's psi so the next step's inputs agree
+        mo.psi[tp1] = N(st.psi[tp1])
+        mo.dpsi[tp1] = N(st.dpsi[tp1])
+        mo.u[tp1] = N(ms.u[tp1])
+        mo.v[tp1] = N(ms.v[tp1])
+        dpsin_o[:, t] = st.dpsin[:, t]
+        # the next step's tendencies: new random du, dv at the new tau level
+        rng = np.random.default_rng(step)
+        mo.du[tp1] = 1e-6 * rng.normal(size=col.shape) * col.maskU
+        mo.dv[tp1] = 1e-6 * rng.normal(size=col.shape) * col.maskV
+        ms.du[tp1].copy_(T(mo.du[tp1]))
+        ms.dv[tp1].copy_(T(mo.dv[tp1]))
+        st.time_goes_by()
+    s.close()
+
+
+def test_levels_argument_errors():
+    """nz outside 1..128 is refused with an error naming nz; nothing runs."""
+    for nz in (0, 129):
+        b0 = basins.beta_plane_basin(seed=0, nx=12, ny=10)
+        if nz:
+            b, col = basins.column_basin(b0, nz)
+            lev = Levels.from_column(col, DEV)
+        else:
+            b, col = basins.column_basin(b0, 1)
+            lev = Levels(np.zeros(0), np.zeros(0), col.maskT[:0], col.maskU[:0], col.maskV[:0],
+                         DEV)
+        s = StreamfunctionSolver.from_basin(b, device=DEV)
+        z = torch.zeros((max(nz, 1),) + b.shape, dtype=torch.float64, device=DEV)[:nz]
+        with pytest.raises(PsiError) as e:
+            s.streamfunction_forcing(lev, z, z.clone(), z.clone(), z, z, z.clone(), s.zeros(),
+                                     s.zeros())
+        assert e.value.code == PSI_BAD_ARG
+        if nz:
+            assert "nz" in str(e.value)
+        with pytest.raises(PsiError):
+            s.velocity_update(lev, 1.0, z, z, z, z, z, z, z, z, s.zeros(), z.clone(), z.clone())
+        s.close()
+
+
+def test_spherical_1024_levels_bitwise():
+    """The kernels at a production-sized plane: 1024^2 spherical basin with
+    24 levels, forcing and velocity update bit for bit with the oracle."""
+    print("\n[levels 1024^2 x 24]", flush=True)
+    b0 = basins.spherical_basin(seed=3, n=1024, n_islands=2)
+    b, col = basins.column_basin(b0, 24)
+    m = basins.momentum_fields(col, 9)
+    s = StreamfunctionSolver.from_basin(b, device=DEV)
+    lev = Levels.from_column(col, DEV)
+    du, dv = T(m.du[1]), T(m.dv[1])
+    p = torch.zeros(col.shape, dtype=torch.float64, device=DEV)
+    fpx, fpy = s.zeros(), s.zeros()
+    s.streamfunction_forcing(lev, T(m.rho[1]), du, dv, T(m.du_mix), T(m.dv_mix), p, fpx, fpy)
+    p_o, du_o, dv_o, fpx_o, fpy_o = O.streamfunction_forcing(b, col, m.rho[1], m.du[1], m.dv[1],
+                                                             m.du_mix, m.dv_mix, GRAV / RHO_0)
+    assert np.array_equal(N(p), p_o) and np.array_equal(N(du), du_o)
+    assert np.array_equal(N(fpx), fpx_o) and np.array_equal(N(fpy), fpy_o)
+    del p, p_o, du_o, dv_o
+    u = torch.empty(col.shape, dtype=torch.float64, device=DEV)
+    v = torch.empty(col.shape, dtype=torch.float64, device=DEV)
+    s.velocity_update(lev, 1800.0, T(m.u[1]), T(m.v[1]), du, dv, T(m.du[0]), T(m.dv[0]),
+                      T(m.du_mix), T(m.dv_mix), T(m.psi[2]), u, v)
+    u_o, v_o = O.velocity_update(b, col, 1800.0, AB, m.u[1], m.v[1], N(du), N(dv), m.du[0],
+                                 m.dv[0], m.du_mix, m.dv_mix, m.psi[2])
+    assert np.array_equal(N(u), u_o) and np.array_equal(N(v), v_o)
+    s.close()

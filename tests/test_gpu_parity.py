@@ -222,29 +222,8 @@ def test_mixed_iteration_cap_keeps_the_iterate():
 
 def _oracle_step(b, init, fpx, fpy, psi, dpsi, dpsin, taum1, tau, taup1, dt, ab):
     """solve_streamfunction's 2-D body (solve_streamfct.f90:49-94) on the oracle."""
-    nx, ny = b.nx, b.ny
-    forc = np.zeros(b.shape)
-    j = np.arange(1, ny + 1)[:, None]
-    i = np.arange(1, nx + 1)[None, :]
-    forc[j + 1, i + 1] = ((fpy[j + 1, i + 2] - fpy[j + 1, i + 1]) / (b.cosu[j + 1] * b.dxu[i + 1]) -
-                          (b.cost[j + 2] * fpx[j + 2, i + 1] - b.cost[j + 1] * fpx[j + 1, i + 1]) /
-                          (b.cosu[j + 1] * b.dyu[j + 1]))
-    dpsi[taup1] = 2 * dpsi[tau] - dpsi[taum1]
-    r = O.congrad(nx, ny, b.cyclic_x, init.cf, init.Z, forc, dpsi[taup1], 1e-12, 3000)
-    dpsi[taup1] = O.setcyclic(nx, ny, b.cyclic_x, r.sol)
-    n = init.islands.nisle
-    if n > 1:
-        lf = np.array([O.line_integral(b, init.islands, k, fpx, fpy) for k in range(n)])
-        u, v = O.psi_velocity(b, dpsi[taup1])
-        l0 = np.array([O.line_integral(b, init.islands, k, u, v) for k in range(n)])
-        x, info = O.dense_solve(init.line_psin[1:, 1:], (lf - l0)[1:])
-        assert info == 0
-        dpsin[1:, tau] = x
-    psi[taup1] = psi[tau] + dt * ((1.5 + ab) * dpsi[taup1] - (0.5 + ab) * dpsi[tau])
-    for k in range(1, n):
-        psi[taup1] = psi[taup1] + dt * ((1.5 + ab) * dpsin[k, tau] -
-                                        (0.5 + ab) * dpsin[k, taum1]) * init.psin[k]
-    return r.iterations
+    return O.solve_streamfunction_2d(b, init, fpx, fpy, psi, dpsi, dpsin, taum1, tau, taup1, dt,
+                                     ab, 1e-12, 3000).iterations
 
 
 def test_solve_streamfunction_dropin_matches_oracle():

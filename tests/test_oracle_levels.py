@@ -1,0 +1,193 @@
+"""Pins the oracle's restatement of the 3-D parts of solve_streamfunction
+(hydrostatic pressure, forcing assembly, velocity update;
+solve_streamfct.f90:22-48, 95-118) before the HIP kernels are checked
+against it: the reference's whole routine (flang build of the pyOM2
+sources, golden fixtures and, where the build exists, fresh runs), a closed
+form, and invariants chosen so that a wrong sign, index or weight fails."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import ref_run
+from pyom2_amd import basins
+from pyom2_amd.streamfunction import AB_EPS_REFERENCE as AB
+from pyom2_amd.streamfunction import GRAV_REFERENCE as GRAV
+from pyom2_amd.streamfunction import RHO_0_REFERENCE as RHO_0
+from tests import fixtures
+
+OUTS = ("p_hydro", "du", "dv", "dpsi", "psi", "u", "v")
+
+
+def _oracle_whole(b, col, mom, dt, eps=1e-12, max_iter=2000):
+    init = O.streamfunction_init(b, eps, max_iter)
+    dpsin = np.zeros((max(init.islands.nisle, 1), 3))
+    p, r = O.solve_streamfunction(b, init, col, mom, dpsin, 0, 1, 2, dt, AB, GRAV, RHO_0, eps,
+                                  max_iter)
+    return dict(p_hydro=p, du=mom.du[1], dv=mom.dv[1], dpsi=mom.dpsi[2], psi=mom.psi[2],
+                u=mom.u[2], v=mom.v[2]), r
+
+
+@pytest.mark.parametrize("name", fixtures.STEP_NAMES)
+def test_whole_routine_matches_reference_bitwise(name):
+    """Oracle composition == the reference's solve_streamfunction, bit for bit,
+    including the congrad iteration count."""
+    b, col, mom, d = fixtures.load_step(name)
+    out, r = _oracle_whole(b, col, mom, float(d["dt_mom"]))
+    assert r.iterations == int(d["iterations"])
+    for k in OUTS:
+        np.testing.assert_array_equal(out[k], d["out_" + k], err_msg=k)
+
+
+@pytest.mark.skipif(not ref_run.step_available(), reason="reference build not present")
+@pytest.mark.parametrize("nz,seed,cyclic", [(1, 3, False), (7, 4, False), (3, 5, True)])
+def test_fresh_cases_match_reference_bitwise(nz, seed, cyclic):
+    """New seeded basins through the reference build (where it exists)."""
+    if cyclic:
+        b = basins.beta_plane_basin(seed=seed, nx=22, ny=14, cyclic_x=True)
+        b.kbot[:, 8] = 0          # join the channel walls: one land mass, no islands
+        b = _rebuild(b)
+    else:
+        b = basins.beta_plane_basin(seed=seed, nx=26, ny=18)
+    b3, col = basins.column_basin(b, nz)
+    mom = basins.momentum_fields(col, seed)
+    ref = ref_run.run_step(b3, col, mom, 1e-12, 2000, 1800.0)
+    out, r = _oracle_whole(b3, col, mom, 1800.0)
+    assert r.iterations == ref["iterations"]
+    for k in OUTS:
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+def _rebuild(b):
+    """Basin b with its masks and depths re-derived from a modified kbot."""
+    ocean = b.kbot[2:-2, 2:-2] != 0
+    kbot, ht, hu, hv, hur, hvr, mT, mU, mV = basins.topo_fields(
+        np.where(ocean, np.maximum(b.ht[2:-2, 2:-2], 200.0), 0.0), ocean, b.cyclic_x)
+    return dataclasses.replace(b, kbot=kbot, ht=ht, hu=hu, hv=hv, hur=hur, hvr=hvr, maskT=mT,
+                               maskU=mU, maskV=mV)
+
+
+def _flat_spherical(n=40, nz=6):
+    """Spherical basin (cost varies with latitude) with a flat bottom: every
+    ocean column holds all nz levels."""
+    b = basins.spherical_basin(seed=1, n=n, n_islands=0, dlon=1.0)
+    b.ht[:] = np.where(b.kbot != 0, 4000.0, 0.0)
+    return basins.column_basin(b, nz, depth=4000.0)
+
+
+def test_hydrostatic_closed_form():
+    """Horizontally and vertically constant rho: p(nz) = rho fxa dzw(nz)/2,
+    p(k) = p(k+1) + rho fxa dzw(k) on every wet level."""
+    b, col = _flat_spherical()
+    rho = 1027.5 * col.maskT
+    z = np.zeros(col.shape)
+    p, *_ = O.streamfunction_forcing(b, col, rho, z, z, z, z, GRAV / RHO_0)
+    fxa = GRAV / RHO_0
+    want = np.zeros(col.nz)
+    want[-1] = 0.5 * 1027.5 * fxa * col.dzw[-1]
+    for k in range(col.nz - 2, -1, -1):
+        want[k] = want[k + 1] + 1027.5 * fxa * col.dzw[k]
+    wet = col.maskT[:, 10, 10] == 1
+    assert wet.all()
+    np.testing.assert_allclose(p[:, 10, 10], want, rtol=1e-14)
+    assert (p[:, col.maskT[0] == 0] == 0).all()
+
+
+def test_pressure_gradient_drops_out_of_the_curl():
+    """With a flat bottom the depth-integrated pressure gradient is a
+    discrete gradient, whose curl vanishes: forc = 0 to round-off wherever
+    the four u/v points around a psi point are wet.  A gradient with cost(j+1)
+    in place of cost(j) -- a plausible index slip -- does not pass."""
+    b, col = _flat_spherical()
+    rng = np.random.default_rng(5)
+    ny4, nx4 = b.shape
+    jj, ii = np.meshgrid(np.arange(ny4), np.arange(nx4), indexing="ij")
+    rho = np.stack([1026.0 + 0.3 * np.sin(0.2 * ii + 0.1 * k) * np.cos(0.15 * jj)
+                    + 0.01 * rng.normal(size=b.shape) for k in range(col.nz)]) * col.maskT
+    z = np.zeros(col.shape)
+    p, du, dv, fpx, fpy = O.streamfunction_forcing(b, col, rho, z, z, z, z, GRAV / RHO_0)
+    forc = O.curl_forcing(b, fpx, fpy)
+    mU, mV = col.maskU[-1], col.maskV[-1]
+    ok = np.zeros(b.shape, bool)
+    ok[2:-2, 2:-2] = ((mU[2:-2, 2:-2] == 1) & (mU[3:-1, 2:-2] == 1) &
+                      (mV[2:-2, 2:-2] == 1) & (mV[2:-2, 3:-1] == 1))
+    assert ok.sum() > 500
+    term = np.zeros(b.shape)
+    term[2:-2, 2:-2] = np.abs(fpy[2:-2, 3:-1] - fpy[2:-2, 2:-2]) / (b.cosu[2:-2, None] *
+                                                                     b.dxu[None, 3:-1])
+    scale = term[ok].max()
+    assert scale > 0
+    assert np.abs(forc[ok]).max() <= 1e-10 * scale
+    # the planted slip: cost(j+1) in the x gradient
+    wrong = fpx.copy()
+    acc = np.zeros(b.shape)
+    for k in range(col.nz):
+        g = np.zeros(b.shape)
+        g[2:-2, 2:-2] = -(p[k][2:-2, 3:-1] - p[k][2:-2, 2:-2]) / (b.dxu[None, 2:-2] *
+                                                                  b.cost[3:-1, None])
+        acc = acc + g * col.maskU[k] * col.dzt[k]
+    wrong[2:-2, 2:-2] = (acc * b.hur)[2:-2, 2:-2]
+    assert np.abs(O.curl_forcing(b, wrong, fpy)[ok]).max() > 1e-6 * scale
+
+
+def test_forcing_of_a_uniform_tendency():
+    """du = c maskU on every level, no pressure gradient (rho uniform in the
+    horizontal), no mixing: fpx = c hur sum_k maskU dzt = c on wet u points."""
+    b0 = basins.beta_plane_basin(seed=2, nx=30, ny=22)
+    b, col = basins.column_basin(b0, 7)
+    rho = (1025.0 + np.linspace(2, 0, col.nz))[:, None, None] * col.maskT
+    z = np.zeros(col.shape)
+    c, e = 3.25e-6, -1.5e-6
+    p, du, dv, fpx, fpy = O.streamfunction_forcing(b, col, rho, c * col.maskU, e * col.maskV, z, z,
+                                                   GRAV / RHO_0)
+    np.testing.assert_array_equal(du, c * col.maskU)     # no gradient where maskU = 1
+    wet = b.hur != 0
+    np.testing.assert_allclose(fpx[wet], c, rtol=1e-14)
+    np.testing.assert_allclose(fpy[b.hvr != 0], e, rtol=1e-14)
+
+
+def test_velocity_depth_integral_is_the_barotropic_velocity():
+    """After the update, sum_k u maskU dzt = -(psi(i,j)-psi(i,j-1))/dyt(j) and
+    sum_k v maskV dzt = (psi(i,j)-psi(i-1,j))/(cosu(j) dxt(i)) on interior wet
+    points (the baroclinic part has zero depth mean, hur = 1/hu), and 0 on
+    the halo.  Pins the AB step's masking, the mean removal and the signs
+    and indices of the barotropic mode."""
+    b0 = basins.spherical_basin(seed=4, n=36, n_islands=0, dlon=1.0)
+    b, col = basins.column_basin(b0, 9)
+    mom = basins.momentum_fields(col, 6)
+    u, v = O.velocity_update(b, col, 1200.0, AB, mom.u[1], mom.v[1], mom.du[1], mom.dv[1],
+                             mom.du[0], mom.dv[0], mom.du_mix, mom.dv_mix, mom.psi[2])
+    su = (u * col.maskU * col.dzt[:, None, None]).sum(axis=0)
+    sv = (v * col.maskV * col.dzt[:, None, None]).sum(axis=0)
+    psi = mom.psi[2]
+    want_u = np.zeros(b.shape)
+    want_v = np.zeros(b.shape)
+    want_u[2:-2, 2:-2] = -(psi[2:-2, 2:-2] - psi[1:-3, 2:-2]) / b.dyt[2:-2, None]
+    want_v[2:-2, 2:-2] = (psi[2:-2, 2:-2] - psi[2:-2, 1:-3]) / (b.cosu[2:-2, None] *
+                                                                b.dxt[None, 2:-2])
+    want_u *= b.hur != 0
+    want_v *= b.hvr != 0
+    np.testing.assert_allclose(su, want_u, rtol=1e-10, atol=1e-12 * np.abs(want_u).max())
+    np.testing.assert_allclose(sv, want_v, rtol=1e-10, atol=1e-12 * np.abs(want_v).max())
+    # a depth-uniform increment is pure depth mean: it is removed entirely
+    z = np.zeros(col.shape)
+    u2, _ = O.velocity_update(b, col, 1200.0, AB, z, z, col.maskU, z, z, z, z, z,
+                              np.zeros(b.shape))
+    assert np.abs(u2).max() <= 1e-12 * 1200.0 * (1.5 + AB)
+
+
+def test_velocity_adams_bashforth_weights():
+    """With hur = hvr = 0 (no mean removal, no barotropic mode) the update is
+    the Adams-Bashforth step alone: u(tau) + dt (du_mix + (1.5+eps) du(tau)
+    - (0.5+eps) du(taum1)) maskU, with eps = 0.1 as a default real."""
+    b0 = basins.beta_plane_basin(seed=2, nx=20, ny=16)
+    b, col = basins.column_basin(b0, 4)
+    b = dataclasses.replace(b, hur=np.zeros(b.shape), hvr=np.zeros(b.shape))
+    z = np.zeros(col.shape)
+    one = np.ones(col.shape)
+    dt = 900.0
+    u, v = O.velocity_update(b, col, dt, AB, z, z, one, z, z, one, z, z, np.ones(b.shape))
+    np.testing.assert_array_equal(u, dt * (1.5 + AB) * col.maskU)
+    np.testing.assert_array_equal(v, -(dt * (0.5 + AB)) * col.maskV)
+    assert AB == float(np.float32(0.1)) and AB != 0.1
