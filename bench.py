@@ -93,6 +93,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def calibration_copy(mib, dev):
+    """Three device copies of `mib` MiB: a kernel of known traffic that
+    calibrates the profiler's byte counters."""
+    if not mib:
+        return
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] calibration copy: 3 x {mib} MiB")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -109,14 +123,7 @@ def main():
     from pyom2_amd.streamfunction import SolveOptions, StreamfunctionSolver, islands_trace
 
     dev = torch.device("cuda:0")
-    if args.copy_mib:
-        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"[bench] calibration copy: 3 x {args.copy_mib} MiB")
-        del src, dst
+    calibration_copy(args.copy_mib, dev)
     t = time.time()
     # the 2048^2 basin's domain (102.4 deg of latitude about the equator) at
     # any resolution: spacing 0.05 deg * 2048 / n
@@ -456,6 +463,7 @@ def step_main(args):
                                           StreamfunctionSolver, islands_trace)
 
     dev = torch.device("cuda:0")
+    calibration_copy(args.copy_mib, dev)
     nz = args.nz
     t = time.time()
     b0 = basins.spherical_basin(seed=0, n=args.n, n_islands=2, dlon=0.05 * 2048 / args.n)

@@ -49,6 +49,8 @@ struct psi_ctx {
     // layer thicknesses (dzt, dzw) as last uploaded
     double *fpx3 = nullptr, *fpy3 = nullptr, *dz = nullptr;
     std::vector<double> h_dz;
+    int vl_nz = 0;
+    VelocityLaunch vl;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *pf32 = nullptr;   // flexible CG
     double* fcg = nullptr;        // flexible-CG device scalars
     int fcg_restart = 1;
@@ -1333,8 +1335,8 @@ static int forcing_async(psi_ctx* c, const psi_levels* L, double grav, double rh
     const int nz = L->nz;
     hipLaunchKernelGGL(k_hydrostatic, dim3(nblk(c->plane)), dim3(256), 0, c->stream, c->plane, nz,
                        fxa, c->dz + nz, rho, L->maskT, p_hydro);
-    const dim3 grid((c->g.pitch + 63) / 64, (c->g.ny + 4 + LV_TY - 1) / LV_TY);
-    hipLaunchKernelGGL(k_streamfunction_forcing, grid, dim3(64, LV_TY), 0, c->stream, c->g, nz,
+    hipLaunchKernelGGL(k_streamfunction_forcing, dim3(nblk(c->plane)), dim3(256), 0, c->stream,
+                       c->g, nz,
                        c->dxu, c->dyu, c->cost, c->dz, L->maskU, L->maskV, c->hur, c->hvr,
                        p_hydro, du, dv, du_mix, dv_mix, fpx, fpy);
     if (c->g.cyclic) {
@@ -1352,14 +1354,16 @@ static int velocity_async(psi_ctx* c, const psi_levels* L, double dt_mom, double
                           const double* du_mix, const double* dv_mix, const double* psi_f,
                           double* u_out, double* v_out) {
     const double w1 = 1.5 + AB_eps, w0 = 0.5 + AB_eps;   // solve_streamfct.f90:96-97
-    const int nz = L->nz, lpt = velocity_lpt(nz);
-    const auto kern = velocity_kernel(lpt);
-    if (!kern) return fail(c, PSI_BAD_ARG, "levels: nz out of range%s");
-    const dim3 grid((c->g.pitch + 63) / 64, c->g.ny + 4);
-    const dim3 block(64, (nz + lpt - 1) / lpt);
-    hipLaunchKernelGGL(kern, grid, block, 0, c->stream, c->g, nz, dt_mom, w1, w0, c->dxt, c->dyt,
-                       c->cosu, c->dz, L->maskU, L->maskV, c->hur, c->hvr, u_tau, v_tau, du_tau,
-                       dv_tau, du_taum1, dv_taum1, du_mix, dv_mix, psi_f, u_out, v_out);
+    const int nz = L->nz;
+    if (c->vl_nz != nz) {   // launch shape per level count, kept until nz changes
+        c->vl = velocity_launch(nz);
+        c->vl_nz = nz;
+    }
+    if (!c->vl.kern) return fail(c, PSI_BAD_ARG, "levels: nz out of range%s");
+    hipLaunchKernelGGL(c->vl.kern, dim3((unsigned)((c->plane + 63) / 64)), dim3(64, c->vl.nw), 0,
+                       c->stream, c->g, nz, dt_mom, w1, w0, c->dxt, c->dyt, c->cosu, c->dz,
+                       L->maskU, L->maskV, c->hur, c->hvr, u_tau, v_tau, du_tau, dv_tau,
+                       du_taum1, dv_taum1, du_mix, dv_mix, psi_f, u_out, v_out);
     return check_launch(c);
 }
 

@@ -43,13 +43,14 @@ __global__ void __launch_bounds__(256) k_hydrostatic(size_t plane, int nz, doubl
 
 // Pressure gradient into du/dv (interior, :30-35) and the depth-integrated
 // forcing fpx = hur sum_k (du+du_mix) maskU dzt(k), fpy likewise (:37-42),
-// bottom level first as the reference sums.  2-D tiles of 64 x LV_TY
-// columns, so the p(i,j+1) neighbour is mostly read by the same workgroup.
+// bottom level first as the reference sums.  One thread per column over the
+// flat plane, so every workgroup reads and writes whole 128-byte lines (rows
+// of nx+4 doubles are not line-aligned); the p(i,j+1) neighbours sit one row
+// (about 8 workgroups of 256) further on, which the round-robin dispatch
+// places on the same XCD shortly after, so they come from its L2.
 // 72 bytes per point and level (p; du, dv read and written; du_mix, dv_mix,
-// maskU, maskV) plus 16 per column (fpx, fpy out; hur, hvr in).
-constexpr int LV_TY = 4;
-
-__global__ void __launch_bounds__(64 * LV_TY) k_streamfunction_forcing(
+// maskU, maskV) plus 32 per column (fpx, fpy out; hur, hvr in).
+__global__ void __launch_bounds__(256) k_streamfunction_forcing(
     Geom g, int nz, const double* __restrict__ dxu, const double* __restrict__ dyu,
     const double* __restrict__ cost, const double* __restrict__ dzt,
     const double* __restrict__ maskU, const double* __restrict__ maskV,
@@ -58,11 +59,10 @@ __global__ void __launch_bounds__(64 * LV_TY) k_streamfunction_forcing(
     const double* __restrict__ du_mix, const double* __restrict__ dv_mix,
     double* __restrict__ fpx, double* __restrict__ fpy) {
 #pragma clang fp contract(off)
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    const int r = blockIdx.y * LV_TY + threadIdx.y;
-    if (c >= g.pitch || r >= g.ny + 4) return;
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
-    const size_t q = (size_t)r * g.pitch + c;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= plane) return;
+    const int r = (int)(q / g.pitch), c = (int)(q - (size_t)r * g.pitch);
     const bool inner = r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1;
     const double denx = inner ? dxu[c] * cost[r] : 1.0;
     const double deny = inner ? dyu[r] : 1.0;
@@ -91,15 +91,16 @@ __global__ void __launch_bounds__(64 * LV_TY) k_streamfunction_forcing(
 //   u' -= (sum_k u' maskU dzt) maskU hur                             (:104-112)
 //   u' -= maskU (psi(i,j)-psi(i,j-1))/dyt(j) hur     on the interior (:114-119)
 // and v likewise.  The depth sum needs every level of u' before any level
-// can be finished, so a workgroup holds one row segment of 64 columns with
-// its levels split over NW waves, LPT levels per thread kept in registers:
+// can be finished, so a workgroup holds 64 consecutive columns of the flat
+// plane (whole 128-byte lines) with their levels split over NW waves, LPT
+// levels per thread kept in registers:
 // u' is written once, never re-read.  The sum runs through the waves in
 // level order (wave w adds its levels to the running sum left by wave w-1),
 // which keeps the reference's summation order.  96 bytes per point and
 // level (u, v, du, dv at tau; du, dv at taum1; du_mix, dv_mix, maskU, maskV
 // in; u, v out).
 template <int LPT>
-__global__ void __launch_bounds__(512) k_velocity_update(
+__global__ void __launch_bounds__(1024) k_velocity_update(
     Geom g, int nz, double dt, double w1, double w0, const double* __restrict__ dxt,
     const double* __restrict__ dyt, const double* __restrict__ cosu,
     const double* __restrict__ dzt, const double* __restrict__ maskU,
@@ -113,10 +114,11 @@ __global__ void __launch_bounds__(512) k_velocity_update(
 #pragma clang fp contract(off)
     __shared__ double sum_u[64], sum_v[64];
     const int lane = threadIdx.x, w = threadIdx.y, nw = blockDim.y;
-    const int c = blockIdx.x * 64 + lane, r = blockIdx.y;
-    const bool col = c < g.pitch;
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
-    const size_t q = (size_t)r * g.pitch + (col ? c : 0);
+    const size_t q0 = (size_t)blockIdx.x * 64 + lane;
+    const bool col = q0 < plane;
+    const size_t q = col ? q0 : 0;
+    const int r = (int)(q / g.pitch), c = (int)(q - (size_t)r * g.pitch);
     const int k0 = w * LPT;
     double un[LPT], vn[LPT], mu[LPT], mv[LPT];
 #pragma unroll
@@ -174,25 +176,33 @@ __global__ void __launch_bounds__(512) k_velocity_update(
     }
 }
 
-// Levels per thread of k_velocity_update for nz levels: the smallest LPT in
-// {2, 4, 8, 16} that needs at most 8 waves (512 threads).  Returns 0 when
-// nz exceeds LV_MAX_NZ.
+// Launch shape of k_velocity_update for nz levels: among LPT in {2, 4, 8}
+// (NW = ceil(nz/LPT) waves, at most 16), the one with the most waves
+// resident per CU, from the occupancy calculator (more waves in flight hide more of the HBM latency;
+// measured at 2048^2: nz 40 -> LPT 4 x 10 waves, nz 60 -> LPT 8 x 8).  Ties
+// go to the smaller LPT.  kern is null when nz is outside 1..LV_MAX_NZ.
 constexpr int LV_MAX_NZ = 128;
-inline int velocity_lpt(int nz) {
-    for (int lpt = 2; lpt <= 16; lpt *= 2)
-        if ((nz + lpt - 1) / lpt <= 8) return lpt;
-    return 0;
-}
-
 using VelocityKernel = decltype(&k_velocity_update<2>);
-inline VelocityKernel velocity_kernel(int lpt) {
-    switch (lpt) {
-    case 2: return k_velocity_update<2>;
-    case 4: return k_velocity_update<4>;
-    case 8: return k_velocity_update<8>;
-    case 16: return k_velocity_update<16>;
-    default: return nullptr;
+struct VelocityLaunch {
+    VelocityKernel kern = nullptr;
+    int lpt = 0, nw = 0, resident = 0;
+};
+
+inline VelocityLaunch velocity_launch(int nz) {
+    VelocityLaunch best;
+    if (nz < 1 || nz > LV_MAX_NZ) return best;
+    const VelocityKernel kerns[3] = {k_velocity_update<2>, k_velocity_update<4>,
+                                     k_velocity_update<8>};
+    for (int i = 0; i < 3; ++i) {
+        const int lpt = 2 << i, nw = (nz + lpt - 1) / lpt;
+        if (nw > 16) continue;
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kerns[i], 64 * nw, 0) !=
+            hipSuccess)
+            occ = 1;
+        if (occ * nw > best.resident) best = {kerns[i], lpt, nw, occ * nw};
     }
+    return best;
 }
 
 }  // namespace psi

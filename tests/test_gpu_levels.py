@@ -60,7 +60,7 @@ def test_forcing_matches_oracle_bitwise(nz, cyclic):
     s.close()
 
 
-# level counts at and around each levels-per-thread switch (2, 4, 8, 16)
+# level counts covering every launch shape (2, 4 or 8 levels per thread, 1 to 16 waves)
 @pytest.mark.parametrize("nz", [1, 2, 3, 16, 17, 33, 64, 65, 100, 128])
 def test_velocity_matches_oracle_bitwise(nz):
     print(f"\n[velocity nz={nz}]", flush=True)
