@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--modes", default="learned,poly,mixed,plain",
+    ap.add_argument("--modes", default="learned,mg,poly,mixed,plain",
                     help="first one is the headline")
     ap.add_argument("--weights", default="", help="learned preconditioner weights (.npz)")
     ap.add_argument("--reprobe-every", type=int, default=50)
@@ -141,7 +141,7 @@ def main():
     for mode in modes:
         # the learned path falls back to the fastest classical preconditioner
         # among the paths measured here
-        fallback = "poly" if "poly" in modes else "mixed"
+        fallback = next((m for m in ("mg", "poly", "mixed") if m in modes), "mixed")
         opts = SolveOptions(mode=fallback if mode == "learned" else mode, criterion="relres",
                             relres_target=args.relres, max_iterations=200000,
                             check_every=args.check_every, inner_reduction=args.inner_reduction,

@@ -45,8 +45,12 @@ typedef enum psi_mode {
     PSI_MODE_PLAIN = 0,    /* FP64 Jacobi-PCG */
     PSI_MODE_MIXED = 1,    /* FP32 inner Jacobi-PCG under FP64 iterative refinement */
     PSI_MODE_LEARNED = 2,  /* flexible CG with the learned preconditioner (FP32 inner) */
-    PSI_MODE_POLY = 3      /* as MIXED, preconditioned by a Chebyshev polynomial of degree
+    PSI_MODE_POLY = 3,     /* as MIXED, preconditioned by a Chebyshev polynomial of degree
                               poly_degree in Z A (Z the Jacobi inverse) times Z */
+    PSI_MODE_MG = 4        /* FP64 flexible CG preconditioned by an FP32 aggregation-multigrid
+                              V-cycle (psi_mg_setup first); PSI_CRIT_RELRES only.  poly_degree
+                              = Chebyshev smoothing steps per level (<= 0: 3), poly_lower = the
+                              smoother's interval [2 poly_lower, 2] (<= 0: 1/15) */
 } psi_mode;
 
 typedef enum psi_criterion {
@@ -264,6 +268,17 @@ int psi_solve_streamfunction_3d(psi_ctx *ctx, const psi_opts *opts, const psi_le
                                 const double *line_psin, int32_t taum1, int32_t tau,
                                 int32_t taup1, double dt_mom, double AB_eps, double *forc_out,
                                 psi_info *info);
+
+/* ---- multigrid preconditioner (no counterpart in the reference) ------------
+ * Builds the aggregation-multigrid hierarchy of the context's operator on the
+ * host (2 x 2 Galerkin coarsening of the area-weighted operator down to at
+ * most 1024 active cells, whose dense inverse is formed) and allocates its
+ * device memory.  Needed once before psi_congrad with PSI_MODE_MG; undone by
+ * psi_ctx_update_coeffs.  psi_mg_levels reports the levels (grid size and
+ * active cells per level, coarsest last) into caller arrays of cap entries. */
+int psi_mg_setup(psi_ctx *ctx);
+int psi_mg_levels(const psi_ctx *ctx, int32_t *nlevels, int32_t *nx, int32_t *ny,
+                  int32_t *nactive, int32_t cap);
 
 /* ---- flexible CG with the learned preconditioner ---------------------------
  * No counterpart in the reference.  The host applies the network of psinn.h

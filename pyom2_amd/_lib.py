@@ -26,6 +26,7 @@ PSI_MODE_PLAIN = 0
 PSI_MODE_MIXED = 1
 PSI_MODE_LEARNED = 2
 PSI_MODE_POLY = 3
+PSI_MODE_MG = 4
 PSI_CRIT_REFERENCE = 0
 PSI_CRIT_RELRES = 1
 
@@ -149,6 +150,8 @@ SIGNATURES = [
     ("psi_fcg_flush", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     ("psi_fcg_scalars", C.c_int, [C.c_void_p, _dp]),
     ("psi_poly_coefficients", C.c_int, [C.c_int32, C.c_double, C.POINTER(C.c_float)]),
+    ("psi_mg_setup", C.c_int, [C.c_void_p]),
+    ("psi_mg_levels", C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, C.c_int32]),
     ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),

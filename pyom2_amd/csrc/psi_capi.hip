@@ -19,6 +19,7 @@
 #include "psi_kernels.hip"
 #include "psi_poly.hip"
 #include "psi_levels.hip"
+#include "psi_mg.hip"
 
 using namespace psi;
 
@@ -51,6 +52,7 @@ struct psi_ctx {
     std::vector<double> h_dz;
     int vl_nz = 0;
     VelocityLaunch vl;
+    MgState* mg = nullptr;           // multigrid hierarchy (psi_mg_setup)
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *pf32 = nullptr;   // flexible CG
     double* fcg = nullptr;        // flexible-CG device scalars
     int fcg_restart = 1;
@@ -780,6 +782,337 @@ int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, 
 
 bool valid_field_ctx(psi_ctx* c) { return c && c->E; }
 
+// ---- multigrid-preconditioned flexible CG ----------------------------------
+void mg_release(psi_ctx* c) {
+    MgState* m = c->mg;
+    if (!m) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (m->gs) (void)hipStreamSynchronize(m->gs);
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    for (void* p : m->allocs) (void)hipFree(p);
+    if (m->h_S) (void)hipHostFree(m->h_S);
+    if (m->e_in) (void)hipEventDestroy(m->e_in);
+    if (m->e_out) (void)hipEventDestroy(m->e_out);
+    if (m->gs) (void)hipStreamDestroy(m->gs);
+    delete m;
+    c->mg = nullptr;
+}
+
+template <class T>
+int mg_alloc(psi_ctx* c, MgState* m, T** p, size_t n) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, n * sizeof(T) + 64);
+    if (e != hipSuccess) return fail(c, PSI_HIP_ERROR, "hipMalloc: %s", hipGetErrorString(e));
+    m->allocs.push_back(v);
+    m->bytes += (int64_t)(n * sizeof(T) + 64);
+    e = hipMemsetAsync(v, 0, n * sizeof(T) + 64, c->stream);
+    if (e != hipSuccess) return fail(c, PSI_HIP_ERROR, "hipMemset: %s", hipGetErrorString(e));
+    *p = (T*)v;
+    return PSI_OK;
+}
+
+// Level 0 of the hierarchy from the stored operator: S = W A over active
+// points (W the psi-cell area), couplings to inactive points dropped.
+MgHostLev mg_level0(psi_ctx* c, const std::vector<double>& E, const std::vector<double>& N,
+                    const std::vector<double>& cC, const std::vector<uint8_t>& act,
+                    double& w_mean) {
+    const Geom& g = c->g;
+    MgHostLev L;
+    L.nx = g.nx;
+    L.ny = g.ny;
+    L.cyc = g.cyclic;
+    const size_t n = (size_t)g.nx * g.ny;
+    L.E.assign(n, 0.0);
+    L.N.assign(n, 0.0);
+    L.C.assign(n, 1.0);
+    L.act.assign(n, 0);
+    double wsum = 0.0;
+    auto o_of = [&](int i, int j) { return (size_t)(j + 2) * g.pitch + (size_t)(i + 2); };
+    for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+            const size_t o = o_of(i, j), p = (size_t)i + (size_t)j * g.nx;
+            if (!act[o]) continue;
+            const double w = c->h_dxu[i + 2] * c->h_cosu[j + 2] * c->h_dyu[j + 2];
+            L.act[p] = 1;
+            ++L.nact;
+            wsum += w;
+            L.C[p] = w * cC[o];
+            const int ie = (i + 1 < g.nx) ? i + 1 : (g.cyclic ? 0 : -1);
+            if (ie >= 0 && act[o_of(ie, j)]) L.E[p] = w * E[o];
+            if (j + 1 < g.ny && act[o_of(i, j + 1)]) L.N[p] = w * N[o];
+        }
+    w_mean = L.nact ? wsum / L.nact : 1.0;
+    return L;
+}
+
+int mg_setup(psi_ctx* c) {
+    mg_release(c);
+    const Geom& g = c->g;
+    std::vector<double> E(c->plane), N(c->plane), cC(c->plane);
+    std::vector<uint8_t> act(c->plane);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(E.data(), c->E, c->plane * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(N.data(), c->N, c->plane * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cC.data(), c->cC, c->plane * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(act.data(), c->act, c->plane, hipMemcpyDeviceToHost));
+    MgState* m = new MgState();
+    c->mg = m;
+    std::vector<MgHostLev> H;
+    H.push_back(mg_level0(c, E, N, cC, act, m->w_mean));
+    m->nact0 = H[0].nact;
+    if (m->nact0 == 0) {
+        mg_release(c);
+        return fail(c, PSI_BAD_ARG, "multigrid: no active points%s");
+    }
+    while (H.back().nact > MG_MAX_DENSE && (int)H.size() < MG_MAX_LEVELS) {
+        MgHostLev next = mg_coarsen(H.back());
+        if ((size_t)next.nx * next.ny >= (size_t)H.back().nx * H.back().ny) break;
+        H.push_back(std::move(next));
+    }
+    if (H.back().nact > MG_MAX_DENSE * 4) {
+        mg_release(c);
+        return fail(c, PSI_BAD_ARG, "multigrid: coarsest level too large%s");
+    }
+    std::vector<int> cells;
+    std::vector<float> inv;
+    if (!mg_dense_inverse(H.back(), cells, inv)) {
+        mg_release(c);
+        return fail(c, PSI_BAD_ARG, "multigrid: singular coarsest operator%s");
+    }
+    m->L = (int)H.size() - 1;
+    m->ncoarse = (int)cells.size();
+    int rc = PSI_OK;
+#define MGTRY(x)          \
+    do {                  \
+        rc = (x);         \
+        if (rc) {         \
+            mg_release(c); \
+            return rc;    \
+        }                 \
+    } while (0)
+    for (const MgHostLev& h : H) {
+        MgLev d{};
+        d.nx = h.nx;
+        d.ny = h.ny;
+        d.pitch = h.nx + 2;
+        d.cyc = h.cyc;
+        const size_t np = (size_t)d.pitch * (h.ny + 2);
+        std::vector<float> hE(np, 0.0f), hN(np, 0.0f), hC(np, 1.0f);
+        std::vector<uint8_t> hA(np, 0);
+        for (int j = 0; j < h.ny; ++j)
+            for (int i = 0; i < h.nx; ++i) {
+                const size_t p = (size_t)i + (size_t)j * h.nx;
+                const size_t q = (size_t)(j + 1) * d.pitch + (i + 1);
+                hE[q] = (float)h.E[p];
+                hN[q] = (float)h.N[p];
+                hC[q] = (float)h.C[p];
+                hA[q] = h.act[p];
+            }
+        float *dE, *dN, *dC;
+        uint8_t* dA;
+        MGTRY(mg_alloc(c, m, &dE, np));
+        MGTRY(mg_alloc(c, m, &dN, np));
+        MGTRY(mg_alloc(c, m, &dC, np));
+        MGTRY(mg_alloc(c, m, &dA, np));
+        HIPCHK(c, hipMemcpy(dE, hE.data(), np * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dN, hN.data(), np * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dC, hC.data(), np * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dA, hA.data(), np, hipMemcpyHostToDevice));
+        d.E = dE;
+        d.N = dN;
+        d.C = dC;
+        d.act = dA;
+        MGTRY(mg_alloc(c, m, &d.b, np));
+        MGTRY(mg_alloc(c, m, &d.x, np));
+        MGTRY(mg_alloc(c, m, &d.y, np));
+        MGTRY(mg_alloc(c, m, &d.r, np));
+        MGTRY(mg_alloc(c, m, &d.d0, np));
+        MGTRY(mg_alloc(c, m, &d.d1, np));
+        m->lev.push_back(d);
+        m->level_nx.push_back(h.nx);
+        m->level_ny.push_back(h.ny);
+        m->level_nact.push_back(h.nact);
+    }
+    MGTRY(mg_alloc(c, m, &m->Ainv, inv.size() + 1));
+    MGTRY(mg_alloc(c, m, &m->cells, cells.size() + 1));
+    if (!inv.empty()) {
+        HIPCHK(c, hipMemcpy(m->Ainv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(m->cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
+    }
+    MGTRY(mg_alloc(c, m, &m->x, c->plane));
+    MGTRY(mg_alloc(c, m, &m->r, c->plane));
+    MGTRY(mg_alloc(c, m, &m->p, c->plane));
+    MGTRY(mg_alloc(c, m, &m->q, c->plane));
+    MGTRY(mg_alloc(c, m, &m->S, 1));
+    MGTRY(mg_alloc(c, m, &m->zscale, 1));
+    m->nblk = (int)nblk((size_t)g.nx * g.ny);
+    MGTRY(mg_alloc(c, m, &m->part, (size_t)MG_NRED * m->nblk));
+#undef MGTRY
+    HIPCHK(c, hipHostMalloc((void**)&m->h_S, sizeof(MgScal), hipHostMallocDefault));
+    HIPCHK(c, hipStreamCreateWithFlags(&m->gs, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&m->e_in, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&m->e_out, hipEventDisableTiming));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSI_OK;
+}
+
+// One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold.
+void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch) {
+    MgState* m = c->mg;
+    const int L = m->L;
+    auto grid = [](const MgLev& v) { return dim3(nblk((size_t)v.nx * v.ny)); };
+    for (int l = 0; l < L; ++l) {
+        const MgLev& v = m->lev[l];
+        hipLaunchKernelGGL(k_mg_pre1, grid(v), dim3(256), 0, s, v, ch, K, (const MgScal*)m->S);
+        for (int k = 2; k < K; ++k)
+            hipLaunchKernelGGL(k_mg_step, grid(v), dim3(256), 0, s, v, ch, k, 0,
+                               (const float*)((k - 1) & 1 ? v.d1 : v.d0), (k & 1) ? v.d1 : v.d0,
+                               (const MgScal*)m->S);
+        const float* dlast = (K == 1) ? v.d0 : (((K - 1) & 1) ? v.d1 : v.d0);
+        hipLaunchKernelGGL(k_mg_restrict, grid(m->lev[l + 1]), dim3(256), 0, s, v, dlast,
+                           m->lev[l + 1], (const MgScal*)m->S);
+    }
+    if (m->ncoarse > 0)
+        hipLaunchKernelGGL(k_mg_dense, dim3((m->ncoarse + 3) / 4), dim3(256), 0, s, m->lev[L],
+                           (const float*)m->Ainv, (const int*)m->cells, m->ncoarse,
+                           (const MgScal*)m->S);
+    for (int l = L - 1; l >= 0; --l) {
+        const MgLev& v = m->lev[l];
+        hipLaunchKernelGGL(k_mg_post1, grid(v), dim3(256), 0, s, v, m->lev[l + 1], ch,
+                           (const MgScal*)m->S);
+        for (int k = 1; k < K; ++k)
+            hipLaunchKernelGGL(k_mg_step, grid(v), dim3(256), 0, s, v, ch, k, 1,
+                               (const float*)((k - 1) & 1 ? v.d1 : v.d0), (k & 1) ? v.d1 : v.d0,
+                               (const MgScal*)m->S);
+    }
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+                       (const double*)c->E, (const double*)c->N, (const double*)c->cC,
+                       (const double*)c->rx, (const double*)c->ry, (const double*)c->dxu,
+                       (const double*)c->cosu, (const double*)c->dyu, (const uint8_t*)c->act,
+                       (const double*)m->p, (const double*)m->q, (const double*)m->r,
+                       (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
+                       m->S);
+    hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+                       (const double*)c->E, (const double*)c->N, (const double*)c->cC,
+                       (const double*)c->rx, (const double*)c->ry, (const double*)c->dxu,
+                       (const double*)c->cosu, (const double*)c->dyu, (const uint8_t*)c->act,
+                       m->p, m->q, m->r, m->x, (const double*)m->zscale, (const MgScal*)m->S,
+                       m->part);
+    hipLaunchKernelGGL(k_mg_fold_b, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
+                       m->nact0, m->S, m->zscale);
+}
+
+int mg_graph(psi_ctx* c, int K, double ratio) {
+    MgState* m = c->mg;
+    if (m->exec && m->graph_K == K && m->graph_ratio == ratio) return PSI_OK;
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->exec = nullptr;
+    m->graph = nullptr;
+    const MgCheb ch = mg_cheb(K, ratio);
+    HIPCHK(c, hipStreamBeginCapture(m->gs, hipStreamCaptureModeThreadLocal));
+    mg_iteration(c, m->gs, K, ch);
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamEndCapture(m->gs, &graph));
+    m->graph = graph;
+    HIPCHK(c, hipGraphInstantiate(&m->exec, graph, nullptr, nullptr, 0));
+    m->graph_K = K;
+    m->graph_ratio = ratio;
+    return PSI_OK;
+}
+
+int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
+    MgState* m = c->mg;
+    if (!m) return fail(c, PSI_BAD_ARG, "multigrid mode: call psi_mg_setup first%s");
+    if (o->criterion != PSI_CRIT_RELRES)
+        return fail(c, PSI_BAD_ARG, "multigrid mode needs the relres criterion%s");
+    const int K = o->poly_degree > 0 ? o->poly_degree : 3;
+    if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
+    const double lower = o->poly_lower > 0 ? o->poly_lower : 1.0 / 15.0;
+    if (!(lower < 1.0)) return fail(c, PSI_BAD_ARG, "multigrid interval: lower < 1%s");
+    const double ratio = 1.0 / lower;
+    const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
+    const int batch = o->check_every > 0 ? o->check_every : 4;
+    int rc = mg_graph(c, K, ratio);
+    if (rc) return rc;
+    const Geom& g = c->g;
+    HIPCHK(c, hipMemcpyAsync(m->x, sol, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
+    double res[4];
+    double bb = 0.0;
+    int n = 0, restarts = 0, status = ST_RUNNING;
+    for (;;) {
+        rc = residual(c, forc, m->x, m->r, res);
+        if (rc) return rc;
+        info->setup_launches += 1;
+        if (restarts == 0) bb = res[1] > 0 ? res[1] : res[0];
+        if (res[0] <= target * target * bb) {
+            status = ST_CONVERGED;
+            break;
+        }
+        if (res[0] != res[0]) {
+            status = ST_NAN;
+            break;
+        }
+        if (n >= o->max_iterations) {
+            status = ST_MAXIT;
+            break;
+        }
+        const double s0 = 1.0 / (m->w_mean * sqrt(res[0] / std::max(res[3], 1.0)));
+        MgScal S0{};
+        S0.s_r = s0;
+        S0.s_pending = s0;
+        S0.rr = res[0];
+        S0.bb = bb;
+        S0.rtarget2 = target * target;
+        S0.n = n;
+        S0.max_it = o->max_iterations;
+        S0.status = ST_RUNNING;
+        *m->h_S = S0;
+        HIPCHK(c, hipMemcpyAsync(m->S, m->h_S, sizeof(MgScal), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(m->zscale, &m->h_S->s_r, sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemsetAsync(m->p, 0, c->plane * 8, c->stream));
+        HIPCHK(c, hipMemsetAsync(m->q, 0, c->plane * 8, c->stream));
+        hipLaunchKernelGGL(k_mg_start, dim3(nblk((size_t)g.nx * g.ny)), dim3(256), 0, c->stream,
+                           g, m->lev[0], (const double*)c->dxu, (const double*)c->cosu,
+                           (const double*)c->dyu, (const uint8_t*)c->act, (const double*)m->r,
+                           s0);
+        rc = check_launch(c);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(m->e_in, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
+        for (;;) {
+            for (int k = 0; k < batch; ++k) HIPCHK(c, hipGraphLaunch(m->exec, m->gs));
+            HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
+            HIPCHK(c, hipStreamSynchronize(m->gs));
+            info->sweep_launches += batch;
+            if (m->h_S->done) break;
+        }
+        HIPCHK(c, hipEventRecord(m->e_out, m->gs));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, m->e_out, 0));
+        n = m->h_S->n;
+        status = m->h_S->status;
+        if (status != ST_CONVERGED) break;
+        ++restarts;   // confirm on the FP64 true residual (top of the loop)
+    }
+    // the FP64 true residual of the final iterate
+    if (status != ST_CONVERGED || restarts == 0) {
+        rc = residual(c, forc, m->x, nullptr, res);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(sol, m->x, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
+    info->iterations = status == ST_MAXIT ? o->max_iterations + 1 : n;
+    info->status = status_of(status);
+    info->converged = status == ST_CONVERGED;
+    info->outer_iterations = restarts;
+    info->relres_true = bb > 0 ? sqrt(res[0] / bb) : sqrt(res[0]);
+    info->relres_recursive = bb > 0 ? sqrt(m->h_S->rr / bb) : 0.0;
+    return info->status;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -792,10 +1125,35 @@ const char* psi_ctx_last_error(const psi_ctx* c) {
 }
 
 int psi_ctx_nisle(const psi_ctx* c) { return c ? c->nisle : 0; }
-int64_t psi_ctx_device_bytes(const psi_ctx* c) { return c ? c->bytes : 0; }
+int64_t psi_ctx_device_bytes(const psi_ctx* c) {
+    return c ? c->bytes + (c->mg ? c->mg->bytes : 0) : 0;
+}
+
+int psi_mg_setup(psi_ctx* c) {
+    if (!valid_field_ctx(c)) return PSI_BAD_ARG;
+    return mg_setup(c);
+}
+
+int psi_mg_levels(const psi_ctx* c, int32_t* nlevels, int32_t* nx, int32_t* ny, int32_t* nact,
+                  int32_t cap) {
+    if (!c || !nlevels) return PSI_BAD_ARG;
+    if (!c->mg) {
+        *nlevels = 0;
+        return PSI_OK;
+    }
+    const int n = (int)c->mg->level_nx.size();
+    *nlevels = n;
+    for (int k = 0; k < n && k < cap; ++k) {
+        if (nx) nx[k] = c->mg->level_nx[k];
+        if (ny) ny[k] = c->mg->level_ny[k];
+        if (nact) nact[k] = c->mg->level_nact[k];
+    }
+    return PSI_OK;
+}
 
 void psi_ctx_destroy(psi_ctx* c) {
     if (!c) return;
+    mg_release(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     else (void)hipDeviceSynchronize();
     for (void* p : c->allocs) (void)hipFree(p);
@@ -1002,6 +1360,7 @@ int psi_ctx_update_coeffs(psi_ctx* c, const double* hur, const double* hvr) {
     if (!valid_field_ctx(c) || !hur || !hvr) return PSI_BAD_ARG;
     c->hur = hur;
     c->hvr = hvr;
+    mg_release(c);   // the hierarchy belongs to the old operator
     int rc = build_operator(c);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1027,6 +1386,7 @@ int psi_congrad(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, 
         case PSI_MODE_PLAIN: rc = solve_plain(c, o, forc, sol, info); break;
         case PSI_MODE_MIXED: rc = solve_mixed(c, o, forc, sol, info); break;
         case PSI_MODE_POLY: rc = solve_poly(c, o, forc, sol, info); break;
+        case PSI_MODE_MG: rc = solve_mg(c, o, forc, sol, info); break;
         case PSI_MODE_LEARNED: return fail(c, PSI_NO_LEARNED_MODEL, "no learned model loaded%s");
         default: return fail(c, PSI_BAD_ARG, "unknown mode%s");
     }

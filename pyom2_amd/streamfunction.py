@@ -39,7 +39,7 @@ AB_EPS_REFERENCE = float(np.float32(0.1))
 GRAV_REFERENCE = float(np.float32(9.81))
 RHO_0_REFERENCE = 1024.0
 
-MODES = {"plain": _lib.PSI_MODE_PLAIN, "mixed": _lib.PSI_MODE_MIXED,
+MODES = {"plain": _lib.PSI_MODE_PLAIN, "mixed": _lib.PSI_MODE_MIXED, "mg": _lib.PSI_MODE_MG,
          "learned": _lib.PSI_MODE_LEARNED, "poly": _lib.PSI_MODE_POLY}
 CRITERIA = {"reference": _lib.PSI_CRIT_REFERENCE, "relres": _lib.PSI_CRIT_RELRES}
 
@@ -172,6 +172,7 @@ class StreamfunctionSolver:
         if rc != _lib.PSI_OK:
             raise PsiError(rc, self.L.psi_ctx_last_error(None).decode())
         self.ctx = ctx
+        self._mg_ready = False
 
     @classmethod
     def from_basin(cls, b, islands: Islands | None = None, device="cuda", **kw):
@@ -214,6 +215,8 @@ class StreamfunctionSolver:
         not raised (the reference prints and continues); NaN raises
         (the reference halts)."""
         opts = opts or SolveOptions(**kw)
+        if opts.mode == "mg" and not self._mg_ready:
+            self.mg_setup()
         info = PsiInfo()
         self._sync_in()
         rc = self.L.psi_congrad(self.ctx, C.byref(opts.as_c()), _dev(forc, self.shape),
@@ -221,6 +224,21 @@ class StreamfunctionSolver:
         self._check(rc, (_lib.PSI_OK, _lib.PSI_NOT_CONVERGED, _lib.PSI_DIVERGED))
         d = info.as_dict()
         return d["iterations"], bool(d["converged"]), d
+
+    def mg_setup(self):
+        """Build the multigrid hierarchy (needed by mode "mg"; done on first use)."""
+        self._sync_in()
+        self._check(self.L.psi_mg_setup(self.ctx))
+        self._mg_ready = True
+
+    def mg_levels(self):
+        """[(nx, ny, active cells)] of the multigrid levels, coarsest last."""
+        cap = 32
+        n = np.zeros(1, np.int32)
+        nx, ny, na = (np.zeros(cap, np.int32) for _ in range(3))
+        self._check(self.L.psi_mg_levels(self.ctx, _i32p(n), _i32p(nx), _i32p(ny), _i32p(na),
+                                         cap))
+        return [(int(nx[k]), int(ny[k]), int(na[k])) for k in range(min(int(n[0]), cap))]
 
     # -- building blocks -------------------------------------------------------
     def apply_op(self, p: torch.Tensor, res: torch.Tensor | None = None):
