@@ -11,9 +11,11 @@ Paths: "learned" is the learned-preconditioner solver with its measured
 policy (flexible CG with the trained network or the fastest classical path
 measured here, whichever needs fewer ms per residual decade on a probe from
 the same first guess; probes at the first solve and every --reprobe-every
-solves); "poly" is FP32-inner / FP64-refined PCG with a Chebyshev polynomial
-of the Jacobi-preconditioned operator (degree 4); "mixed" is FP32-inner /
-FP64-refined Jacobi-PCG; "plain" is FP64 Jacobi-PCG.
+solves); "mg" is FP64 flexible CG preconditioned by an FP32
+aggregation-multigrid V-cycle (the fastest classical path, so the learned
+policy's fallback); "poly" is FP32-inner / FP64-refined PCG with a Chebyshev
+polynomial of the Jacobi-preconditioned operator (degree 4); "mixed" is
+FP32-inner / FP64-refined Jacobi-PCG; "plain" is FP64 Jacobi-PCG.
 
 Prints one JSON line with the headline path's ms per solve, the other
 paths beside it, the fused sweep kernel's roofline figure and the CPU
@@ -41,6 +43,18 @@ BYTES_PER_PT = {"plain": 6 * 8 + 2 * 8 + 1, "mixed": 6 * 4 + 2 * 4 + 1,
                 # polynomial-preconditioned FP32 sweep: x, r, z = M r and p
                 # read and written, E and N, the mask
                 "poly": 8 * 4 + 2 * 4 + 1}
+# multigrid path: algorithmic bytes per point of the finest level for its
+# heaviest kernels (FP32 level vectors, FP64 outer vectors): prolongation +
+# post-smoothing reads x, b, E, N, C, the mask and a quarter of the coarse
+# correction and writes y; pre-smoothing + restriction reads b, E, N, C and
+# writes x and a quarter of the coarse right-hand side; outer pass A reads z,
+# E, N, q, r, the mask and writes Az; outer pass B reads z, Az, p, q, r, x,
+# the mask and writes p, q, r, x and the next V-cycle input
+MG_KERNEL_BYTES = {"post_level0": 5 * 4 + 1 + 1 + 4, "pre_level0": 4 * 4 + 4 + 1,
+                   "fcg_a": 4 + 2 * 8 + 2 * 8 + 1 + 8, "fcg_b": 4 + 8 + 4 * 8 + 1 + 4 * 8 + 4}
+MG_KERNEL_NAMES = {"post_level0": "k_mg_post<4> (finest level)",
+                   "pre_level0": "k_mg_pre<4> (finest level)",
+                   "fcg_a": "k_mg_fcg_a", "fcg_b": "k_mg_fcg_b"}
 # HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
 # profiles/k_cg_sweep_spherical_2048.md, profiles/k_poly_pcg_spherical_2048.md
@@ -193,13 +207,25 @@ def main():
         avg_launch_us = 1e3 * sweep_ms / max(launches, 1)
         bytes_per_launch = BYTES_PER_PT[mode if mode in BYTES_PER_PT else "mixed"] * npts * \
             its.sum() / max(launches, 1)
-        achieved = bytes_per_launch / (avg_launch_us * 1e-6) / 1e9
+        achieved = bytes_per_launch / (avg_launch_us * 1e-6) / 1e9 if avg_launch_us > 0 else 0.0
+        if mode == "mg":
+            # the solve's heaviest kernels, timed with HIP events between solves
+            kt = s.mg_kernel_times(degree=args.poly_degree, reps=20)
+            top = max(kt, key=kt.get)
+            mg_kernels = {k: dict(us=round(v, 3), bytes_per_pt=MG_KERNEL_BYTES[k],
+                                  GBs=round(MG_KERNEL_BYTES[k] * npts / (v * 1e-6) / 1e9, 1))
+                          for k, v in kt.items()}
+            avg_launch_us = kt[top]
+            achieved = MG_KERNEL_BYTES[top] * npts / (avg_launch_us * 1e-6) / 1e9
         results[mode] = dict(ms_per_solve=1e3 * total / args.steps, iterations_mean=float(its.mean()),
                              relres_max=max(p["relres"] for p in per),
                              outer_mean=float(np.mean([p["outer"] for p in per])),
                              sweep_avg_launch_us=avg_launch_us, sweep_launches=launches,
                              achieved_GBs=achieved, frac=achieved / HBM_PEAK_GBS,
                              total_s=total)
+        if mode == "mg":
+            results[mode].update(us_per_iteration=1e3 * sweep_ms / max(launches, 1),
+                                 kernels=mg_kernels, dominant=top)
         if ls is not None:
             timed = choices[args.warmup:]
             ch = max(set(timed), key=timed.count)
@@ -213,8 +239,8 @@ def main():
     r = results[head]
     # the roofline figure is the fused sweep kernel of the Jacobi path that
     # carries the solve (the learned policy's fallback is the mixed path)
-    kpath = head if head != "learned" else next((m for m in ("poly", "mixed") if m in results),
-                                                   None)
+    kpath = head if head != "learned" else next((m for m in ("mg", "poly", "mixed")
+                                                    if m in results), None)
     rk = results.get(kpath, {}) if kpath else {}
     cpu = cpu_all = None
     if not args.skip_cpu:
@@ -242,14 +268,16 @@ def main():
         "paths": {m: {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
                       for k, v in d.items()} for m, d in results.items()},
         "roofline": ({"bound": "hbm",
-                      "kernel": ("k_poly_pcg<4, sweep>" if kpath == "poly" else
+                      "kernel": (MG_KERNEL_NAMES[rk["dominant"]] if kpath == "mg" else
+                                 "k_poly_pcg<4, sweep>" if kpath == "poly" else
                                  f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>"),
                       "path": kpath, "achieved": round(rk["achieved_GBs"], 1),
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rk["frac"], 4),
                       "traffic": (TRAFFIC_2048.get(kpath) if b.nx == b.ny == 2048 else
                                   None),
                       "traffic_unit": "bytes per launch",
-                      "bytes_per_pt_iter": BYTES_PER_PT[kpath],
+                      "bytes_per_pt_iter": (MG_KERNEL_BYTES[rk["dominant"]] if kpath == "mg"
+                                            else BYTES_PER_PT[kpath]),
                       "avg_launch_us": round(rk["sweep_avg_launch_us"], 3)} if rk else None),
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,

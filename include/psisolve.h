@@ -49,7 +49,7 @@ typedef enum psi_mode {
                               poly_degree in Z A (Z the Jacobi inverse) times Z */
     PSI_MODE_MG = 4        /* FP64 flexible CG preconditioned by an FP32 aggregation-multigrid
                               V-cycle (psi_mg_setup first); PSI_CRIT_RELRES only.  poly_degree
-                              = Chebyshev smoothing steps per level (<= 0: 3), poly_lower = the
+                              = Chebyshev smoothing steps per level (<= 0: 4), poly_lower = the
                               smoother's interval [2 poly_lower, 2] (<= 0: 1/15) */
 } psi_mode;
 
@@ -279,6 +279,13 @@ int psi_solve_streamfunction_3d(psi_ctx *ctx, const psi_opts *opts, const psi_le
 int psi_mg_setup(psi_ctx *ctx);
 int psi_mg_levels(const psi_ctx *ctx, int32_t *nlevels, int32_t *nx, int32_t *ny,
                   int32_t *nactive, int32_t cap);
+/* Performance report: average duration in microseconds of `reps` launches of
+ * each of the solve's heaviest kernels, timed with HIP events on the solve's
+ * stream: us[0] prolongation + post-smoothing on the finest level, us[1]
+ * pre-smoothing + restriction on the finest level, us[2] and us[3] the two
+ * passes of the outer iteration (n >= 4).  Their vectors are scratch: call
+ * between solves. */
+int psi_mg_kernel_times(psi_ctx *ctx, int32_t degree, int32_t reps, double *us, int32_t n);
 
 /* ---- flexible CG with the learned preconditioner ---------------------------
  * No counterpart in the reference.  The host applies the network of psinn.h

@@ -152,6 +152,7 @@ SIGNATURES = [
     ("psi_poly_coefficients", C.c_int, [C.c_int32, C.c_double, C.POINTER(C.c_float)]),
     ("psi_mg_setup", C.c_int, [C.c_void_p]),
     ("psi_mg_levels", C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, C.c_int32]),
+    ("psi_mg_kernel_times", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32]),
     ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),

@@ -794,6 +794,8 @@ void mg_release(psi_ctx* c) {
     if (m->h_S) (void)hipHostFree(m->h_S);
     if (m->e_in) (void)hipEventDestroy(m->e_in);
     if (m->e_out) (void)hipEventDestroy(m->e_out);
+    if (m->t0) (void)hipEventDestroy(m->t0);
+    if (m->t1) (void)hipEventDestroy(m->t1);
     if (m->gs) (void)hipStreamDestroy(m->gs);
     delete m;
     c->mg = nullptr;
@@ -926,9 +928,6 @@ int mg_setup(psi_ctx* c) {
         MGTRY(mg_alloc(c, m, &d.b, np));
         MGTRY(mg_alloc(c, m, &d.x, np));
         MGTRY(mg_alloc(c, m, &d.y, np));
-        MGTRY(mg_alloc(c, m, &d.r, np));
-        MGTRY(mg_alloc(c, m, &d.d0, np));
-        MGTRY(mg_alloc(c, m, &d.d1, np));
         m->lev.push_back(d);
         m->level_nx.push_back(h.nx);
         m->level_ny.push_back(h.ny);
@@ -944,63 +943,70 @@ int mg_setup(psi_ctx* c) {
     MGTRY(mg_alloc(c, m, &m->r, c->plane));
     MGTRY(mg_alloc(c, m, &m->p, c->plane));
     MGTRY(mg_alloc(c, m, &m->q, c->plane));
+    MGTRY(mg_alloc(c, m, &m->az, c->plane));
     MGTRY(mg_alloc(c, m, &m->S, 1));
+    MGTRY(mg_alloc(c, m, &m->S_time, 1));
     MGTRY(mg_alloc(c, m, &m->zscale, 1));
-    m->nblk = (int)nblk((size_t)g.nx * g.ny);
+    m->nblk = (int)std::min<size_t>(nblk((size_t)g.nx * g.ny), MG_OUTER_BLOCKS);
     MGTRY(mg_alloc(c, m, &m->part, (size_t)MG_NRED * m->nblk));
 #undef MGTRY
     HIPCHK(c, hipHostMalloc((void**)&m->h_S, sizeof(MgScal), hipHostMallocDefault));
     HIPCHK(c, hipStreamCreateWithFlags(&m->gs, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&m->e_in, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&m->e_out, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreate(&m->t0));
+    HIPCHK(c, hipEventCreate(&m->t1));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PSI_OK;
 }
 
 // One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold.
-void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch) {
-    MgState* m = c->mg;
+template <int K>
+void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch) {
     const int L = m->L;
-    auto grid = [](const MgLev& v) { return dim3(nblk((size_t)v.nx * v.ny)); };
-    for (int l = 0; l < L; ++l) {
-        const MgLev& v = m->lev[l];
-        hipLaunchKernelGGL(k_mg_pre1, grid(v), dim3(256), 0, s, v, ch, K, (const MgScal*)m->S);
-        for (int k = 2; k < K; ++k)
-            hipLaunchKernelGGL(k_mg_step, grid(v), dim3(256), 0, s, v, ch, k, 0,
-                               (const float*)((k - 1) & 1 ? v.d1 : v.d0), (k & 1) ? v.d1 : v.d0,
-                               (const MgScal*)m->S);
-        const float* dlast = (K == 1) ? v.d0 : (((K - 1) & 1) ? v.d1 : v.d0);
-        hipLaunchKernelGGL(k_mg_restrict, grid(m->lev[l + 1]), dim3(256), 0, s, v, dlast,
+    auto tiles = [](const MgLev& v) {
+        return dim3((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + MT_Y - 1) / MT_Y)));
+    };
+    for (int l = 0; l < L; ++l)
+        hipLaunchKernelGGL(k_mg_pre<K>, tiles(m->lev[l]), dim3(MT_NT), 0, s, m->lev[l], ch,
                            m->lev[l + 1], (const MgScal*)m->S);
-    }
     if (m->ncoarse > 0)
         hipLaunchKernelGGL(k_mg_dense, dim3((m->ncoarse + 3) / 4), dim3(256), 0, s, m->lev[L],
                            (const float*)m->Ainv, (const int*)m->cells, m->ncoarse,
                            (const MgScal*)m->S);
-    for (int l = L - 1; l >= 0; --l) {
-        const MgLev& v = m->lev[l];
-        hipLaunchKernelGGL(k_mg_post1, grid(v), dim3(256), 0, s, v, m->lev[l + 1], ch,
-                           (const MgScal*)m->S);
-        for (int k = 1; k < K; ++k)
-            hipLaunchKernelGGL(k_mg_step, grid(v), dim3(256), 0, s, v, ch, k, 1,
-                               (const float*)((k - 1) & 1 ? v.d1 : v.d0), (k & 1) ? v.d1 : v.d0,
-                               (const MgScal*)m->S);
+    for (int l = L - 1; l >= 0; --l)
+        hipLaunchKernelGGL(k_mg_post<K>, tiles(m->lev[l]), dim3(MT_NT), 0, s, m->lev[l],
+                           m->lev[l + 1], ch, (const MgScal*)m->S);
+}
+
+void mg_vcycle(MgState* m, hipStream_t s, int K, const MgCheb& ch) {
+    switch (K) {
+        case 1: mg_vcycle_fused<1>(m, s, ch); break;
+        case 2: mg_vcycle_fused<2>(m, s, ch); break;
+        case 3: mg_vcycle_fused<3>(m, s, ch); break;
+        case 4: mg_vcycle_fused<4>(m, s, ch); break;
+        case 5: mg_vcycle_fused<5>(m, s, ch); break;
+        default: mg_vcycle_fused<6>(m, s, ch); break;
     }
+}
+
+// One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold.
+void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch) {
+    MgState* m = c->mg;
+    mg_vcycle(m, s, K, ch);
     const Geom& g = c->g;
     hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
-                       (const double*)c->E, (const double*)c->N, (const double*)c->cC,
-                       (const double*)c->rx, (const double*)c->ry, (const double*)c->dxu,
-                       (const double*)c->cosu, (const double*)c->dyu, (const uint8_t*)c->act,
-                       (const double*)m->p, (const double*)m->q, (const double*)m->r,
+                       (const double*)c->E, (const double*)c->N, (const double*)c->rx,
+                       (const double*)c->ry, (const double*)c->dxu, (const double*)c->cosu,
+                       (const double*)c->dyu, (const uint8_t*)c->act,
+                       (const double*)m->q, (const double*)m->r, m->az,
                        (const double*)m->zscale, (const MgScal*)m->S, m->part);
     hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->S);
     hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
-                       (const double*)c->E, (const double*)c->N, (const double*)c->cC,
-                       (const double*)c->rx, (const double*)c->ry, (const double*)c->dxu,
-                       (const double*)c->cosu, (const double*)c->dyu, (const uint8_t*)c->act,
-                       m->p, m->q, m->r, m->x, (const double*)m->zscale, (const MgScal*)m->S,
-                       m->part);
+                       (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
+                       (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
+                       (const double*)m->zscale, (const MgScal*)m->S, m->part);
     hipLaunchKernelGGL(k_mg_fold_b, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->nact0, m->S, m->zscale);
 }
@@ -1024,12 +1030,39 @@ int mg_graph(psi_ctx* c, int K, double ratio) {
     return PSI_OK;
 }
 
+// Average duration (us) of `reps` back-to-back launches queued by fn on the
+// multigrid stream, timed with HIP events.
+template <class Fn>
+double mg_time(MgState* m, int reps, Fn fn) {
+    if (hipEventRecord(m->t0, m->gs) != hipSuccess) return -1.0;
+    for (int k = 0; k < reps; ++k) fn();
+    if (hipEventRecord(m->t1, m->gs) != hipSuccess || hipEventSynchronize(m->t1) != hipSuccess)
+        return -1.0;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, m->t0, m->t1);
+    return 1e3 * ms / reps;
+}
+
+template <int K>
+void mg_time_levels(MgState* m, int reps, double* us) {
+    const MgCheb ch = mg_cheb(K, 15.0);
+    const MgLev& v = m->lev[0];
+    const dim3 tiles((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + MT_Y - 1) / MT_Y)));
+    const MgScal* S = m->S_time;
+    us[0] = mg_time(m, reps, [&] {
+        hipLaunchKernelGGL(k_mg_post<K>, tiles, dim3(MT_NT), 0, m->gs, v, m->lev[1], ch, S);
+    });
+    us[1] = mg_time(m, reps, [&] {
+        hipLaunchKernelGGL(k_mg_pre<K>, tiles, dim3(MT_NT), 0, m->gs, v, ch, m->lev[1], S);
+    });
+}
+
 int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
     MgState* m = c->mg;
     if (!m) return fail(c, PSI_BAD_ARG, "multigrid mode: call psi_mg_setup first%s");
     if (o->criterion != PSI_CRIT_RELRES)
         return fail(c, PSI_BAD_ARG, "multigrid mode needs the relres criterion%s");
-    const int K = o->poly_degree > 0 ? o->poly_degree : 3;
+    const int K = o->poly_degree > 0 ? o->poly_degree : 4;
     if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
     const double lower = o->poly_lower > 0 ? o->poly_lower : 1.0 / 15.0;
     if (!(lower < 1.0)) return fail(c, PSI_BAD_ARG, "multigrid interval: lower < 1%s");
@@ -1084,13 +1117,20 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
         if (rc) return rc;
         HIPCHK(c, hipEventRecord(m->e_in, c->stream));
         HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
+        const int n_start = n;
+        HIPCHK(c, hipEventRecord(m->t0, m->gs));
         for (;;) {
             for (int k = 0; k < batch; ++k) HIPCHK(c, hipGraphLaunch(m->exec, m->gs));
             HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
             HIPCHK(c, hipStreamSynchronize(m->gs));
-            info->sweep_launches += batch;
             if (m->h_S->done) break;
         }
+        HIPCHK(c, hipEventRecord(m->t1, m->gs));
+        HIPCHK(c, hipEventSynchronize(m->t1));
+        float span = 0.0f;
+        (void)hipEventElapsedTime(&span, m->t0, m->t1);
+        info->sweep_ms += span;                          // graph replays of this leg
+        info->sweep_launches += m->h_S->n - n_start;     // outer iterations they ran
         HIPCHK(c, hipEventRecord(m->e_out, m->gs));
         HIPCHK(c, hipStreamWaitEvent(c->stream, m->e_out, 0));
         n = m->h_S->n;
@@ -1132,6 +1172,45 @@ int64_t psi_ctx_device_bytes(const psi_ctx* c) {
 int psi_mg_setup(psi_ctx* c) {
     if (!valid_field_ctx(c)) return PSI_BAD_ARG;
     return mg_setup(c);
+}
+
+int psi_mg_kernel_times(psi_ctx* c, int32_t degree, int32_t reps, double* us, int32_t n) {
+    if (!valid_field_ctx(c) || !us || n < 4 || reps < 1) return PSI_BAD_ARG;
+    MgState* m = c->mg;
+    if (!m) return fail(c, PSI_BAD_ARG, "call psi_mg_setup first%s");
+    if (m->L < 1) return fail(c, PSI_BAD_ARG, "multigrid: a single level has no V-cycle kernels%s");
+    const int K = degree > 0 ? degree : 4;
+    if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
+    MgScal z{};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(m->S_time, &z, sizeof z, hipMemcpyHostToDevice));
+    switch (K) {
+        case 1: mg_time_levels<1>(m, reps, us); break;
+        case 2: mg_time_levels<2>(m, reps, us); break;
+        case 3: mg_time_levels<3>(m, reps, us); break;
+        case 4: mg_time_levels<4>(m, reps, us); break;
+        case 5: mg_time_levels<5>(m, reps, us); break;
+        default: mg_time_levels<6>(m, reps, us); break;
+    }
+    const Geom& g = c->g;
+    us[2] = mg_time(m, reps, [&] {
+        hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
+                           (const double*)c->E, (const double*)c->N, (const double*)c->rx,
+                           (const double*)c->ry, (const double*)c->dxu, (const double*)c->cosu,
+                           (const double*)c->dyu, (const uint8_t*)c->act, (const double*)m->q,
+                           (const double*)m->r, m->az, (const double*)m->zscale,
+                           (const MgScal*)m->S_time, m->part);
+    });
+    us[3] = mg_time(m, reps, [&] {
+        hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
+                           (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
+                           (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
+                           (const double*)m->zscale, (const MgScal*)m->S_time, m->part);
+    });
+    HIPCHK(c, hipGetLastError());
+    for (int k = 0; k < 4; ++k)
+        if (us[k] < 0) return fail(c, PSI_HIP_ERROR, "multigrid kernel timing failed%s");
+    return PSI_OK;
 }
 
 int psi_mg_levels(const psi_ctx* c, int32_t* nlevels, int32_t* nx, int32_t* ny, int32_t* nact,

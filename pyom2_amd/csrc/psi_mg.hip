@@ -19,8 +19,9 @@
 // [2/ratio, 2] from zero (Gershgorin bounds D^-1 S by 2 on every level),
 // the residual restricted to the next level, the coarse correction added
 // with weight MG_SCALE, and K Chebyshev steps again.  Vectors are FP32 in a
-// 1-cell-halo layout per level; each kernel applies one stencil to a vector
-// its predecessor wrote.
+// 1-cell-halo layout per level; a level costs two LDS-tiled kernels per
+// cycle (pre-smoothing with the restriction, prolongation with the
+// post-smoothing).
 //
 // The outer iteration is flexible CG (FCG(1)) on A x = b in FP64 with
 // area-weighted inner products, the V-cycle applied to the area-weighted
@@ -36,13 +37,15 @@ constexpr int MG_MAX_DENSE = 1024;   // active cells of the coarsest level at mo
 constexpr int MG_MAX_LEVELS = 16;
 constexpr int MG_MAXK = 6;           // Chebyshev steps per smoothing at most
 constexpr float MG_SCALE = 1.7f;     // weight of the coarse-grid correction
-constexpr int MG_NRED = 6;
+constexpr int MG_NRED = 3;           // reduction slots of the outer passes
 
 struct MgLev {
     int nx, ny, pitch, cyc;          // interior cells; pitch = nx + 2
     const float *E, *N, *C;
     const uint8_t* act;
-    float *b, *x, *y, *r, *d0, *d1;
+    float *b;                        // right-hand side (restricted residual)
+    float *x;                        // pre-smoothed solution
+    float *y;                        // solution after the coarse correction and post-smoothing
 };
 
 struct MgCheb {
@@ -59,128 +62,6 @@ struct MgScal {
     int n, done, status, max_it;
 };
 
-__device__ __forceinline__ void mg_cell(const MgLev& L, int t, int& I, int& J, int& q) {
-    I = t % L.nx;
-    J = t / L.nx;
-    q = (J + 1) * L.pitch + (I + 1);
-}
-
-__device__ __forceinline__ void mg_nbrs(const MgLev& L, int I, int q, int& qE, int& qW) {
-    qE = (L.cyc && I == L.nx - 1) ? q - (L.nx - 1) : q + 1;
-    qW = (L.cyc && I == 0) ? q + (L.nx - 1) : q - 1;
-}
-
-// S v at cell q
-__device__ __forceinline__ float mg_apply(const MgLev& L, const float* v, int q, int qE, int qW) {
-    return L.C[q] * v[q] + L.E[q] * v[qE] + L.E[qW] * v[qW] + L.N[q] * v[q + L.pitch] +
-           L.N[q - L.pitch] * v[q - L.pitch];
-}
-
-// Pre-smoothing, steps 0 and 1 from x = 0: d0 = D^-1 b / theta at the cell
-// and its neighbours, r1 = b - S d0, d1 = c1 d0 + c2 D^-1 r1, x = d0 + d1.
-// With K = 1 only d0 is formed (x = d0, r = b).
-__global__ void __launch_bounds__(256) k_mg_pre1(MgLev L, MgCheb ch, int K,
-                                                 const MgScal* __restrict__ S) {
-    if (S->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.nx * L.ny) return;
-    int I, J, q, qE, qW;
-    mg_cell(L, t, I, J, q);
-    mg_nbrs(L, I, q, qE, qW);
-    const float d0 = L.b[q] * ch.c0 / L.C[q];
-    if (K == 1) {
-        L.x[q] = d0;
-        L.r[q] = L.b[q];
-        L.d0[q] = d0;
-        return;
-    }
-    const float sd = L.C[q] * d0 + L.E[q] * (L.b[qE] * ch.c0 / L.C[qE]) +
-                     L.E[qW] * (L.b[qW] * ch.c0 / L.C[qW]) +
-                     L.N[q] * (L.b[q + L.pitch] * ch.c0 / L.C[q + L.pitch]) +
-                     L.N[q - L.pitch] * (L.b[q - L.pitch] * ch.c0 / L.C[q - L.pitch]);
-    const float r1 = L.b[q] - sd;
-    const float d1 = ch.c1[1] * d0 + ch.c2[1] * (r1 / L.C[q]);
-    L.r[q] = r1;
-    L.d1[q] = d1;
-    L.x[q] = d0 + d1;
-}
-
-// Chebyshev step k >= 1 (k = 1 only after post1): r -= S d_{k-1};
-// d_k = c1 d_{k-1} + c2 D^-1 r; v += d_k, v = x (pre) or y (post).
-// d_{k-1} is din, d_k goes to dout.
-__global__ void __launch_bounds__(256) k_mg_step(MgLev L, MgCheb ch, int k, int post,
-                                                 const float* __restrict__ din, float* dout,
-                                                 const MgScal* __restrict__ S) {
-    if (S->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.nx * L.ny) return;
-    int I, J, q, qE, qW;
-    mg_cell(L, t, I, J, q);
-    mg_nbrs(L, I, q, qE, qW);
-    const float r = L.r[q] - mg_apply(L, din, q, qE, qW);
-    const float d = ch.c1[k] * din[q] + ch.c2[k] * (r / L.C[q]);
-    L.r[q] = r;
-    dout[q] = d;
-    float* v = post ? L.y : L.x;
-    v[q] = v[q] + d;
-}
-
-// Residual after pre-smoothing, r - S d_last, restricted to the coarse
-// level: one thread per coarse cell sums its (up to) 2 x 2 fine cells.
-__global__ void __launch_bounds__(256) k_mg_restrict(MgLev L, const float* __restrict__ dlast,
-                                                     MgLev Lc, const MgScal* __restrict__ S) {
-    if (S->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= Lc.nx * Lc.ny) return;
-    int IC, JC, Q;
-    mg_cell(Lc, t, IC, JC, Q);
-    float s = 0.0f;
-    for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di) {
-            const int I = 2 * IC + di, J = 2 * JC + dj;
-            if (I >= L.nx || J >= L.ny) continue;
-            const int q = (J + 1) * L.pitch + (I + 1);
-            if (!L.act[q]) continue;
-            int qE, qW;
-            mg_nbrs(L, I, q, qE, qW);
-            s += L.r[q] - mg_apply(L, dlast, q, qE, qW);
-        }
-    Lc.b[Q] = Lc.act[Q] ? s : 0.0f;
-}
-
-// Post-smoothing step 0: y' = x + MG_SCALE * P e_c at the cell and its
-// neighbours (active cells only), r = b - S y', d0 = D^-1 r / theta,
-// y = y' + d0.
-__device__ __forceinline__ float mg_prolonged(const MgLev& L, const MgLev& Lc, int q, int I,
-                                              int J) {
-    if (!L.act[q]) return 0.0f;
-    return L.x[q] + MG_SCALE * Lc.y[(J / 2 + 1) * Lc.pitch + (I / 2 + 1)];
-}
-
-__global__ void __launch_bounds__(256) k_mg_post1(MgLev L, MgLev Lc, MgCheb ch,
-                                                  const MgScal* __restrict__ S) {
-    if (S->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.nx * L.ny) return;
-    int I, J, q, qE, qW;
-    mg_cell(L, t, I, J, q);
-    mg_nbrs(L, I, q, qE, qW);
-    const int IE = (I == L.nx - 1) ? (L.cyc ? 0 : -1) : I + 1;
-    const int IW = (I == 0) ? (L.cyc ? L.nx - 1 : -1) : I - 1;
-    const float v = mg_prolonged(L, Lc, q, I, J);
-    const float vE = IE >= 0 ? mg_prolonged(L, Lc, qE, IE, J) : 0.0f;
-    const float vW = IW >= 0 ? mg_prolonged(L, Lc, qW, IW, J) : 0.0f;
-    const float vN = J + 1 < L.ny ? mg_prolonged(L, Lc, q + L.pitch, I, J + 1) : 0.0f;
-    const float vS = J > 0 ? mg_prolonged(L, Lc, q - L.pitch, I, J - 1) : 0.0f;
-    const float sv = L.C[q] * v + L.E[q] * vE + L.E[qW] * vW + L.N[q] * vN +
-                     L.N[q - L.pitch] * vS;
-    const float r = L.b[q] - sv;
-    const float d = ch.c0 * (r / L.C[q]);
-    L.r[q] = r;
-    L.d0[q] = d;
-    L.y[q] = v + d;
-}
-
 // Coarsest level: y = Ainv b over its active cells (one wave per row).
 __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restrict__ Ainv,
                                                   const int* __restrict__ cells, int n,
@@ -195,6 +76,208 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
     if (lane == 0) L.y[cells[row]] = s;
 }
 
+// ---- fused level kernels ------------------------------------------------------
+// One workgroup per 64 x 16 tile of a level with a K-cell frame around it in
+// LDS: the K smoothing steps recompute the frame's shrinking margins instead
+// of writing every step to memory, so a level costs two kernels per V-cycle
+// (pre-smoothing + restriction, prolongation + post-smoothing).
+constexpr int MT_X = 64, MT_Y = 16, MT_NT = 256;
+
+template <int K>
+struct MgFrame {
+    static constexpr int FX = MT_X + 2 * K, FY = MT_Y + 2 * K, FN = FX * FY;
+    static constexpr int PER = (FN + MT_NT - 1) / MT_NT;
+};
+
+// Level cell of frame cell (fx, fy) of the tile at (I0, J0); -1 outside the
+// level (cyclic in x wraps).
+__device__ __forceinline__ int mg_frame_cell(const MgLev& L, int I, int J) {
+    if (J < 0 || J >= L.ny) return -1;
+    if (I < 0 || I >= L.nx) {
+        if (!L.cyc) return -1;
+        I = (I % L.nx + L.nx) % L.nx;
+    }
+    return (J + 1) * L.pitch + (I + 1);
+}
+
+// S v at frame index f (row stride FX)
+template <int FX>
+__device__ __forceinline__ float mg_fs(const float* v, const float* sE, const float* sN,
+                                       const float* sC, int f) {
+    return sC[f] * v[f] + sE[f] * v[f + 1] + sE[f - 1] * v[f - 1] + sN[f] * v[f + FX] +
+           sN[f - FX] * v[f - FX];
+}
+
+// Pre-smoothing from x = 0 (K Chebyshev steps), the residual, and its 2 x 2
+// restriction to the next level's right-hand side.  Each thread keeps r and
+// x of its frame cells in registers; only the direction d, whose neighbours
+// the next step reads, goes through LDS (double-buffered: one barrier per
+// step).
+template <int K>
+__global__ void __launch_bounds__(MT_NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
+                                                  const MgScal* __restrict__ S) {
+    using F = MgFrame<K>;
+    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER;
+    __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
+    const int done = S->done;   // read beside the frame loads, checked before any store
+    const int tiles_x = (L.nx + MT_X - 1) / MT_X;
+    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * MT_Y;
+    float r[PER], x[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * MT_NT;
+        r[u] = 0.0f;
+        x[u] = 0.0f;
+        if (f < FN) {
+            const int fy = f / FX, fx = f - fy * FX;
+            const int q = mg_frame_cell(L, I0 - K + fx, J0 - K + fy);
+            const float b = q >= 0 ? L.b[q] : 0.0f, c = q >= 0 ? L.C[q] : 1.0f;
+            sE[f] = q >= 0 ? L.E[q] : 0.0f;
+            sN[f] = q >= 0 ? L.N[q] : 0.0f;
+            sC[f] = c;
+            const float d0 = b * ch.c0 / c;
+            sD[0][f] = d0;
+            r[u] = b;
+            x[u] = d0;
+        }
+    }
+    if (done) return;
+    __syncthreads();
+    // steps 1..K-1 on margins >= k; step K: the residual on the tile
+#pragma unroll
+    for (int k = 1; k <= K; ++k) {
+        const float* din = sD[(k - 1) & 1];
+        float* dout = sD[k & 1];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int f = threadIdx.x + u * MT_NT;
+            const int fy = f / FX, fx = f - fy * FX;
+            if (f < FN && fx >= k && fx < FX - k && fy >= k && fy < FY - k) {
+                const float rn = r[u] - mg_fs<FX>(din, sE, sN, sC, f);
+                r[u] = rn;
+                if (k < K) {
+                    const float dn = ch.c1[k] * din[f] + ch.c2[k] * (rn / sC[f]);
+                    dout[f] = dn;
+                    x[u] = x[u] + dn;
+                }
+            }
+        }
+        if (k < K) __syncthreads();
+    }
+    // the tile's x, its residual into LDS (reusing sE) for the restriction
+    __syncthreads();
+    float* sRes = sE;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * MT_NT;
+        const int fy = f / FX, fx = f - fy * FX;
+        if (f < FN && fx >= K && fx < FX - K && fy >= K && fy < FY - K) {
+            sRes[f] = r[u];
+            const int I = I0 + fx - K, J = J0 + fy - K;
+            if (I < L.nx && J < L.ny) L.x[(J + 1) * L.pitch + (I + 1)] = x[u];
+        }
+    }
+    __syncthreads();
+    {
+        const int l = threadIdx.x;   // one coarse cell per thread: 32 x 8
+        const int cy = l / (MT_X / 2), cx = l - cy * (MT_X / 2);
+        const int IC = I0 / 2 + cx, JC = J0 / 2 + cy;
+        if (IC < Lc.nx && JC < Lc.ny) {
+            float sum = 0.0f;
+            for (int dj = 0; dj < 2; ++dj)
+                for (int di = 0; di < 2; ++di) {
+                    const int tx = 2 * cx + di, ty = 2 * cy + dj;
+                    if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sRes[(ty + K) * FX + tx + K];
+                }
+            const int Q = (JC + 1) * Lc.pitch + (IC + 1);
+            Lc.b[Q] = Lc.act[Q] ? sum : 0.0f;
+        }
+    }
+}
+
+// Prolongation of the coarse correction (weight MG_SCALE, active cells) and
+// K Chebyshev steps of post-smoothing; the result goes to y.  y' (needed
+// with its neighbours once) and then d go through LDS, r and y stay in
+// registers.
+template <int K>
+__global__ void __launch_bounds__(MT_NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
+                                                   const MgScal* __restrict__ S) {
+    using F = MgFrame<K>;
+    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER;
+    __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
+    const int done = S->done;
+    const int tiles_x = (L.nx + MT_X - 1) / MT_X;
+    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * MT_Y;
+    float r[PER], y[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * MT_NT;
+        r[u] = 0.0f;
+        y[u] = 0.0f;
+        if (f < FN) {
+            const int fy = f / FX, fx = f - fy * FX;
+            int I = I0 - K + fx;
+            const int J = J0 - K + fy;
+            const int q = mg_frame_cell(L, I, J);
+            float yv = 0.0f;
+            if (q >= 0 && L.act[q]) {
+                if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
+                yv = L.x[q] + MG_SCALE * Lc.y[(J / 2 + 1) * Lc.pitch + (I / 2 + 1)];
+            }
+            sE[f] = q >= 0 ? L.E[q] : 0.0f;
+            sN[f] = q >= 0 ? L.N[q] : 0.0f;
+            sC[f] = q >= 0 ? L.C[q] : 1.0f;
+            sD[1][f] = yv;                       // y' for step 0's stencil
+            r[u] = q >= 0 ? L.b[q] : 0.0f;       // b
+            y[u] = yv;
+        }
+    }
+    if (done) return;
+    __syncthreads();
+    // step 0 (margin >= 1): r = b - S y', d0 = D^-1 r / theta, y = y' + d0
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * MT_NT;
+        const int fy = f / FX, fx = f - fy * FX;
+        if (f < FN && fx >= 1 && fx < FX - 1 && fy >= 1 && fy < FY - 1) {
+            const float rn = r[u] - mg_fs<FX>(sD[1], sE, sN, sC, f);
+            const float dn = ch.c0 * (rn / sC[f]);
+            r[u] = rn;
+            sD[0][f] = dn;
+            y[u] = y[u] + dn;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+        const int m = k + 1;
+        const float* din = sD[(k - 1) & 1];
+        float* dout = sD[k & 1];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int f = threadIdx.x + u * MT_NT;
+            const int fy = f / FX, fx = f - fy * FX;
+            if (f < FN && fx >= m && fx < FX - m && fy >= m && fy < FY - m) {
+                const float rn = r[u] - mg_fs<FX>(din, sE, sN, sC, f);
+                const float dn = ch.c1[k] * din[f] + ch.c2[k] * (rn / sC[f]);
+                r[u] = rn;
+                dout[f] = dn;
+                y[u] = y[u] + dn;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * MT_NT;
+        const int fy = f / FX, fx = f - fy * FX;
+        if (f < FN && fx >= K && fx < FX - K && fy >= K && fy < FY - K) {
+            const int I = I0 + fx - K, J = J0 + fy - K;
+            if (I < L.nx && J < L.ny) L.y[(J + 1) * L.pitch + (I + 1)] = y[u];
+        }
+    }
+}
+
 // ---- outer flexible CG (FP64, solver grid layout) ---------------------------
 // z(i,j) = y0(i,j) / s_r with wrap in x (cyclic) and 0 outside the interior.
 __device__ __forceinline__ double mg_z(const Geom& g, const MgLev& L0, int i, int j, double inv) {
@@ -206,25 +289,28 @@ __device__ __forceinline__ double mg_z(const Geom& g, const MgLev& L0, int i, in
     return (double)L0.y[(j + 1) * L0.pitch + (i + 1)] * inv;
 }
 
-// A z at interior point (i,j) (ctx index o): the stored operator in FP64.
+// A z at interior point (i,j) (ctx index o) with the stored operator in
+// FP64: W = E(i-1,j) rx(i), S = N(i,j-1) ry(j), C = -E-W-N-S.
 __device__ __forceinline__ double mg_Az(const Geom& g, const MgLev& L0,
                                         const double* __restrict__ E,
                                         const double* __restrict__ N,
-                                        const double* __restrict__ cC,
                                         const double* __restrict__ rx,
                                         const double* __restrict__ ry, int i, int j, size_t o,
                                         double inv, double& z) {
     z = mg_z(g, L0, i, j, inv);
     const int c = i + 2, r = j + 2;
-    return cC[o] * z + E[o] * mg_z(g, L0, i + 1, j, inv) +
-           E[o - 1] * rx[c] * mg_z(g, L0, i - 1, j, inv) + N[o] * mg_z(g, L0, i, j + 1, inv) +
-           N[o - g.pitch] * ry[r] * mg_z(g, L0, i, j - 1, inv);
+    const double e = E[o], w = E[o - 1] * rx[c], n = N[o], s = N[o - g.pitch] * ry[r];
+    const double cc = (((-e) - w) - n) - s;
+    return cc * z + e * mg_z(g, L0, i + 1, j, inv) + w * mg_z(g, L0, i - 1, j, inv) +
+           n * mg_z(g, L0, i, j + 1, inv) + s * mg_z(g, L0, i, j - 1, inv);
 }
 
 __device__ __forceinline__ double mg_area(const double* dxu, const double* cosu,
                                           const double* dyu, int i, int j) {
     return dxu[i + 2] * cosu[j + 2] * dyu[j + 2];
 }
+
+constexpr int MG_OUTER_BLOCKS = 2048;   // workgroups of the outer passes (grid-stride)
 
 template <int NR>
 __device__ __forceinline__ void mg_block_partials(double (&v)[NR], double* part) {
@@ -245,35 +331,36 @@ __device__ __forceinline__ void mg_block_partials(double (&v)[NR], double* part)
                                                        sh[k * 4 + 3];
 }
 
-// Pass A: with z = M r from the V-cycle, the weighted products
-// <Az,p>, <z,Az>, <z,q>, <r,z>, <r,p> over active points.
+// Pass A: with z = M r from the V-cycle, Az (kept for pass B) and the
+// weighted products <z,Az>, <z,q>, <r,z> over active points.  Flexible CG
+// also uses <Az,p> and <r,p>; the first equals <z,q> (W A is symmetric and
+// q = A p) and the second vanishes by the previous line search, so p is not
+// read here.
 __global__ void __launch_bounds__(256) k_mg_fcg_a(
     Geom g, MgLev L0, const double* __restrict__ E, const double* __restrict__ N,
-    const double* __restrict__ cC, const double* __restrict__ rx, const double* __restrict__ ry,
+    const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ dxu, const double* __restrict__ cosu,
     const double* __restrict__ dyu, const uint8_t* __restrict__ act,
-    const double* __restrict__ p, const double* __restrict__ q, const double* __restrict__ r,
-    const double* __restrict__ zscale, const MgScal* __restrict__ S, double* __restrict__ part) {
+    const double* __restrict__ q, const double* __restrict__ r, double* __restrict__ az_out,
+    const double* __restrict__ zscale, const MgScal* __restrict__ S,
+    double* __restrict__ part) {
     if (S->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[5] = {0, 0, 0, 0, 0};
-    if (t < g.nx * g.ny) {
+    const double inv = 1.0 / *zscale;   // the scale of the V-cycle's input
+    double acc[3] = {0, 0, 0};
+    const int n = g.nx * g.ny;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const int i = t % g.nx, j = t / g.nx;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
-        if (act[o]) {
-            const double inv = 1.0 / *zscale;   // the scale of the V-cycle's input
-            double z;
-            const double az = mg_Az(g, L0, E, N, cC, rx, ry, i, j, o, inv, z);
-            const double w = mg_area(dxu, cosu, dyu, i, j);
-            const double pv = p[o], qv = q[o], rv = r[o];
-            acc[0] = w * az * pv;
-            acc[1] = w * z * az;
-            acc[2] = w * z * qv;
-            acc[3] = w * rv * z;
-            acc[4] = w * rv * pv;
-        }
+        if (!act[o]) continue;
+        double z;
+        const double az = mg_Az(g, L0, E, N, rx, ry, i, j, o, inv, z);
+        az_out[o] = az;
+        const double w = mg_area(dxu, cosu, dyu, i, j);
+        acc[0] += w * z * az;
+        acc[1] += w * z * q[o];
+        acc[2] += w * r[o] * z;
     }
-    mg_block_partials<5>(acc, part);
+    mg_block_partials<3>(acc, part);
 }
 
 // Fold of a pass's partials by one workgroup (fixed order).
@@ -301,17 +388,17 @@ __device__ __forceinline__ void mg_fold(const double* __restrict__ part, int nbl
         }
 }
 
-// beta = -<z,q>/<p,q>_old, <p,q> = <z,Az> + beta(<Az,p> + <z,q>) + beta^2 <p,q>_old,
-// alpha = (<r,z> + beta <r,p>) / <p,q>.
+// beta = -<z,q>/<p,q>_old, <p,q> = <z,Az> + 2 beta <z,q> + beta^2 <p,q>_old,
+// alpha = <r,z> / <p,q>.
 __global__ void __launch_bounds__(1024) k_mg_fold_a(const double* __restrict__ part, int nblocks,
                                                     MgScal* S) {
     if (S->done) return;
-    double v[5];
-    mg_fold<5>(part, nblocks, v);
+    double v[3];
+    mg_fold<3>(part, nblocks, v);
     if (threadIdx.x == 0) {
-        const double beta = S->pq_old != 0.0 ? -v[2] / S->pq_old : 0.0;
-        const double pq = v[1] + beta * (v[0] + v[2]) + beta * beta * S->pq_old;
-        const double alpha = (v[3] + beta * v[4]) / pq;
+        const double beta = S->pq_old != 0.0 ? -v[1] / S->pq_old : 0.0;
+        const double pq = v[0] + 2.0 * beta * v[1] + beta * beta * S->pq_old;
+        const double alpha = v[2] / pq;
         S->beta = beta;
         S->pq = pq;
         S->alpha = alpha;
@@ -324,42 +411,38 @@ __global__ void __launch_bounds__(1024) k_mg_fold_a(const double* __restrict__ p
 }
 
 // Pass B: p = z + beta p, q = Az + beta q, x += alpha p, r -= alpha q; the
-// V-cycle's next input w r * s_r on level 0; |r|^2 (active) and |w r|^2.
+// V-cycle's next input w r s_r on level 0; |r|^2 (active) and |w r|^2.
 __global__ void __launch_bounds__(256) k_mg_fcg_b(
-    Geom g, MgLev L0, const double* __restrict__ E, const double* __restrict__ N,
-    const double* __restrict__ cC, const double* __restrict__ rx, const double* __restrict__ ry,
-    const double* __restrict__ dxu, const double* __restrict__ cosu,
+    Geom g, MgLev L0, const double* __restrict__ dxu, const double* __restrict__ cosu,
     const double* __restrict__ dyu, const uint8_t* __restrict__ act, double* __restrict__ p,
     double* __restrict__ q, double* __restrict__ r, double* __restrict__ x,
-    const double* __restrict__ zscale_src, const MgScal* __restrict__ S,
-    double* __restrict__ part) {
+    const double* __restrict__ az_in, const double* __restrict__ zscale,
+    const MgScal* __restrict__ S, double* __restrict__ part) {
     if (S->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const double inv = 1.0 / *zscale;
+    const double beta = S->beta, alpha = S->alpha, sr = S->s_r;
     double acc[2] = {0, 0};
-    if (t < g.nx * g.ny) {
+    const int n = g.nx * g.ny;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const int i = t % g.nx, j = t / g.nx;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
         const int ql = (j + 1) * L0.pitch + (i + 1);
-        if (act[o]) {
-            // z was scaled by the s_r in force before fold_a replaced it
-            const double inv = 1.0 / *zscale_src;
-            double z;
-            const double az = mg_Az(g, L0, E, N, cC, rx, ry, i, j, o, inv, z);
-            const double beta = S->beta, alpha = S->alpha;
-            const double pn = z + beta * p[o];
-            const double qn = az + beta * q[o];
-            p[o] = pn;
-            q[o] = qn;
-            x[o] = x[o] + alpha * pn;
-            const double rn = r[o] - alpha * qn;
-            r[o] = rn;
-            const double w = mg_area(dxu, cosu, dyu, i, j);
-            L0.b[ql] = (float)(w * rn * S->s_r);
-            acc[0] = rn * rn;
-            acc[1] = (w * rn) * (w * rn);
-        } else {
+        if (!act[o]) {
             L0.b[ql] = 0.0f;
+            continue;
         }
+        const double z = (double)L0.y[ql] * inv;
+        const double pn = z + beta * p[o];
+        const double qn = az_in[o] + beta * q[o];
+        p[o] = pn;
+        q[o] = qn;
+        x[o] = x[o] + alpha * pn;
+        const double rn = r[o] - alpha * qn;
+        r[o] = rn;
+        const double w = mg_area(dxu, cosu, dyu, i, j);
+        L0.b[ql] = (float)(w * rn * sr);
+        acc[0] += rn * rn;
+        acc[1] += (w * rn) * (w * rn);
     }
     mg_block_partials<2>(acc, part);
 }
@@ -527,8 +610,10 @@ struct MgState {
     int* cells = nullptr;
     int nact0 = 0;
     double w_mean = 1.0;             // mean psi-cell area over active points
-    double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
+    double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *az = nullptr;
     MgScal* S = nullptr;
+    MgScal* S_time = nullptr;        // scalars of psi_mg_kernel_times (never done)
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     MgScal* h_S = nullptr;           // pinned
     double* zscale = nullptr;
     double* part = nullptr;

@@ -240,6 +240,14 @@ class StreamfunctionSolver:
                                          cap))
         return [(int(nx[k]), int(ny[k]), int(na[k])) for k in range(min(int(n[0]), cap))]
 
+    def mg_kernel_times(self, degree=0, reps=20):
+        """Microseconds per launch of the multigrid path's heaviest kernels:
+        {post_level0, pre_level0, fcg_a, fcg_b} (HIP events, between solves)."""
+        us = np.zeros(4)
+        self._sync_in()
+        self._check(self.L.psi_mg_kernel_times(self.ctx, degree, reps, _f64p(us), 4))
+        return dict(zip(("post_level0", "pre_level0", "fcg_a", "fcg_b"), us.tolist()))
+
     # -- building blocks -------------------------------------------------------
     def apply_op(self, p: torch.Tensor, res: torch.Tensor | None = None):
         """apply_op (solve_pressure.f90:254-274): res = A p."""
