@@ -87,7 +87,7 @@ def parse():
                     help="mixed path: residual reduction per FP32 leg (0: library default)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--nz", type=int, default=40, help="levels of the step workload")
-    ap.add_argument("--step-mode", default="poly", choices=["poly", "mixed", "plain"],
+    ap.add_argument("--step-mode", default="mg", choices=["mg", "poly", "mixed", "plain"],
                     help="solve path of the step workload")
     ap.add_argument("--workload", default="solve", choices=["solve", "ensemble", "step"],
                     help="solve: the 2048^2 two-island solve (headline); ensemble: 512 "

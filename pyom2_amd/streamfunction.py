@@ -214,9 +214,7 @@ class StreamfunctionSolver:
         (iterations, converged, info dict).  Non-convergence is reported,
         not raised (the reference prints and continues); NaN raises
         (the reference halts)."""
-        opts = opts or SolveOptions(**kw)
-        if opts.mode == "mg" and not self._mg_ready:
-            self.mg_setup()
+        opts = self._prepare(opts or SolveOptions(**kw))
         info = PsiInfo()
         self._sync_in()
         rc = self.L.psi_congrad(self.ctx, C.byref(opts.as_c()), _dev(forc, self.shape),
@@ -224,6 +222,12 @@ class StreamfunctionSolver:
         self._check(rc, (_lib.PSI_OK, _lib.PSI_NOT_CONVERGED, _lib.PSI_DIVERGED))
         d = info.as_dict()
         return d["iterations"], bool(d["converged"]), d
+
+    def _prepare(self, opts: "SolveOptions"):
+        # the multigrid path needs its hierarchy (built once per operator)
+        if opts.mode == "mg" and not self._mg_ready:
+            self.mg_setup()
+        return opts
 
     def mg_setup(self):
         """Build the multigrid hierarchy (needed by mode "mg"; done on first use)."""
@@ -314,7 +318,7 @@ class StreamfunctionSolver:
         """Solver part of streamfunction_init (streamfct_init.f90:269-303).
         Returns (psin (nisle, ny+4, nx+4) tensor, line_psin (nisle, nisle)
         numpy [n, isle], iterations list)."""
-        opts = opts or SolveOptions(**kw)
+        opts = self._prepare(opts or SolveOptions(**kw))
         n = self.nisle
         psin = torch.zeros((max(n, 1),) + self.shape, dtype=torch.float64, device=self.device)
         line_psin = np.zeros((max(n, 1), max(n, 1)))
@@ -329,7 +333,7 @@ class StreamfunctionSolver:
                              opts: SolveOptions | None = None, forc_out=None, **kw):
         """The 2-D body of solve_streamfunction (solve_streamfct.f90:39-94):
         curl forcing, warm start, solve, island constraint, AB step of psi."""
-        opts = opts or SolveOptions(**kw)
+        opts = self._prepare(opts or SolveOptions(**kw))
         info = PsiInfo()
         dpsin = np.ascontiguousarray(state.dpsin)
         lp = np.ascontiguousarray(state.line_psin, np.float64) if self.nisle > 1 else np.zeros(1)
@@ -384,7 +388,7 @@ class StreamfunctionSolver:
         no open boundaries) on the model state: p_hydro, du/dv(tau) updated,
         psi/dpsi(taup1), dpsin(tau), u/v(taup1).  Returns
         (iterations, converged, info) like solve_streamfunction."""
-        opts = opts or SolveOptions(**kw)
+        opts = self._prepare(opts or SolveOptions(**kw))
         if mom.nz != lev.nz:
             raise ValueError("momentum state and levels differ in nz")
         info = PsiInfo()
