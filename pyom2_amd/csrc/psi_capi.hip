@@ -897,7 +897,7 @@ int mg_setup(psi_ctx* c) {
         MgLev d{};
         d.nx = h.nx;
         d.ny = h.ny;
-        d.pitch = h.nx + 2;
+        d.pitch = mg_pitch(h.nx);
         d.cyc = h.cyc;
         const size_t np = (size_t)d.pitch * (h.ny + 2);
         std::vector<float> hE(np, 0.0f), hN(np, 0.0f), hC(np, 1.0f);
@@ -905,7 +905,7 @@ int mg_setup(psi_ctx* c) {
         for (int j = 0; j < h.ny; ++j)
             for (int i = 0; i < h.nx; ++i) {
                 const size_t p = (size_t)i + (size_t)j * h.nx;
-                const size_t q = (size_t)(j + 1) * d.pitch + (i + 1);
+                const size_t q = (size_t)(j + 1) * d.pitch + i + MG_XOFF;
                 hE[q] = (float)h.E[p];
                 hN[q] = (float)h.N[p];
                 hC[q] = (float)h.C[p];
@@ -964,18 +964,19 @@ int mg_setup(psi_ctx* c) {
 template <int K>
 void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch) {
     const int L = m->L;
+    using F = MgFrame<K>;
     auto tiles = [](const MgLev& v) {
-        return dim3((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + MT_Y - 1) / MT_Y)));
+        return dim3((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + F::TY - 1) / F::TY)));
     };
     for (int l = 0; l < L; ++l)
-        hipLaunchKernelGGL(k_mg_pre<K>, tiles(m->lev[l]), dim3(MT_NT), 0, s, m->lev[l], ch,
+        hipLaunchKernelGGL(k_mg_pre<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l], ch,
                            m->lev[l + 1], (const MgScal*)m->S);
     if (m->ncoarse > 0)
         hipLaunchKernelGGL(k_mg_dense, dim3((m->ncoarse + 3) / 4), dim3(256), 0, s, m->lev[L],
                            (const float*)m->Ainv, (const int*)m->cells, m->ncoarse,
                            (const MgScal*)m->S);
     for (int l = L - 1; l >= 0; --l)
-        hipLaunchKernelGGL(k_mg_post<K>, tiles(m->lev[l]), dim3(MT_NT), 0, s, m->lev[l],
+        hipLaunchKernelGGL(k_mg_post<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l],
                            m->lev[l + 1], ch, (const MgScal*)m->S);
 }
 
@@ -1047,13 +1048,14 @@ template <int K>
 void mg_time_levels(MgState* m, int reps, double* us) {
     const MgCheb ch = mg_cheb(K, 15.0);
     const MgLev& v = m->lev[0];
-    const dim3 tiles((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + MT_Y - 1) / MT_Y)));
+    using F = MgFrame<K>;
+    const dim3 tiles((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + F::TY - 1) / F::TY)));
     const MgScal* S = m->S_time;
     us[0] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(k_mg_post<K>, tiles, dim3(MT_NT), 0, m->gs, v, m->lev[1], ch, S);
+        hipLaunchKernelGGL(k_mg_post<K>, tiles, dim3(F::NT), 0, m->gs, v, m->lev[1], ch, S);
     });
     us[1] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(k_mg_pre<K>, tiles, dim3(MT_NT), 0, m->gs, v, ch, m->lev[1], S);
+        hipLaunchKernelGGL(k_mg_pre<K>, tiles, dim3(F::NT), 0, m->gs, v, ch, m->lev[1], S);
     });
 }
 

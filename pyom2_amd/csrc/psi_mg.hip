@@ -39,8 +39,16 @@ constexpr int MG_MAXK = 6;           // Chebyshev steps per smoothing at most
 constexpr float MG_SCALE = 1.7f;     // weight of the coarse-grid correction
 constexpr int MG_NRED = 3;           // reduction slots of the outer passes
 
+// Level layout: cell (I, J) at (J + 1) * pitch + I + MG_XOFF, one halo row
+// above and below, MG_XOFF halo columns on the left (so the LDS frames of
+// the smoothing kernels, which start MG_XOFF cells left of a 64-cell tile
+// boundary at the default degree, begin on a 128-byte line), pitch a
+// multiple of 32 floats with at least 4 halo columns on the right.
+constexpr int MG_XOFF = 4;
+inline int mg_pitch(int nx) { return ((nx + MG_XOFF + 4 + 31) / 32) * 32; }
+
 struct MgLev {
-    int nx, ny, pitch, cyc;          // interior cells; pitch = nx + 2
+    int nx, ny, pitch, cyc;          // interior cells; pitch = mg_pitch(nx)
     const float *E, *N, *C;
     const uint8_t* act;
     float *b;                        // right-hand side (restricted residual)
@@ -77,16 +85,25 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
 }
 
 // ---- fused level kernels ------------------------------------------------------
-// One workgroup per 64 x 16 tile of a level with a K-cell frame around it in
+// One workgroup per 64 x TY tile of a level with a K-cell frame around it in
 // LDS: the K smoothing steps recompute the frame's shrinking margins instead
 // of writing every step to memory, so a level costs two kernels per V-cycle
 // (pre-smoothing + restriction, prolongation + post-smoothing).
-constexpr int MT_X = 64, MT_Y = 16, MT_NT = 256;
+// Tile MT_X x MT_Y with one thread per tile cell: each thread carries about
+// two frame cells per step (tile shape settable at build time).
+#ifndef PSI_MG_TX
+#define PSI_MG_TX 64
+#endif
+#ifndef PSI_MG_TY
+#define PSI_MG_TY 16
+#endif
+constexpr int MT_X = PSI_MG_TX;
 
 template <int K>
 struct MgFrame {
-    static constexpr int FX = MT_X + 2 * K, FY = MT_Y + 2 * K, FN = FX * FY;
-    static constexpr int PER = (FN + MT_NT - 1) / MT_NT;
+    static constexpr int TY = PSI_MG_TY, NT = MT_X * TY;
+    static constexpr int FX = MT_X + 2 * K, FY = TY + 2 * K, FN = FX * FY;
+    static constexpr int PER = (FN + NT - 1) / NT;
 };
 
 // Level cell of frame cell (fx, fy) of the tile at (I0, J0); -1 outside the
@@ -97,7 +114,7 @@ __device__ __forceinline__ int mg_frame_cell(const MgLev& L, int I, int J) {
         if (!L.cyc) return -1;
         I = (I % L.nx + L.nx) % L.nx;
     }
-    return (J + 1) * L.pitch + (I + 1);
+    return (J + 1) * L.pitch + I + MG_XOFF;
 }
 
 // S v at frame index f (row stride FX)
@@ -114,18 +131,18 @@ __device__ __forceinline__ float mg_fs(const float* v, const float* sE, const fl
 // the next step reads, goes through LDS (double-buffered: one barrier per
 // step).
 template <int K>
-__global__ void __launch_bounds__(MT_NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
+__global__ void __launch_bounds__(MgFrame<K>::NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
                                                   const MgScal* __restrict__ S) {
     using F = MgFrame<K>;
-    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER;
+    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
     __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
     const int done = S->done;   // read beside the frame loads, checked before any store
     const int tiles_x = (L.nx + MT_X - 1) / MT_X;
-    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * MT_Y;
+    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * TY;
     float r[PER], x[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * MT_NT;
+        const int f = threadIdx.x + u * NT;
         r[u] = 0.0f;
         x[u] = 0.0f;
         if (f < FN) {
@@ -150,7 +167,7 @@ __global__ void __launch_bounds__(MT_NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
         float* dout = sD[k & 1];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int f = threadIdx.x + u * MT_NT;
+            const int f = threadIdx.x + u * NT;
             const int fy = f / FX, fx = f - fy * FX;
             if (f < FN && fx >= k && fx < FX - k && fy >= k && fy < FY - k) {
                 const float rn = r[u] - mg_fs<FX>(din, sE, sN, sC, f);
@@ -169,17 +186,17 @@ __global__ void __launch_bounds__(MT_NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
     float* sRes = sE;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * MT_NT;
+        const int f = threadIdx.x + u * NT;
         const int fy = f / FX, fx = f - fy * FX;
         if (f < FN && fx >= K && fx < FX - K && fy >= K && fy < FY - K) {
             sRes[f] = r[u];
             const int I = I0 + fx - K, J = J0 + fy - K;
-            if (I < L.nx && J < L.ny) L.x[(J + 1) * L.pitch + (I + 1)] = x[u];
+            if (I < L.nx && J < L.ny) L.x[(J + 1) * L.pitch + I + MG_XOFF] = x[u];
         }
     }
     __syncthreads();
-    {
-        const int l = threadIdx.x;   // one coarse cell per thread: 32 x 8
+    if (threadIdx.x < (MT_X / 2) * (TY / 2)) {
+        const int l = threadIdx.x;   // one coarse cell per thread
         const int cy = l / (MT_X / 2), cx = l - cy * (MT_X / 2);
         const int IC = I0 / 2 + cx, JC = J0 / 2 + cy;
         if (IC < Lc.nx && JC < Lc.ny) {
@@ -189,7 +206,7 @@ __global__ void __launch_bounds__(MT_NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
                     const int tx = 2 * cx + di, ty = 2 * cy + dj;
                     if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sRes[(ty + K) * FX + tx + K];
                 }
-            const int Q = (JC + 1) * Lc.pitch + (IC + 1);
+            const int Q = (JC + 1) * Lc.pitch + IC + MG_XOFF;
             Lc.b[Q] = Lc.act[Q] ? sum : 0.0f;
         }
     }
@@ -200,18 +217,18 @@ __global__ void __launch_bounds__(MT_NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
 // with its neighbours once) and then d go through LDS, r and y stay in
 // registers.
 template <int K>
-__global__ void __launch_bounds__(MT_NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
+__global__ void __launch_bounds__(MgFrame<K>::NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
                                                    const MgScal* __restrict__ S) {
     using F = MgFrame<K>;
-    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER;
+    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
     __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
     const int done = S->done;
     const int tiles_x = (L.nx + MT_X - 1) / MT_X;
-    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * MT_Y;
+    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * TY;
     float r[PER], y[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * MT_NT;
+        const int f = threadIdx.x + u * NT;
         r[u] = 0.0f;
         y[u] = 0.0f;
         if (f < FN) {
@@ -222,7 +239,7 @@ __global__ void __launch_bounds__(MT_NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
             float yv = 0.0f;
             if (q >= 0 && L.act[q]) {
                 if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
-                yv = L.x[q] + MG_SCALE * Lc.y[(J / 2 + 1) * Lc.pitch + (I / 2 + 1)];
+                yv = L.x[q] + MG_SCALE * Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
             }
             sE[f] = q >= 0 ? L.E[q] : 0.0f;
             sN[f] = q >= 0 ? L.N[q] : 0.0f;
@@ -237,7 +254,7 @@ __global__ void __launch_bounds__(MT_NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
     // step 0 (margin >= 1): r = b - S y', d0 = D^-1 r / theta, y = y' + d0
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * MT_NT;
+        const int f = threadIdx.x + u * NT;
         const int fy = f / FX, fx = f - fy * FX;
         if (f < FN && fx >= 1 && fx < FX - 1 && fy >= 1 && fy < FY - 1) {
             const float rn = r[u] - mg_fs<FX>(sD[1], sE, sN, sC, f);
@@ -255,7 +272,7 @@ __global__ void __launch_bounds__(MT_NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
         float* dout = sD[k & 1];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int f = threadIdx.x + u * MT_NT;
+            const int f = threadIdx.x + u * NT;
             const int fy = f / FX, fx = f - fy * FX;
             if (f < FN && fx >= m && fx < FX - m && fy >= m && fy < FY - m) {
                 const float rn = r[u] - mg_fs<FX>(din, sE, sN, sC, f);
@@ -269,11 +286,11 @@ __global__ void __launch_bounds__(MT_NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * MT_NT;
+        const int f = threadIdx.x + u * NT;
         const int fy = f / FX, fx = f - fy * FX;
         if (f < FN && fx >= K && fx < FX - K && fy >= K && fy < FY - K) {
             const int I = I0 + fx - K, J = J0 + fy - K;
-            if (I < L.nx && J < L.ny) L.y[(J + 1) * L.pitch + (I + 1)] = y[u];
+            if (I < L.nx && J < L.ny) L.y[(J + 1) * L.pitch + I + MG_XOFF] = y[u];
         }
     }
 }
@@ -286,7 +303,7 @@ __device__ __forceinline__ double mg_z(const Geom& g, const MgLev& L0, int i, in
         if (!g.cyclic) return 0.0;
         i = (i + g.nx) % g.nx;
     }
-    return (double)L0.y[(j + 1) * L0.pitch + (i + 1)] * inv;
+    return (double)L0.y[(j + 1) * L0.pitch + i + MG_XOFF] * inv;
 }
 
 // A z at interior point (i,j) (ctx index o) with the stored operator in
@@ -426,7 +443,7 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const int i = t % g.nx, j = t / g.nx;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
-        const int ql = (j + 1) * L0.pitch + (i + 1);
+        const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
         if (!act[o]) {
             L0.b[ql] = 0.0f;
             continue;
@@ -485,7 +502,7 @@ __global__ void __launch_bounds__(256) k_mg_start(Geom g, MgLev L0, const double
     if (t >= g.nx * g.ny) return;
     const int i = t % g.nx, j = t / g.nx;
     const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
-    const int ql = (j + 1) * L0.pitch + (i + 1);
+    const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
     L0.b[ql] = act[o] ? (float)(mg_area(dxu, cosu, dyu, i, j) * r[o] * s) : 0.0f;
 }
 
@@ -544,7 +561,7 @@ inline bool mg_dense_inverse(const MgHostLev& L, std::vector<int>& cells,
         for (int i = 0; i < L.nx; ++i)
             if (L.act[(size_t)i + (size_t)j * L.nx]) {
                 id[(size_t)i + (size_t)j * L.nx] = (int)cells.size();
-                cells.push_back((j + 1) * (L.nx + 2) + (i + 1));   // level layout index
+                cells.push_back((j + 1) * mg_pitch(L.nx) + i + MG_XOFF);   // level layout index
             }
     const int n = (int)cells.size();
     std::vector<double> M((size_t)n * n, 0.0), X((size_t)n * n, 0.0);
