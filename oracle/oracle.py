@@ -386,3 +386,87 @@ def solve_streamfunction(b, init, col, mom, dpsin, taum1, tau, taup1, dt, ab, gr
     mom.u[taup1] = u
     mom.v[taup1] = v
     return p, r
+
+
+# ---- the surface-pressure solver (solve_pressure.f90) -------------------------
+def _press_lib():
+    L = lib()
+    if not getattr(L, "_press_set", False):
+        I, D = C.c_int, C.c_double
+        L.orc_make_coeff_surf_press.argtypes = [I, I] + [_dp] * 9 + [I, D, D, _dp]
+        L.orc_make_coeff_surf_press.restype = None
+        L.orc_congrad_surf_press.argtypes = [I, I, I, _dp, _dp, _dp, _dp, D, I, C.POINTER(I),
+                                             C.POINTER(I), C.POINTER(D)]
+        L.orc_congrad_surf_press.restype = I
+        L.orc_pressure_forcing.argtypes = [I, I, I, I, D, D] + [_dp] * 17 + [I, D, _dp, _dp, _dp]
+        L.orc_pressure_forcing.restype = None
+        L.orc_pressure_velocity.argtypes = [I, I, I, D] + [_dp] * 8
+        L.orc_pressure_velocity.restype = None
+        L._press_set = True
+    return L
+
+
+def make_coeff_surf_press(b, hu, hv, maskT_top, free_surface, grav, dt_mom):
+    """make_coeff_surf_press (solve_pressure.f90:87-152): 9 planes."""
+    cf = np.zeros((9,) + b.shape)
+    _press_lib().orc_make_coeff_surf_press(b.nx, b.ny, _c(hu), _c(hv), _c(maskT_top), _c(b.dxt),
+                                           _c(b.dxu), _c(b.dyt), _c(b.dyu), _c(b.cost),
+                                           _c(b.cosu), int(free_surface), grav, dt_mom, cf)
+    return cf
+
+
+def congrad_surf_press(b, cf, maskT_top, forc, psi, eps=1e-12, max_iter=1000):
+    """congrad_surf_press (solve_pressure.f90:157-249); psi is copied."""
+    s = _c(psi).copy()
+    it, cv, est = C.c_int(), C.c_int(), C.c_double()
+    st = _press_lib().orc_congrad_surf_press(b.nx, b.ny, int(b.cyclic_x), _c(cf), _c(maskT_top),
+                                             _c(forc), s, float(eps), int(max_iter),
+                                             C.byref(it), C.byref(cv), C.byref(est))
+    return CGResult(s, it.value, bool(cv.value), est.value, st, None)
+
+
+def pressure_forcing(b, col, dt, ab, u_tau, v_tau, du_tau, dv_tau, du_taum1, dv_taum1, du_mix,
+                     dv_mix, psi_tau, free_surface, grav):
+    """AB step of u, v to taup1, depth-integrated transport, its divergence
+    and the free-surface term (solve_pressure.f90:31-58).  Returns
+    (u_taup1, v_taup1, forc)."""
+    u = np.zeros(col.shape)
+    v = np.zeros(col.shape)
+    forc = np.zeros(b.shape)
+    _press_lib().orc_pressure_forcing(
+        b.nx, b.ny, col.nz, int(b.cyclic_x), dt, ab, _c(b.dxt), _c(b.dyt), _c(b.cost),
+        _c(b.cosu), _c(col.dzt), _c(col.maskT), _c(col.maskU), _c(col.maskV), _c(u_tau),
+        _c(v_tau), _c(du_tau), _c(dv_tau), _c(du_taum1), _c(dv_taum1), _c(du_mix), _c(dv_mix),
+        _c(psi_tau), int(free_surface), grav, u, v, forc)
+    return u, v, forc
+
+
+def pressure_velocity(b, col, dt, psi_taup1, u_taup1, v_taup1):
+    """u, v -= dt grad psi(taup1) on the interior (solve_pressure.f90:64-70);
+    returns updated copies."""
+    u = _c(u_taup1).copy()
+    v = _c(v_taup1).copy()
+    _press_lib().orc_pressure_velocity(b.nx, b.ny, col.nz, dt, _c(b.dxu), _c(b.dyu), _c(b.cost),
+                                       _c(col.maskU), _c(col.maskV), _c(psi_taup1), u, v)
+    return u, v
+
+
+def solve_pressure(b, col, hu, hv, mom, taum1, tau, taup1, dt, ab, grav, rho_0, free_surface,
+                   eps=1e-12, max_iter=3000, cf=None):
+    """The whole solve_pressure (solve_pressure.f90:9-72) on a basins.Momentum
+    state (psi holds the surface pressure), updated in place.  Returns
+    (p_hydro, congrad result)."""
+    p, du, dv, _, _ = streamfunction_forcing(b, col, mom.rho[tau], mom.du[tau], mom.dv[tau],
+                                             mom.du_mix, mom.dv_mix, grav / rho_0)
+    mom.du[tau] = du
+    mom.dv[tau] = dv
+    u, v, forc = pressure_forcing(b, col, dt, ab, mom.u[tau], mom.v[tau], du, dv, mom.du[taum1],
+                                  mom.dv[taum1], mom.du_mix, mom.dv_mix, mom.psi[tau],
+                                  free_surface, grav)
+    mom.psi[taup1] = 2 * mom.psi[tau] - mom.psi[taum1]
+    if cf is None:
+        cf = make_coeff_surf_press(b, hu, hv, col.maskT[-1], free_surface, grav, dt)
+    r = congrad_surf_press(b, cf, col.maskT[-1], forc, mom.psi[taup1], eps, max_iter)
+    mom.psi[taup1] = setcyclic(b.nx, b.ny, b.cyclic_x, r.sol)
+    mom.u[taup1], mom.v[taup1] = pressure_velocity(b, col, dt, mom.psi[taup1], u, v)
+    return p, r

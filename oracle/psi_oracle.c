@@ -597,3 +597,213 @@ void orc_velocity_update(int nx, int ny, int nz, double dt_mom, double AB_eps,
     free(fpx);
     free(fpy);
 }
+
+/* ---------------------------------------------------------------------- */
+/* The surface-pressure solver (for_src/external/solve_pressure.f90), the
+ * external mode used instead of the streamfunction when
+ * enable_streamfunction is off (momentum.f90:158-166).  maskT_top is
+ * maskT(:,:,nz), the mask every routine below uses (maskM, :102). */
+
+/* make_coeff_surf_press (solve_pressure.f90:87-152): 5-point coefficients
+ * of 1/cos^2 ((h p_x)_x + cos (cos h p_y)_y) with the mask products of
+ * neighbouring T cells, plus -1/(grav dt^2) on ocean points with a free
+ * surface (:139-145).  Planes as in orc_make_coeff. */
+void orc_make_coeff_surf_press(int nx, int ny, const double *hu, const double *hv,
+                               const double *maskT_top, const double *dxt, const double *dxu,
+                               const double *dyt, const double *dyu, const double *cost,
+                               const double *cosu, int free_surface, double grav, double dt_mom,
+                               double *cf)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    double *c00 = cf + 4 * plane, *cE = cf + 5 * plane, *cW = cf + 3 * plane;
+    double *cN = cf + 7 * plane, *cS = cf + 1 * plane;
+    memset(cf, 0, 9 * plane * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const size_t q = IX(i, j, nx);
+            const int I = i + 1, J = j + 1;   /* 1-D metric index of Fortran i, j */
+            double mp = maskT_top[q] * maskT_top[IX(i + 1, j, nx)];
+            double mm = maskT_top[q] * maskT_top[IX(i - 1, j, nx)];
+            const double c2 = cost[J] * cost[J];
+            c00[q] = c00[q] - mp * hu[q] / dxu[I] / dxt[I] / c2;
+            cE[q] = cE[q] + mp * hu[q] / dxu[I] / dxt[I] / c2;
+            c00[q] = c00[q] - mm * hu[IX(i - 1, j, nx)] / dxu[I - 1] / dxt[I] / c2;
+            cW[q] = cW[q] + mm * hu[IX(i - 1, j, nx)] / dxu[I - 1] / dxt[I] / c2;
+            mp = maskT_top[q] * maskT_top[IX(i, j + 1, nx)];
+            mm = maskT_top[q] * maskT_top[IX(i, j - 1, nx)];
+            c00[q] = c00[q] - mp * hv[q] / dyu[J] / dyt[J] * cosu[J] / cost[J];
+            cN[q] = cN[q] + mp * hv[q] / dyu[J] / dyt[J] * cosu[J] / cost[J];
+            c00[q] = c00[q] - mm * hv[IX(i, j - 1, nx)] / dyu[J - 1] / dyt[J] * cosu[J - 1] / cost[J];
+            cS[q] = cS[q] + mm * hv[IX(i, j - 1, nx)] / dyu[J - 1] / dyt[J] * cosu[J - 1] / cost[J];
+        }
+    if (free_surface)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = IX(i, j, nx);
+                c00[q] = c00[q] - 1.0 / (grav * (dt_mom * dt_mom)) * maskT_top[q];
+            }
+}
+
+/* dot_sfp / absmax_sfp (solve_pressure.f90:277-312): masked by maskT(nz). */
+static double dot_sfp(int nx, int ny, const double *m, const double *a, const double *b)
+{
+    double s2 = 0.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            s2 = s2 + a[IX(i, j, nx)] * b[IX(i, j, nx)] * m[IX(i, j, nx)];
+    return s2;
+}
+
+static double absmax_sfp(int nx, int ny, const double *m, const double *a)
+{
+    double s2 = 0.0;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const double v = fabs(a[IX(i, j, nx)] * m[IX(i, j, nx)]);
+            s2 = (v > s2) ? v : s2;
+        }
+    return s2;
+}
+
+/* congrad_surf_press (solve_pressure.f90:157-249): conjugate gradients
+ * without preconditioner on A psi = forc, psi (= psi(:,:,taup1)) the first
+ * guess on entry.  Tests per iteration in the reference's order: p.Ap = 0
+ * (normal end, :196), divergence of r.r (:206-215), the extrapolated step
+ * with max|p_new| (:219-229), NaN (:233-236).  Returns 0 (normal end),
+ * 1 (diverged or iteration cap), 2 (NaN estimate). */
+int orc_congrad_surf_press(int nx, int ny, int cyclic_x, const double *cf, const double *maskT_top,
+                           const double *forc, double *psi, double congr_epsilon,
+                           int congr_max_iterations, int *iterations, int *converged,
+                           double *est_error)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    double *res = calloc(plane, sizeof(double)), *p = calloc(plane, sizeof(double));
+    double *Ap = calloc(plane, sizeof(double));
+    double rsold, rsnew, alpha, fxa, step, step1 = 0.0, rs_min = 0.0, smax;
+    double estimated_error = 0.0, convergence_rate;
+    int n, normal = 0;
+
+    orc_apply_op(nx, ny, cf, psi, res);
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) res[IX(i, j, nx)] = forc[IX(i, j, nx)] - res[IX(i, j, nx)];
+    memcpy(p, res, plane * sizeof(double));
+    orc_setcyclic(nx, ny, cyclic_x, p);
+    rsold = dot_sfp(nx, ny, maskT_top, res, res);
+    if (rsold == 0.0) {
+        n = 0;
+        normal = 1;
+        goto done;
+    }
+    for (n = 1; n <= congr_max_iterations; ++n) {
+        orc_apply_op(nx, ny, cf, p, Ap);
+        fxa = dot_sfp(nx, ny, maskT_top, p, Ap);
+        if (fxa == 0.0) { normal = 1; goto done; }
+        alpha = rsold / fxa;
+        for (size_t q = 0; q < plane; ++q) psi[q] = psi[q] + alpha * p[q];
+        for (size_t q = 0; q < plane; ++q) res[q] = res[q] - alpha * Ap[q];
+        rsnew = dot_sfp(nx, ny, maskT_top, res, res);
+        for (size_t q = 0; q < plane; ++q) p[q] = res[q] + rsnew / rsold * p[q];
+        orc_setcyclic(nx, ny, cyclic_x, p);
+        rsold = rsnew;
+        if (n == 1) {
+            rs_min = fabs(rsnew);
+        } else if (n > 2) {
+            rs_min = (rs_min < fabs(rsnew)) ? rs_min : fabs(rsnew);
+            if (fabs(rsnew) > 100.0 * rs_min) goto done;
+        }
+        smax = absmax_sfp(nx, ny, maskT_top, p);
+        step = fabs(alpha) * smax;
+        if (n == 1) {
+            step1 = step;
+            estimated_error = step;
+            if (step < congr_epsilon) { normal = 1; goto done; }
+        } else if (step < congr_epsilon) {
+            convergence_rate = exp(log(step / step1) / (double)(n - 1));
+            estimated_error = step * convergence_rate / (1.0 - convergence_rate);
+            if (estimated_error < congr_epsilon) { normal = 1; goto done; }
+        }
+        if (estimated_error != estimated_error) goto done;
+    }
+done:
+    *iterations = n;
+    *converged = normal;
+    *est_error = estimated_error;
+    free(res); free(p); free(Ap);
+    if (normal) return 0;
+    return (estimated_error != estimated_error) ? 2 : 1;
+}
+
+/* The parts of solve_pressure around the solve (solve_pressure.f90:31-58):
+ * Adams-Bashforth step of u, v to taup1 (whole array), the depth-integrated
+ * transport over dt (interior, bottom level first) with its cyclic halo, its
+ * divergence as the forcing, and the free-surface term.  du, dv are the tau
+ * levels after the hydrostatic pressure gradient (same as
+ * orc_streamfunction_forcing's, :19-28). */
+void orc_pressure_forcing(int nx, int ny, int nz, int cyclic_x, double dt_mom, double AB_eps,
+                          const double *dxt, const double *dyt, const double *cost,
+                          const double *cosu, const double *dzt, const double *maskT,
+                          const double *maskU, const double *maskV, const double *u_tau,
+                          const double *v_tau, const double *du_tau, const double *dv_tau,
+                          const double *du_taum1, const double *dv_taum1, const double *du_mix,
+                          const double *dv_mix, const double *psi_tau, int free_surface,
+                          double grav, double *u_taup1, double *v_taup1, double *forc)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const size_t n3 = (size_t)nz * plane;
+    const double *maskT_top = maskT + (size_t)(nz - 1) * plane;
+    for (size_t q = 0; q < n3; ++q) {
+        u_taup1[q] = u_tau[q] + dt_mom * (du_mix[q] + (1.5 + AB_eps) * du_tau[q] -
+                                          (0.5 + AB_eps) * du_taum1[q]) * maskU[q];
+        v_taup1[q] = v_tau[q] + dt_mom * (dv_mix[q] + (1.5 + AB_eps) * dv_tau[q] -
+                                          (0.5 + AB_eps) * dv_taum1[q]) * maskV[q];
+    }
+    double *fpx = calloc(plane, sizeof(double)), *fpy = calloc(plane, sizeof(double));
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = IX(i, j, nx);
+                fpx[q] = fpx[q] + u_taup1[o + q] * maskU[o + q] * dzt[k] / dt_mom;
+                fpy[q] = fpy[q] + v_taup1[o + q] * maskV[o + q] * dzt[k] / dt_mom;
+            }
+    }
+    orc_setcyclic(nx, ny, cyclic_x, fpx);
+    orc_setcyclic(nx, ny, cyclic_x, fpy);
+    memset(forc, 0, plane * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const size_t q = IX(i, j, nx);
+            forc[q] = (fpx[q] - fpx[IX(i - 1, j, nx)]) / (cost[j + 1] * dxt[i + 1]) +
+                      (cosu[j + 1] * fpy[q] - cosu[j] * fpy[IX(i, j - 1, nx)]) /
+                          (cost[j + 1] * dyt[j + 1]);
+        }
+    if (free_surface)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = IX(i, j, nx);
+                forc[q] = forc[q] - psi_tau[q] / (grav * (dt_mom * dt_mom)) * maskT_top[q];
+            }
+    free(fpx);
+    free(fpy);
+}
+
+/* The surface-pressure gradient removed from u, v at taup1 on the interior
+ * (solve_pressure.f90:64-70). */
+void orc_pressure_velocity(int nx, int ny, int nz, double dt_mom, const double *dxu,
+                           const double *dyu, const double *cost, const double *maskU,
+                           const double *maskV, const double *psi_taup1, double *u_taup1,
+                           double *v_taup1)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            const size_t p = IX(i, j, nx);
+            for (int k = 0; k < nz; ++k) {
+                const size_t q = (size_t)k * plane + p;
+                u_taup1[q] = u_taup1[q] - dt_mom * (psi_taup1[IX(i + 1, j, nx)] - psi_taup1[p]) /
+                                              (dxu[i + 1] * cost[j + 1]) * maskU[q];
+                v_taup1[q] = v_taup1[q] - dt_mom * (psi_taup1[IX(i, j + 1, nx)] - psi_taup1[p]) /
+                                              dyu[j + 1] * maskV[q];
+            }
+        }
+}

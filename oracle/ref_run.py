@@ -126,3 +126,46 @@ def run_step(b, col, mom, eps=1e-12, max_iter=1000, dt_mom=3600.0):
                               else (ny + 4, nx + 4))
     out.update(nisle=int(nisle), iterations=int(itts), log=p.stdout.decode(errors="replace"))
     return out
+
+
+PRESS = os.path.join(HERE, "_ref", "ref_press")
+
+
+def press_available() -> bool:
+    return os.path.exists(PRESS)
+
+
+def run_press(b, col, hu, hv, mom, free_surface, eps=1e-12, max_iter=1000, dt_mom=3600.0):
+    """Run the reference's solve_pressure on basin b with levels col, depths
+    hu/hv at u/v points and state mom (time levels taum1=0, tau=1, taup1=2;
+    psi holds the surface pressure).  Returns a dict of the outputs."""
+    nx, ny, nz = b.nx, b.ny, col.nz
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nx, ny, nz, int(b.cyclic_x), max_iter, int(free_surface)],
+                     np.int32).tofile(f)
+            np.array([eps, dt_mom], np.float64).tofile(f)
+            for a in (b.dxt, b.dxu, b.dyt, b.dyu, b.cost, b.cosu, col.dzt, col.dzw):
+                f64(a).tofile(f)
+            for a in (hu, hv, col.maskT, col.maskU, col.maskV, mom.rho[1], mom.du[1], mom.du[0],
+                      mom.dv[1], mom.dv[0], mom.du_mix, mom.dv_mix, mom.u[1], mom.v[1],
+                      mom.psi[1], mom.psi[0]):
+                f64(a).tofile(f)
+        p = subprocess.run([PRESS, fin, fout], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=3600)
+        if not os.path.exists(fout):
+            raise RuntimeError("reference solve_pressure failed:\n" + p.stdout.decode()[-2000:])
+        raw = open(fout, "rb").read()
+    plane = (ny + 4) * (nx + 4)
+    itts = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    off = 4
+    out = {}
+    for name in ("p_hydro", "du", "dv", "u", "v", "psi"):
+        n = 1 if name == "psi" else nz
+        a = np.frombuffer(raw, np.float64, n * plane, off).copy()
+        off += 8 * n * plane
+        out[name] = a.reshape((ny + 4, nx + 4) if name == "psi" else (nz, ny + 4, nx + 4))
+    out.update(iterations=itts, log=p.stdout.decode(errors="replace"))
+    return out

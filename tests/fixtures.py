@@ -42,3 +42,28 @@ def load_step(name):
                    np.stack([d["dv_taum1"], d["dv_tau"], z3]), d["du_mix"], d["dv_mix"],
                    np.stack([z2, d["psi_tau"], z2]), np.stack([d["dpsi_taum1"], d["dpsi_tau"], z2]))
     return b, col, mom, d
+
+
+PRESS_NAMES = ["press_fs_24x20_nz5", "press_rigid_26x22_nz3", "press_rigid_cyclic_30x16_nz4"]
+
+
+def load_press(name):
+    """A surface-pressure fixture: (basin with hu/hv, Column, Momentum at
+    taum1=0/tau=1 with psi the surface pressure, dict).  Time level 2
+    (taup1) of the state is zero on entry."""
+    from pyom2_amd.basins import Column, Momentum
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    d = {k: z[k] for k in z.files}
+    nx, ny, nz = int(d["nx"]), int(d["ny"]), int(d["nz"])
+    kbot = d["kbot_levels"]
+    b = Basin(nx, ny, bool(d["cyclic_x"]), d["dxt"], d["dxu"], d["dyt"], d["dyu"], d["cost"],
+              d["cosu"], kbot, None, d["hu"], d["hv"], None, None, d["maskT"][-1],
+              d["maskU"][-1], d["maskV"][-1], np.zeros((ny + 4, nx + 4)), name)
+    col = Column(nz, d["dzt"], d["dzw"], kbot, d["maskT"], d["maskU"], d["maskV"])
+    z3 = np.zeros((nz, ny + 4, nx + 4))
+    z2 = np.zeros((ny + 4, nx + 4))
+    mom = Momentum(np.stack([z3, d["rho"], z3]), np.stack([z3, d["u_tau"], z3]),
+                   np.stack([z3, d["v_tau"], z3]), np.stack([d["du_taum1"], d["du_tau"], z3]),
+                   np.stack([d["dv_taum1"], d["dv_tau"], z3]), d["du_mix"], d["dv_mix"],
+                   np.stack([d["psi_taum1"], d["psi_tau"], z2]), np.stack([z2, z2, z2]))
+    return b, col, mom, d

@@ -11,6 +11,8 @@ fields and line_psin matrix, the solution, iteration count and converged
 flag of congrad_streamfunction, apply_op of that solution, and the island
 line integrals of a seeded random (u, v) pair.
 
+The ``press_*`` fixtures come from oracle/_ref/ref_press, the reference's
+solve_pressure (surface-pressure mode, free surface and rigid lid).
 The ``step_*`` fixtures come from oracle/_ref/ref_step, the reference's
 whole solve_streamfunction on a basin with vertical levels and no islands:
 inputs (levels, masks, the model state at tau/taum1) and outputs (p_hydro,
@@ -114,6 +116,45 @@ def make_step(name):
     print(f"{name}: iterations={r['iterations']} -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
+PRESS_CASES = {
+    # free surface, a closed basin with an island, 5 levels
+    "press_fs_24x20_nz5": lambda: (basins.beta_plane_basin(seed=1, nx=24, ny=20, n_islands=1),
+                                   5, 2, 1),
+    # rigid lid (a singular, consistent system), closed basin with an island
+    "press_rigid_26x22_nz3": lambda: (basins.beta_plane_basin(seed=2, nx=26, ny=22, n_islands=1),
+                                      3, 4, 0),
+    # rigid lid, zonally periodic channel: the reference ends on its divergence
+    # test (the residual norm grows 100-fold over its minimum)
+    "press_rigid_cyclic_30x16_nz4": lambda: (basins.beta_plane_basin(seed=6, nx=30, ny=16,
+                                                                     cyclic_x=True), 4, 3, 0),
+}
+
+
+def make_press(name):
+    b0, nz, seed, fs = PRESS_CASES[name]()
+    b, col = basins.column_basin(b0, nz)
+    mom = basins.momentum_fields(col, seed)
+    mom.psi[:] = mom.psi * 1e-6          # surface pressure, m^2/s^2
+    eps, max_iter, dt = 1e-12, 3000, 1200.0
+    r = ref_run.run_press(b, col, b.hu, b.hv, mom, fs, eps, max_iter, dt)
+    out = dict(
+        nx=b.nx, ny=b.ny, nz=nz, cyclic_x=int(b.cyclic_x), free_surface=fs, eps=eps,
+        max_iter=max_iter, dt_mom=dt, dxt=b.dxt, dxu=b.dxu, dyt=b.dyt, dyu=b.dyu, cost=b.cost,
+        cosu=b.cosu, hu=b.hu, hv=b.hv, dzt=col.dzt, dzw=col.dzw, kbot_levels=col.kbot,
+        maskT=col.maskT, maskU=col.maskU, maskV=col.maskV,
+        rho=mom.rho[1], du_tau=mom.du[1], du_taum1=mom.du[0], dv_tau=mom.dv[1],
+        dv_taum1=mom.dv[0], du_mix=mom.du_mix, dv_mix=mom.dv_mix, u_tau=mom.u[1],
+        v_tau=mom.v[1], psi_tau=mom.psi[1], psi_taum1=mom.psi[0],
+        out_p_hydro=r["p_hydro"], out_du=r["du"], out_dv=r["dv"], out_u=r["u"], out_v=r["v"],
+        out_psi=r["psi"], iterations=r["iterations"],
+        ref_warning=int("WARNING" in r["log"]),
+    )
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: iterations={r['iterations']} warning={out['ref_warning']} -> "
+          f"{os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def make(name):
     b, kw = CASES[name]()
     rng = np.random.default_rng(1234)
@@ -144,5 +185,5 @@ def make(name):
 if __name__ == "__main__":
     if not ref_run.build():
         sys.exit("reference sources not available: cannot build oracle/_ref")
-    for n in (sys.argv[1:] or list(CASES) + list(STEP_CASES)):
-        (make_step if n in STEP_CASES else make)(n)
+    for n in (sys.argv[1:] or list(CASES) + list(STEP_CASES) + list(PRESS_CASES)):
+        (make_step if n in STEP_CASES else make_press if n in PRESS_CASES else make)(n)
