@@ -287,6 +287,62 @@ int psi_mg_levels(const psi_ctx *ctx, int32_t *nlevels, int32_t *nx, int32_t *ny
  * between solves. */
 int psi_mg_kernel_times(psi_ctx *ctx, int32_t degree, int32_t reps, double *us, int32_t n);
 
+/* ---- the surface-pressure solver (for_src/external/solve_pressure.f90) -----
+ * pyOM2's external mode when enable_streamfunction is off (momentum.f90:
+ * 158-166): A p = forc with A = 1/cos^2 ((h p_x)_x + cos (cos h p_y)_y) (minus
+ * 1/(grav dt^2) with a free surface), solved by conjugate gradients without a
+ * preconditioner.  Its own context (a grid, no islands); every field is a
+ * device (ny+4, nx+4) array or nz such planes as in psi_levels. */
+typedef struct psi_press psi_press;
+
+typedef struct psi_press_fields {
+    const double *hu, *hv;     /* device: depth at u and v points (main_module.f90:99) */
+    const double *maskT;       /* device: maskT(:,:,nz), values 0 or 1 */
+    int32_t free_surface;      /* enable_free_surface */
+    int32_t reserved;
+    double grav, dt_mom;       /* enter the free-surface term, fixed at creation as the
+                                  reference's SAVE'd coefficients are at its first call */
+} psi_press_fields;
+
+/* make_coeff_surf_press (solve_pressure.f90:87-152) on the device, once. */
+int psi_press_create(psi_press **out, const psi_grid *grid, const psi_press_fields *fields,
+                     void *stream);
+void psi_press_destroy(psi_press *ctx);
+int psi_press_set_stream(psi_press *ctx, void *stream);
+const char *psi_press_last_error(const psi_press *ctx);
+int64_t psi_press_device_bytes(const psi_press *ctx);
+/* The coefficients as the reference's cf(:,:,3,3): 9 device planes, plane
+ * (ii+1) + 3 (jj+1) holding cf(:,:,ii+2,jj+2). */
+int psi_make_coeff_surf_press(psi_press *ctx, double *cf);
+/* apply_op (solve_pressure.f90:254-274) with these coefficients: res = A p on
+ * the interior, zero on the halo. */
+int psi_press_apply_op(psi_press *ctx, const double *p, double *res);
+/* congrad_surf_press (solve_pressure.f90:157-249): psi (device, in place) is
+ * the first guess and the result; the reference's tests in its order (p.Ap = 0,
+ * divergence of r.r, the extrapolated step bound congr_epsilon, NaN, the
+ * iteration cap).  opts: mode PSI_MODE_PLAIN, criterion PSI_CRIT_REFERENCE,
+ * congr_epsilon, max_iterations, check_every.  info->iterations is the
+ * reference's congr_itts; relres_true is not computed (-1).  Returns PSI_OK,
+ * PSI_NOT_CONVERGED, PSI_DIVERGED or PSI_NAN. */
+int psi_congrad_surf_press(psi_press *ctx, const psi_opts *opts, const double *forc, double *psi,
+                           psi_info *info);
+/* The whole solve_pressure (solve_pressure.f90:9-84, serial): hydrostatic
+ * pressure into p_hydro and its gradient into du/dv (tau), u/v at taup1 by
+ * Adams-Bashforth, the depth-integrated transport's divergence as forcing
+ * (into forc_out when given), the first guess psi(taup1) = 2 psi(tau) -
+ * psi(taum1), congrad_surf_press, setcyclic_xy, and the surface-pressure
+ * gradient removed from u/v at taup1.  psi: 3 planes (the surface pressure).
+ * Returns the solve's status; on PSI_NAN (where the reference halts) the
+ * velocities are not corrected. */
+int psi_solve_pressure(psi_press *ctx, const psi_opts *opts, const psi_levels *lev,
+                       const psi_momentum *mom, double grav, double rho_0, double *psi,
+                       int32_t taum1, int32_t tau, int32_t taup1, double dt_mom, double AB_eps,
+                       double *forc_out, psi_info *info);
+/* Performance report: average duration in microseconds of `reps` launches of
+ * the iteration's kernels (us[0] direction + apply, us[1] update, us[2] and
+ * us[3] the two folds; n >= 4), HIP events; call between solves. */
+int psi_press_kernel_times(psi_press *ctx, int32_t reps, double *us, int32_t n);
+
 /* ---- flexible CG with the learned preconditioner ---------------------------
  * No counterpart in the reference.  The host applies the network of psinn.h
  * to the current residual between psi_fcg_input_scale and psi_fcg_step.

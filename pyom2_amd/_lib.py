@@ -104,6 +104,12 @@ class PsiMomentum(C.Structure):
                 ("p_hydro", C.c_void_p)]
 
 
+class PsiPressFields(C.Structure):
+    _fields_ = [("hu", C.c_void_p), ("hv", C.c_void_p), ("maskT", C.c_void_p),
+                ("free_surface", C.c_int32), ("reserved", C.c_int32), ("grav", C.c_double),
+                ("dt_mom", C.c_double)]
+
+
 # (name, restype, argtypes) of every entry point in include/*.h
 SIGNATURES = [
     ("psi_islands_trace", C.c_int, [C.c_int32, C.c_int32, C.c_int32, _ip, _ip, _ip, C.c_int32, _ip,
@@ -153,6 +159,22 @@ SIGNATURES = [
     ("psi_mg_setup", C.c_int, [C.c_void_p]),
     ("psi_mg_levels", C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, C.c_int32]),
     ("psi_mg_kernel_times", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32]),
+    # the surface-pressure solver
+    ("psi_press_create", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PsiGrid),
+                                   C.POINTER(PsiPressFields), C.c_void_p]),
+    ("psi_press_destroy", None, [C.c_void_p]),
+    ("psi_press_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_press_last_error", C.c_char_p, [C.c_void_p]),
+    ("psi_press_device_bytes", C.c_int64, [C.c_void_p]),
+    ("psi_make_coeff_surf_press", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_press_apply_op", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psi_congrad_surf_press", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.c_void_p, C.c_void_p,
+                                         C.POINTER(PsiInfo)]),
+    ("psi_solve_pressure", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.POINTER(PsiLevels),
+                                     C.POINTER(PsiMomentum), C.c_double, C.c_double, C.c_void_p,
+                                     C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                     C.c_void_p, C.POINTER(PsiInfo)]),
+    ("psi_press_kernel_times", C.c_int, [C.c_void_p, C.c_int32, _dp, C.c_int32]),
     ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),

@@ -1232,8 +1232,9 @@ int psi_mg_levels(const psi_ctx* c, int32_t* nlevels, int32_t* nx, int32_t* ny, 
     return PSI_OK;
 }
 
-void psi_ctx_destroy(psi_ctx* c) {
-    if (!c) return;
+// Frees what a context holds (device memory, pinned buffers, events), not the
+// object itself.
+static void ctx_release(psi_ctx* c) {
     mg_release(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     else (void)hipDeviceSynchronize();
@@ -1247,6 +1248,12 @@ void psi_ctx_destroy(psi_ctx* c) {
     if (c->ts0) (void)hipEventDestroy(c->ts0);
     for (auto& e : c->ts1)
         if (e) (void)hipEventDestroy(e);
+    c->allocs.clear();
+}
+
+void psi_ctx_destroy(psi_ctx* c) {
+    if (!c) return;
+    ctx_release(c);
     delete c;
 }
 
@@ -2011,3 +2018,6 @@ int psi_fcg_scalars(psi_ctx* c, double* out10) {
 // kernels and helpers.
 #include "../../include/psibatch.h"
 #include "psi_batch.hip"
+
+// The surface-pressure solver (psi_press_*) likewise.
+#include "psi_press.hip"

@@ -1,0 +1,199 @@
+"""The surface-pressure solver on the GPU (psi_press_*, psi_solve_pressure)
+through the C ABI, against the oracle (solve_pressure.f90 restated in C) and
+the reference's fixtures.
+
+Bar: the coefficients, apply_op, the forcing (p_hydro, du, dv, forc) and the
+velocity correction are bit-identical; congrad_surf_press sums its inner
+products in a different order than the reference's serial loops, so its
+iterates agree to rounding: the iteration count within 2 of the
+reference's, psi within 1e-9 of max|psi| (the solve stops at an estimated
+error of 1e-12 absolute on psi of order 1 to 10).  At 512^2 the corrected
+velocities satisfy the discrete continuity equation to the solver's residual."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pyom2_amd import _lib, basins
+from pyom2_amd._lib import PsiError
+from pyom2_amd.pressure import SurfacePressureSolver
+from pyom2_amd.streamfunction import AB_EPS_REFERENCE as AB
+from pyom2_amd.streamfunction import GRAV_REFERENCE as GRAV
+from pyom2_amd.streamfunction import RHO_0_REFERENCE as RHO_0
+from pyom2_amd.streamfunction import Levels, MomentumState, SolveOptions
+from tests import fixtures
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+PSI_TOL = 1e-9
+
+
+def T(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=DEV)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def _case(seed, nx, ny, nz, cyclic, islands=1):
+    b = basins.beta_plane_basin(seed=seed, nx=nx, ny=ny, n_islands=0 if cyclic else islands,
+                                cyclic_x=cyclic)
+    b3, col = basins.column_basin(b, nz)
+    mom = basins.momentum_fields(col, seed)
+    mom.psi[:] = mom.psi * 1e-6
+    if cyclic:
+        for a in (mom.rho, mom.u, mom.v, mom.du, mom.dv, mom.du_mix, mom.dv_mix, mom.psi):
+            basins._setcyclic(a, b3.nx)
+    return b3, col, mom
+
+
+def _solver(b, col, fs, dt):
+    return SurfacePressureSolver.from_basin(b, col.maskT[-1], free_surface=fs, dt_mom=dt,
+                                            device=DEV)
+
+
+@pytest.mark.parametrize("fs,cyclic", [(1, False), (0, False), (1, True)])
+def test_coefficients_and_apply_bitwise(fs, cyclic):
+    print(f"\n[press coeff fs={fs} cyclic={cyclic}]", flush=True)
+    b, col, _ = _case(3, 40, 28, 4, cyclic, islands=2)
+    s = _solver(b, col, fs, 900.0)
+    cf = O.make_coeff_surf_press(b, b.hu, b.hv, col.maskT[-1], fs, GRAV, 900.0)
+    np.testing.assert_array_equal(N(s.make_coeff_surf_press()), cf)
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(b.shape)
+    np.testing.assert_array_equal(N(s.apply_op(T(p))), O.apply_op(b.nx, b.ny, cf, p))
+    s.close()
+
+
+@pytest.mark.parametrize("name", fixtures.PRESS_NAMES)
+def test_solve_pressure_matches_reference(name):
+    """The whole routine against the reference's fixture: the parts before the
+    solve bit for bit, the solve to rounding, the correction bit for bit given
+    the solve's psi."""
+    print(f"\n[press fixture {name}]", flush=True)
+    b, col, mom, d = fixtures.load_press(name)
+    fs, dt = int(d["free_surface"]), float(d["dt_mom"])
+    s = _solver(b, col, fs, dt)
+    lev = Levels.from_column(col, DEV)
+    m = MomentumState.from_numpy(mom, DEV)
+    psi = T(mom.psi)
+    forc = s.zeros()
+    opts = SolveOptions(congr_epsilon=float(d["eps"]), max_iterations=int(d["max_iter"]))
+    its, conv, info = s.solve_pressure(psi, lev, m, 0, 1, 2, opts, dt_mom=dt, forc_out=forc)
+    warn = int(d["ref_warning"])
+    assert conv == (not warn)
+    assert info["status"] == (_lib.PSI_DIVERGED if warn else _lib.PSI_OK)
+    assert abs(its - int(d["iterations"])) <= 2
+    # before the solve: bit for bit
+    np.testing.assert_array_equal(N(m.p_hydro), d["out_p_hydro"])
+    np.testing.assert_array_equal(N(m.du[1]), d["out_du"])
+    np.testing.assert_array_equal(N(m.dv[1]), d["out_dv"])
+    u_star, v_star, forc_o = O.pressure_forcing(b, col, dt, AB, mom.u[1], mom.v[1], d["out_du"],
+                                                d["out_dv"], mom.du[0], mom.dv[0], mom.du_mix,
+                                                mom.dv_mix, mom.psi[1], fs, GRAV)
+    f = N(forc)
+    np.testing.assert_array_equal(f[2:-2, 2:-2], forc_o[2:-2, 2:-2])
+    # the solve
+    p_new = N(psi[2])
+    scale = np.abs(d["out_psi"]).max()
+    if not warn:
+        assert np.abs(p_new - d["out_psi"]).max() <= PSI_TOL * scale
+    # the correction, given the GPU's psi(taup1): bit for bit
+    u_o, v_o = O.pressure_velocity(b, col, dt, p_new, u_star, v_star)
+    np.testing.assert_array_equal(N(m.u[2]), u_o)
+    np.testing.assert_array_equal(N(m.v[2]), v_o)
+    if not warn:
+        for k in ("u", "v"):
+            ref = d["out_" + k]
+            assert np.abs(N(getattr(m, k)[2]) - ref).max() <= 1e-8 * np.abs(ref).max()
+    s.close()
+
+
+def test_congrad_against_oracle_and_exits():
+    """congrad_surf_press alone: same iterations (within 2) and solution as the
+    oracle; a zero residual ends at once (0 iterations); the iteration cap
+    ends with cap + 1 and PSI_NOT_CONVERGED, the iterate kept."""
+    print("\n[press congrad exits]", flush=True)
+    b, col, _ = _case(2, 36, 30, 3, False)
+    dt = 1200.0
+    s = _solver(b, col, 1, dt)
+    cf = O.make_coeff_surf_press(b, b.hu, b.hv, col.maskT[-1], 1, GRAV, dt)
+    rng = np.random.default_rng(2)
+    act = col.maskT[-1] != 0
+    forc = np.where(act, 1e-5 * rng.standard_normal(b.shape), 0.0)
+    forc[:2], forc[-2:], forc[:, :2], forc[:, -2:] = 0, 0, 0, 0
+    x0 = np.where(act, rng.standard_normal(b.shape), 0.0)
+    ref = O.congrad_surf_press(b, cf, col.maskT[-1], forc, x0, 1e-12, 5000)
+    x = T(x0)
+    its, conv, info = s.congrad_surf_press(T(forc), x, congr_epsilon=1e-12, max_iterations=5000)
+    assert conv and ref.converged
+    assert abs(its - ref.iterations) <= 2
+    assert np.abs(N(x) - ref.sol).max() <= PSI_TOL * np.abs(ref.sol).max()
+    assert info["est_error"] < 1e-12
+    # zero residual: forc = A x exactly
+    x1 = T(ref.sol)
+    f1 = T(O.apply_op(b.nx, b.ny, cf, ref.sol))
+    its, conv, info = s.congrad_surf_press(f1, x1, congr_epsilon=1e-12, max_iterations=100)
+    assert its == 0 and conv and info["status"] == _lib.PSI_OK
+    np.testing.assert_array_equal(N(x1), ref.sol)
+    # the cap
+    for cap in (1, 3, 4):
+        x2 = T(x0)
+        its, conv, info = s.congrad_surf_press(T(forc), x2, congr_epsilon=1e-30,
+                                               max_iterations=cap)
+        r = O.congrad_surf_press(b, cf, col.maskT[-1], forc, x0, 1e-30, cap)
+        assert its == cap + 1 == r.iterations and not conv
+        assert info["status"] == _lib.PSI_NOT_CONVERGED
+        assert np.abs(N(x2) - r.sol).max() <= 1e-12 * np.abs(r.sol).max()
+    # options the reference algorithm does not take
+    with pytest.raises(PsiError):
+        s.congrad_surf_press(T(forc), x2, mode="mg", criterion="relres")
+    s.close()
+
+
+def test_mask_values_checked():
+    b, col, _ = _case(2, 20, 14, 2, False)
+    m = col.maskT[-1].copy()
+    m[5, 5] = 0.5
+    with pytest.raises(PsiError):
+        SurfacePressureSolver.from_basin(b, m, free_surface=True, dt_mom=600.0, device=DEV)
+
+
+@pytest.mark.parametrize("fs", [1, 0])
+def test_continuity_at_512(fs):
+    """512^2 x 8 levels: the forcing is bit-identical to the oracle's; the
+    corrected transport satisfies div(U_new)/dt - fs (p_tau - p_new)/(g dt^2)
+    = forc - A p_new (the residual) to rounding, and that residual is small."""
+    print(f"\n[press 512 fs={fs}]", flush=True)
+    b, col, mom = _case(12, 512, 512, 8, False, islands=3)
+    dt = 1800.0
+    s = _solver(b, col, fs, dt)
+    lev = Levels.from_column(col, DEV)
+    m = MomentumState.from_numpy(mom, DEV)
+    psi = T(mom.psi)
+    forc = s.zeros()
+    its, conv, info = s.solve_pressure(psi, lev, m, 0, 1, 2, dt_mom=dt, forc_out=forc,
+                                       congr_epsilon=1e-10, max_iterations=20000)
+    print(f"  its={its} status={info['status']} {info['elapsed_ms']:.1f} ms", flush=True)
+    assert conv
+    du, dv = N(m.du[1]), N(m.dv[1])
+    p, du_o, dv_o, _, _ = O.streamfunction_forcing(b, col, mom.rho[1], mom.du[1], mom.dv[1],
+                                                   mom.du_mix, mom.dv_mix, GRAV / RHO_0)
+    np.testing.assert_array_equal(N(m.p_hydro), p)
+    np.testing.assert_array_equal(du, du_o)
+    _, _, forc_o = O.pressure_forcing(b, col, dt, AB, mom.u[1], mom.v[1], du, dv, mom.du[0],
+                                      mom.dv[0], mom.du_mix, mom.dv_mix, mom.psi[1], fs, GRAV)
+    np.testing.assert_array_equal(N(forc)[2:-2, 2:-2], forc_o[2:-2, 2:-2])
+    p_new = N(psi[2])
+    cf = O.make_coeff_surf_press(b, b.hu, b.hv, col.maskT[-1], fs, GRAV, dt)
+    res = (forc_o - O.apply_op(b.nx, b.ny, cf, p_new))[2:-2, 2:-2]
+    z = np.zeros_like(du)
+    _, _, div_new = O.pressure_forcing(b, col, dt, 0.0, N(m.u[2]), N(m.v[2]), z, z, z, z, z, z,
+                                       p_new, 0, GRAV)
+    lhs = (div_new - fs * (mom.psi[1] - p_new) / (GRAV * dt * dt) * col.maskT[-1])[2:-2, 2:-2]
+    act = col.maskT[-1][2:-2, 2:-2] != 0
+    scale = np.abs(forc_o).max()
+    assert np.abs(lhs - res)[act].max() <= 1e-10 * scale
+    assert np.abs(res)[act].max() <= 1e-4 * scale
+    s.close()
