@@ -89,7 +89,9 @@ def parse():
     ap.add_argument("--nz", type=int, default=40, help="levels of the step workload")
     ap.add_argument("--step-mode", default="mg", choices=["mg", "poly", "mixed", "plain"],
                     help="solve path of the step workload")
-    ap.add_argument("--workload", default="solve", choices=["solve", "ensemble", "step"],
+    ap.add_argument("--free-surface", type=int, default=1,
+                    help="press workload: enable_free_surface (0: rigid lid)")
+    ap.add_argument("--workload", default="solve", choices=["solve", "ensemble", "step", "press"],
                     help="solve: the 2048^2 two-island solve (headline); ensemble: 512 "
                          "batched basins of 64^2-512^2")
     ap.add_argument("--members", type=int, default=512)
@@ -131,6 +133,8 @@ def main():
         return ensemble_main(args)
     if args.workload == "step":
         return step_main(args)
+    if args.workload == "press":
+        return press_main(args)
     import torch
 
     from pyom2_amd import basins
@@ -621,6 +625,174 @@ def step_main(args):
         "cpu_baseline_levels": cpu,
     }
     print(json.dumps(finite(line)), flush=True)
+
+
+# algorithmic bytes per point of the surface-pressure CG kernels: k_sp_dir reads
+# res, the old direction, 4 coefficient planes and the byte mask, writes the new
+# direction and A p; k_sp_update reads psi, res, p, A p and the mask, writes psi
+# and res.
+PRESS_BYTES = {"dir_apply": 8 + 8 + 32 + 1 + 16, "update": 32 + 1 + 16}
+PRESS_LEVEL_BYTES = {"forcing": 24 + 120, "velocity": 48}
+
+
+def press_main(args):
+    """The whole solve_pressure per step (solve_pressure.f90:9-84): hydrostatic
+    pressure, forcing, congrad_surf_press with the reference's stopping rule
+    (congr_epsilon 1e-12 absolute on the extrapolated step) from the
+    2 psi(tau) - psi(taum1) first guess, velocity correction, on the 2048^2
+    spherical basin with two islands and nz levels.  Tendencies drift from
+    step to step (outside the timed calls)."""
+    import torch
+
+    from pyom2_amd import basins
+    from pyom2_amd.pressure import SurfacePressureSolver
+    from pyom2_amd.streamfunction import Levels, MomentumState, SolveOptions
+
+    dev = torch.device("cuda:0")
+    calibration_copy(args.copy_mib, dev)
+    nz, fs, dt = args.nz, bool(args.free_surface), 3600.0
+    t = time.time()
+    b0 = basins.spherical_basin(seed=0, n=args.n, n_islands=2, dlon=0.05 * 2048 / args.n)
+    dzt, dzw, kb = basins.column_levels(b0, nz)
+    kb_d = torch.as_tensor(kb, device=dev)
+    lvl = torch.arange(1, nz + 1, device=dev)[:, None, None]
+    maskT = ((kb_d[None] != 0) & (kb_d[None] <= lvl)).double()
+    maskU = maskT.clone()
+    maskU[:, :, :-1] = torch.minimum(maskT[:, :, :-1], maskT[:, :, 1:])
+    maskV = maskT.clone()
+    maskV[:, :-1, :] = torch.minimum(maskT[:, :-1, :], maskT[:, 1:, :])
+    hu = torch.zeros(b0.shape, dtype=torch.float64, device=dev)
+    hv = torch.zeros_like(hu)
+    for k in range(nz):
+        hu = hu + maskU[k] * float(dzt[k])
+        hv = hv + maskV[k] * float(dzt[k])
+    s = SurfacePressureSolver(b0.nx, b0.ny, b0.dxt, b0.dxu, b0.dyt, b0.dyu, b0.cost, b0.cosu,
+                              hu, hv, maskT[-1], free_surface=fs, dt_mom=dt, device=dev)
+    host_op = (hu.cpu().numpy(), hv.cpu().numpy(), maskT[-1].cpu().numpy())
+    lev = Levels(dzt, dzw, maskT, maskU, maskV, dev)
+    del maskT, maskU, maskV, hu, hv
+    opts = SolveOptions(congr_epsilon=1e-12, max_iterations=200000)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    sh3 = (nz,) + b0.shape
+
+    def rnd(scale, mask, levels=None):
+        shape = sh3 if levels is None else (levels,) + sh3
+        return scale * torch.randn(shape, generator=g, dtype=torch.float64, device=dev) * mask
+
+    strat = torch.linspace(4.0, 0.0, nz, dtype=torch.float64, device=dev)[:, None, None]
+    rho = (1024.0 + strat + 0.05 * torch.randn((3,) + sh3, generator=g, dtype=torch.float64,
+                                                device=dev)) * lev.maskT
+    ms = MomentumState(rho, rnd(0.1, lev.maskU, 3), rnd(0.1, lev.maskV, 3),
+                       rnd(1e-6, lev.maskU, 3), rnd(1e-6, lev.maskV, 3), rnd(1e-7, lev.maskU),
+                       rnd(1e-7, lev.maskV), torch.zeros(sh3, dtype=torch.float64, device=dev))
+    psi = torch.zeros((3,) + b0.shape, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    log(f"[press] basin {b0.nx}x{b0.ny} x {nz} levels, free surface {int(fs)}, "
+        f"set up in {time.time() - t:.1f}s")
+    c = float(np.sqrt(1 - 0.99 ** 2))
+    taum1, tau, taup1 = 0, 1, 2
+    per_step, its, solve_ms = [], [], []
+    for k in range(args.warmup + args.steps):
+        ms.du[tau] = 0.99 * ms.du[taum1] + rnd(c * 1e-6, lev.maskU)
+        ms.dv[tau] = 0.99 * ms.dv[taum1] + rnd(c * 1e-6, lev.maskV)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        n_it, conv, info = s.solve_pressure(psi, lev, ms, taum1, tau, taup1, opts)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        if not conv:
+            raise SystemExit(f"step {k}: surface pressure did not converge ({info})")
+        if k >= args.warmup:
+            per_step.append(te - ts)
+            its.append(n_it)
+            solve_ms.append(info["elapsed_ms"])
+        log(f"[press] {k}: {(te - ts) * 1e3:.2f} ms, solve {info['elapsed_ms']:.2f} ms, "
+            f"{n_it} iterations, est. error {info['est_error']:.2e}")
+        taum1, tau, taup1 = tau, taup1, taum1
+    kt = s.kernel_times(reps=20)
+    pts = b0.shape[0] * b0.shape[1]
+    kern = {}
+    for name in ("dir_apply", "update"):
+        gbs = PRESS_BYTES[name] * pts / (kt[name] * 1e-6) / 1e9
+        kern[name] = {"us": round(kt[name], 2), "achieved": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_pt": PRESS_BYTES[name]}
+    kern["fold_a_us"] = round(kt["fold_a"], 2)
+    kern["fold_b_us"] = round(kt["fold_b"], 2)
+    dom = max(("dir_apply", "update"), key=lambda n: kt[n])
+    total = sum(per_step)
+    it_mean = float(np.mean(its))
+    cpu = None if args.skip_cpu else press_cpu_baseline(b0, host_op, nz, it_mean, fs,
+                                                        budget_s=args.cpu_seconds)
+    line = {
+        "metric": f"ms per {args.n}^2 x {nz}-level solve_pressure step (forcing, surface-"
+                  f"pressure CG to congr_epsilon 1e-12, velocity correction)",
+        "value": round(1e3 * total / args.steps, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"spherical basin {b0.nx}x{b0.ny}, {nz} levels, 2 islands, "
+                               f"{'free surface' if fs else 'rigid lid'}, dt 3600 s, drifting "
+                               f"tendencies, first guess 2 psi(tau) - psi(taum1)"},
+        "solve_ms_mean": round(float(np.mean(solve_ms)), 3),
+        "iterations_mean": round(it_mean, 1),
+        "us_per_iteration": round(1e3 * float(np.mean(solve_ms)) / max(it_mean, 1.0), 2),
+        "press_kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": "k_sp_dir" if dom == "dir_apply" else
+                     "k_sp_update", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(finite(line)), flush=True)
+
+
+def press_cpu_baseline(b, host_op, nz, iterations, fs, budget_s=10.0, n3=512):
+    """The oracle's solve_pressure on the CPU (1 core): CG iterations on the
+    benchmark's own 2048^2 operator for about budget_s, and the forcing and
+    velocity parts on an n3^2 x nz basin scaled per point-level, summed to one
+    step of the GPU run's iteration count."""
+    from oracle import oracle as O
+    from pyom2_amd import basins
+    from pyom2_amd.streamfunction import AB_EPS_REFERENCE, GRAV_REFERENCE, RHO_0_REFERENCE
+    dt = 3600.0
+    hu, hv, mT = host_op
+    cf = O.make_coeff_surf_press(b, hu, hv, mT, fs, GRAV_REFERENCE, dt)
+    rng = np.random.default_rng(0)
+    forc = np.where(mT != 0, 1e-9 * rng.standard_normal(b.shape), 0.0)
+    k = 2
+    while True:
+        t2 = time.perf_counter()
+        O.congrad_surf_press(b, cf, mT, forc, np.zeros(b.shape), 1e-30, k)
+        el = time.perf_counter() - t2
+        if el * 4 > budget_s or k >= 4096:
+            break
+        k *= 4
+    per_it = el / k
+    b3, col = basins.column_basin(basins.spherical_basin(seed=0, n=n3, n_islands=2,
+                                                         dlon=0.05 * 2048 / n3), nz)
+    m = basins.momentum_fields(col, 0)
+    t0 = time.perf_counter()
+    p, du, dv, _, _ = O.streamfunction_forcing(b3, col, m.rho[1], m.du[1], m.dv[1], m.du_mix,
+                                               m.dv_mix, GRAV_REFERENCE / RHO_0_REFERENCE)
+    u, v, f3 = O.pressure_forcing(b3, col, dt, AB_EPS_REFERENCE, m.u[1], m.v[1], du, dv,
+                                  m.du[0], m.dv[0], m.du_mix, m.dv_mix, m.psi[1], fs,
+                                  GRAV_REFERENCE)
+    O.pressure_velocity(b3, col, dt, m.psi[1], u, v)
+    lev_s = (time.perf_counter() - t0) * (b.shape[0] * b.shape[1]) / (b3.shape[0] * b3.shape[1])
+    total = lev_s + per_it * iterations
+    return {"value": round(1e3 * total, 1), "unit": "ms per step (extrapolated)", "cores": 1,
+            "kind": "port",
+            "sample": f"{k} CG iterations on the {b.nx}x{b.ny} operator ({per_it * 1e3:.1f} ms "
+                      f"each) x {iterations:.0f} iterations of the GPU run, plus forcing and "
+                      f"velocity correction on {b3.nx}x{b3.ny}x{nz} scaled per point-level "
+                      f"({lev_s:.1f} s); C restatement, 1 thread"}
 
 
 def levels_cpu_baseline(nz, budget_s=10.0, n=512):

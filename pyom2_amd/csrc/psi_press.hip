@@ -22,7 +22,10 @@ struct SpScal {
     int n, done, status, iterations, max_it, pad;
 };
 
-constexpr int SP_BLOCKS = 2048;   // workgroups of the grid-stride passes
+// Workgroups of the grid-stride passes (k_sp_dir: measured at 2048^2, 51 us
+// with 8192 against 54 with 1024-4096).
+constexpr int SP_BLOCKS = 2048;
+constexpr int SP_DBLOCKS = 8192;
 
 __device__ __forceinline__ bool sp_inner(const Geom& g, int row, int col) {
     return row >= 2 && row <= g.ny + 1 && col >= 2 && col <= g.nx + 1;
@@ -266,6 +269,39 @@ __global__ void __launch_bounds__(256) k_sp_dir(
     sp_block_partials<2, 1>(acc, part);
 }
 
+// k_sp_update on pairs of points (q, q+1), q even: 16-byte loads and stores.
+// Needs an even pitch (pairs never straddle rows) and 16-byte aligned planes.
+__global__ void __launch_bounds__(256) k_sp_update2(Geom g, const uint8_t* __restrict__ m8,
+                                                    double* __restrict__ x,
+                                                    double* __restrict__ r,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ Ap,
+                                                    const SpScal* __restrict__ S,
+                                                    double* __restrict__ part) {
+#pragma clang fp contract(off)
+    if (S->done) return;
+    const double alpha = S->alpha;
+    const size_t P = g.pitch;
+    const size_t npair = P * (size_t)(g.ny + 4) / 2;
+    double acc[1] = {0.0};
+    for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < npair;
+         h += (size_t)gridDim.x * blockDim.x) {
+        const size_t q = 2 * h;
+        const double2 xv = *(const double2*)(x + q), pv = *(const double2*)(p + q);
+        const double2 rv = *(const double2*)(r + q), av = *(const double2*)(Ap + q);
+        *(double2*)(x + q) = make_double2(xv.x + alpha * pv.x, xv.y + alpha * pv.y);
+        const double2 rn = make_double2(rv.x - alpha * av.x, rv.y - alpha * av.y);
+        *(double2*)(r + q) = rn;
+        const int row = (int)(q / P), col = (int)(q - (size_t)row * P);
+        if (row >= 2 && row <= g.ny + 1) {
+            const uchar2 mm = *(const uchar2*)(m8 + q);
+            if (col >= 2 && col <= g.nx + 1) acc[0] = acc[0] + rn.x * rn.x * (double)mm.x;
+            if (col + 1 >= 2 && col + 1 <= g.nx + 1) acc[0] = acc[0] + rn.y * rn.y * (double)mm.y;
+        }
+    }
+    sp_block_partials<1, 1>(acc, part);
+}
+
 // The tests of the reference's loop in its order: for iteration n-1 the step
 // test with max|p| (:219-229), then its NaN test (:233-236) and the iteration
 // cap (:238-239); for iteration n the p.Ap = 0 end (:185) and alpha (:186).
@@ -485,20 +521,27 @@ void press_release_graphs(psi_press* c) {
     c->graphs.clear();
 }
 
+// The pair kernels need an even pitch and a 16-byte aligned solution (the
+// context's own planes come from hipMalloc).
+bool press_pairs(const psi_press* c, const double* x) {
+    return c->g.pitch % 2 == 0 && ((uintptr_t)x & 15) == 0;
+}
+
 // Two iterations (p buffers 0 -> 1 -> 0) on the graph stream.
 void press_iterations(psi_press* c, hipStream_t s, double* x) {
     using namespace psi;
     const Geom& g = c->g;
+    const bool pairs = press_pairs(c, x);
     for (int h = 0; h < 2; ++h) {
-        hipLaunchKernelGGL(k_sp_dir, dim3(SP_BLOCKS), dim3(256), 0, s, g, (const double*)c->cE,
+        hipLaunchKernelGGL(k_sp_dir, dim3(SP_DBLOCKS), dim3(256), 0, s, g, (const double*)c->cE,
                            (const double*)c->cW, (const double*)c->cN, (const double*)c->cS,
                            (const uint8_t*)c->m8, c->fs, c->kfs, (const double*)c->r,
                            (const double*)c->pv[h], c->pv[1 - h], c->Ap, (const SpScal*)c->S,
                            c->sp_part);
         hipLaunchKernelGGL(k_sp_fold_a, dim3(1), dim3(1024), 0, s, (const double*)c->sp_part,
-                           SP_BLOCKS, c->S);
-        hipLaunchKernelGGL(k_sp_update, dim3(SP_BLOCKS), dim3(256), 0, s, g,
-                           (const uint8_t*)c->m8, x, c->r, (const double*)c->pv[1 - h],
+                           SP_DBLOCKS, c->S);
+        hipLaunchKernelGGL(pairs ? k_sp_update2 : k_sp_update, dim3(SP_BLOCKS), dim3(256), 0, s,
+                           g, (const uint8_t*)c->m8, x, c->r, (const double*)c->pv[1 - h],
                            (const double*)c->Ap, (const SpScal*)c->S, c->sp_part);
         hipLaunchKernelGGL(k_sp_fold_b, dim3(1), dim3(1024), 0, s, (const double*)c->sp_part,
                            SP_BLOCKS, c->S);
@@ -677,7 +720,7 @@ int psi_press_create(psi_press** out, const psi_grid* grid, const psi_press_fiel
         TRY(dalloc(c, &c->fpx, c->plane));
         TRY(dalloc(c, &c->fpy, c->plane));
         TRY(dalloc(c, &c->forc, c->plane));
-        TRY(dalloc(c, &c->sp_part, 2 * (size_t)SP_BLOCKS));
+        TRY(dalloc(c, &c->sp_part, 2 * (size_t)std::max(SP_BLOCKS, SP_DBLOCKS)));
         TRY(dalloc(c, &c->S, 1));
         TRY(dalloc(c, &c->S_time, 1));
         TRY(dalloc(c, &c->d_bad, 1));
@@ -842,22 +885,24 @@ int psi_press_kernel_times(psi_press* c, int32_t reps, double* us, int32_t n) {
         return 1e3 * ms / reps;
     };
     const SpScal* St = c->S_time;
+    const bool pairs = press_pairs(c, c->pv[0]);
     us[0] = timed([&] {
-        hipLaunchKernelGGL(k_sp_dir, dim3(SP_BLOCKS), dim3(256), 0, c->gs, g,
+        hipLaunchKernelGGL(k_sp_dir, dim3(SP_DBLOCKS), dim3(256), 0, c->gs, g,
                            (const double*)c->cE, (const double*)c->cW, (const double*)c->cN,
                            (const double*)c->cS, (const uint8_t*)c->m8, c->fs, c->kfs,
                            (const double*)c->r, (const double*)c->pv[0], c->pv[1], c->Ap, St,
                            c->sp_part);
     });
     us[1] = timed([&] {
-        hipLaunchKernelGGL(k_sp_update, dim3(SP_BLOCKS), dim3(256), 0, c->gs, g,
+        hipLaunchKernelGGL(pairs ? k_sp_update2 : k_sp_update, dim3(SP_BLOCKS), dim3(256), 0,
+                           c->gs, g,
                            (const uint8_t*)c->m8, c->pv[0], c->r, (const double*)c->pv[1],
                            (const double*)c->Ap, St, c->sp_part);
     });
     // the folds write the scalars: give them a copy that is never read
     us[2] = timed([&] {
         hipLaunchKernelGGL(k_sp_fold_a, dim3(1), dim3(1024), 0, c->gs,
-                           (const double*)c->sp_part, SP_BLOCKS, c->S_time);
+                           (const double*)c->sp_part, SP_DBLOCKS, c->S_time);
     });
     us[3] = timed([&] {
         hipLaunchKernelGGL(k_sp_fold_b, dim3(1), dim3(1024), 0, c->gs,

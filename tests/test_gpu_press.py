@@ -73,6 +73,28 @@ def test_solve_pressure_matches_reference(name):
     the solve's psi."""
     print(f"\n[press fixture {name}]", flush=True)
     b, col, mom, d = fixtures.load_press(name)
+    _check_whole(b, col, mom, d)
+
+
+@pytest.mark.parametrize("nx,cyclic,fs", [(33, True, 1), (31, False, 0)])
+def test_solve_pressure_odd_pitch(nx, cyclic, fs):
+    """An odd row length (the per-point kernels, not the pair ones) against
+    the oracle's whole routine."""
+    print(f"\n[press odd nx={nx} cyclic={cyclic}]", flush=True)
+    b, col, mom = _case(9, nx, 22, 3, cyclic)
+    dt, eps, max_iter = 1200.0, 1e-12, 5000
+    m0 = basins.Momentum(*[a.copy() for a in (mom.rho, mom.u, mom.v, mom.du, mom.dv,
+                                              mom.du_mix, mom.dv_mix, mom.psi, mom.dpsi)])
+    p, r = O.solve_pressure(b, col, b.hu, b.hv, m0, 0, 1, 2, dt, AB, GRAV, RHO_0, fs, eps,
+                            max_iter)
+    d = dict(free_surface=fs, dt_mom=dt, eps=eps, max_iter=max_iter, iterations=r.iterations,
+             ref_warning=int(not r.converged), out_p_hydro=p, out_du=m0.du[1], out_dv=m0.dv[1],
+             out_psi=m0.psi[2], out_u=m0.u[2], out_v=m0.v[2])
+    assert r.converged
+    _check_whole(b, col, mom, d)
+
+
+def _check_whole(b, col, mom, d):
     fs, dt = int(d["free_surface"]), float(d["dt_mom"])
     s = _solver(b, col, fs, dt)
     lev = Levels.from_column(col, DEV)
@@ -110,12 +132,14 @@ def test_solve_pressure_matches_reference(name):
     s.close()
 
 
-def test_congrad_against_oracle_and_exits():
+@pytest.mark.parametrize("nx", [36, 35])
+def test_congrad_against_oracle_and_exits(nx):
     """congrad_surf_press alone: same iterations (within 2) and solution as the
     oracle; a zero residual ends at once (0 iterations); the iteration cap
-    ends with cap + 1 and PSI_NOT_CONVERGED, the iterate kept."""
-    print("\n[press congrad exits]", flush=True)
-    b, col, _ = _case(2, 36, 30, 3, False)
+    ends with cap + 1 and PSI_NOT_CONVERGED, the iterate kept.  nx 36 runs
+    the pair kernels, 35 (odd row length) the per-point ones."""
+    print(f"\n[press congrad exits nx={nx}]", flush=True)
+    b, col, _ = _case(2, nx, 30, 3, False)
     dt = 1200.0
     s = _solver(b, col, 1, dt)
     cf = O.make_coeff_surf_press(b, b.hu, b.hv, col.maskT[-1], 1, GRAV, dt)
