@@ -59,6 +59,13 @@ MG_KERNEL_NAMES = {"post_level0": "k_mg_post<4> (finest level)",
 # (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
 # profiles/k_cg_sweep_spherical_2048.md, profiles/k_poly_pcg_spherical_2048.md
 TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6, "poly": 175.5e6}
+# ... of the multigrid path's kernels (profiles/k_mg_spherical_2048.md)
+MG_TRAFFIC_2048 = {"post_level0": 168.9e6, "pre_level0": 130.3e6, "fcg_a": 201.8e6,
+                   "fcg_b": 357.1e6}
+# ... of the surface-pressure CG kernels (profiles/k_sp_press_2048x40.md) and of the
+# velocity update at 2048^2 x 40 levels (profiles/levels_kernels_step_2048x40.md)
+PRESS_TRAFFIC_2048 = {"dir_apply": 283.2e6, "update": 206.4e6}
+VELOCITY_TRAFFIC_2048x40 = 16.288e9
 
 
 def parse():
@@ -277,8 +284,9 @@ def main():
                                  f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>"),
                       "path": kpath, "achieved": round(rk["achieved_GBs"], 1),
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rk["frac"], 4),
-                      "traffic": (TRAFFIC_2048.get(kpath) if b.nx == b.ny == 2048 else
-                                  None),
+                      "traffic": (None if not b.nx == b.ny == 2048 else
+                                  MG_TRAFFIC_2048.get(rk["dominant"]) if kpath == "mg" else
+                                  TRAFFIC_2048.get(kpath)),
                       "traffic_unit": "bytes per launch",
                       "bytes_per_pt_iter": (MG_KERNEL_BYTES[rk["dominant"]] if kpath == "mg"
                                             else BYTES_PER_PT[kpath]),
@@ -621,7 +629,9 @@ def step_main(args):
         "levels_kernels": kern,
         "roofline": {"bound": "hbm", "kernel": "k_velocity_update",
                      "achieved": kern["velocity"]["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": kern["velocity"]["frac"], "traffic": None},
+                     "unit": "GB/s", "frac": kern["velocity"]["frac"],
+                     "traffic": (VELOCITY_TRAFFIC_2048x40 if b.nx == b.ny == 2048 and nz == 40
+                                 else None)},
         "cpu_baseline_levels": cpu,
     }
     print(json.dumps(finite(line)), flush=True)
@@ -747,7 +757,8 @@ def press_main(args):
         "press_kernels": kern,
         "roofline": {"bound": "hbm", "kernel": "k_sp_dir" if dom == "dir_apply" else
                      "k_sp_update", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None},
+                     "unit": "GB/s", "frac": kern[dom]["frac"],
+                     "traffic": (PRESS_TRAFFIC_2048[dom] if b0.nx == b0.ny == 2048 else None)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(finite(line)), flush=True)
