@@ -470,3 +470,106 @@ def solve_pressure(b, col, hu, hv, mom, taum1, tau, taup1, dt, ab, grav, rho_0, 
     mom.psi[taup1] = setcyclic(b.nx, b.ny, b.cyclic_x, r.sol)
     mom.u[taup1], mom.v[taup1] = pressure_velocity(b, col, dt, mom.psi[taup1], u, v)
     return p, r
+
+
+# ---- the non-hydrostatic pressure solver (non_hydrostatic.f90) ------------------
+NH_PLANES = 7   # below, south, west, centre, east, north, above
+
+
+def _nh_lib():
+    L = lib()
+    if not getattr(L, "_nh_set", False):
+        I, D = C.c_int, C.c_double
+        L.orc_make_coeff_non_hydro.argtypes = [I, I, I] + [_dp] * 12
+        L.orc_make_coeff_non_hydro.restype = None
+        L.orc_apply_op_3d.argtypes = [I, I, I, _dp, _dp, _dp]
+        L.orc_apply_op_3d.restype = None
+        L.orc_congrad_non_hydro.argtypes = [I, I, I, I, _dp, _dp, _dp, _dp, D, I,
+                                            C.POINTER(I), C.POINTER(I), C.POINTER(D)]
+        L.orc_congrad_non_hydro.restype = I
+        L.orc_non_hydro_forcing.argtypes = [I, I, I, I, D, D] + [_dp] * 14
+        L.orc_non_hydro_forcing.restype = None
+        L.orc_non_hydro_velocity.argtypes = [I, I, I, D] + [_dp] * 11
+        L.orc_non_hydro_velocity.restype = None
+        L._nh_set = True
+    return L
+
+
+def mask_w(maskT):
+    """maskW of calc_topo (numerics.f90:246-249): min(maskT(k), maskT(k+1)),
+    maskT at the top level."""
+    w = np.array(maskT, dtype=np.float64, copy=True)
+    w[:-1] = np.minimum(maskT[:-1], maskT[1:])
+    return w
+
+
+def make_coeff_non_hydro(b, col, maskW):
+    """make_coeff_non_hydro (non_hydrostatic.f90:72-141): (7, nz, ny+4, nx+4)."""
+    cf = np.zeros((NH_PLANES,) + col.shape)
+    _nh_lib().orc_make_coeff_non_hydro(b.nx, b.ny, col.nz, _c(col.maskU), _c(col.maskV),
+                                       _c(maskW), _c(b.dxt), _c(b.dxu), _c(b.dyt), _c(b.dyu),
+                                       _c(b.cost), _c(b.cosu), _c(col.dzt), _c(col.dzw), cf)
+    return cf
+
+
+def apply_op_3d(b, nz, cf, p):
+    """apply_op_3D (non_hydrostatic.f90:247-270)."""
+    res = np.zeros(p.shape)
+    _nh_lib().orc_apply_op_3d(b.nx, b.ny, nz, _c(cf), _c(p), res)
+    return res
+
+
+def congrad_non_hydro(b, col, cf, forc, p, eps=1e-12, max_iter=1000):
+    """congrad_non_hydro (non_hydrostatic.f90:145-242); p is copied."""
+    s = _c(p).copy()
+    it, cv, est = C.c_int(), C.c_int(), C.c_double()
+    st = _nh_lib().orc_congrad_non_hydro(b.nx, b.ny, col.nz, int(b.cyclic_x), _c(cf),
+                                         _c(col.maskT), _c(forc), s, float(eps), int(max_iter),
+                                         C.byref(it), C.byref(cv), C.byref(est))
+    return CGResult(s, it.value, bool(cv.value), est.value, st, None)
+
+
+def non_hydro_forcing(b, col, maskW, dt, ab, w_tau, dw_tau, dw_taum1, dw_mix, u_taup1, v_taup1,
+                      w_taup1):
+    """w(taup1) by Adams-Bashforth for k < nz, setcyclic_xyz of u, v, w at
+    taup1 and the forcing (non_hydrostatic.f90:15-39).  u, v, w are updated
+    copies; returns (u, v, w, forc)."""
+    u, v, w = (_c(a).copy() for a in (u_taup1, v_taup1, w_taup1))
+    forc = np.zeros(col.shape)
+    _nh_lib().orc_non_hydro_forcing(b.nx, b.ny, col.nz, int(b.cyclic_x), dt, ab, _c(b.dxt),
+                                    _c(b.dyt), _c(b.cost), _c(b.cosu), _c(col.dzt), _c(maskW),
+                                    _c(w_tau), _c(dw_tau), _c(dw_taum1), _c(dw_mix), u, v, w,
+                                    forc)
+    return u, v, w, forc
+
+
+def non_hydro_velocity(b, col, maskW, dt, p, u, v, w):
+    """The pressure gradient out of u, v, w at taup1 (non_hydrostatic.f90:53-63);
+    returns updated copies."""
+    u, v, w = (_c(a).copy() for a in (u, v, w))
+    _nh_lib().orc_non_hydro_velocity(b.nx, b.ny, col.nz, dt, _c(b.dxu), _c(b.dyu), _c(b.cost),
+                                     _c(col.dzw), _c(col.maskU), _c(col.maskV), _c(maskW),
+                                     _c(p), u, v, w)
+    return u, v, w
+
+
+def solve_non_hydrostatic(b, col, maskW, nh, taum1, tau, taup1, dt, ab, eps=1e-12,
+                          max_iter=1000, itt=1, cf=None):
+    """The whole solve_non_hydrostatic (non_hydrostatic.f90:6-64) on a state
+    nh with u, v, w, dw (3 time levels), dw_mix and p_non_hydro (3 time
+    levels), updated in place.  Returns the congrad result."""
+    u, v, w, forc = non_hydro_forcing(b, col, maskW, dt, ab, nh.w[tau], nh.dw[tau], nh.dw[taum1],
+                                      nh.dw_mix, nh.u[taup1], nh.v[taup1], nh.w[taup1])
+    nh.p[taup1] = 2 * nh.p[tau] - nh.p[taum1]
+    if cf is None:
+        cf = make_coeff_non_hydro(b, col, maskW)
+    r = congrad_non_hydro(b, col, cf, forc, nh.p[taup1], eps, max_iter)
+    p = r.sol
+    for k in range(col.nz):
+        setcyclic(b.nx, b.ny, b.cyclic_x, p[k])
+    nh.p[taup1] = p
+    if itt == 0:
+        nh.p[tau] = p
+        nh.p[taum1] = p
+    nh.u[taup1], nh.v[taup1], nh.w[taup1] = non_hydro_velocity(b, col, maskW, dt, p, u, v, w)
+    return r, forc

@@ -807,3 +807,255 @@ void orc_pressure_velocity(int nx, int ny, int nz, double dt_mom, const double *
             }
         }
 }
+
+/* ---------------------------------------------------------------------- */
+/* The non-hydrostatic pressure solver (for_src/non_hydrostatic/
+ * non_hydrostatic.f90), used when enable_hydrostatic is off
+ * (momentum.f90:167).  3-D fields are nz consecutive planes, level k (1 =
+ * bottom) at offset (k-1)*plane.  The operator's coefficients are stored as
+ * 7 blocks of nz planes in apply_op_3D's summation order: below (kk=-1),
+ * south, west, centre, east, north, above (kk=+1); the other 20 entries of
+ * the reference's cf(:,:,:,3,3,3) are zero. */
+enum { NH_D = 0, NH_S, NH_W, NH_C, NH_E, NH_N, NH_U, NH_PLANES };
+
+/* make_coeff_non_hydro (non_hydrostatic.f90:72-141). */
+void orc_make_coeff_non_hydro(int nx, int ny, int nz, const double *maskU, const double *maskV,
+                              const double *maskW, const double *dxt, const double *dxu,
+                              const double *dyt, const double *dyu, const double *cost,
+                              const double *cosu, const double *dzt, const double *dzw,
+                              double *cf)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const size_t n3 = (size_t)nz * plane;
+    memset(cf, 0, NH_PLANES * n3 * sizeof(double));
+    double *c0 = cf + NH_C * n3;
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = o + IX(i, j, nx);
+                const int I = i + 1, J = j + 1;
+                const double c2 = cost[J] * cost[J];
+                double mp = maskU[q], mm = maskU[q - 1];
+                c0[q] = c0[q] - mp / dxu[I] / dxt[I] / c2;
+                cf[NH_E * n3 + q] = cf[NH_E * n3 + q] + mp / dxu[I] / dxt[I] / c2;
+                c0[q] = c0[q] - mm / dxu[I - 1] / dxt[I] / c2;
+                cf[NH_W * n3 + q] = cf[NH_W * n3 + q] + mm / dxu[I - 1] / dxt[I] / c2;
+                mp = maskV[q];
+                mm = maskV[q - (size_t)(nx + 4)];
+                c0[q] = c0[q] - mp / dyu[J] / dyt[J] * cosu[J] / cost[J];
+                cf[NH_N * n3 + q] = cf[NH_N * n3 + q] + mp / dyu[J] / dyt[J] * cosu[J] / cost[J];
+                c0[q] = c0[q] - mm / dyu[J - 1] / dyt[J] * cosu[J - 1] / cost[J];
+                cf[NH_S * n3 + q] =
+                    cf[NH_S * n3 + q] + mm / dyu[J - 1] / dyt[J] * cosu[J - 1] / cost[J];
+            }
+    }
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = o + IX(i, j, nx);
+                if (k < nz - 1) {   /* k = 1 .. nz-1 (Fortran): the level above */
+                    const double mp = maskW[q];
+                    c0[q] = c0[q] - mp / dzw[k] / dzt[k];
+                    cf[NH_U * n3 + q] = cf[NH_U * n3 + q] + mp / dzw[k] / dzt[k];
+                }
+                if (k > 0) {        /* k = 2 .. nz: the level below */
+                    const double mm = maskW[q - plane];
+                    c0[q] = c0[q] - mm / dzw[k - 1] / dzt[k];
+                    cf[NH_D * n3 + q] = cf[NH_D * n3 + q] + mm / dzw[k - 1] / dzt[k];
+                }
+            }
+    }
+}
+
+/* apply_op_3D (non_hydrostatic.f90:247-270): res = A p1 on the interior of
+ * every level, zero elsewhere; the vertical neighbour index is clamped to
+ * [1, nz] as the reference's kpkk (the clamped terms have zero
+ * coefficients).  Terms in the reference's order (kk, jj, ii outer to
+ * inner); its 20 zero-coefficient terms add nothing for finite p1. */
+void orc_apply_op_3d(int nx, int ny, int nz, const double *cf, const double *p1, double *res)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const size_t n3 = (size_t)nz * plane;
+    const size_t P = (size_t)(nx + 4);
+    memset(res, 0, n3 * sizeof(double));
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        const size_t dn = k > 0 ? plane : 0, up = k < nz - 1 ? plane : 0;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = o + IX(i, j, nx);
+                double a = res[q];
+                a = a + cf[NH_D * n3 + q] * p1[q - dn];
+                a = a + cf[NH_S * n3 + q] * p1[q - P];
+                a = a + cf[NH_W * n3 + q] * p1[q - 1];
+                a = a + cf[NH_C * n3 + q] * p1[q];
+                a = a + cf[NH_E * n3 + q] * p1[q + 1];
+                a = a + cf[NH_N * n3 + q] * p1[q + P];
+                a = a + cf[NH_U * n3 + q] * p1[q + up];
+                res[q] = a;
+            }
+    }
+}
+
+static double dot_3d(int nx, int ny, int nz, const double *m, const double *a, const double *b)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    double s2 = 0.0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = (size_t)k * plane + IX(i, j, nx);
+                s2 = s2 + a[q] * b[q] * m[q];
+            }
+    return s2;
+}
+
+static double absmax_3d(int nx, int ny, int nz, const double *m, const double *a)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    double s2 = 0.0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = (size_t)k * plane + IX(i, j, nx);
+                const double v = fabs(a[q] * m[q]);
+                s2 = (v > s2) ? v : s2;
+            }
+    return s2;
+}
+
+static void setcyclic_3d(int nx, int ny, int nz, int cyclic_x, double *a)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    for (int k = 0; k < nz; ++k) orc_setcyclic(nx, ny, cyclic_x, a + (size_t)k * plane);
+}
+
+/* congrad_non_hydro (non_hydrostatic.f90:145-242): conjugate gradients
+ * without preconditioner, p (= p_non_hydro(:,:,:,taup1)) the first guess on
+ * entry.  The tests in the reference's order: p.Ap = 0 (normal end),
+ * divergence of r.r, the extrapolated step (no NaN test in this solver).  A
+ * zero initial residual ends with 0 iterations (the reference leaves n
+ * unset there).  Returns 0 (normal end) or 1 (diverged or iteration cap). */
+int orc_congrad_non_hydro(int nx, int ny, int nz, int cyclic_x, const double *cf,
+                          const double *maskT, const double *forc, double *p_sol,
+                          double congr_epsilon, int max_itts, int *iterations, int *converged,
+                          double *est_error)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const size_t n3 = (size_t)nz * plane;
+    double *res = calloc(n3, sizeof(double)), *p = calloc(n3, sizeof(double));
+    double *Ap = calloc(n3, sizeof(double));
+    double rsold, rsnew, alpha, fxa, step, step1 = 0.0, rs_min = 0.0, smax;
+    double estimated_error = 0.0, convergence_rate;
+    int n = 0, normal = 0;
+
+    orc_apply_op_3d(nx, ny, nz, cf, p_sol, res);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = (size_t)k * plane + IX(i, j, nx);
+                res[q] = forc[q] - res[q];
+            }
+    memcpy(p, res, n3 * sizeof(double));
+    setcyclic_3d(nx, ny, nz, cyclic_x, p);
+    rsold = dot_3d(nx, ny, nz, maskT, res, res);
+    if (rsold == 0.0) {
+        normal = 1;
+        goto done;
+    }
+    for (n = 1; n <= max_itts; ++n) {
+        orc_apply_op_3d(nx, ny, nz, cf, p, Ap);
+        fxa = dot_3d(nx, ny, nz, maskT, p, Ap);
+        if (fxa == 0.0) { normal = 1; goto done; }
+        alpha = rsold / fxa;
+        for (size_t q = 0; q < n3; ++q) p_sol[q] = p_sol[q] + alpha * p[q];
+        for (size_t q = 0; q < n3; ++q) res[q] = res[q] - alpha * Ap[q];
+        rsnew = dot_3d(nx, ny, nz, maskT, res, res);
+        for (size_t q = 0; q < n3; ++q) p[q] = res[q] + rsnew / rsold * p[q];
+        setcyclic_3d(nx, ny, nz, cyclic_x, p);
+        rsold = rsnew;
+        if (n == 1) {
+            rs_min = fabs(rsnew);
+        } else if (n > 2) {
+            rs_min = (rs_min < fabs(rsnew)) ? rs_min : fabs(rsnew);
+            if (fabs(rsnew) > 100.0 * rs_min) goto done;
+        }
+        smax = absmax_3d(nx, ny, nz, maskT, p);
+        step = fabs(alpha) * smax;
+        if (n == 1) {
+            step1 = step;
+            estimated_error = step;
+            if (step < congr_epsilon) { normal = 1; goto done; }
+        } else if (step < congr_epsilon) {
+            convergence_rate = exp(log(step / step1) / (double)(n - 1));
+            estimated_error = step * convergence_rate / (1.0 - convergence_rate);
+            if (estimated_error < congr_epsilon) { normal = 1; goto done; }
+        }
+    }
+done:
+    *iterations = n;
+    *converged = normal;
+    *est_error = estimated_error;
+    free(res); free(p); free(Ap);
+    return normal ? 0 : 1;
+}
+
+/* The parts of solve_non_hydrostatic around the solve (non_hydrostatic.f90:
+ * 15-39): Adams-Bashforth step of w to taup1 for k < nz (whole planes),
+ * setcyclic_xyz of u, v, w at taup1 (in place, as the reference does), and
+ * the forcing (divergence of u, v and the vertical difference of w over
+ * dzt, divided by dt) on the interior. */
+void orc_non_hydro_forcing(int nx, int ny, int nz, int cyclic_x, double dt_mom, double AB_eps,
+                           const double *dxt, const double *dyt, const double *cost,
+                           const double *cosu, const double *dzt, const double *maskW,
+                           const double *w_tau, const double *dw_tau, const double *dw_taum1,
+                           const double *dw_mix, double *u_taup1, double *v_taup1,
+                           double *w_taup1, double *forc)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    const size_t n3 = (size_t)nz * plane;
+    for (size_t q = 0; q < (size_t)(nz - 1) * plane; ++q)
+        w_taup1[q] = w_tau[q] + dt_mom * (dw_mix[q] + (1.5 + AB_eps) * dw_tau[q] -
+                                          (0.5 + AB_eps) * dw_taum1[q]) * maskW[q];
+    setcyclic_3d(nx, ny, nz, cyclic_x, u_taup1);
+    setcyclic_3d(nx, ny, nz, cyclic_x, v_taup1);
+    setcyclic_3d(nx, ny, nz, cyclic_x, w_taup1);
+    memset(forc, 0, n3 * sizeof(double));
+    for (int k = 0; k < nz; ++k) {
+        const size_t o = (size_t)k * plane;
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t q = o + IX(i, j, nx);
+                double f = (u_taup1[q] - u_taup1[q - 1]) / (cost[j + 1] * dxt[i + 1]) +
+                           (cosu[j + 1] * v_taup1[q] - cosu[j] * v_taup1[q - (size_t)(nx + 4)]) /
+                               (cost[j + 1] * dyt[j + 1]);
+                if (k == 0) f = f + w_taup1[q] / dzt[0];
+                else f = f + (w_taup1[q] - w_taup1[q - plane]) / dzt[k];
+                forc[q] = f / dt_mom;
+            }
+    }
+}
+
+/* The non-hydrostatic pressure gradient removed from u, v (interior, every
+ * level) and w (whole planes, k < nz) at taup1 (non_hydrostatic.f90:53-63). */
+void orc_non_hydro_velocity(int nx, int ny, int nz, double dt_mom, const double *dxu,
+                            const double *dyu, const double *cost, const double *dzw,
+                            const double *maskU, const double *maskV, const double *maskW,
+                            const double *p, double *u, double *v, double *w)
+{
+    const size_t plane = (size_t)(nx + 4) * (size_t)(ny + 4);
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+            for (int k = 0; k < nz; ++k) {
+                const size_t q = (size_t)k * plane + IX(i, j, nx);
+                u[q] = u[q] - dt_mom * (p[q + 1] - p[q]) / (dxu[i + 1] * cost[j + 1]) * maskU[q];
+                v[q] = v[q] - dt_mom * (p[q + (size_t)(nx + 4)] - p[q]) / dyu[j + 1] * maskV[q];
+            }
+    for (int k = 0; k < nz - 1; ++k)
+        for (size_t q2 = 0; q2 < plane; ++q2) {
+            const size_t q = (size_t)k * plane + q2;
+            w[q] = w[q] - dt_mom * (p[q + plane] - p[q]) / dzw[k] * maskW[q];
+        }
+}

@@ -169,3 +169,47 @@ def run_press(b, col, hu, hv, mom, free_surface, eps=1e-12, max_iter=1000, dt_mo
         out[name] = a.reshape((ny + 4, nx + 4) if name == "psi" else (nz, ny + 4, nx + 4))
     out.update(iterations=itts, log=p.stdout.decode(errors="replace"))
     return out
+
+
+NONHYDRO = os.path.join(HERE, "_ref", "ref_nonhydro")
+
+
+def nonhydro_available() -> bool:
+    return os.path.exists(NONHYDRO)
+
+
+def run_nonhydro(b, col, maskW, nh, eps=1e-12, max_iter=1000, dt_mom=3600.0, itt=1):
+    """Run the reference's solve_non_hydrostatic on basin b with levels col,
+    maskW and state nh (basins.NonHydro; time levels taum1=0, tau=1,
+    taup1=2).  Returns a dict of the outputs and make_coeff_non_hydro's cf as
+    (27, nz, ny+4, nx+4), plane (ii+1) + 3 (jj+1) + 9 (kk+1)."""
+    nx, ny, nz = b.nx, b.ny, col.nz
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([nx, ny, nz, int(b.cyclic_x), max_iter, itt], np.int32).tofile(f)
+            np.array([eps, dt_mom], np.float64).tofile(f)
+            for a in (b.dxt, b.dxu, b.dyt, b.dyu, b.cost, b.cosu, col.dzt, col.dzw):
+                f64(a).tofile(f)
+            for a in (col.maskT, col.maskU, col.maskV, maskW, nh.u[2], nh.v[2], nh.w[1],
+                      nh.w[2], nh.dw[1], nh.dw[0], nh.dw_mix, nh.p[1], nh.p[0]):
+                f64(a).tofile(f)
+        p = subprocess.run([NONHYDRO, fin, fout], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=3600)
+        if not os.path.exists(fout):
+            raise RuntimeError("reference solve_non_hydrostatic failed:\n" +
+                               p.stdout.decode()[-2000:])
+        raw = open(fout, "rb").read()
+    n3 = nz * (ny + 4) * (nx + 4)
+    itts = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    off = 4
+    out = {}
+    for name in ("u", "v", "w", "p", "p_tau", "p_taum1"):
+        out[name] = np.frombuffer(raw, np.float64, n3, off).copy().reshape(col.shape)
+        off += 8 * n3
+    cf = np.frombuffer(raw, np.float64, 27 * n3, off).copy()
+    # Fortran cf(i,j,k,ii,jj,kk): i fastest, kk slowest
+    out["cf"] = cf.reshape(27, nz, ny + 4, nx + 4)
+    out.update(iterations=itts, log=p.stdout.decode(errors="replace"))
+    return out

@@ -404,12 +404,12 @@ def _setcyclic(a: np.ndarray, nx: int):
     a[..., 0:2] = a[..., nx:nx + 2]
 
 
-def column_levels(b: Basin, nz: int, depth: float | None = None):
+def column_levels(b: Basin, nz: int, depth: float | None = None, stretch: float = 1.08):
     """Layer thicknesses and bottom levels of basin b: each ocean column is
     wet from the level whose centre lies above its depth ht up to the surface
     (at least one level).  Returns (dzt, dzw, kbot) with kbot 0 on land."""
     depth = float(b.ht.max()) if depth is None else depth
-    dzt, dzw = vertical_grid(nz, depth)
+    dzt, dzw = vertical_grid(nz, depth, stretch)
     ztop = np.concatenate([np.cumsum(dzt[::-1])[::-1][1:], [0.0]])   # depth of each level's top
     zc = ztop + 0.5 * dzt                                              # centre depth
     wet_levels = np.zeros(b.shape, np.int64)
@@ -420,13 +420,13 @@ def column_levels(b: Basin, nz: int, depth: float | None = None):
     return dzt, dzw, kbot
 
 
-def column_basin(b: Basin, nz: int, depth: float | None = None):
+def column_basin(b: Basin, nz: int, depth: float | None = None, stretch: float = 1.08):
     """Levels for basin b (column_levels); masks follow calc_topo
     (numerics.f90:215-247) and the inverse depths hur = 1/sum_k maskU dzt,
     hvr likewise (:253-259).  Returns the basin with hu/hv/hur/hvr and the
     surface masks replaced by those of the levels (so the 2-D solve and the
     3-D fields agree) and the Column."""
-    dzt, dzw, kbot = column_levels(b, nz, depth)
+    dzt, dzw, kbot = column_levels(b, nz, depth, stretch)
     k = np.arange(1, nz + 1)[:, None, None]
     maskT = ((kbot[None] != 0) & (kbot[None] <= k)).astype(np.float64)
     if b.cyclic_x:
@@ -493,3 +493,66 @@ def momentum_fields(col: Column, seed: int = 0) -> Momentum:
     psi = 1e6 * rng.normal(size=(3, ny4, nx4))
     dpsi = 1e4 * rng.normal(size=(3, ny4, nx4))
     return Momentum(rho, u, v, du, dv, du_mix, dv_mix, psi, dpsi)
+
+
+def mask_w(maskT: np.ndarray) -> np.ndarray:
+    """maskW of calc_topo (numerics.f90:246-249): min(maskT(k), maskT(k+1)) for
+    k < nz, maskT(nz) at the top level."""
+    w = np.array(maskT, dtype=np.float64, copy=True)
+    w[:-1] = np.minimum(maskT[:-1], maskT[1:])
+    return w
+
+
+@dataclasses.dataclass
+class NonHydro:
+    """The state solve_non_hydrostatic reads and writes (main_module.f90:
+    143-145, 216-220): u, v, w, dw and p_non_hydro with 3 time levels
+    (3, nz, ny+4, nx+4), dw_mix (nz, ny+4, nx+4)."""
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    dw: np.ndarray
+    dw_mix: np.ndarray
+    p: np.ndarray
+
+
+def nonhydro_fields(col: Column, seed: int = 0, cyclic_nx: int | None = None) -> NonHydro:
+    """Seeded synthetic non-hydrostatic state: velocities ~0.1 m/s (w ~1e-4,
+    zero at the surface), w tendencies ~1e-8 m/s^2, pressure ~1e-2 m^2/s^2;
+    all masked.  With cyclic_nx the x-halos are made periodic, as a model
+    state's are."""
+    rng = np.random.default_rng(1000 + seed)
+    nz, ny4, nx4 = col.shape
+    mW = mask_w(col.maskT)
+    sh = (3, nz, ny4, nx4)
+    u = 0.1 * rng.normal(size=sh) * col.maskU
+    v = 0.1 * rng.normal(size=sh) * col.maskV
+    w = 1e-4 * rng.normal(size=sh) * mW
+    dw = 1e-8 * rng.normal(size=sh) * mW
+    dw_mix = 1e-9 * rng.normal(size=(nz, ny4, nx4)) * mW
+    p = 1e-2 * rng.normal(size=sh) * col.maskT
+    # w at the top face is zero (rigid lid), so the divergence forcing of the
+    # closed basin sums to zero as the pressure equation needs
+    w[:, -1] = 0.0
+    dw[:, -1] = 0.0
+    dw_mix[-1] = 0.0
+    if cyclic_nx is not None:
+        for a in (u, v, w, dw, dw_mix, p):
+            _setcyclic(a, cyclic_nx)
+    return NonHydro(u, v, w, dw, dw_mix, p)
+
+
+def nonhydro_box(seed: int, nx: int, ny: int, nz: int, cyclic_x: bool = False,
+                 dx: float = 100.0, depth: float = 1000.0, stretch: float = 1.0,
+                 n_islands: int = 1):
+    """A small-scale basin for the non-hydrostatic solver: the beta-plane
+    basin's land and seamounts with the horizontal spacing scaled to dx
+    metres and the depth to `depth` (so dx is comparable to the layer
+    thickness, as in non-hydrostatic configurations), levels as in
+    column_basin.  Returns (basin, Column)."""
+    b = beta_plane_basin(seed=seed, nx=nx, ny=ny, n_islands=0 if cyclic_x else n_islands,
+                         cyclic_x=cyclic_x)
+    s = dx / 20e3
+    b = dataclasses.replace(b, dxt=b.dxt * s, dxu=b.dxu * s, dyt=b.dyt * s, dyu=b.dyu * s,
+                            ht=b.ht * (depth / b.ht.max()))
+    return column_basin(b, nz, depth=depth, stretch=stretch)

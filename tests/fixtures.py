@@ -67,3 +67,26 @@ def load_press(name):
                    np.stack([d["dv_taum1"], d["dv_tau"], z3]), d["du_mix"], d["dv_mix"],
                    np.stack([d["psi_taum1"], d["psi_tau"], z2]), np.stack([z2, z2, z2]))
     return b, col, mom, d
+
+
+NH_NAMES = ["nh_box_20x16_nz10", "nh_cyclic_22x14_nz8", "nh_stretched_18x16_nz10"]
+
+
+def load_nonhydro(name):
+    """A non-hydrostatic fixture: (basin, Column, maskW, NonHydro state with
+    taum1=0/tau=1/taup1=2, dict)."""
+    from pyom2_amd.basins import Column, NonHydro
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    d = {k: z[k] for k in z.files}
+    nx, ny, nz = int(d["nx"]), int(d["ny"]), int(d["nz"])
+    kbot = d["kbot_levels"]
+    b = Basin(nx, ny, bool(d["cyclic_x"]), d["dxt"], d["dxu"], d["dyt"], d["dyu"], d["cost"],
+              d["cosu"], kbot, None, None, None, None, None, d["maskT"][-1], d["maskU"][-1],
+              d["maskV"][-1], np.zeros((ny + 4, nx + 4)), name)
+    col = Column(nz, d["dzt"], d["dzw"], kbot, d["maskT"], d["maskU"], d["maskV"])
+    z3 = np.zeros((nz, ny + 4, nx + 4))
+    nh = NonHydro(np.stack([z3, z3, d["u_taup1"]]), np.stack([z3, z3, d["v_taup1"]]),
+                  np.stack([z3, d["w_tau"], d["w_taup1"]]),
+                  np.stack([d["dw_taum1"], d["dw_tau"], z3]), d["dw_mix"],
+                  np.stack([d["p_taum1"], d["p_tau"], z3]))
+    return b, col, d["maskW"], nh, d

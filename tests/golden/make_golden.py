@@ -13,6 +13,8 @@ line integrals of a seeded random (u, v) pair.
 
 The ``press_*`` fixtures come from oracle/_ref/ref_press, the reference's
 solve_pressure (surface-pressure mode, free surface and rigid lid).
+The ``nh_*`` fixtures come from oracle/_ref/ref_nonhydro, the reference's
+solve_non_hydrostatic (and its make_coeff_non_hydro coefficients).
 The ``step_*`` fixtures come from oracle/_ref/ref_step, the reference's
 whole solve_streamfunction on a basin with vertical levels and no islands:
 inputs (levels, masks, the model state at tau/taum1) and outputs (p_hydro,
@@ -130,6 +132,40 @@ PRESS_CASES = {
 }
 
 
+NH_CASES = {
+    # (seed, nx, ny, nz, cyclic, dx, depth, stretch, itt, dt)
+    "nh_box_20x16_nz10": (1, 20, 16, 10, False, 50.0, 1000.0, 1.0, 1, 60.0),
+    "nh_cyclic_22x14_nz8": (2, 22, 14, 8, True, 100.0, 800.0, 1.0, 0, 60.0),
+    "nh_stretched_18x16_nz10": (1, 18, 16, 10, False, 100.0, 1000.0, 1.05, 1, 60.0),
+}
+
+
+def make_nonhydro(name):
+    seed, nx, ny, nz, cyc, dx, depth, stretch, itt, dt = NH_CASES[name]
+    b, col = basins.nonhydro_box(seed, nx, ny, nz, cyc, dx, depth, stretch)
+    mW = basins.mask_w(col.maskT)
+    nh = basins.nonhydro_fields(col, seed, b.nx if cyc else None)
+    eps, max_iter = 1e-12, 5000
+    r = ref_run.run_nonhydro(b, col, mW, nh, eps, max_iter, dt, itt)
+    out = dict(
+        nx=b.nx, ny=b.ny, nz=nz, cyclic_x=int(b.cyclic_x), eps=eps, max_iter=max_iter, dt_mom=dt,
+        itt=itt, dxt=b.dxt, dxu=b.dxu, dyt=b.dyt, dyu=b.dyu, cost=b.cost, cosu=b.cosu,
+        dzt=col.dzt, dzw=col.dzw, kbot_levels=col.kbot, maskT=col.maskT, maskU=col.maskU,
+        maskV=col.maskV, maskW=mW, u_taup1=nh.u[2], v_taup1=nh.v[2], w_tau=nh.w[1],
+        w_taup1=nh.w[2], dw_tau=nh.dw[1], dw_taum1=nh.dw[0], dw_mix=nh.dw_mix, p_tau=nh.p[1],
+        p_taum1=nh.p[0], out_u=r["u"], out_v=r["v"], out_w=r["w"], out_p=r["p"],
+        out_p_tau=r["p_tau"], out_p_taum1=r["p_taum1"], iterations=r["iterations"],
+        ref_warning=int("WARNING" in r["log"]),
+    )
+    # the 7 non-zero planes of make_coeff_non_hydro's cf (below, S, W, C, E, N, above)
+    out["cf7"] = r["cf"][[4, 10, 12, 13, 14, 16, 22]]
+    out["cf_other_max"] = np.abs(np.delete(r["cf"], [4, 10, 12, 13, 14, 16, 22], axis=0)).max()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: iterations={r['iterations']} warning={out['ref_warning']} -> "
+          f"{os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def make_press(name):
     b0, nz, seed, fs = PRESS_CASES[name]()
     b, col = basins.column_basin(b0, nz)
@@ -185,5 +221,7 @@ def make(name):
 if __name__ == "__main__":
     if not ref_run.build():
         sys.exit("reference sources not available: cannot build oracle/_ref")
-    for n in (sys.argv[1:] or list(CASES) + list(STEP_CASES) + list(PRESS_CASES)):
-        (make_step if n in STEP_CASES else make_press if n in PRESS_CASES else make)(n)
+    for n in (sys.argv[1:] or list(CASES) + list(STEP_CASES) + list(PRESS_CASES) +
+              list(NH_CASES)):
+        (make_step if n in STEP_CASES else make_press if n in PRESS_CASES else
+         make_nonhydro if n in NH_CASES else make)(n)
