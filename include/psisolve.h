@@ -343,6 +343,61 @@ int psi_solve_pressure(psi_press *ctx, const psi_opts *opts, const psi_levels *l
  * us[3] the two folds; n >= 4), HIP events; call between solves. */
 int psi_press_kernel_times(psi_press *ctx, int32_t reps, double *us, int32_t n);
 
+/* ---- the non-hydrostatic pressure solver (for_src/non_hydrostatic/
+ * non_hydrostatic.f90) ---------------------------------------------------------
+ * Used when enable_hydrostatic is off (momentum.f90:167): the 3-D Poisson
+ * problem of make_coeff_non_hydro (7-point, Neumann through the masks) solved
+ * by congrad_non_hydro (conjugate gradients without a preconditioner).  3-D
+ * fields are nz device planes as in psi_levels; fields with time levels are
+ * 3 such blocks. */
+typedef struct psi_nh psi_nh;
+
+/* The context of one basin: grid, levels (dzt, dzw; maskT, maskU, maskV) and
+ * maskW (nz device planes, main_module.f90:96-99).  Every mask value must be 0
+ * or 1; nz >= 2 (make_coeff_non_hydro, non_hydrostatic.f90:72-141, reads
+ * maskW(:,:,k-1) and dzw(k-1) at k = nz).  The coefficients are rebuilt in the
+ * kernels from the masks and 1-D metric factors, as the reference's SAVE'd cf
+ * holds them (:159-163). */
+int psi_nh_create(psi_nh **out, const psi_grid *grid, const psi_levels *lev,
+                  const double *maskW, void *stream);
+void psi_nh_destroy(psi_nh *ctx);
+int psi_nh_set_stream(psi_nh *ctx, void *stream);
+const char *psi_nh_last_error(const psi_nh *ctx);
+int64_t psi_nh_device_bytes(const psi_nh *ctx);
+/* The coefficients: 7 blocks of nz device planes, in apply_op_3D's order —
+ * cf(:,:,:,2,2,1) (below), (2,1,2) (south), (1,2,2) (west), (2,2,2) (centre),
+ * (3,2,2) (east), (2,3,2) (north), (2,2,3) (above); the other 20 entries of
+ * the reference's cf(:,:,:,3,3,3) are zero. */
+int psi_make_coeff_non_hydro(psi_nh *ctx, double *cf);
+/* apply_op_3D (non_hydrostatic.f90:247-270): res = A p on the interior of
+ * every level, zero elsewhere. */
+int psi_apply_op_3d(psi_nh *ctx, const double *p, double *res);
+/* congrad_non_hydro (non_hydrostatic.f90:145-242): p (nz device planes, in
+ * place) is the first guess and the result; the reference's tests in its
+ * order (p.Ap = 0, divergence of r.r, the extrapolated step bound
+ * congr_epsilon_non_hydro, the iteration cap; no NaN test).  opts: mode
+ * PSI_MODE_PLAIN, criterion PSI_CRIT_REFERENCE, congr_epsilon (the
+ * reference's congr_epsilon_non_hydro), max_iterations
+ * (congr_max_itts_non_hydro), check_every.  A zero initial residual returns 0
+ * iterations.  Returns PSI_OK, PSI_NOT_CONVERGED or PSI_DIVERGED. */
+int psi_congrad_non_hydro(psi_nh *ctx, const psi_opts *opts, const double *forc, double *p,
+                          psi_info *info);
+/* The whole solve_non_hydrostatic (non_hydrostatic.f90:6-64, serial): w at
+ * taup1 by Adams-Bashforth (k < nz), setcyclic_xyz of u, v, w at taup1 (in
+ * place, as the reference), the forcing (into forc_out when given), the
+ * first guess p(taup1) = 2 p(tau) - p(taum1), congrad_non_hydro,
+ * setcyclic_xyz, at itt = 0 p(taup1) copied to tau and taum1, and the
+ * pressure gradient removed from u, v, w at taup1.  u, v, w, dw, p_nh: 3 x nz
+ * planes; dw_mix: nz planes.  Returns the solve's status. */
+int psi_solve_non_hydrostatic(psi_nh *ctx, const psi_opts *opts, double *u, double *v,
+                              double *w, const double *dw, const double *dw_mix, double *p_nh,
+                              int32_t taum1, int32_t tau, int32_t taup1, double dt_mom,
+                              double AB_eps, int32_t itt, double *forc_out, psi_info *info);
+/* Performance report: average duration in microseconds of `reps` launches of
+ * the iteration's kernels (us[0] direction + apply, us[1] update, us[2] and
+ * us[3] the two folds; n >= 4), HIP events; call between solves. */
+int psi_nh_kernel_times(psi_nh *ctx, int32_t reps, double *us, int32_t n);
+
 /* ---- flexible CG with the learned preconditioner ---------------------------
  * No counterpart in the reference.  The host applies the network of psinn.h
  * to the current residual between psi_fcg_input_scale and psi_fcg_step.

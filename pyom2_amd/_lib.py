@@ -175,6 +175,21 @@ SIGNATURES = [
                                      C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                      C.c_void_p, C.POINTER(PsiInfo)]),
     ("psi_press_kernel_times", C.c_int, [C.c_void_p, C.c_int32, _dp, C.c_int32]),
+    # the non-hydrostatic pressure solver
+    ("psi_nh_create", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PsiGrid), C.POINTER(PsiLevels),
+                                C.c_void_p, C.c_void_p]),
+    ("psi_nh_destroy", None, [C.c_void_p]),
+    ("psi_nh_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_nh_last_error", C.c_char_p, [C.c_void_p]),
+    ("psi_nh_device_bytes", C.c_int64, [C.c_void_p]),
+    ("psi_make_coeff_non_hydro", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psi_apply_op_3d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("psi_congrad_non_hydro", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.c_void_p, C.c_void_p,
+                                        C.POINTER(PsiInfo)]),
+    ("psi_solve_non_hydrostatic", C.c_int, [C.c_void_p, C.POINTER(PsiOpts)] + [C.c_void_p] * 6 +
+     [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_void_p,
+      C.POINTER(PsiInfo)]),
+    ("psi_nh_kernel_times", C.c_int, [C.c_void_p, C.c_int32, _dp, C.c_int32]),
     ("psi_nn_anorm_loss", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_nn_weighted_dot", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),

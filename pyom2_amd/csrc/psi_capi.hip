@@ -2019,5 +2019,7 @@ int psi_fcg_scalars(psi_ctx* c, double* out10) {
 #include "../../include/psibatch.h"
 #include "psi_batch.hip"
 
-// The surface-pressure solver (psi_press_*) likewise.
+// The surface-pressure solver (psi_press_*) and the non-hydrostatic one
+// (psi_nh_*) likewise.
 #include "psi_press.hip"
+#include "psi_nonhydro.hip"

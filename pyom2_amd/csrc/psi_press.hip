@@ -19,7 +19,8 @@ namespace psi {
 // fold after that apply, when max|p| of the new direction is known.
 struct SpScal {
     double rsold, alpha, beta, rs_min, step1, est, eps;
-    int n, done, status, iterations, max_it, pad;
+    int n, done, status, iterations, max_it;
+    int nan_test;   // congrad_surf_press ends on a NaN estimate; congrad_non_hydro does not
 };
 
 // Workgroups of the grid-stride passes (k_sp_dir: measured at 2048^2, 51 us
@@ -29,6 +30,15 @@ constexpr int SP_DBLOCKS = 8192;
 
 __device__ __forceinline__ bool sp_inner(const Geom& g, int row, int col) {
     return row >= 2 && row <= g.ny + 1 && col >= 2 && col <= g.nx + 1;
+}
+
+// Row and column of plane offset q in 32-bit arithmetic (planes hold fewer
+// than 2^31 points; a 64-bit division costs far more).
+__device__ __forceinline__ void sp_rc(const Geom& g, size_t q, int& row, int& col) {
+    const unsigned u = (unsigned)q, P = (unsigned)g.pitch;
+    const unsigned r = u / P;
+    row = (int)r;
+    col = (int)(u - r * P);
 }
 
 // apply_op (solve_pressure.f90:254-274) at one interior point: the nine terms
@@ -169,7 +179,8 @@ __global__ void __launch_bounds__(256) k_sp_init(
     double acc[1] = {0.0};
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
          q += (size_t)gridDim.x * blockDim.x) {
-        const int row = (int)(q / P), col = (int)(q - (size_t)row * P);
+        int row, col;
+        sp_rc(g, q, row, col);
         double rv = 0.0;
         if (sp_inner(g, row, col)) {
             const double m = (double)m8[q];
@@ -194,7 +205,8 @@ __global__ void __launch_bounds__(256) k_sp_apply(
     const size_t P = g.pitch;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= plane) return;
-    const int row = (int)(q / P), col = (int)(q - (size_t)row * P);
+    int row, col;
+    sp_rc(g, q, row, col);
     double a = 0.0;
     if (sp_inner(g, row, col))
         a = sp_apply(cE[q], cW[q], cN[q], cS[q], (double)m8[q], fs, kfs, p[q - P], p[q - 1],
@@ -243,7 +255,8 @@ __global__ void __launch_bounds__(256) k_sp_dir(
     double acc[2] = {0.0, 0.0};
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
          q += (size_t)gridDim.x * blockDim.x) {
-        const int row = (int)(q / P), col = (int)(q - (size_t)row * P);
+        int row, col;
+        sp_rc(g, q, row, col);
         size_t src = q;
         if (g.cyclic) {
             if (col < 2) src = q + nx;
@@ -292,7 +305,8 @@ __global__ void __launch_bounds__(256) k_sp_update2(Geom g, const uint8_t* __res
         *(double2*)(x + q) = make_double2(xv.x + alpha * pv.x, xv.y + alpha * pv.y);
         const double2 rn = make_double2(rv.x - alpha * av.x, rv.y - alpha * av.y);
         *(double2*)(r + q) = rn;
-        const int row = (int)(q / P), col = (int)(q - (size_t)row * P);
+        int row, col;
+        sp_rc(g, q, row, col);
         if (row >= 2 && row <= g.ny + 1) {
             const uchar2 mm = *(const uchar2*)(m8 + q);
             if (col >= 2 && col <= g.nx + 1) acc[0] = acc[0] + rn.x * rn.x * (double)mm.x;
@@ -325,7 +339,7 @@ __global__ void __launch_bounds__(1024) k_sp_fold_a(const double* __restrict__ p
             S->est = step * rate / (1.0 - rate);
             if (S->est < S->eps) return sp_finish(S, ST_CONVERGED, m);
         }
-        if (S->est != S->est) return sp_finish(S, ST_NAN, m);
+        if (S->nan_test && S->est != S->est) return sp_finish(S, ST_NAN, m);
         if (m >= S->max_it) return sp_finish(S, ST_MAXIT, S->max_it + 1);
     }
     if (v[0] == 0.0) return sp_finish(S, ST_CONVERGED, n);
@@ -351,7 +365,8 @@ __global__ void __launch_bounds__(256) k_sp_update(Geom g, const uint8_t* __rest
         x[q] = x[q] + alpha * p[q];
         const double rn = r[q] - alpha * Ap[q];
         r[q] = rn;
-        const int row = (int)(q / P), col = (int)(q - (size_t)row * P);
+        int row, col;
+        sp_rc(g, q, row, col);
         if (sp_inner(g, row, col)) acc[0] = acc[0] + rn * rn * (double)m8[q];
     }
     sp_block_partials<1, 1>(acc, part);
@@ -590,6 +605,7 @@ int press_congrad(psi_press* c, const psi_opts* o, const double* forc, double* x
     S0.eps = o->congr_epsilon;
     S0.max_it = o->max_iterations;
     S0.n = 1;
+    S0.nan_test = 1;
     *c->h_S = S0;
     HIPCHK(c, hipMemcpyAsync(c->S, c->h_S, sizeof(SpScal), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_sp_init, dim3(SP_BLOCKS), dim3(256), 0, c->stream, g,
@@ -673,6 +689,8 @@ int psi_press_create(psi_press** out, const psi_grid* grid, const psi_press_fiel
         return fail(nullptr, PSI_BAD_ARG, "bad grid%s");
     if (f->free_surface && !(f->grav * f->dt_mom * f->dt_mom != 0.0))
         return fail(nullptr, PSI_BAD_ARG, "free surface needs grav and dt_mom%s");
+    if ((double)(grid->nx + 4) * (double)(grid->ny + 4) >= 2147483648.0)
+        return fail(nullptr, PSI_BAD_ARG, "a plane must hold fewer than 2^31 points%s");
     psi_press* c = new psi_press();
     c->stream = (hipStream_t)stream;
     (void)hipGetDevice(&c->device);
