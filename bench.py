@@ -96,9 +96,12 @@ def parse():
     ap.add_argument("--nz", type=int, default=40, help="levels of the step workload")
     ap.add_argument("--step-mode", default="mg", choices=["mg", "poly", "mixed", "plain"],
                     help="solve path of the step workload")
+    ap.add_argument("--nh-n", type=int, default=512, help="nonhydro workload: nx = ny")
+    ap.add_argument("--nh-nz", type=int, default=32, help="nonhydro workload: levels")
     ap.add_argument("--free-surface", type=int, default=1,
                     help="press workload: enable_free_surface (0: rigid lid)")
-    ap.add_argument("--workload", default="solve", choices=["solve", "ensemble", "step", "press"],
+    ap.add_argument("--workload", default="solve",
+                    choices=["solve", "ensemble", "step", "press", "nonhydro"],
                     help="solve: the 2048^2 two-island solve (headline); ensemble: 512 "
                          "batched basins of 64^2-512^2")
     ap.add_argument("--members", type=int, default=512)
@@ -142,6 +145,8 @@ def main():
         return step_main(args)
     if args.workload == "press":
         return press_main(args)
+    if args.workload == "nonhydro":
+        return nonhydro_main(args)
     import torch
 
     from pyom2_amd import basins
@@ -762,6 +767,139 @@ def press_main(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(finite(line)), flush=True)
+
+
+# algorithmic bytes per point of the non-hydrostatic CG kernels: k_nh_dir reads
+# res, the old direction and the packed mask byte, writes the new direction and
+# A p (the operator is rebuilt from the mask and 1-D factors); k_nh_update as
+# k_sp_update.
+NH_BYTES = {"dir_apply": 8 + 8 + 1 + 16, "update": 32 + 1 + 16}
+
+
+def nonhydro_main(args):
+    """The whole solve_non_hydrostatic per step (non_hydrostatic.f90:6-64): w
+    step, forcing, congrad_non_hydro with the reference's stopping rule
+    (congr_epsilon_non_hydro 1e-12 on the extrapolated step) from the
+    2 p(tau) - p(taum1) first guess, velocity correction; a 512^2 x 32-level
+    basin at 50 m horizontal and 25 m vertical spacing (basins.nonhydro_box),
+    seeded state, tendencies of w redrawn each step (outside the timed calls)."""
+    import torch
+
+    from pyom2_amd import basins
+    from pyom2_amd.nonhydro import NonHydrostaticSolver
+    from pyom2_amd.streamfunction import SolveOptions
+
+    dev = torch.device("cuda:0")
+    calibration_copy(args.copy_mib, dev)
+    n, nz, dt = args.nh_n, args.nh_nz, 60.0
+    t = time.time()
+    b, col = basins.nonhydro_box(0, n, n, nz, False, 50.0, 25.0 * nz, 1.0, n_islands=2)
+    mW = basins.mask_w(col.maskT)
+    nh = basins.nonhydro_fields(col, 0)
+    s = NonHydrostaticSolver.from_column(b, col, mW, device=dev)
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    u, v, w, dw, dw_mix, p = (T(a) for a in (nh.u, nh.v, nh.w, nh.dw, nh.dw_mix, nh.p))
+    u0, v0 = u.clone(), v.clone()
+    mWd = T(mW)
+    opts = SolveOptions(congr_epsilon=1e-12, max_iterations=200000)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    torch.cuda.synchronize()
+    log(f"[nonhydro] basin {b.nx}x{b.ny} x {nz} levels, set up in {time.time() - t:.1f}s")
+    taum1, tau, taup1 = 0, 1, 2
+    per_step, its, solve_ms = [], [], []
+    for k in range(args.warmup + args.steps):
+        # u, v at taup1 as the hydrostatic step leaves them; new w tendencies
+        u[taup1] = u0[taup1]
+        v[taup1] = v0[taup1]
+        dw[tau] = 1e-8 * torch.randn(col.shape, generator=g, dtype=torch.float64,
+                                     device=dev) * mWd
+        dw[tau][-1] = 0.0
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        n_it, conv, info = s.solve_non_hydrostatic(u, v, w, dw, dw_mix, p, dt, taum1, tau,
+                                                   taup1, 1, opts)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        if not conv:
+            raise SystemExit(f"step {k}: non-hydrostatic solve did not converge ({info})")
+        if k >= args.warmup:
+            per_step.append(te - ts)
+            its.append(n_it)
+            solve_ms.append(info["elapsed_ms"])
+        log(f"[nonhydro] {k}: {(te - ts) * 1e3:.2f} ms, solve {info['elapsed_ms']:.2f} ms, "
+            f"{n_it} iterations, est. error {info['est_error']:.2e}")
+        taum1, tau, taup1 = tau, taup1, taum1
+    kt = s.kernel_times(reps=20)
+    pts = nz * b.shape[0] * b.shape[1]
+    kern = {}
+    for name in ("dir_apply", "update"):
+        gbs = NH_BYTES[name] * pts / (kt[name] * 1e-6) / 1e9
+        kern[name] = {"us": round(kt[name], 2), "achieved": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_pt": NH_BYTES[name]}
+    kern["fold_a_us"] = round(kt["fold_a"], 2)
+    kern["fold_b_us"] = round(kt["fold_b"], 2)
+    dom = max(("dir_apply", "update"), key=lambda x: kt[x])
+    total = sum(per_step)
+    it_mean = float(np.mean(its))
+    cpu = None if args.skip_cpu else nonhydro_cpu_baseline(b, col, mW, nh, it_mean,
+                                                           budget_s=args.cpu_seconds)
+    line = {
+        "metric": f"ms per {n}^2 x {nz}-level solve_non_hydrostatic step (w step, forcing, "
+                  f"congrad_non_hydro to congr_epsilon_non_hydro 1e-12, velocity correction)",
+        "value": round(1e3 * total / args.steps, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"basin {b.nx}x{b.ny}, {nz} levels, 50 m x 25 m spacing, "
+                               f"2 islands, dt 60 s, first guess 2 p(tau) - p(taum1)"},
+        "solve_ms_mean": round(float(np.mean(solve_ms)), 3),
+        "iterations_mean": round(it_mean, 1),
+        "us_per_iteration": round(1e3 * float(np.mean(solve_ms)) / max(it_mean, 1.0), 2),
+        "nonhydro_kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": "k_nh_dir" if dom == "dir_apply" else
+                     "k_nh_update", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(finite(line)), flush=True)
+
+
+def nonhydro_cpu_baseline(b, col, mW, nh, iterations, budget_s=10.0):
+    """The oracle's congrad_non_hydro on the same operator (1 core) for about
+    budget_s, per iteration, scaled to the GPU run's iteration count, plus the
+    oracle's forcing and velocity correction once."""
+    from oracle import oracle as O
+    from pyom2_amd.streamfunction import AB_EPS_REFERENCE
+    t0 = time.perf_counter()
+    u, v, w, forc = O.non_hydro_forcing(b, col, mW, 60.0, AB_EPS_REFERENCE, nh.w[1], nh.dw[1],
+                                        nh.dw[0], nh.dw_mix, nh.u[2], nh.v[2], nh.w[2])
+    O.non_hydro_velocity(b, col, mW, 60.0, nh.p[1], u, v, w)
+    lev_s = time.perf_counter() - t0
+    cf = O.make_coeff_non_hydro(b, col, mW)
+    k = 2
+    while True:
+        t2 = time.perf_counter()
+        O.congrad_non_hydro(b, col, cf, forc, nh.p[1], 1e-30, k)
+        el = time.perf_counter() - t2
+        if el * 4 > budget_s or k >= 4096:
+            break
+        k *= 4
+    per_it = el / k
+    total = lev_s + per_it * iterations
+    return {"value": round(1e3 * total, 1), "unit": "ms per step (extrapolated)", "cores": 1,
+            "kind": "port",
+            "sample": f"{k} CG iterations on the same {b.nx}x{b.ny}x{col.nz} operator "
+                      f"({per_it * 1e3:.1f} ms each) x {iterations:.0f} iterations of the GPU "
+                      f"run, plus forcing and velocity correction once ({lev_s:.1f} s); C "
+                      f"restatement, 1 thread"}
 
 
 def press_cpu_baseline(b, host_op, nz, iterations, fs, budget_s=10.0, n3=512):

@@ -22,7 +22,8 @@ enum : uint8_t { NH_MU = 1, NH_MV = 2, NH_MW = 4, NH_MT = 8 };
 // 1-D factors of make_coeff_non_hydro (non_hydrostatic.f90:87-141) with mp = 1:
 // ex_e(i) = (1/dxu(i))/dxt(i), ex_w(i) = (1/dxu(i-1))/dxt(i), c2(j) = cost(j)^2,
 // tn(j) = ((1/dyu(j))/dyt(j) * cosu(j))/cost(j), ts(j) likewise with j-1,
-// tu(k) = (1/dzw(k))/dzt(k), td(k) = (1/dzw(k-1))/dzt(k).
+// tu(k) = (1/dzw(k))/dzt(k), td(k) = (1/dzw(k-1))/dzt(k).  (Keeping ex/c2 as
+// two 2-D planes instead of dividing in the kernel measured no faster.)
 struct NhTables {
     const double *ex_e, *ex_w, *c2, *tn, *ts, *tu, *td;
 };
@@ -131,12 +132,14 @@ __global__ void __launch_bounds__(256) k_nh_apply(Geom g, int nz, NhTables t,
 }
 
 // res = forc - A x on the interior, zero elsewhere (non_hydrostatic.f90:169-177),
-// and the partials of dot_3D(res, res) (:181).  x's halo is read as it stands.
+// and the partials of dot_3D(res, res) (:181).  x's halo is read as it stands;
+// p0 (the direction buffer iteration 1 reads as the old one) is zeroed.
 __global__ void __launch_bounds__(256) k_nh_init(Geom g, int nz, NhTables t,
                                                  const uint8_t* __restrict__ m8,
                                                  const double* __restrict__ forc,
                                                  const double* __restrict__ x,
                                                  double* __restrict__ r,
+                                                 double* __restrict__ p0,
                                                  double* __restrict__ part) {
 #pragma clang fp contract(off)
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
@@ -160,6 +163,7 @@ __global__ void __launch_bounds__(256) k_nh_init(Geom g, int nz, NhTables t,
             acc[0] = acc[0] + rv * rv * (double)((m >> 3) & 1);
         }
         r[o] = rv;
+        p0[o] = 0.0;   // the old direction of iteration 1
     }
     // one partial per workgroup of the (gridDim.x, nz) grid
     const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
@@ -174,48 +178,52 @@ __global__ void __launch_bounds__(256) k_nh_init(Geom g, int nz, NhTables t,
 // The direction of iteration n, p = res + (rsnew/rsold) p (:190; p = res for
 // n = 1, :179) with setcyclic_xyz (:191-192), Ap = A p (:185; zero off the
 // interior), and the partials of dot_3D(p, Ap) (:186) and absmax_3D(p)
-// (:215).  Partials: slot 0 the sums, slot 1 the maxima, one per workgroup
-// of the (gridDim.x, nz) grid.
+// (:215).  Workgroup row blockIdx.y is level k; within a level, consecutive
+// chunks of the plane go to the workgroups of one XCD (gridDim.x is a
+// multiple of 8, so the chunk's levels above and below run on that XCD too),
+// and its L2 serves the neighbours.  Partials: slot 0 the sums, slot 1 the
+// maxima, one per workgroup.
 __global__ void __launch_bounds__(256) k_nh_dir(
     Geom g, int nz, NhTables t, const uint8_t* __restrict__ m8, const double* __restrict__ r,
     const double* __restrict__ p_old, double* __restrict__ p_new, double* __restrict__ Ap,
     const SpScal* __restrict__ S, double* __restrict__ part) {
 #pragma clang fp contract(off)
     if (S->done) return;
-    const bool first = S->n == 1;
+    // iteration 1: beta = 0 and the old direction was zeroed by k_nh_init
     const double beta = S->beta;
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
     const size_t P = g.pitch, nx = (size_t)g.nx;
     const int k = blockIdx.y;
     const size_t base = (size_t)k * plane;
-    auto dir = [&](size_t o) -> double { return first ? r[o] : r[o] + beta * p_old[o]; };
+    // vertical neighbour offsets (zero where the reference's clamped index
+    // meets a zero coefficient) and an interior stand-in for halo points
+    const size_t dn = k > 0 ? plane : 0, up = k < nz - 1 ? plane : 0;
+    const size_t q_in = 2 * P + 2;
+    const bool cyc = g.cyclic != 0;
+    auto dir = [&](size_t o) -> double { return r[o] + beta * p_old[o]; };
     double s_dot = 0.0, s_max = 0.0;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
+    const size_t chunk = (size_t)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    for (size_t q = chunk * blockDim.x + threadIdx.x; q < plane;
          q += (size_t)gridDim.x * blockDim.x) {
         int row, col;
         sp_rc(g, q, row, col);
         const size_t o = base + q;
-        size_t src = o;
-        if (g.cyclic) {
-            if (col < 2) src = o + nx;
-            else if (col >= g.nx + 2) src = o - nx;
-        }
+        const size_t src = o + ((cyc && col < 2) ? nx : 0) - ((cyc && col >= g.nx + 2) ? nx : 0);
         const double v = dir(src);
         p_new[o] = v;
-        double a = 0.0;
-        if (sp_inner(g, row, col)) {
-            const size_t ow = (g.cyclic && col == 2) ? o - 1 + nx : o - 1;
-            const size_t oe = (g.cyclic && col == g.nx + 1) ? o + 1 - nx : o + 1;
-            const uint8_t m = m8[o];
-            const NhCoef f = nh_coef(t, row, col, k, nz, m, m8[o - 1], m8[o - P],
-                                     k > 0 ? m8[o - plane] : 0);
-            a = nh_apply(f, k > 0 ? dir(o - plane) : v, dir(o - P), dir(ow), v, dir(oe),
-                         dir(o + P), k < nz - 1 ? dir(o + plane) : v);
-            const double mt = (double)((m >> 3) & 1);
-            s_dot = s_dot + v * a * mt;
-            s_max = fmax(fabs(v * mt), s_max);
-        }
-        Ap[o] = a;
+        const bool inner = sp_inner(g, row, col);
+        const size_t oc = base + (inner ? q : q_in);
+        const size_t ow = oc - 1 + ((cyc && col == 2) ? nx : 0);
+        const size_t oe = oc + 1 - ((cyc && col == g.nx + 1) ? nx : 0);
+        const uint8_t m = m8[oc];
+        const uint8_t md = m8[oc - dn];
+        const NhCoef f = nh_coef(t, row, col, k, nz, m, m8[oc - 1], m8[oc - P], md);
+        const double a = nh_apply(f, dir(oc - dn), dir(oc - P), dir(ow), v, dir(oe), dir(oc + P),
+                                  dir(oc + up));
+        const double mt = inner ? (double)((m >> 3) & 1) : 0.0;
+        Ap[o] = inner ? a : 0.0;
+        s_dot = s_dot + v * (inner ? a : 0.0) * mt;
+        s_max = fmax(fabs(v * mt), s_max);
     }
     double acc[2] = {s_dot, s_max};
     __shared__ double sh[8];
@@ -470,7 +478,7 @@ int nh_congrad(psi_nh* c, const psi_opts* o, const double* forc, double* x, psi_
     HIPCHK(c, hipMemcpyAsync(c->S, c->h_S, sizeof(SpScal), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_nh_init, dim3((unsigned)c->gx, (unsigned)c->nz), dim3(256), 0,
                        c->stream, c->g, c->nz, c->T, (const uint8_t*)c->m8, forc,
-                       (const double*)x, c->r, c->part);
+                       (const double*)x, c->r, c->pv[0], c->part);
     hipLaunchKernelGGL(k_sp_fold_init, dim3(1), dim3(1024), 0, c->stream, (const double*)c->part,
                        nb, c->S);
     rc = check_launch(c);
@@ -561,8 +569,10 @@ int psi_nh_create(psi_nh** out, const psi_grid* grid, const psi_levels* lev,
     c->plane = (size_t)(g.nx + 4) * (g.ny + 4);
     c->nz = lev->nz;
     c->n3 = (size_t)c->nz * c->plane;
-    c->gx = std::max(1, 8192 / c->nz);
-    c->gx = std::min<int>(c->gx, (int)nblk(c->plane));
+    // workgroups per level: a multiple of 8 (the XCD chunk mapping), about
+    // 8192 in all, at most one per 256 points of a level
+    c->gx = std::max(8, (8192 / c->nz) & ~7);
+    c->gx = std::min<int>(c->gx, std::max<int>(8, (int)nblk(c->plane) & ~7));
     const size_t nxh = g.nx + 4, nyh = g.ny + 4, nz = (size_t)c->nz;
     c->h_dxt.assign(grid->dxt, grid->dxt + nxh);
     c->h_dxu.assign(grid->dxu, grid->dxu + nxh);
@@ -652,6 +662,7 @@ int psi_nh_create(psi_nh** out, const psi_grid* grid, const psi_levels* lev,
         c->T.td = c->T.tu + nz;
         c->d_dzt = c->T.td + nz;
         c->d_dzw = c->d_dzt + nz;
+
         hipLaunchKernelGGL(k_nh_masks, dim3(nblk(c->n3)), dim3(256), 0, c->stream, c->n3,
                            lev->maskT, lev->maskU, lev->maskV, maskW, c->m8, c->d_bad);
         TRY(check_launch(c));

@@ -168,11 +168,13 @@ __device__ __forceinline__ void sp_fold(const double* __restrict__ part, int nbl
 
 // res = forc - A psi on the interior, zero on the halo (:163-169), and the
 // partial sums of dot_sfp(res, res) (:172).  psi's halo is read as it stands.
+// p0 (the direction buffer iteration 1 reads as the old one) is zeroed.
 __global__ void __launch_bounds__(256) k_sp_init(
     Geom g, const double* __restrict__ cE, const double* __restrict__ cW,
     const double* __restrict__ cN, const double* __restrict__ cS,
     const uint8_t* __restrict__ m8, int fs, double kfs, const double* __restrict__ forc,
-    const double* __restrict__ x, double* __restrict__ r, double* __restrict__ part) {
+    const double* __restrict__ x, double* __restrict__ r, double* __restrict__ p0,
+    double* __restrict__ part) {
 #pragma clang fp contract(off)
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
     const size_t P = g.pitch;
@@ -190,6 +192,7 @@ __global__ void __launch_bounds__(256) k_sp_init(
             acc[0] = acc[0] + rv * rv * m;
         }
         r[q] = rv;
+        p0[q] = 0.0;   // the old direction of iteration 1
     }
     sp_block_partials<1, 1>(acc, part);
 }
@@ -246,38 +249,36 @@ __global__ void __launch_bounds__(256) k_sp_dir(
     const SpScal* __restrict__ S, double* __restrict__ part) {
 #pragma clang fp contract(off)
     if (S->done) return;
-    const bool first = S->n == 1;
+    // iteration 1: beta = 0 and the old direction was zeroed by k_sp_init, so
+    // p = res + 0 (the reference's p = res but for the sign of a zero)
     const double beta = S->beta;
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
     const size_t P = g.pitch;
     const size_t nx = (size_t)g.nx;
-    auto dir = [&](size_t q) -> double { return first ? r[q] : r[q] + beta * p_old[q]; };
+    const size_t q_in = 2 * P + 2;   // an interior point: stands in for the halo's neighbours
+    auto dir = [&](size_t q) -> double { return r[q] + beta * p_old[q]; };
     double acc[2] = {0.0, 0.0};
+    // branch-free body (selects instead of branches) so the loads of a point
+    // are issued together
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
          q += (size_t)gridDim.x * blockDim.x) {
         int row, col;
         sp_rc(g, q, row, col);
-        size_t src = q;
-        if (g.cyclic) {
-            if (col < 2) src = q + nx;
-            else if (col >= g.nx + 2) src = q - nx;
-        }
+        const bool cyc = g.cyclic != 0;
+        const size_t src = q + ((cyc && col < 2) ? nx : 0) - ((cyc && col >= g.nx + 2) ? nx : 0);
         const double v = dir(src);
         p_new[q] = v;
-        double a = 0.0;
-        if (sp_inner(g, row, col)) {
-            size_t qw = q - 1, qe = q + 1;
-            if (g.cyclic) {
-                if (col == 2) qw = q - 1 + nx;
-                if (col == g.nx + 1) qe = q + 1 - nx;
-            }
-            const double m = (double)m8[q];
-            a = sp_apply(cE[q], cW[q], cN[q], cS[q], m, fs, kfs, dir(q - P), dir(qw), v, dir(qe),
-                         dir(q + P));
-            acc[0] = acc[0] + v * a * m;
-            acc[1] = fmax(fabs(v * m), acc[1]);
-        }
-        Ap[q] = a;
+        const bool inner = sp_inner(g, row, col);
+        const size_t qc = inner ? q : q_in;
+        const size_t qw = qc - 1 + ((cyc && col == 2) ? nx : 0);
+        const size_t qe = qc + 1 - ((cyc && col == g.nx + 1) ? nx : 0);
+        const double m = (double)m8[qc];
+        const double a = sp_apply(cE[qc], cW[qc], cN[qc], cS[qc], m, fs, kfs, dir(qc - P), dir(qw),
+                                  v, dir(qe), dir(qc + P));
+        const double mi = inner ? m : 0.0;
+        Ap[q] = inner ? a : 0.0;
+        acc[0] = acc[0] + v * (inner ? a : 0.0) * mi;
+        acc[1] = fmax(fabs(v * mi), acc[1]);
     }
     sp_block_partials<2, 1>(acc, part);
 }
@@ -611,7 +612,7 @@ int press_congrad(psi_press* c, const psi_opts* o, const double* forc, double* x
     hipLaunchKernelGGL(k_sp_init, dim3(SP_BLOCKS), dim3(256), 0, c->stream, g,
                        (const double*)c->cE, (const double*)c->cW, (const double*)c->cN,
                        (const double*)c->cS, (const uint8_t*)c->m8, c->fs, c->kfs, forc,
-                       (const double*)x, c->r, c->sp_part);
+                       (const double*)x, c->r, c->pv[0], c->sp_part);
     hipLaunchKernelGGL(k_sp_fold_init, dim3(1), dim3(1024), 0, c->stream,
                        (const double*)c->sp_part, SP_BLOCKS, c->S);
     rc = check_launch(c);
