@@ -213,3 +213,73 @@ def run_nonhydro(b, col, maskW, nh, eps=1e-12, max_iter=1000, dt_mom=3600.0, itt
     out["cf"] = cf.reshape(27, nz, ny + 4, nx + 4)
     out.update(iterations=itts, log=p.stdout.decode(errors="replace"))
     return out
+
+
+RESTART = os.path.join(HERE, "_ref", "ref_restart")
+RESTART_FIELDS = ("rho", "Nsqr", "Hd", "int_drhodT", "int_drhodS", "dHd", "temp", "dtemp",
+                  "salt", "dsalt", "u", "du", "v", "dv", "w")
+
+
+def restart_available() -> bool:
+    return os.path.exists(RESTART)
+
+
+def _restart_header(nx, ny, nz, itt, nisle, sf):
+    return np.array([nx, ny, nz, itt, nisle, int(sf)], np.int32).tobytes()
+
+
+def run_restart_write(nx, ny, nz, itt, fields3, psi, dpsi, dpsin, sf=True):
+    """The reference's write_restart on the given state (fields3[name] =
+    (tau, taum1); psi = (taum1, tau); dpsi = (taum1, tau, taup1); dpsin =
+    (tau, taum1)).  Returns the bytes of restart_PE_00000.dta."""
+    nisle = 0 if dpsin is None else int(np.shape(dpsin[0])[0])
+    f64 = lambda a: np.ascontiguousarray(a, np.float64).tobytes()
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        with open(fin, "wb") as f:
+            f.write(_restart_header(nx, ny, nz, itt, nisle, sf))
+            for name in RESTART_FIELDS:
+                f.write(f64(fields3[name][0]) + f64(fields3[name][1]))
+            for a in tuple(psi) + tuple(dpsi):
+                f.write(f64(a))
+            if nisle:
+                f.write(f64(dpsin[0]) + f64(dpsin[1]))
+        p = subprocess.run([RESTART, "write", fin], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=600, cwd=td)
+        out = os.path.join(td, "restart_PE_00000.dta")
+        if not os.path.exists(out):
+            raise RuntimeError("reference write_restart failed:\n" + p.stdout.decode()[-2000:])
+        return open(out, "rb").read()
+
+
+def run_restart_read(data, nx, ny, nz, nisle, sf=True):
+    """The reference's read_restart of the file bytes `data`; returns
+    (itt, fields3, psi, dpsi, dpsin) as run_restart_write takes them."""
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        open(fin, "wb").write(_restart_header(nx, ny, nz, 0, nisle, sf))
+        open(os.path.join(td, "restart_PE_00000.dta"), "wb").write(data)
+        p = subprocess.run([RESTART, "read", fin, fout], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=600, cwd=td)
+        if not os.path.exists(fout):
+            raise RuntimeError("reference read_restart failed:\n" + p.stdout.decode()[-2000:])
+        raw = open(fout, "rb").read()
+    n3, n2 = nz * (ny + 4) * (nx + 4), (ny + 4) * (nx + 4)
+    itt = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    off = 4
+    take = lambda n: np.frombuffer(raw, np.float64, n, off).copy()
+    fields = {}
+    for name in RESTART_FIELDS:
+        a = take(n3).reshape(nz, ny + 4, nx + 4)
+        off += 8 * n3
+        b = take(n3).reshape(nz, ny + 4, nx + 4)
+        off += 8 * n3
+        fields[name] = (a, b)
+    two = []
+    for _ in range(5):
+        two.append(take(n2).reshape(ny + 4, nx + 4))
+        off += 8 * n2
+    dpsin = None
+    if nisle:
+        dpsin = (take(nisle), np.frombuffer(raw, np.float64, nisle, off + 8 * nisle).copy())
+    return itt, fields, tuple(two[:2]), tuple(two[2:]), dpsin
