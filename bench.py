@@ -65,6 +65,8 @@ MG_TRAFFIC_2048 = {"post_level0": 168.9e6, "pre_level0": 130.3e6, "fcg_a": 201.8
 # ... of the surface-pressure CG kernels (profiles/k_sp_press_2048x40.md) and of the
 # velocity update at 2048^2 x 40 levels (profiles/levels_kernels_step_2048x40.md)
 PRESS_TRAFFIC_2048 = {"dir_apply": 283.2e6, "update": 206.4e6}
+# ... of the non-hydrostatic CG kernels at 512^2 x 32 (profiles/k_nh_nonhydro_512x32.md)
+NH_TRAFFIC_512x32 = {"dir_apply": 344.0e6, "update": 421.4e6}
 VELOCITY_TRAFFIC_2048x40 = 16.288e9
 
 
@@ -866,7 +868,8 @@ def nonhydro_main(args):
         "nonhydro_kernels": kern,
         "roofline": {"bound": "hbm", "kernel": "k_nh_dir" if dom == "dir_apply" else
                      "k_nh_update", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None},
+                     "unit": "GB/s", "frac": kern[dom]["frac"],
+                     "traffic": (NH_TRAFFIC_512x32[dom] if (n, nz) == (512, 32) else None)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(finite(line)), flush=True)
