@@ -323,9 +323,19 @@ int psi_press_apply_op(psi_press *ctx, const double *p, double *res);
  * iteration cap).  opts: mode PSI_MODE_PLAIN, criterion PSI_CRIT_REFERENCE,
  * congr_epsilon, max_iterations, check_every.  info->iterations is the
  * reference's congr_itts; relres_true is not computed (-1).  Returns PSI_OK,
- * PSI_NOT_CONVERGED, PSI_DIVERGED or PSI_NAN. */
+ * PSI_NOT_CONVERGED, PSI_DIVERGED or PSI_NAN.
+ * Also PSI_MODE_MG with PSI_CRIT_RELRES (no counterpart in the reference,
+ * free surface only): flexible CG in FP64 preconditioned by the aggregation
+ * multigrid V-cycle of PSI_MODE_MG on S = W A, W the T-cell area, to the FP64
+ * true relative residual relres_target (poly_degree, poly_lower and
+ * check_every as for psi_congrad); the hierarchy is built on first use or by
+ * psi_press_mg_setup.  A cyclic solution gets its halo set. */
 int psi_congrad_surf_press(psi_press *ctx, const psi_opts *opts, const double *forc, double *psi,
                            psi_info *info);
+/* Builds the multigrid hierarchy of the surface-pressure operator (no
+ * counterpart in the reference; free surface only, PSI_BAD_ARG with a rigid
+ * lid, whose operator is singular). */
+int psi_press_mg_setup(psi_press *ctx);
 /* The whole solve_pressure (solve_pressure.f90:9-84, serial): hydrostatic
  * pressure into p_hydro and its gradient into du/dv (tau), u/v at taup1 by
  * Adams-Bashforth, the depth-integrated transport's divergence as forcing

@@ -848,6 +848,8 @@ MgHostLev mg_level0(psi_ctx* c, const std::vector<double>& E, const std::vector<
     return L;
 }
 
+int mg_build(psi_ctx* c, std::vector<MgHostLev>& H);
+
 int mg_setup(psi_ctx* c) {
     mg_release(c);
     const Geom& g = c->g;
@@ -862,6 +864,16 @@ int mg_setup(psi_ctx* c) {
     c->mg = m;
     std::vector<MgHostLev> H;
     H.push_back(mg_level0(c, E, N, cC, act, m->w_mean));
+    return mg_build(c, H);
+}
+
+// The rest of the hierarchy from its level 0 (H[0], m->w_mean set): 2 x 2
+// coarsening down to at most MG_MAX_DENSE active cells, the coarsest level's
+// dense inverse, device copies of every level, the outer-iteration vectors.
+// Shared by the streamfunction and the surface-pressure contexts.
+int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
+    MgState* m = c->mg;
+    const Geom& g = c->g;
     m->nact0 = H[0].nact;
     if (m->nact0 == 0) {
         mg_release(c);

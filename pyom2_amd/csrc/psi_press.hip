@@ -499,6 +499,96 @@ __global__ void __launch_bounds__(256) k_press_velocity(
     }
 }
 
+// ---- multigrid mode (no counterpart in the reference) ------------------------
+// The aggregation multigrid of the streamfunction path on S = W A with W the
+// T-cell area cost dxt dyt (symmetric: W(i) cE(i) = W(i+1) cW(i+1)); FP64
+// flexible CG outside, the same V-cycle kernels inside.
+
+// Pass A: Az (kept for pass B) with the surface-pressure operator in FP64 and
+// the area-weighted products <z,Az>, <z,q>, <r,z> over active points.
+__global__ void __launch_bounds__(256) k_spmg_fcg_a(
+    Geom g, MgLev L0, const double* __restrict__ cE, const double* __restrict__ cW,
+    const double* __restrict__ cN, const double* __restrict__ cS,
+    const uint8_t* __restrict__ m8, int fs, double kfs, const double* __restrict__ dxt,
+    const double* __restrict__ cost, const double* __restrict__ dyt,
+    const uint8_t* __restrict__ act, const double* __restrict__ q,
+    const double* __restrict__ r, double* __restrict__ az_out,
+    const double* __restrict__ zscale, const MgScal* __restrict__ S,
+    double* __restrict__ part) {
+    if (S->done) return;
+    const double inv = 1.0 / *zscale;
+    double acc[3] = {0, 0, 0};
+    const int n = g.nx * g.ny;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int i = t % g.nx, j = t / g.nx;
+        const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+        if (!act[o]) continue;
+        const double z = mg_z(g, L0, i, j, inv);
+        const double az = sp_apply(cE[o], cW[o], cN[o], cS[o], (double)m8[o], fs, kfs,
+                                   mg_z(g, L0, i, j - 1, inv), mg_z(g, L0, i - 1, j, inv), z,
+                                   mg_z(g, L0, i + 1, j, inv), mg_z(g, L0, i, j + 1, inv));
+        az_out[o] = az;
+        const double w = mg_area(dxt, cost, dyt, i, j);
+        acc[0] += w * z * az;
+        acc[1] += w * z * q[o];
+        acc[2] += w * r[o] * z;
+    }
+    mg_block_partials<3>(acc, part);
+}
+
+// r = b - A x on active points (zero elsewhere) and the partial sums of |r|^2,
+// |b|^2 and the active count (slots 0, 1, 2).
+__global__ void __launch_bounds__(256) k_spmg_resid(
+    Geom g, const double* __restrict__ cE, const double* __restrict__ cW,
+    const double* __restrict__ cN, const double* __restrict__ cS,
+    const uint8_t* __restrict__ m8, int fs, double kfs, const uint8_t* __restrict__ act,
+    const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r_out,
+    double* __restrict__ part) {
+    double acc[3] = {0, 0, 0};
+    const int n = g.nx * g.ny;
+    const size_t P = g.pitch;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int i = t % g.nx, j = t / g.nx;
+        const size_t o = (size_t)(j + 2) * P + (i + 2);
+        double rv = 0.0;
+        if (act[o]) {
+            size_t ow = o - 1, oe = o + 1;
+            if (g.cyclic && i == 0) ow = o - 1 + g.nx;
+            if (g.cyclic && i == g.nx - 1) oe = o + 1 - g.nx;
+            rv = b[o] - sp_apply(cE[o], cW[o], cN[o], cS[o], (double)m8[o], fs, kfs, x[o - P],
+                                 x[ow], x[o], x[oe], x[o + P]);
+            acc[0] += rv * rv;
+            acc[1] += b[o] * b[o];
+            acc[2] += 1.0;
+        }
+        if (r_out) r_out[o] = rv;
+    }
+    mg_block_partials<3>(acc, part);
+}
+
+__global__ void __launch_bounds__(1024) k_spmg_resid_fold(const double* __restrict__ part,
+                                                          int nblocks, double* __restrict__ res) {
+    double v[3];
+    mg_fold<3>(part, nblocks, v);
+    if (threadIdx.x == 0) {
+        res[0] = v[0];
+        res[1] = v[1];
+        res[2] = 0.0;
+        res[3] = v[2];
+    }
+}
+
+// Active points of the multigrid mode: interior ocean cells (maskT(:,:,nz) = 1).
+__global__ void __launch_bounds__(256) k_spmg_act(Geom g, const uint8_t* __restrict__ m8,
+                                                  uint8_t* __restrict__ act) {
+    const size_t plane = (size_t)g.pitch * (g.ny + 4);
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= plane) return;
+    int row, col;
+    sp_rc(g, q, row, col);
+    act[q] = sp_inner(g, row, col) && m8[q];
+}
+
 }  // namespace psi
 
 // ---------------------------------------------------------------------------
@@ -590,12 +680,17 @@ bool valid_press(const psi_press* c) { return c && c->cE && c->S; }
 
 // congrad_surf_press on x (device, in place) with the reference's stopping
 // rule; forc on the interior.
+int sp_solve_mg(psi_press* c, const psi_opts* o, const double* forc, double* sol,
+                psi_info* info);
+
 int press_congrad(psi_press* c, const psi_opts* o, const double* forc, double* x,
                   psi_info* info) {
     using namespace psi;
+    if (o->mode == PSI_MODE_MG) return sp_solve_mg(c, o, forc, x, info);
     if (o->mode != PSI_MODE_PLAIN || o->criterion != PSI_CRIT_REFERENCE)
         return fail(c, PSI_BAD_ARG,
-                    "surface pressure: mode plain with the reference criterion only%s");
+                    "surface pressure: mode plain with the reference criterion, or mode mg "
+                    "with the relres criterion%s");
     const int batch = o->check_every > 0 ? (o->check_every + 1) / 2 : 16;
     hipGraphExec_t exec = nullptr;
     int rc = press_graph(c, x, &exec);
@@ -657,6 +752,238 @@ int press_congrad(psi_press* c, const psi_opts* o, const double* forc, double* x
     return info->status;
 }
 
+
+// ---- multigrid mode --------------------------------------------------------------
+int sp_mg_setup(psi_press* c) {
+    using namespace psi;
+    mg_release(c);
+    if (!c->fs)
+        return fail(c, PSI_BAD_ARG,
+                    "multigrid mode needs the free surface (the rigid-lid operator is "
+                    "singular)%s");
+    const Geom& g = c->g;
+    const size_t P = c->plane;
+    std::vector<double> E(P), W(P), N(P), S(P);
+    std::vector<uint8_t> m8(P);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(E.data(), c->cE, P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(W.data(), c->cW, P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(N.data(), c->cN, P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(S.data(), c->cS, P * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(m8.data(), c->m8, P, hipMemcpyDeviceToHost));
+    if (!c->act) {
+        int rc = dalloc(c, &c->act, P);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_spmg_act, dim3(nblk(P)), dim3(256), 0, c->stream, g,
+                           (const uint8_t*)c->m8, c->act);
+        rc = check_launch(c);
+        if (rc) return rc;
+    }
+    // level 0: S = W A over ocean cells, W = cost dxt dyt; couplings to land
+    // are zero already (the mask products of make_coeff_surf_press)
+    MgHostLev L;
+    L.nx = g.nx;
+    L.ny = g.ny;
+    L.cyc = g.cyclic;
+    const size_t n = (size_t)g.nx * g.ny;
+    L.E.assign(n, 0.0);
+    L.N.assign(n, 0.0);
+    L.C.assign(n, 1.0);
+    L.act.assign(n, 0);
+    auto o_of = [&](int i, int j) { return (size_t)(j + 2) * g.pitch + (size_t)(i + 2); };
+    double wsum = 0.0;
+    for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+            const size_t o = o_of(i, j), p = (size_t)i + (size_t)j * g.nx;
+            if (!m8[o]) continue;
+            const double w = c->h_dxt[i + 2] * c->h_cost[j + 2] * c->h_dyt[j + 2];
+            L.act[p] = 1;
+            ++L.nact;
+            wsum += w;
+            L.C[p] = w * ((((0.0 - E[o]) - W[o]) - N[o]) - S[o] - c->kfs);
+            const int ie = (i + 1 < g.nx) ? i + 1 : (g.cyclic ? 0 : -1);
+            if (ie >= 0 && m8[o_of(ie, j)]) L.E[p] = w * E[o];
+            if (j + 1 < g.ny && m8[o_of(i, j + 1)]) L.N[p] = w * N[o];
+        }
+    MgState* m = new MgState();
+    c->mg = m;
+    m->w_mean = L.nact ? wsum / L.nact : 1.0;
+    std::vector<MgHostLev> H;
+    H.push_back(std::move(L));
+    return mg_build(c, H);
+}
+
+int sp_residual(psi_press* c, const double* b, const double* x, double* r_out, double res[4]) {
+    using namespace psi;
+    MgState* m = c->mg;
+    hipLaunchKernelGGL(k_spmg_resid, dim3(m->nblk), dim3(256), 0, c->stream, c->g,
+                       (const double*)c->cE, (const double*)c->cW, (const double*)c->cN,
+                       (const double*)c->cS, (const uint8_t*)c->m8, c->fs, c->kfs,
+                       (const uint8_t*)c->act, b, x, r_out, m->part);
+    hipLaunchKernelGGL(k_spmg_resid_fold, dim3(1), dim3(1024), 0, c->stream,
+                       (const double*)m->part, m->nblk, c->sp_part);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    double h[4];
+    HIPCHK(c, hipMemcpyAsync(h, c->sp_part, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; ++k) res[k] = h[k];
+    return PSI_OK;
+}
+
+void sp_mg_iteration(psi_press* c, hipStream_t s, int K, const psi::MgCheb& ch) {
+    using namespace psi;
+    MgState* m = c->mg;
+    mg_vcycle(m, s, K, ch);
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_spmg_fcg_a, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+                       (const double*)c->cE, (const double*)c->cW, (const double*)c->cN,
+                       (const double*)c->cS, (const uint8_t*)c->m8, c->fs, c->kfs,
+                       (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
+                       (const uint8_t*)c->act, (const double*)m->q, (const double*)m->r, m->az,
+                       (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
+                       m->S);
+    hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+                       (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
+                       (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
+                       (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    hipLaunchKernelGGL(k_mg_fold_b, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
+                       m->nact0, m->S, m->zscale);
+}
+
+int sp_mg_graph(psi_press* c, int K, double ratio) {
+    using namespace psi;
+    MgState* m = c->mg;
+    if (m->exec && m->graph_K == K && m->graph_ratio == ratio) return PSI_OK;
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->exec = nullptr;
+    m->graph = nullptr;
+    const MgCheb ch = mg_cheb(K, ratio);
+    HIPCHK(c, hipStreamBeginCapture(m->gs, hipStreamCaptureModeThreadLocal));
+    sp_mg_iteration(c, m->gs, K, ch);
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamEndCapture(m->gs, &graph));
+    m->graph = graph;
+    HIPCHK(c, hipGraphInstantiate(&m->exec, graph, nullptr, nullptr, 0));
+    m->graph_K = K;
+    m->graph_ratio = ratio;
+    return PSI_OK;
+}
+
+// Multigrid-preconditioned flexible CG to the FP64 true relative residual
+// (the streamfunction path's PSI_MODE_MG on the surface-pressure operator).
+int sp_solve_mg(psi_press* c, const psi_opts* o, const double* forc, double* sol,
+                psi_info* info) {
+    using namespace psi;
+    if (o->criterion != PSI_CRIT_RELRES)
+        return fail(c, PSI_BAD_ARG, "multigrid mode needs the relres criterion%s");
+    if (!c->mg) {
+        int rc = sp_mg_setup(c);
+        if (rc) return rc;
+    }
+    MgState* m = c->mg;
+    const int K = o->poly_degree > 0 ? o->poly_degree : 4;
+    if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
+    const double lower = o->poly_lower > 0 ? o->poly_lower : 1.0 / 15.0;
+    if (!(lower < 1.0)) return fail(c, PSI_BAD_ARG, "multigrid interval: lower < 1%s");
+    const double ratio = 1.0 / lower;
+    const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
+    const int batch = o->check_every > 0 ? o->check_every : 4;
+    int rc = sp_mg_graph(c, K, ratio);
+    if (rc) return rc;
+    const Geom& g = c->g;
+    HIPCHK(c, hipEventRecord(c->t0, c->stream));
+    HIPCHK(c, hipMemcpyAsync(m->x, sol, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
+    double res[4];
+    double bb = 0.0;
+    int n = 0, restarts = 0, status = ST_RUNNING;
+    for (;;) {
+        rc = sp_residual(c, forc, m->x, m->r, res);
+        if (rc) return rc;
+        info->setup_launches += 2;
+        if (restarts == 0) bb = res[1] > 0 ? res[1] : res[0];
+        if (res[0] <= target * target * bb) {
+            status = ST_CONVERGED;
+            break;
+        }
+        if (res[0] != res[0]) {
+            status = ST_NAN;
+            break;
+        }
+        if (n >= o->max_iterations) {
+            status = ST_MAXIT;
+            break;
+        }
+        const double s0 = 1.0 / (m->w_mean * sqrt(res[0] / std::max(res[3], 1.0)));
+        MgScal S0{};
+        S0.s_r = s0;
+        S0.s_pending = s0;
+        S0.rr = res[0];
+        S0.bb = bb;
+        S0.rtarget2 = target * target;
+        S0.n = n;
+        S0.max_it = o->max_iterations;
+        S0.status = ST_RUNNING;
+        *m->h_S = S0;
+        HIPCHK(c, hipMemcpyAsync(m->S, m->h_S, sizeof(MgScal), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(m->zscale, &m->h_S->s_r, sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemsetAsync(m->p, 0, c->plane * 8, c->stream));
+        HIPCHK(c, hipMemsetAsync(m->q, 0, c->plane * 8, c->stream));
+        hipLaunchKernelGGL(k_mg_start, dim3(nblk((size_t)g.nx * g.ny)), dim3(256), 0, c->stream,
+                           g, m->lev[0], (const double*)c->dxt, (const double*)c->cost,
+                           (const double*)c->dyt, (const uint8_t*)c->act, (const double*)m->r,
+                           s0);
+        rc = check_launch(c);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(m->e_in, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
+        const int n_start = n;
+        HIPCHK(c, hipEventRecord(m->t0, m->gs));
+        for (;;) {
+            for (int k = 0; k < batch; ++k) HIPCHK(c, hipGraphLaunch(m->exec, m->gs));
+            HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
+            HIPCHK(c, hipStreamSynchronize(m->gs));
+            if (m->h_S->done) break;
+        }
+        HIPCHK(c, hipEventRecord(m->t1, m->gs));
+        HIPCHK(c, hipEventSynchronize(m->t1));
+        float span = 0.0f;
+        (void)hipEventElapsedTime(&span, m->t0, m->t1);
+        info->sweep_ms += span;
+        info->sweep_launches += m->h_S->n - n_start;
+        HIPCHK(c, hipEventRecord(m->e_out, m->gs));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, m->e_out, 0));
+        n = m->h_S->n;
+        status = m->h_S->status;
+        if (status != ST_CONVERGED) break;
+        ++restarts;   // confirm on the FP64 true residual (top of the loop)
+    }
+    if (status != ST_CONVERGED || restarts == 0) {
+        rc = sp_residual(c, forc, m->x, nullptr, res);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(sol, m->x, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (g.cyclic)
+        hipLaunchKernelGGL(k_setcyclic, dim3(nblk(g.ny + 4)), dim3(256), 0, c->stream, g, sol);
+    HIPCHK(c, hipEventRecord(c->t1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->t1));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->t0, c->t1);
+    info->elapsed_ms = ms;
+    info->iterations = status == ST_MAXIT ? o->max_iterations + 1 : n;
+    info->status = status_of(status);
+    info->converged = status == ST_CONVERGED;
+    info->outer_iterations = restarts;
+    info->relres_true = bb > 0 ? sqrt(res[0] / bb) : sqrt(res[0]);
+    info->relres_recursive = bb > 0 ? sqrt(m->h_S->rr / bb) : 0.0;
+    if (info->status == PSI_NOT_CONVERGED)
+        fail(c, PSI_NOT_CONVERGED, "surface pressure (multigrid): iteration cap reached%s");
+    return info->status;
+}
 }  // namespace
 
 extern "C" {
@@ -793,7 +1120,14 @@ int psi_press_set_stream(psi_press* c, void* stream) {
     return PSI_OK;
 }
 
-int64_t psi_press_device_bytes(const psi_press* c) { return c ? c->bytes : 0; }
+int64_t psi_press_device_bytes(const psi_press* c) {
+    return c ? c->bytes + (c->mg ? c->mg->bytes : 0) : 0;
+}
+
+int psi_press_mg_setup(psi_press* c) {
+    if (!valid_press(c)) return PSI_BAD_ARG;
+    return sp_mg_setup(c);
+}
 
 int psi_make_coeff_surf_press(psi_press* c, double* cf) {
     using namespace psi;

@@ -111,11 +111,20 @@ class SurfacePressureSolver:
                                               _dev(res, self.shape)))
         return res
 
+    def mg_setup(self):
+        """Build the multigrid hierarchy of mode "mg" (free surface only; done
+        on first use otherwise)."""
+        self._sync_in()
+        self._check(self.L.psi_press_mg_setup(self.ctx))
+
     # -- the solve ------------------------------------------------------------
     def congrad_surf_press(self, forc: torch.Tensor, psi: torch.Tensor,
                            opts: SolveOptions | None = None, **kw):
         """congrad_surf_press (solve_pressure.f90:157-249) on psi in place
-        (first guess in, solution out).  Returns (iterations, converged, info)."""
+        (first guess in, solution out).  mode "plain" with criterion
+        "reference" runs the reference's algorithm; mode "mg" with criterion
+        "relres" the multigrid-preconditioned solve to relres_target (free
+        surface only).  Returns (iterations, converged, info)."""
         opts = opts or SolveOptions(**kw)
         info = PsiInfo()
         self._sync_in()

@@ -170,9 +170,11 @@ def test_congrad_against_oracle_and_exits(nx):
         assert its == cap + 1 == r.iterations and not conv
         assert info["status"] == _lib.PSI_NOT_CONVERGED
         assert np.abs(N(x2) - r.sol).max() <= 1e-12 * np.abs(r.sol).max()
-    # options the reference algorithm does not take
+    # options neither the reference algorithm nor the multigrid mode takes
     with pytest.raises(PsiError):
-        s.congrad_surf_press(T(forc), x2, mode="mg", criterion="relres")
+        s.congrad_surf_press(T(forc), x2, mode="poly", criterion="relres")
+    with pytest.raises(PsiError):
+        s.congrad_surf_press(T(forc), x2, mode="mg", criterion="reference")
     s.close()
 
 
@@ -221,3 +223,83 @@ def test_continuity_at_512(fs):
     assert np.abs(lhs - res)[act].max() <= 1e-10 * scale
     assert np.abs(res)[act].max() <= 1e-4 * scale
     s.close()
+
+
+# ---- multigrid mode (no counterpart in the reference) ---------------------------
+def _relres_oracle(b, col, fs, dt, forc, x):
+    """FP64 ||forc - A x|| / ||forc|| over ocean cells, with the oracle's
+    coefficients and apply_op (x's cyclic halo set first)."""
+    cf = O.make_coeff_surf_press(b, b.hu, b.hv, col.maskT[-1], fs, GRAV, dt)
+    xs = O.setcyclic(b.nx, b.ny, b.cyclic_x, np.ascontiguousarray(x).copy())
+    r = forc - O.apply_op(b.nx, b.ny, cf, xs)
+    act = np.zeros(b.shape, bool)
+    act[2:-2, 2:-2] = col.maskT[-1][2:-2, 2:-2] != 0
+    return float(np.sqrt((r[act] ** 2).sum() / (forc[act] ** 2).sum()))
+
+
+@pytest.mark.parametrize("n,cyclic,islands", [(24, False, 1), (64, True, 0), (512, False, 3)])
+def test_mg_mode_reaches_the_residual_target(n, cyclic, islands):
+    """Multigrid FCG to relres 1e-10: the oracle's FP64 residual agrees, the
+    iteration count stays small, and the solution matches the reference
+    algorithm's (run to congr_epsilon 1e-12) within 1e-6 of max|psi|."""
+    print(f"\n[press mg n={n} cyclic={cyclic}]", flush=True)
+    b, col, mom = _case(13, n, n, 4, cyclic, islands=islands)
+    dt = 1800.0
+    s = _solver(b, col, 1, dt)
+    p, du, dv, _, _ = O.streamfunction_forcing(b, col, mom.rho[1], mom.du[1], mom.dv[1],
+                                               mom.du_mix, mom.dv_mix, GRAV / RHO_0)
+    _, _, forc = O.pressure_forcing(b, col, dt, AB, mom.u[1], mom.v[1], du, dv, mom.du[0],
+                                    mom.dv[0], mom.du_mix, mom.dv_mix, mom.psi[1], 1, GRAV)
+    x0 = 2 * mom.psi[1] - mom.psi[0]
+    x = T(x0)
+    its, conv, info = s.congrad_surf_press(T(forc), x, mode="mg", criterion="relres",
+                                           relres_target=1e-10, max_iterations=500)
+    print(f"  mg its={its} relres={info['relres_true']:.2e} {info['elapsed_ms']:.2f} ms",
+          flush=True)
+    assert conv and info["status"] == _lib.PSI_OK
+    assert its <= 40
+    rel = _relres_oracle(b, col, 1, dt, forc, N(x))
+    assert rel <= 1.01e-10
+    assert abs(info["relres_true"] - rel) <= 1e-3 * 1e-10 + 1e-6 * rel
+    xp = T(x0)
+    its_p, conv_p, _ = s.congrad_surf_press(T(forc), xp, congr_epsilon=1e-12,
+                                            max_iterations=100000)
+    assert conv_p and its_p > its
+    assert np.abs(N(x) - N(xp)).max() <= 1e-6 * np.abs(N(xp)).max()
+    s.close()
+
+
+def test_mg_mode_whole_routine_and_rigid_lid():
+    """solve_pressure with the multigrid solve: the corrected transport obeys
+    the continuity equation to the residual; a rigid lid is refused."""
+    print("\n[press mg whole routine]", flush=True)
+    b, col, mom = _case(14, 96, 80, 5, False, islands=2)
+    dt = 1200.0
+    s = _solver(b, col, 1, dt)
+    lev = Levels.from_column(col, DEV)
+    m = MomentumState.from_numpy(mom, DEV)
+    psi = T(mom.psi)
+    forc = s.zeros()
+    its, conv, info = s.solve_pressure(psi, lev, m, 0, 1, 2, dt_mom=dt, forc_out=forc,
+                                       mode="mg", criterion="relres", relres_target=1e-10,
+                                       max_iterations=500)
+    assert conv
+    p_new = N(psi[2])
+    cf = O.make_coeff_surf_press(b, b.hu, b.hv, col.maskT[-1], 1, GRAV, dt)
+    f = N(forc)
+    res = (f - O.apply_op(b.nx, b.ny, cf, p_new))[2:-2, 2:-2]
+    z = np.zeros(col.shape)
+    _, _, div_new = O.pressure_forcing(b, col, dt, 0.0, N(m.u[2]), N(m.v[2]), z, z, z, z, z, z,
+                                       p_new, 0, GRAV)
+    lhs = (div_new - (mom.psi[1] - p_new) / (GRAV * dt * dt) * col.maskT[-1])[2:-2, 2:-2]
+    act = col.maskT[-1][2:-2, 2:-2] != 0
+    scale = np.abs(f).max()
+    assert np.abs(lhs - res)[act].max() <= 1e-10 * scale
+    assert np.sqrt((res[act] ** 2).sum() / (f[2:-2, 2:-2][act] ** 2).sum()) <= 1.01e-10
+    s.close()
+    s0 = _solver(b, col, 0, dt)
+    with pytest.raises(PsiError):
+        s0.mg_setup()
+    with pytest.raises(PsiError):
+        s0.congrad_surf_press(forc, s0.zeros(), mode="mg", criterion="relres")
+    s0.close()
