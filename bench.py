@@ -100,6 +100,9 @@ def parse():
                     help="solve path of the step workload")
     ap.add_argument("--nh-n", type=int, default=512, help="nonhydro workload: nx = ny")
     ap.add_argument("--nh-nz", type=int, default=32, help="nonhydro workload: levels")
+    ap.add_argument("--press-modes", default="mg,plain",
+                    help="press workload: solve paths, the first the headline (mg: multigrid "
+                         "to --relres; plain: the reference's CG and stopping rule)")
     ap.add_argument("--free-surface", type=int, default=1,
                     help="press workload: enable_free_surface (0: rigid lid)")
     ap.add_argument("--workload", default="solve",
@@ -650,6 +653,17 @@ def step_main(args):
 # and res.
 PRESS_BYTES = {"dir_apply": 8 + 8 + 32 + 1 + 16, "update": 32 + 1 + 16}
 PRESS_LEVEL_BYTES = {"forcing": 24 + 120, "velocity": 48}
+# ... of the multigrid mode's kernels per interior point: the level kernels as
+# in MG_KERNEL_BYTES; outer pass A reads z, the 4 coefficient planes, the two
+# masks, q and r and writes Az; pass B as MG_KERNEL_BYTES
+PRESS_MG_BYTES = {"post_level0": 26, "pre_level0": 21, "fcg_a": 4 + 32 + 1 + 1 + 16 + 8,
+                  "fcg_b": 81}
+PRESS_MG_NAMES = {"post_level0": "k_mg_post<4> (finest level)",
+                  "pre_level0": "k_mg_pre<4> (finest level)", "fcg_a": "k_spmg_fcg_a",
+                  "fcg_b": "k_mg_fcg_b"}
+# ... HBM bytes per launch at 2048^2 (profiles/k_spmg_press_2048x40.md)
+PRESS_MG_TRAFFIC_2048 = {"post_level0": 169.4e6, "pre_level0": 130.4e6, "fcg_a": 279.7e6,
+                         "fcg_b": 357.9e6}
 
 
 def press_main(args):
@@ -688,7 +702,6 @@ def press_main(args):
     host_op = (hu.cpu().numpy(), hv.cpu().numpy(), maskT[-1].cpu().numpy())
     lev = Levels(dzt, dzw, maskT, maskU, maskV, dev)
     del maskT, maskU, maskV, hu, hv
-    opts = SolveOptions(congr_epsilon=1e-12, max_iterations=200000)
     g = torch.Generator(device=dev)
     g.manual_seed(0)
     sh3 = (nz,) + b0.shape
@@ -708,66 +721,111 @@ def press_main(args):
     log(f"[press] basin {b0.nx}x{b0.ny} x {nz} levels, free surface {int(fs)}, "
         f"set up in {time.time() - t:.1f}s")
     c = float(np.sqrt(1 - 0.99 ** 2))
-    taum1, tau, taup1 = 0, 1, 2
-    per_step, its, solve_ms = [], [], []
-    for k in range(args.warmup + args.steps):
-        ms.du[tau] = 0.99 * ms.du[taum1] + rnd(c * 1e-6, lev.maskU)
-        ms.dv[tau] = 0.99 * ms.dv[taum1] + rnd(c * 1e-6, lev.maskV)
-        torch.cuda.synchronize()
-        ts = time.perf_counter()
-        n_it, conv, info = s.solve_pressure(psi, lev, ms, taum1, tau, taup1, opts)
-        torch.cuda.synchronize()
-        te = time.perf_counter()
-        if not conv:
-            raise SystemExit(f"step {k}: surface pressure did not converge ({info})")
-        if k >= args.warmup:
-            per_step.append(te - ts)
-            its.append(n_it)
-            solve_ms.append(info["elapsed_ms"])
-        log(f"[press] {k}: {(te - ts) * 1e3:.2f} ms, solve {info['elapsed_ms']:.2f} ms, "
-            f"{n_it} iterations, est. error {info['est_error']:.2e}")
-        taum1, tau, taup1 = tau, taup1, taum1
-    kt = s.kernel_times(reps=20)
+    du0, dv0 = ms.du.clone(), ms.dv.clone()
+    modes = [m for m in args.press_modes.split(",") if m]
+
+    def run(mode):
+        """warmup + steps of the whole routine with the given solve path, from
+        the same initial state and the same drift for every path."""
+        g.manual_seed(1)
+        ms.du.copy_(du0)
+        ms.dv.copy_(dv0)
+        psi.zero_()
+        opts = (SolveOptions(mode="mg", criterion="relres", relres_target=args.relres,
+                             max_iterations=10000) if mode == "mg" else
+                SolveOptions(congr_epsilon=1e-12, max_iterations=200000))
+        taum1, tau, taup1 = 0, 1, 2
+        per_step, its, solve_ms = [], [], []
+        for k in range(args.warmup + args.steps):
+            ms.du[tau] = 0.99 * ms.du[taum1] + rnd(c * 1e-6, lev.maskU)
+            ms.dv[tau] = 0.99 * ms.dv[taum1] + rnd(c * 1e-6, lev.maskV)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            n_it, conv, info = s.solve_pressure(psi, lev, ms, taum1, tau, taup1, opts)
+            torch.cuda.synchronize()
+            te = time.perf_counter()
+            if not conv:
+                raise SystemExit(f"{mode} step {k}: surface pressure did not converge ({info})")
+            if k >= args.warmup:
+                per_step.append(te - ts)
+                its.append(n_it)
+                solve_ms.append(info["elapsed_ms"])
+            log(f"[press {mode}] {k}: {(te - ts) * 1e3:.2f} ms, solve {info['elapsed_ms']:.2f} "
+                f"ms, {n_it} iterations, est. error {info['est_error']:.2e}, relres "
+                f"{info['relres_true']:.2e}")
+            taum1, tau, taup1 = tau, taup1, taum1
+        return {"ms_per_step": round(1e3 * sum(per_step) / args.steps, 3),
+                "solve_ms_mean": round(float(np.mean(solve_ms)), 3),
+                "iterations_mean": round(float(np.mean(its)), 1),
+                "us_per_iteration": round(1e3 * float(np.mean(solve_ms)) /
+                                          max(float(np.mean(its)), 1.0), 2)}
+
+    paths = {m: run(m) for m in modes}
+    head = modes[0]
     pts = b0.shape[0] * b0.shape[1]
-    kern = {}
-    for name in ("dir_apply", "update"):
-        gbs = PRESS_BYTES[name] * pts / (kt[name] * 1e-6) / 1e9
-        kern[name] = {"us": round(kt[name], 2), "achieved": round(gbs, 1),
-                      "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_pt": PRESS_BYTES[name]}
-    kern["fold_a_us"] = round(kt["fold_a"], 2)
-    kern["fold_b_us"] = round(kt["fold_b"], 2)
-    dom = max(("dir_apply", "update"), key=lambda n: kt[n])
-    total = sum(per_step)
-    it_mean = float(np.mean(its))
-    cpu = None if args.skip_cpu else press_cpu_baseline(b0, host_op, nz, it_mean, fs,
-                                                        budget_s=args.cpu_seconds)
+    if head == "mg":
+        kt = s.mg_kernel_times(reps=20)
+        pts_i = b0.nx * b0.ny
+        kern = {}
+        for name, us in kt.items():
+            gbs = PRESS_MG_BYTES[name] * pts_i / (us * 1e-6) / 1e9
+            kern[name] = {"us": round(us, 2), "achieved": round(gbs, 1),
+                          "frac": round(gbs / HBM_PEAK_GBS, 4),
+                          "bytes_per_pt": PRESS_MG_BYTES[name]}
+        dom = max(kt, key=lambda x: kt[x])
+        roof = {"bound": "hbm", "kernel": PRESS_MG_NAMES[dom], "path": "mg",
+                "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": kern[dom]["frac"],
+                "traffic": (PRESS_MG_TRAFFIC_2048.get(dom) if b0.nx == b0.ny == 2048 else None)}
+    else:
+        kt = s.kernel_times(reps=20)
+        kern = {}
+        for name in ("dir_apply", "update"):
+            gbs = PRESS_BYTES[name] * pts / (kt[name] * 1e-6) / 1e9
+            kern[name] = {"us": round(kt[name], 2), "achieved": round(gbs, 1),
+                          "frac": round(gbs / HBM_PEAK_GBS, 4),
+                          "bytes_per_pt": PRESS_BYTES[name]}
+        kern["fold_a_us"] = round(kt["fold_a"], 2)
+        kern["fold_b_us"] = round(kt["fold_b"], 2)
+        dom = max(("dir_apply", "update"), key=lambda n: kt[n])
+        roof = {"bound": "hbm", "kernel": "k_sp_dir" if dom == "dir_apply" else "k_sp_update2",
+                "path": "plain", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": kern[dom]["frac"],
+                "traffic": (PRESS_TRAFFIC_2048[dom] if b0.nx == b0.ny == 2048 else None)}
+    plain_its = paths.get("plain", {}).get("iterations_mean")
+    cpu = None if (args.skip_cpu or plain_its is None) else press_cpu_baseline(
+        b0, host_op, nz, plain_its, fs, budget_s=args.cpu_seconds)
+    r = paths[head]
     line = {
         "metric": f"ms per {args.n}^2 x {nz}-level solve_pressure step (forcing, surface-"
-                  f"pressure CG to congr_epsilon 1e-12, velocity correction)",
-        "value": round(1e3 * total / args.steps, 3),
+                  f"pressure solve, velocity correction)",
+        "value": r["ms_per_step"],
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "plain" else "f64 outer / f32 V-cycle",
         "data": "synthetic",
         "config": {"workload": f"spherical basin {b0.nx}x{b0.ny}, {nz} levels, 2 islands, "
                                f"{'free surface' if fs else 'rigid lid'}, dt 3600 s, drifting "
-                               f"tendencies, first guess 2 psi(tau) - psi(taum1)"},
-        "solve_ms_mean": round(float(np.mean(solve_ms)), 3),
-        "iterations_mean": round(it_mean, 1),
-        "us_per_iteration": round(1e3 * float(np.mean(solve_ms)) / max(it_mean, 1.0), 2),
+                               f"tendencies, first guess 2 psi(tau) - psi(taum1)",
+                   "path": head,
+                   "stopping": ("FP64 true relative residual <= %g" % args.relres
+                                if head == "mg" else
+                                "the reference's rule: congr_epsilon 1e-12 on the extrapolated "
+                                "step")},
+        "paths": paths,
         "press_kernels": kern,
-        "roofline": {"bound": "hbm", "kernel": "k_sp_dir" if dom == "dir_apply" else
-                     "k_sp_update", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": kern[dom]["frac"],
-                     "traffic": (PRESS_TRAFFIC_2048[dom] if b0.nx == b0.ny == 2048 else None)},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if "plain" in paths and head != "plain":
+        line["speedup_vs_reference_algorithm"] = round(
+            paths["plain"]["ms_per_step"] / r["ms_per_step"], 2)
     print(json.dumps(finite(line)), flush=True)
 
 

@@ -336,6 +336,11 @@ int psi_congrad_surf_press(psi_press *ctx, const psi_opts *opts, const double *f
  * counterpart in the reference; free surface only, PSI_BAD_ARG with a rigid
  * lid, whose operator is singular). */
 int psi_press_mg_setup(psi_press *ctx);
+/* Performance report of the multigrid mode, as psi_mg_kernel_times (us[0]
+ * post-smoothing and us[1] pre-smoothing of the finest level, us[2] and us[3]
+ * the outer passes; n >= 4); after psi_press_mg_setup, between solves. */
+int psi_press_mg_kernel_times(psi_press *ctx, int32_t degree, int32_t reps, double *us,
+                              int32_t n);
 /* The whole solve_pressure (solve_pressure.f90:9-84, serial): hydrostatic
  * pressure into p_hydro and its gradient into du/dv (tau), u/v at taup1 by
  * Adams-Bashforth, the depth-integrated transport's divergence as forcing

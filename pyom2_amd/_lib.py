@@ -176,6 +176,7 @@ SIGNATURES = [
                                      C.c_void_p, C.POINTER(PsiInfo)]),
     ("psi_press_kernel_times", C.c_int, [C.c_void_p, C.c_int32, _dp, C.c_int32]),
     ("psi_press_mg_setup", C.c_int, [C.c_void_p]),
+    ("psi_press_mg_kernel_times", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32]),
     # the non-hydrostatic pressure solver
     ("psi_nh_create", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(PsiGrid), C.POINTER(PsiLevels),
                                 C.c_void_p, C.c_void_p]),

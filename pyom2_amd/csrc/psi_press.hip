@@ -1129,6 +1129,50 @@ int psi_press_mg_setup(psi_press* c) {
     return sp_mg_setup(c);
 }
 
+// The multigrid mode's heaviest kernels, as psi_mg_kernel_times: us[0]
+// prolongation + post-smoothing and us[1] pre-smoothing + restriction on the
+// finest level, us[2] and us[3] the outer passes.
+int psi_press_mg_kernel_times(psi_press* c, int32_t degree, int32_t reps, double* us,
+                              int32_t n) {
+    using namespace psi;
+    if (!valid_press(c) || !us || n < 4 || reps < 1) return PSI_BAD_ARG;
+    MgState* m = c->mg;
+    if (!m) return fail(c, PSI_BAD_ARG, "call psi_press_mg_setup first%s");
+    if (m->L < 1) return fail(c, PSI_BAD_ARG, "multigrid: a single level has no V-cycle kernels%s");
+    const int K = degree > 0 ? degree : 4;
+    if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
+    MgScal z{};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(m->S_time, &z, sizeof z, hipMemcpyHostToDevice));
+    switch (K) {
+        case 1: mg_time_levels<1>(m, reps, us); break;
+        case 2: mg_time_levels<2>(m, reps, us); break;
+        case 3: mg_time_levels<3>(m, reps, us); break;
+        case 4: mg_time_levels<4>(m, reps, us); break;
+        case 5: mg_time_levels<5>(m, reps, us); break;
+        default: mg_time_levels<6>(m, reps, us); break;
+    }
+    const Geom& g = c->g;
+    us[2] = mg_time(m, reps, [&] {
+        hipLaunchKernelGGL(k_spmg_fcg_a, dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
+                           (const double*)c->cE, (const double*)c->cW, (const double*)c->cN,
+                           (const double*)c->cS, (const uint8_t*)c->m8, c->fs, c->kfs,
+                           (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
+                           (const uint8_t*)c->act, (const double*)m->q, (const double*)m->r,
+                           m->az, (const double*)m->zscale, (const MgScal*)m->S_time, m->part);
+    });
+    us[3] = mg_time(m, reps, [&] {
+        hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
+                           (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
+                           (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
+                           (const double*)m->zscale, (const MgScal*)m->S_time, m->part);
+    });
+    HIPCHK(c, hipGetLastError());
+    for (int k = 0; k < 4; ++k)
+        if (us[k] < 0) return fail(c, PSI_HIP_ERROR, "multigrid kernel timing failed%s");
+    return PSI_OK;
+}
+
 int psi_make_coeff_surf_press(psi_press* c, double* cf) {
     using namespace psi;
     if (!valid_press(c) || !cf) return PSI_BAD_ARG;

@@ -162,6 +162,14 @@ class SurfacePressureSolver:
         d = info.as_dict()
         return d["iterations"], bool(d["converged"]), d
 
+    def mg_kernel_times(self, degree=0, reps=20):
+        """Microseconds per launch of the multigrid mode's heaviest kernels:
+        {post_level0, pre_level0, fcg_a, fcg_b} (HIP events, between solves)."""
+        us = np.zeros(4)
+        self._sync_in()
+        self._check(self.L.psi_press_mg_kernel_times(self.ctx, degree, reps, _f64p(us), 4))
+        return dict(zip(("post_level0", "pre_level0", "fcg_a", "fcg_b"), us.tolist()))
+
     def kernel_times(self, reps=20):
         """Microseconds per launch of the iteration's kernels:
         {dir_apply, update, fold_a, fold_b} (HIP events, between solves)."""
