@@ -66,7 +66,7 @@ MG_TRAFFIC_2048 = {"post_level0": 168.9e6, "pre_level0": 130.3e6, "fcg_a": 201.8
 # velocity update at 2048^2 x 40 levels (profiles/levels_kernels_step_2048x40.md)
 PRESS_TRAFFIC_2048 = {"dir_apply": 283.2e6, "update": 206.4e6}
 # ... of the non-hydrostatic CG kernels at 512^2 x 32 (profiles/k_nh_nonhydro_512x32.md)
-NH_TRAFFIC_512x32 = {"dir_apply": 344.0e6, "update": 421.4e6}
+NH_TRAFFIC_512x32 = {"dir": 216.9e6, "apply": 156.4e6, "update": 421.4e6}
 VELOCITY_TRAFFIC_2048x40 = 16.288e9
 
 
@@ -829,11 +829,12 @@ def press_main(args):
     print(json.dumps(finite(line)), flush=True)
 
 
-# algorithmic bytes per point of the non-hydrostatic CG kernels: k_nh_dir reads
-# res, the old direction and the packed mask byte, writes the new direction and
-# A p (the operator is rebuilt from the mask and 1-D factors); k_nh_update as
-# k_sp_update.
-NH_BYTES = {"dir_apply": 8 + 8 + 1 + 16, "update": 32 + 1 + 16}
+# algorithmic bytes per point of the non-hydrostatic CG kernels: k_nh_p reads
+# res, the old direction and the packed mask byte and writes the new
+# direction; k_nh_ap reads it and the mask and writes A p (the operator is
+# rebuilt from the mask and 1-D factors); k_nh_update as k_sp_update.
+NH_BYTES = {"dir": 8 + 8 + 1 + 8, "apply": 8 + 1 + 8, "update": 32 + 1 + 16}
+NH_NAMES = {"dir": "k_nh_p", "apply": "k_nh_ap", "update": "k_nh_update"}
 
 
 def nonhydro_main(args):
@@ -893,13 +894,14 @@ def nonhydro_main(args):
     kt = s.kernel_times(reps=20)
     pts = nz * b.shape[0] * b.shape[1]
     kern = {}
-    for name in ("dir_apply", "update"):
+    for name in ("dir", "apply", "update"):
         gbs = NH_BYTES[name] * pts / (kt[name] * 1e-6) / 1e9
-        kern[name] = {"us": round(kt[name], 2), "achieved": round(gbs, 1),
-                      "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_pt": NH_BYTES[name]}
+        kern[name] = {"kernel": NH_NAMES[name], "us": round(kt[name], 2),
+                      "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "bytes_per_pt": NH_BYTES[name]}
     kern["fold_a_us"] = round(kt["fold_a"], 2)
     kern["fold_b_us"] = round(kt["fold_b"], 2)
-    dom = max(("dir_apply", "update"), key=lambda x: kt[x])
+    dom = max(("dir", "apply", "update"), key=lambda x: kt[x])
     total = sum(per_step)
     it_mean = float(np.mean(its))
     cpu = None if args.skip_cpu else nonhydro_cpu_baseline(b, col, mW, nh, it_mean,
@@ -924,10 +926,9 @@ def nonhydro_main(args):
         "iterations_mean": round(it_mean, 1),
         "us_per_iteration": round(1e3 * float(np.mean(solve_ms)) / max(it_mean, 1.0), 2),
         "nonhydro_kernels": kern,
-        "roofline": {"bound": "hbm", "kernel": "k_nh_dir" if dom == "dir_apply" else
-                     "k_nh_update", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": kern[dom]["frac"],
-                     "traffic": (NH_TRAFFIC_512x32[dom] if (n, nz) == (512, 32) else None)},
+        "roofline": {"bound": "hbm", "kernel": NH_NAMES[dom], "achieved": kern[dom]["achieved"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern[dom]["frac"],
+                     "traffic": (NH_TRAFFIC_512x32.get(dom) if (n, nz) == (512, 32) else None)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(finite(line)), flush=True)

@@ -409,8 +409,9 @@ int psi_solve_non_hydrostatic(psi_nh *ctx, const psi_opts *opts, double *u, doub
                               int32_t taum1, int32_t tau, int32_t taup1, double dt_mom,
                               double AB_eps, int32_t itt, double *forc_out, psi_info *info);
 /* Performance report: average duration in microseconds of `reps` launches of
- * the iteration's kernels (us[0] direction + apply, us[1] update, us[2] and
- * us[3] the two folds; n >= 4), HIP events; call between solves. */
+ * the iteration's kernels (us[0] the direction pair, us[1] update, us[2] and
+ * us[3] the two folds, and with n >= 6 us[4], us[5] the two direction
+ * kernels alone), HIP events; call between solves. */
 int psi_nh_kernel_times(psi_nh *ctx, int32_t reps, double *us, int32_t n);
 
 /* ---- flexible CG with the learned preconditioner ---------------------------

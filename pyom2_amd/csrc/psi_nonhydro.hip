@@ -177,74 +177,90 @@ __global__ void __launch_bounds__(256) k_nh_init(Geom g, int nz, NhTables t,
 
 // The direction of iteration n, p = res + (rsnew/rsold) p (:190; p = res for
 // n = 1, :179) with setcyclic_xyz (:191-192), Ap = A p (:185; zero off the
-// interior), and the partials of dot_3D(p, Ap) (:186) and absmax_3D(p)
-// (:215).  Workgroup row blockIdx.y is level k; within a level, consecutive
-// chunks of the plane go to the workgroups of one XCD (gridDim.x is a
-// multiple of 8, so the chunk's levels above and below run on that XCD too),
-// and its L2 serves the neighbours.  Partials: slot 0 the sums, slot 1 the
-// maxima, one per workgroup.
-__global__ void __launch_bounds__(256) k_nh_dir(
-    Geom g, int nz, NhTables t, const uint8_t* __restrict__ m8, const double* __restrict__ r,
-    const double* __restrict__ p_old, double* __restrict__ p_new, double* __restrict__ Ap,
-    const SpScal* __restrict__ S, double* __restrict__ part) {
+// interior) and the partials of dot_3D(p, Ap) (:186) and absmax_3D(p)
+// (:215), in two passes: rebuilding the seven neighbours' new directions in
+// the stencil (one pass, 33 B/pt) made it bound by instruction issue, and the
+// split (41 B/pt) measured 18% faster.
+// k_nh_p: p = res + beta p_old over the whole array with setcyclic_xyz, and
+// the partial maxima of absmax_3D(p) (slot 1).
+__global__ void __launch_bounds__(256) k_nh_p(Geom g, const uint8_t* __restrict__ m8,
+                                              const double* __restrict__ r,
+                                              const double* __restrict__ p_old,
+                                              double* __restrict__ p_new,
+                                              const SpScal* __restrict__ S,
+                                              double* __restrict__ part) {
 #pragma clang fp contract(off)
     if (S->done) return;
-    // iteration 1: beta = 0 and the old direction was zeroed by k_nh_init
-    const double beta = S->beta;
+    const double beta = S->beta;   // 0 in iteration 1, with the old direction zeroed
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
-    const size_t P = g.pitch, nx = (size_t)g.nx;
-    const int k = blockIdx.y;
-    const size_t base = (size_t)k * plane;
-    // vertical neighbour offsets (zero where the reference's clamped index
-    // meets a zero coefficient) and an interior stand-in for halo points
-    const size_t dn = k > 0 ? plane : 0, up = k < nz - 1 ? plane : 0;
-    const size_t q_in = 2 * P + 2;
+    const size_t nx = (size_t)g.nx;
+    const size_t base = (size_t)blockIdx.y * plane;
     const bool cyc = g.cyclic != 0;
-    auto dir = [&](size_t o) -> double { return r[o] + beta * p_old[o]; };
-    double s_dot = 0.0, s_max = 0.0;
-    const size_t chunk = (size_t)xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    for (size_t q = chunk * blockDim.x + threadIdx.x; q < plane;
+    double s_max = 0.0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
          q += (size_t)gridDim.x * blockDim.x) {
         int row, col;
         sp_rc(g, q, row, col);
         const size_t o = base + q;
         const size_t src = o + ((cyc && col < 2) ? nx : 0) - ((cyc && col >= g.nx + 2) ? nx : 0);
-        const double v = dir(src);
+        const double v = r[src] + beta * p_old[src];
         p_new[o] = v;
-        const bool inner = sp_inner(g, row, col);
-        const size_t oc = base + (inner ? q : q_in);
-        const size_t ow = oc - 1 + ((cyc && col == 2) ? nx : 0);
-        const size_t oe = oc + 1 - ((cyc && col == g.nx + 1) ? nx : 0);
-        const uint8_t m = m8[oc];
-        const uint8_t md = m8[oc - dn];
-        const NhCoef f = nh_coef(t, row, col, k, nz, m, m8[oc - 1], m8[oc - P], md);
-        const double a = nh_apply(f, dir(oc - dn), dir(oc - P), dir(ow), v, dir(oe), dir(oc + P),
-                                  dir(oc + up));
-        const double mt = inner ? (double)((m >> 3) & 1) : 0.0;
-        Ap[o] = inner ? a : 0.0;
-        s_dot = s_dot + v * (inner ? a : 0.0) * mt;
+        const double mt = sp_inner(g, row, col) ? (double)((m8[o] >> 3) & 1) : 0.0;
         s_max = fmax(fabs(v * mt), s_max);
     }
-    double acc[2] = {s_dot, s_max};
-    __shared__ double sh[8];
-    for (int l = 0; l < 2; ++l) {
-        double x = acc[l];
-        for (int off = 32; off > 0; off >>= 1) {
-            const double y = __shfl_xor(x, off, 64);
-            x = l == 0 ? x + y : fmax(x, y);
-        }
-        acc[l] = x;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        sh[threadIdx.x >> 6] = acc[0];
-        sh[4 + (threadIdx.x >> 6)] = acc[1];
-    }
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) s_max = fmax(s_max, __shfl_xor(s_max, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s_max;
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
-        part[b] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-        part[nb + b] = fmax(fmax(sh[4], sh[5]), fmax(sh[6], sh[7]));
+        part[nb + b] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
     }
+}
+
+// k_nh_ap: Ap = A p on the interior (zero elsewhere; p's halo columns hold
+// their cyclic copies) and the partial sums of dot_3D(p, Ap) (slot 0).
+// Workgroup row blockIdx.y is level k; within a level, consecutive chunks of
+// the plane go to the workgroups of one XCD (gridDim.x is a multiple of 8, so
+// the chunk's levels above and below run on that XCD too), and its L2 serves
+// the neighbours (a plain grid-stride walk fetched 3.5x the reads).
+__global__ void __launch_bounds__(256) k_nh_ap(Geom g, int nz, NhTables t,
+                                               const uint8_t* __restrict__ m8,
+                                               const double* __restrict__ p,
+                                               double* __restrict__ Ap,
+                                               const SpScal* __restrict__ S,
+                                               double* __restrict__ part) {
+#pragma clang fp contract(off)
+    if (S->done) return;
+    const size_t plane = (size_t)g.pitch * (g.ny + 4);
+    const size_t P = g.pitch;
+    const int k = blockIdx.y;
+    const size_t base = (size_t)k * plane;
+    const size_t dn = k > 0 ? plane : 0, up = k < nz - 1 ? plane : 0;
+    const size_t q_in = 2 * P + 2;
+    double s_dot = 0.0;
+    const size_t chunk = (size_t)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    for (size_t q = chunk * blockDim.x + threadIdx.x; q < plane;
+         q += (size_t)gridDim.x * blockDim.x) {
+        int row, col;
+        sp_rc(g, q, row, col);
+        const bool inner = sp_inner(g, row, col);
+        const size_t oc = base + (inner ? q : q_in);
+        const uint8_t m = m8[oc];
+        const NhCoef f = nh_coef(t, row, col, k, nz, m, m8[oc - 1], m8[oc - P], m8[oc - dn]);
+        const double pc = p[oc];
+        const double a = nh_apply(f, p[oc - dn], p[oc - P], p[oc - 1], pc, p[oc + 1],
+                                  p[oc + P], p[oc + up]);
+        const double mt = inner ? (double)((m >> 3) & 1) : 0.0;
+        Ap[base + q] = inner ? a : 0.0;
+        s_dot = s_dot + pc * (inner ? a : 0.0) * mt;
+    }
+    __shared__ double sh[4];
+    for (int off = 32; off > 0; off >>= 1) s_dot += __shfl_xor(s_dot, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s_dot;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        part[blockIdx.y * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
 // p_non_hydro += alpha p, res -= alpha Ap over the whole array (:188-189), and
@@ -421,9 +437,12 @@ void nh_iterations(psi_nh* c, hipStream_t s, double* x) {
     const dim3 grid((unsigned)c->gx, (unsigned)c->nz);
     const int nb = c->gx * c->nz;
     for (int h = 0; h < 2; ++h) {
-        hipLaunchKernelGGL(k_nh_dir, grid, dim3(256), 0, s, c->g, c->nz, c->T,
-                           (const uint8_t*)c->m8, (const double*)c->r, (const double*)c->pv[h],
-                           c->pv[1 - h], c->Ap, (const SpScal*)c->S, c->part);
+        hipLaunchKernelGGL(k_nh_p, grid, dim3(256), 0, s, c->g, (const uint8_t*)c->m8,
+                           (const double*)c->r, (const double*)c->pv[h], c->pv[1 - h],
+                           (const SpScal*)c->S, c->part);
+        hipLaunchKernelGGL(k_nh_ap, grid, dim3(256), 0, s, c->g, c->nz, c->T,
+                           (const uint8_t*)c->m8, (const double*)c->pv[1 - h], c->Ap,
+                           (const SpScal*)c->S, c->part);
         hipLaunchKernelGGL(k_sp_fold_a, dim3(1), dim3(1024), 0, s, (const double*)c->part, nb,
                            c->S);
         hipLaunchKernelGGL(k_nh_update, grid, dim3(256), 0, s, c->g, c->nz,
@@ -775,8 +794,9 @@ int psi_solve_non_hydrostatic(psi_nh* c, const psi_opts* o, double* u, double* v
 }
 
 // Average durations (us) of `reps` launches of the iteration's kernels: [0]
-// k_nh_dir, [1] k_nh_update, [2] k_sp_fold_a, [3] k_sp_fold_b; HIP events on
-// the graph stream, on the context's own vectors (call between solves).
+// the direction pair k_nh_p + k_nh_ap, [1] k_nh_update, [2] k_sp_fold_a, [3]
+// k_sp_fold_b, and with n >= 6 [4] k_nh_p and [5] k_nh_ap alone; HIP events
+// on the graph stream, on the context's own vectors (call between solves).
 int psi_nh_kernel_times(psi_nh* c, int32_t reps, double* us, int32_t n) {
     using namespace psi;
     if (!valid_nh(c) || !us || n < 4 || reps < 1) return PSI_BAD_ARG;
@@ -801,10 +821,23 @@ int psi_nh_kernel_times(psi_nh* c, int32_t reps, double* us, int32_t n) {
     const int nb = c->gx * c->nz;
     const SpScal* St = c->S_time;
     us[0] = timed([&] {
-        hipLaunchKernelGGL(k_nh_dir, grid, dim3(256), 0, c->gs, c->g, c->nz, c->T,
-                           (const uint8_t*)c->m8, (const double*)c->r, (const double*)c->pv[0],
-                           c->pv[1], c->Ap, St, c->part);
+        hipLaunchKernelGGL(k_nh_p, grid, dim3(256), 0, c->gs, c->g, (const uint8_t*)c->m8,
+                           (const double*)c->r, (const double*)c->pv[0], c->pv[1], St, c->part);
+        hipLaunchKernelGGL(k_nh_ap, grid, dim3(256), 0, c->gs, c->g, c->nz, c->T,
+                           (const uint8_t*)c->m8, (const double*)c->pv[1], c->Ap, St, c->part);
     });
+    if (n >= 6) {
+        us[4] = timed([&] {
+            hipLaunchKernelGGL(k_nh_p, grid, dim3(256), 0, c->gs, c->g, (const uint8_t*)c->m8,
+                               (const double*)c->r, (const double*)c->pv[0], c->pv[1], St,
+                               c->part);
+        });
+        us[5] = timed([&] {
+            hipLaunchKernelGGL(k_nh_ap, grid, dim3(256), 0, c->gs, c->g, c->nz, c->T,
+                               (const uint8_t*)c->m8, (const double*)c->pv[1], c->Ap, St,
+                               c->part);
+        });
+    }
     us[1] = timed([&] {
         hipLaunchKernelGGL(k_nh_update, grid, dim3(256), 0, c->gs, c->g, c->nz,
                            (const uint8_t*)c->m8, c->pv[0], c->r, (const double*)c->pv[1],

@@ -143,9 +143,11 @@ class NonHydrostaticSolver:
         return d["iterations"], bool(d["converged"]), d
 
     def kernel_times(self, reps=20):
-        """Microseconds per launch of the iteration's kernels:
-        {dir_apply, update, fold_a, fold_b} (HIP events, between solves)."""
-        us = np.zeros(4)
+        """Microseconds per launch of the iteration's kernels: {dir_apply (the
+        direction pair), update, fold_a, fold_b, dir (k_nh_p), apply
+        (k_nh_ap)} (HIP events, between solves)."""
+        us = np.zeros(6)
         self._sync_in()
-        self._check(self.L.psi_nh_kernel_times(self.ctx, reps, _f64p(us), 4))
-        return dict(zip(("dir_apply", "update", "fold_a", "fold_b"), us.tolist()))
+        self._check(self.L.psi_nh_kernel_times(self.ctx, reps, _f64p(us), 6))
+        return dict(zip(("dir_apply", "update", "fold_a", "fold_b", "dir", "apply"),
+                        us.tolist()))
