@@ -59,6 +59,7 @@ struct psi_ctx {
     double *part = nullptr, *result = nullptr;
     double* part2 = nullptr;      // solver partials, 2 x NRED x G
     int G = 0, G32 = 0;           // resident workgroups of the FP64 / FP32 solver kernels
+    int ncu = 1;                  // compute units of the device
     unsigned* counter = nullptr;
     CgState* st = nullptr;        // 2 states (double-buffered by launch)
     CgState* h_st = nullptr;      // pinned: [0..1] poll slots, [2] init
@@ -533,8 +534,12 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
 // FP64 residual of (b, x): returns |r|^2, |b|^2 (active points), max|r|, count.
 int residual(psi_ctx* c, const double* b, const double* x, double* r_out, double res[4]) {
     const Geom& g = c->g;
-    hipLaunchKernelGGL(k_residual, dim3(g.ntiles), dim3(NT), 0, c->stream, g, c->E, c->N, c->rx,
-                       c->ry, c->act, b, x, r_out, c->part, c->counter, c->result);
+    // 4 workgroups per CU, a multiple of 8; part holds NRED x (ntiles + 1024)
+    const int Gr = std::max(8, std::min(4 * c->ncu, g.ntiles + 1024) & ~7);
+    hipLaunchKernelGGL(k_residual, dim3(Gr), dim3(NT), 0, c->stream, g, c->E, c->N, c->rx, c->ry,
+                       c->act, b, x, r_out, c->part);
+    hipLaunchKernelGGL(k_residual_fold, dim3(1), dim3(NT), 0, c->stream, (const double*)c->part,
+                       Gr, c->result);
     int rc = check_launch(c);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->h_res, c->result, 4 * sizeof(double), hipMemcpyDeviceToHost,
@@ -1365,6 +1370,7 @@ int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
                 if (G >= 8) G &= ~7;
                 return G;
             };
+            c->ncu = ncu;
             c->G = grid_for(occ64);
             c->G32 = grid_for(occ32);
         }

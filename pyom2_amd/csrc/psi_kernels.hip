@@ -152,20 +152,24 @@ __device__ __forceinline__ void load_metrics_1d(const Geom& g, int c0, int r0, c
 // a contiguous range of tiles, so neighbouring tiles, which share halo rows,
 // are swept through the same L2.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void tile_schedule(const Geom& g, int& first, int& stride, int& end) {
+__device__ __forceinline__ void range_schedule(int n, int& first, int& stride, int& end) {
     const int b = blockIdx.x, G = gridDim.x;
     if (G < 8 || (G & 7)) {
         first = b;
         stride = G;
-        end = g.ntiles;
+        end = n;
         return;
     }
     const int x = b & 7, k = b >> 3;
-    const int q = g.ntiles >> 3, rem = g.ntiles & 7;
+    const int q = n >> 3, rem = n & 7;
     const int start = x * q + (x < rem ? x : rem);
     first = start + k;
     stride = G >> 3;
     end = start + q + (x < rem ? 1 : 0);
+}
+
+__device__ __forceinline__ void tile_schedule(const Geom& g, int& first, int& stride, int& end) {
+    range_schedule(g.ntiles, first, stride, end);
 }
 
 // Fold the previous kernel's partials (slot-major, G per slot) in a fixed
@@ -734,64 +738,63 @@ __global__ __launch_bounds__(NT) void k_cg_refresh(
 // ---------------------------------------------------------------------------
 // FP64 residual r = b - A x with |r|^2 and |b|^2 over active points and
 // max|r| (the true residual of the refinement loop and of every report).
-// r may be null (norms only).
+// r may be null (norms only).  One pass over row segments of NT points;
+// each XCD takes a contiguous range of rows, so a row's neighbours above and
+// below come from the same L2 and the columns either side from the same
+// cache lines.  Partials go to k_residual_fold: a last-arriving-workgroup
+// fold here would need a device-scope fence per workgroup, which writes
+// back the XCD's L2 each time (with one workgroup per tile and an LDS frame
+// the kernel ran at a fifth of the stream rate).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_residual(
     Geom g, const double* __restrict__ E, const double* __restrict__ N,
     const double* __restrict__ rx, const double* __restrict__ ry, const uint8_t* __restrict__ act,
     const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r_out,
-    double* __restrict__ part, unsigned* __restrict__ counter, double* __restrict__ result) {
-    __shared__ double sX[HX * HY], sE[HX * HY], sN[HX * HY];
-    __shared__ double sRX[HX], sRY[HY];
+    double* __restrict__ part) {
     __shared__ double sRed[NRED * NW];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
-    const int c0 = tx * TX, r0 = ty * TY;
-    for (int l = threadIdx.x; l < HX * HY; l += NT) {
-        const int gg = l / HX, f = l - gg * HX;
-        const int r = r0 + gg, c = map_col(c0 + f, g);
-        double xv = 0, e = 0, n = 0;
-        if (in_array(r, c, g)) {
-            const size_t o = (size_t)r * g.pitch + c;
-            xv = x[o];
-            e = E[o];
-            n = N[o];
-        }
-        sX[l] = xv;
-        sE[l] = e;
-        sN[l] = n;
-    }
-    load_metrics_1d(g, c0, r0, rx, ry, sRX, sRY);
-    __syncthreads();
     double acc[NRED] = {0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < (TX * TY) / NT; ++k) {
-        const int l = threadIdx.x + k * NT;
-        const int gg = 2 + l / TX, f = 2 + l % TX;
-        const int r = r0 + gg, c = c0 + f;
-        if (r <= g.ny + 1 && c <= g.nx + 1) {
-            const int idx = gg * HX + f;
-            double cc;
-            const double ax = stencil5(sX, sE, sN, idx, sRX[f], sRY[gg], cc);
-            const size_t o = (size_t)r * g.pitch + c;
-            const double bv = b[o];
-            const double rv = bv - ax;
-            if (act[o]) {
-                acc[0] += rv * rv;
-                acc[1] += bv * bv;
-                acc[2] = fmax(acc[2], fabs(rv));
-                acc[3] += 1.0;
-            }
-            if (r_out) r_out[o] = act[o] ? rv : 0.0;
+    const int segs = (g.nx + NT - 1) / NT;
+    int first, stride, end;
+    range_schedule(g.ny * segs, first, stride, end);
+    const size_t P = g.pitch;
+    for (int it = first; it < end; it += stride) {
+        const int j = it / segs, i = (it - j * segs) * NT + threadIdx.x;
+        if (i >= g.nx) continue;
+        const int r = j + 2, c = i + 2;
+        const size_t o = (size_t)r * P + c;
+        const size_t ow = (size_t)r * P + map_col(c - 1, g), oe = (size_t)r * P + map_col(c + 1, g);
+        // the 5-point operator in stencil5's order and arithmetic
+        const double e = E[o], w = E[ow] * rx[c], n = N[o], so = N[o - P] * ry[r];
+        const double cc = (((-e) - w) - n) - so;
+        double ax = so * x[o - P];
+        ax = ax + w * x[ow];
+        ax = ax + cc * x[o];
+        ax = ax + e * x[oe];
+        ax = ax + n * x[o + P];
+        const double bv = b[o];
+        const double rv = bv - ax;
+        const bool a = act[o] != 0;
+        if (a) {
+            acc[0] += rv * rv;
+            acc[1] += bv * bv;
+            acc[2] = fmax(acc[2], fabs(rv));
+            acc[3] += 1.0;
         }
+        if (r_out) r_out[o] = a ? rv : 0.0;
     }
     block_reduce<NRED>(acc, 1u << 2, sRed);
-    if (publish_partials<NRED>(acc, part, counter, blockIdx.x, gridDim.x)) {
-        fold_partials<NRED>(acc, 1u << 2, part, gridDim.x, sRed);
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < 4; ++k) result[k] = acc[k];
-            *counter = 0u;
-        }
-    }
+    write_partials<NRED>(acc, part);
+}
+
+// Fold k_residual's G partials (one workgroup): result = {|r|^2, |b|^2,
+// max|r|, count}.
+__global__ __launch_bounds__(NT) void k_residual_fold(const double* __restrict__ part, int G,
+                                                     double* __restrict__ result) {
+    __shared__ double sRed[NRED * NW];
+    double v[NRED];
+    fold_prev<NRED>(part, G, 1u << 2, v, sRed);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) result[k] = v[k];
 }
 
 // x += s * e on the interior, then e = 0 (FP64 accumulate of an inner
