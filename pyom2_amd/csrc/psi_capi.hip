@@ -950,10 +950,17 @@ int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
         m->level_ny.push_back(h.ny);
         m->level_nact.push_back(h.nact);
     }
-    MGTRY(mg_alloc(c, m, &m->Ainv, inv.size() + 1));
+    // rows padded to a multiple of 4 floats (16-byte loads), zeros past n
+    const int nc = m->ncoarse, ld = (nc + 3) & ~3;
+    std::vector<float> padded((size_t)nc * ld, 0.0f);
+    for (int i = 0; i < nc; ++i)
+        std::copy(inv.begin() + (size_t)i * nc, inv.begin() + (size_t)(i + 1) * nc,
+                  padded.begin() + (size_t)i * ld);
+    m->dense_ld = ld;
+    MGTRY(mg_alloc(c, m, &m->Ainv, padded.size() + 4));
     MGTRY(mg_alloc(c, m, &m->cells, cells.size() + 1));
     if (!inv.empty()) {
-        HIPCHK(c, hipMemcpy(m->Ainv, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(m->Ainv, padded.data(), padded.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(m->cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
     }
     MGTRY(mg_alloc(c, m, &m->x, c->plane));
@@ -990,7 +997,7 @@ void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch) {
                            m->lev[l + 1], (const MgScal*)m->S);
     if (m->ncoarse > 0)
         hipLaunchKernelGGL(k_mg_dense, dim3((m->ncoarse + 3) / 4), dim3(256), 0, s, m->lev[L],
-                           (const float*)m->Ainv, (const int*)m->cells, m->ncoarse,
+                           (const float*)m->Ainv, m->dense_ld, (const int*)m->cells, m->ncoarse,
                            (const MgScal*)m->S);
     for (int l = L - 1; l >= 0; --l)
         hipLaunchKernelGGL(k_mg_post<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l],

@@ -70,16 +70,26 @@ struct MgScal {
     int n, done, status, max_it;
 };
 
-// Coarsest level: y = Ainv b over its active cells (one wave per row).
-__global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restrict__ Ainv,
+// Coarsest level: y = Ainv b over its active cells.  The workgroup gathers b
+// into LDS once; each wave takes one row of Ainv (row stride ld, a multiple
+// of 4 zero-padded past n) in 16-byte loads, all in flight together.
+__global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restrict__ Ainv, int ld,
                                                   const int* __restrict__ cells, int n,
                                                   const MgScal* __restrict__ S) {
+    __shared__ float4 sB[MG_MAX_DENSE];   // 4 MG_MAX_DENSE floats: the largest coarsest level
     if (S->done) return;
+    float* sb = reinterpret_cast<float*>(sB);
+    for (int k = threadIdx.x; k < ld; k += blockDim.x) sb[k] = k < n ? L.b[cells[k]] : 0.0f;
+    __syncthreads();
     const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
+    const float4* a = reinterpret_cast<const float4*>(Ainv + (size_t)row * ld);
     float s = 0.0f;
-    for (int k = lane; k < n; k += 64) s += Ainv[(size_t)row * n + k] * L.b[cells[k]];
+    for (int k = lane; k < ld / 4; k += 64) {
+        const float4 v = a[k], w = sB[k];
+        s += ((v.x * w.x + v.y * w.y) + v.z * w.z) + v.w * w.w;
+    }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) L.y[cells[row]] = s;
 }
@@ -623,7 +633,8 @@ struct MgState {
     std::vector<MgLev> lev;          // smoothing levels 0..L-1 and the coarsest L
     int L = 0;                       // index of the coarsest level
     int ncoarse = 0;                 // its active cells
-    float* Ainv = nullptr;
+    float* Ainv = nullptr;            // ncoarse rows of dense_ld floats
+    int dense_ld = 0;
     int* cells = nullptr;
     int nact0 = 0;
     double w_mean = 1.0;             // mean psi-cell area over active points
