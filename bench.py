@@ -180,7 +180,8 @@ def main():
         opts = SolveOptions(mode=fallback if mode == "learned" else mode, criterion="relres",
                             relres_target=args.relres, max_iterations=200000,
                             check_every=args.check_every, inner_reduction=args.inner_reduction,
-                            poly_degree=args.poly_degree, poly_lower=args.poly_lower)
+                            poly_degree=args.poly_degree, poly_lower=args.poly_lower,
+                            time_kernels=(mode == "mg"))
         ls = None
         if mode == "learned":
             from pyom2_amd import learned
@@ -210,9 +211,11 @@ def main():
             if not conv:
                 raise SystemExit(f"{mode} solve {k} did not converge: {info}")
             if k >= args.warmup:
+                ns = info.get("kernel_samples", 0)
                 per.append(dict(ms=(te - ts) * 1e3, its=its, relres=info["relres_true"],
                                 sweep_ms=info["sweep_ms"], launches=info["sweep_launches"],
-                                outer=info["outer_iterations"]))
+                                outer=info["outer_iterations"], ksamples=ns,
+                                ksum=np.array(info.get("kernel_us", [0.0] * 4)) * ns))
             if args.verbose:
                 log(f"[bench] {mode} step {k}: {(te - ts) * 1e3:.2f} ms its={its} "
                     f"relres={info['relres_true']:.2e} outer={info['outer_iterations']}")
@@ -230,8 +233,15 @@ def main():
             its.sum() / max(launches, 1)
         achieved = bytes_per_launch / (avg_launch_us * 1e-6) / 1e9 if avg_launch_us > 0 else 0.0
         if mode == "mg":
-            # the solve's heaviest kernels, timed with HIP events between solves
-            kt = s.mg_kernel_times(degree=args.poly_degree, reps=20)
+            # the solve's heaviest kernels, timed inside the timed solves: HIP
+            # events captured around them in the outer iteration's graph, read
+            # at the host polls (the last iteration of every full batch)
+            nsamp = sum(p["ksamples"] for p in per)
+            if nsamp == 0:
+                raise SystemExit("mg: no in-solve kernel timings were recorded")
+            ksum = sum(p["ksum"] for p in per)
+            kt = dict(zip(("pre_level0", "post_level0", "fcg_a", "fcg_b"),
+                          (float(v) / nsamp for v in ksum)))
             top = max(kt, key=kt.get)
             mg_kernels = {k: dict(us=round(v, 3), bytes_per_pt=MG_KERNEL_BYTES[k],
                                   GBs=round(MG_KERNEL_BYTES[k] * npts / (v * 1e-6) / 1e9, 1))
@@ -246,7 +256,7 @@ def main():
                              total_s=total)
         if mode == "mg":
             results[mode].update(us_per_iteration=1e3 * sweep_ms / max(launches, 1),
-                                 kernels=mg_kernels, dominant=top)
+                                 kernels=mg_kernels, dominant=top, kernel_samples=nsamp)
         if ls is not None:
             timed = choices[args.warmup:]
             ch = max(set(timed), key=timed.count)

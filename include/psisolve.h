@@ -101,7 +101,10 @@ typedef struct psi_opts {
     double poly_lower;         /* PSI_MODE_POLY: the polynomial's interval is
                                   [2 poly_lower, 2]; <= 0 picks 0.005 */
     int32_t leg_max_iterations;/* mixed/poly: inner iterations per FP32 leg; <= 0: no cap */
-    int32_t reserved[3];
+    int32_t time_kernels;      /* PSI_MODE_MG: nonzero records HIP events around the
+                                  four heaviest kernels of every outer iteration and
+                                  reports their mean durations in psi_info.kernel_us */
+    int32_t reserved[2];
 } psi_opts;
 
 typedef struct psi_info {
@@ -117,6 +120,9 @@ typedef struct psi_info {
     int32_t sweep_launches;    /* sweep kernels launched (incl. ones that found
                                   the solve finished and returned at once) */
     int32_t setup_launches;    /* setup + residual kernels launched */
+    int32_t kernel_samples;    /* outer iterations timed (psi_opts.time_kernels) */
+    double kernel_us[4];       /* their mean durations (us) of the finest level's
+                                  pre-smoothing, post-smoothing, pass A, pass B */
 } psi_info;
 
 typedef struct psi_ctx psi_ctx;

@@ -55,7 +55,8 @@ class PsiOpts(C.Structure):
                 ("max_iterations", C.c_int32), ("relres_target", C.c_double),
                 ("check_every", C.c_int32), ("inner_reduction", C.c_double),
                 ("max_outer", C.c_int32), ("poly_degree", C.c_int32), ("poly_lower", C.c_double),
-                ("leg_max_iterations", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("leg_max_iterations", C.c_int32), ("time_kernels", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class PsiInfo(C.Structure):
@@ -63,10 +64,13 @@ class PsiInfo(C.Structure):
                 ("outer_iterations", C.c_int32), ("est_error", C.c_double),
                 ("relres_recursive", C.c_double), ("relres_true", C.c_double),
                 ("elapsed_ms", C.c_double), ("sweep_ms", C.c_double),
-                ("sweep_launches", C.c_int32), ("setup_launches", C.c_int32)]
+                ("sweep_launches", C.c_int32), ("setup_launches", C.c_int32),
+                ("kernel_samples", C.c_int32), ("kernel_us", C.c_double * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["kernel_us"] = list(self.kernel_us)
+        return d
 
 
 class PsiBatchMember(C.Structure):

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -795,6 +796,10 @@ void mg_release(psi_ctx* c) {
     if (m->gs) (void)hipStreamSynchronize(m->gs);
     if (m->exec) (void)hipGraphExecDestroy(m->exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->exec_t) (void)hipGraphExecDestroy(m->exec_t);
+    if (m->graph_t) (void)hipGraphDestroy(m->graph_t);
+    for (hipEvent_t& e : m->kev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : m->allocs) (void)hipFree(p);
     if (m->h_S) (void)hipHostFree(m->h_S);
     if (m->e_in) (void)hipEventDestroy(m->e_in);
@@ -980,45 +985,81 @@ int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
     HIPCHK(c, hipEventCreateWithFlags(&m->e_out, hipEventDisableTiming));
     HIPCHK(c, hipEventCreate(&m->t0));
     HIPCHK(c, hipEventCreate(&m->t1));
+    for (hipEvent_t& e : m->kev) HIPCHK(c, hipEventCreate(&e));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PSI_OK;
 }
 
 // One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold.
+// Timing events of a captured outer iteration.  hipEventRecord on a
+// capturing stream adds no node to the graph, so each event goes in as an
+// explicit event-record node after the stream's current capture
+// dependencies, which then become that node.
+struct MgEvents {
+    const hipEvent_t* ev;
+    hipError_t err = hipSuccess;
+    void rec(int k, hipStream_t s) {
+        if (err != hipSuccess) return;
+        hipStreamCaptureStatus st;
+        unsigned long long id = 0;
+        hipGraph_t g = nullptr;
+        const hipGraphNode_t* deps = nullptr;
+        size_t nd = 0;
+        err = hipStreamGetCaptureInfo_v2(s, &st, &id, &g, &deps, &nd);
+        if (err != hipSuccess) return;
+        if (st != hipStreamCaptureStatusActive) {
+            err = hipErrorInvalidValue;   // only used while capturing
+            return;
+        }
+        hipGraphNode_t node;
+        err = hipGraphAddEventRecordNode(&node, g, deps, nd, ev[k]);
+        if (err == hipSuccess)
+            err = hipStreamUpdateCaptureDependencies(s, &node, 1, hipStreamSetCaptureDependencies);
+    }
+};
+
 template <int K>
-void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch) {
+void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, MgEvents* ev) {
     const int L = m->L;
     using F = MgFrame<K>;
     auto tiles = [](const MgLev& v) {
         return dim3((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + F::TY - 1) / F::TY)));
     };
-    for (int l = 0; l < L; ++l)
+    for (int l = 0; l < L; ++l) {
+        if (ev && l == 0) ev->rec(0, s);
         hipLaunchKernelGGL(k_mg_pre<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l], ch,
                            m->lev[l + 1], (const MgScal*)m->S);
+        if (ev && l == 0) ev->rec(1, s);
+    }
     if (m->ncoarse > 0)
         hipLaunchKernelGGL(k_mg_dense, dim3((m->ncoarse + 3) / 4), dim3(256), 0, s, m->lev[L],
                            (const float*)m->Ainv, m->dense_ld, (const int*)m->cells, m->ncoarse,
                            (const MgScal*)m->S);
-    for (int l = L - 1; l >= 0; --l)
+    for (int l = L - 1; l >= 0; --l) {
+        if (ev && l == 0) ev->rec(2, s);
         hipLaunchKernelGGL(k_mg_post<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l],
                            m->lev[l + 1], ch, (const MgScal*)m->S);
+    }
+    if (ev) ev->rec(3, s);
 }
 
-void mg_vcycle(MgState* m, hipStream_t s, int K, const MgCheb& ch) {
+// ev: null, or the 7 events of MgState::kev (the first 4 are recorded here)
+void mg_vcycle(MgState* m, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev = nullptr) {
     switch (K) {
-        case 1: mg_vcycle_fused<1>(m, s, ch); break;
-        case 2: mg_vcycle_fused<2>(m, s, ch); break;
-        case 3: mg_vcycle_fused<3>(m, s, ch); break;
-        case 4: mg_vcycle_fused<4>(m, s, ch); break;
-        case 5: mg_vcycle_fused<5>(m, s, ch); break;
-        default: mg_vcycle_fused<6>(m, s, ch); break;
+        case 1: mg_vcycle_fused<1>(m, s, ch, ev); break;
+        case 2: mg_vcycle_fused<2>(m, s, ch, ev); break;
+        case 3: mg_vcycle_fused<3>(m, s, ch, ev); break;
+        case 4: mg_vcycle_fused<4>(m, s, ch, ev); break;
+        case 5: mg_vcycle_fused<5>(m, s, ch, ev); break;
+        default: mg_vcycle_fused<6>(m, s, ch, ev); break;
     }
 }
 
-// One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold.
-void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch) {
+// One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold;
+// with ev, the kernel-timing events of MgState::kev recorded around them.
+void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev = nullptr) {
     MgState* m = c->mg;
-    mg_vcycle(m, s, K, ch);
+    mg_vcycle(m, s, K, ch, ev);
     const Geom& g = c->g;
     hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
                        (const double*)c->E, (const double*)c->N, (const double*)c->rx,
@@ -1026,32 +1067,42 @@ void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch) {
                        (const double*)c->dyu, (const uint8_t*)c->act,
                        (const double*)m->q, (const double*)m->r, m->az,
                        (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    if (ev) ev->rec(4, s);
     hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->S);
+    if (ev) ev->rec(5, s);
     hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
                        (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
                        (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
                        (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    if (ev) ev->rec(6, s);
     hipLaunchKernelGGL(k_mg_fold_b, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->nact0, m->S, m->zscale);
 }
 
-int mg_graph(psi_ctx* c, int K, double ratio) {
+// The captured outer iteration (timed: with the kernel-timing events).
+int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false) {
     MgState* m = c->mg;
-    if (m->exec && m->graph_K == K && m->graph_ratio == ratio) return PSI_OK;
-    if (m->exec) (void)hipGraphExecDestroy(m->exec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
-    m->exec = nullptr;
-    m->graph = nullptr;
+    hipGraph_t& graph = timed ? m->graph_t : m->graph;
+    hipGraphExec_t& exec = timed ? m->exec_t : m->exec;
+    int& gK = timed ? m->graph_t_K : m->graph_K;
+    double& gratio = timed ? m->graph_t_ratio : m->graph_ratio;
+    if (exec && gK == K && gratio == ratio) return PSI_OK;
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
     const MgCheb ch = mg_cheb(K, ratio);
     HIPCHK(c, hipStreamBeginCapture(m->gs, hipStreamCaptureModeThreadLocal));
-    mg_iteration(c, m->gs, K, ch);
-    hipGraph_t graph = nullptr;
-    HIPCHK(c, hipStreamEndCapture(m->gs, &graph));
-    m->graph = graph;
-    HIPCHK(c, hipGraphInstantiate(&m->exec, graph, nullptr, nullptr, 0));
-    m->graph_K = K;
-    m->graph_ratio = ratio;
+    MgEvents ev{m->kev};
+    mg_iteration(c, m->gs, K, ch, timed ? &ev : nullptr);
+    hipGraph_t cap = nullptr;
+    HIPCHK(c, hipStreamEndCapture(m->gs, &cap));
+    graph = cap;
+    HIPCHK(c, ev.err);
+    HIPCHK(c, hipGraphInstantiate(&exec, cap, nullptr, nullptr, 0));
+    gK = K;
+    gratio = ratio;
     return PSI_OK;
 }
 
@@ -1095,7 +1146,11 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
     const double ratio = 1.0 / lower;
     const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
     const int batch = o->check_every > 0 ? o->check_every : 4;
+    // with time_kernels the last replay of every poll batch runs the timed
+    // graph (its event nodes cost about 3 us each)
+    const bool timed = o->time_kernels != 0;
     int rc = mg_graph(c, K, ratio);
+    if (rc == PSI_OK && timed) rc = mg_graph(c, K, ratio, true);
     if (rc) return rc;
     const Geom& g = c->g;
     HIPCHK(c, hipMemcpyAsync(m->x, sol, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
@@ -1145,11 +1200,37 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
         HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
         const int n_start = n;
         HIPCHK(c, hipEventRecord(m->t0, m->gs));
+        int n_poll = n_start, next = batch;
+        double rr_poll = S0.rr;
         for (;;) {
-            for (int k = 0; k < batch; ++k) HIPCHK(c, hipGraphLaunch(m->exec, m->gs));
+            for (int k = 0; k < next; ++k)
+                HIPCHK(c, hipGraphLaunch(timed && k == next - 1 ? m->exec_t : m->exec, m->gs));
             HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
             HIPCHK(c, hipStreamSynchronize(m->gs));
-            if (m->h_S->done) break;
+            const MgScal& h = *m->h_S;
+            if (timed && h.n - n_poll == next) {
+                // every replay of the batch iterated, so the events hold the
+                // last one's kernels
+                const int pairs[4][2] = {{0, 1}, {2, 3}, {3, 4}, {5, 6}};
+                for (int k = 0; k < 4; ++k) {
+                    float ms = 0.0f;
+                    HIPCHK(c, hipEventElapsedTime(&ms, m->kev[pairs[k][0]], m->kev[pairs[k][1]]));
+                    info->kernel_us[k] += 1e3 * ms;
+                }
+                info->kernel_samples += 1;
+            }
+            if (h.done) break;
+            // Replays after convergence still run the smoothing kernels (they
+            // test the flag beside their loads), so the next batch is cut to
+            // the iterations the observed contraction of |r|^2 says remain.
+            next = batch;
+            if (h.n > n_poll && rr_poll > 0.0 && h.rr > 0.0 && h.rr < rr_poll) {
+                const double rate = std::pow(h.rr / rr_poll, 1.0 / (h.n - n_poll));
+                const double need = std::log(h.rtarget2 * h.bb / h.rr) / std::log(rate);
+                if (need > 0.0 && need < batch) next = std::max(1, (int)std::ceil(need));
+            }
+            n_poll = h.n;
+            rr_poll = h.rr;
         }
         HIPCHK(c, hipEventRecord(m->t1, m->gs));
         HIPCHK(c, hipEventSynchronize(m->t1));
@@ -1170,6 +1251,8 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
         if (rc) return rc;
     }
     HIPCHK(c, hipMemcpyAsync(sol, m->x, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (info->kernel_samples > 0)
+        for (int k = 0; k < 4; ++k) info->kernel_us[k] /= info->kernel_samples;
     info->iterations = status == ST_MAXIT ? o->max_iterations + 1 : n;
     info->status = status_of(status);
     info->converged = status == ST_CONVERGED;

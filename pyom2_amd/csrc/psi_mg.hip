@@ -652,6 +652,14 @@ struct MgState {
     hipGraphExec_t exec = nullptr;
     int graph_K = 0;
     double graph_ratio = 0.0;
+    // the same iteration with HIP events around its four heaviest kernels
+    // (psi_opts.time_kernels): before/after pre level 0, before/after post
+    // level 0, after pass A, before/after pass B
+    hipEvent_t kev[7] = {};
+    hipGraph_t graph_t = nullptr;
+    hipGraphExec_t exec_t = nullptr;
+    int graph_t_K = 0;
+    double graph_t_ratio = 0.0;
     std::vector<void*> allocs;
     int64_t bytes = 0;
     std::vector<int> level_nx, level_ny, level_nact;

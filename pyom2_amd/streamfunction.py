@@ -113,6 +113,7 @@ class SolveOptions:
     max_outer: int = 0
     poly_degree: int = 0
     poly_lower: float = 0.0
+    time_kernels: bool = False
 
     def as_c(self):
         o = PsiOpts()
@@ -126,6 +127,7 @@ class SolveOptions:
         o.max_outer = self.max_outer
         o.poly_degree = self.poly_degree
         o.poly_lower = self.poly_lower
+        o.time_kernels = int(self.time_kernels)
         return o
 
 

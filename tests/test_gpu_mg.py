@@ -172,3 +172,30 @@ def test_mg_spherical_1024_two_islands():
             assert its <= 40
     assert relerr(sols["mg"], sols["poly"]) < 1e-6
     s.close()
+
+
+def test_mg_in_solve_kernel_timing():
+    """time_kernels records HIP events inside the iteration graph: the solve
+    is bit-identical to the untimed one, every full poll batch gives a sample,
+    and the four durations are positive and below the iteration's time."""
+    print("\n[mg kernel timing 512]", flush=True)
+    b = basins.spherical_basin(seed=4, n=512, n_islands=2)
+    isl = islands_trace(b.kbot)
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=DEV)
+    out = {}
+    for timed in (False, True, True):
+        sol = s.zeros()
+        its, conv, info = s.congrad_streamfunction(T(b.forc), sol, mode="mg", criterion="relres",
+                                                   relres_target=1e-10, max_iterations=500,
+                                                   check_every=2, time_kernels=timed)
+        assert conv
+        if timed:
+            assert info["kernel_samples"] > 0 and info["kernel_samples"] >= its // 2 - 4
+            us_iter = 1e3 * info["sweep_ms"] / info["sweep_launches"]
+            assert all(0.0 < u < us_iter for u in info["kernel_us"])
+            np.testing.assert_array_equal(N(sol), out[False][1])
+            assert its == out[False][0]
+        else:
+            assert info["kernel_samples"] == 0 and info["kernel_us"] == [0.0] * 4
+        out[timed] = (its, N(sol))
+    s.close()
