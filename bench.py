@@ -742,10 +742,11 @@ def press_main(args):
         ms.dv.copy_(dv0)
         psi.zero_()
         opts = (SolveOptions(mode="mg", criterion="relres", relres_target=args.relres,
-                             max_iterations=10000) if mode == "mg" else
+                             max_iterations=10000, time_kernels=True) if mode == "mg" else
                 SolveOptions(congr_epsilon=1e-12, max_iterations=200000))
         taum1, tau, taup1 = 0, 1, 2
         per_step, its, solve_ms = [], [], []
+        ksum, nsamp = np.zeros(4), 0
         for k in range(args.warmup + args.steps):
             ms.du[tau] = 0.99 * ms.du[taum1] + rnd(c * 1e-6, lev.maskU)
             ms.dv[tau] = 0.99 * ms.dv[taum1] + rnd(c * 1e-6, lev.maskV)
@@ -760,6 +761,8 @@ def press_main(args):
                 per_step.append(te - ts)
                 its.append(n_it)
                 solve_ms.append(info["elapsed_ms"])
+                ksum += np.array(info["kernel_us"]) * info["kernel_samples"]
+                nsamp += info["kernel_samples"]
             log(f"[press {mode}] {k}: {(te - ts) * 1e3:.2f} ms, solve {info['elapsed_ms']:.2f} "
                 f"ms, {n_it} iterations, est. error {info['est_error']:.2e}, relres "
                 f"{info['relres_true']:.2e}")
@@ -768,13 +771,20 @@ def press_main(args):
                 "solve_ms_mean": round(float(np.mean(solve_ms)), 3),
                 "iterations_mean": round(float(np.mean(its)), 1),
                 "us_per_iteration": round(1e3 * float(np.mean(solve_ms)) /
-                                          max(float(np.mean(its)), 1.0), 2)}
+                                          max(float(np.mean(its)), 1.0), 2),
+                "_kernels": (ksum, nsamp)}
 
     paths = {m: run(m) for m in modes}
     head = modes[0]
     pts = b0.shape[0] * b0.shape[1]
     if head == "mg":
-        kt = s.mg_kernel_times(reps=20)
+        # the heaviest kernels timed inside the timed solves (HIP events in the
+        # iteration graph, the last iteration of every poll batch)
+        ksum, nsamp = paths["mg"]["_kernels"]
+        if nsamp == 0:
+            raise SystemExit("press mg: no in-solve kernel timings were recorded")
+        kt = dict(zip(("pre_level0", "post_level0", "fcg_a", "fcg_b"),
+                      (float(v) / nsamp for v in ksum)))
         pts_i = b0.nx * b0.ny
         kern = {}
         for name, us in kt.items():
@@ -802,6 +812,8 @@ def press_main(args):
                 "path": "plain", "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": kern[dom]["frac"],
                 "traffic": (PRESS_TRAFFIC_2048[dom] if b0.nx == b0.ny == 2048 else None)}
+    for v in paths.values():
+        v.pop("_kernels", None)
     plain_its = paths.get("plain", {}).get("iterations_mean")
     cpu = None if (args.skip_cpu or plain_its is None) else press_cpu_baseline(
         b0, host_op, nz, plain_its, fs, budget_s=args.cpu_seconds)

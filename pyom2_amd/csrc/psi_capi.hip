@@ -1025,6 +1025,10 @@ void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, MgEvents* ev) 
     auto tiles = [](const MgLev& v) {
         return dim3((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + F::TY - 1) / F::TY)));
     };
+    if (ev && L == 0) {   // no smoothing level: empty intervals
+        ev->rec(0, s);
+        ev->rec(1, s);
+    }
     for (int l = 0; l < L; ++l) {
         if (ev && l == 0) ev->rec(0, s);
         hipLaunchKernelGGL(k_mg_pre<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l], ch,
@@ -1035,6 +1039,7 @@ void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, MgEvents* ev) 
         hipLaunchKernelGGL(k_mg_dense, dim3((m->ncoarse + 3) / 4), dim3(256), 0, s, m->lev[L],
                            (const float*)m->Ainv, m->dense_ld, (const int*)m->cells, m->ncoarse,
                            (const MgScal*)m->S);
+    if (ev && L == 0) ev->rec(2, s);
     for (int l = L - 1; l >= 0; --l) {
         if (ev && l == 0) ev->rec(2, s);
         hipLaunchKernelGGL(k_mg_post<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l],
@@ -1081,8 +1086,10 @@ void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* 
 }
 
 // The captured outer iteration (timed: with the kernel-timing events).
-int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false) {
-    MgState* m = c->mg;
+// Capture one outer iteration, iter(stream, cheb, events), into the graph
+// (timed: the copy with the kernel-timing events) unless it is current.
+template <class Iter>
+int mg_capture(psi_ctx* c, MgState* m, int K, double ratio, bool timed, Iter iter) {
     hipGraph_t& graph = timed ? m->graph_t : m->graph;
     hipGraphExec_t& exec = timed ? m->exec_t : m->exec;
     int& gK = timed ? m->graph_t_K : m->graph_K;
@@ -1095,7 +1102,7 @@ int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false) {
     const MgCheb ch = mg_cheb(K, ratio);
     HIPCHK(c, hipStreamBeginCapture(m->gs, hipStreamCaptureModeThreadLocal));
     MgEvents ev{m->kev};
-    mg_iteration(c, m->gs, K, ch, timed ? &ev : nullptr);
+    iter(m->gs, ch, timed ? &ev : nullptr);
     hipGraph_t cap = nullptr;
     HIPCHK(c, hipStreamEndCapture(m->gs, &cap));
     graph = cap;
@@ -1104,6 +1111,52 @@ int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false) {
     gK = K;
     gratio = ratio;
     return PSI_OK;
+}
+
+int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false) {
+    return mg_capture(c, c->mg, K, ratio, timed, [&](hipStream_t s, const MgCheb& ch,
+                                                     MgEvents* ev) {
+        mg_iteration(c, s, K, ch, ev);
+    });
+}
+
+// Replay the outer-iteration graph on m->gs from iteration n_start (|r|^2 =
+// rr0) until the device flags the end, polling the scalars after every batch.
+// Replays after the end still run the smoothing kernels (they test the flag
+// beside their loads), so a batch is cut to the iterations the observed
+// contraction of |r|^2 says remain.  timed: the last replay of each batch
+// is the timed graph, whose kernel durations are added to info.
+int mg_replay(psi_ctx* c, MgState* m, int batch, bool timed, int n_start, double rr0,
+              psi_info* info) {
+    int n_poll = n_start, next = batch;
+    double rr_poll = rr0;
+    for (;;) {
+        for (int k = 0; k < next; ++k)
+            HIPCHK(c, hipGraphLaunch(timed && k == next - 1 ? m->exec_t : m->exec, m->gs));
+        HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
+        HIPCHK(c, hipStreamSynchronize(m->gs));
+        const MgScal& h = *m->h_S;
+        if (timed && h.n - n_poll == next) {
+            // every replay of the batch iterated, so the events hold the last
+            // one's kernels
+            const int pairs[4][2] = {{0, 1}, {2, 3}, {3, 4}, {5, 6}};
+            for (int k = 0; k < 4; ++k) {
+                float ms = 0.0f;
+                HIPCHK(c, hipEventElapsedTime(&ms, m->kev[pairs[k][0]], m->kev[pairs[k][1]]));
+                info->kernel_us[k] += 1e3 * ms;
+            }
+            info->kernel_samples += 1;
+        }
+        if (h.done) return PSI_OK;
+        next = batch;
+        if (h.n > n_poll && rr_poll > 0.0 && h.rr > 0.0 && h.rr < rr_poll) {
+            const double rate = std::pow(h.rr / rr_poll, 1.0 / (h.n - n_poll));
+            const double need = std::log(h.rtarget2 * h.bb / h.rr) / std::log(rate);
+            if (need > 0.0 && need < batch) next = std::max(1, (int)std::ceil(need));
+        }
+        n_poll = h.n;
+        rr_poll = h.rr;
+    }
 }
 
 // Average duration (us) of `reps` back-to-back launches queued by fn on the
@@ -1200,38 +1253,8 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
         HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
         const int n_start = n;
         HIPCHK(c, hipEventRecord(m->t0, m->gs));
-        int n_poll = n_start, next = batch;
-        double rr_poll = S0.rr;
-        for (;;) {
-            for (int k = 0; k < next; ++k)
-                HIPCHK(c, hipGraphLaunch(timed && k == next - 1 ? m->exec_t : m->exec, m->gs));
-            HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
-            HIPCHK(c, hipStreamSynchronize(m->gs));
-            const MgScal& h = *m->h_S;
-            if (timed && h.n - n_poll == next) {
-                // every replay of the batch iterated, so the events hold the
-                // last one's kernels
-                const int pairs[4][2] = {{0, 1}, {2, 3}, {3, 4}, {5, 6}};
-                for (int k = 0; k < 4; ++k) {
-                    float ms = 0.0f;
-                    HIPCHK(c, hipEventElapsedTime(&ms, m->kev[pairs[k][0]], m->kev[pairs[k][1]]));
-                    info->kernel_us[k] += 1e3 * ms;
-                }
-                info->kernel_samples += 1;
-            }
-            if (h.done) break;
-            // Replays after convergence still run the smoothing kernels (they
-            // test the flag beside their loads), so the next batch is cut to
-            // the iterations the observed contraction of |r|^2 says remain.
-            next = batch;
-            if (h.n > n_poll && rr_poll > 0.0 && h.rr > 0.0 && h.rr < rr_poll) {
-                const double rate = std::pow(h.rr / rr_poll, 1.0 / (h.n - n_poll));
-                const double need = std::log(h.rtarget2 * h.bb / h.rr) / std::log(rate);
-                if (need > 0.0 && need < batch) next = std::max(1, (int)std::ceil(need));
-            }
-            n_poll = h.n;
-            rr_poll = h.rr;
-        }
+        rc = mg_replay(c, m, batch, timed, n_start, S0.rr, info);
+        if (rc) return rc;
         HIPCHK(c, hipEventRecord(m->t1, m->gs));
         HIPCHK(c, hipEventSynchronize(m->t1));
         float span = 0.0f;

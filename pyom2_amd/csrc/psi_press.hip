@@ -832,10 +832,11 @@ int sp_residual(psi_press* c, const double* b, const double* x, double* r_out, d
     return PSI_OK;
 }
 
-void sp_mg_iteration(psi_press* c, hipStream_t s, int K, const psi::MgCheb& ch) {
+void sp_mg_iteration(psi_press* c, hipStream_t s, int K, const psi::MgCheb& ch,
+                     MgEvents* ev = nullptr) {
     using namespace psi;
     MgState* m = c->mg;
-    mg_vcycle(m, s, K, ch);
+    mg_vcycle(m, s, K, ch, ev);
     const Geom& g = c->g;
     hipLaunchKernelGGL(k_spmg_fcg_a, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
                        (const double*)c->cE, (const double*)c->cW, (const double*)c->cN,
@@ -843,34 +844,25 @@ void sp_mg_iteration(psi_press* c, hipStream_t s, int K, const psi::MgCheb& ch) 
                        (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
                        (const uint8_t*)c->act, (const double*)m->q, (const double*)m->r, m->az,
                        (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    if (ev) ev->rec(4, s);
     hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->S);
+    if (ev) ev->rec(5, s);
     hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
                        (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
                        (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
                        (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    if (ev) ev->rec(6, s);
     hipLaunchKernelGGL(k_mg_fold_b, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->nact0, m->S, m->zscale);
 }
 
-int sp_mg_graph(psi_press* c, int K, double ratio) {
+int sp_mg_graph(psi_press* c, int K, double ratio, bool timed = false) {
     using namespace psi;
-    MgState* m = c->mg;
-    if (m->exec && m->graph_K == K && m->graph_ratio == ratio) return PSI_OK;
-    if (m->exec) (void)hipGraphExecDestroy(m->exec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
-    m->exec = nullptr;
-    m->graph = nullptr;
-    const MgCheb ch = mg_cheb(K, ratio);
-    HIPCHK(c, hipStreamBeginCapture(m->gs, hipStreamCaptureModeThreadLocal));
-    sp_mg_iteration(c, m->gs, K, ch);
-    hipGraph_t graph = nullptr;
-    HIPCHK(c, hipStreamEndCapture(m->gs, &graph));
-    m->graph = graph;
-    HIPCHK(c, hipGraphInstantiate(&m->exec, graph, nullptr, nullptr, 0));
-    m->graph_K = K;
-    m->graph_ratio = ratio;
-    return PSI_OK;
+    return mg_capture(c, c->mg, K, ratio, timed, [&](hipStream_t s, const MgCheb& ch,
+                                                     MgEvents* ev) {
+        sp_mg_iteration(c, s, K, ch, ev);
+    });
 }
 
 // Multigrid-preconditioned flexible CG to the FP64 true relative residual
@@ -892,7 +884,9 @@ int sp_solve_mg(psi_press* c, const psi_opts* o, const double* forc, double* sol
     const double ratio = 1.0 / lower;
     const double target = o->relres_target > 0 ? o->relres_target : 1e-10;
     const int batch = o->check_every > 0 ? o->check_every : 4;
+    const bool timed = o->time_kernels != 0;
     int rc = sp_mg_graph(c, K, ratio);
+    if (rc == PSI_OK && timed) rc = sp_mg_graph(c, K, ratio, true);
     if (rc) return rc;
     const Geom& g = c->g;
     HIPCHK(c, hipEventRecord(c->t0, c->stream));
@@ -943,12 +937,8 @@ int sp_solve_mg(psi_press* c, const psi_opts* o, const double* forc, double* sol
         HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
         const int n_start = n;
         HIPCHK(c, hipEventRecord(m->t0, m->gs));
-        for (;;) {
-            for (int k = 0; k < batch; ++k) HIPCHK(c, hipGraphLaunch(m->exec, m->gs));
-            HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
-            HIPCHK(c, hipStreamSynchronize(m->gs));
-            if (m->h_S->done) break;
-        }
+        rc = mg_replay(c, m, batch, timed, n_start, S0.rr, info);
+        if (rc) return rc;
         HIPCHK(c, hipEventRecord(m->t1, m->gs));
         HIPCHK(c, hipEventSynchronize(m->t1));
         float span = 0.0f;
@@ -974,6 +964,8 @@ int sp_solve_mg(psi_press* c, const psi_opts* o, const double* forc, double* sol
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, c->t0, c->t1);
     info->elapsed_ms = ms;
+    if (info->kernel_samples > 0)
+        for (int k = 0; k < 4; ++k) info->kernel_us[k] /= info->kernel_samples;
     info->iterations = status == ST_MAXIT ? o->max_iterations + 1 : n;
     info->status = status_of(status);
     info->converged = status == ST_CONVERGED;

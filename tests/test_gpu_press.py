@@ -266,6 +266,16 @@ def test_mg_mode_reaches_the_residual_target(n, cyclic, islands):
                                             max_iterations=100000)
     assert conv_p and its_p > its
     assert np.abs(N(x) - N(xp)).max() <= 1e-6 * np.abs(N(xp)).max()
+    # in-solve kernel timing: the same solve bit for bit, positive durations
+    xt = T(x0)
+    its_t, conv_t, info_t = s.congrad_surf_press(T(forc), xt, mode="mg", criterion="relres",
+                                                 relres_target=1e-10, max_iterations=500,
+                                                 check_every=2, time_kernels=True)
+    assert conv_t and its_t == its
+    np.testing.assert_array_equal(N(xt), N(x))
+    # (the smallest basin has no smoothing level: its pre/post intervals are empty)
+    assert info_t["kernel_samples"] > 0 and all(u >= 0.0 for u in info_t["kernel_us"])
+    assert info_t["kernel_us"][2] > 0.0 and info_t["kernel_us"][3] > 0.0
     s.close()
 
 
