@@ -332,6 +332,28 @@ __device__ __forceinline__ double mg_Az(const Geom& g, const MgLev& L0,
            n * mg_z(g, L0, i, j + 1, inv) + s * mg_z(g, L0, i, j - 1, inv);
 }
 
+// z at (i, j) and its four neighbours, each as mg_z gives it.  Every load is
+// inside the level layout (its halo row above and below, MG_XOFF columns to
+// the left and at least 4 to the right), so all five are issued before the
+// out-of-interior ones are replaced by zero.
+__device__ __forceinline__ void mg_z5(const Geom& g, const MgLev& L0, int i, int j, double inv,
+                                      double& zc, double& zs, double& zw, double& ze,
+                                      double& zn) {
+    const int P = L0.pitch, q = (j + 1) * P + i + MG_XOFF;
+    int dw = -1, de = 1;
+    if (g.cyclic) {
+        if (i == 0) dw = g.nx - 1;
+        if (i == g.nx - 1) de = 1 - g.nx;
+    }
+    const float yc = L0.y[q], ys = L0.y[q - P], yn = L0.y[q + P];
+    const float yw = L0.y[q + dw], ye = L0.y[q + de];
+    zc = (double)yc * inv;
+    zs = j > 0 ? (double)ys * inv : 0.0;
+    zn = j < g.ny - 1 ? (double)yn * inv : 0.0;
+    zw = (g.cyclic || i > 0) ? (double)yw * inv : 0.0;
+    ze = (g.cyclic || i < g.nx - 1) ? (double)ye * inv : 0.0;
+}
+
 __device__ __forceinline__ double mg_area(const double* dxu, const double* cosu,
                                           const double* dyu, int i, int j) {
     return dxu[i + 2] * cosu[j + 2] * dyu[j + 2];
@@ -374,18 +396,27 @@ __global__ void __launch_bounds__(256) k_mg_fcg_a(
     if (S->done) return;
     const double inv = 1.0 / *zscale;   // the scale of the V-cycle's input
     double acc[3] = {0, 0, 0};
-    const int n = g.nx * g.ny;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const int i = t % g.nx, j = t / g.nx;
+    // row segments of 256 points, each XCD a contiguous range of rows
+    const int segs = (g.nx + 255) / 256;
+    int first, stride, end;
+    range_schedule(g.ny * segs, first, stride, end);
+    for (int it = first; it < end; it += stride) {
+        const int j = it / segs, i = (it - j * segs) * 256 + threadIdx.x;
+        if (i >= g.nx) continue;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+        const int c = i + 2, rr = j + 2;
+        // every load in bounds, issued before the active test
+        double zc, zs, zw, ze, zn;
+        mg_z5(g, L0, i, j, inv, zc, zs, zw, ze, zn);
+        const double e = E[o], w = E[o - 1] * rx[c], n = N[o], so = N[o - g.pitch] * ry[rr];
+        const double qv = q[o], rv = r[o], wa = mg_area(dxu, cosu, dyu, i, j);
         if (!act[o]) continue;
-        double z;
-        const double az = mg_Az(g, L0, E, N, rx, ry, i, j, o, inv, z);
+        const double cc = (((-e) - w) - n) - so;
+        const double az = cc * zc + e * ze + w * zw + n * zn + so * zs;
         az_out[o] = az;
-        const double w = mg_area(dxu, cosu, dyu, i, j);
-        acc[0] += w * z * az;
-        acc[1] += w * z * q[o];
-        acc[2] += w * r[o] * z;
+        acc[0] += wa * zc * az;
+        acc[1] += wa * zc * qv;
+        acc[2] += wa * rv * zc;
     }
     mg_block_partials<3>(acc, part);
 }
@@ -449,24 +480,31 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
     const double inv = 1.0 / *zscale;
     const double beta = S->beta, alpha = S->alpha, sr = S->s_r;
     double acc[2] = {0, 0};
-    const int n = g.nx * g.ny;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const int i = t % g.nx, j = t / g.nx;
+    // row segments of 256 points, each XCD a contiguous range of rows
+    const int segs = (g.nx + 255) / 256;
+    int first, stride, end;
+    range_schedule(g.ny * segs, first, stride, end);
+    for (int it = first; it < end; it += stride) {
+        const int j = it / segs, i = (it - j * segs) * 256 + threadIdx.x;
+        if (i >= g.nx) continue;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
         const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
+        // every load in bounds, issued before the active test
+        const float yv = L0.y[ql];
+        const double pv = p[o], qv = q[o], xv = x[o], rv = r[o], azv = az_in[o];
+        const double w = mg_area(dxu, cosu, dyu, i, j);
         if (!act[o]) {
             L0.b[ql] = 0.0f;
             continue;
         }
-        const double z = (double)L0.y[ql] * inv;
-        const double pn = z + beta * p[o];
-        const double qn = az_in[o] + beta * q[o];
+        const double z = (double)yv * inv;
+        const double pn = z + beta * pv;
+        const double qn = azv + beta * qv;
         p[o] = pn;
         q[o] = qn;
-        x[o] = x[o] + alpha * pn;
-        const double rn = r[o] - alpha * qn;
+        x[o] = xv + alpha * pn;
+        const double rn = rv - alpha * qn;
         r[o] = rn;
-        const double w = mg_area(dxu, cosu, dyu, i, j);
         L0.b[ql] = (float)(w * rn * sr);
         acc[0] += rn * rn;
         acc[1] += (w * rn) * (w * rn);

@@ -518,20 +518,25 @@ __global__ void __launch_bounds__(256) k_spmg_fcg_a(
     if (S->done) return;
     const double inv = 1.0 / *zscale;
     double acc[3] = {0, 0, 0};
-    const int n = g.nx * g.ny;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const int i = t % g.nx, j = t / g.nx;
+    // row segments of 256 points, each XCD a contiguous range of rows
+    const int segs = (g.nx + 255) / 256;
+    int first, stride, end;
+    range_schedule(g.ny * segs, first, stride, end);
+    for (int it = first; it < end; it += stride) {
+        const int j = it / segs, i = (it - j * segs) * 256 + threadIdx.x;
+        if (i >= g.nx) continue;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+        // every load in bounds, issued before the active test
+        double z, zs, zw, ze, zn;
+        mg_z5(g, L0, i, j, inv, z, zs, zw, ze, zn);
+        const double e = cE[o], w = cW[o], n = cN[o], so = cS[o], mk = (double)m8[o];
+        const double qv = q[o], rv = r[o], wa = mg_area(dxt, cost, dyt, i, j);
         if (!act[o]) continue;
-        const double z = mg_z(g, L0, i, j, inv);
-        const double az = sp_apply(cE[o], cW[o], cN[o], cS[o], (double)m8[o], fs, kfs,
-                                   mg_z(g, L0, i, j - 1, inv), mg_z(g, L0, i - 1, j, inv), z,
-                                   mg_z(g, L0, i + 1, j, inv), mg_z(g, L0, i, j + 1, inv));
+        const double az = sp_apply(e, w, n, so, mk, fs, kfs, zs, zw, z, ze, zn);
         az_out[o] = az;
-        const double w = mg_area(dxt, cost, dyt, i, j);
-        acc[0] += w * z * az;
-        acc[1] += w * z * q[o];
-        acc[2] += w * r[o] * z;
+        acc[0] += wa * z * az;
+        acc[1] += wa * z * qv;
+        acc[2] += wa * rv * z;
     }
     mg_block_partials<3>(acc, part);
 }
