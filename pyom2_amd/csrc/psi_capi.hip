@@ -1018,21 +1018,45 @@ struct MgEvents {
     }
 };
 
+// The level kernels at level v's tile height.
+template <int K>
+void mg_launch_pre(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
+                   hipStream_t s) {
+    if (mg_tile_y(v) == MG_TY_LARGE) {
+        using F = MgFrame<K, MG_TY_LARGE>;
+        hipLaunchKernelGGL((k_mg_pre<K, MG_TY_LARGE>), dim3(mg_tiles(v, F::TY)), dim3(F::NT), 0,
+                           s, v, ch, vc, S);
+    } else {
+        using F = MgFrame<K, MG_TY_SMALL>;
+        hipLaunchKernelGGL((k_mg_pre<K, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT), 0,
+                           s, v, ch, vc, S);
+    }
+}
+
+template <int K>
+void mg_launch_post(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
+                    hipStream_t s) {
+    if (mg_tile_y(v) == MG_TY_LARGE) {
+        using F = MgFrame<K, MG_TY_LARGE>;
+        hipLaunchKernelGGL((k_mg_post<K, MG_TY_LARGE>), dim3(mg_tiles(v, F::TY)), dim3(F::NT),
+                           0, s, v, vc, ch, S);
+    } else {
+        using F = MgFrame<K, MG_TY_SMALL>;
+        hipLaunchKernelGGL((k_mg_post<K, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT),
+                           0, s, v, vc, ch, S);
+    }
+}
+
 template <int K>
 void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, MgEvents* ev) {
     const int L = m->L;
-    using F = MgFrame<K>;
-    auto tiles = [](const MgLev& v) {
-        return dim3((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + F::TY - 1) / F::TY)));
-    };
     if (ev && L == 0) {   // no smoothing level: empty intervals
         ev->rec(0, s);
         ev->rec(1, s);
     }
     for (int l = 0; l < L; ++l) {
         if (ev && l == 0) ev->rec(0, s);
-        hipLaunchKernelGGL(k_mg_pre<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l], ch,
-                           m->lev[l + 1], (const MgScal*)m->S);
+        mg_launch_pre<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
         if (ev && l == 0) ev->rec(1, s);
     }
     if (m->ncoarse > 0)
@@ -1042,8 +1066,7 @@ void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, MgEvents* ev) 
     if (ev && L == 0) ev->rec(2, s);
     for (int l = L - 1; l >= 0; --l) {
         if (ev && l == 0) ev->rec(2, s);
-        hipLaunchKernelGGL(k_mg_post<K>, tiles(m->lev[l]), dim3(F::NT), 0, s, m->lev[l],
-                           m->lev[l + 1], ch, (const MgScal*)m->S);
+        mg_launch_post<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
     }
     if (ev) ev->rec(3, s);
 }
@@ -1176,15 +1199,9 @@ template <int K>
 void mg_time_levels(MgState* m, int reps, double* us) {
     const MgCheb ch = mg_cheb(K, 15.0);
     const MgLev& v = m->lev[0];
-    using F = MgFrame<K>;
-    const dim3 tiles((unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + F::TY - 1) / F::TY)));
     const MgScal* S = m->S_time;
-    us[0] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(k_mg_post<K>, tiles, dim3(F::NT), 0, m->gs, v, m->lev[1], ch, S);
-    });
-    us[1] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(k_mg_pre<K>, tiles, dim3(F::NT), 0, m->gs, v, ch, m->lev[1], S);
-    });
+    us[0] = mg_time(m, reps, [&] { mg_launch_post<K>(v, m->lev[1], ch, S, m->gs); });
+    us[1] = mg_time(m, reps, [&] { mg_launch_pre<K>(v, m->lev[1], ch, S, m->gs); });
 }
 
 int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {

@@ -99,22 +99,35 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
 // LDS: the K smoothing steps recompute the frame's shrinking margins instead
 // of writing every step to memory, so a level costs two kernels per V-cycle
 // (pre-smoothing + restriction, prolongation + post-smoothing).
-// Tile MT_X x MT_Y with one thread per tile cell: each thread carries about
-// two frame cells per step (tile shape settable at build time).
-#ifndef PSI_MG_TX
-#define PSI_MG_TX 64
-#endif
-#ifndef PSI_MG_TY
-#define PSI_MG_TY 16
-#endif
-constexpr int MT_X = PSI_MG_TX;
+// Tiles are MT_X wide and TY tall (16, or 32 on the largest levels, where
+// the frame's share of the reads drops from 1.69x to 1.41x the tile and
+// there are still several tiles per CU); at most 1024 threads, each
+// carrying two or three frame cells per step.
+constexpr int MT_X = 64;
+constexpr int MG_TY_SMALL = 16, MG_TY_LARGE = 32;
 
-template <int K>
+template <int TY>
+struct MgThreads {
+    static constexpr int NT = MT_X * TY < 1024 ? MT_X * TY : 1024;
+};
+
+template <int K, int TY_>
 struct MgFrame {
-    static constexpr int TY = PSI_MG_TY, NT = MT_X * TY;
+    static constexpr int TY = TY_, NT = MgThreads<TY_>::NT;
     static constexpr int FX = MT_X + 2 * K, FY = TY + 2 * K, FN = FX * FY;
     static constexpr int PER = (FN + NT - 1) / NT;
 };
+
+// Tile height of a level: the tall tile where it still leaves at least
+// 1024 tiles (4 per CU).
+inline int mg_tile_y(const MgLev& v) {
+    const long t = (long)((v.nx + MT_X - 1) / MT_X) * ((v.ny + MG_TY_LARGE - 1) / MG_TY_LARGE);
+    return t >= 1024 ? MG_TY_LARGE : MG_TY_SMALL;
+}
+
+inline unsigned mg_tiles(const MgLev& v, int ty) {
+    return (unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + ty - 1) / ty));
+}
 
 // Level cell of frame cell (fx, fy) of the tile at (I0, J0); -1 outside the
 // level (cyclic in x wraps).
@@ -140,10 +153,10 @@ __device__ __forceinline__ float mg_fs(const float* v, const float* sE, const fl
 // x of its frame cells in registers; only the direction d, whose neighbours
 // the next step reads, goes through LDS (double-buffered: one barrier per
 // step).
-template <int K>
-__global__ void __launch_bounds__(MgFrame<K>::NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
-                                                  const MgScal* __restrict__ S) {
-    using F = MgFrame<K>;
+template <int K, int TY_>
+__global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
+                                                       const MgScal* __restrict__ S) {
+    using F = MgFrame<K, TY_>;
     constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
     __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
     const int done = S->done;   // read beside the frame loads, checked before any store
@@ -226,10 +239,10 @@ __global__ void __launch_bounds__(MgFrame<K>::NT) k_mg_pre(MgLev L, MgCheb ch, M
 // K Chebyshev steps of post-smoothing; the result goes to y.  y' (needed
 // with its neighbours once) and then d go through LDS, r and y stay in
 // registers.
-template <int K>
-__global__ void __launch_bounds__(MgFrame<K>::NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
-                                                   const MgScal* __restrict__ S) {
-    using F = MgFrame<K>;
+template <int K, int TY_>
+__global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
+                                                        const MgScal* __restrict__ S) {
+    using F = MgFrame<K, TY_>;
     constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
     __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
     const int done = S->done;
