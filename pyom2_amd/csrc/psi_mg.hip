@@ -161,7 +161,10 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb c
     __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
     const int done = S->done;   // read beside the frame loads, checked before any store
     const int tiles_x = (L.nx + MT_X - 1) / MT_X;
-    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * TY;
+    // each XCD sweeps a contiguous band of tiles, so the frame margins shared
+    // with the neighbouring tiles are read through the same L2
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int I0 = (tile % tiles_x) * MT_X, J0 = (tile / tiles_x) * TY;
     float r[PER], x[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -247,7 +250,10 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev L
     __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
     const int done = S->done;
     const int tiles_x = (L.nx + MT_X - 1) / MT_X;
-    const int I0 = (blockIdx.x % tiles_x) * MT_X, J0 = (blockIdx.x / tiles_x) * TY;
+    // each XCD sweeps a contiguous band of tiles, so the frame margins shared
+    // with the neighbouring tiles are read through the same L2
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int I0 = (tile % tiles_x) * MT_X, J0 = (tile / tiles_x) * TY;
     float r[PER], y[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
