@@ -52,16 +52,16 @@ BYTES_PER_PT = {"plain": 6 * 8 + 2 * 8 + 1, "mixed": 6 * 4 + 2 * 4 + 1,
 # the mask and writes p, q, r, x and the next V-cycle input
 MG_KERNEL_BYTES = {"post_level0": 5 * 4 + 1 + 1 + 4, "pre_level0": 4 * 4 + 4 + 1,
                    "fcg_a": 4 + 2 * 8 + 2 * 8 + 1 + 8, "fcg_b": 4 + 8 + 4 * 8 + 1 + 4 * 8 + 4}
-MG_KERNEL_NAMES = {"post_level0": "k_mg_post<4> (finest level)",
-                   "pre_level0": "k_mg_pre<4> (finest level)",
+MG_KERNEL_NAMES = {"post_level0": "k_mg_post<4, 32> (finest level)",
+                   "pre_level0": "k_mg_pre<4, 32> (finest level)",
                    "fcg_a": "k_mg_fcg_a", "fcg_b": "k_mg_fcg_b"}
 # HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
 # profiles/k_cg_sweep_spherical_2048.md, profiles/k_poly_pcg_spherical_2048.md
 TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6, "poly": 175.5e6}
 # ... of the multigrid path's kernels (profiles/k_mg_spherical_2048.md)
-MG_TRAFFIC_2048 = {"post_level0": 168.9e6, "pre_level0": 130.3e6, "fcg_a": 201.8e6,
-                   "fcg_b": 357.1e6}
+MG_TRAFFIC_2048 = {"post_level0": 113.3e6, "pre_level0": 92.3e6, "fcg_a": 194.0e6,
+                   "fcg_b": 341.4e6}
 # ... of the surface-pressure CG kernels (profiles/k_sp_press_2048x40.md) and of the
 # velocity update at 2048^2 x 40 levels (profiles/levels_kernels_step_2048x40.md)
 PRESS_TRAFFIC_2048 = {"dir_apply": 283.2e6, "update": 206.4e6}
@@ -668,12 +668,12 @@ PRESS_LEVEL_BYTES = {"forcing": 24 + 120, "velocity": 48}
 # masks, q and r and writes Az; pass B as MG_KERNEL_BYTES
 PRESS_MG_BYTES = {"post_level0": 26, "pre_level0": 21, "fcg_a": 4 + 32 + 1 + 1 + 16 + 8,
                   "fcg_b": 81}
-PRESS_MG_NAMES = {"post_level0": "k_mg_post<4> (finest level)",
-                  "pre_level0": "k_mg_pre<4> (finest level)", "fcg_a": "k_spmg_fcg_a",
+PRESS_MG_NAMES = {"post_level0": "k_mg_post<4, 32> (finest level)",
+                  "pre_level0": "k_mg_pre<4, 32> (finest level)", "fcg_a": "k_spmg_fcg_a",
                   "fcg_b": "k_mg_fcg_b"}
 # ... HBM bytes per launch at 2048^2 (profiles/k_spmg_press_2048x40.md)
-PRESS_MG_TRAFFIC_2048 = {"post_level0": 169.4e6, "pre_level0": 130.4e6, "fcg_a": 279.7e6,
-                         "fcg_b": 357.9e6}
+PRESS_MG_TRAFFIC_2048 = {"post_level0": 113.3e6, "pre_level0": 92.3e6, "fcg_a": 265.2e6,
+                         "fcg_b": 341.9e6}
 
 
 def press_main(args):
