@@ -280,7 +280,8 @@ def main():
         cpu_all = cpu_baseline(b, isl, args.cpu_iters, its_ref, threaded=True,
                                budget_s=args.cpu_seconds / 2)
     line = {
-        "metric": "ms per 2048\u00b2 psi solve to 1e-10 rel. FP64 residual: learned vs Jacobi PCG",
+        "metric": f"ms per {args.n}\u00b2 psi solve to 1e-10 rel. FP64 residual: learned vs "
+                  f"Jacobi PCG",
         "value": round(r["ms_per_solve"], 3),
         "unit": "ms",
         "n_gpus": 1,
