@@ -325,35 +325,10 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev L
 }
 
 // ---- outer flexible CG (FP64, solver grid layout) ---------------------------
-// z(i,j) = y0(i,j) / s_r with wrap in x (cyclic) and 0 outside the interior.
-__device__ __forceinline__ double mg_z(const Geom& g, const MgLev& L0, int i, int j, double inv) {
-    if (j < 0 || j >= g.ny) return 0.0;
-    if (i < 0 || i >= g.nx) {
-        if (!g.cyclic) return 0.0;
-        i = (i + g.nx) % g.nx;
-    }
-    return (double)L0.y[(j + 1) * L0.pitch + i + MG_XOFF] * inv;
-}
-
-// A z at interior point (i,j) (ctx index o) with the stored operator in
-// FP64: W = E(i-1,j) rx(i), S = N(i,j-1) ry(j), C = -E-W-N-S.
-__device__ __forceinline__ double mg_Az(const Geom& g, const MgLev& L0,
-                                        const double* __restrict__ E,
-                                        const double* __restrict__ N,
-                                        const double* __restrict__ rx,
-                                        const double* __restrict__ ry, int i, int j, size_t o,
-                                        double inv, double& z) {
-    z = mg_z(g, L0, i, j, inv);
-    const int c = i + 2, r = j + 2;
-    const double e = E[o], w = E[o - 1] * rx[c], n = N[o], s = N[o - g.pitch] * ry[r];
-    const double cc = (((-e) - w) - n) - s;
-    return cc * z + e * mg_z(g, L0, i + 1, j, inv) + w * mg_z(g, L0, i - 1, j, inv) +
-           n * mg_z(g, L0, i, j + 1, inv) + s * mg_z(g, L0, i, j - 1, inv);
-}
-
-// z at (i, j) and its four neighbours, each as mg_z gives it.  Every load is
-// inside the level layout (its halo row above and below, MG_XOFF columns to
-// the left and at least 4 to the right), so all five are issued before the
+// z = y0 / s_r at interior point (i, j) and its four neighbours, zero
+// outside the interior and wrapped in x when cyclic.  Every load is inside
+// the level layout (its halo row above and below, MG_XOFF columns to the
+// left and at least 4 to the right), so all five are issued before the
 // out-of-interior ones are replaced by zero.
 __device__ __forceinline__ void mg_z5(const Geom& g, const MgLev& L0, int i, int j, double inv,
                                       double& zc, double& zs, double& zw, double& ze,
