@@ -199,3 +199,21 @@ def test_mg_in_solve_kernel_timing():
             assert info["kernel_samples"] == 0 and info["kernel_us"] == [0.0] * 4
         out[timed] = (its, N(sol))
     s.close()
+
+
+def test_mg_cyclic_tall_tiles():
+    """A cyclic basin large enough for the 64 x 32 tiles on its finest level
+    (2048 x 512: 512 tiles), whose frames wrap across the seam: the FP64 true
+    residual recomputed on the CPU meets the target."""
+    print("\n[mg cyclic 2048x512]", flush=True)
+    b = basins.beta_plane_basin(seed=5, nx=2048, ny=512, n_islands=2, cyclic_x=True)
+    isl = islands_trace(b.kbot, cyclic_x=True)
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=DEV)
+    sol = s.zeros()
+    its, conv, info = s.congrad_streamfunction(T(b.forc), sol, mode="mg", criterion="relres",
+                                               relres_target=1e-10, max_iterations=500)
+    print(f"  its={its} relres={info['relres_true']:.2e}", flush=True)
+    assert conv and its <= 60
+    rel, _ = _oracle_relres(b, isl, N(sol))
+    assert rel <= 1.01e-10
+    s.close()
