@@ -52,15 +52,22 @@ BYTES_PER_PT = {"plain": 6 * 8 + 2 * 8 + 1, "mixed": 6 * 4 + 2 * 4 + 1,
 # the mask and writes p, q, r, x and the next V-cycle input
 MG_KERNEL_BYTES = {"post_level0": 5 * 4 + 1 + 1 + 4, "pre_level0": 4 * 4 + 4 + 1,
                    "fcg_a": 4 + 2 * 8 + 2 * 8 + 1 + 8, "fcg_b": 4 + 8 + 4 * 8 + 1 + 4 * 8 + 4}
-MG_KERNEL_NAMES = {"post_level0": "k_mg_post<4, 32> (finest level)",
-                   "pre_level0": "k_mg_pre<4, 32> (finest level)",
-                   "fcg_a": "k_mg_fcg_a", "fcg_b": "k_mg_fcg_b"}
+# the library's default smoothing degrees (psisolve.h): 5 for the
+# streamfunction, 4 for the surface pressure
+MG_DEFAULT_DEGREE = 5
+PRESS_MG_DEFAULT_DEGREE = 4
+
+
+def mg_kernel_names(degree):
+    return {"post_level0": f"k_mg_post<{degree}, 32> (finest level)",
+            "pre_level0": f"k_mg_pre<{degree}, 32> (finest level)",
+            "fcg_a": "k_mg_fcg_a", "fcg_b": "k_mg_fcg_b"}
 # HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
 # profiles/k_cg_sweep_spherical_2048.md, profiles/k_poly_pcg_spherical_2048.md
 TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6, "poly": 175.5e6}
 # ... of the multigrid path's kernels (profiles/k_mg_spherical_2048.md)
-MG_TRAFFIC_2048 = {"post_level0": 113.3e6, "pre_level0": 92.3e6, "fcg_a": 194.0e6,
+MG_TRAFFIC_2048 = {"post_level0": 114.0e6, "pre_level0": 92.8e6, "fcg_a": 194.0e6,
                    "fcg_b": 341.4e6}
 # ... of the surface-pressure CG kernels (profiles/k_sp_press_2048x40.md) and of the
 # velocity update at 2048^2 x 40 levels (profiles/levels_kernels_step_2048x40.md)
@@ -300,7 +307,8 @@ def main():
         "paths": {m: {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
                       for k, v in d.items()} for m, d in results.items()},
         "roofline": ({"bound": "hbm",
-                      "kernel": (MG_KERNEL_NAMES[rk["dominant"]] if kpath == "mg" else
+                      "kernel": (mg_kernel_names(args.poly_degree or MG_DEFAULT_DEGREE)
+                                 [rk["dominant"]] if kpath == "mg" else
                                  "k_poly_pcg<4, sweep>" if kpath == "poly" else
                                  f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>"),
                       "path": kpath, "achieved": round(rk["achieved_GBs"], 1),
@@ -669,9 +677,6 @@ PRESS_LEVEL_BYTES = {"forcing": 24 + 120, "velocity": 48}
 # masks, q and r and writes Az; pass B as MG_KERNEL_BYTES
 PRESS_MG_BYTES = {"post_level0": 26, "pre_level0": 21, "fcg_a": 4 + 32 + 1 + 1 + 16 + 8,
                   "fcg_b": 81}
-PRESS_MG_NAMES = {"post_level0": "k_mg_post<4, 32> (finest level)",
-                  "pre_level0": "k_mg_pre<4, 32> (finest level)", "fcg_a": "k_spmg_fcg_a",
-                  "fcg_b": "k_mg_fcg_b"}
 # ... HBM bytes per launch at 2048^2 (profiles/k_spmg_press_2048x40.md)
 PRESS_MG_TRAFFIC_2048 = {"post_level0": 113.3e6, "pre_level0": 92.3e6, "fcg_a": 265.2e6,
                          "fcg_b": 341.9e6}
@@ -743,7 +748,9 @@ def press_main(args):
         ms.dv.copy_(dv0)
         psi.zero_()
         opts = (SolveOptions(mode="mg", criterion="relres", relres_target=args.relres,
-                             max_iterations=10000, time_kernels=True) if mode == "mg" else
+                             max_iterations=10000, time_kernels=True,
+                             poly_degree=args.poly_degree, poly_lower=args.poly_lower)
+                if mode == "mg" else
                 SolveOptions(congr_epsilon=1e-12, max_iterations=200000))
         taum1, tau, taup1 = 0, 1, 2
         per_step, its, solve_ms = [], [], []
@@ -794,7 +801,9 @@ def press_main(args):
                           "frac": round(gbs / HBM_PEAK_GBS, 4),
                           "bytes_per_pt": PRESS_MG_BYTES[name]}
         dom = max(kt, key=lambda x: kt[x])
-        roof = {"bound": "hbm", "kernel": PRESS_MG_NAMES[dom], "path": "mg",
+        names = dict(mg_kernel_names(args.poly_degree or PRESS_MG_DEFAULT_DEGREE),
+                     fcg_a="k_spmg_fcg_a")
+        roof = {"bound": "hbm", "kernel": names[dom], "path": "mg",
                 "achieved": kern[dom]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": kern[dom]["frac"],
                 "traffic": (PRESS_MG_TRAFFIC_2048.get(dom) if b0.nx == b0.ny == 2048 else None)}

@@ -49,8 +49,9 @@ typedef enum psi_mode {
                               poly_degree in Z A (Z the Jacobi inverse) times Z */
     PSI_MODE_MG = 4        /* FP64 flexible CG preconditioned by an FP32 aggregation-multigrid
                               V-cycle (psi_mg_setup first); PSI_CRIT_RELRES only.  poly_degree
-                              = Chebyshev smoothing steps per level (<= 0: 4), poly_lower = the
-                              smoother's interval [2 poly_lower, 2] (<= 0: 1/15) */
+                              = Chebyshev smoothing steps per level (<= 0: 5; 4 for the
+                              surface pressure), poly_lower = the smoother's interval
+                              [2 poly_lower, 2] (<= 0: 1/20; 1/15 for the surface pressure) */
 } psi_mode;
 
 typedef enum psi_criterion {
