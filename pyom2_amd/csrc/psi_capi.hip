@@ -1332,7 +1332,7 @@ int psi_mg_kernel_times(psi_ctx* c, int32_t degree, int32_t reps, double* us, in
     MgState* m = c->mg;
     if (!m) return fail(c, PSI_BAD_ARG, "call psi_mg_setup first%s");
     if (m->L < 1) return fail(c, PSI_BAD_ARG, "multigrid: a single level has no V-cycle kernels%s");
-    const int K = degree > 0 ? degree : 4;
+    const int K = degree > 0 ? degree : 5;   // the solve's default
     if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
     MgScal z{};
     HIPCHK(c, hipStreamSynchronize(c->stream));

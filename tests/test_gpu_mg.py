@@ -198,6 +198,10 @@ def test_mg_in_solve_kernel_timing():
         else:
             assert info["kernel_samples"] == 0 and info["kernel_us"] == [0.0] * 4
         out[timed] = (its, N(sol))
+    # the back-to-back timings of the same four kernels between solves
+    kt = s.mg_kernel_times(reps=3)
+    assert set(kt) == {"post_level0", "pre_level0", "fcg_a", "fcg_b"}
+    assert all(u > 0.0 for u in kt.values())
     s.close()
 
 

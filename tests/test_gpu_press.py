@@ -276,6 +276,9 @@ def test_mg_mode_reaches_the_residual_target(n, cyclic, islands):
     # (the smallest basin has no smoothing level: its pre/post intervals are empty)
     assert info_t["kernel_samples"] > 0 and all(u >= 0.0 for u in info_t["kernel_us"])
     assert info_t["kernel_us"][2] > 0.0 and info_t["kernel_us"][3] > 0.0
+    if n == 512:   # a smoothing level to time back to back
+        kt = s.mg_kernel_times(reps=3)
+        assert all(u > 0.0 for u in kt.values())
     s.close()
 
 
