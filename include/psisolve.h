@@ -51,7 +51,7 @@ typedef enum psi_mode {
                               V-cycle (psi_mg_setup first); PSI_CRIT_RELRES only.  poly_degree
                               = Chebyshev smoothing steps per level (<= 0: 5; 4 for the
                               surface pressure), poly_lower = the smoother's interval
-                              [2 poly_lower, 2] (<= 0: 1/20; 1/15 for the surface pressure) */
+                              [2 poly_lower, 2] (<= 0: 1/15) */
 } psi_mode;
 
 typedef enum psi_criterion {

@@ -1122,7 +1122,7 @@ int mg_capture(psi_ctx* c, MgState* m, int K, double ratio, bool timed, Iter ite
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
-    const MgCheb ch = mg_cheb(K, ratio);
+    const MgCheb ch = mg_cheb(K, ratio, m->cscale);
     HIPCHK(c, hipStreamBeginCapture(m->gs, hipStreamCaptureModeThreadLocal));
     MgEvents ev{m->kev};
     iter(m->gs, ch, timed ? &ev : nullptr);
@@ -1197,7 +1197,7 @@ double mg_time(MgState* m, int reps, Fn fn) {
 
 template <int K>
 void mg_time_levels(MgState* m, int reps, double* us) {
-    const MgCheb ch = mg_cheb(K, 15.0);
+    const MgCheb ch = mg_cheb(K, 15.0, m->cscale);
     const MgLev& v = m->lev[0];
     const MgScal* S = m->S_time;
     us[0] = mg_time(m, reps, [&] { mg_launch_post<K>(v, m->lev[1], ch, S, m->gs); });
@@ -1209,11 +1209,12 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
     if (!m) return fail(c, PSI_BAD_ARG, "multigrid mode: call psi_mg_setup first%s");
     if (o->criterion != PSI_CRIT_RELRES)
         return fail(c, PSI_BAD_ARG, "multigrid mode needs the relres criterion%s");
-    // defaults: 5 Chebyshev steps on [2/20, 2] (a sweep of 3-6 steps and
-    // interval 1/10-1/30 at 2048^2 put this 3% ahead of 4 steps on [2/15, 2])
+    // defaults: 5 Chebyshev steps on [2/15, 2] (with the coarse-correction
+    // weight 1.9, a sweep of 3-6 steps and intervals 1/10-1/30 at 2048^2 put
+    // this ahead)
     const int K = o->poly_degree > 0 ? o->poly_degree : 5;
     if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
-    const double lower = o->poly_lower > 0 ? o->poly_lower : 1.0 / 20.0;
+    const double lower = o->poly_lower > 0 ? o->poly_lower : 1.0 / 15.0;
     if (!(lower < 1.0)) return fail(c, PSI_BAD_ARG, "multigrid interval: lower < 1%s");
     const double ratio = 1.0 / lower;
     const double target = o->relres_target > 0 ? o->relres_target : 1e-10;

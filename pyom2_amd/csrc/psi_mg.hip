@@ -18,7 +18,7 @@
 // One V-cycle: on each level K steps of Chebyshev smoothing of D^-1 S on
 // [2/ratio, 2] from zero (Gershgorin bounds D^-1 S by 2 on every level),
 // the residual restricted to the next level, the coarse correction added
-// with weight MG_SCALE, and K Chebyshev steps again.  Vectors are FP32 in a
+// with a weight (MG_SCALE_*), and K Chebyshev steps again.  Vectors are FP32 in a
 // 1-cell-halo layout per level; a level costs two LDS-tiled kernels per
 // cycle (pre-smoothing with the restriction, prolongation with the
 // post-smoothing).
@@ -36,7 +36,10 @@ namespace psi {
 constexpr int MG_MAX_DENSE = 1024;   // active cells of the coarsest level at most
 constexpr int MG_MAX_LEVELS = 16;
 constexpr int MG_MAXK = 6;           // Chebyshev steps per smoothing at most
-constexpr float MG_SCALE = 1.7f;     // weight of the coarse-grid correction
+// weight of the coarse-grid correction, per operator (swept 1.5-2.4 at 2048^2:
+// 1.9 for the streamfunction with 5 smoothing steps, 1.7 for the surface
+// pressure with 4)
+constexpr double MG_SCALE_PSI = 1.9, MG_SCALE_PRESS = 1.7;
 constexpr int MG_NRED = 3;           // reduction slots of the outer passes
 
 // Level layout: cell (I, J) at (J + 1) * pitch + I + MG_XOFF, one halo row
@@ -60,6 +63,7 @@ struct MgCheb {
     float c0;               // 1/theta
     float c1[MG_MAXK];      // rho_k rho_{k-1}
     float c2[MG_MAXK];      // 2 rho_k / delta
+    float cscale;           // weight of the coarse-grid correction
 };
 
 // Device scalars of the outer iteration.
@@ -238,7 +242,7 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb c
     }
 }
 
-// Prolongation of the coarse correction (weight MG_SCALE, active cells) and
+// Prolongation of the coarse correction (weight ch.cscale, active cells) and
 // K Chebyshev steps of post-smoothing; the result goes to y.  y' (needed
 // with its neighbours once) and then d go through LDS, r and y stay in
 // registers.
@@ -268,7 +272,7 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev L
             float yv = 0.0f;
             if (q >= 0 && L.act[q]) {
                 if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
-                yv = L.x[q] + MG_SCALE * Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
+                yv = L.x[q] + ch.cscale * Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
             }
             sE[f] = q >= 0 ? L.E[q] : 0.0f;
             sN[f] = q >= 0 ? L.N[q] : 0.0f;
@@ -695,11 +699,14 @@ struct MgState {
     std::vector<void*> allocs;
     int64_t bytes = 0;
     std::vector<int> level_nx, level_ny, level_nact;
+    double cscale = MG_SCALE_PSI;    // coarse-correction weight of this operator
 };
 
-// Chebyshev scalars of K steps on [2/ratio, 2].
-inline MgCheb mg_cheb(int K, double ratio) {
+// Chebyshev scalars of K steps on [2/ratio, 2], and the coarse-correction
+// weight.
+inline MgCheb mg_cheb(int K, double ratio, double cscale) {
     MgCheb ch{};
+    ch.cscale = (float)cscale;
     const double hi = 2.0, lo = hi / ratio;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     double rho = 1.0 / sigma;

@@ -813,6 +813,7 @@ int sp_mg_setup(psi_press* c) {
     MgState* m = new MgState();
     c->mg = m;
     m->w_mean = L.nact ? wsum / L.nact : 1.0;
+    m->cscale = MG_SCALE_PRESS;
     std::vector<MgHostLev> H;
     H.push_back(std::move(L));
     return mg_build(c, H);
