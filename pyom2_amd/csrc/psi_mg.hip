@@ -36,10 +36,11 @@ namespace psi {
 constexpr int MG_MAX_DENSE = 1024;   // active cells of the coarsest level at most
 constexpr int MG_MAX_LEVELS = 16;
 constexpr int MG_MAXK = 6;           // Chebyshev steps per smoothing at most
-// weight of the coarse-grid correction, per operator (swept 1.5-2.4 at 2048^2:
-// 1.9 for the streamfunction with 5 smoothing steps, 1.7 for the surface
-// pressure with 4)
-constexpr double MG_SCALE_PSI = 1.9, MG_SCALE_PRESS = 1.7;
+// weight of the coarse-grid correction, per operator (swept 1.3-2.4 at 2048^2:
+// 1.9 for the streamfunction with 5 smoothing steps, 1.5 for the surface
+// pressure, whose free-surface term makes the operator better conditioned,
+// with 4)
+constexpr double MG_SCALE_PSI = 1.9, MG_SCALE_PRESS = 1.5;
 constexpr int MG_NRED = 3;           // reduction slots of the outer passes
 
 // Level layout: cell (I, J) at (J + 1) * pitch + I + MG_XOFF, one halo row
