@@ -23,9 +23,9 @@ writes the file a run with those modules off writes, and reads the records
 above from any file.  A 3-D field in this package's layout, C-ordered
 (nz, ny+4, nx+4), is byte for byte the Fortran a(-1:nx+2,-1:ny+2,1:nz).
 
-Records over 2^31-1 bytes (a 2048^2 x 40 field pair is 2.7 GB) follow
+Records over 2^31-9 bytes (a 2048^2 x 40 field pair is 2.7 GB) follow
 gfortran's subrecord convention: the record is split into subrecords of at
-most 2^31-1 bytes, each framed by markers holding its length, the leading
+most 2^31-9 bytes (gfortran's default -fmax-subrecord-length), each framed by markers holding its length, the leading
 marker negated when another subrecord follows, the trailing one negated when
 another precedes.  Files of that size were not written by the reference's
 writer here, so that part is checked only against this module's own reader."""
@@ -38,7 +38,15 @@ import numpy as np
 
 CORE_3D = ("rho", "Nsqr", "Hd", "int_drhodT", "int_drhodS", "dHd", "temp", "dtemp", "salt",
            "dsalt", "u", "du", "v", "dv", "w")
-MAX_SUB = 2 ** 31 - 1   # bytes per subrecord
+# bytes per subrecord: gfortran's default -fmax-subrecord-length, 2**31 - 9
+MAX_SUB = 2147483639
+
+
+def subrecord_sizes(n: int) -> list:
+    """Payload bytes of each subrecord a record of n bytes is split into."""
+    if n <= MAX_SUB:
+        return [n]
+    return [min(MAX_SUB, n - off) for off in range(0, n, MAX_SUB)]
 
 
 def restart_filename(pe: int = 0) -> str:
@@ -74,14 +82,14 @@ class RestartError(RuntimeError):
 # ---- records ---------------------------------------------------------------
 def _write_record(f, payload: bytes, endian: str):
     n = len(payload)
-    if n <= MAX_SUB:
+    sizes = subrecord_sizes(n)
+    if len(sizes) == 1:
         m = np.array([n], endian + "i4").tobytes()
         f.write(m + payload + m)
         return
     off = 0
     first = True
-    while off < n:
-        k = min(MAX_SUB, n - off)
+    for k in sizes:
         more = off + k < n
         lead = np.array([-k if more else k], endian + "i4").tobytes()
         trail = np.array([-k if not first else k], endian + "i4").tobytes()

@@ -37,6 +37,35 @@ def test_kernel_names_and_bytes(degree):
     # pass B: z (FP32), Az, p, q, r, x read (FP64), mask, p, q, r, x written, next input (FP32)
     assert bench.MG_KERNEL_BYTES["fcg_b"] == 4 + 8 + 4 * 8 + 1 + 4 * 8 + 4 == 81
     assert bench.MG_KERNEL_BYTES["fcg_a"] == 45
-    for k, v in bench.MG_TRAFFIC_2048.items():
-        # measured traffic within 10% of the algorithmic bytes at 2048^2
-        assert 1.0 <= v / (bench.MG_KERNEL_BYTES[k] * 2048 * 2048) <= 1.1
+
+
+def _profile_traffic(name):
+    """kernel -> traffic bytes from the HBM-traffic table of a committed
+    profile summary: FETCH_SIZE KiB x 2.0 + WRITE_SIZE KiB x 1.0 (the copy
+    calibration the summary states), and the traffic MB column it prints."""
+    rows = {}
+    text = open(os.path.join(ROOT, "profiles", name)).read()
+    section = text.split("## HBM traffic per dispatch", 1)[1].split("\n## ", 1)[0]
+    for line in section.splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) < 4 or not cells[0].startswith("`"):
+            continue
+        try:
+            fetch, write = (float(c.replace(",", "")) for c in cells[1:3])
+        except ValueError:
+            continue
+        rows[cells[0]] = ((2.0 * fetch + write) * 1024, float(cells[3].split()[0]) * 1e6)
+    return rows
+
+
+def test_mg_traffic_constants_come_from_the_committed_profile():
+    """The traffic the bench line reports for the multigrid kernels is the
+    one the committed rocprofv3 summary measured, and that summary's counters
+    add up to its own traffic column."""
+    rows = _profile_traffic("k_mg_spherical_2048.md")
+    names = {"fcg_b": "`k_mg_fcg_b`", "fcg_a": "`k_mg_fcg_a`",
+             "post_level0": "`k_mg_post<5, 32>`, 2048²", "pre_level0": "`k_mg_pre<5, 32>`, 2048²"}
+    for k, row in names.items():
+        counted, printed = rows[row]
+        assert abs(counted - printed) <= 0.002 * printed, (k, counted, printed)
+        assert bench.MG_TRAFFIC_2048[k] == pytest.approx(printed, rel=1e-6), k

@@ -98,7 +98,15 @@ def test_record_framing_and_subrecords(tmp_path, monkeypatch):
     raw = open(path, "rb").read()
     assert int(np.frombuffer(raw[:4], ">i4")[0]) == 16
     assert tuple(np.frombuffer(raw[4:20], ">i4")) == (8, 6, 2, st.itt)
+    # gfortran's default subrecord length (-fmax-subrecord-length): a 2048^2 x 40
+    # field pair (2.7 GB) is split at 2**31 - 9 bytes
+    assert R.MAX_SUB == 2 ** 31 - 9
+    n = 2 * 40 * 2052 * 2052 * 8
+    assert R.subrecord_sizes(n) == [2 ** 31 - 9, n - (2 ** 31 - 9)]
+    assert R.subrecord_sizes(2 ** 31 - 9) == [2 ** 31 - 9]
+    assert R.subrecord_sizes(2 ** 31 - 8) == [2 ** 31 - 9, 1]
     monkeypatch.setattr(R, "MAX_SUB", 1000)
+    assert R.subrecord_sizes(3840) == [1000, 1000, 1000, 840]
     p2 = str(tmp_path / "split.dta")
     R.write_restart(p2, st)
     raw2 = open(p2, "rb").read()
