@@ -342,30 +342,53 @@ def curl_forcing(b, fpx, fpy):
     return forc
 
 
+def island_rhs(b, init, fpx, fpy, dpsi_new):
+    """Right-hand side of the island system (solve_streamfct.f90:61-78):
+    line_forc - line_psi0 for every island, line_forc(k) the line integral of
+    (fpx, fpy) around island k, line_psi0(k) that of the barotropic velocity of
+    the new interior streamfunction tendency.  Entry 0 (island 1, the basin
+    rim, whose streamfunction is held fixed) is formed but not used."""
+    n = init.islands.nisle
+    lf = np.array([line_integral(b, init.islands, k, fpx, fpy) for k in range(n)])
+    u, v = psi_velocity(b, dpsi_new)
+    l0 = np.array([line_integral(b, init.islands, k, u, v) for k in range(n)])
+    return lf - l0
+
+
+def island_solve(init, rhs):
+    """DGESV of line_psin(2:nisle, 2:nisle) dpsin = rhs(2:nisle)
+    (solve_streamfct.f90:79-88); returns dpsin(2:nisle, tau)."""
+    x, info = dense_solve(init.line_psin[1:, 1:], rhs[1:])
+    if info != 0:
+        raise ValueError("island system singular")
+    return x
+
+
+def ab_psi(psi, dpsi, dpsin, psin, taum1, tau, taup1, dt, ab):
+    """Adams-Bashforth step of psi with the island terms (solve_streamfct.f90:
+    90-94), in the reference's order; psi[taup1] is overwritten."""
+    psi[taup1] = psi[tau] + dt * ((1.5 + ab) * dpsi[taup1] - (0.5 + ab) * dpsi[tau])
+    for k in range(1, dpsin.shape[0]):
+        psi[taup1] = psi[taup1] + dt * ((1.5 + ab) * dpsin[k, tau] -
+                                        (0.5 + ab) * dpsin[k, taum1]) * psin[k]
+
+
 def solve_streamfunction_2d(b, init, fpx, fpy, psi, dpsi, dpsin, taum1, tau, taup1, dt, ab,
-                            eps=1e-12, max_iter=3000):
+                            eps=1e-12, max_iter=3000, congrad_fn=None):
     """solve_streamfunction's 2-D body (solve_streamfct.f90:49-94): curl, warm
     start, congrad, island system, Adams-Bashforth step of psi.  psi, dpsi
     (3, ny+4, nx+4) and dpsin (nisle, 3) are updated in place.  Returns the
-    congrad result."""
+    congrad result.  congrad_fn (default: the serial congrad) may be
+    congrad_threads for grids where the serial oracle is too slow."""
     nx, ny = b.nx, b.ny
     forc = curl_forcing(b, fpx, fpy)
     dpsi[taup1] = 2 * dpsi[tau] - dpsi[taum1]
-    r = congrad(nx, ny, b.cyclic_x, init.cf, init.Z, forc, dpsi[taup1], eps, max_iter)
+    r = (congrad_fn or congrad)(nx, ny, b.cyclic_x, init.cf, init.Z, forc, dpsi[taup1], eps,
+                                max_iter)
     dpsi[taup1] = setcyclic(nx, ny, b.cyclic_x, r.sol)
-    n = init.islands.nisle
-    if n > 1:
-        lf = np.array([line_integral(b, init.islands, k, fpx, fpy) for k in range(n)])
-        u, v = psi_velocity(b, dpsi[taup1])
-        l0 = np.array([line_integral(b, init.islands, k, u, v) for k in range(n)])
-        x, info = dense_solve(init.line_psin[1:, 1:], (lf - l0)[1:])
-        if info != 0:
-            raise ValueError("island system singular")
-        dpsin[1:, tau] = x
-    psi[taup1] = psi[tau] + dt * ((1.5 + ab) * dpsi[taup1] - (0.5 + ab) * dpsi[tau])
-    for k in range(1, n):
-        psi[taup1] = psi[taup1] + dt * ((1.5 + ab) * dpsin[k, tau] -
-                                        (0.5 + ab) * dpsin[k, taum1]) * init.psin[k]
+    if init.islands.nisle > 1:
+        dpsin[1:, tau] = island_solve(init, island_rhs(b, init, fpx, fpy, dpsi[taup1]))
+    ab_psi(psi, dpsi, dpsin, init.psin, taum1, tau, taup1, dt, ab)
     return r
 
 
