@@ -36,6 +36,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# how oracle/Makefile builds the C restatement the CPU baseline times
+ORACLE_FLAGS = "gcc -O3 -march=x86-64-v3 -ffp-contract=off"
 # algorithmic bytes per point per fused PCG iteration: x, r (as u = Z r) and p
 # read and written (6 words), hur/hvr-derived coefficients read (2 words),
 # a 1-byte Dirichlet/land mask
@@ -92,6 +94,9 @@ def parse():
                          "kernel of known traffic that calibrates HBM byte counters")
     ap.add_argument("--relres", type=float, default=1e-10)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-oracle", action="store_true",
+                    help="solve workload: no oracle solve of the last timed forcing (no "
+                         "psi_err / oracle_relres per path)")
     ap.add_argument("--cpu-iters", type=int, default=100,
                     help="CPU baseline: iterations per oracle run (repeated until the budget)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
@@ -180,6 +185,7 @@ def main():
     log(f"[bench] {nsteps} forcings resident on the GPU")
     modes = args.modes.split(",")
     results = {}
+    last_sol = {}
     for mode in modes:
         # the learned path falls back to the fastest classical preconditioner
         # among the paths measured here
@@ -229,6 +235,9 @@ def main():
             taum1, tau, taup1 = tau, taup1, taum1
         torch.cuda.synchronize()
         total = time.perf_counter() - t0
+        # the last timed solve's solution (level tau after the rotation), read
+        # back after the timed region for the oracle comparison
+        last_sol[mode] = dpsi[tau].cpu().numpy()
         its = np.array([p["its"] for p in per])
         sweep_ms = sum(p["sweep_ms"] for p in per)
         launches = sum(p["launches"] for p in per)
@@ -255,7 +264,8 @@ def main():
                           for k, v in kt.items()}
             avg_launch_us = kt[top]
             achieved = MG_KERNEL_BYTES[top] * npts / (avg_launch_us * 1e-6) / 1e9
-        results[mode] = dict(ms_per_solve=1e3 * total / args.steps, iterations_mean=float(its.mean()),
+        results[mode] = dict(ms_per_solve=1e3 * total / args.steps,
+                             solves_per_s=args.steps / total, iterations_mean=float(its.mean()),
                              relres_max=max(p["relres"] for p in per),
                              outer_mean=float(np.mean([p["outer"] for p in per])),
                              sweep_avg_launch_us=avg_launch_us, sweep_launches=launches,
@@ -281,11 +291,30 @@ def main():
                                                     if m in results), None)
     rk = results.get(kpath, {}) if kpath else {}
     cpu = cpu_all = None
+    oracle_ref = None
+    if not args.skip_oracle:
+        # every path's last solution against one oracle solve of the same
+        # (last timed) forcing: the reference's CG and stopping rule
+        # (congr_epsilon 1e-12) on all host threads, from a zero first guess
+        oracle_ref = oracle_compare(b, isl, forcings[-1].cpu().numpy(), last_sol)
+        for m, e in oracle_ref.pop("paths").items():
+            results[m].update(e)
     if not args.skip_cpu:
         its_ref = results.get("plain", r)["iterations_mean"]
         cpu = cpu_baseline(b, isl, args.cpu_iters, its_ref, budget_s=args.cpu_seconds)
-        cpu_all = cpu_baseline(b, isl, args.cpu_iters, its_ref, threaded=True,
-                               budget_s=args.cpu_seconds / 2)
+        if oracle_ref is not None:
+            cpu_all = {"value": round(oracle_ref["ms"] * its_ref / oracle_ref["iterations"], 1),
+                       "unit": "ms", "cores": oracle_ref["threads"], "kind": "port",
+                       "sample": f"one whole solve of the last timed forcing from a zero first "
+                                 f"guess at the reference's stopping rule: "
+                                 f"{oracle_ref['iterations']} iterations in "
+                                 f"{oracle_ref['ms'] / 1e3:.1f} s (OpenMP, "
+                                 f"{oracle_ref['threads']} threads, C restatement "
+                                 f"{ORACLE_FLAGS}), scaled to {its_ref:.0f} iterations "
+                                 f"per solve"}
+        else:
+            cpu_all = cpu_baseline(b, isl, args.cpu_iters, its_ref, threaded=True,
+                                   budget_s=args.cpu_seconds / 2)
     line = {
         "metric": f"ms per {args.n}\u00b2 psi solve to 1e-10 rel. FP64 residual: learned vs "
                   f"Jacobi PCG",
@@ -323,13 +352,29 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }
+    line["solves_per_s"] = round(r["solves_per_s"], 2)
+    if oracle_ref is not None:
+        # the headline path's last solution against the oracle's solve of the
+        # same forcing, and its FP64 true residual recomputed by the oracle
+        line["psi_err"] = float(f"{r['psi_err']:.3g}")
+        line["oracle_relres"] = float(f"{r['oracle_relres']:.3g}")
+        line["oracle"] = {"iterations": oracle_ref["iterations"],
+                          "converged": oracle_ref["converged"],
+                          "relres": float(f"{oracle_ref['relres']:.3g}"),
+                          "threads": oracle_ref["threads"],
+                          "s": round(oracle_ref["ms"] / 1e3, 2),
+                          "forcing": "the last timed forcing, zero first guess, "
+                                     "congr_epsilon 1e-12 (the reference's stopping rule)"}
     if "plain" in results and head != "plain":
         line["speedup_vs_plain_fp64"] = round(results["plain"]["ms_per_solve"] / r["ms_per_solve"], 3)
-    if head == "learned" and "mixed" in results:
-        line["learned_vs_jacobi_mixed"] = round(results["mixed"]["ms_per_solve"] /
-                                                r["ms_per_solve"], 3)
     if head == "learned":
-        line["config"]["path"] = f"learned policy (chose {r.get('choice')})"
+        ch = r.get("choice")
+        if ch == "learned":
+            line["config"]["path"] = "learned (flexible CG with the trained network)"
+        else:
+            # the timed solves ran the fallback path: name it
+            line["config"]["path"] = (f"{ch.split(': ')[-1]} (the learned policy's fallback: "
+                                      f"the network's flexible CG lost the probe)")
     print(json.dumps(finite(line)), flush=True)
 
 
@@ -359,9 +404,34 @@ def cpu_baseline(b, isl, iters, plain_its, threaded=False, budget_s=10.0):
     how = f"OpenMP, {cores} threads" if threaded else "1 thread"
     return {"value": round(per_it * plain_its * 1e3, 1), "unit": "ms", "cores": cores,
             "kind": "port",
-            "sample": f"{n} PCG iterations of the {b.nx}x{b.ny} solve in {dt:.1f} s "
-                      f"({per_it * 1e3:.2f} ms/iteration, C restatement -O2, {how}), "
-                      f"scaled to {plain_its:.0f} iterations per solve"}
+            "sample": f"{n} PCG iterations of the {b.nx}x{b.ny} solve from a zero first guess "
+                      f"in {dt:.1f} s ({per_it * 1e3:.2f} ms/iteration, C restatement "
+                      f"{ORACLE_FLAGS}, {how}) = {100.0 * n / plain_its:.1f}% of the "
+                      f"{plain_its:.0f} iterations a solve takes, scaled to a whole solve"}
+
+
+def oracle_compare(b, isl, forc, sols):
+    """One oracle solve of `forc` (the reference's CG and stopping rule on all
+    host threads, zero first guess) and, for every path's solution of the
+    same forcing, max|psi - psi_ref| / max|psi_ref| and the FP64 true
+    relative residual recomputed by the oracle."""
+    from oracle import oracle as O
+    oisl = O.Islands(isl.nisle, isl.offsets, isl.bi, isl.bj, isl.di, isl.dj, None)
+    cf = O.make_coeff(b)
+    Z = O.make_inv(b.nx, b.ny, cf, oisl)
+    t = time.perf_counter()
+    ref = O.congrad_threads(b.nx, b.ny, b.cyclic_x, cf, Z, forc, np.zeros(b.shape), 1e-12,
+                            200000)
+    ms = 1e3 * (time.perf_counter() - t)
+    scale = np.abs(ref.sol).max()
+    out = {"ms": ms, "iterations": ref.iterations, "converged": ref.converged,
+           "threads": O.threads(), "relres": O.true_relres(b, cf, Z, forc, ref.sol), "paths": {}}
+    for m, x in sols.items():
+        out["paths"][m] = {"psi_err": float(np.abs(x - ref.sol).max() / scale),
+                           "oracle_relres": O.true_relres(b, cf, Z, forc, x)}
+    log(f"[bench] oracle: {ref.iterations} iterations in {ms / 1e3:.1f}s on {out['threads']} "
+        f"threads, relres {out['relres']:.2e}; {out['paths']}")
+    return out
 
 
 def timed_oracle(b, cf, Z, iters, budget_s, threaded):

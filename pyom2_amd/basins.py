@@ -37,6 +37,7 @@ import dataclasses
 import math
 
 import numpy as np
+import scipy.ndimage
 
 PI = 3.14159265358979323846264338327950588
 RADIUS = 6370.0e3
@@ -302,11 +303,27 @@ def beta_plane_basin(seed: int = 0, nx: int = 128, ny: int = 128, dx: float = 20
                      1e-10, cyclic_x, placed)
 
 
+def _land_below(ht: np.ndarray, h_min: float) -> np.ndarray:
+    """Ocean map (interior, bool) of a closed basin: a 1-cell wall rim plus
+    every cell shallower than h_min that is 8-connected to the rim.  Shallow
+    patches not joined to the rim would each become an island with its own
+    line-integral constraint; they are filled to h_min instead (ht is raised
+    in place), so a basin's islands are only the ones placed on purpose."""
+    land = ht < h_min
+    land[0, :] = land[-1, :] = land[:, 0] = land[:, -1] = True
+    lab, _ = scipy.ndimage.label(land, structure=np.ones((3, 3), bool))
+    coast = lab == lab[0, 0]
+    np.maximum(ht, np.where(coast, ht, h_min), out=ht)
+    return ~coast
+
+
 def spherical_basin(seed: int = 0, n: int = 2048, ny: int | None = None,
                     n_islands: int = 2, n_features: int = 64, n_modes: int = 16,
                     kmax: int = 32, dlon: float = 0.05) -> Basin:
     """Spherical basin on a dlon-degree grid centred on the equator: 5000 m
-    base minus n_features Gaussians, land = rim + seeded elliptical islands."""
+    base minus n_features Gaussians, land where H < 100 m joined to the wall
+    rim (isolated shallows are filled to 100 m) plus n_islands seeded
+    elliptical islands."""
     nx = n
     ny = n if ny is None else ny
     rng = np.random.default_rng(seed)
@@ -319,12 +336,7 @@ def spherical_basin(seed: int = 0, n: int = 2048, ny: int | None = None,
     scale = n / 2048.0
     _gaussians(rng, ht, xt, yt, n_features, 1000.0, 4500.0,
                max(2.0, 20 * scale) * dxm, max(4.0, 120 * scale) * dxm)
-    ht = np.maximum(ht, 150.0)
-    ocean = ht >= 100.0
-    ocean[0, :] = False
-    ocean[-1, :] = False
-    ocean[:, 0] = False
-    ocean[:, -1] = False
+    ocean = _land_below(ht, 100.0)
     placed = 0
     if n_islands:
         a_lo, a_hi = 20.0 * scale, 60.0 * scale

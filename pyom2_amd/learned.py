@@ -521,10 +521,12 @@ class LearnedSolver:
             j_rate = self._ms_per_decade(j_ms, f_rel0, jinfo["relres_true"])
             f_rate = self._ms_per_decade(f_ms, f_rel0, f_rel)
             self.choice = "learned" if f_rate < j_rate else "jacobi"
-            self.last_probe = {"jacobi_ms_per_decade": j_rate, "learned_ms_per_decade": f_rate,
-                               "jacobi_iterations": j_its, "learned_iterations": f_its,
-                               "relres_start": f_rel0, "jacobi_relres": jinfo["relres_true"],
-                               "learned_relres": f_rel}
+            fb = jac.mode
+            self.last_probe = {f"{fb}_ms_per_decade": j_rate, "learned_ms_per_decade": f_rate,
+                               f"{fb}_iterations": j_its, "learned_iterations": f_its,
+                               "relres_start": f_rel0, f"{fb}_relres": jinfo["relres_true"],
+                               "learned_relres": f_rel, f"{fb}_ms": j_ms, "learned_ms": f_ms,
+                               "fallback_mode": fb}
             info["probe"] = self.last_probe
             if self.choice == "jacobi":
                 sol.copy_(x_jac)           # keep the Jacobi probe's progress

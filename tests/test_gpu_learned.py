@@ -288,5 +288,5 @@ def test_training_reduces_loss_and_fcg_converges():
     assert conv2 and info2["choice"] in ("learned", "jacobi")
     pr = info2["probe"]
     assert info2["choice"] == ("learned" if pr["learned_ms_per_decade"] <
-                               pr["jacobi_ms_per_decade"] else "jacobi")
+                               pr[f"{pr['fallback_mode']}_ms_per_decade"] else "jacobi")
     assert O.true_relres(b, cf, Z, b.forc, sol2.cpu().numpy()) <= 1.01e-10
