@@ -169,6 +169,27 @@ int64_t psi_ctx_device_bytes(const psi_ctx *ctx);
 int psi_congrad(psi_ctx *ctx, const psi_opts *opts, const double *forc, double *sol,
                 psi_info *info);
 
+/* ---- the plain solve without a context and without allocating -----------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (a serving loop, a harness that forbids hipMalloc, a program that
+ * captures HIP graphs around its other work).  psi_congrad_noalloc runs
+ * psi_congrad's plain FP64 Jacobi-PCG (mode PSI_MODE_PLAIN, either
+ * criterion; the same kernels, so the same iterations and solution) on the
+ * basin described by grid / fields / islands, without a context: it calls no
+ * hipMalloc, hipFree, hipHostMalloc or event creation.  Its operator, active
+ * mask, CG vectors, partials and solver state live in one static device
+ * array of 19.5 MiB per GPU (held until the process ends); the first call on
+ * a device looks up that array's address there.  Limits: (nx+4)(ny+4) <=
+ * 516^2 (a 512 x 512 interior) with nx, ny <= 2048, and at most 2^18 island
+ * perimeter points; larger inputs return PSI_BAD_ARG with the reason in
+ * psi_ctx_last_error(NULL).  fields->maskU/maskV are not read.  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (two threads,
+ * two streams) race.  The call returns after the solve has finished on
+ * `stream`; info->elapsed_ms is host wall time, info->sweep_ms is -1. */
+int psi_congrad_noalloc(const psi_grid *grid, const psi_fields *fields,
+                        const psi_islands *islands, const psi_opts *opts, const double *forc,
+                        double *sol, psi_info *info, void *stream);
+
 /* ---- building blocks, exposed for the drop-in and for parity tests ----- */
 /* apply_op (solve_pressure.f90:254-274) with the cached operator: res = A p
  * on the interior, res = 0 on the halo. */

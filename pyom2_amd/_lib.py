@@ -128,6 +128,9 @@ SIGNATURES = [
     ("psi_ctx_device_bytes", C.c_int64, [C.c_void_p]),
     ("psi_congrad", C.c_int, [C.c_void_p, C.POINTER(PsiOpts), C.c_void_p, C.c_void_p,
                               C.POINTER(PsiInfo)]),
+    ("psi_congrad_noalloc", C.c_int, [C.POINTER(PsiGrid), C.POINTER(PsiFields),
+                                      C.POINTER(PsiIslands), C.POINTER(PsiOpts), C.c_void_p,
+                                      C.c_void_p, C.POINTER(PsiInfo), C.c_void_p]),
     ("psi_apply_op", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("psi_make_coeff", C.c_int, [C.c_void_p, C.c_void_p]),
     ("psi_make_inv", C.c_int, [C.c_void_p, C.c_void_p]),

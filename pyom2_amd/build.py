@@ -15,7 +15,7 @@ ARCH = os.environ.get("PSI_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["psi_capi.hip", "psi_nn.hip", "islands.cpp"]
 DEPS = ["psi_capi.hip", "psi_kernels.hip", "psi_batch.hip", "psi_levels.hip", "psi_poly.hip",
-        "psi_mg.hip", "psi_press.hip", "psi_nonhydro.hip",
+        "psi_mg.hip", "psi_press.hip", "psi_nonhydro.hip", "psi_noalloc.hip",
         "psi_device.h", "psi_nn.hip",
         "islands.cpp"]
 

@@ -481,7 +481,10 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
     int slot = 0, queue[2], qn = 0;
     long issued_at[2] = {0, 0};
     const long L0 = Ln.L;
-    HIPCHK(c, hipEventRecord(c->ts0, c->stream));
+    // a context without events (psi_congrad_noalloc) polls with stream
+    // synchronisations and reports no sweep time
+    const bool events = c->ev[0] != nullptr;
+    if (events) HIPCHK(c, hipEventRecord(c->ts0, c->stream));
     auto issue = [&]() -> int {
         for (int k = 0; k < batch && Ln.L - L0 < max_launch; ++k, ++Ln.L) {
             if (P.poly > 0)
@@ -496,12 +499,12 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
         }
         int r = check_launch(c);
         if (r) return r;
-        HIPCHK(c, hipEventRecord(c->ts1[slot], c->stream));
+        if (events) HIPCHK(c, hipEventRecord(c->ts1[slot], c->stream));
         issued_at[slot] = Ln.L - L0;
         // the latest state is the one the last launch wrote
         HIPCHK(c, hipMemcpyAsync(&c->h_st[slot], c->st + ((Ln.L - 1) & 1), sizeof(CgState),
                                  hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+        if (events) HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
         queue[qn++] = slot;
         slot ^= 1;
         return PSI_OK;
@@ -515,16 +518,17 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
         const int s = queue[0];
         queue[0] = queue[1];
         --qn;
-        HIPCHK(c, hipEventSynchronize(c->ev[s]));
+        if (events) HIPCHK(c, hipEventSynchronize(c->ev[s]));
+        else HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->h_st[s].done) {
             out->fin = c->h_st[s];
             float ms = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&ms, c->ts0, c->ts1[s]));
+            if (events) HIPCHK(c, hipEventElapsedTime(&ms, c->ts0, c->ts1[s]));
             out->sweep_ms += ms;
             out->launches += issued_at[s];
             // drain the batch queued behind this poll so the next leg starts clean
             if (qn) {
-                HIPCHK(c, hipEventSynchronize(c->ev[queue[0]]));
+                if (events) HIPCHK(c, hipEventSynchronize(c->ev[queue[0]]));
                 out->launches += issued_at[queue[0]] - issued_at[s];
             }
             return PSI_OK;
@@ -2176,3 +2180,6 @@ int psi_fcg_scalars(psi_ctx* c, double* out10) {
 // (psi_nh_*) likewise.
 #include "psi_press.hip"
 #include "psi_nonhydro.hip"
+
+// The allocation-free plain solve (psi_congrad_noalloc) and its static scratch.
+#include "psi_noalloc.hip"
