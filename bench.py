@@ -104,6 +104,8 @@ def parse():
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--poly-degree", type=int, default=0, help="0: library default")
     ap.add_argument("--poly-lower", type=float, default=0.0)
+    ap.add_argument("--mg-coarse-degree", type=int, default=0,
+                    help="mg path: smoothing steps on the coarse levels (0: library default)")
     ap.add_argument("--inner-reduction", type=float, default=0.0,
                     help="mixed path: residual reduction per FP32 leg (0: library default)")
     ap.add_argument("--verbose", action="store_true")
@@ -194,7 +196,8 @@ def main():
                             relres_target=args.relres, max_iterations=200000,
                             check_every=args.check_every, inner_reduction=args.inner_reduction,
                             poly_degree=args.poly_degree, poly_lower=args.poly_lower,
-                            time_kernels=(mode == "mg"))
+                            time_kernels=(mode == "mg"),
+                            mg_coarse_degree=args.mg_coarse_degree)
         ls = None
         if mode == "learned":
             from pyom2_amd import learned

@@ -105,7 +105,11 @@ typedef struct psi_opts {
     int32_t time_kernels;      /* PSI_MODE_MG: nonzero records HIP events around the
                                   four heaviest kernels of every outer iteration and
                                   reports their mean durations in psi_info.kernel_us */
-    int32_t reserved[2];
+    int32_t mg_coarse_degree;  /* PSI_MODE_MG: Chebyshev smoothing steps on the coarse
+                                  levels (at most 256 tiles of 64 x 16 cells, whose
+                                  kernels are bound by launch latency); 1..10, <= 0
+                                  picks the library default */
+    int32_t reserved;
 } psi_opts;
 
 typedef struct psi_info {

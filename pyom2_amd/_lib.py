@@ -56,7 +56,7 @@ class PsiOpts(C.Structure):
                 ("check_every", C.c_int32), ("inner_reduction", C.c_double),
                 ("max_outer", C.c_int32), ("poly_degree", C.c_int32), ("poly_lower", C.c_double),
                 ("leg_max_iterations", C.c_int32), ("time_kernels", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("mg_coarse_degree", C.c_int32), ("reserved", C.c_int32)]
 
 
 class PsiInfo(C.Structure):

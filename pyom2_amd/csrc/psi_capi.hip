@@ -1051,16 +1051,58 @@ void mg_launch_post(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgS
     }
 }
 
+// A coarse level's kernel at degree Kc (64 x 16 tiles).
+template <int Kc>
+void mg_launch_coarse(bool pre, const MgLev& v, const MgLev& vc, const MgCheb& ch,
+                      const MgScal* S, hipStream_t s) {
+    using F = MgFrame<Kc, MG_TY_SMALL>;
+    if (pre)
+        hipLaunchKernelGGL((k_mg_pre<Kc, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT), 0,
+                           s, v, ch, vc, S);
+    else
+        hipLaunchKernelGGL((k_mg_post<Kc, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT),
+                           0, s, v, vc, ch, S);
+}
+
+void mg_launch_coarse(int Kc, bool pre, const MgLev& v, const MgLev& vc, const MgCheb& ch,
+                      const MgScal* S, hipStream_t s) {
+    switch (Kc) {
+        case 1: mg_launch_coarse<1>(pre, v, vc, ch, S, s); break;
+        case 2: mg_launch_coarse<2>(pre, v, vc, ch, S, s); break;
+        case 3: mg_launch_coarse<3>(pre, v, vc, ch, S, s); break;
+        case 4: mg_launch_coarse<4>(pre, v, vc, ch, S, s); break;
+        case 5: mg_launch_coarse<5>(pre, v, vc, ch, S, s); break;
+        case 6: mg_launch_coarse<6>(pre, v, vc, ch, S, s); break;
+        case 7: mg_launch_coarse<7>(pre, v, vc, ch, S, s); break;
+        case 8: mg_launch_coarse<8>(pre, v, vc, ch, S, s); break;
+        case 9: mg_launch_coarse<9>(pre, v, vc, ch, S, s); break;
+        default: mg_launch_coarse<10>(pre, v, vc, ch, S, s); break;
+    }
+}
+
+// Coarse smoothing of a V-cycle: degree Kc with scalars chc on the coarse
+// levels (mg_coarse_level); Kc <= 0 keeps the finest levels' degree.
+struct MgCoarse {
+    int Kc = 0;
+    MgCheb chc{};
+};
+
 template <int K>
-void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, MgEvents* ev) {
+void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, const MgCoarse& co,
+                     MgEvents* ev) {
     const int L = m->L;
     if (ev && L == 0) {   // no smoothing level: empty intervals
         ev->rec(0, s);
         ev->rec(1, s);
     }
+    auto coarse = [&](int l) { return co.Kc > 0 && l > 0 && mg_coarse_level(m->lev[l]); };
     for (int l = 0; l < L; ++l) {
         if (ev && l == 0) ev->rec(0, s);
-        mg_launch_pre<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
+        if (coarse(l))
+            mg_launch_coarse(co.Kc, true, m->lev[l], m->lev[l + 1], co.chc,
+                             (const MgScal*)m->S, s);
+        else
+            mg_launch_pre<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
         if (ev && l == 0) ev->rec(1, s);
     }
     if (m->ncoarse > 0)
@@ -1070,28 +1112,34 @@ void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, MgEvents* ev) 
     if (ev && L == 0) ev->rec(2, s);
     for (int l = L - 1; l >= 0; --l) {
         if (ev && l == 0) ev->rec(2, s);
-        mg_launch_post<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
+        if (coarse(l))
+            mg_launch_coarse(co.Kc, false, m->lev[l], m->lev[l + 1], co.chc,
+                             (const MgScal*)m->S, s);
+        else
+            mg_launch_post<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
     }
     if (ev) ev->rec(3, s);
 }
 
 // ev: null, or the 7 events of MgState::kev (the first 4 are recorded here)
-void mg_vcycle(MgState* m, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev = nullptr) {
+void mg_vcycle(MgState* m, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev = nullptr,
+               const MgCoarse& co = MgCoarse{}) {
     switch (K) {
-        case 1: mg_vcycle_fused<1>(m, s, ch, ev); break;
-        case 2: mg_vcycle_fused<2>(m, s, ch, ev); break;
-        case 3: mg_vcycle_fused<3>(m, s, ch, ev); break;
-        case 4: mg_vcycle_fused<4>(m, s, ch, ev); break;
-        case 5: mg_vcycle_fused<5>(m, s, ch, ev); break;
-        default: mg_vcycle_fused<6>(m, s, ch, ev); break;
+        case 1: mg_vcycle_fused<1>(m, s, ch, co, ev); break;
+        case 2: mg_vcycle_fused<2>(m, s, ch, co, ev); break;
+        case 3: mg_vcycle_fused<3>(m, s, ch, co, ev); break;
+        case 4: mg_vcycle_fused<4>(m, s, ch, co, ev); break;
+        case 5: mg_vcycle_fused<5>(m, s, ch, co, ev); break;
+        default: mg_vcycle_fused<6>(m, s, ch, co, ev); break;
     }
 }
 
 // One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold;
 // with ev, the kernel-timing events of MgState::kev recorded around them.
-void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev = nullptr) {
+void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev = nullptr,
+                  const MgCoarse& co = MgCoarse{}) {
     MgState* m = c->mg;
-    mg_vcycle(m, s, K, ch, ev);
+    mg_vcycle(m, s, K, ch, ev, co);
     const Geom& g = c->g;
     hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
                        (const double*)c->E, (const double*)c->N, (const double*)c->rx,
@@ -1116,12 +1164,13 @@ void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* 
 // Capture one outer iteration, iter(stream, cheb, events), into the graph
 // (timed: the copy with the kernel-timing events) unless it is current.
 template <class Iter>
-int mg_capture(psi_ctx* c, MgState* m, int K, double ratio, bool timed, Iter iter) {
+int mg_capture(psi_ctx* c, MgState* m, int K, double ratio, bool timed, Iter iter, int Kc = 0) {
     hipGraph_t& graph = timed ? m->graph_t : m->graph;
     hipGraphExec_t& exec = timed ? m->exec_t : m->exec;
     int& gK = timed ? m->graph_t_K : m->graph_K;
     double& gratio = timed ? m->graph_t_ratio : m->graph_ratio;
-    if (exec && gK == K && gratio == ratio) return PSI_OK;
+    int& gKc = timed ? m->graph_t_Kc : m->graph_Kc;
+    if (exec && gK == K && gratio == ratio && gKc == Kc) return PSI_OK;
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
@@ -1137,14 +1186,19 @@ int mg_capture(psi_ctx* c, MgState* m, int K, double ratio, bool timed, Iter ite
     HIPCHK(c, hipGraphInstantiate(&exec, cap, nullptr, nullptr, 0));
     gK = K;
     gratio = ratio;
+    gKc = Kc;
     return PSI_OK;
 }
 
-int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false) {
-    return mg_capture(c, c->mg, K, ratio, timed, [&](hipStream_t s, const MgCheb& ch,
-                                                     MgEvents* ev) {
-        mg_iteration(c, s, K, ch, ev);
-    });
+// Kc: the coarse levels' smoothing degree (0: K)
+int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false, int Kc = 0) {
+    MgCoarse co;
+    co.Kc = Kc;
+    if (Kc > 0) co.chc = mg_cheb(Kc, ratio, c->mg->cscale);
+    return mg_capture(
+        c, c->mg, K, ratio, timed,
+        [&](hipStream_t s, const MgCheb& ch, MgEvents* ev) { mg_iteration(c, s, K, ch, ev, co); },
+        Kc);
 }
 
 // Replay the outer-iteration graph on m->gs from iteration n_start (|r|^2 =
@@ -1218,6 +1272,8 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
     // this ahead)
     const int K = o->poly_degree > 0 ? o->poly_degree : 5;
     if (K > MG_MAXK) return fail(c, PSI_BAD_ARG, "multigrid smoothing degree 1..6%s");
+    const int Kc = o->mg_coarse_degree > 0 ? o->mg_coarse_degree : PSI_MG_COARSE_DEGREE;
+    if (Kc > MG_MAXKC) return fail(c, PSI_BAD_ARG, "multigrid coarse smoothing degree 1..10%s");
     const double lower = o->poly_lower > 0 ? o->poly_lower : 1.0 / 15.0;
     if (!(lower < 1.0)) return fail(c, PSI_BAD_ARG, "multigrid interval: lower < 1%s");
     const double ratio = 1.0 / lower;
@@ -1228,8 +1284,8 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
     // with time_kernels the last replay of every poll batch runs the timed
     // graph (its event nodes cost about 3 us each)
     const bool timed = o->time_kernels != 0;
-    int rc = mg_graph(c, K, ratio);
-    if (rc == PSI_OK && timed) rc = mg_graph(c, K, ratio, true);
+    int rc = mg_graph(c, K, ratio, false, Kc == K ? 0 : Kc);
+    if (rc == PSI_OK && timed) rc = mg_graph(c, K, ratio, true, Kc == K ? 0 : Kc);
     if (rc) return rc;
     const Geom& g = c->g;
     HIPCHK(c, hipMemcpyAsync(m->x, sol, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));

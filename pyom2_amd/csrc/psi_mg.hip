@@ -35,12 +35,22 @@ namespace psi {
 
 constexpr int MG_MAX_DENSE = 1024;   // active cells of the coarsest level at most
 constexpr int MG_MAX_LEVELS = 16;
-constexpr int MG_MAXK = 6;           // Chebyshev steps per smoothing at most
+constexpr int MG_MAXK = 6;           // Chebyshev steps per smoothing at most (finest levels)
+constexpr int MG_MAXKC = 10;         // ... on the coarse levels (psi_opts.mg_coarse_degree)
+#ifndef PSI_MG_COARSE_DEGREE
+#define PSI_MG_COARSE_DEGREE 5       // default degree of the coarse levels
+#endif
+// A level is coarse when it has at most this many 64 x 16 tiles (one per CU
+// or fewer): its kernels are bound by launch latency, not by their work.
+constexpr long MG_COARSE_TILES = 256;
 // weight of the coarse-grid correction, per operator (swept 1.3-2.4 at 2048^2:
 // 1.9 for the streamfunction with 5 smoothing steps, 1.5 for the surface
 // pressure, whose free-surface term makes the operator better conditioned,
 // with 4)
-constexpr double MG_SCALE_PSI = 1.9, MG_SCALE_PRESS = 1.5;
+#ifndef PSI_MG_SCALE_PSI
+#define PSI_MG_SCALE_PSI 1.9
+#endif
+constexpr double MG_SCALE_PSI = PSI_MG_SCALE_PSI, MG_SCALE_PRESS = 1.5;
 constexpr int MG_NRED = 3;           // reduction slots of the outer passes
 
 // Level layout: cell (I, J) at (J + 1) * pitch + I + MG_XOFF, one halo row
@@ -62,8 +72,8 @@ struct MgLev {
 
 struct MgCheb {
     float c0;               // 1/theta
-    float c1[MG_MAXK];      // rho_k rho_{k-1}
-    float c2[MG_MAXK];      // 2 rho_k / delta
+    float c1[MG_MAXKC];     // rho_k rho_{k-1}
+    float c2[MG_MAXKC];     // 2 rho_k / delta
     float cscale;           // weight of the coarse-grid correction
 };
 
@@ -128,6 +138,11 @@ struct MgFrame {
 inline int mg_tile_y(const MgLev& v) {
     const long t = (long)((v.nx + MT_X - 1) / MT_X) * ((v.ny + MG_TY_LARGE - 1) / MG_TY_LARGE);
     return t >= 512 ? MG_TY_LARGE : MG_TY_SMALL;
+}
+
+inline bool mg_coarse_level(const MgLev& v) {
+    return (long)((v.nx + MT_X - 1) / MT_X) * ((v.ny + MG_TY_SMALL - 1) / MG_TY_SMALL) <=
+           MG_COARSE_TILES;
 }
 
 inline unsigned mg_tiles(const MgLev& v, int ty) {
@@ -687,7 +702,7 @@ struct MgState {
     hipEvent_t e_in = nullptr, e_out = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int graph_K = 0;
+    int graph_K = 0, graph_Kc = 0;
     double graph_ratio = 0.0;
     // the same iteration with HIP events around its four heaviest kernels
     // (psi_opts.time_kernels): before/after pre level 0, before/after post
@@ -695,7 +710,7 @@ struct MgState {
     hipEvent_t kev[7] = {};
     hipGraph_t graph_t = nullptr;
     hipGraphExec_t exec_t = nullptr;
-    int graph_t_K = 0;
+    int graph_t_K = 0, graph_t_Kc = 0;
     double graph_t_ratio = 0.0;
     std::vector<void*> allocs;
     int64_t bytes = 0;
@@ -712,7 +727,7 @@ inline MgCheb mg_cheb(int K, double ratio, double cscale) {
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     double rho = 1.0 / sigma;
     ch.c0 = (float)(1.0 / theta);
-    for (int k = 1; k < K && k < MG_MAXK; ++k) {
+    for (int k = 1; k < K && k < MG_MAXKC; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
         ch.c1[k] = (float)(rn * rho);
         ch.c2[k] = (float)(2.0 * rn / delta);

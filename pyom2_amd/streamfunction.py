@@ -114,6 +114,7 @@ class SolveOptions:
     poly_degree: int = 0
     poly_lower: float = 0.0
     time_kernels: bool = False
+    mg_coarse_degree: int = 0
 
     def as_c(self):
         o = PsiOpts()
@@ -128,6 +129,7 @@ class SolveOptions:
         o.poly_degree = self.poly_degree
         o.poly_lower = self.poly_lower
         o.time_kernels = int(self.time_kernels)
+        o.mg_coarse_degree = self.mg_coarse_degree
         return o
 
 

@@ -130,6 +130,27 @@ def test_mg_smoothing_degrees(degree):
     s.close()
 
 
+@pytest.mark.parametrize("coarse", [3, 8, 10])
+def test_mg_coarse_smoothing_degree(coarse):
+    """The coarse levels (at most 256 tiles of 64 x 16) smoothed with their
+    own degree: same FP64 residual bound, and a degree above 10 refused."""
+    b = basins.spherical_basin(seed=5, n=512, n_islands=2)
+    s = StreamfunctionSolver.from_basin(b, device=DEV)
+    isl = s.islands
+    sol = s.zeros()
+    its, conv, info = s.congrad_streamfunction(T(b.forc), sol, mode="mg", criterion="relres",
+                                               relres_target=1e-10, max_iterations=500,
+                                               mg_coarse_degree=coarse)
+    print(f"  coarse degree {coarse}: {its} iterations", flush=True)
+    assert conv
+    rel, _ = _oracle_relres(b, isl, N(sol))
+    assert rel <= 1.01e-10
+    with pytest.raises(PsiError):
+        s.congrad_streamfunction(T(b.forc), s.zeros(), mode="mg", criterion="relres",
+                                 relres_target=1e-10, max_iterations=10, mg_coarse_degree=11)
+    s.close()
+
+
 def test_mg_argument_errors():
     b, d = fixtures.load("islands_64")
     isl = Islands(int(d["nisle"]), d["offsets"], d["bi"], d["bj"], d["di"], d["dj"])
