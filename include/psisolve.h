@@ -463,6 +463,12 @@ int psi_fcg_input_scale(psi_ctx *ctx, const float *Z32, const float **r32, const
 /* One flexible-CG step with preconditioned residual z (fp32, network layout
  * H x W, solver grid in its top-left corner); no host sync. */
 int psi_fcg_step(psi_ctx *ctx, const float *znet, int H, int W);
+/* The same step with the network composed with the multigrid V-cycle
+ * (psi_mg_setup first): the preconditioned residual is z = c n + V(r - c A n),
+ * n the network's output znet, c = (n, W r) / (n, S n) its A-optimal scale,
+ * V one V-cycle (5 Chebyshev steps per level) on what the network leaves; no
+ * host sync. */
+int psi_fcg_step_mg(psi_ctx *ctx, const float *znet, int H, int W);
 /* Recursively updated relative residual (host sync). */
 int psi_fcg_read(psi_ctx *ctx, double *relres_recursive);
 /* sol += correction; r = forc - A sol in FP64 (residual replacement);

@@ -159,6 +159,7 @@ SIGNATURES = [
     ("psi_fcg_input_scale", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p)]),
     ("psi_fcg_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    ("psi_fcg_step_mg", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
     ("psi_fcg_read", C.c_int, [C.c_void_p, _dp]),
     ("psi_fcg_flush", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     ("psi_fcg_scalars", C.c_int, [C.c_void_p, _dp]),

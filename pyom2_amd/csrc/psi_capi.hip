@@ -57,6 +57,7 @@ struct psi_ctx {
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *pf32 = nullptr;   // flexible CG
     double* fcg = nullptr;        // flexible-CG device scalars
     int fcg_restart = 1;
+    int fcg_mg_ready = 0;         // the multigrid scalars of psi_fcg_step_mg are set
     double *part = nullptr, *result = nullptr;
     double* part2 = nullptr;      // solver partials, 2 x NRED x G
     int G = 0, G32 = 0;           // resident workgroups of the FP64 / FP32 solver kernels
@@ -157,6 +158,8 @@ __global__ void k_fcg_scalar(double* sc, int what) {
         st_dev(sc + 1, 0.0);
         st_dev(sc + 3, 0.0);
         st_dev(sc + 4, 0.0);
+        st_dev(sc + 10, 0.0);
+        st_dev(sc + 11, 0.0);
     }
 }
 
@@ -238,6 +241,103 @@ __global__ void k_fcg_update(Geom g, const float* __restrict__ p, const float* _
     }
     block_reduce<1>(acc, 0u, sh);
     if (threadIdx.x == 0) atomicAdd(&sc[4], acc[0]);
+}
+
+// ---- flexible CG with the network composed with the multigrid V-cycle ----
+// z = c n + V(r - c A n): n the network's estimate (network layout H x W,
+// masked to active points), c = (n, W r) / (n, S n) its A-optimal scale
+// (S = W A), V the aggregation-multigrid V-cycle on what the network leaves.
+// Scalars sc[10] = (n, W r), sc[11] = (n, S n).
+__device__ __forceinline__ float fcg_net(const float* __restrict__ znet, int H, int W,
+                                         const uint8_t* __restrict__ act, size_t idx, int P) {
+    const int r = (int)(idx / P), c = (int)(idx % P);
+    return (act[idx] && r < H && c < W) ? znet[(size_t)r * W + c] : 0.f;
+}
+
+// A n (FP32 operator) and the two products of the scale.
+__global__ void k_fcgmg_scale(Geom g, const float* __restrict__ znet, int H, int W,
+                              const uint8_t* __restrict__ act, const float* __restrict__ E,
+                              const float* __restrict__ N, const float* __restrict__ rx,
+                              const float* __restrict__ ry, const float* __restrict__ r32,
+                              const double* __restrict__ dxu, const double* __restrict__ cosu,
+                              const double* __restrict__ dyu, float* __restrict__ an,
+                              double* __restrict__ sc) {
+    __shared__ double sh[2 * NW];
+    double acc[2] = {0.0, 0.0};
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const int P = g.pitch;
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
+        const int r = (int)(idx / P), c = (int)(idx % P);
+        float v = 0.f;
+        if (r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
+            const int cl = map_col(c - 1, g), cr = map_col(c + 1, g);
+            const size_t il = (size_t)r * P + cl, ir = (size_t)r * P + cr;
+            const float e = E[idx], w = E[il] * rx[c], n = N[idx], s = N[idx - P] * ry[r];
+            const float cc = (((-e) - w) - n) - s;
+            const float nc = fcg_net(znet, H, W, act, idx, P);
+            v = s * fcg_net(znet, H, W, act, idx - P, P) + w * fcg_net(znet, H, W, act, il, P) +
+                cc * nc + e * fcg_net(znet, H, W, act, ir, P) +
+                n * fcg_net(znet, H, W, act, idx + P, P);
+            const double wa = area_w(g, dxu, cosu, dyu, r, c);
+            acc[0] += (double)nc * (double)r32[idx] * wa;
+            acc[1] += (double)nc * (double)v * wa;
+        }
+        an[idx] = v;
+    }
+    block_reduce<2>(acc, 0u, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(&sc[10], acc[0]);
+        atomicAdd(&sc[11], acc[1]);
+    }
+}
+
+// c minimises the energy norm of e - c n (e = A^-1 r); S is negative definite,
+// so (n, S n) < 0 for any n != 0 on active points
+__device__ __forceinline__ float fcgmg_c(const double* sc) {
+    const double a = ld_dev(sc + 10), q = ld_dev(sc + 11);
+    const double cv = q != 0.0 ? a / q : 0.0;
+    return cv == cv ? (float)cv : 0.f;
+}
+
+// The V-cycle's input on level 0: (w / w_mean) (r - c A n) on active cells.
+__global__ void k_fcgmg_input(Geom g, MgLev L0, const uint8_t* __restrict__ act,
+                              const float* __restrict__ r32, const float* __restrict__ an,
+                              const double* __restrict__ dxu, const double* __restrict__ cosu,
+                              const double* __restrict__ dyu, double inv_wmean,
+                              const double* sc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nx * g.ny) return;
+    const int i = t % g.nx, j = t / g.nx;
+    const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+    const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
+    const float cn = fcgmg_c(sc);
+    L0.b[ql] = act[o] ? (float)(area_w(g, dxu, cosu, dyu, j + 2, i + 2) * inv_wmean *
+                                (double)(r32[o] - cn * an[o]))
+                      : 0.0f;
+}
+
+// zs = c n + w_mean y on active points; (zs, W q_old)
+__global__ void k_fcgmg_scatter(Geom g, MgLev L0, const float* __restrict__ znet, int H, int W,
+                                const uint8_t* __restrict__ act, const float* __restrict__ q32,
+                                const double* __restrict__ dxu, const double* __restrict__ cosu,
+                                const double* __restrict__ dyu, float w_mean,
+                                float* __restrict__ zs, double* __restrict__ sc) {
+    __shared__ double sh[NW];
+    double acc[1] = {0.0};
+    const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
+    const float cn = fcgmg_c(sc);
+    for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
+        const int r = (int)(idx / g.pitch), c = (int)(idx % g.pitch);
+        float z = 0.f;
+        if (act[idx]) {
+            const float y = L0.y[(r - 1) * L0.pitch + (c - 2) + MG_XOFF];
+            z = cn * fcg_net(znet, H, W, act, idx, g.pitch) + w_mean * y;
+        }
+        zs[idx] = z;
+        acc[0] += (double)z * (double)q32[idx] * area_w(g, dxu, cosu, dyu, r, c);
+    }
+    block_reduce<1>(acc, 0u, sh);
+    if (threadIdx.x == 0) atomicAdd(&sc[0], acc[0]);
 }
 
 }  // namespace
@@ -795,6 +895,7 @@ bool valid_field_ctx(psi_ctx* c) { return c && c->E; }
 // ---- multigrid-preconditioned flexible CG ----------------------------------
 void mg_release(psi_ctx* c) {
     MgState* m = c->mg;
+    c->fcg_mg_ready = 0;
     if (!m) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (m->gs) (void)hipStreamSynchronize(m->gs);
@@ -1085,6 +1186,7 @@ void mg_launch_coarse(int Kc, bool pre, const MgLev& v, const MgLev& vc, const M
 struct MgCoarse {
     int Kc = 0;
     MgCheb chc{};
+    const MgScal* S = nullptr;   // scalars the kernels test (null: the solve's, m->S)
 };
 
 template <int K>
@@ -1096,27 +1198,28 @@ void mg_vcycle_fused(MgState* m, hipStream_t s, const MgCheb& ch, const MgCoarse
         ev->rec(1, s);
     }
     auto coarse = [&](int l) { return co.Kc > 0 && l > 0 && mg_coarse_level(m->lev[l]); };
+    const MgScal* S = co.S ? co.S : (const MgScal*)m->S;
     for (int l = 0; l < L; ++l) {
         if (ev && l == 0) ev->rec(0, s);
         if (coarse(l))
             mg_launch_coarse(co.Kc, true, m->lev[l], m->lev[l + 1], co.chc,
-                             (const MgScal*)m->S, s);
+                             S, s);
         else
-            mg_launch_pre<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
+            mg_launch_pre<K>(m->lev[l], m->lev[l + 1], ch, S, s);
         if (ev && l == 0) ev->rec(1, s);
     }
     if (m->ncoarse > 0)
         hipLaunchKernelGGL(k_mg_dense, dim3((m->ncoarse + 3) / 4), dim3(256), 0, s, m->lev[L],
                            (const float*)m->Ainv, m->dense_ld, (const int*)m->cells, m->ncoarse,
-                           (const MgScal*)m->S);
+                           S);
     if (ev && L == 0) ev->rec(2, s);
     for (int l = L - 1; l >= 0; --l) {
         if (ev && l == 0) ev->rec(2, s);
         if (coarse(l))
             mg_launch_coarse(co.Kc, false, m->lev[l], m->lev[l + 1], co.chc,
-                             (const MgScal*)m->S, s);
+                             S, s);
         else
-            mg_launch_post<K>(m->lev[l], m->lev[l + 1], ch, (const MgScal*)m->S, s);
+            mg_launch_post<K>(m->lev[l], m->lev[l + 1], ch, S, s);
     }
     if (ev) ev->rec(3, s);
 }
@@ -2143,6 +2246,42 @@ int psi_fcg_step(psi_ctx* c, const float* znet, int H, int W) {
     hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(1), 0, c->stream, c->fcg, 1);
     hipLaunchKernelGGL(k_fcg_scatter, dim3(1024), dim3(NT), 0, c->stream, g, znet, H, W, c->act,
                        c->q32, c->dxu, c->cosu, c->dyu, c->z32, c->fcg);
+    hipLaunchKernelGGL(k_fcg_dir, dim3(nblk(c->plane)), dim3(256), 0, c->stream, g, c->z32, c->pf32,
+                       c->fcg, c->fcg_restart);
+    hipLaunchKernelGGL(k_fcg_apply, dim3(1024), dim3(NT), 0, c->stream, g, c->E32, c->N32, c->rx32,
+                       c->ry32, c->pf32, c->r32, c->dxu, c->cosu, c->dyu, c->q32, c->fcg);
+    hipLaunchKernelGGL(k_fcg_update, dim3(1024), dim3(NT), 0, c->stream, g, c->pf32, c->q32, c->act,
+                       c->e32, c->r32, c->fcg);
+    c->fcg_restart = 0;
+    return check_launch(c);
+}
+
+int psi_fcg_step_mg(psi_ctx* c, const float* znet, int H, int W) {
+    if (!valid_field_ctx(c) || !znet || H < 1 || W < 1) return PSI_BAD_ARG;
+    MgState* m = c->mg;
+    if (!m) return fail(c, PSI_BAD_ARG, "psi_fcg_step_mg: call psi_mg_setup first%s");
+    const Geom& g = c->g;
+    if (!c->fcg_mg_ready) {
+        // the V-cycle kernels test S_time's done flag, which stays clear
+        MgScal z{};
+        HIPCHK(c, hipMemcpyAsync(m->S_time, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->fcg_mg_ready = 1;
+    }
+    hipLaunchKernelGGL(k_fcg_scalar, dim3(1), dim3(1), 0, c->stream, c->fcg, 1);
+    float* an = c->rp32[0];
+    hipLaunchKernelGGL(k_fcgmg_scale, dim3(1024), dim3(NT), 0, c->stream, g, znet, H, W, c->act,
+                       c->E32, c->N32, c->rx32, c->ry32, c->r32, c->dxu, c->cosu, c->dyu, an,
+                       c->fcg);
+    hipLaunchKernelGGL(k_fcgmg_input, dim3(nblk((size_t)g.nx * g.ny)), dim3(256), 0, c->stream, g,
+                       m->lev[0], c->act, c->r32, (const float*)an, c->dxu, c->cosu, c->dyu,
+                       1.0 / m->w_mean, c->fcg);
+    MgCoarse co;
+    co.S = m->S_time;
+    mg_vcycle(m, c->stream, 5, mg_cheb(5, 15.0, m->cscale), nullptr, co);
+    hipLaunchKernelGGL(k_fcgmg_scatter, dim3(1024), dim3(NT), 0, c->stream, g, m->lev[0], znet, H,
+                       W, c->act, c->q32, c->dxu, c->cosu, c->dyu, (float)m->w_mean, c->z32,
+                       c->fcg);
     hipLaunchKernelGGL(k_fcg_dir, dim3(nblk(c->plane)), dim3(256), 0, c->stream, g, c->z32, c->pf32,
                        c->fcg, c->fcg_restart);
     hipLaunchKernelGGL(k_fcg_apply, dim3(1024), dim3(NT), 0, c->stream, g, c->E32, c->N32, c->rx32,

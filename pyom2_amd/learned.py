@@ -322,10 +322,12 @@ def smooth_error(rng, solver: StreamfunctionSolver, n_modes=24, kmax=None, noise
 
 
 def harvest_pairs(solver: StreamfunctionSolver, forc, rng, n_synthetic=24,
-                  cg_iters=(5, 20, 60, 150, 400, 1000, 2500)):
-    """(r, e) pairs for one basin: errors of plain FP64 CG iterates
-    (x* - x_k, r_k = forc - A x_k, k in cg_iters) and synthetic red-spectrum
-    errors e with r = A e.  Returns FP64 device tensors R, E (n, ny+4, nx+4)."""
+                  cg_iters=(5, 20, 60, 150, 400, 1000, 2500), mode="plain"):
+    """(r, e) pairs for one basin: errors of the solver's own iterates
+    (x* - x_k, r_k = forc - A x_k, k in cg_iters) -- plain FP64 Jacobi-PCG, or
+    with mode="mg" the multigrid flexible CG, whose iterates leave the error
+    modes a V-cycle reduces slowest -- and synthetic red-spectrum errors e
+    with r = A e.  Returns FP64 device tensors R, E (n, ny+4, nx+4)."""
     dev = solver.device
     Z = solver.make_inv_sfc()
     act = (Z != 0).double()
@@ -335,7 +337,7 @@ def harvest_pairs(solver: StreamfunctionSolver, forc, rng, n_synthetic=24,
     Rs, Es = [], []
     for k in cg_iters:
         x = solver.zeros()
-        solver.congrad_streamfunction(forc, x, SolveOptions(criterion="relres",
+        solver.congrad_streamfunction(forc, x, SolveOptions(mode=mode, criterion="relres",
                                                             relres_target=1e-30, max_iterations=k))
         e = (xs - x) * act
         if float(e.abs().max()) == 0:
@@ -431,9 +433,16 @@ class LearnedSolver:
     `max_iterations`); policy="jacobi" runs Jacobi-PCG only."""
 
     def __init__(self, solver: StreamfunctionSolver, net: PrecondNet, reprobe_every=50,
-                 probe=32, probe_jacobi=512):
+                 probe=32, probe_jacobi=512, compose_mg=False):
         self.solver = solver
         self.net = net
+        # compose_mg: each step's preconditioned residual is the network's
+        # estimate plus one multigrid V-cycle on what it leaves
+        # (psi_fcg_step_mg), so the global modes the network cannot reach are
+        # handled too
+        self.compose_mg = compose_mg
+        if compose_mg and not solver._mg_ready:
+            solver.mg_setup()
         self.inputs = SolverInputs(solver)
         self.x16 = torch.zeros((1, self.inputs.H, self.inputs.W, 16), dtype=BF16,
                                device=solver.device)
@@ -467,7 +476,8 @@ class LearnedSolver:
                                        s.ny + 4, s.nx + 4, H, W, self.scale_ptr, _p(self.x16), st),
                      "net_input")
                 z = self.net.forward(self.x16, 1, H, W)
-                _chk(L.psi_fcg_step(ctx, _p(z), H, W), "fcg_step")
+                step = L.psi_fcg_step_mg if self.compose_mg else L.psi_fcg_step
+                _chk(step(ctx, _p(z), H, W), "fcg_step")
                 its += 1
             _chk(L.psi_fcg_read(ctx, C.byref(rr)), "fcg_read")
             rec = rr.value

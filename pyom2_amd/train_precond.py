@@ -1,12 +1,18 @@
 """Train the learned preconditioner on the GPU and evaluate it.
 
-    python -m pyom2_amd.train_precond --steps 3000 --out pyom2_amd/weights/precond_unet.npz
+    python -m pyom2_amd.train_precond --pairs mg --steps 12500 --basins 24 \
+        --init pyom2_amd/weights/precond_unet.npz --out pyom2_amd/weights/precond_unet.npz
 
 Training data: residual/error pairs on seeded basins of 256^2-1024^2 (the
-spherical and beta-plane constructions, 0-3 islands): errors of plain FP64
-Jacobi-PCG iterates (x* - x_k with r_k = forc - A x_k) harvested once per
-basin, plus red-spectrum synthetic errors e with r = A e drawn afresh for
-every step.  Loss: the A-norm cosine (pyom2_amd.learned); optimiser: Adam.
+spherical and beta-plane constructions, 0-3 islands): errors of the solver's
+own iterates (x* - x_k with r_k = forc - A x_k) harvested once per basin --
+multigrid flexible-CG iterates with --pairs mg (the errors a V-cycle leaves,
+which the network composed with the V-cycle has to catch), plain FP64
+Jacobi-PCG iterates with --pairs plain -- plus red-spectrum synthetic errors
+e with r = A e drawn afresh every --refresh-every steps.  Loss: the A-norm
+cosine (pyom2_amd.learned); optimiser: Adam.  The shipped weights are the
+command above: 12,500 steps of batch 4 (50,000 pairs) on 24 basins, started
+from weights trained on plain-CG pairs.
 """
 from __future__ import annotations
 
@@ -44,13 +50,19 @@ def training_basins(seed, count, sizes):
     return out
 
 
-def build_sets(bs, rng, device="cuda", cg_iters=(5, 20, 60, 150, 400, 1000, 2500), n_synthetic=8):
+# iterates harvested per basin: plain Jacobi-PCG, or multigrid flexible CG
+# (whose errors are the modes the composed preconditioner must catch)
+PAIR_ITERS = {"plain": (5, 20, 60, 150, 400, 1000, 2500), "mg": (1, 2, 3, 4, 5, 6, 8, 10)}
+
+
+def build_sets(bs, rng, device="cuda", pairs="plain", n_synthetic=8):
     sets = []
     for b in bs:
         t = time.time()
         s = StreamfunctionSolver.from_basin(b, device=device)
         forc = torch.as_tensor(b.forc, device=device)
-        ts = learned.make_trainset(s, forc, rng, n_synthetic=n_synthetic, cg_iters=cg_iters)
+        ts = learned.make_trainset(s, forc, rng, n_synthetic=n_synthetic,
+                                   cg_iters=PAIR_ITERS[pairs], mode=pairs)
         sets.append(ts)
         log(f"[data] {b.name} {b.nx}x{b.ny} nisle={s.nisle} pairs={ts.R.shape[0]} "
             f"({time.time() - t:.1f}s)")
@@ -76,7 +88,7 @@ def refresh_synthetic(ts, rng, k_synth):
 def train_main(args):
     rng = np.random.default_rng(args.seed)
     bs = training_basins(args.seed, args.basins, [int(x) for x in args.sizes.split(",")])
-    sets = build_sets(bs, rng, n_synthetic=args.synthetic)
+    sets = build_sets(bs, rng, pairs=args.pairs, n_synthetic=args.synthetic)
     net = learned.PrecondNet(seed=args.seed)
     if args.init and os.path.exists(args.init):
         net = learned.PrecondNet.load(args.init)
@@ -99,8 +111,19 @@ def train_main(args):
     return net
 
 
+def zero_net(device="cuda"):
+    """The network with every weight zero: the composed preconditioner without
+    the network's term (its ablation)."""
+    net = learned.PrecondNet(device=device)
+    for k in net.w:
+        net.w[k].zero_()
+        net.b[k].zero_()
+    net.repack()
+    return net
+
+
 def evaluate(net, n, seed=11, relres=1e-10, modes=("learned", "mixed", "plain")):
-    b = basins.spherical_basin(seed=seed, n=n, n_islands=2)
+    b = basins.spherical_basin(seed=seed, n=n, n_islands=2, dlon=0.05 * 2048 / n)
     s = StreamfunctionSolver.from_basin(b)
     forc = torch.as_tensor(b.forc, device=s.device)
     res = {"n": n}
@@ -108,8 +131,9 @@ def evaluate(net, n, seed=11, relres=1e-10, modes=("learned", "mixed", "plain"))
         sol = s.zeros()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        if mode == "learned":
-            ls = learned.LearnedSolver(s, net)
+        if mode in ("learned", "learned+mg", "mg-fcg32"):
+            ls = learned.LearnedSolver(s, zero_net() if mode == "mg-fcg32" else net,
+                                       compose_mg=(mode != "learned"))
             its, conv, info = ls.solve(forc, sol, relres_target=relres, max_iterations=20000,
                                        policy="learned")
         else:
@@ -137,13 +161,17 @@ def main():
     ap.add_argument("--init", default="")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--eval", default="512,1024,2048")
+    ap.add_argument("--eval-modes", default="learned,mixed,plain")
+    ap.add_argument("--pairs", default="plain", choices=sorted(PAIR_ITERS),
+                    help="iterates the (residual, error) pairs are harvested from")
     ap.add_argument("--eval-only", action="store_true")
     args = ap.parse_args()
     if args.eval_only:
         net = learned.PrecondNet.load(args.out)
     else:
         net = train_main(args)
-    out = [evaluate(net, int(n)) for n in args.eval.split(",") if n]
+    out = [evaluate(net, int(n), modes=tuple(args.eval_modes.split(",")))
+           for n in args.eval.split(",") if n]
     print(json.dumps(out), flush=True)
 
 
