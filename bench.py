@@ -7,19 +7,18 @@ forcings (AR(1), rho = 0.99) solved one after another with the reference's
 warm start dpsi(taup1) = 2 dpsi(tau) - dpsi(taum1).  A step is one solve.
 Inputs are resident in HBM before the timed region starts.
 
-Paths: "learned" is the learned-preconditioner solver with its measured
-policy (flexible CG with the trained network or the fastest classical path
-measured here, whichever needs fewer ms per residual decade on a probe from
-the same first guess; probes at the first solve and every --reprobe-every
-solves); "mg" is FP64 flexible CG preconditioned by an FP32
-aggregation-multigrid V-cycle (the fastest classical path, so the learned
-policy's fallback); "poly" is FP32-inner / FP64-refined PCG with a Chebyshev
-polynomial of the Jacobi-preconditioned operator (degree 4); "mixed" is
-FP32-inner / FP64-refined Jacobi-PCG; "plain" is FP64 Jacobi-PCG.
+Paths: "mg" (the headline: the fastest) is FP64 flexible CG preconditioned
+by an FP32 aggregation-multigrid V-cycle; "learned" is flexible CG (FP32
+inner, FP64 residual replacement) preconditioned by the trained network
+composed with one V-cycle per step; "poly" is FP32-inner / FP64-refined PCG
+with a Chebyshev polynomial of the Jacobi-preconditioned operator (degree
+4); "mixed" is FP32-inner / FP64-refined Jacobi-PCG; "plain" is FP64
+Jacobi-PCG.
 
-Prints one JSON line with the headline path's ms per solve, the other
-paths beside it, the fused sweep kernel's roofline figure and the CPU
-oracle timed on a bounded sample on this host.
+Prints one JSON line with the headline path's ms per solve, the other paths
+beside it (the learned path's own figures under "learned"), every path's
+solution against one oracle solve of the last timed forcing, the heaviest
+kernel's roofline figure and the CPU oracle timed on this host.
 """
 from __future__ import annotations
 
@@ -85,10 +84,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--modes", default="learned,mg,poly,mixed,plain",
+    ap.add_argument("--modes", default="mg,learned,poly,mixed,plain",
                     help="first one is the headline")
     ap.add_argument("--weights", default="", help="learned preconditioner weights (.npz)")
-    ap.add_argument("--reprobe-every", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (3 times): a "
                          "kernel of known traffic that calibrates HBM byte counters")
@@ -189,10 +187,7 @@ def main():
     results = {}
     last_sol = {}
     for mode in modes:
-        # the learned path falls back to the fastest classical preconditioner
-        # among the paths measured here
-        fallback = next((m for m in ("mg", "poly", "mixed") if m in modes), "mixed")
-        opts = SolveOptions(mode=fallback if mode == "learned" else mode, criterion="relres",
+        opts = SolveOptions(mode="mg" if mode == "learned" else mode, criterion="relres",
                             relres_target=args.relres, max_iterations=200000,
                             check_every=args.check_every, inner_reduction=args.inner_reduction,
                             poly_degree=args.poly_degree, poly_lower=args.poly_lower,
@@ -200,12 +195,15 @@ def main():
                             mg_coarse_degree=args.mg_coarse_degree)
         ls = None
         if mode == "learned":
+            # flexible CG with the trained network composed with one
+            # multigrid V-cycle per step (FP32 inner, FP64 residual
+            # replacement), convergence polled after every step
             from pyom2_amd import learned
             net = learned.PrecondNet.load(args.weights or learned.DEFAULT_WEIGHTS, device=dev)
-            ls = learned.LearnedSolver(s, net, reprobe_every=args.reprobe_every)
+            ls = learned.LearnedSolver(s, net, compose_mg=True)
         dpsi = torch.zeros((3,) + b.shape, dtype=torch.float64, device=dev)
         taum1, tau, taup1 = 0, 1, 2
-        per, probes, choices = [], [], []
+        per, choices = [], []
         for k in range(nsteps):
             if k == args.warmup:
                 torch.cuda.synchronize()
@@ -214,9 +212,7 @@ def main():
             dpsi[taup1] = 2 * dpsi[tau] - dpsi[taum1]
             if ls is not None:
                 its, conv, linfo = ls.solve(forcings[k], dpsi[taup1], relres_target=args.relres,
-                                            fallback_opts=opts)
-                if linfo["probe"]:
-                    probes.append(dict(step=k, **linfo["probe"]))
+                                            max_iterations=2000, policy="learned", poll_every=1)
                 choices.append(linfo["choice"])
                 info = dict(relres_true=linfo["relres_true"], sweep_ms=0.0, sweep_launches=0,
                             outer_iterations=linfo["flushes"])
@@ -278,18 +274,16 @@ def main():
             results[mode].update(us_per_iteration=1e3 * sweep_ms / max(launches, 1),
                                  kernels=mg_kernels, dominant=top, kernel_samples=nsamp)
         if ls is not None:
-            timed = choices[args.warmup:]
-            ch = max(set(timed), key=timed.count)
-            results[mode].update(choice=ch if ch == "learned" else f"fallback: {fallback}",
-                                 probes=probes, reprobe_every=args.reprobe_every)
+            results[mode].update(preconditioner="network + multigrid V-cycle",
+                                 all_steps_learned=all(c == "learned" for c in choices))
             for key in ("sweep_avg_launch_us", "sweep_launches", "achieved_GBs", "frac"):
                 results[mode].pop(key)
         log(f"[bench] {mode}: {results[mode]}")
 
     head = modes[0]
     r = results[head]
-    # the roofline figure is the fused sweep kernel of the Jacobi path that
-    # carries the solve (the learned policy's fallback is the mixed path)
+    # the roofline figure is the heaviest kernel of the headline path (of the
+    # multigrid path when the learned one leads: its V-cycle is that path's)
     kpath = head if head != "learned" else next((m for m in ("mg", "poly", "mixed")
                                                     if m in results), None)
     rk = results.get(kpath, {}) if kpath else {}
@@ -370,14 +364,25 @@ def main():
                                      "congr_epsilon 1e-12 (the reference's stopping rule)"}
     if "plain" in results and head != "plain":
         line["speedup_vs_plain_fp64"] = round(results["plain"]["ms_per_solve"] / r["ms_per_solve"], 3)
-    if head == "learned":
-        ch = r.get("choice")
-        if ch == "learned":
-            line["config"]["path"] = "learned (flexible CG with the trained network)"
-        else:
-            # the timed solves ran the fallback path: name it
-            line["config"]["path"] = (f"{ch.split(': ')[-1]} (the learned policy's fallback: "
-                                      f"the network's flexible CG lost the probe)")
+    if "learned" in results:
+        # the learned path's own figures (flexible CG, network + V-cycle)
+        lr = results["learned"]
+        line["learned"] = {"ms_per_solve": round(lr["ms_per_solve"], 3),
+                           "iterations_mean": lr["iterations_mean"],
+                           "relres_max": float(f"{lr['relres_max']:.3g}")}
+        if "plain" in results:
+            line["learned"]["speedup_vs_plain_fp64"] = round(
+                results["plain"]["ms_per_solve"] / lr["ms_per_solve"], 2)
+        if "mg" in results:
+            line["learned"]["vs_mg_ms"] = round(lr["ms_per_solve"] / results["mg"]["ms_per_solve"], 2)
+            line["learned"]["vs_mg_iterations"] = round(
+                lr["iterations_mean"] / results["mg"]["iterations_mean"], 3)
+        if "psi_err" in lr:
+            line["learned"]["psi_err"] = float(f"{lr['psi_err']:.3g}")
+    names = {"mg": "mg (FP64 flexible CG, FP32 multigrid V-cycle)",
+             "learned": "learned (flexible CG, trained network + multigrid V-cycle)",
+             "poly": "poly", "mixed": "mixed", "plain": "plain FP64 Jacobi-PCG"}
+    line["config"]["path"] = names.get(head, head)
     print(json.dumps(finite(line)), flush=True)
 
 

@@ -25,7 +25,7 @@ def test_defaults(monkeypatch):
     monkeypatch.setattr(sys, "argv", ["bench.py"])
     a = bench.parse()
     assert (a.gpus, a.n, a.workload, a.relres) == (1, 2048, "solve", 1e-10)
-    assert a.modes.split(",")[0] == "learned" and "mg" in a.modes.split(",")
+    assert a.modes.split(",")[0] == "mg" and "learned" in a.modes.split(",")
     assert a.steps >= 1 and a.warmup >= 0
 
 
