@@ -18,7 +18,8 @@
 extern "C" {
 #endif
 
-/* D = A B for 16x16 bf16 A, B (row-major) via one MFMA: lane-layout check. */
+/* D = A B for bf16 A (16 x 32) and B (32 x 16), row-major, fp32 D (16 x 16),
+ * via one v_mfma_f32_16x16x32_bf16, the convolutions' MFMA: lane-layout check. */
 int psinn_mfma_probe(const uint16_t *A, const uint16_t *B, float *D, void *stream);
 
 /* 3x3 conv, zero padding, input = [in1 (C1, nearest x2 upsampled if up1) |
@@ -36,7 +37,9 @@ int psinn_conv3x3_wgrad(const uint16_t *in1, int C1, int up1, const uint16_t *in
                         void *stream);
 
 /* Pack fp32 w[tap][cin][cout] into MFMA B fragments (transposed = 1: the
- * flipped, transposed weights of the data-gradient convolution). */
+ * flipped, transposed weights of the data-gradient convolution); k16 and n16
+ * are the 16-channel blocks along K and N, and pack holds k16 * n16 * 2560
+ * bf16 (per 16 input channels, 5 K steps of two taps, the tenth zero). */
 int psinn_pack_weights(const float *w, int cin, int cout, int transposed, int k16, int n16,
                        uint16_t *pack, void *stream);
 

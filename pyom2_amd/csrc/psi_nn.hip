@@ -4,8 +4,10 @@
 // Adam, the A-norm training loss and the flexible-CG helpers that use it.
 //
 // Activations are NHWC bfloat16 ([batch][H][W][C], C a multiple of 16).
-// MFMA v_mfma_f32_16x16x16_bf16 (wave64): lane l supplies A[l%16][4(l/16)..+3]
-// and B[4(l/16)..+3][l%16] and receives D[4(l/16)+i][l%16], i = 0..3.
+// Convolutions: MFMA v_mfma_f32_16x16x32_bf16 (gfx950, wave64): lane l
+// supplies A[l%16][8(l/16)..+7] and B[8(l/16)..+7][l%16] and receives
+// D[4(l/16)+i][l%16], i = 0..3.  Weight gradients: v_mfma_f32_16x16x16_bf16
+// (A[l%16][4(l/16)..+3], B[4(l/16)..+3][l%16], the same D).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -25,33 +27,64 @@ __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
     return (uint16_t)(u >> 16);
 }
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
 __device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+__device__ __forceinline__ float4_t mfma32(uint4 a, uint4 b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                   __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
+
 // ---------------------------------------------------------------------------
-// Lane-layout probe: D = A (16x16) x B (16x16), bf16 in, fp32 out, row-major.
+// Lane-layout probe: D = A (16x32) x B (32x16), bf16 in, fp32 out, row-major,
+// through the convolutions' MFMA.
 __global__ void k_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
     const int l = threadIdx.x;
-    short4_t a, b;
-    for (int j = 0; j < 4; ++j) {
-        a[j] = (short)A[(l & 15) * 16 + 4 * (l >> 4) + j];
-        b[j] = (short)B[(4 * (l >> 4) + j) * 16 + (l & 15)];
+    uint16_t a[8], b[8];
+    for (int j = 0; j < 8; ++j) {
+        a[j] = A[(l & 15) * 32 + 8 * (l >> 4) + j];
+        b[j] = B[(8 * (l >> 4) + j) * 16 + (l & 15)];
     }
     float4_t d = {0, 0, 0, 0};
-    d = mfma16(a, b, d);
+    d = mfma32(__builtin_bit_cast(uint4, a), __builtin_bit_cast(uint4, b), d);
     for (int i = 0; i < 4; ++i) D[(4 * (l >> 4) + i) * 16 + (l & 15)] = d[i];
 }
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution, stride 1, zero padding, implicit GEMM on MFMA.
 // Input channels are the concatenation [in1 (C1) | in2 (C2)]; in1 may be
-// read through a nearest x2 upsampling (up1: in1 has H/2 x W/2).  Each wave
-// computes 16 consecutive pixels of one row for all 16*NB output channels.
-// Weights are pre-packed per (tap, kb, nb) as 64 lanes x 4 bf16 (B fragment
-// order).  Epilogue: + bias, optional ReLU; output bf16 or fp32 with
-// cout_store channels (<= 16*NB).
+// read through a nearest x2 upsampling (up1: in1 has H/2 x W/2).
+//
+// A workgroup (4 waves) computes a CV_TH x CV_TW tile of output pixels for
+// all 16 NB output channels.  The input channels are taken 16 at a time: for
+// each chunk the (CV_TH+2) x (CV_TW+2) x 16 input tile (with its halo) and
+// the chunk's packed weights are staged in LDS once, shared by the four
+// waves and reused for all 9 taps.  A K step of the MFMA covers two taps of
+// the chunk (lane group g of a 16-lane quarter takes tap 2 ks + g/2,
+// channels 8 (g%2) .. +7; the tenth tap is zero), so a chunk is 5 K steps.
+// Wave w computes output row w of the tile: MR segments of 16 pixels x NB
+// channel blocks in accumulators.  The next chunk's loads are issued into
+// registers before the current chunk's MFMAs.
+// Weights are pre-packed per (chunk, K step, nb) as 64 lanes x 8 bf16 (B
+// fragment order, k_pack_weights).  Epilogue: + bias, optional ReLU; output
+// bf16 or fp32 with cout_store channels (<= 16 NB).
 // ---------------------------------------------------------------------------
+constexpr int CV_TH = 4, CV_TW = 64;
+constexpr int CV_FX = CV_TW + 2, CV_FY = CV_TH + 2;
+constexpr int CV_PS = 24;                       // LDS bf16 per frame pixel (16 + 8 pad)
+constexpr int CV_IN16 = CV_FX * CV_FY * 2;      // 16-byte units of an input chunk
+
+template <int NB>
+struct ConvShape {
+    static constexpr int MR = NB <= 4 ? 4 : (NB <= 8 ? 2 : 1);   // 16-pixel segments per wave
+    static constexpr int W16 = 5 * NB * 64;                       // 16-byte units of a weight chunk
+    static constexpr int PER_IN = (CV_IN16 + 255) / 256;
+    static constexpr int PER_W = (W16 + 255) / 256;
+};
+
 template <int NB>
 __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in1, int C1, int up1,
                                                  const uint16_t* __restrict__ in2, int C2, int H,
@@ -59,59 +92,142 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
                                                  const float* __restrict__ bias, int relu,
                                                  uint16_t* __restrict__ out16,
                                                  float* __restrict__ out32, int cout_store) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 16;
-    const int y = blockIdx.y * 4 + wave;
+    using Sh = ConvShape<NB>;
+    constexpr int MR = Sh::MR, PER_IN = Sh::PER_IN, PER_W = Sh::PER_W, W16 = Sh::W16;
+    // input frame [fy][fx][24 bf16] and the chunk's weights [ks][nb][lane][8]
+    __shared__ uint4 sIn[CV_FX * CV_FY * CV_PS / 8];
+    __shared__ uint4 sW[W16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * CV_TW, y0 = blockIdx.y * CV_TH;
     const int bimg = blockIdx.z;
-    if (y >= H) return;
-    const int KB = (C1 + C2) >> 4;
+    const int nchunk = (C1 + C2) >> 4;
     const int H1 = up1 ? (H >> 1) : H, W1 = up1 ? (W >> 1) : W;
     const uint16_t* img1 = in1 + (size_t)bimg * H1 * W1 * C1;
     const uint16_t* img2 = in2 ? in2 + (size_t)bimg * H * W * C2 : nullptr;
-    float4_t acc[NB];
+
+    // global -> registers: input unit u = (frame pixel, half of the 16 channels)
+    auto load_in = [&](int ch, uint4 (&v)[PER_IN]) {
+        const int c0 = ch * 16;
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = (float4_t){0, 0, 0, 0};
-    const int px_l = x0 + (lane & 15);
-    const int cq = 4 * (lane >> 4);
-    for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const int yy = y + dy, xx = px_l + dx;
-        const bool inb = yy >= 0 && yy < H && xx >= 0 && xx < W;
-        for (int kb = 0; kb < KB; ++kb) {
-            const int c = kb * 16 + cq;
-            short4_t a = {0, 0, 0, 0};
-            if (inb) {
-                const uint16_t* src;
-                if (c < C1) {
-                    const int ys = up1 ? (yy >> 1) : yy, xs = up1 ? (xx >> 1) : xx;
-                    src = img1 + ((size_t)ys * W1 + xs) * C1 + c;
-                } else {
-                    src = img2 + ((size_t)yy * W + xx) * C2 + (c - C1);
+        for (int k = 0; k < PER_IN; ++k) {
+            const int u = tid + k * 256;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (u < CV_IN16) {
+                const int px = u >> 1, half = u & 1;
+                const int fy = px / CV_FX, fx = px - fy * CV_FX;
+                const int yy = y0 + fy - 1, xx = x0 + fx - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    const int c = c0 + 8 * half;
+                    const uint16_t* src;
+                    if (c < C1) {
+                        const int ys = up1 ? (yy >> 1) : yy, xs = up1 ? (xx >> 1) : xx;
+                        src = img1 + ((size_t)ys * W1 + xs) * C1 + c;
+                    } else {
+                        src = img2 + ((size_t)yy * W + xx) * C2 + (c - C1);
+                    }
+                    val = *reinterpret_cast<const uint4*>(src);
                 }
-                a = *(const short4_t*)src;
             }
-            const uint16_t* wp = wpack + ((size_t)(tap * KB + kb) * NB) * 256 + lane * 4;
+            v[k] = val;
+        }
+    };
+    auto load_w = [&](int ch, uint4 (&v)[PER_W]) {
+        const uint4* src = reinterpret_cast<const uint4*>(wpack) + (size_t)ch * W16;
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const short4_t b = *(const short4_t*)(wp + nb * 256);
-                acc[nb] = mfma16(a, b, acc[nb]);
+        for (int k = 0; k < PER_W; ++k) {
+            const int u = tid + k * 256;
+            v[k] = u < W16 ? src[u] : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    auto store = [&](const uint4 (&vi)[PER_IN], const uint4 (&vw)[PER_W]) {
+#pragma unroll
+        for (int k = 0; k < PER_IN; ++k) {
+            const int u = tid + k * 256;
+            if (u < CV_IN16) sIn[(u >> 1) * (CV_PS / 8) + (u & 1)] = vi[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PER_W; ++k) {
+            const int u = tid + k * 256;
+            if (u < W16) sW[u] = vw[k];
+        }
+    };
+
+    float4_t acc[MR][NB];
+#pragma unroll
+    for (int m = 0; m < MR; ++m)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[m][nb] = (float4_t){0, 0, 0, 0};
+
+    const int g = lane >> 4, pl = lane & 15;
+    // this wave's output row; its pixels [x0 + 16 m + pl] for m < MR, of the
+    // CV_TW / 16 segments of the tile (waves with MR < 4 cover the tile in
+    // several passes below)
+    uint4 ri[PER_IN], rw[PER_W];
+    load_in(0, ri);
+    load_w(0, rw);
+    constexpr int SEG = CV_TW / 16, PASSES = SEG / MR;
+    static_assert(SEG % MR == 0, "segments per wave");
+    // With MR < SEG the tile's segments are processed in PASSES rounds, each
+    // over all chunks (accumulators are per round).
+#pragma unroll 1
+    for (int pass = 0; pass < PASSES; ++pass) {
+        if (pass > 0) {
+#pragma unroll
+            for (int m = 0; m < MR; ++m)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[m][nb] = (float4_t){0, 0, 0, 0};
+            load_in(0, ri);
+            load_w(0, rw);
+        }
+#pragma unroll 1
+        for (int ch = 0; ch < nchunk; ++ch) {
+            __syncthreads();   // the previous chunk's MFMAs have read the LDS
+            store(ri, rw);
+            __syncthreads();
+            if (ch + 1 < nchunk) {
+                load_in(ch + 1, ri);
+                load_w(ch + 1, rw);
+            }
+#pragma unroll
+            for (int ks = 0; ks < 5; ++ks) {
+                const int tap = 2 * ks + (g >> 1);
+                // tap 9 is padding: its A is zero, addressed at tap 8's offsets
+                const int tt = tap < 9 ? tap : 8;
+                const int dy = tt / 3 - 1, dx = tt - 3 * (tt / 3) - 1;
+                uint4 a[MR];
+#pragma unroll
+                for (int m = 0; m < MR; ++m) {
+                    const int fx = (pass * MR + m) * 16 + pl + dx + 1, fy = wave + dy + 1;
+                    a[m] = tap < 9 ? sIn[(fy * CV_FX + fx) * (CV_PS / 8) + (g & 1)]
+                                   : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    const uint4 b = sW[(ks * NB + nb) * 64 + lane];
+#pragma unroll
+                    for (int m = 0; m < MR; ++m) acc[m][nb] = mfma32(a[m], b, acc[m][nb]);
+                }
             }
         }
-    }
-    const int co_l = lane & 15;
+        const int y = y0 + wave;
+        if (y < H) {
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-        const int co = nb * 16 + co_l;
-        const float bv = bias ? bias[co] : 0.f;
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + pl;
+                const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int px = x0 + cq + i;
-            if (px >= W || co >= cout_store) continue;
-            float v = acc[nb][i] + bv;
-            if (relu) v = fmaxf(v, 0.f);
-            const size_t o = (((size_t)bimg * H + y) * W + px) * cout_store + co;
-            if (out32) out32[o] = v;
-            else out16[o] = f2bf(v);
+                for (int m = 0; m < MR; ++m)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int px = x0 + (pass * MR + m) * 16 + 4 * g + i;
+                        if (px >= W || co >= cout_store) continue;
+                        float v = acc[m][nb][i] + bv;
+                        if (relu) v = fmaxf(v, 0.f);
+                        const size_t o = (((size_t)bimg * H + y) * W + px) * cout_store + co;
+                        if (out32) out32[o] = v;
+                        else out16[o] = f2bf(v);
+                    }
+            }
         }
     }
 }
@@ -179,28 +295,37 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(
 }
 
 // Pack fp32 master weights w[tap][ci][co] (CIN x COUT, tap = 3 ky + kx) into
-// B fragments.  transposed = 0: forward convolution (K = ci, N = co).
+// the B fragments of k_conv3x3: per input chunk of 16 channels, 5 K steps of
+// two taps (the tenth zero), per 16-channel output block nb, 64 lanes x 8
+// bf16: lane l, element j holds K = (tap 2 ks + g/2, channel 16 chunk +
+// 8 (g%2) + j), N = 16 nb + l%16, with g = l/16.
+// transposed = 0: forward convolution (K = ci, N = co).
 // transposed = 1: data-gradient convolution (K = co, N = ci, taps flipped):
 // dIn[p][ci] = sum_tap sum_co dOut[p - d(tap)][co] w[tap][ci][co]
 //            = sum_tap' sum_co dOut[p + d(tap')][co] w[8 - tap'][ci][co].
-// K16/N16: number of 16-blocks along K and N (zero padded).
+// K16/N16: number of 16-blocks along K and N (zero padded); the pack holds
+// K16 * 5 * N16 * 512 bf16.
 __global__ void k_pack_weights(const float* __restrict__ w, int CIN, int COUT, int transposed,
                                int K16, int N16, uint16_t* __restrict__ pack) {
-    const long total = 9L * K16 * N16 * 256;
+    const long total = (long)K16 * 5 * N16 * 512;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    const int j = (int)(idx & 3), lane = (int)((idx >> 2) & 63);
-    long t = idx >> 8;
+    const int j = (int)(idx & 7), lane = (int)((idx >> 3) & 63);
+    long t = idx >> 9;
     const int nb = (int)(t % N16);
     t /= N16;
-    const int kb = (int)(t % K16);
-    const int tap = (int)(t / K16);
-    const int k = kb * 16 + 4 * (lane >> 4) + j, n = nb * 16 + (lane & 15);
+    const int ks = (int)(t % 5);
+    const int chunk = (int)(t / 5);
+    const int g = lane >> 4;
+    const int tap = 2 * ks + (g >> 1);
+    const int k = chunk * 16 + 8 * (g & 1) + j, n = nb * 16 + (lane & 15);
     float v = 0.f;
-    if (!transposed) {
-        if (k < CIN && n < COUT) v = w[((size_t)tap * CIN + k) * COUT + n];
-    } else {
-        if (k < COUT && n < CIN) v = w[((size_t)(8 - tap) * CIN + n) * COUT + k];
+    if (tap < 9) {
+        if (!transposed) {
+            if (k < CIN && n < COUT) v = w[((size_t)tap * CIN + k) * COUT + n];
+        } else {
+            if (k < COUT && n < CIN) v = w[((size_t)(8 - tap) * CIN + n) * COUT + k];
+        }
     }
     pack[idx] = f2bf(v);
 }
@@ -343,7 +468,7 @@ int psinn_conv3x3(const uint16_t* in1, int C1, int up1, const uint16_t* in2, int
     if ((C1 & 15) || (C2 & 15) || (cout_pad & 15) || C1 <= 0 || B <= 0 || H <= 0 || W <= 0 ||
         (up1 && ((H & 1) || (W & 1))) || (!out16 && !out32) || cout_store > cout_pad)
         return 5;
-    const dim3 grid((W + 15) / 16, (H + 3) / 4, B), blk(256);
+    const dim3 grid((W + CV_TW - 1) / CV_TW, (H + CV_TH - 1) / CV_TH, B), blk(256);
     hipStream_t s = (hipStream_t)stream;
     switch (cout_pad / 16) {
         case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
@@ -374,7 +499,7 @@ int psinn_conv3x3_wgrad(const uint16_t* in1, int C1, int up1, const uint16_t* in
 
 int psinn_pack_weights(const float* w, int cin, int cout, int transposed, int k16, int n16,
                        uint16_t* pack, void* stream) {
-    const long total = 9L * k16 * n16 * 256;
+    const long total = (long)k16 * 5 * n16 * 512;
     hipLaunchKernelGGL(k_pack_weights, dim3(nb256(total)), dim3(256), 0, (hipStream_t)stream, w,
                        cin, cout, transposed, k16, n16, pack);
     return launch_status();

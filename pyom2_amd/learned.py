@@ -76,6 +76,11 @@ LAYERS = [
 N_INPUTS = 3   # residual, inverse depth, mask
 
 
+def pack_elems(k16, n16):
+    """bf16 elements of packed convolution weights (psinn_pack_weights)."""
+    return k16 * n16 * 2560
+
+
 class PrecondNet:
     """Parameters, activations and the forward/backward passes."""
 
@@ -99,9 +104,9 @@ class PrecondNet:
                 d[ly.name] = torch.zeros_like(self.w[ly.name])
             for d in (self.gb, self.mb, self.vb):
                 d[ly.name] = torch.zeros_like(self.b[ly.name])
-            self.pack_f[ly.name] = torch.empty(9 * (ly.cin // 16) * (ly.cout_pad // 16) * 256,
+            self.pack_f[ly.name] = torch.empty(pack_elems(ly.cin // 16, ly.cout_pad // 16),
                                                dtype=torch.int16, device=self.dev)
-            self.pack_t[ly.name] = torch.empty(9 * (ly.cout_pad // 16) * (ly.cin // 16) * 256,
+            self.pack_t[ly.name] = torch.empty(pack_elems(ly.cout_pad // 16, ly.cin // 16),
                                                dtype=torch.int16, device=self.dev)
         self.step_count = 0
         self.repack()

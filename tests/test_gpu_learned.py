@@ -38,8 +38,8 @@ def cos(a, b):
 def test_mfma_lane_layout():
     L = _lib.lib()
     g = torch.Generator().manual_seed(0)
-    A = torch.randn(16, 16, generator=g).to(torch.bfloat16)
-    B = torch.randn(16, 16, generator=g).to(torch.bfloat16)
+    A = torch.randn(16, 32, generator=g).to(torch.bfloat16)
+    B = torch.randn(32, 16, generator=g).to(torch.bfloat16)
     D = torch.zeros(16, 16, device=DEV)
     Ad, Bd = A.to(DEV).contiguous(), B.to(DEV).contiguous()
     assert L.psinn_mfma_probe(p(Ad), p(Bd), p(D), None) == 0
@@ -73,7 +73,11 @@ def _torch_conv(in1, in2, w, b, up1, relu, cout):
 
 @pytest.mark.parametrize("case", [(2, 16, 24, 32, True, 16, 48, True),
                                   (1, 8, 40, 16, False, 0, 32, True),
-                                  (2, 12, 16, 64, True, 32, 32, False)])
+                                  (2, 12, 16, 64, True, 32, 32, False),
+                                  # several 4 x 64 tiles with ragged edges; 192 input
+                                  # channels (the data gradient has 12 output blocks)
+                                  (1, 10, 136, 128, True, 64, 64, True),
+                                  (1, 9, 70, 16, False, 0, 16, True)])
 def test_conv3x3_forward_and_gradients(case):
     Bn, H, W, c1, up1, c2, cout, relu = case
     L = _lib.lib()
@@ -81,7 +85,7 @@ def test_conv3x3_forward_and_gradients(case):
     cin = c1 + c2
     d1, d2 = in1.to(DEV).contiguous(), (in2.to(DEV).contiguous() if in2 is not None else None)
     wd, bd = w.to(DEV).contiguous(), b.to(DEV).contiguous()
-    pf = torch.empty(9 * (cin // 16) * (cpad // 16) * 256, dtype=torch.int16, device=DEV)
+    pf = torch.empty(learned.pack_elems(cin // 16, cpad // 16), dtype=torch.int16, device=DEV)
     pt = torch.empty_like(pf)
     assert L.psinn_pack_weights(p(wd), cin, cpad, 0, cin // 16, cpad // 16, p(pf), None) == 0
     assert L.psinn_pack_weights(p(wd), cin, cpad, 1, cpad // 16, cin // 16, p(pt), None) == 0
