@@ -23,8 +23,9 @@ extern "C" {
 int psinn_mfma_probe(const uint16_t *A, const uint16_t *B, float *D, void *stream);
 
 /* 3x3 conv, zero padding, input = [in1 (C1, nearest x2 upsampled if up1) |
- * in2 (C2)], output cout_store channels of cout_pad computed (bf16 out16 or
- * fp32 out32), + bias, optional ReLU.  wpack from psinn_pack_weights. */
+ * in2 (C2)], output cout_store channels of cout_pad computed (bf16 out16,
+ * cout_store a multiple of 16, or fp32 out32), + bias, optional ReLU.  wpack
+ * from psinn_pack_weights. */
 int psinn_conv3x3(const uint16_t *in1, int C1, int up1, const uint16_t *in2, int C2, int B, int H,
                   int W, const uint16_t *wpack, int cout_pad, const float *bias, int relu,
                   uint16_t *out16, float *out32, int cout_store, void *stream);

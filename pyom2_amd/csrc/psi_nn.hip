@@ -85,12 +85,15 @@ struct ConvShape {
     static constexpr int PER_W = (W16 + 255) / 256;
 };
 
+// Workgroups of a small image may split the output blocks: blockIdx.z =
+// image * groups + group, the group computing blocks [group NB, group NB +
+// NB) of nb_total.
 template <int NB>
 __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in1, int C1, int up1,
                                                  const uint16_t* __restrict__ in2, int C2, int H,
                                                  int W, const uint16_t* __restrict__ wpack,
-                                                 const float* __restrict__ bias, int relu,
-                                                 uint16_t* __restrict__ out16,
+                                                 int nb_total, const float* __restrict__ bias,
+                                                 int relu, uint16_t* __restrict__ out16,
                                                  float* __restrict__ out32, int cout_store) {
     using Sh = ConvShape<NB>;
     constexpr int MR = Sh::MR, PER_IN = Sh::PER_IN, PER_W = Sh::PER_W, W16 = Sh::W16;
@@ -99,7 +102,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
     __shared__ uint4 sW[W16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int x0 = blockIdx.x * CV_TW, y0 = blockIdx.y * CV_TH;
-    const int bimg = blockIdx.z;
+    const int groups = nb_total / NB;
+    const int bimg = blockIdx.z / groups, nb0 = (blockIdx.z % groups) * NB;
     const int nchunk = (C1 + C2) >> 4;
     const int H1 = up1 ? (H >> 1) : H, W1 = up1 ? (W >> 1) : W;
     const uint16_t* img1 = in1 + (size_t)bimg * H1 * W1 * C1;
@@ -132,11 +136,14 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
         }
     };
     auto load_w = [&](int ch, uint4 (&v)[PER_W]) {
-        const uint4* src = reinterpret_cast<const uint4*>(wpack) + (size_t)ch * W16;
+        // this group's blocks of each K step: NB x 64 consecutive units
+        const uint4* src = reinterpret_cast<const uint4*>(wpack) + (size_t)ch * 5 * nb_total * 64;
 #pragma unroll
         for (int k = 0; k < PER_W; ++k) {
             const int u = tid + k * 256;
-            v[k] = u < W16 ? src[u] : make_uint4(0u, 0u, 0u, 0u);
+            const int ks = u / (NB * 64), rem = u - ks * (NB * 64);
+            v[k] = u < W16 ? src[(size_t)(ks * nb_total + nb0) * 64 + rem]
+                           : make_uint4(0u, 0u, 0u, 0u);
         }
     };
     auto store = [&](const uint4 (&vi)[PER_IN], const uint4 (&vw)[PER_W]) {
@@ -209,24 +216,52 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
                 }
             }
         }
-        const int y = y0 + wave;
-        if (y < H) {
+        // the output through LDS, one 16-channel block at a time (the input
+        // frame's LDS is free: all MFMAs have read it): rows [pixel][16 ch]
+        // in fp32, then written as 16-byte bf16 units or as consecutive
+        // fp32 channels of consecutive pixels
+        // (staged in bf16 for a bf16 output, in fp32 for an fp32 one)
+        float* sOut = reinterpret_cast<float*>(sIn);
+        uint16_t* sOut16 = reinterpret_cast<uint16_t*>(sIn);
+        constexpr int TPX = CV_TH * MR * 16;   // pixels of this pass
+        static_assert(TPX * 16 * 4 <= (int)sizeof(sIn), "output staging fits the frame's LDS");
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                const int co = nb * 16 + pl;
-                const float bv = bias ? bias[co] : 0.f;
+        for (int nb = 0; nb < NB; ++nb) {
+            const int cb = (nb0 + nb) * 16;
+            const float bv = bias ? bias[cb + pl] : 0.f;
+            __syncthreads();
 #pragma unroll
-                for (int m = 0; m < MR; ++m)
+            for (int m = 0; m < MR; ++m)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int px = x0 + (pass * MR + m) * 16 + 4 * g + i;
-                        if (px >= W || co >= cout_store) continue;
-                        float v = acc[m][nb][i] + bv;
-                        if (relu) v = fmaxf(v, 0.f);
-                        const size_t o = (((size_t)bimg * H + y) * W + px) * cout_store + co;
-                        if (out32) out32[o] = v;
-                        else out16[o] = f2bf(v);
-                    }
+                for (int i = 0; i < 4; ++i) {
+                    float v = acc[m][nb][i] + bv;
+                    if (relu) v = fmaxf(v, 0.f);
+                    const int q = ((wave * MR + m) * 16 + 4 * g + i) * 16 + pl;
+                    if (out16) sOut16[q] = f2bf(v);
+                    else sOut[q] = v;
+                }
+            __syncthreads();
+            const int nval = min(16, cout_store - cb);
+            if (nval <= 0) continue;
+            if (out16) {
+                for (int u = tid; u < TPX * 2; u += 256) {
+                    const int tp = u >> 1, half = u & 1;
+                    const int wy = tp / (MR * 16), px = x0 + pass * MR * 16 + (tp - wy * MR * 16);
+                    const int yy = y0 + wy;
+                    if (yy < H && px < W)
+                        *reinterpret_cast<uint4*>(out16 + (((size_t)bimg * H + yy) * W + px) *
+                                                              cout_store + cb + 8 * half) =
+                            reinterpret_cast<const uint4*>(sOut16)[tp * 2 + half];
+                }
+            } else {
+                for (int u = tid; u < TPX * nval; u += 256) {
+                    const int tp = u / nval, c = u - tp * nval;
+                    const int wy = tp / (MR * 16), px = x0 + pass * MR * 16 + (tp - wy * MR * 16);
+                    const int yy = y0 + wy;
+                    if (yy < H && px < W)
+                        out32[(((size_t)bimg * H + yy) * W + px) * cout_store + cb + c] =
+                            sOut[tp * 16 + c];
+                }
             }
         }
     }
@@ -330,21 +365,30 @@ __global__ void k_pack_weights(const float* __restrict__ w, int CIN, int COUT, i
     pack[idx] = f2bf(v);
 }
 
-// 2x2 average pool, bf16 NHWC.
+// 2x2 average pool, bf16 NHWC, 8 channels (16 bytes) per thread.
 __global__ void k_avgpool2(const uint16_t* __restrict__ in, int B, int H, int W, int C,
                            uint16_t* __restrict__ out) {
-    const long n = (long)B * (H / 2) * (W / 2) * C;
+    const int C8 = C >> 3;
+    const long n = (long)B * (H / 2) * (W / 2) * C8;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    const int c = (int)(idx % C);
-    long t = idx / C;
+    const int c = (int)(idx % C8) * 8;
+    long t = idx / C8;
     const int x = (int)(t % (W / 2));
     t /= (W / 2);
     const int y = (int)(t % (H / 2));
     const int b = (int)(t / (H / 2));
     const uint16_t* p = in + (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
-    const float s = bf2f(p[0]) + bf2f(p[C]) + bf2f(p[(size_t)W * C]) + bf2f(p[(size_t)W * C + C]);
-    out[idx] = f2bf(0.25f * s);
+    uint16_t v[4][8], o[8];
+    *reinterpret_cast<uint4*>(v[0]) = *reinterpret_cast<const uint4*>(p);
+    *reinterpret_cast<uint4*>(v[1]) = *reinterpret_cast<const uint4*>(p + C);
+    *reinterpret_cast<uint4*>(v[2]) = *reinterpret_cast<const uint4*>(p + (size_t)W * C);
+    *reinterpret_cast<uint4*>(v[3]) = *reinterpret_cast<const uint4*>(p + (size_t)W * C + C);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        o[k] = f2bf(0.25f * (((bf2f(v[0][k]) + bf2f(v[1][k])) + bf2f(v[2][k])) + bf2f(v[3][k])));
+    *reinterpret_cast<uint4*>(out + (((size_t)b * (H / 2) + y) * (W / 2) + x) * C + c) =
+        *reinterpret_cast<const uint4*>(o);
 }
 
 // Sum of the 4 fine children (gradient of a nearest x2 upsampling) for the
@@ -466,20 +510,32 @@ int psinn_conv3x3(const uint16_t* in1, int C1, int up1, const uint16_t* in2, int
                   int W, const uint16_t* wpack, int cout_pad, const float* bias, int relu,
                   uint16_t* out16, float* out32, int cout_store, void* stream) {
     if ((C1 & 15) || (C2 & 15) || (cout_pad & 15) || C1 <= 0 || B <= 0 || H <= 0 || W <= 0 ||
-        (up1 && ((H & 1) || (W & 1))) || (!out16 && !out32) || cout_store > cout_pad)
+        (up1 && ((H & 1) || (W & 1))) || (!out16 && !out32) || cout_store > cout_pad ||
+        (out16 && (cout_store & 15)))
         return 5;
-    const dim3 grid((W + CV_TW - 1) / CV_TW, (H + CV_TH - 1) / CV_TH, B), blk(256);
+    const int NB = cout_pad / 16;
+    const long tiles = (long)((W + CV_TW - 1) / CV_TW) * ((H + CV_TH - 1) / CV_TH) * B;
+    // a small image with many output blocks: split them over workgroups
+    int sub = NB;
+    if (NB > 4 && tiles < 1024) sub = (NB % 4 == 0) ? 4 : (NB % 3 == 0 ? 3 : NB);
+    const dim3 grid((W + CV_TW - 1) / CV_TW, (H + CV_TH - 1) / CV_TH, B * (NB / sub)), blk(256);
     hipStream_t s = (hipStream_t)stream;
-    switch (cout_pad / 16) {
-        case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
-        case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
-        case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
-        case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
-        case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
-        case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
-        case 12: hipLaunchKernelGGL(k_conv3x3<12>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, bias, relu, out16, out32, cout_store); break;
+#define CONV_CASE(K)                                                                          \
+    case K:                                                                                   \
+        hipLaunchKernelGGL(k_conv3x3<K>, grid, blk, 0, s, in1, C1, up1, in2, C2, H, W, wpack, \
+                           NB, bias, relu, out16, out32, cout_store);                         \
+        break;
+    switch (sub) {
+        CONV_CASE(1)
+        CONV_CASE(2)
+        CONV_CASE(3)
+        CONV_CASE(4)
+        CONV_CASE(6)
+        CONV_CASE(8)
+        CONV_CASE(12)
         default: return 5;
     }
+#undef CONV_CASE
     return launch_status();
 }
 
@@ -506,7 +562,8 @@ int psinn_pack_weights(const float* w, int cin, int cout, int transposed, int k1
 }
 
 int psinn_avgpool2(const uint16_t* in, int B, int H, int W, int C, uint16_t* out, void* stream) {
-    const long n = (long)B * (H / 2) * (W / 2) * C;
+    if (C & 7) return 5;
+    const long n = (long)B * (H / 2) * (W / 2) * (C / 8);
     hipLaunchKernelGGL(k_avgpool2, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream, in, B, H, W,
                        C, out);
     return launch_status();

@@ -147,6 +147,28 @@ def evaluate(net, n, seed=11, relres=1e-10, modes=("learned", "mixed", "plain"))
     return res
 
 
+def time_forward(n, reps=10, weights=DEFAULT_WEIGHTS):
+    """Average time of the network's forward pass on an n x n solver grid's
+    network layout (random input), HIP events around reps passes."""
+    net = learned.PrecondNet.load(weights)
+    H = (n + 4 + 7) // 8 * 8
+    x = torch.randn(1, H, H, 16, device=net.dev).to(learned.BF16)
+    net.forward(x, 1, H, H)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(net.stream):
+        e0.record()
+        for _ in range(reps):
+            net.forward(x, 1, H, H)
+        e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flop = sum(2 * 9 * ly.cin * ly.cout_pad * (H >> lvl) ** 2
+               for ly, lvl in zip(learned.LAYERS, (0, 1, 2, 3, 2, 1, 0, 0)))
+    log(f"[forward] {H}x{H}: {ms:.3f} ms, {flop / 1e9:.1f} GFLOP, "
+        f"{flop / (ms * 1e-3) / 1e12:.0f} TFLOP/s")
+    return ms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=3000)
@@ -165,7 +187,12 @@ def main():
     ap.add_argument("--pairs", default="plain", choices=sorted(PAIR_ITERS),
                     help="iterates the (residual, error) pairs are harvested from")
     ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--time-forward", type=int, default=0,
+                    help="only time the network's forward pass on an n x n grid")
     args = ap.parse_args()
+    if args.time_forward:
+        time_forward(args.time_forward)
+        return
     if args.eval_only:
         net = learned.PrecondNet.load(args.out)
     else:
