@@ -156,6 +156,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_mib:
+        import torch
+        calibration_copy(args.copy_mib, torch.device("cuda:0"))
     if args.workload == "ensemble":
         return ensemble_main(args)
     if args.workload == "step":
@@ -170,7 +173,6 @@ def main():
     from pyom2_amd.streamfunction import SolveOptions, StreamfunctionSolver, islands_trace
 
     dev = torch.device("cuda:0")
-    calibration_copy(args.copy_mib, dev)
     t = time.time()
     # the 2048^2 basin's domain (102.4 deg of latitude about the equator) at
     # any resolution: spacing 0.05 deg * 2048 / n

@@ -676,8 +676,17 @@ __device__ __forceinline__ void poly_run(const BMember& m, const BArrays& A, con
 // The batched step with the degree-KP polynomial preconditioner in the FP32
 // legs (the member's tiles are those of the poly tiling, (64 - 2R) x
 // (4 RPT - 2R)); work distribution and state machine as k_batch_step.
+// Waves per SIMD the register allocation must allow.  The register-resident
+// polynomial tile (8 arrays of RPT rows per thread) and the member state
+// machine take 236 VGPRs at degree 4, so 2 waves per SIMD; forcing 3 or 4
+// spills 64 / 122 VGPRs to scratch and measured 17% / 44% fewer ensemble
+// solves per second at 512 members.
+#ifndef PSI_BATCH_POLY_WAVES
+#define PSI_BATCH_POLY_WAVES 1
+#endif
 template <int KP>
-__global__ __launch_bounds__(NT) void k_batch_poly(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PSI_BATCH_POLY_WAVES)))
+void k_batch_poly(
     const BMember* __restrict__ mem, const int2* __restrict__ items,
     const int* __restrict__ nitems_p, BArrays A, BParams P, const BState* __restrict__ st_in,
     BState* __restrict__ st_out, const double* __restrict__ part_in,
