@@ -160,52 +160,60 @@ __device__ __forceinline__ int mg_frame_cell(const MgLev& L, int I, int J) {
     return (J + 1) * L.pitch + I + MG_XOFF;
 }
 
-// S v at frame index f (row stride FX)
-template <int FX>
-__device__ __forceinline__ float mg_fs(const float* v, const float* sE, const float* sN,
-                                       const float* sC, int f) {
-    return sC[f] * v[f] + sE[f] * v[f + 1] + sE[f - 1] * v[f - 1] + sN[f] * v[f + FX] +
-           sN[f - FX] * v[f - FX];
-}
-
 // Pre-smoothing from x = 0 (K Chebyshev steps), the residual, and its 2 x 2
-// restriction to the next level's right-hand side.  Each thread keeps r and
-// x of its frame cells in registers; only the direction d, whose neighbours
-// the next step reads, goes through LDS (double-buffered: one barrier per
-// step).
+// restriction to the next level's right-hand side.  Each thread keeps, per
+// frame cell, r, x, the direction d, the cell's five couplings (its own C,
+// E, N and its west and south neighbours' E and N) and 1/C in registers;
+// only d, whose neighbours the next step reads, goes through LDS
+// (double-buffered: one barrier per step), so a step costs four LDS reads
+// and one write per cell.
 template <int K, int TY_>
 __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
                                                        const MgScal* __restrict__ S) {
     using F = MgFrame<K, TY_>;
     constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
-    __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
+    __shared__ float sE[FN], sN[FN], sD[2][FN];
     const int done = S->done;   // read beside the frame loads, checked before any store
     const int tiles_x = (L.nx + MT_X - 1) / MT_X;
     // each XCD sweeps a contiguous band of tiles, so the frame margins shared
     // with the neighbouring tiles are read through the same L2
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int I0 = (tile % tiles_x) * MT_X, J0 = (tile / tiles_x) * TY;
-    float r[PER], x[PER];
+    float r[PER], x[PER], d[PER], cC[PER], cE[PER], cN[PER], cW[PER], cS[PER], di[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int f = threadIdx.x + u * NT;
-        r[u] = 0.0f;
-        x[u] = 0.0f;
+        r[u] = x[u] = d[u] = cE[u] = cN[u] = cW[u] = cS[u] = 0.0f;
+        cC[u] = di[u] = 1.0f;
         if (f < FN) {
             const int fy = f / FX, fx = f - fy * FX;
             const int q = mg_frame_cell(L, I0 - K + fx, J0 - K + fy);
             const float b = q >= 0 ? L.b[q] : 0.0f, c = q >= 0 ? L.C[q] : 1.0f;
-            sE[f] = q >= 0 ? L.E[q] : 0.0f;
-            sN[f] = q >= 0 ? L.N[q] : 0.0f;
-            sC[f] = c;
-            const float d0 = b * ch.c0 / c;
+            const float e = q >= 0 ? L.E[q] : 0.0f, n = q >= 0 ? L.N[q] : 0.0f;
+            sE[f] = e;
+            sN[f] = n;
+            cC[u] = c;
+            cE[u] = e;
+            cN[u] = n;
+            di[u] = 1.0f / c;
+            const float d0 = b * ch.c0 * di[u];
             sD[0][f] = d0;
             r[u] = b;
             x[u] = d0;
+            d[u] = d0;
         }
     }
     if (done) return;
     __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * NT;
+        const int fy = f / FX, fx = f - fy * FX;
+        if (f < FN && fx >= 1 && fy >= 1) {
+            cW[u] = sE[f - 1];
+            cS[u] = sN[f - FX];
+        }
+    }
     // steps 1..K-1 on margins >= k; step K: the residual on the tile
 #pragma unroll
     for (int k = 1; k <= K; ++k) {
@@ -216,11 +224,14 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb c
             const int f = threadIdx.x + u * NT;
             const int fy = f / FX, fx = f - fy * FX;
             if (f < FN && fx >= k && fx < FX - k && fy >= k && fy < FY - k) {
-                const float rn = r[u] - mg_fs<FX>(din, sE, sN, sC, f);
+                const float sv = cC[u] * d[u] + cE[u] * din[f + 1] + cW[u] * din[f - 1] +
+                                 cN[u] * din[f + FX] + cS[u] * din[f - FX];
+                const float rn = r[u] - sv;
                 r[u] = rn;
                 if (k < K) {
-                    const float dn = ch.c1[k] * din[f] + ch.c2[k] * (rn / sC[f]);
+                    const float dn = ch.c1[k] * d[u] + ch.c2[k] * (rn * di[u]);
                     dout[f] = dn;
+                    d[u] = dn;
                     x[u] = x[u] + dn;
                 }
             }
@@ -248,8 +259,8 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb c
         if (IC < Lc.nx && JC < Lc.ny) {
             float sum = 0.0f;
             for (int dj = 0; dj < 2; ++dj)
-                for (int di = 0; di < 2; ++di) {
-                    const int tx = 2 * cx + di, ty = 2 * cy + dj;
+                for (int di2 = 0; di2 < 2; ++di2) {
+                    const int tx = 2 * cx + di2, ty = 2 * cy + dj;
                     if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sRes[(ty + K) * FX + tx + K];
                 }
             const int Q = (JC + 1) * Lc.pitch + IC + MG_XOFF;
@@ -260,26 +271,26 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb c
 
 // Prolongation of the coarse correction (weight ch.cscale, active cells) and
 // K Chebyshev steps of post-smoothing; the result goes to y.  y' (needed
-// with its neighbours once) and then d go through LDS, r and y stay in
-// registers.
+// with its neighbours once) and then d go through LDS; r, y, d, the cell's
+// couplings and 1/C stay in registers.
 template <int K, int TY_>
 __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
                                                         const MgScal* __restrict__ S) {
     using F = MgFrame<K, TY_>;
     constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
-    __shared__ float sE[FN], sN[FN], sC[FN], sD[2][FN];
+    __shared__ float sE[FN], sN[FN], sD[2][FN];
     const int done = S->done;
     const int tiles_x = (L.nx + MT_X - 1) / MT_X;
     // each XCD sweeps a contiguous band of tiles, so the frame margins shared
     // with the neighbouring tiles are read through the same L2
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int I0 = (tile % tiles_x) * MT_X, J0 = (tile / tiles_x) * TY;
-    float r[PER], y[PER];
+    float r[PER], y[PER], d[PER], cC[PER], cE[PER], cN[PER], cW[PER], cS[PER], di[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int f = threadIdx.x + u * NT;
-        r[u] = 0.0f;
-        y[u] = 0.0f;
+        r[u] = y[u] = d[u] = cE[u] = cN[u] = cW[u] = cS[u] = 0.0f;
+        cC[u] = di[u] = 1.0f;
         if (f < FN) {
             const int fy = f / FX, fx = f - fy * FX;
             int I = I0 - K + fx;
@@ -290,9 +301,14 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev L
                 if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
                 yv = L.x[q] + ch.cscale * Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
             }
-            sE[f] = q >= 0 ? L.E[q] : 0.0f;
-            sN[f] = q >= 0 ? L.N[q] : 0.0f;
-            sC[f] = q >= 0 ? L.C[q] : 1.0f;
+            const float c = q >= 0 ? L.C[q] : 1.0f;
+            const float e = q >= 0 ? L.E[q] : 0.0f, n = q >= 0 ? L.N[q] : 0.0f;
+            sE[f] = e;
+            sN[f] = n;
+            cC[u] = c;
+            cE[u] = e;
+            cN[u] = n;
+            di[u] = 1.0f / c;
             sD[1][f] = yv;                       // y' for step 0's stencil
             r[u] = q >= 0 ? L.b[q] : 0.0f;       // b
             y[u] = yv;
@@ -306,10 +322,16 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev L
         const int f = threadIdx.x + u * NT;
         const int fy = f / FX, fx = f - fy * FX;
         if (f < FN && fx >= 1 && fx < FX - 1 && fy >= 1 && fy < FY - 1) {
-            const float rn = r[u] - mg_fs<FX>(sD[1], sE, sN, sC, f);
-            const float dn = ch.c0 * (rn / sC[f]);
+            const float* v = sD[1];
+            cW[u] = sE[f - 1];
+            cS[u] = sN[f - FX];
+            const float sv = cC[u] * y[u] + cE[u] * v[f + 1] + cW[u] * v[f - 1] +
+                             cN[u] * v[f + FX] + cS[u] * v[f - FX];
+            const float rn = r[u] - sv;
+            const float dn = ch.c0 * (rn * di[u]);
             r[u] = rn;
             sD[0][f] = dn;
+            d[u] = dn;
             y[u] = y[u] + dn;
         }
     }
@@ -324,10 +346,13 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev L
             const int f = threadIdx.x + u * NT;
             const int fy = f / FX, fx = f - fy * FX;
             if (f < FN && fx >= m && fx < FX - m && fy >= m && fy < FY - m) {
-                const float rn = r[u] - mg_fs<FX>(din, sE, sN, sC, f);
-                const float dn = ch.c1[k] * din[f] + ch.c2[k] * (rn / sC[f]);
+                const float sv = cC[u] * d[u] + cE[u] * din[f + 1] + cW[u] * din[f - 1] +
+                                 cN[u] * din[f + FX] + cS[u] * din[f - FX];
+                const float rn = r[u] - sv;
+                const float dn = ch.c1[k] * d[u] + ch.c2[k] * (rn * di[u]);
                 r[u] = rn;
                 dout[f] = dn;
+                d[u] = dn;
                 y[u] = y[u] + dn;
             }
         }
