@@ -1244,7 +1244,7 @@ void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* 
     MgState* m = c->mg;
     mg_vcycle(m, s, K, ch, ev, co);
     const Geom& g = c->g;
-    hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+    hipLaunchKernelGGL(mg_pass_a(g), dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
                        (const double*)c->E, (const double*)c->N, (const double*)c->rx,
                        (const double*)c->ry, (const double*)c->dxu, (const double*)c->cosu,
                        (const double*)c->dyu, (const uint8_t*)c->act,
@@ -1254,7 +1254,7 @@ void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* 
     hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->S);
     if (ev) ev->rec(5, s);
-    hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+    hipLaunchKernelGGL(mg_pass_b(g), dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
                        (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
                        (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
                        (const double*)m->zscale, (const MgScal*)m->S, m->part);
@@ -1511,7 +1511,7 @@ int psi_mg_kernel_times(psi_ctx* c, int32_t degree, int32_t reps, double* us, in
     }
     const Geom& g = c->g;
     us[2] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
+        hipLaunchKernelGGL(mg_pass_a(g), dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
                            (const double*)c->E, (const double*)c->N, (const double*)c->rx,
                            (const double*)c->ry, (const double*)c->dxu, (const double*)c->cosu,
                            (const double*)c->dyu, (const uint8_t*)c->act, (const double*)m->q,
@@ -1519,7 +1519,7 @@ int psi_mg_kernel_times(psi_ctx* c, int32_t degree, int32_t reps, double* us, in
                            (const MgScal*)m->S_time, m->part);
     });
     us[3] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(k_mg_fcg_b, dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
+        hipLaunchKernelGGL(mg_pass_b(g), dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
                            (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
                            (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
                            (const double*)m->zscale, (const MgScal*)m->S_time, m->part);

@@ -578,6 +578,168 @@ __global__ void __launch_bounds__(1024) k_mg_fold_b(const double* __restrict__ p
     }
 }
 
+// ---- the outer passes on pairs of points -----------------------------------
+// The same two passes with each thread taking the points (i, i+1), i even,
+// of a 512-point row segment through 16-byte loads and stores (used when nx
+// is even, so every row of the solver layout, pitch nx + 4, starts 16-byte
+// aligned and no row ends in a single point).  Per point the arithmetic and
+// its order are those of k_mg_fcg_a / k_mg_fcg_b.
+struct MgZ2 {
+    double c0, c1;   // z at (i, j), (i+1, j)
+    double w0, e1;   // z west of i, east of i+1
+    double s0, s1, n0, n1;
+};
+
+__device__ __forceinline__ MgZ2 mg_z_pair(const Geom& g, const MgLev& L0, int i, int j,
+                                         double inv) {
+    const int P = L0.pitch, q = (j + 1) * P + i + MG_XOFF;   // even: 8-byte aligned pairs
+    int dw = -1, de = 2;
+    if (g.cyclic) {
+        if (i == 0) dw = g.nx - 1;
+        if (i + 1 == g.nx - 1) de = 2 - g.nx;
+    }
+    const float2 yc = *reinterpret_cast<const float2*>(L0.y + q);
+    const float2 ys = *reinterpret_cast<const float2*>(L0.y + q - P);
+    const float2 yn = *reinterpret_cast<const float2*>(L0.y + q + P);
+    const float yw = L0.y[q + dw], ye = L0.y[q + de];
+    const bool hs = j > 0, hn = j < g.ny - 1;
+    MgZ2 z;
+    z.c0 = (double)yc.x * inv;
+    z.c1 = (double)yc.y * inv;
+    z.s0 = hs ? (double)ys.x * inv : 0.0;
+    z.s1 = hs ? (double)ys.y * inv : 0.0;
+    z.n0 = hn ? (double)yn.x * inv : 0.0;
+    z.n1 = hn ? (double)yn.y * inv : 0.0;
+    z.w0 = (g.cyclic || i > 0) ? (double)yw * inv : 0.0;
+    z.e1 = (g.cyclic || i + 2 < g.nx) ? (double)ye * inv : 0.0;
+    return z;
+}
+
+__global__ void __launch_bounds__(256) k_mg_fcg_a2(
+    Geom g, MgLev L0, const double* __restrict__ E, const double* __restrict__ N,
+    const double* __restrict__ rx, const double* __restrict__ ry,
+    const double* __restrict__ dxu, const double* __restrict__ cosu,
+    const double* __restrict__ dyu, const uint8_t* __restrict__ act,
+    const double* __restrict__ q, const double* __restrict__ r, double* __restrict__ az_out,
+    const double* __restrict__ zscale, const MgScal* __restrict__ S,
+    double* __restrict__ part) {
+    if (S->done) return;
+    const double inv = 1.0 / *zscale;   // the scale of the V-cycle's input
+    double acc[3] = {0, 0, 0};
+    // row segments of 512 points, each XCD a contiguous range of rows
+    const int segs = (g.nx + 511) / 512;
+    int first, stride, end;
+    range_schedule(g.ny * segs, first, stride, end);
+    for (int it = first; it < end; it += stride) {
+        const int j = it / segs, i = (it - j * segs) * 512 + 2 * threadIdx.x;
+        if (i >= g.nx) continue;
+        const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+        const int c = i + 2, rr = j + 2;
+        // every load in bounds, issued before the active test
+        const MgZ2 z = mg_z_pair(g, L0, i, j, inv);
+        const double2 e2 = *reinterpret_cast<const double2*>(E + o);
+        const double2 n2 = *reinterpret_cast<const double2*>(N + o);
+        const double2 s2 = *reinterpret_cast<const double2*>(N + o - g.pitch);
+        const double2 q2 = *reinterpret_cast<const double2*>(q + o);
+        const double2 r2 = *reinterpret_cast<const double2*>(r + o);
+        const double em = E[o - 1], rx0 = rx[c], rx1 = rx[c + 1], ryv = ry[rr];
+        const double wa0 = dxu[c] * cosu[rr] * dyu[rr], wa1 = dxu[c + 1] * cosu[rr] * dyu[rr];
+        const uint16_t a2 = *reinterpret_cast<const uint16_t*>(act + o);
+        if (!a2) continue;
+        double2 az2 = make_double2(0.0, 0.0);
+        if (a2 & 0xff) {
+            const double e = e2.x, w = em * rx0, n = n2.x, so = s2.x * ryv;
+            const double cc = (((-e) - w) - n) - so;
+            const double az = cc * z.c0 + e * z.c1 + w * z.w0 + n * z.n0 + so * z.s0;
+            az2.x = az;
+            acc[0] += wa0 * z.c0 * az;
+            acc[1] += wa0 * z.c0 * q2.x;
+            acc[2] += wa0 * r2.x * z.c0;
+        }
+        if (a2 >> 8) {
+            const double e = e2.y, w = e2.x * rx1, n = n2.y, so = s2.y * ryv;
+            const double cc = (((-e) - w) - n) - so;
+            const double az = cc * z.c1 + e * z.e1 + w * z.c0 + n * z.n1 + so * z.s1;
+            az2.y = az;
+            acc[0] += wa1 * z.c1 * az;
+            acc[1] += wa1 * z.c1 * q2.y;
+            acc[2] += wa1 * r2.y * z.c1;
+        }
+        *reinterpret_cast<double2*>(az_out + o) = az2;
+    }
+    mg_block_partials<3>(acc, part);
+}
+
+__global__ void __launch_bounds__(256) k_mg_fcg_b2(
+    Geom g, MgLev L0, const double* __restrict__ dxu, const double* __restrict__ cosu,
+    const double* __restrict__ dyu, const uint8_t* __restrict__ act, double* __restrict__ p,
+    double* __restrict__ q, double* __restrict__ r, double* __restrict__ x,
+    const double* __restrict__ az_in, const double* __restrict__ zscale,
+    const MgScal* __restrict__ S, double* __restrict__ part) {
+    if (S->done) return;
+    const double inv = 1.0 / *zscale;
+    const double beta = S->beta, alpha = S->alpha, sr = S->s_r;
+    double acc[2] = {0, 0};
+    // row segments of 512 points, each XCD a contiguous range of rows
+    const int segs = (g.nx + 511) / 512;
+    int first, stride, end;
+    range_schedule(g.ny * segs, first, stride, end);
+    for (int it = first; it < end; it += stride) {
+        const int j = it / segs, i = (it - j * segs) * 512 + 2 * threadIdx.x;
+        if (i >= g.nx) continue;
+        const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+        const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
+        const int c = i + 2, rr = j + 2;
+        // every load in bounds, issued before the active test
+        const float2 y2 = *reinterpret_cast<const float2*>(L0.y + ql);
+        const double2 p2 = *reinterpret_cast<const double2*>(p + o);
+        const double2 q2 = *reinterpret_cast<const double2*>(q + o);
+        const double2 x2 = *reinterpret_cast<const double2*>(x + o);
+        const double2 r2 = *reinterpret_cast<const double2*>(r + o);
+        const double2 a2v = *reinterpret_cast<const double2*>(az_in + o);
+        const double w0 = dxu[c] * cosu[rr] * dyu[rr], w1 = dxu[c + 1] * cosu[rr] * dyu[rr];
+        const uint16_t a2 = *reinterpret_cast<const uint16_t*>(act + o);
+        float2 b2 = make_float2(0.0f, 0.0f);
+        if (a2) {
+            double2 pn = p2, qn = q2, xn = x2, rn = r2;
+            if (a2 & 0xff) {
+                const double z = (double)y2.x * inv;
+                pn.x = z + beta * p2.x;
+                qn.x = a2v.x + beta * q2.x;
+                xn.x = x2.x + alpha * pn.x;
+                rn.x = r2.x - alpha * qn.x;
+                b2.x = (float)(w0 * rn.x * sr);
+                acc[0] += rn.x * rn.x;
+                acc[1] += (w0 * rn.x) * (w0 * rn.x);
+            }
+            if (a2 >> 8) {
+                const double z = (double)y2.y * inv;
+                pn.y = z + beta * p2.y;
+                qn.y = a2v.y + beta * q2.y;
+                xn.y = x2.y + alpha * pn.y;
+                rn.y = r2.y - alpha * qn.y;
+                b2.y = (float)(w1 * rn.y * sr);
+                acc[0] += rn.y * rn.y;
+                acc[1] += (w1 * rn.y) * (w1 * rn.y);
+            }
+            *reinterpret_cast<double2*>(p + o) = pn;
+            *reinterpret_cast<double2*>(q + o) = qn;
+            *reinterpret_cast<double2*>(x + o) = xn;
+            *reinterpret_cast<double2*>(r + o) = rn;
+        }
+        *reinterpret_cast<float2*>(L0.b + ql) = b2;
+    }
+    mg_block_partials<2>(acc, part);
+}
+
+// The outer passes for a grid: on pairs of points when nx is even.
+inline decltype(&k_mg_fcg_a) mg_pass_a(const Geom& g) {
+    return (g.nx & 1) ? k_mg_fcg_a : k_mg_fcg_a2;
+}
+inline decltype(&k_mg_fcg_b) mg_pass_b(const Geom& g) {
+    return (g.nx & 1) ? k_mg_fcg_b : k_mg_fcg_b2;
+}
+
 // Start of a (re)started outer iteration from the residual r: the V-cycle
 // input w r s on level 0 with s = 1/rms(w r) (computed by the host).
 __global__ void __launch_bounds__(256) k_mg_start(Geom g, MgLev L0, const double* __restrict__ dxu,

@@ -242,3 +242,24 @@ def test_mg_cyclic_tall_tiles():
     rel, _ = _oracle_relres(b, isl, N(sol))
     assert rel <= 1.01e-10
     s.close()
+
+
+@pytest.mark.parametrize("nx,ny,cyclic", [(63, 50, False), (63, 50, True), (130, 66, False),
+                                          (130, 66, True), (514, 258, True)])
+def test_mg_odd_and_even_widths(nx, ny, cyclic):
+    """The outer passes run on pairs of points when nx is even and point by
+    point when it is odd; both, closed and cyclic (where the pair at the
+    seam wraps its west or east neighbour), reach the target on the FP64 true
+    residual recomputed on the CPU, and the 514-wide basin has frames that
+    wrap across the seam on its 64 x 32 tiles."""
+    print(f"\n[mg width {nx}x{ny} cyclic={cyclic}]", flush=True)
+    b = basins.beta_plane_basin(seed=nx + ny, nx=nx, ny=ny, n_islands=1, cyclic_x=cyclic)
+    isl = islands_trace(b.kbot, cyclic_x=cyclic)
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=DEV)
+    sol = s.zeros()
+    its, conv, info = s.congrad_streamfunction(T(b.forc), sol, mode="mg", criterion="relres",
+                                               relres_target=1e-10, max_iterations=500)
+    rel, _ = _oracle_relres(b, isl, N(sol))
+    print(f"  its={its} relres={info['relres_true']:.2e} oracle {rel:.2e}", flush=True)
+    assert conv and rel <= 1.01e-10
+    s.close()
