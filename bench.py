@@ -60,8 +60,8 @@ PRESS_MG_DEFAULT_DEGREE = 4
 
 
 def mg_kernel_names(degree):
-    return {"post_level0": f"k_mg_post<{degree}, 32> (finest level)",
-            "pre_level0": f"k_mg_pre<{degree}, 32> (finest level)",
+    return {"post_level0": f"k_mg_post<{degree}> (finest level)",
+            "pre_level0": f"k_mg_pre<{degree}> (finest level)",
             "fcg_a": "k_mg_fcg_a", "fcg_b": "k_mg_fcg_b"}
 # HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):

@@ -1123,46 +1123,47 @@ struct MgEvents {
     }
 };
 
-// The level kernels at level v's tile height.
+// The level kernels of level v at degree K (rows per lane by the level's size).
+template <int K, int R>
+void mg_launch_pre_r(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
+                     hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_pre<K, R>), dim3(mg_tiles(v, K, R)), dim3(MG_LEVEL_THREADS), 0, s,
+                       v, ch, vc, S);
+}
+
+template <int K, int R>
+void mg_launch_post_r(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
+                      hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_post<K, R>), dim3(mg_tiles(v, K, R)), dim3(MG_LEVEL_THREADS), 0, s,
+                       v, vc, ch, S);
+}
+
 template <int K>
 void mg_launch_pre(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
                    hipStream_t s) {
-    if (mg_tile_y(v) == MG_TY_LARGE) {
-        using F = MgFrame<K, MG_TY_LARGE>;
-        hipLaunchKernelGGL((k_mg_pre<K, MG_TY_LARGE>), dim3(mg_tiles(v, F::TY)), dim3(F::NT), 0,
-                           s, v, ch, vc, S);
-    } else {
-        using F = MgFrame<K, MG_TY_SMALL>;
-        hipLaunchKernelGGL((k_mg_pre<K, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT), 0,
-                           s, v, ch, vc, S);
-    }
+    if (mg_rows_per_lane(v, K) == MG_RPT_LARGE)
+        mg_launch_pre_r<K, MG_RPT_LARGE>(v, vc, ch, S, s);
+    else
+        mg_launch_pre_r<K, mg_rpt_small(K)>(v, vc, ch, S, s);
 }
 
 template <int K>
 void mg_launch_post(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
                     hipStream_t s) {
-    if (mg_tile_y(v) == MG_TY_LARGE) {
-        using F = MgFrame<K, MG_TY_LARGE>;
-        hipLaunchKernelGGL((k_mg_post<K, MG_TY_LARGE>), dim3(mg_tiles(v, F::TY)), dim3(F::NT),
-                           0, s, v, vc, ch, S);
-    } else {
-        using F = MgFrame<K, MG_TY_SMALL>;
-        hipLaunchKernelGGL((k_mg_post<K, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT),
-                           0, s, v, vc, ch, S);
-    }
+    if (mg_rows_per_lane(v, K) == MG_RPT_LARGE)
+        mg_launch_post_r<K, MG_RPT_LARGE>(v, vc, ch, S, s);
+    else
+        mg_launch_post_r<K, mg_rpt_small(K)>(v, vc, ch, S, s);
 }
 
-// A coarse level's kernel at degree Kc (64 x 16 tiles).
+// A coarse level's kernel at degree Kc.
 template <int Kc>
 void mg_launch_coarse(bool pre, const MgLev& v, const MgLev& vc, const MgCheb& ch,
                       const MgScal* S, hipStream_t s) {
-    using F = MgFrame<Kc, MG_TY_SMALL>;
     if (pre)
-        hipLaunchKernelGGL((k_mg_pre<Kc, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT), 0,
-                           s, v, ch, vc, S);
+        mg_launch_pre<Kc>(v, vc, ch, S, s);
     else
-        hipLaunchKernelGGL((k_mg_post<Kc, MG_TY_SMALL>), dim3(mg_tiles(v, F::TY)), dim3(F::NT),
-                           0, s, v, vc, ch, S);
+        mg_launch_post<Kc>(v, vc, ch, S, s);
 }
 
 void mg_launch_coarse(int Kc, bool pre, const MgLev& v, const MgLev& vc, const MgCheb& ch,

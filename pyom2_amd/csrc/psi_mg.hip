@@ -110,43 +110,61 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
 }
 
 // ---- fused level kernels ------------------------------------------------------
-// One workgroup per 64 x TY tile of a level with a K-cell frame around it in
-// LDS: the K smoothing steps recompute the frame's shrinking margins instead
-// of writing every step to memory, so a level costs two kernels per V-cycle
+// One workgroup per tile of a level with a K-cell frame around it: the K
+// smoothing steps recompute the frame's shrinking margins instead of writing
+// every step to memory, so a level costs two kernels per V-cycle
 // (pre-smoothing + restriction, prolongation + post-smoothing).
-// Tiles are MT_X wide and TY tall (16, or 32 on the largest levels, where
-// the frame's share of the reads drops from 1.69x to 1.41x the tile and
-// there are still at least 2 tiles per CU); at most 1024 threads, each
-// carrying two or three frame cells per step.
-constexpr int MT_X = 64;
-constexpr int MG_TY_SMALL = 16, MG_TY_LARGE = 32;
-
-template <int TY>
-struct MgThreads {
-    static constexpr int NT = MT_X * TY < 1024 ? MT_X * TY : 1024;
-};
-
-template <int K, int TY_>
-struct MgFrame {
-    static constexpr int TY = TY_, NT = MgThreads<TY_>::NT;
-    static constexpr int FX = MT_X + 2 * K, FY = TY + 2 * K, FN = FX * FY;
-    static constexpr int PER = (FN + NT - 1) / NT;
-};
-
-// Tile height of a level: the tall tile where it still leaves at least 512
-// tiles (2 per CU; measured 1% faster per iteration than keeping 4 per CU).
-inline int mg_tile_y(const MgLev& v) {
-    const long t = (long)((v.nx + MT_X - 1) / MT_X) * ((v.ny + MG_TY_LARGE - 1) / MG_TY_LARGE);
-    return t >= 512 ? MG_TY_LARGE : MG_TY_SMALL;
+//
+// The frame is 64 columns (one per lane) x MG_NW * RPT rows; wave w holds
+// rows [w RPT, (w+1) RPT) of every column in registers, with each cell's
+// couplings (its own C, E, N, its west neighbour's E and its south
+// neighbour's N) and 1/C.  A stencil takes the west and east neighbours from
+// the adjacent lanes (DPP whole-wave shifts), the rows inside the band from
+// registers, and only the rows just below and above the band from LDS (one
+// exchange, one barrier per step).  Every cell is updated at every step:
+// a cell at margin m is exact after step m, and garbage from the frame edge
+// (where a shift brings in zero) moves in by one cell per step, so the own
+// tile (margin >= K) is exact at the end.
+// Rows per lane: MG_RPT_LARGE on levels that keep at least MG_LARGE_TILES
+// tiles with it, MG_RPT_SMALL on the others, whose kernels are bound by
+// latency and gain from more, smaller workgroups.
+constexpr int MG_NW = 16, MG_FW = 64;
+constexpr int MG_LEVEL_THREADS = 64 * MG_NW;
+#ifndef PSI_MG_RPT_SMALL
+#define PSI_MG_RPT_SMALL 1
+#endif
+#ifndef PSI_MG_RPT_LARGE
+#define PSI_MG_RPT_LARGE 3
+#endif
+#ifndef PSI_MG_LARGE_TILES
+#define PSI_MG_LARGE_TILES 512
+#endif
+constexpr int MG_RPT_LARGE = PSI_MG_RPT_LARGE;
+constexpr long MG_LARGE_TILES = PSI_MG_LARGE_TILES;
+// the small levels' rows per lane at degree K: PSI_MG_RPT_SMALL where that
+// leaves a tile at least 4 rows tall, else the fewest that do
+constexpr int mg_rpt_small(int K) {
+    return MG_NW * PSI_MG_RPT_SMALL - 2 * K >= 4 ? PSI_MG_RPT_SMALL
+                                                 : (MG_NW * 2 - 2 * K >= 4 ? 2 : 3);
 }
 
+// Tile of a level kernel at degree K and RPT rows per lane:
+// (64 - 2K) x (MG_NW RPT - 2K), both even.
+inline int mg_tile_x(int K) { return MG_FW - 2 * K; }
+inline int mg_tile_h(int K, int rpt) { return MG_NW * rpt - 2 * K; }
+inline unsigned mg_tiles(const MgLev& v, int K, int rpt) {
+    return (unsigned)(((v.nx + mg_tile_x(K) - 1) / mg_tile_x(K)) *
+                      ((v.ny + mg_tile_h(K, rpt) - 1) / mg_tile_h(K, rpt)));
+}
+inline int mg_rows_per_lane(const MgLev& v, int K) {
+    return mg_tiles(v, K, MG_RPT_LARGE) >= MG_LARGE_TILES ? MG_RPT_LARGE : mg_rpt_small(K);
+}
+
+// A level is coarse when it has at most MG_COARSE_TILES 64 x 16 blocks of
+// cells (one per CU or fewer): its kernels are bound by launch latency, not
+// by their work.
 inline bool mg_coarse_level(const MgLev& v) {
-    return (long)((v.nx + MT_X - 1) / MT_X) * ((v.ny + MG_TY_SMALL - 1) / MG_TY_SMALL) <=
-           MG_COARSE_TILES;
-}
-
-inline unsigned mg_tiles(const MgLev& v, int ty) {
-    return (unsigned)(((v.nx + MT_X - 1) / MT_X) * ((v.ny + ty - 1) / ty));
+    return (long)((v.nx + 63) / 64) * ((v.ny + 15) / 16) <= MG_COARSE_TILES;
 }
 
 // Level cell of frame cell (fx, fy) of the tile at (I0, J0); -1 outside the
@@ -160,108 +178,132 @@ __device__ __forceinline__ int mg_frame_cell(const MgLev& L, int I, int J) {
     return (J + 1) * L.pitch + I + MG_XOFF;
 }
 
+// The value of the lane to the west (lane - 1) and to the east (lane + 1);
+// the frame's edge lanes get zero.
+__device__ __forceinline__ float mg_from_west(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float mg_from_east(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+// Rows below and above each wave's band: v[buf][0 = first row | 1 = last
+// row][wave][lane].  Buffers alternate, so one is rewritten only after the
+// barrier that follows its last read.
+struct MgXch {
+    float v[2][2][MG_NW][64];
+};
+
+__device__ __forceinline__ void mg_xch(MgXch& X, int& buf, float first, float last, float& below,
+                                       float& above) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    X.v[buf][0][w][lane] = first;
+    X.v[buf][1][w][lane] = last;
+    __syncthreads();
+    below = w > 0 ? X.v[buf][1][w - 1][lane] : 0.0f;
+    above = w < MG_NW - 1 ? X.v[buf][0][w + 1][lane] : 0.0f;
+    buf ^= 1;
+}
+
+// S v for the band's cells (sum C, E, W, N, S), given the rows below and
+// above the band.
+template <int R>
+__device__ __forceinline__ void mg_band_apply(const float (&v)[R], float below, float above,
+                                              const float (&cC)[R], const float (&cE)[R],
+                                              const float (&cW)[R], const float (&cN)[R],
+                                              const float (&cS)[R], float (&out)[R]) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const float vs = k ? v[k - 1] : below, vn = k < R - 1 ? v[k + 1] : above;
+        const float vw = mg_from_west(v[k]), ve = mg_from_east(v[k]);
+        out[k] = cC[k] * v[k] + cE[k] * ve + cW[k] * vw + cN[k] * vn + cS[k] * vs;
+    }
+}
+
+// The cell's west and south couplings (its neighbours' E and N).
+template <int R>
+__device__ __forceinline__ void mg_band_couplings(MgXch& X, int& buf, const float (&cE)[R],
+                                                  const float (&cN)[R], float (&cW)[R],
+                                                  float (&cS)[R]) {
+    float below, above;
+    mg_xch(X, buf, cN[0], cN[R - 1], below, above);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        cW[k] = mg_from_west(cE[k]);
+        cS[k] = k ? cN[k - 1] : below;
+    }
+}
+
 // Pre-smoothing from x = 0 (K Chebyshev steps), the residual, and its 2 x 2
-// restriction to the next level's right-hand side.  Each thread keeps, per
-// frame cell, r, x, the direction d, the cell's five couplings (its own C,
-// E, N and its west and south neighbours' E and N) and 1/C in registers;
-// only d, whose neighbours the next step reads, goes through LDS
-// (double-buffered: one barrier per step), so a step costs four LDS reads
-// and one write per cell.
-template <int K, int TY_>
-__global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
-                                                       const MgScal* __restrict__ S) {
-    using F = MgFrame<K, TY_>;
-    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
-    __shared__ float sE[FN], sN[FN], sD[2][FN];
+// restriction to the next level's right-hand side.
+template <int K, int R>
+__global__ void __launch_bounds__(MG_LEVEL_THREADS) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
+                                                         const MgScal* __restrict__ S) {
+    constexpr int FH = MG_NW * R, TX = MG_FW - 2 * K, TY = FH - 2 * K;
+    static_assert(TY >= 2, "tile too short for the degree");
+    __shared__ MgXch X;
+    __shared__ float sR[FH][MG_FW];   // the residual, for the restriction
     const int done = S->done;   // read beside the frame loads, checked before any store
-    const int tiles_x = (L.nx + MT_X - 1) / MT_X;
+    const int tiles_x = (L.nx + TX - 1) / TX;
     // each XCD sweeps a contiguous band of tiles, so the frame margins shared
     // with the neighbouring tiles are read through the same L2
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int I0 = (tile % tiles_x) * MT_X, J0 = (tile / tiles_x) * TY;
-    float r[PER], x[PER], d[PER], cC[PER], cE[PER], cN[PER], cW[PER], cS[PER], di[PER];
+    const int I0 = (tile % tiles_x) * TX, J0 = (tile / tiles_x) * TY;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int I = I0 - K + lane;
+    float r[R], x[R], d[R], cC[R], cE[R], cN[R], cW[R], cS[R], di[R];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * NT;
-        r[u] = x[u] = d[u] = cE[u] = cN[u] = cW[u] = cS[u] = 0.0f;
-        cC[u] = di[u] = 1.0f;
-        if (f < FN) {
-            const int fy = f / FX, fx = f - fy * FX;
-            const int q = mg_frame_cell(L, I0 - K + fx, J0 - K + fy);
-            const float b = q >= 0 ? L.b[q] : 0.0f, c = q >= 0 ? L.C[q] : 1.0f;
-            const float e = q >= 0 ? L.E[q] : 0.0f, n = q >= 0 ? L.N[q] : 0.0f;
-            sE[f] = e;
-            sN[f] = n;
-            cC[u] = c;
-            cE[u] = e;
-            cN[u] = n;
-            di[u] = 1.0f / c;
-            const float d0 = b * ch.c0 * di[u];
-            sD[0][f] = d0;
-            r[u] = b;
-            x[u] = d0;
-            d[u] = d0;
-        }
+    for (int k = 0; k < R; ++k) {
+        const int q = mg_frame_cell(L, I, J0 - K + w * R + k);
+        const float b = q >= 0 ? L.b[q] : 0.0f, c = q >= 0 ? L.C[q] : 1.0f;
+        cE[k] = q >= 0 ? L.E[q] : 0.0f;
+        cN[k] = q >= 0 ? L.N[q] : 0.0f;
+        cC[k] = c;
+        di[k] = 1.0f / c;
+        const float d0 = b * ch.c0 * di[k];
+        r[k] = b;
+        x[k] = d0;
+        d[k] = d0;
     }
     if (done) return;
-    __syncthreads();
+    int buf = 0;
+    mg_band_couplings(X, buf, cE, cN, cW, cS);
+    // steps 1..K-1; step K: the residual
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * NT;
-        const int fy = f / FX, fx = f - fy * FX;
-        if (f < FN && fx >= 1 && fy >= 1) {
-            cW[u] = sE[f - 1];
-            cS[u] = sN[f - FX];
-        }
-    }
-    // steps 1..K-1 on margins >= k; step K: the residual on the tile
+    for (int s = 1; s <= K; ++s) {
+        float below, above, sv[R];
+        mg_xch(X, buf, d[0], d[R - 1], below, above);
+        mg_band_apply(d, below, above, cC, cE, cW, cN, cS, sv);
 #pragma unroll
-    for (int k = 1; k <= K; ++k) {
-        const float* din = sD[(k - 1) & 1];
-        float* dout = sD[k & 1];
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int f = threadIdx.x + u * NT;
-            const int fy = f / FX, fx = f - fy * FX;
-            if (f < FN && fx >= k && fx < FX - k && fy >= k && fy < FY - k) {
-                const float sv = cC[u] * d[u] + cE[u] * din[f + 1] + cW[u] * din[f - 1] +
-                                 cN[u] * din[f + FX] + cS[u] * din[f - FX];
-                const float rn = r[u] - sv;
-                r[u] = rn;
-                if (k < K) {
-                    const float dn = ch.c1[k] * d[u] + ch.c2[k] * (rn * di[u]);
-                    dout[f] = dn;
-                    d[u] = dn;
-                    x[u] = x[u] + dn;
-                }
+        for (int k = 0; k < R; ++k) {
+            r[k] = r[k] - sv[k];
+            if (s < K) {
+                const float dn = ch.c1[s] * d[k] + ch.c2[s] * (r[k] * di[k]);
+                d[k] = dn;
+                x[k] = x[k] + dn;
             }
         }
-        if (k < K) __syncthreads();
     }
-    // the tile's x, its residual into LDS (reusing sE) for the restriction
-    __syncthreads();
-    float* sRes = sE;
+    // the tile's x, the residual into LDS for the restriction
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * NT;
-        const int fy = f / FX, fx = f - fy * FX;
-        if (f < FN && fx >= K && fx < FX - K && fy >= K && fy < FY - K) {
-            sRes[f] = r[u];
-            const int I = I0 + fx - K, J = J0 + fy - K;
-            if (I < L.nx && J < L.ny) L.x[(J + 1) * L.pitch + I + MG_XOFF] = x[u];
+    for (int k = 0; k < R; ++k) {
+        const int fy = w * R + k;
+        sR[fy][lane] = r[k];
+        if (lane >= K && lane < K + TX && fy >= K && fy < K + TY) {
+            const int J = J0 - K + fy;
+            if (I < L.nx && J < L.ny) L.x[(J + 1) * L.pitch + I + MG_XOFF] = x[k];
         }
     }
     __syncthreads();
-    if (threadIdx.x < (MT_X / 2) * (TY / 2)) {
-        const int l = threadIdx.x;   // one coarse cell per thread
-        const int cy = l / (MT_X / 2), cx = l - cy * (MT_X / 2);
+    for (int l = threadIdx.x; l < (TX / 2) * (TY / 2); l += MG_LEVEL_THREADS) {
+        const int cy = l / (TX / 2), cx = l - cy * (TX / 2);   // one coarse cell per thread
         const int IC = I0 / 2 + cx, JC = J0 / 2 + cy;
         if (IC < Lc.nx && JC < Lc.ny) {
             float sum = 0.0f;
             for (int dj = 0; dj < 2; ++dj)
-                for (int di2 = 0; di2 < 2; ++di2) {
-                    const int tx = 2 * cx + di2, ty = 2 * cy + dj;
-                    if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sRes[(ty + K) * FX + tx + K];
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int tx = 2 * cx + dx, ty = 2 * cy + dj;
+                    if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sR[ty + K][tx + K];
                 }
             const int Q = (JC + 1) * Lc.pitch + IC + MG_XOFF;
             Lc.b[Q] = Lc.act[Q] ? sum : 0.0f;
@@ -270,101 +312,74 @@ __global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_pre(MgLev L, MgCheb c
 }
 
 // Prolongation of the coarse correction (weight ch.cscale, active cells) and
-// K Chebyshev steps of post-smoothing; the result goes to y.  y' (needed
-// with its neighbours once) and then d go through LDS; r, y, d, the cell's
-// couplings and 1/C stay in registers.
-template <int K, int TY_>
-__global__ void __launch_bounds__(MgThreads<TY_>::NT) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
-                                                        const MgScal* __restrict__ S) {
-    using F = MgFrame<K, TY_>;
-    constexpr int FX = F::FX, FY = F::FY, FN = F::FN, PER = F::PER, NT = F::NT, TY = F::TY;
-    __shared__ float sE[FN], sN[FN], sD[2][FN];
+// K Chebyshev steps of post-smoothing; the result goes to y.
+template <int K, int R>
+__global__ void __launch_bounds__(MG_LEVEL_THREADS) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
+                                                          const MgScal* __restrict__ S) {
+    constexpr int FH = MG_NW * R, TX = MG_FW - 2 * K, TY = FH - 2 * K;
+    static_assert(TY >= 2, "tile too short for the degree");
+    __shared__ MgXch X;
     const int done = S->done;
-    const int tiles_x = (L.nx + MT_X - 1) / MT_X;
+    const int tiles_x = (L.nx + TX - 1) / TX;
     // each XCD sweeps a contiguous band of tiles, so the frame margins shared
     // with the neighbouring tiles are read through the same L2
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int I0 = (tile % tiles_x) * MT_X, J0 = (tile / tiles_x) * TY;
-    float r[PER], y[PER], d[PER], cC[PER], cE[PER], cN[PER], cW[PER], cS[PER], di[PER];
+    const int I0 = (tile % tiles_x) * TX, J0 = (tile / tiles_x) * TY;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float r[R], y[R], d[R], cC[R], cE[R], cN[R], cW[R], cS[R], di[R];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * NT;
-        r[u] = y[u] = d[u] = cE[u] = cN[u] = cW[u] = cS[u] = 0.0f;
-        cC[u] = di[u] = 1.0f;
-        if (f < FN) {
-            const int fy = f / FX, fx = f - fy * FX;
-            int I = I0 - K + fx;
-            const int J = J0 - K + fy;
-            const int q = mg_frame_cell(L, I, J);
-            float yv = 0.0f;
-            if (q >= 0 && L.act[q]) {
-                if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
-                yv = L.x[q] + ch.cscale * Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
-            }
-            const float c = q >= 0 ? L.C[q] : 1.0f;
-            const float e = q >= 0 ? L.E[q] : 0.0f, n = q >= 0 ? L.N[q] : 0.0f;
-            sE[f] = e;
-            sN[f] = n;
-            cC[u] = c;
-            cE[u] = e;
-            cN[u] = n;
-            di[u] = 1.0f / c;
-            sD[1][f] = yv;                       // y' for step 0's stencil
-            r[u] = q >= 0 ? L.b[q] : 0.0f;       // b
-            y[u] = yv;
+    for (int k = 0; k < R; ++k) {
+        int I = I0 - K + lane;
+        const int J = J0 - K + w * R + k;
+        const int q = mg_frame_cell(L, I, J);
+        float yv = 0.0f;
+        if (q >= 0 && L.act[q]) {
+            if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
+            yv = L.x[q] + ch.cscale * Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
         }
+        const float c = q >= 0 ? L.C[q] : 1.0f;
+        cE[k] = q >= 0 ? L.E[q] : 0.0f;
+        cN[k] = q >= 0 ? L.N[q] : 0.0f;
+        cC[k] = c;
+        di[k] = 1.0f / c;
+        r[k] = q >= 0 ? L.b[q] : 0.0f;   // b
+        y[k] = yv;
     }
     if (done) return;
-    __syncthreads();
-    // step 0 (margin >= 1): r = b - S y', d0 = D^-1 r / theta, y = y' + d0
+    int buf = 0;
+    mg_band_couplings(X, buf, cE, cN, cW, cS);
+    // step 0: r = b - S y', d0 = D^-1 r / theta, y = y' + d0
+    {
+        float below, above, sv[R];
+        mg_xch(X, buf, y[0], y[R - 1], below, above);
+        mg_band_apply(y, below, above, cC, cE, cW, cN, cS, sv);
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * NT;
-        const int fy = f / FX, fx = f - fy * FX;
-        if (f < FN && fx >= 1 && fx < FX - 1 && fy >= 1 && fy < FY - 1) {
-            const float* v = sD[1];
-            cW[u] = sE[f - 1];
-            cS[u] = sN[f - FX];
-            const float sv = cC[u] * y[u] + cE[u] * v[f + 1] + cW[u] * v[f - 1] +
-                             cN[u] * v[f + FX] + cS[u] * v[f - FX];
-            const float rn = r[u] - sv;
-            const float dn = ch.c0 * (rn * di[u]);
-            r[u] = rn;
-            sD[0][f] = dn;
-            d[u] = dn;
-            y[u] = y[u] + dn;
+        for (int k = 0; k < R; ++k) {
+            r[k] = r[k] - sv[k];
+            d[k] = ch.c0 * (r[k] * di[k]);
+            y[k] = y[k] + d[k];
         }
     }
-    __syncthreads();
 #pragma unroll
-    for (int k = 1; k < K; ++k) {
-        const int m = k + 1;
-        const float* din = sD[(k - 1) & 1];
-        float* dout = sD[k & 1];
+    for (int s = 1; s < K; ++s) {
+        float below, above, sv[R];
+        mg_xch(X, buf, d[0], d[R - 1], below, above);
+        mg_band_apply(d, below, above, cC, cE, cW, cN, cS, sv);
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int f = threadIdx.x + u * NT;
-            const int fy = f / FX, fx = f - fy * FX;
-            if (f < FN && fx >= m && fx < FX - m && fy >= m && fy < FY - m) {
-                const float sv = cC[u] * d[u] + cE[u] * din[f + 1] + cW[u] * din[f - 1] +
-                                 cN[u] * din[f + FX] + cS[u] * din[f - FX];
-                const float rn = r[u] - sv;
-                const float dn = ch.c1[k] * d[u] + ch.c2[k] * (rn * di[u]);
-                r[u] = rn;
-                dout[f] = dn;
-                d[u] = dn;
-                y[u] = y[u] + dn;
-            }
+        for (int k = 0; k < R; ++k) {
+            r[k] = r[k] - sv[k];
+            const float dn = ch.c1[s] * d[k] + ch.c2[s] * (r[k] * di[k]);
+            d[k] = dn;
+            y[k] = y[k] + dn;
         }
-        __syncthreads();
     }
+    const int I = I0 - K + lane;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int f = threadIdx.x + u * NT;
-        const int fy = f / FX, fx = f - fy * FX;
-        if (f < FN && fx >= K && fx < FX - K && fy >= K && fy < FY - K) {
-            const int I = I0 + fx - K, J = J0 + fy - K;
-            if (I < L.nx && J < L.ny) L.y[(J + 1) * L.pitch + I + MG_XOFF] = y[u];
+    for (int k = 0; k < R; ++k) {
+        const int fy = w * R + k;
+        if (lane >= K && lane < K + TX && fy >= K && fy < K + TY) {
+            const int J = J0 - K + fy;
+            if (I < L.nx && J < L.ny) L.y[(J + 1) * L.pitch + I + MG_XOFF] = y[k];
         }
     }
 }
@@ -424,7 +439,7 @@ __device__ __forceinline__ void mg_block_partials(double (&v)[NR], double* part)
 // also uses <Az,p> and <r,p>; the first equals <z,q> (W A is symmetric and
 // q = A p) and the second vanishes by the previous line search, so p is not
 // read here.
-__global__ void __launch_bounds__(256) k_mg_fcg_a(
+__global__ void __launch_bounds__(256) k_mg_fcg_a1(
     Geom g, MgLev L0, const double* __restrict__ E, const double* __restrict__ N,
     const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ dxu, const double* __restrict__ cosu,
@@ -509,7 +524,7 @@ __global__ void __launch_bounds__(1024) k_mg_fold_a(const double* __restrict__ p
 
 // Pass B: p = z + beta p, q = Az + beta q, x += alpha p, r -= alpha q; the
 // V-cycle's next input w r s_r on level 0; |r|^2 (active) and |w r|^2.
-__global__ void __launch_bounds__(256) k_mg_fcg_b(
+__global__ void __launch_bounds__(256) k_mg_fcg_b1(
     Geom g, MgLev L0, const double* __restrict__ dxu, const double* __restrict__ cosu,
     const double* __restrict__ dyu, const uint8_t* __restrict__ act, double* __restrict__ p,
     double* __restrict__ q, double* __restrict__ r, double* __restrict__ x,
@@ -583,7 +598,7 @@ __global__ void __launch_bounds__(1024) k_mg_fold_b(const double* __restrict__ p
 // of a 512-point row segment through 16-byte loads and stores (used when nx
 // is even, so every row of the solver layout, pitch nx + 4, starts 16-byte
 // aligned and no row ends in a single point).  Per point the arithmetic and
-// its order are those of k_mg_fcg_a / k_mg_fcg_b.
+// its order are those of k_mg_fcg_a1 / k_mg_fcg_b1.
 struct MgZ2 {
     double c0, c1;   // z at (i, j), (i+1, j)
     double w0, e1;   // z west of i, east of i+1
@@ -615,7 +630,7 @@ __device__ __forceinline__ MgZ2 mg_z_pair(const Geom& g, const MgLev& L0, int i,
     return z;
 }
 
-__global__ void __launch_bounds__(256) k_mg_fcg_a2(
+__global__ void __launch_bounds__(256) k_mg_fcg_a(
     Geom g, MgLev L0, const double* __restrict__ E, const double* __restrict__ N,
     const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ dxu, const double* __restrict__ cosu,
@@ -670,7 +685,7 @@ __global__ void __launch_bounds__(256) k_mg_fcg_a2(
     mg_block_partials<3>(acc, part);
 }
 
-__global__ void __launch_bounds__(256) k_mg_fcg_b2(
+__global__ void __launch_bounds__(256) k_mg_fcg_b(
     Geom g, MgLev L0, const double* __restrict__ dxu, const double* __restrict__ cosu,
     const double* __restrict__ dyu, const uint8_t* __restrict__ act, double* __restrict__ p,
     double* __restrict__ q, double* __restrict__ r, double* __restrict__ x,
@@ -733,11 +748,11 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b2(
 }
 
 // The outer passes for a grid: on pairs of points when nx is even.
-inline decltype(&k_mg_fcg_a) mg_pass_a(const Geom& g) {
-    return (g.nx & 1) ? k_mg_fcg_a : k_mg_fcg_a2;
+inline decltype(&k_mg_fcg_a1) mg_pass_a(const Geom& g) {
+    return (g.nx & 1) ? k_mg_fcg_a1 : k_mg_fcg_a;
 }
-inline decltype(&k_mg_fcg_b) mg_pass_b(const Geom& g) {
-    return (g.nx & 1) ? k_mg_fcg_b : k_mg_fcg_b2;
+inline decltype(&k_mg_fcg_b1) mg_pass_b(const Geom& g) {
+    return (g.nx & 1) ? k_mg_fcg_b1 : k_mg_fcg_b;
 }
 
 // Start of a (re)started outer iteration from the residual r: the V-cycle

@@ -32,8 +32,8 @@ def test_defaults(monkeypatch):
 @pytest.mark.parametrize("degree", [4, 5])
 def test_kernel_names_and_bytes(degree):
     names = bench.mg_kernel_names(degree)
-    assert names["post_level0"].startswith(f"k_mg_post<{degree}, 32>")
-    assert names["pre_level0"].startswith(f"k_mg_pre<{degree}, 32>")
+    assert names["post_level0"].startswith(f"k_mg_post<{degree}>")
+    assert names["pre_level0"].startswith(f"k_mg_pre<{degree}>")
     # pass B: z (FP32), Az, p, q, r, x read (FP64), mask, p, q, r, x written, next input (FP32)
     assert bench.MG_KERNEL_BYTES["fcg_b"] == 4 + 8 + 4 * 8 + 1 + 4 * 8 + 4 == 81
     assert bench.MG_KERNEL_BYTES["fcg_a"] == 45
