@@ -333,9 +333,11 @@ __global__ void __launch_bounds__(MG_LEVEL_THREADS) k_mg_post(MgLev L, MgLev Lc,
         const int J = J0 - K + w * R + k;
         const int q = mg_frame_cell(L, I, J);
         float yv = 0.0f;
-        if (q >= 0 && L.act[q]) {
+        if (q >= 0) {
+            // x, the coarse correction and the mask in one round of loads
             if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
-            yv = L.x[q] + ch.cscale * Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
+            const float xv = L.x[q], yc = Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
+            yv = L.act[q] ? xv + ch.cscale * yc : 0.0f;
         }
         const float c = q >= 0 ? L.C[q] : 1.0f;
         cE[k] = q >= 0 ? L.E[q] : 0.0f;
