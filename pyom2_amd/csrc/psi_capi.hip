@@ -1123,37 +1123,38 @@ struct MgEvents {
     }
 };
 
-// The level kernels of level v at degree K (rows per lane by the level's size).
-template <int K, int R>
+// The level kernels of level v at degree K (waves and rows per lane by the
+// level's size).
+template <int K, int NW, int R>
 void mg_launch_pre_r(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
                      hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_pre<K, R>), dim3(mg_tiles(v, K, R)), dim3(MG_LEVEL_THREADS), 0, s,
+    hipLaunchKernelGGL((k_mg_pre<K, NW, R>), dim3(mg_tiles(v, K, NW, R)), dim3(64 * NW), 0, s,
                        v, ch, vc, S);
 }
 
-template <int K, int R>
+template <int K, int NW, int R>
 void mg_launch_post_r(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
                       hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_post<K, R>), dim3(mg_tiles(v, K, R)), dim3(MG_LEVEL_THREADS), 0, s,
+    hipLaunchKernelGGL((k_mg_post<K, NW, R>), dim3(mg_tiles(v, K, NW, R)), dim3(64 * NW), 0, s,
                        v, vc, ch, S);
 }
 
 template <int K>
 void mg_launch_pre(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
                    hipStream_t s) {
-    if (mg_rows_per_lane(v, K) == MG_RPT_LARGE)
-        mg_launch_pre_r<K, MG_RPT_LARGE>(v, vc, ch, S, s);
+    if (mg_large_level(v, K))
+        mg_launch_pre_r<K, MG_NW_LARGE, MG_RPT_LARGE>(v, vc, ch, S, s);
     else
-        mg_launch_pre_r<K, mg_rpt_small(K)>(v, vc, ch, S, s);
+        mg_launch_pre_r<K, MG_NW_SMALL, mg_rpt_small(K)>(v, vc, ch, S, s);
 }
 
 template <int K>
 void mg_launch_post(const MgLev& v, const MgLev& vc, const MgCheb& ch, const MgScal* S,
                     hipStream_t s) {
-    if (mg_rows_per_lane(v, K) == MG_RPT_LARGE)
-        mg_launch_post_r<K, MG_RPT_LARGE>(v, vc, ch, S, s);
+    if (mg_large_level(v, K))
+        mg_launch_post_r<K, MG_NW_LARGE, MG_RPT_LARGE>(v, vc, ch, S, s);
     else
-        mg_launch_post_r<K, mg_rpt_small(K)>(v, vc, ch, S, s);
+        mg_launch_post_r<K, MG_NW_SMALL, mg_rpt_small(K)>(v, vc, ch, S, s);
 }
 
 // A coarse level's kernel at degree Kc.

@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
 // every step to memory, so a level costs two kernels per V-cycle
 // (pre-smoothing + restriction, prolongation + post-smoothing).
 //
-// The frame is 64 columns (one per lane) x MG_NW * RPT rows; wave w holds
+// The frame is 64 columns (one per lane) x NW * RPT rows (NW waves); wave w holds
 // rows [w RPT, (w+1) RPT) of every column in registers, with each cell's
 // couplings (its own C, E, N, its west neighbour's E and its south
 // neighbour's N) and 1/C.  A stencil takes the west and east neighbours from
@@ -125,39 +125,47 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
 // a cell at margin m is exact after step m, and garbage from the frame edge
 // (where a shift brings in zero) moves in by one cell per step, so the own
 // tile (margin >= K) is exact at the end.
-// Rows per lane: MG_RPT_LARGE on levels that keep at least MG_LARGE_TILES
-// tiles with it, MG_RPT_SMALL on the others, whose kernels are bound by
-// latency and gain from more, smaller workgroups.
-constexpr int MG_NW = 16, MG_FW = 64;
-constexpr int MG_LEVEL_THREADS = 64 * MG_NW;
+// Waves and rows per lane: (MG_NW_LARGE, MG_RPT_LARGE) on levels that keep
+// at least MG_LARGE_TILES tiles with it, (MG_NW_SMALL, mg_rpt_small) on the
+// others, whose kernels are bound by latency and gain from more, smaller
+// workgroups.
+constexpr int MG_FW = 64;
+#ifndef PSI_MG_NW_LARGE
+#define PSI_MG_NW_LARGE 8
+#endif
+#ifndef PSI_MG_RPT_LARGE
+#define PSI_MG_RPT_LARGE 4
+#endif
+#ifndef PSI_MG_NW_SMALL
+#define PSI_MG_NW_SMALL 16
+#endif
 #ifndef PSI_MG_RPT_SMALL
 #define PSI_MG_RPT_SMALL 1
 #endif
-#ifndef PSI_MG_RPT_LARGE
-#define PSI_MG_RPT_LARGE 3
-#endif
 #ifndef PSI_MG_LARGE_TILES
-#define PSI_MG_LARGE_TILES 512
+#define PSI_MG_LARGE_TILES 200
 #endif
-constexpr int MG_RPT_LARGE = PSI_MG_RPT_LARGE;
+constexpr int MG_NW_LARGE = PSI_MG_NW_LARGE, MG_RPT_LARGE = PSI_MG_RPT_LARGE;
+constexpr int MG_NW_SMALL = PSI_MG_NW_SMALL;
 constexpr long MG_LARGE_TILES = PSI_MG_LARGE_TILES;
 // the small levels' rows per lane at degree K: PSI_MG_RPT_SMALL where that
 // leaves a tile at least 4 rows tall, else the fewest that do
 constexpr int mg_rpt_small(int K) {
-    return MG_NW * PSI_MG_RPT_SMALL - 2 * K >= 4 ? PSI_MG_RPT_SMALL
-                                                 : (MG_NW * 2 - 2 * K >= 4 ? 2 : 3);
+    return MG_NW_SMALL * PSI_MG_RPT_SMALL - 2 * K >= 4
+               ? PSI_MG_RPT_SMALL
+               : (MG_NW_SMALL * 2 - 2 * K >= 4 ? 2 : 3);
 }
 
-// Tile of a level kernel at degree K and RPT rows per lane:
-// (64 - 2K) x (MG_NW RPT - 2K), both even.
+// Tile of a level kernel at degree K, NW waves and RPT rows per lane:
+// (64 - 2K) x (NW RPT - 2K), both even.
 inline int mg_tile_x(int K) { return MG_FW - 2 * K; }
-inline int mg_tile_h(int K, int rpt) { return MG_NW * rpt - 2 * K; }
-inline unsigned mg_tiles(const MgLev& v, int K, int rpt) {
+inline int mg_tile_h(int K, int nw, int rpt) { return nw * rpt - 2 * K; }
+inline unsigned mg_tiles(const MgLev& v, int K, int nw, int rpt) {
     return (unsigned)(((v.nx + mg_tile_x(K) - 1) / mg_tile_x(K)) *
-                      ((v.ny + mg_tile_h(K, rpt) - 1) / mg_tile_h(K, rpt)));
+                      ((v.ny + mg_tile_h(K, nw, rpt) - 1) / mg_tile_h(K, nw, rpt)));
 }
-inline int mg_rows_per_lane(const MgLev& v, int K) {
-    return mg_tiles(v, K, MG_RPT_LARGE) >= MG_LARGE_TILES ? MG_RPT_LARGE : mg_rpt_small(K);
+inline bool mg_large_level(const MgLev& v, int K) {
+    return mg_tiles(v, K, MG_NW_LARGE, MG_RPT_LARGE) >= MG_LARGE_TILES;
 }
 
 // A level is coarse when it has at most MG_COARSE_TILES 64 x 16 blocks of
@@ -190,18 +198,20 @@ __device__ __forceinline__ float mg_from_east(float v) {
 // Rows below and above each wave's band: v[buf][0 = first row | 1 = last
 // row][wave][lane].  Buffers alternate, so one is rewritten only after the
 // barrier that follows its last read.
+template <int NW>
 struct MgXch {
-    float v[2][2][MG_NW][64];
+    float v[2][2][NW][64];
 };
 
-__device__ __forceinline__ void mg_xch(MgXch& X, int& buf, float first, float last, float& below,
-                                       float& above) {
+template <int NW>
+__device__ __forceinline__ void mg_xch(MgXch<NW>& X, int& buf, float first, float last,
+                                       float& below, float& above) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     X.v[buf][0][w][lane] = first;
     X.v[buf][1][w][lane] = last;
     __syncthreads();
     below = w > 0 ? X.v[buf][1][w - 1][lane] : 0.0f;
-    above = w < MG_NW - 1 ? X.v[buf][0][w + 1][lane] : 0.0f;
+    above = w < NW - 1 ? X.v[buf][0][w + 1][lane] : 0.0f;
     buf ^= 1;
 }
 
@@ -221,8 +231,8 @@ __device__ __forceinline__ void mg_band_apply(const float (&v)[R], float below, 
 }
 
 // The cell's west and south couplings (its neighbours' E and N).
-template <int R>
-__device__ __forceinline__ void mg_band_couplings(MgXch& X, int& buf, const float (&cE)[R],
+template <int NW, int R>
+__device__ __forceinline__ void mg_band_couplings(MgXch<NW>& X, int& buf, const float (&cE)[R],
                                                   const float (&cN)[R], float (&cW)[R],
                                                   float (&cS)[R]) {
     float below, above;
@@ -236,12 +246,12 @@ __device__ __forceinline__ void mg_band_couplings(MgXch& X, int& buf, const floa
 
 // Pre-smoothing from x = 0 (K Chebyshev steps), the residual, and its 2 x 2
 // restriction to the next level's right-hand side.
-template <int K, int R>
-__global__ void __launch_bounds__(MG_LEVEL_THREADS) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
-                                                         const MgScal* __restrict__ S) {
-    constexpr int FH = MG_NW * R, TX = MG_FW - 2 * K, TY = FH - 2 * K;
+template <int K, int NW, int R>
+__global__ void __launch_bounds__(64 * NW) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc,
+                                                  const MgScal* __restrict__ S) {
+    constexpr int FH = NW * R, TX = MG_FW - 2 * K, TY = FH - 2 * K;
     static_assert(TY >= 2, "tile too short for the degree");
-    __shared__ MgXch X;
+    __shared__ MgXch<NW> X;
     __shared__ float sR[FH][MG_FW];   // the residual, for the restriction
     const int done = S->done;   // read beside the frame loads, checked before any store
     const int tiles_x = (L.nx + TX - 1) / TX;
@@ -295,7 +305,7 @@ __global__ void __launch_bounds__(MG_LEVEL_THREADS) k_mg_pre(MgLev L, MgCheb ch,
         }
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < (TX / 2) * (TY / 2); l += MG_LEVEL_THREADS) {
+    for (int l = threadIdx.x; l < (TX / 2) * (TY / 2); l += 64 * NW) {
         const int cy = l / (TX / 2), cx = l - cy * (TX / 2);   // one coarse cell per thread
         const int IC = I0 / 2 + cx, JC = J0 / 2 + cy;
         if (IC < Lc.nx && JC < Lc.ny) {
@@ -313,12 +323,12 @@ __global__ void __launch_bounds__(MG_LEVEL_THREADS) k_mg_pre(MgLev L, MgCheb ch,
 
 // Prolongation of the coarse correction (weight ch.cscale, active cells) and
 // K Chebyshev steps of post-smoothing; the result goes to y.
-template <int K, int R>
-__global__ void __launch_bounds__(MG_LEVEL_THREADS) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
-                                                          const MgScal* __restrict__ S) {
-    constexpr int FH = MG_NW * R, TX = MG_FW - 2 * K, TY = FH - 2 * K;
+template <int K, int NW, int R>
+__global__ void __launch_bounds__(64 * NW) k_mg_post(MgLev L, MgLev Lc, MgCheb ch,
+                                                   const MgScal* __restrict__ S) {
+    constexpr int FH = NW * R, TX = MG_FW - 2 * K, TY = FH - 2 * K;
     static_assert(TY >= 2, "tile too short for the degree");
-    __shared__ MgXch X;
+    __shared__ MgXch<NW> X;
     const int done = S->done;
     const int tiles_x = (L.nx + TX - 1) / TX;
     // each XCD sweeps a contiguous band of tiles, so the frame margins shared
