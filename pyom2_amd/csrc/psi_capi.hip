@@ -1083,6 +1083,20 @@ int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
     MGTRY(mg_alloc(c, m, &m->zscale, 1));
     m->nblk = (int)std::min<size_t>(nblk((size_t)g.nx * g.ny), MG_OUTER_BLOCKS);
     MGTRY(mg_alloc(c, m, &m->part, (size_t)MG_NRED * m->nblk));
+    {
+        // pass A on pairs: two 1024-thread workgroups per CU, at most one per
+        // row segment
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+                hipSuccess ||
+            ncu < 1)
+            ncu = 1;
+        const long segs = (long)g.ny * ((g.nx + 2 * MG_PASS_A_THREADS - 1) / (2 * MG_PASS_A_THREADS));
+        long ga = std::min<long>(2L * ncu, segs);
+        if (ga >= 8) ga &= ~7L;
+        m->nblk_a = (int)std::max<long>(ga, 1);
+        MGTRY(mg_alloc(c, m, &m->part_a, (size_t)MG_NRED * m->nblk_a));
+    }
 #undef MGTRY
     HIPCHK(c, hipHostMalloc((void**)&m->h_S, sizeof(MgScal), hipHostMallocDefault));
     HIPCHK(c, hipStreamCreateWithFlags(&m->gs, hipStreamNonBlocking));
@@ -1239,27 +1253,42 @@ void mg_vcycle(MgState* m, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev 
     }
 }
 
-// One outer iteration on stream s: V-cycle, pass A, fold, pass B, fold;
+// One outer iteration on stream s: V-cycle, pass A, (fold,) pass B, fold;
 // with ev, the kernel-timing events of MgState::kev recorded around them.
+// On even-width grids pass B folds pass A's partials itself.
 void mg_iteration(psi_ctx* c, hipStream_t s, int K, const MgCheb& ch, MgEvents* ev = nullptr,
                   const MgCoarse& co = MgCoarse{}) {
     MgState* m = c->mg;
     mg_vcycle(m, s, K, ch, ev, co);
     const Geom& g = c->g;
-    hipLaunchKernelGGL(mg_pass_a(g), dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
-                       (const double*)c->E, (const double*)c->N, (const double*)c->rx,
-                       (const double*)c->ry, (const double*)c->dxu, (const double*)c->cosu,
-                       (const double*)c->dyu, (const uint8_t*)c->act,
-                       (const double*)m->q, (const double*)m->r, m->az,
-                       (const double*)m->zscale, (const MgScal*)m->S, m->part);
-    if (ev) ev->rec(4, s);
-    hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
-                       m->S);
-    if (ev) ev->rec(5, s);
-    hipLaunchKernelGGL(mg_pass_b(g), dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
-                       (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
-                       (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
-                       (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    if (!(g.nx & 1)) {
+        hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk_a), dim3(MG_PASS_A_THREADS), 0, s, g,
+                           m->lev[0], (const double*)c->E, (const double*)c->N,
+                           (const double*)c->rx, (const double*)c->ry, (const double*)c->dxu,
+                           (const double*)c->cosu, (const double*)c->dyu,
+                           (const uint8_t*)c->act, (const double*)m->q, (const double*)m->r,
+                           m->az, (const double*)m->zscale, (const MgScal*)m->S, m->part_a);
+        if (ev) ev->rec(4, s);
+        if (ev) ev->rec(5, s);
+        hipLaunchKernelGGL(k_mg_fcg_b<true>, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+                           (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
+                           (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
+                           (const double*)m->zscale, m->S, (const double*)m->part_a, m->nblk_a,
+                           m->part);
+    } else {
+        hipLaunchKernelGGL(k_mg_fcg_a1, dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
+                           (const double*)c->E, (const double*)c->N, (const double*)c->rx,
+                           (const double*)c->ry, (const double*)c->dxu, (const double*)c->cosu,
+                           (const double*)c->dyu, (const uint8_t*)c->act,
+                           (const double*)m->q, (const double*)m->r, m->az,
+                           (const double*)m->zscale, (const MgScal*)m->S, m->part);
+        if (ev) ev->rec(4, s);
+        hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part,
+                           m->nblk, m->S);
+        if (ev) ev->rec(5, s);
+        mg_launch_pass_b(g, m->nblk, s, m->lev[0], c->dxu, c->cosu, c->dyu, c->act, m->p, m->q,
+                         m->r, m->x, m->az, m->zscale, m->S, m->part);
+    }
     if (ev) ev->rec(6, s);
     hipLaunchKernelGGL(k_mg_fold_b, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->nact0, m->S, m->zscale);
@@ -1513,18 +1542,27 @@ int psi_mg_kernel_times(psi_ctx* c, int32_t degree, int32_t reps, double* us, in
     }
     const Geom& g = c->g;
     us[2] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(mg_pass_a(g), dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
-                           (const double*)c->E, (const double*)c->N, (const double*)c->rx,
-                           (const double*)c->ry, (const double*)c->dxu, (const double*)c->cosu,
-                           (const double*)c->dyu, (const uint8_t*)c->act, (const double*)m->q,
-                           (const double*)m->r, m->az, (const double*)m->zscale,
-                           (const MgScal*)m->S_time, m->part);
+        if (g.nx & 1)
+            hipLaunchKernelGGL(k_mg_fcg_a1, dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
+                               (const double*)c->E, (const double*)c->N, (const double*)c->rx,
+                               (const double*)c->ry, (const double*)c->dxu,
+                               (const double*)c->cosu, (const double*)c->dyu,
+                               (const uint8_t*)c->act, (const double*)m->q, (const double*)m->r,
+                               m->az, (const double*)m->zscale, (const MgScal*)m->S_time,
+                               m->part);
+        else
+            hipLaunchKernelGGL(k_mg_fcg_a, dim3(m->nblk_a), dim3(MG_PASS_A_THREADS), 0, m->gs, g,
+                               m->lev[0], (const double*)c->E, (const double*)c->N,
+                               (const double*)c->rx, (const double*)c->ry,
+                               (const double*)c->dxu, (const double*)c->cosu,
+                               (const double*)c->dyu, (const uint8_t*)c->act,
+                               (const double*)m->q, (const double*)m->r, m->az,
+                               (const double*)m->zscale, (const MgScal*)m->S_time, m->part_a);
     });
+    // pass B with its scalars from S_time (the in-solve pass B also folds pass A's partials)
     us[3] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(mg_pass_b(g), dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
-                           (const double*)c->dxu, (const double*)c->cosu, (const double*)c->dyu,
-                           (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
-                           (const double*)m->zscale, (const MgScal*)m->S_time, m->part);
+        mg_launch_pass_b(g, m->nblk, m->gs, m->lev[0], c->dxu, c->cosu, c->dyu, c->act, m->p, m->q,
+                         m->r, m->x, m->az, m->zscale, m->S_time, m->part);
     });
     HIPCHK(c, hipGetLastError());
     for (int k = 0; k < 4; ++k)

@@ -427,9 +427,10 @@ __device__ __forceinline__ double mg_area(const double* dxu, const double* cosu,
 
 constexpr int MG_OUTER_BLOCKS = 2048;   // workgroups of the outer passes (grid-stride)
 
-template <int NR>
+// One partial per slot for this workgroup of NW waves (waves summed in order).
+template <int NR, int NW = 4>
 __device__ __forceinline__ void mg_block_partials(double (&v)[NR], double* part) {
-    __shared__ double sh[NR * 4];
+    __shared__ double sh[NR * NW];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         double x = v[k];
@@ -438,12 +439,14 @@ __device__ __forceinline__ void mg_block_partials(double (&v)[NR], double* part)
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0)
-        for (int k = 0; k < NR; ++k) sh[k * 4 + wid] = v[k];
+        for (int k = 0; k < NR; ++k) sh[k * NW + wid] = v[k];
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int k = 0; k < NR; ++k)
-            part[(size_t)k * gridDim.x + blockIdx.x] = ((sh[k * 4] + sh[k * 4 + 1]) + sh[k * 4 + 2]) +
-                                                       sh[k * 4 + 3];
+        for (int k = 0; k < NR; ++k) {
+            double a = sh[k * NW];
+            for (int w = 1; w < NW; ++w) a += sh[k * NW + w];
+            part[(size_t)k * gridDim.x + blockIdx.x] = a;
+        }
 }
 
 // Pass A: with z = M r from the V-cycle, Az (kept for pass B) and the
@@ -607,10 +610,13 @@ __global__ void __launch_bounds__(1024) k_mg_fold_b(const double* __restrict__ p
 
 // ---- the outer passes on pairs of points -----------------------------------
 // The same two passes with each thread taking the points (i, i+1), i even,
-// of a 512-point row segment through 16-byte loads and stores (used when nx
-// is even, so every row of the solver layout, pitch nx + 4, starts 16-byte
-// aligned and no row ends in a single point).  Per point the arithmetic and
-// its order are those of k_mg_fcg_a1 / k_mg_fcg_b1.
+// of a row segment through 16-byte loads and stores (used when nx is even,
+// so every row of the solver layout, pitch nx + 4, starts 16-byte aligned and
+// no row ends in a single point).  Per point the arithmetic and its order are
+// those of k_mg_fcg_a1 / k_mg_fcg_b1.  Pass A runs on MG_PASS_A_THREADS-thread
+// workgroups, two per CU, so that it leaves few partials: pass B folds them
+// itself instead of a one-workgroup fold kernel between the passes.
+constexpr int MG_PASS_A_THREADS = 1024;
 struct MgZ2 {
     double c0, c1;   // z at (i, j), (i+1, j)
     double w0, e1;   // z west of i, east of i+1
@@ -642,7 +648,7 @@ __device__ __forceinline__ MgZ2 mg_z_pair(const Geom& g, const MgLev& L0, int i,
     return z;
 }
 
-__global__ void __launch_bounds__(256) k_mg_fcg_a(
+__global__ void __launch_bounds__(MG_PASS_A_THREADS) k_mg_fcg_a(
     Geom g, MgLev L0, const double* __restrict__ E, const double* __restrict__ N,
     const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ dxu, const double* __restrict__ cosu,
@@ -653,12 +659,12 @@ __global__ void __launch_bounds__(256) k_mg_fcg_a(
     if (S->done) return;
     const double inv = 1.0 / *zscale;   // the scale of the V-cycle's input
     double acc[3] = {0, 0, 0};
-    // row segments of 512 points, each XCD a contiguous range of rows
-    const int segs = (g.nx + 511) / 512;
+    // row segments of 2 MG_PASS_A_THREADS points, each XCD a contiguous range of rows
+    const int segs = (g.nx + 2 * MG_PASS_A_THREADS - 1) / (2 * MG_PASS_A_THREADS);
     int first, stride, end;
     range_schedule(g.ny * segs, first, stride, end);
     for (int it = first; it < end; it += stride) {
-        const int j = it / segs, i = (it - j * segs) * 512 + 2 * threadIdx.x;
+        const int j = it / segs, i = (it - j * segs) * (2 * MG_PASS_A_THREADS) + 2 * threadIdx.x;
         if (i >= g.nx) continue;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
         const int c = i + 2, rr = j + 2;
@@ -694,77 +700,154 @@ __global__ void __launch_bounds__(256) k_mg_fcg_a(
         }
         *reinterpret_cast<double2*>(az_out + o) = az2;
     }
-    mg_block_partials<3>(acc, part);
+    mg_block_partials<3, MG_PASS_A_THREADS / 64>(acc, part);
 }
 
+// Pass B's operands of the points (i, i+1) of a row.
+struct MgB2 {
+    float2 y;
+    double2 p, q, x, r, az;
+    double w0, w1;
+    uint16_t a;
+};
+
+__device__ __forceinline__ void mg_b2_load(const Geom& g, const MgLev& L0,
+                                           const double* __restrict__ dxu,
+                                           const double* __restrict__ cosu,
+                                           const double* __restrict__ dyu,
+                                           const uint8_t* __restrict__ act, const double* p,
+                                           const double* q, const double* r, const double* x,
+                                           const double* __restrict__ az_in, int j, int i,
+                                           MgB2& v) {
+    const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+    const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
+    const int c = i + 2, rr = j + 2;
+    v.y = *reinterpret_cast<const float2*>(L0.y + ql);
+    v.p = *reinterpret_cast<const double2*>(p + o);
+    v.q = *reinterpret_cast<const double2*>(q + o);
+    v.x = *reinterpret_cast<const double2*>(x + o);
+    v.r = *reinterpret_cast<const double2*>(r + o);
+    v.az = *reinterpret_cast<const double2*>(az_in + o);
+    v.w0 = dxu[c] * cosu[rr] * dyu[rr];
+    v.w1 = dxu[c + 1] * cosu[rr] * dyu[rr];
+    v.a = *reinterpret_cast<const uint16_t*>(act + o);
+}
+
+// FOLD: the scalars come from pass A's partials (ga per slot), folded by
+// every workgroup in the same fixed order, instead of from a separate fold
+// kernel; workgroup 0 records them in S for the fold after pass B.  Else they
+// are read from S.  (Holding the first row segment's operands in flight
+// across the fold needs 72 VGPRs, 7 waves per SIMD, and spills at 8.)
+template <bool FOLD>
 __global__ void __launch_bounds__(256) k_mg_fcg_b(
     Geom g, MgLev L0, const double* __restrict__ dxu, const double* __restrict__ cosu,
     const double* __restrict__ dyu, const uint8_t* __restrict__ act, double* __restrict__ p,
     double* __restrict__ q, double* __restrict__ r, double* __restrict__ x,
-    const double* __restrict__ az_in, const double* __restrict__ zscale,
-    const MgScal* __restrict__ S, double* __restrict__ part) {
+    const double* __restrict__ az_in, const double* __restrict__ zscale, MgScal* S,
+    const double* __restrict__ part_a, int ga, double* __restrict__ part) {
+    __shared__ double sf[3 * 4];
     if (S->done) return;
     const double inv = 1.0 / *zscale;
-    const double beta = S->beta, alpha = S->alpha, sr = S->s_r;
-    double acc[2] = {0, 0};
     // row segments of 512 points, each XCD a contiguous range of rows
     const int segs = (g.nx + 511) / 512;
     int first, stride, end;
     range_schedule(g.ny * segs, first, stride, end);
+    double beta, alpha, sr;
+    if (FOLD) {
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int b = threadIdx.x; b < ga; b += 256)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] += part_a[(size_t)k * ga + b];
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double t = v[k];
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+            if (lane == 0) sf[k * 4 + wid] = t;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = ((sf[k * 4] + sf[k * 4 + 1]) + sf[k * 4 + 2]) + sf[k * 4 + 3];
+        // beta = -<z,q>/<p,q>_old, <p,q> = <z,Az> + 2 beta <z,q> + beta^2 <p,q>_old,
+        // alpha = <r,z> / <p,q>
+        const double pq_old = S->pq_old;
+        beta = pq_old != 0.0 ? -v[1] / pq_old : 0.0;
+        const double pq = v[0] + 2.0 * beta * v[1] + beta * beta * pq_old;
+        alpha = v[2] / pq;
+        sr = S->s_pending;
+        const bool bad = !(pq != 0.0) || alpha != alpha;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S->beta = beta;
+            S->pq = pq;
+            S->alpha = alpha;
+            S->s_r = sr;
+            if (bad) {
+                S->status = ST_NAN;
+                S->done = 1;
+            }
+        }
+        if (bad) return;
+    } else {
+        beta = S->beta;
+        alpha = S->alpha;
+        sr = S->s_r;
+    }
+    double acc[2] = {0, 0};
     for (int it = first; it < end; it += stride) {
         const int j = it / segs, i = (it - j * segs) * 512 + 2 * threadIdx.x;
-        if (i >= g.nx) continue;
-        const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
-        const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
-        const int c = i + 2, rr = j + 2;
-        // every load in bounds, issued before the active test
-        const float2 y2 = *reinterpret_cast<const float2*>(L0.y + ql);
-        const double2 p2 = *reinterpret_cast<const double2*>(p + o);
-        const double2 q2 = *reinterpret_cast<const double2*>(q + o);
-        const double2 x2 = *reinterpret_cast<const double2*>(x + o);
-        const double2 r2 = *reinterpret_cast<const double2*>(r + o);
-        const double2 a2v = *reinterpret_cast<const double2*>(az_in + o);
-        const double w0 = dxu[c] * cosu[rr] * dyu[rr], w1 = dxu[c + 1] * cosu[rr] * dyu[rr];
-        const uint16_t a2 = *reinterpret_cast<const uint16_t*>(act + o);
-        float2 b2 = make_float2(0.0f, 0.0f);
-        if (a2) {
-            double2 pn = p2, qn = q2, xn = x2, rn = r2;
-            if (a2 & 0xff) {
-                const double z = (double)y2.x * inv;
-                pn.x = z + beta * p2.x;
-                qn.x = a2v.x + beta * q2.x;
-                xn.x = x2.x + alpha * pn.x;
-                rn.x = r2.x - alpha * qn.x;
-                b2.x = (float)(w0 * rn.x * sr);
-                acc[0] += rn.x * rn.x;
-                acc[1] += (w0 * rn.x) * (w0 * rn.x);
+        if (i < g.nx) {
+            MgB2 cur;
+            mg_b2_load(g, L0, dxu, cosu, dyu, act, p, q, r, x, az_in, j, i, cur);
+            const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
+            const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
+            float2 b2 = make_float2(0.0f, 0.0f);
+            if (cur.a) {
+                double2 pn = cur.p, qn = cur.q, xn = cur.x, rn = cur.r;
+                if (cur.a & 0xff) {
+                    const double z = (double)cur.y.x * inv;
+                    pn.x = z + beta * cur.p.x;
+                    qn.x = cur.az.x + beta * cur.q.x;
+                    xn.x = cur.x.x + alpha * pn.x;
+                    rn.x = cur.r.x - alpha * qn.x;
+                    b2.x = (float)(cur.w0 * rn.x * sr);
+                    acc[0] += rn.x * rn.x;
+                    acc[1] += (cur.w0 * rn.x) * (cur.w0 * rn.x);
+                }
+                if (cur.a >> 8) {
+                    const double z = (double)cur.y.y * inv;
+                    pn.y = z + beta * cur.p.y;
+                    qn.y = cur.az.y + beta * cur.q.y;
+                    xn.y = cur.x.y + alpha * pn.y;
+                    rn.y = cur.r.y - alpha * qn.y;
+                    b2.y = (float)(cur.w1 * rn.y * sr);
+                    acc[0] += rn.y * rn.y;
+                    acc[1] += (cur.w1 * rn.y) * (cur.w1 * rn.y);
+                }
+                *reinterpret_cast<double2*>(p + o) = pn;
+                *reinterpret_cast<double2*>(q + o) = qn;
+                *reinterpret_cast<double2*>(x + o) = xn;
+                *reinterpret_cast<double2*>(r + o) = rn;
             }
-            if (a2 >> 8) {
-                const double z = (double)y2.y * inv;
-                pn.y = z + beta * p2.y;
-                qn.y = a2v.y + beta * q2.y;
-                xn.y = x2.y + alpha * pn.y;
-                rn.y = r2.y - alpha * qn.y;
-                b2.y = (float)(w1 * rn.y * sr);
-                acc[0] += rn.y * rn.y;
-                acc[1] += (w1 * rn.y) * (w1 * rn.y);
-            }
-            *reinterpret_cast<double2*>(p + o) = pn;
-            *reinterpret_cast<double2*>(q + o) = qn;
-            *reinterpret_cast<double2*>(x + o) = xn;
-            *reinterpret_cast<double2*>(r + o) = rn;
+            *reinterpret_cast<float2*>(L0.b + ql) = b2;
         }
-        *reinterpret_cast<float2*>(L0.b + ql) = b2;
     }
     mg_block_partials<2>(acc, part);
 }
 
 // The outer passes for a grid: on pairs of points when nx is even.
-inline decltype(&k_mg_fcg_a1) mg_pass_a(const Geom& g) {
-    return (g.nx & 1) ? k_mg_fcg_a1 : k_mg_fcg_a;
-}
-inline decltype(&k_mg_fcg_b1) mg_pass_b(const Geom& g) {
-    return (g.nx & 1) ? k_mg_fcg_b1 : k_mg_fcg_b;
+constexpr int MG_PASS_A_THREADS_DECL = 0;
+// Pass B after a separate fold of pass A's partials (the scalars read from S).
+template <class... A>
+void mg_launch_pass_b(const Geom& g, unsigned nblk, hipStream_t s, const MgLev& L0,
+                      const double* dxu, const double* cosu, const double* dyu,
+                      const uint8_t* act, double* p, double* q, double* r, double* x,
+                      const double* az, const double* zscale, MgScal* S, double* part) {
+    if (g.nx & 1)
+        hipLaunchKernelGGL(k_mg_fcg_b1, dim3(nblk), dim3(256), 0, s, g, L0, dxu, cosu, dyu, act, p,
+                           q, r, x, az, zscale, (const MgScal*)S, part);
+    else
+        hipLaunchKernelGGL(k_mg_fcg_b<false>, dim3(nblk), dim3(256), 0, s, g, L0, dxu, cosu, dyu,
+                           act, p, q, r, x, az, zscale, S, (const double*)nullptr, 0, part);
 }
 
 // Start of a (re)started outer iteration from the residual r: the V-cycle
@@ -912,6 +995,8 @@ struct MgState {
     double* zscale = nullptr;
     double* part = nullptr;
     int nblk = 0;                    // workgroups of the outer passes
+    double* part_a = nullptr;        // pass A's partials when pass B folds them
+    int nblk_a = 0;                  // ... and pass A's workgroups then
     hipStream_t gs = nullptr;        // stream the graph is replayed on
     hipEvent_t e_in = nullptr, e_out = nullptr;
     hipGraph_t graph = nullptr;

@@ -854,10 +854,8 @@ void sp_mg_iteration(psi_press* c, hipStream_t s, int K, const psi::MgCheb& ch,
     hipLaunchKernelGGL(k_mg_fold_a, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->S);
     if (ev) ev->rec(5, s);
-    hipLaunchKernelGGL(mg_pass_b(g), dim3(m->nblk), dim3(256), 0, s, g, m->lev[0],
-                       (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
-                       (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
-                       (const double*)m->zscale, (const MgScal*)m->S, m->part);
+    mg_launch_pass_b(g, m->nblk, s, m->lev[0], c->dxt, c->cost, c->dyt, c->act, m->p, m->q, m->r,
+                     m->x, m->az, m->zscale, m->S, m->part);
     if (ev) ev->rec(6, s);
     hipLaunchKernelGGL(k_mg_fold_b, dim3(1), dim3(1024), 0, s, (const double*)m->part, m->nblk,
                        m->nact0, m->S, m->zscale);
@@ -1162,10 +1160,8 @@ int psi_press_mg_kernel_times(psi_press* c, int32_t degree, int32_t reps, double
                            m->az, (const double*)m->zscale, (const MgScal*)m->S_time, m->part);
     });
     us[3] = mg_time(m, reps, [&] {
-        hipLaunchKernelGGL(mg_pass_b(g), dim3(m->nblk), dim3(256), 0, m->gs, g, m->lev[0],
-                           (const double*)c->dxt, (const double*)c->cost, (const double*)c->dyt,
-                           (const uint8_t*)c->act, m->p, m->q, m->r, m->x, (const double*)m->az,
-                           (const double*)m->zscale, (const MgScal*)m->S_time, m->part);
+        mg_launch_pass_b(g, m->nblk, m->gs, m->lev[0], c->dxt, c->cost, c->dyt, c->act, m->p, m->q,
+                         m->r, m->x, m->az, m->zscale, m->S_time, m->part);
     });
     HIPCHK(c, hipGetLastError());
     for (int k = 0; k < 4; ++k)
