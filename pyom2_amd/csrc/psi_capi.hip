@@ -1081,7 +1081,7 @@ int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
     MGTRY(mg_alloc(c, m, &m->S, 1));
     MGTRY(mg_alloc(c, m, &m->S_time, 1));
     MGTRY(mg_alloc(c, m, &m->zscale, 1));
-    m->nblk = (int)std::min<size_t>(nblk((size_t)g.nx * g.ny), MG_OUTER_BLOCKS);
+    m->nblk = (int)std::min<size_t>(nblk((size_t)g.nx * g.ny), PSI_MG_OUTER_BLOCKS);
     MGTRY(mg_alloc(c, m, &m->part, (size_t)MG_NRED * m->nblk));
     {
         // pass A on pairs: two 1024-thread workgroups per CU, at most one per
@@ -1092,7 +1092,7 @@ int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
             ncu < 1)
             ncu = 1;
         const long segs = (long)g.ny * ((g.nx + 2 * MG_PASS_A_THREADS - 1) / (2 * MG_PASS_A_THREADS));
-        long ga = std::min<long>(2L * ncu, segs);
+        long ga = std::min<long>((long)PSI_MG_PASS_A_PER_CU * ncu, segs);
         if (ga >= 8) ga &= ~7L;
         m->nblk_a = (int)std::max<long>(ga, 1);
         MGTRY(mg_alloc(c, m, &m->part_a, (size_t)MG_NRED * m->nblk_a));

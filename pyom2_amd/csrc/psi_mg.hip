@@ -425,7 +425,13 @@ __device__ __forceinline__ double mg_area(const double* dxu, const double* cosu,
     return dxu[i + 2] * cosu[j + 2] * dyu[j + 2];
 }
 
-constexpr int MG_OUTER_BLOCKS = 2048;   // workgroups of the outer passes (grid-stride)
+// workgroups of the outer passes (grid-stride; pass A on pairs: per CU)
+#ifndef PSI_MG_OUTER_BLOCKS
+#define PSI_MG_OUTER_BLOCKS 2048
+#endif
+#ifndef PSI_MG_PASS_A_PER_CU
+#define PSI_MG_PASS_A_PER_CU 2
+#endif
 
 // One partial per slot for this workgroup of NW waves (waves summed in order).
 template <int NR, int NW = 4>
