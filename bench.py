@@ -60,16 +60,16 @@ PRESS_MG_DEFAULT_DEGREE = 4
 
 
 def mg_kernel_names(degree):
-    return {"post_level0": f"k_mg_post<{degree}> (finest level)",
-            "pre_level0": f"k_mg_pre<{degree}> (finest level)",
+    return {"post_level0": f"k_mg_post<{degree}, 8, 4> (finest level)",
+            "pre_level0": f"k_mg_pre<{degree}, 8, 4> (finest level)",
             "fcg_a": "k_mg_fcg_a", "fcg_b": "k_mg_fcg_b"}
 # HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
 # profiles/k_cg_sweep_spherical_2048.md, profiles/k_poly_pcg_spherical_2048.md
 TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6, "poly": 175.5e6}
 # ... of the multigrid path's kernels (profiles/k_mg_spherical_2048.md)
-MG_TRAFFIC_2048 = {"post_level0": 114.0e6, "pre_level0": 92.8e6, "fcg_a": 194.0e6,
-                   "fcg_b": 341.4e6}
+MG_TRAFFIC_2048 = {"post_level0": 116.4e6, "pre_level0": 92.9e6, "fcg_a": 194.1e6,
+                   "fcg_b": 342.8e6}
 # ... of the surface-pressure CG kernels (profiles/k_sp_press_2048x40.md) and of the
 # velocity update at 2048^2 x 40 levels (profiles/levels_kernels_step_2048x40.md)
 PRESS_TRAFFIC_2048 = {"dir_apply": 283.2e6, "update": 206.4e6}

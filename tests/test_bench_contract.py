@@ -32,8 +32,8 @@ def test_defaults(monkeypatch):
 @pytest.mark.parametrize("degree", [4, 5])
 def test_kernel_names_and_bytes(degree):
     names = bench.mg_kernel_names(degree)
-    assert names["post_level0"].startswith(f"k_mg_post<{degree}>")
-    assert names["pre_level0"].startswith(f"k_mg_pre<{degree}>")
+    assert names["post_level0"].startswith(f"k_mg_post<{degree}, 8, 4>")
+    assert names["pre_level0"].startswith(f"k_mg_pre<{degree}, 8, 4>")
     # pass B: z (FP32), Az, p, q, r, x read (FP64), mask, p, q, r, x written, next input (FP32)
     assert bench.MG_KERNEL_BYTES["fcg_b"] == 4 + 8 + 4 * 8 + 1 + 4 * 8 + 4 == 81
     assert bench.MG_KERNEL_BYTES["fcg_a"] == 45
@@ -64,7 +64,7 @@ def test_mg_traffic_constants_come_from_the_committed_profile():
     add up to its own traffic column."""
     rows = _profile_traffic("k_mg_spherical_2048.md")
     names = {"fcg_b": "`k_mg_fcg_b`", "fcg_a": "`k_mg_fcg_a`",
-             "post_level0": "`k_mg_post<5, 32>`, 2048²", "pre_level0": "`k_mg_pre<5, 32>`, 2048²"}
+             "post_level0": "`k_mg_post<5, 8, 4>`, 2048²", "pre_level0": "`k_mg_pre<5, 8, 4>`, 2048²"}
     for k, row in names.items():
         counted, printed = rows[row]
         assert abs(counted - printed) <= 0.002 * printed, (k, counted, printed)
