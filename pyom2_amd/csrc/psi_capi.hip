@@ -192,9 +192,12 @@ __global__ void k_fcg_dir(Geom g, const float* __restrict__ zs, float* __restric
     p32[idx] = zs[idx] - beta * p32[idx];
 }
 
-// q = A p (FP32 operator), (p, W q), (r, W p)
-__global__ void k_fcg_apply(Geom g, const float* __restrict__ E, const float* __restrict__ N,
-                            const float* __restrict__ rx, const float* __restrict__ ry,
+// q = A p, (p, W q), (r, W p).  A is the FP64 operator (q stored in FP32): with
+// the FP32 one the recursion would follow a system whose difference from
+// A x = b grows with |A||x| rather than with the residual, which holds the
+// FP32 legs between residual replacements to small reductions.
+__global__ void k_fcg_apply(Geom g, const double* __restrict__ E, const double* __restrict__ N,
+                            const double* __restrict__ rx, const double* __restrict__ ry,
                             const float* __restrict__ p, const float* __restrict__ r32,
                             const double* __restrict__ dxu, const double* __restrict__ cosu,
                             const double* __restrict__ dyu, float* __restrict__ q,
@@ -209,11 +212,13 @@ __global__ void k_fcg_apply(Geom g, const float* __restrict__ E, const float* __
         if (r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
             const int cl = map_col(c - 1, g), cr = map_col(c + 1, g);
             const size_t il = (size_t)r * P + cl, ir = (size_t)r * P + cr;
-            const float e = E[idx], w = E[il] * rx[c], n = N[idx], s = N[idx - P] * ry[r];
-            const float cc = (((-e) - w) - n) - s;
-            v = s * p[idx - P] + w * p[il] + cc * p[idx] + e * p[ir] + n * p[idx + P];
+            const double e = E[idx], w = E[il] * rx[c], n = N[idx], s = N[idx - P] * ry[r];
+            const double cc = (((-e) - w) - n) - s;
+            const double vd = s * (double)p[idx - P] + w * (double)p[il] + cc * (double)p[idx] +
+                              e * (double)p[ir] + n * (double)p[idx + P];
+            v = (float)vd;
             const double wa = area_w(g, dxu, cosu, dyu, r, c);
-            acc[0] += (double)p[idx] * (double)v * wa;
+            acc[0] += (double)p[idx] * vd * wa;
             acc[1] += (double)r32[idx] * (double)p[idx] * wa;
         }
         q[idx] = v;
@@ -2288,8 +2293,8 @@ int psi_fcg_step(psi_ctx* c, const float* znet, int H, int W) {
                        c->q32, c->dxu, c->cosu, c->dyu, c->z32, c->fcg);
     hipLaunchKernelGGL(k_fcg_dir, dim3(nblk(c->plane)), dim3(256), 0, c->stream, g, c->z32, c->pf32,
                        c->fcg, c->fcg_restart);
-    hipLaunchKernelGGL(k_fcg_apply, dim3(1024), dim3(NT), 0, c->stream, g, c->E32, c->N32, c->rx32,
-                       c->ry32, c->pf32, c->r32, c->dxu, c->cosu, c->dyu, c->q32, c->fcg);
+    hipLaunchKernelGGL(k_fcg_apply, dim3(1024), dim3(NT), 0, c->stream, g, c->E, c->N, c->rx,
+                       c->ry, c->pf32, c->r32, c->dxu, c->cosu, c->dyu, c->q32, c->fcg);
     hipLaunchKernelGGL(k_fcg_update, dim3(1024), dim3(NT), 0, c->stream, g, c->pf32, c->q32, c->act,
                        c->e32, c->r32, c->fcg);
     c->fcg_restart = 0;
@@ -2324,8 +2329,8 @@ int psi_fcg_step_mg(psi_ctx* c, const float* znet, int H, int W) {
                        c->fcg);
     hipLaunchKernelGGL(k_fcg_dir, dim3(nblk(c->plane)), dim3(256), 0, c->stream, g, c->z32, c->pf32,
                        c->fcg, c->fcg_restart);
-    hipLaunchKernelGGL(k_fcg_apply, dim3(1024), dim3(NT), 0, c->stream, g, c->E32, c->N32, c->rx32,
-                       c->ry32, c->pf32, c->r32, c->dxu, c->cosu, c->dyu, c->q32, c->fcg);
+    hipLaunchKernelGGL(k_fcg_apply, dim3(1024), dim3(NT), 0, c->stream, g, c->E, c->N, c->rx,
+                       c->ry, c->pf32, c->r32, c->dxu, c->cosu, c->dyu, c->q32, c->fcg);
     hipLaunchKernelGGL(k_fcg_update, dim3(1024), dim3(NT), 0, c->stream, g, c->pf32, c->q32, c->act,
                        c->e32, c->r32, c->fcg);
     c->fcg_restart = 0;
