@@ -61,6 +61,9 @@ struct psi_ctx {
     double *part = nullptr, *result = nullptr;
     double* part2 = nullptr;      // solver partials, 2 x NRED x G
     int G = 0, G32 = 0;           // resident workgroups of the FP64 / FP32 solver kernels
+    int march = 0;                // wave-column sweep shape (SEG * 10 + D), 0: LDS-tile sweep
+    int Gmarch = 0;               // its grid
+    int Gpart = 0;                // partials per slot the part2 halves hold
     int ncu = 1;                  // compute units of the device
     unsigned* counter = nullptr;
     CgState* st = nullptr;        // 2 states (double-buffered by launch)
@@ -490,6 +493,7 @@ struct CgPath {
     T *u[2], *p[2];
     int G;
     int poly = 0;                  // > 0: Chebyshev-preconditioned FP32 kernels of degree poly
+    int march = 0;                 // > 0: Jacobi sweeps by k_cg_march of this shape
     ChebCoef ch{};
     T* rb[2] = {nullptr, nullptr}; // their residual buffers (u holds z = M r)
 };
@@ -540,14 +544,62 @@ int launch_poly_checked(psi_ctx* c, int mode, const CgPath<T>& P, T* x, const do
 
 // Launch bookkeeping of one solve: launch L reads state/partials (L-1) & 1
 // and writes L & 1 (the setup is launch 0 and reads the host-written state 0).
+// Each half of part2 holds G partials per slot; Gprev is how many the last
+// launch wrote (its grid).
 struct Launches {
     long L = 0;
     int G = 0;
+    int Gprev = 0;
     CgState* st_in(psi_ctx* c) const { return c->st + ((L - 1) & 1); }
     CgState* st_out(psi_ctx* c) const { return c->st + (L & 1); }
     double* part_in(psi_ctx* c) const { return c->part2 + ((L - 1) & 1) * (size_t)NRED * G; }
     double* part_out(psi_ctx* c) const { return c->part2 + (L & 1) * (size_t)NRED * G; }
 };
+
+// Shapes of the wave-column sweep (k_cg_march<T, SEG, D>): SEG rows per
+// wave, D rows of loads in flight; code SEG * 10 + D.  Per launch on the
+// 2048^2 two-island basin, FP64 / FP32: 24 x 6 52.4 / 32.2 us, 32 x 8
+// 55.1 / 33.4 us, 48 x 8 54.3 / 33.2 us; the LDS-tile sweep 54.6 / 35.3 us.
+#define PSI_MARCH_SHAPES(X) X(24, 6) X(32, 8) X(48, 8)
+#ifndef PSI_MARCH_DEFAULT
+#define PSI_MARCH_DEFAULT 246
+#endif
+
+inline int march_grid(const Geom& g, int ncu, int code) {
+#define PSI_MARCH_GRID(S, D) \
+    if (code == S * 10 + D) return march_groups<S>(g, ncu);
+    PSI_MARCH_SHAPES(PSI_MARCH_GRID)
+#undef PSI_MARCH_GRID
+    return 0;
+}
+
+// The Jacobi sweeps' kernel (PSI_MARCH=SEG*10+D selects a march shape, 0 the
+// LDS-tile sweep) and the partials per slot every solver launch fits in;
+// needs c->g, c->G and c->G32.
+void choose_march(psi_ctx* c) {
+    c->march = PSI_MARCH_DEFAULT;
+    if (const char* mv = getenv("PSI_MARCH")) {
+        const int m = atoi(mv);
+        if (m == 0 || march_grid(c->g, c->ncu, m) > 0) c->march = m;
+    }
+    c->Gpart = std::max(c->G, c->G32);
+    c->Gmarch = c->march ? march_grid(c->g, c->ncu, c->march) : 0;
+    c->Gpart = std::max(c->Gpart, c->Gmarch);
+}
+
+template <class T>
+void launch_march(psi_ctx* c, const CgPath<T>& P, T* x, const CgState* si, CgState* so,
+                  const double* pi, int Gprev, double* po) {
+#define PSI_MARCH_LAUNCH(S, D)                                                                \
+    if (P.march == S * 10 + D) {                                                              \
+        hipLaunchKernelGGL((k_cg_march<T, S, D>), dim3(c->Gmarch), dim3(NT), 0,              \
+                           c->stream, c->g, P.E, P.N, P.rx, P.ry, c->act, x, P.u[0], P.u[1],  \
+                           P.p[0], P.p[1], si, so, pi, Gprev, po);                            \
+        return;                                                                               \
+    }
+    PSI_MARCH_SHAPES(PSI_MARCH_LAUNCH)
+#undef PSI_MARCH_LAUNCH
+}
 
 template <class T, class TB>
 int cg_setup(psi_ctx* c, const CgPath<T>& P, const TB* b, double bscale, T* x, bool zero_guess,
@@ -555,7 +607,8 @@ int cg_setup(psi_ctx* c, const CgPath<T>& P, const TB* b, double bscale, T* x, b
     c->h_st[2] = init;
     HIPCHK(c, hipMemcpyAsync(c->st, &c->h_st[2], sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     Ln.L = 0;
-    Ln.G = P.G;
+    Ln.G = c->Gpart;
+    Ln.Gprev = P.G;
     if (P.poly > 0) {
         // the polynomial path starts from a zero first guess on the residual b
         if constexpr (std::is_same<TB, double>::value) {
@@ -592,15 +645,20 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
     if (events) HIPCHK(c, hipEventRecord(c->ts0, c->stream));
     auto issue = [&]() -> int {
         for (int k = 0; k < batch && Ln.L - L0 < max_launch; ++k, ++Ln.L) {
-            if (P.poly > 0)
+            if (P.poly > 0) {
                 (void)launch_poly_checked<T>(c, PM_SWEEP, P, x, nullptr, 0.0, nullptr, 0.0, 0,
                                              Ln.st_in(c), Ln.st_out(c), Ln.part_in(c),
                                              Ln.part_out(c));
-            else
+            } else if (P.march > 0) {
+                launch_march<T>(c, P, x, Ln.st_in(c), Ln.st_out(c), Ln.part_in(c), Ln.Gprev,
+                                Ln.part_out(c));
+                Ln.Gprev = c->Gmarch;
+            } else {
                 hipLaunchKernelGGL((k_cg_sweep<T>), dim3(P.G), dim3(NT), 0, c->stream, c->g,
                                    P.E, P.N, P.rx, P.ry, c->act, x, P.u[0], P.u[1], P.p[0],
                                    P.p[1], Ln.st_in(c), Ln.st_out(c), Ln.part_in(c),
                                    Ln.part_out(c));
+            }
         }
         int r = check_launch(c);
         if (r) return r;
@@ -673,6 +731,7 @@ int cg_refresh(psi_ctx* c, const CgPath<T>& P, double rscale, double rr_thresh, 
                            (const T*)P.p[1], c->result, rr_thresh, new_max_it, Ln.st_in(c),
                            Ln.st_out(c), Ln.part_out(c));
     }
+    Ln.Gprev = P.G;
     ++Ln.L;
     return check_launch(c);
 }
@@ -806,6 +865,7 @@ int solve_pcg(psi_ctx* c, const psi_opts* o, const CgPath<T>& P, const double* f
 
 int solve_plain(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
     CgPath<double> P{c->E, c->N, c->rx, c->ry, {c->u64[0], c->u64[1]}, {c->p64[0], c->p64[1]}, c->G};
+    P.march = c->march;
     return solve_pcg<double>(c, o, P, forc, sol, sol, 1.0, true, 0, 0.0, info);
 }
 
@@ -845,7 +905,7 @@ int poly_grid(psi_ctx* c, int k) {
     const char* ov = getenv("PSI_GRID_PER_CU");
     if (ov && atoi(ov) > 0) occ = std::min(occ, atoi(ov));
     int G = std::max(1, ncu * std::max(occ, 1));
-    G = std::min(G, std::max(c->G, c->G32));   // the partials buffer holds max(G, G32)
+    G = std::min(G, c->Gpart);   // the partials buffer holds Gpart per slot
     G = std::min(G, poly_geom(c->g, k).ntiles);
     if (G >= 8) G &= ~7;
     c->Gpoly[k] = G;
@@ -885,6 +945,7 @@ int solve_poly(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, p
 int solve_mixed(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi_info* info) {
     CgPath<float> P{c->E32, c->N32, c->rx32, c->ry32, {c->u32[0], c->u32[1]}, {c->p32[0], c->p32[1]},
                     c->G32};
+    P.march = c->march;
     // FP32 legs end after each `inner_reduction` drop of the residual (the
     // recursive FP32 residual is then replaced by the FP64 true one).  The
     // default 0.25 was measured on the 2048^2 two-island basin: legs of 0.2-0.5
@@ -1716,8 +1777,9 @@ int psi_ctx_create(psi_ctx** out, const psi_grid* grid, const psi_fields* f,
             c->ncu = ncu;
             c->G = grid_for(occ64);
             c->G32 = grid_for(occ32);
+            choose_march(c);
         }
-        TRY(dalloc(c, &c->part2, (size_t)2 * NRED * std::max(c->G, c->G32)));
+        TRY(dalloc(c, &c->part2, (size_t)2 * NRED * c->Gpart));
         TRY(dalloc(c, &c->result, 8));
         TRY(dalloc(c, &c->counter, 4));
         TRY(dalloc(c, &c->st, 2));

@@ -176,6 +176,32 @@ __device__ __forceinline__ void tile_schedule(const Geom& g, int& first, int& st
 // order.  Every workgroup does this and gets the bit-identical result, so
 // no grid-wide synchronisation (and no cross-XCD cache maintenance) is
 // needed inside a kernel.  Result valid in thread 0.
+// The loads are unconditional (slots past G read partial 0 and are dropped
+// in fold_add), so all 4 NR of them are in flight at once instead of one
+// round of loads per predicated group.
+template <int NR>
+__device__ __forceinline__ void fold_load(const double* __restrict__ part, int G, int b0,
+                                          double (&x)[4][NR]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int b = b0 + q * NT < G ? b0 + q * NT : 0;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) x[q][k] = part[(size_t)k * G + b];
+    }
+}
+template <int NR>
+__device__ __forceinline__ void fold_add(const double (&x)[4][NR], int G, int b0, unsigned maxmask,
+                                         double (&v)[NR]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool in = b0 + q * NT < G;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const double y = in ? x[q][k] : 0.0;
+            v[k] = ((maxmask >> k) & 1u) ? fmax(v[k], y) : v[k] + y;
+        }
+    }
+}
 template <int NR>
 __device__ __forceinline__ void fold_prev(const double* __restrict__ part, int G, unsigned maxmask,
                                           double (&v)[NR], double* sh) {
@@ -183,17 +209,8 @@ __device__ __forceinline__ void fold_prev(const double* __restrict__ part, int G
     for (int k = 0; k < NR; ++k) v[k] = 0.0;
     for (int b0 = threadIdx.x; b0 < G; b0 += 4 * NT) {
         double x[4][NR];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int b = b0 + q * NT;
-#pragma unroll
-            for (int k = 0; k < NR; ++k) x[q][k] = (b < G) ? part[(size_t)k * G + b] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int k = 0; k < NR; ++k)
-                v[k] = ((maxmask >> k) & 1u) ? fmax(v[k], x[q][k]) : v[k] + x[q][k];
+        fold_load<NR>(part, G, b0, x);
+        fold_add<NR>(x, G, b0, maxmask, v);
     }
     block_reduce<NR>(v, maxmask, sh);
 }
@@ -343,15 +360,37 @@ __device__ void cg_from_refresh(CgState& s, const double* red) {
     cg_prologue(s, gam, d1, d2, d3);
 }
 
-// Every workgroup: fold the previous partials, advance the state, publish it
-// (workgroup 0) and return it in LDS.
+// The state update from the folded partials `red` (valid in thread 0):
+// workgroup 0 publishes it, every workgroup gets it in LDS.
+__device__ __forceinline__ void advance_state_folded(const CgState* __restrict__ st_in,
+                                                     CgState* __restrict__ st_out,
+                                                     const double (&red)[NRED], int done_in,
+                                                     int next_phase, CgState& sst) {
+    if (threadIdx.x == 0) {
+        CgState s = *st_in;
+        if (!done_in) {
+            if (s.phase == PHASE_SETUP) cg_from_setup(s, red);
+            else if (s.phase == PHASE_REFRESH) cg_from_refresh(s, red);
+            else cg_from_sweep(s, red);
+            s.phase = next_phase;
+        }
+        sst = s;
+        if (blockIdx.x == 0) *st_out = s;
+    }
+    __syncthreads();
+}
+
+// Every workgroup: fold the previous partials (G_prev of them per slot; by
+// default the previous launch ran on this launch's grid), advance the state,
+// publish it (workgroup 0) and return it in LDS.
 __device__ __forceinline__ void advance_state(const CgState* __restrict__ st_in,
                                               CgState* __restrict__ st_out,
                                               const double* __restrict__ part_in, int next_phase,
-                                              CgState& sst, double* sRed) {
+                                              CgState& sst, double* sRed, int G_prev = -1) {
     const int done_in = st_in->done;
     double red[NRED];
-    if (!done_in) fold_prev<NRED>(part_in, gridDim.x, 1u << 2, red, sRed);
+    if (!done_in)
+        fold_prev<NRED>(part_in, G_prev > 0 ? G_prev : (int)gridDim.x, 1u << 2, red, sRed);
     if (threadIdx.x == 0) {
         CgState s = *st_in;
         if (!done_in) {
@@ -634,6 +673,259 @@ __global__ __launch_bounds__(NT) void k_cg_sweep(
             }
         }
         __syncthreads();
+    }
+    block_reduce<NRED>(acc, 1u << 2, sRed);
+    write_partials<NRED>(acc, part_out);
+}
+
+// ---------------------------------------------------------------------------
+// The same fused iteration as a wave-column march.  Each wave owns a strip
+// of MOWN = 60 columns and a segment of SEG rows: one frame column per lane
+// (the strip with its 2-column margin each side), and it walks down the
+// segment row by row with every value in registers.  West/east neighbours
+// come from the adjacent lanes by DPP whole-wave shifts; north/south
+// neighbours are the rows before and after in the march.  No LDS frame and
+// no barrier until the final reduction: each row is loaded once (plus the
+// two rows either side of the segment), the loads of the next D rows are in
+// flight while a row is computed, and the per-point arithmetic is
+// k_cg_sweep's (stencil5's terms and order, the same tests), so both give
+// the same iterates up to the order of the partial sums.
+//
+// Per loaded row r (march step i = r - R0 + 2):
+//   p(r) = u(r) + beta p_old(r)
+//   row r-1 (i >= 2): A p, u_new = u - alpha Z (A p) on active points
+//   row r-2 (i >= 4, an own row): w = A u_new, s = A p, outputs, reductions
+// ---------------------------------------------------------------------------
+constexpr int MW = 64;            // frame columns per wave (one per lane)
+constexpr int MOWN = MW - 4;      // own columns per strip
+
+__device__ __forceinline__ int lane_shift_i(int v, bool west) {
+    // wave_shr:1 (0x138): lane l gets lane l-1; wave_shl:1 (0x130): l gets l+1;
+    // the lane with no source gets 0
+    return west ? __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true)
+                : __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ double lane_shift(double v, bool west) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = lane_shift_i((int)(unsigned)(b & 0xffffffffull), west);
+    const int hi = lane_shift_i((int)(unsigned)(b >> 32), west);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+__device__ __forceinline__ float lane_shift(float v, bool west) {
+    return __int_as_float(lane_shift_i(__float_as_int(v), west));
+}
+
+// The 5-point operator of one lane's cell given its coefficients and the
+// values south, here and north; west/east from the neighbouring lanes.
+// Same terms and order as stencil5.
+template <class T>
+__device__ __forceinline__ T march_apply(T so, T w, T cc, T e, T n, T vs, T vc, T vn) {
+    const T vw = lane_shift(vc, true), ve = lane_shift(vc, false);
+    T r = so * vs;
+    r = r + w * vw;
+    r = r + cc * vc;
+    r = r + e * ve;
+    r = r + n * vn;
+    return r;
+}
+
+template <class T>
+struct MarchRow {   // one loaded row of a lane's column, as loaded
+    T u, po, e, n, x, ry;
+    int a;
+};
+
+// Work items of the march (strips x segments) and its grid: a multiple of
+// the CU count, so that every CU holds the same number of workgroups, and
+// the items spread evenly over the workgroups (at most NW each, one per
+// wave).  A CU's vector memory pipeline is shared by its waves: with an
+// uneven grid the CUs holding one workgroup more set the launch's length.
+template <int SEG>
+__host__ __device__ inline int march_items(const Geom& g) {
+    return ((g.nx + MOWN - 1) / MOWN) * ((g.ny + SEG - 1) / SEG);
+}
+template <int SEG>
+inline int march_groups(const Geom& g, int ncu) {
+    const long items = march_items<SEG>(g), per = (long)NW * ncu;
+    return (int)(((items + per - 1) / per) * ncu);
+}
+
+template <class T, int SEG, int D>
+__global__ __launch_bounds__(NT) void k_cg_march(
+    Geom g, const T* __restrict__ E, const T* __restrict__ N, const T* __restrict__ rx,
+    const T* __restrict__ ry, const uint8_t* __restrict__ act, T* __restrict__ x, T* ub0, T* ub1,
+    T* pb0, T* pb1, const CgState* __restrict__ st_in, CgState* __restrict__ st_out,
+    const double* __restrict__ part_in, int G_prev, double* __restrict__ part_out) {
+    __shared__ double sRed[NRED * NW];
+    __shared__ CgState sst;
+    const int phase_in = st_in->phase, n_in = st_in->n;
+    const int n_pred = st_in->done ? n_in
+                                   : (phase_in == PHASE_SETUP ? 0
+                                                              : (phase_in == PHASE_SWEEP ? n_in + 1
+                                                                                         : n_in));
+    const int odd = n_pred & 1;
+    const T* __restrict__ u_in = odd ? ub1 : ub0;
+    const T* __restrict__ p_in = odd ? pb1 : pb0;
+    T* __restrict__ u_out = odd ? ub0 : ub1;
+    T* __restrict__ p_out = odd ? pb0 : pb1;
+
+    // work item of this wave: strips in order, the segments of a strip
+    // consecutive, each XCD a contiguous range
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nseg = (g.ny + SEG - 1) / SEG;
+    // (wave-uniform values in scalar registers: the march's loop and branches
+    // are then scalar, and no load result is copied at a join)
+    const long nitems = march_items<SEG>(g), wg = xcd_tile(blockIdx.x, gridDim.x);
+    const int first = (int)(wg * nitems / gridDim.x), end = (int)((wg + 1) * nitems / gridDim.x);
+    const int item = __builtin_amdgcn_readfirstlane(first + wv);
+    const bool live = item < end;
+    const int strip = live ? item / nseg : 0, seg = live ? item - strip * nseg : 0;
+    const int c = strip * MOWN + lane;              // frame column (array column before wrap)
+    // Lanes past the array's last column (closed basins) load column 0
+    // instead: an own point's results depend only on lanes up to nx+3, so
+    // their values are never used.  Own columns (2 .. nx+1) map to
+    // themselves, also when the grid is cyclic.
+    const int cm = map_col(c, g);
+    const int cs = (cm >= 0 && cm < g.nx + 4) ? cm : 0;
+    const bool own_col = lane >= 2 && lane < MW - 2 && c <= g.nx + 1;
+    const int R0 = 2 + seg * SEG;
+    const int H = live ? min(SEG, g.ny + 2 - R0) : 0;
+    const int nrows = __builtin_amdgcn_readfirstlane(live ? H + 4 : 0);   // rows R0-2 .. R0+H+1
+    const int rlast = R0 + H + 1;
+    // Odd segments march up, even ones down: two neighbouring segments then
+    // reach the two rows either side of their common edge at the same time
+    // (both at the start or both at the end of the march), and the second
+    // reads them from L2 instead of memory.
+    const bool up = seg & 1;
+    const int rstart = up ? rlast : R0 - 2, dir = up ? -1 : 1;
+    const size_t P = g.pitch;
+    const T rxv = rx[cs];
+
+    // Loads only, every one in bounds and none under a branch, so they stay
+    // in flight until the march consumes the row D steps later.  Steps past
+    // the end reload the last row (unused); x is read on every loaded row
+    // but used only on own rows.  Idle waves (live false) load rows 0..3 of
+    // strip 0 and use nothing.
+    auto load = [&](MarchRow<T>& m, int i) {
+        const int r = up ? max(rlast - i, R0 - 2) : min(R0 - 2 + i, rlast);
+        const size_t o = (size_t)r * P + cs;
+        m.u = u_in[o];
+        m.po = p_in[o];
+        m.e = E[o];
+        m.n = N[o];
+        m.a = act[o];
+        m.x = x[o];
+        m.ry = ry[r];
+    };
+
+    // the previous launch's partials first (up to 4 NT of them per slot), then
+    // the first D rows: vector loads complete in order, so the fold then
+    // waits for its own loads only
+    const int done_in = st_in->done;
+    double pre[4][NRED];
+    fold_load<NRED>(part_in, G_prev, threadIdx.x, pre);   // (valid memory also when done)
+    MarchRow<T> ring[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) load(ring[q], q);   // (finished solves return below)
+    __builtin_amdgcn_sched_barrier(0);   // keep every load above the fold's first use
+    {
+        double red[NRED] = {0, 0, 0, 0, 0, 0};
+        if (!done_in) {
+            fold_add<NRED>(pre, G_prev, threadIdx.x, 1u << 2, red);
+            for (int b0 = threadIdx.x + 4 * NT; b0 < G_prev; b0 += 4 * NT) {   // larger grids
+                double x[4][NRED];
+                fold_load<NRED>(part_in, G_prev, b0, x);
+                fold_add<NRED>(x, G_prev, b0, 1u << 2, red);
+            }
+            block_reduce<NRED>(red, 1u << 2, sRed);
+        }
+        advance_state_folded(st_in, st_out, red, done_in, PHASE_SWEEP, sst);
+    }
+    if (sst.done) return;
+    const T alpha = (T)sst.alpha, beta = (T)sst.beta;
+
+    double acc[NRED] = {0, 0, 0, 0, 0, 0};
+    // march state, for the step that loads row r (the rows before it in the
+    // march are r-1, r-2, r-3, written for a downward march): p of rows r-2,
+    // r-1; the loaded row r-1; N of row r-2; the finished row r-2
+    // (coefficients, A p, u_new, p, x, mask); u_new of r-3
+    T p_m2 = 0, p_m1 = 0, n_m2 = 0, un_m3 = 0;
+    MarchRow<T> L1{};
+    T c1_e = 0, c1_w = 0, c1_n = 0, c1_s = 0, c1_cc = 0, c1_ap = 0, c1_un = 0, c1_p = 0,
+      c1_x = 0;
+    int c1_a = 0;
+    T ry_m1 = 0;
+    // The march is unrolled whole (SEG + 4 steps), so no load in flight is
+    // copied at a loop edge; a segment cut short by the grid runs the same
+    // steps, those past its last row keeping nothing.
+    if (live) {   // (idle waves still join the final reduction)
+#pragma unroll
+        for (int i0 = 0; i0 < SEG + 4; i0 += D) {
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                const int i = i0 + q;
+                if (i < SEG + 4) {
+                    const MarchRow<T> cur = ring[q];
+                    const int r = rstart + dir * i;
+                    const T ry_r = cur.ry;
+                    const T p0 = cur.u + beta * cur.po;
+                    T c0_e = 0, c0_w = 0, c0_n = 0, c0_s = 0, c0_cc = 0, c0_ap = 0, c0_un = 0;
+                    if (i >= 2) {
+                        // row r-1: its coefficients, A p and u_new
+                        c0_e = L1.e;
+                        c0_w = lane_shift(L1.e, true) * rxv;
+                        c0_n = L1.n;
+                        c0_s = (up ? cur.n : n_m2) * ry_m1;
+                        c0_cc = (((-c0_e) - c0_w) - c0_n) - c0_s;
+                        c0_ap = march_apply(c0_s, c0_w, c0_cc, c0_e, c0_n, up ? p0 : p_m2, p_m1,
+                                            up ? p_m2 : p0);
+                        c0_un = L1.a ? L1.u - alpha * ((T(1) / c0_cc) * c0_ap) : L1.u;
+                    }
+                    if (i >= 4 && i < nrows) {
+                        // row r-2, an own row: w = A u_new, s = A p, outputs
+                        const T wv2 = march_apply(c1_s, c1_w, c1_cc, c1_e, c1_n, up ? c0_un : un_m3,
+                                                  c1_un, up ? un_m3 : c0_un);
+                        if (own_col) {
+                            const size_t o = (size_t)(r - 2 * dir) * P + c;
+                            put(x + o, c1_x + alpha * c1_p);
+                            put(u_out + o, c1_un);
+                            put(p_out + o, c1_p);
+                            const double ud = (double)c1_un, pd = (double)c1_p, ccd = (double)c1_cc;
+                            const double wd = (double)wv2, sd = (double)c1_ap;
+                            acc[0] += ud * ud * ccd;
+                            acc[1] += ud * wd;
+                            acc[2] = fmax(acc[2], fabs(pd));
+                            if (c1_a) {
+                                const double rv = ud * ccd;
+                                acc[3] += rv * rv;
+                            }
+                            acc[4] += ud * sd + pd * wd;
+                            acc[5] += pd * sd;
+                        }
+                    }
+                    // shift the march by one row
+                    un_m3 = c1_un;
+                    c1_e = c0_e;
+                    c1_w = c0_w;
+                    c1_n = c0_n;
+                    c1_s = c0_s;
+                    c1_cc = c0_cc;
+                    c1_ap = c0_ap;
+                    c1_un = c0_un;
+                    c1_p = p_m1;
+                    c1_x = L1.x;
+                    c1_a = L1.a;
+                    n_m2 = L1.n;
+                    ry_m1 = ry_r;
+                    L1 = cur;
+                    p_m2 = p_m1;
+                    p_m1 = p0;
+                    // the slot's row is consumed: reload it (in the same registers)
+                    if (i + D < SEG + 4) load(ring[q], i + D);
+                }
+            }
+        }
     }
     block_reduce<NRED>(acc, 1u << 2, sRed);
     write_partials<NRED>(acc, part_out);

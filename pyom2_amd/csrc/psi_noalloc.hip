@@ -200,6 +200,8 @@ extern "C" int psi_congrad_noalloc(const psi_grid* grid, const psi_fields* f,
     c.G = std::min(d->G, g.ntiles);
     if (c.G >= 8) c.G &= ~7;
     c.G32 = c.G;
+    choose_march(&c);   // the same sweep kernel as psi_congrad's
+    if (c.Gpart > NA_MAX_PART) return fail(nullptr, PSI_BAD_ARG, "internal: partials%s");
     memset(d->h_st, 0, sizeof d->h_st);
 
 #define NA_HIP(x)                                                                         \
