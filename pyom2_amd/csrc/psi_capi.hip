@@ -565,19 +565,35 @@ struct Launches {
 #define PSI_MARCH_DEFAULT 246
 #endif
 
-inline int march_grid(const Geom& g, int ncu, int code) {
-#define PSI_MARCH_GRID(S, D) \
-    if (code == S * 10 + D) return march_groups<S>(g, ncu);
+// Grid of march shape `code` (0: not a shape), from the resident workgroups
+// per CU of its FP64 kernel; *one_round: every wave has at most one item.
+inline int march_grid(const Geom& g, int ncu, int code, bool* one_round = nullptr) {
+    int occ = 0;
+#define PSI_MARCH_GRID(S, D)                                                                  \
+    if (code == S * 10 + D) {                                                                 \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg_march<double, S, D>, NT,  \
+                                                         0) != hipSuccess)                    \
+            occ = 1;                                                                          \
+        if (const char* ov = getenv("PSI_GRID_PER_CU"))                                       \
+            if (atoi(ov) > 0) occ = std::min(occ, atoi(ov));                                  \
+        const int G = march_groups<S>(g, ncu, occ);                                           \
+        if (one_round) *one_round = march_items<S>(g) <= (long)NW * G;                        \
+        return G;                                                                             \
+    }
     PSI_MARCH_SHAPES(PSI_MARCH_GRID)
 #undef PSI_MARCH_GRID
     return 0;
 }
 
-// The Jacobi sweeps' kernel (PSI_MARCH=SEG*10+D selects a march shape, 0 the
-// LDS-tile sweep) and the partials per slot every solver launch fits in;
-// needs c->g, c->G and c->G32.
+// The Jacobi sweeps' kernel and the partials per slot every solver launch
+// fits in; needs c->g, c->G and c->G32.  The march when one round of waves
+// covers the grid (2048^2 FP64 52.5 us against 54.6 us for the LDS-tile
+// sweep), else the LDS-tile sweep: with several items per wave the march
+// streams HBM worse (4096^2: 316 us against 250 us).  PSI_MARCH=SEG*10+D
+// forces a march shape, PSI_MARCH=0 the LDS-tile sweep.
 void choose_march(psi_ctx* c) {
-    c->march = PSI_MARCH_DEFAULT;
+    bool one = false;
+    c->march = march_grid(c->g, c->ncu, PSI_MARCH_DEFAULT, &one) > 0 && one ? PSI_MARCH_DEFAULT : 0;
     if (const char* mv = getenv("PSI_MARCH")) {
         const int m = atoi(mv);
         if (m == 0 || march_grid(c->g, c->ncu, m) > 0) c->march = m;

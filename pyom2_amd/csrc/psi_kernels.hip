@@ -17,6 +17,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "psi_device.h"
 
 namespace psi {
@@ -738,17 +740,52 @@ struct MarchRow {   // one loaded row of a lane's column, as loaded
 
 // Work items of the march (strips x segments) and its grid: a multiple of
 // the CU count, so that every CU holds the same number of workgroups, and
-// the items spread evenly over the workgroups (at most NW each, one per
-// wave).  A CU's vector memory pipeline is shared by its waves: with an
-// uneven grid the CUs holding one workgroup more set the launch's length.
+// the items spread evenly over the workgroups.  A CU's vector memory
+// pipeline is shared by its waves: with an uneven grid the CUs holding one
+// workgroup more set the launch's length.  Up to NW items per workgroup
+// (one per wave) the grid is ceil(items / (NW CUs)) workgroups per CU; past
+// the resident limit `per_cu` the waves take several items in turn.
 template <int SEG>
 __host__ __device__ inline int march_items(const Geom& g) {
     return ((g.nx + MOWN - 1) / MOWN) * ((g.ny + SEG - 1) / SEG);
 }
 template <int SEG>
-inline int march_groups(const Geom& g, int ncu) {
+inline int march_groups(const Geom& g, int ncu, int per_cu) {
     const long items = march_items<SEG>(g), per = (long)NW * ncu;
-    return (int)(((items + per - 1) / per) * ncu);
+    const long k = std::min<long>((items + per - 1) / per, std::max(per_cu, 1));
+    return (int)(k * ncu);
+}
+
+// One work item's place: the segment's rows R0 .. R0+H-1 of a strip, the
+// lane's column, and the direction of its march.
+struct MarchItem {
+    int R0, H, rlast, c, cs, up;
+    bool live, own_col;
+};
+
+template <int SEG>
+__device__ __forceinline__ MarchItem march_item(const Geom& g, int item, int end) {
+    MarchItem m;
+    const int lane = threadIdx.x & 63, nseg = (g.ny + SEG - 1) / SEG;
+    m.live = item < end;
+    const int strip = m.live ? item / nseg : 0, seg = m.live ? item - strip * nseg : 0;
+    m.c = strip * MOWN + lane;                      // frame column (array column before wrap)
+    // Lanes past the array's last column (closed basins) load column 0
+    // instead: an own point's results depend only on lanes up to nx+3, so
+    // their values are never used.  Own columns (2 .. nx+1) map to
+    // themselves, also when the grid is cyclic.
+    const int cm = map_col(m.c, g);
+    m.cs = (cm >= 0 && cm < g.nx + 4) ? cm : 0;
+    m.own_col = lane >= 2 && lane < MW - 2 && m.c <= g.nx + 1;
+    m.R0 = 2 + seg * SEG;
+    m.H = m.live ? min(SEG, g.ny + 2 - m.R0) : 0;
+    m.rlast = m.R0 + m.H + 1;
+    // Odd segments march up, even ones down: two neighbouring segments then
+    // reach the two rows either side of their common edge at the same time
+    // (both at the start or both at the end of the march), and the second
+    // reads them from L2 instead of memory.
+    m.up = seg & 1;
+    return m;
 }
 
 template <class T, int SEG, int D>
@@ -770,46 +807,27 @@ __global__ __launch_bounds__(NT) void k_cg_march(
     T* __restrict__ u_out = odd ? ub0 : ub1;
     T* __restrict__ p_out = odd ? pb0 : pb1;
 
-    // work item of this wave: strips in order, the segments of a strip
-    // consecutive, each XCD a contiguous range
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nseg = (g.ny + SEG - 1) / SEG;
-    // (wave-uniform values in scalar registers: the march's loop and branches
-    // are then scalar, and no load result is copied at a join)
+    // Work items of this workgroup: a contiguous range of the items (strips
+    // in order, the segments of a strip consecutive), each XCD a contiguous
+    // range of workgroups; wave w takes items first + w, first + w + NW, ...
+    // (The item index is wave-uniform and kept in a scalar register: the
+    // march's branches are then scalar, and no load result is copied at a
+    // join.)
+    const int wv = threadIdx.x >> 6;
     const long nitems = march_items<SEG>(g), wg = xcd_tile(blockIdx.x, gridDim.x);
     const int first = (int)(wg * nitems / gridDim.x), end = (int)((wg + 1) * nitems / gridDim.x);
-    const int item = __builtin_amdgcn_readfirstlane(first + wv);
-    const bool live = item < end;
-    const int strip = live ? item / nseg : 0, seg = live ? item - strip * nseg : 0;
-    const int c = strip * MOWN + lane;              // frame column (array column before wrap)
-    // Lanes past the array's last column (closed basins) load column 0
-    // instead: an own point's results depend only on lanes up to nx+3, so
-    // their values are never used.  Own columns (2 .. nx+1) map to
-    // themselves, also when the grid is cyclic.
-    const int cm = map_col(c, g);
-    const int cs = (cm >= 0 && cm < g.nx + 4) ? cm : 0;
-    const bool own_col = lane >= 2 && lane < MW - 2 && c <= g.nx + 1;
-    const int R0 = 2 + seg * SEG;
-    const int H = live ? min(SEG, g.ny + 2 - R0) : 0;
-    const int nrows = __builtin_amdgcn_readfirstlane(live ? H + 4 : 0);   // rows R0-2 .. R0+H+1
-    const int rlast = R0 + H + 1;
-    // Odd segments march up, even ones down: two neighbouring segments then
-    // reach the two rows either side of their common edge at the same time
-    // (both at the start or both at the end of the march), and the second
-    // reads them from L2 instead of memory.
-    const bool up = seg & 1;
-    const int rstart = up ? rlast : R0 - 2, dir = up ? -1 : 1;
+    int item = __builtin_amdgcn_readfirstlane(first + wv);
+    MarchItem cur = march_item<SEG>(g, item, end);
     const size_t P = g.pitch;
-    const T rxv = rx[cs];
 
     // Loads only, every one in bounds and none under a branch, so they stay
-    // in flight until the march consumes the row D steps later.  Steps past
-    // the end reload the last row (unused); x is read on every loaded row
-    // but used only on own rows.  Idle waves (live false) load rows 0..3 of
-    // strip 0 and use nothing.
-    auto load = [&](MarchRow<T>& m, int i) {
-        const int r = up ? max(rlast - i, R0 - 2) : min(R0 - 2 + i, rlast);
-        const size_t o = (size_t)r * P + cs;
+    // in flight until the march consumes the row D steps later.  Row i of an
+    // item's march; steps past its end reload its last row (unused); x is
+    // read on every loaded row but used only on own rows.  Idle waves load
+    // rows of strip 0 and use nothing.
+    auto load = [&](MarchRow<T>& m, const MarchItem& it, int i) {
+        const int r = it.up ? max(it.rlast - i, it.R0 - 2) : min(it.R0 - 2 + i, it.rlast);
+        const size_t o = (size_t)r * P + it.cs;
         m.u = u_in[o];
         m.po = p_in[o];
         m.e = E[o];
@@ -827,7 +845,7 @@ __global__ __launch_bounds__(NT) void k_cg_march(
     fold_load<NRED>(part_in, G_prev, threadIdx.x, pre);   // (valid memory also when done)
     MarchRow<T> ring[D];
 #pragma unroll
-    for (int q = 0; q < D; ++q) load(ring[q], q);   // (finished solves return below)
+    for (int q = 0; q < D; ++q) load(ring[q], cur, q);   // (finished solves return below)
     __builtin_amdgcn_sched_barrier(0);   // keep every load above the fold's first use
     {
         double red[NRED] = {0, 0, 0, 0, 0, 0};
@@ -846,37 +864,47 @@ __global__ __launch_bounds__(NT) void k_cg_march(
     const T alpha = (T)sst.alpha, beta = (T)sst.beta;
 
     double acc[NRED] = {0, 0, 0, 0, 0, 0};
-    // march state, for the step that loads row r (the rows before it in the
-    // march are r-1, r-2, r-3, written for a downward march): p of rows r-2,
-    // r-1; the loaded row r-1; N of row r-2; the finished row r-2
-    // (coefficients, A p, u_new, p, x, mask); u_new of r-3
-    T p_m2 = 0, p_m1 = 0, n_m2 = 0, un_m3 = 0;
-    MarchRow<T> L1{};
-    T c1_e = 0, c1_w = 0, c1_n = 0, c1_s = 0, c1_cc = 0, c1_ap = 0, c1_un = 0, c1_p = 0,
-      c1_x = 0;
-    int c1_a = 0;
-    T ry_m1 = 0;
-    // The march is unrolled whole (SEG + 4 steps), so no load in flight is
-    // copied at a loop edge; a segment cut short by the grid runs the same
-    // steps, those past its last row keeping nothing.
-    if (live) {   // (idle waves still join the final reduction)
+    // (idle waves skip the loop and still join the final reduction)
+    for (; item < end; item += NW) {
+        // the next item of this wave, whose first D rows the last D steps of
+        // this march load
+        const int inext = __builtin_amdgcn_readfirstlane(item + NW);
+        const MarchItem nxt = march_item<SEG>(g, inext, end);
+        const T rxv = rx[cur.cs];
+        const int rstart = cur.up ? cur.rlast : cur.R0 - 2, dir = cur.up ? -1 : 1;
+        const int nrows = cur.H + 4;                 // rows R0-2 .. R0+H+1
+        const bool up = cur.up, own_col = cur.own_col;
+        const int c = cur.c;
+        // march state, for the step that loads row r (the rows before it in
+        // the march are r-1, r-2, r-3, written for a downward march): p of
+        // rows r-2, r-1; the loaded row r-1; N of row r-2; the finished row
+        // r-2 (coefficients, A p, u_new, p, x, mask); u_new of r-3
+        T p_m2 = 0, p_m1 = 0, n_m2 = 0, un_m3 = 0;
+        MarchRow<T> L1{};
+        T c1_e = 0, c1_w = 0, c1_n = 0, c1_s = 0, c1_cc = 0, c1_ap = 0, c1_un = 0, c1_p = 0,
+          c1_x = 0;
+        int c1_a = 0;
+        T ry_m1 = 0;
+        // The march is unrolled whole (SEG + 4 steps), so no load in flight
+        // is copied at an edge inside it; a segment cut short by the grid
+        // runs the same steps, those past its last row keeping nothing.
 #pragma unroll
         for (int i0 = 0; i0 < SEG + 4; i0 += D) {
 #pragma unroll
             for (int q = 0; q < D; ++q) {
                 const int i = i0 + q;
                 if (i < SEG + 4) {
-                    const MarchRow<T> cur = ring[q];
+                    const MarchRow<T> rw = ring[q];
                     const int r = rstart + dir * i;
-                    const T ry_r = cur.ry;
-                    const T p0 = cur.u + beta * cur.po;
+                    const T ry_r = rw.ry;
+                    const T p0 = rw.u + beta * rw.po;
                     T c0_e = 0, c0_w = 0, c0_n = 0, c0_s = 0, c0_cc = 0, c0_ap = 0, c0_un = 0;
                     if (i >= 2) {
                         // row r-1: its coefficients, A p and u_new
                         c0_e = L1.e;
                         c0_w = lane_shift(L1.e, true) * rxv;
                         c0_n = L1.n;
-                        c0_s = (up ? cur.n : n_m2) * ry_m1;
+                        c0_s = (up ? rw.n : n_m2) * ry_m1;
                         c0_cc = (((-c0_e) - c0_w) - c0_n) - c0_s;
                         c0_ap = march_apply(c0_s, c0_w, c0_cc, c0_e, c0_n, up ? p0 : p_m2, p_m1,
                                             up ? p_m2 : p0);
@@ -884,14 +912,15 @@ __global__ __launch_bounds__(NT) void k_cg_march(
                     }
                     if (i >= 4 && i < nrows) {
                         // row r-2, an own row: w = A u_new, s = A p, outputs
-                        const T wv2 = march_apply(c1_s, c1_w, c1_cc, c1_e, c1_n, up ? c0_un : un_m3,
-                                                  c1_un, up ? un_m3 : c0_un);
+                        const T wv2 = march_apply(c1_s, c1_w, c1_cc, c1_e, c1_n,
+                                                  up ? c0_un : un_m3, c1_un, up ? un_m3 : c0_un);
                         if (own_col) {
                             const size_t o = (size_t)(r - 2 * dir) * P + c;
                             put(x + o, c1_x + alpha * c1_p);
                             put(u_out + o, c1_un);
                             put(p_out + o, c1_p);
-                            const double ud = (double)c1_un, pd = (double)c1_p, ccd = (double)c1_cc;
+                            const double ud = (double)c1_un, pd = (double)c1_p,
+                                         ccd = (double)c1_cc;
                             const double wd = (double)wv2, sd = (double)c1_ap;
                             acc[0] += ud * ud * ccd;
                             acc[1] += ud * wd;
@@ -918,14 +947,20 @@ __global__ __launch_bounds__(NT) void k_cg_march(
                     c1_a = L1.a;
                     n_m2 = L1.n;
                     ry_m1 = ry_r;
-                    L1 = cur;
+                    L1 = rw;
                     p_m2 = p_m1;
                     p_m1 = p0;
-                    // the slot's row is consumed: reload it (in the same registers)
-                    if (i + D < SEG + 4) load(ring[q], i + D);
+                    // the slot's row is consumed: reload it (in the same
+                    // registers) with the row D steps on; in the last D steps
+                    // with the next item's row that its march reads from
+                    // this slot first (row q: step q reads slot q), or, with
+                    // no next item, this item's last row again (in L2)
+                    if (i + D < SEG + 4) load(ring[q], cur, i + D);
+                    else load(ring[q], nxt.live ? nxt : cur, nxt.live ? q : i + D);
                 }
             }
         }
+        cur = nxt;
     }
     block_reduce<NRED>(acc, 1u << 2, sRed);
     write_partials<NRED>(acc, part_out);

@@ -277,3 +277,44 @@ def test_spherical_1024_two_islands_properties():
         assert O.true_relres(b, cf, Z, b.forc, N(sol)) <= 1.01e-10
         sols[mode] = N(sol)
     assert relerr(sols["mixed"], sols["plain"]) < 1e-5
+
+
+@pytest.mark.parametrize("shape,per_cu", [("0", None), ("246", None), ("246", "1"), ("328", "1"),
+                                          ("488", None)])
+def test_jacobi_sweep_kernels_agree(shape, per_cu, monkeypatch):
+    """The Jacobi sweep's kernels (PSI_MARCH: the LDS-tile sweep, or a
+    wave-column march shape; PSI_GRID_PER_CU=1 puts several work items on
+    each wave of the march, whose last steps then prefetch the next item's
+    rows) on a 1440^2 spherical basin (1,440 march items of 24 rows): FP64
+    and FP32-inner solves reach the oracle-recomputed FP64 residual and
+    agree with the LDS-tile sweep's solutions."""
+    b = basins.spherical_basin(seed=5, n=1440, n_islands=2)
+    isl = islands_trace(b.kbot)
+    cf = O.make_coeff(b)
+    Z = O.make_inv(b.nx, b.ny, cf, O.Islands(isl.nisle, isl.offsets, isl.bi, isl.bj, isl.di,
+                                             isl.dj, None))
+    monkeypatch.setenv("PSI_MARCH", shape)
+    if per_cu:
+        monkeypatch.setenv("PSI_GRID_PER_CU", per_cu)
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=DEV)
+    try:
+        for mode in ("plain", "mixed"):
+            sol = s.zeros()
+            its, conv, info = s.congrad_streamfunction(T(b.forc), sol, mode=mode,
+                                                       criterion="relres", relres_target=1e-10,
+                                                       max_iterations=40000)
+            print(f"  march {shape} per_cu {per_cu} {mode}: its={its} conv={conv}")
+            assert conv
+            assert O.true_relres(b, cf, Z, b.forc, N(sol)) <= 1.01e-10
+            key = mode
+            if shape == "0":
+                _SWEEP_REF[key] = (its, N(sol))
+            elif key in _SWEEP_REF:
+                its0, sol0 = _SWEEP_REF[key]
+                assert relerr(N(sol), sol0) <= 1e-9
+                assert abs(its - its0) <= 3 + 0.02 * its0
+    finally:
+        s.close()
+
+
+_SWEEP_REF = {}
