@@ -64,9 +64,22 @@ def mg_kernel_names(degree):
             "pre_level0": f"k_mg_pre<{degree}, 8, 4> (finest level)",
             "fcg_a": "k_mg_fcg_a", "fcg_b": "k_mg_fcg_b"}
 # HBM bytes per sweep launch at 2048^2 from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back):
-# profiles/k_cg_sweep_spherical_2048.md, profiles/k_poly_pcg_spherical_2048.md
-TRAFFIC_2048 = {"plain": 272.1e6, "mixed": 140.7e6, "poly": 175.5e6}
+# (scaled by 2.0 / 1.0, the factors a 1 GiB device copy reads back): the
+# Jacobi sweeps' wave-column march, profiles/k_cg_march_spherical_2048.md, and
+# profiles/k_poly_pcg_spherical_2048.md
+TRAFFIC_2048 = {"plain": 313.9e6, "mixed": 169.0e6, "poly": 175.5e6}
+
+
+def sweep_kernel(n, fp64):
+    """The Jacobi sweep kernel the library runs on an n x n grid: the
+    wave-column march (strips of 60 columns x segments of 24 rows) when one
+    round of its waves covers the grid (4 waves x 3 workgroups per CU on the
+    MI355X's 256 CUs), else the LDS-tile sweep."""
+    t = "double" if fp64 else "float"
+    items = -(-n // 60) * -(-n // 24)
+    return f"k_cg_march<{t}, 24, 6>" if items <= 4 * 3 * 256 else f"k_cg_sweep<{t}>"
+
+
 # ... of the multigrid path's kernels (profiles/k_mg_spherical_2048.md)
 MG_TRAFFIC_2048 = {"post_level0": 116.4e6, "pre_level0": 92.9e6, "fcg_a": 194.1e6,
                    "fcg_b": 342.8e6}
@@ -338,7 +351,7 @@ def main():
                       "kernel": (mg_kernel_names(args.poly_degree or MG_DEFAULT_DEGREE)
                                  [rk["dominant"]] if kpath == "mg" else
                                  "k_poly_pcg<4, sweep>" if kpath == "poly" else
-                                 f"k_cg_sweep<{'double' if kpath == 'plain' else 'float'}>"),
+                                 sweep_kernel(b.nx, kpath == "plain")),
                       "path": kpath, "achieved": round(rk["achieved_GBs"], 1),
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rk["frac"], 4),
                       "traffic": (None if not b.nx == b.ny == 2048 else

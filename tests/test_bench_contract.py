@@ -69,3 +69,16 @@ def test_mg_traffic_constants_come_from_the_committed_profile():
         counted, printed = rows[row]
         assert abs(counted - printed) <= 0.002 * printed, (k, counted, printed)
         assert bench.MG_TRAFFIC_2048[k] == pytest.approx(printed, rel=1e-6), k
+
+
+def test_sweep_traffic_constants_come_from_the_committed_profile():
+    """The traffic the bench line reports for the Jacobi sweeps (plain FP64,
+    mixed FP32) is the committed rocprofv3 summary's for the kernel the bench
+    names at 2048^2."""
+    rows = _profile_traffic("k_cg_march_spherical_2048.md")
+    for mode, fp64 in (("plain", True), ("mixed", False)):
+        name = bench.sweep_kernel(2048, fp64)
+        assert name.startswith("k_cg_march<")
+        counted, _ = rows["`" + name.replace(", 24, 6", "") + "`"]
+        assert bench.TRAFFIC_2048[mode] == pytest.approx(counted, rel=2e-3), mode
+    assert bench.sweep_kernel(4096, True) == "k_cg_sweep<double>"
