@@ -263,6 +263,88 @@ __global__ void __launch_bounds__(256) k_nh_ap(Geom g, int nz, NhTables t,
         part[blockIdx.y * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+// k_nh_ap on LPT consecutive levels per thread (nz a multiple of LPT):
+// the point's row, column and the divisions of its horizontal coefficients
+// (level-independent: ex_e / c2, ex_w / c2) are formed once for its LPT
+// levels, and each level's centre value and mask serve the next level as
+// its "below" ones, so a level costs five loads of p and three mask bytes
+// instead of seven and four.  The coefficients, the stencil and every
+// partial are those of k_nh_ap bit for bit: workgroup (x, y) takes the
+// same chunk of each of its levels as workgroup x of that level in k_nh_ap,
+// and writes one partial per level, in the same slot.
+template <int LPT>
+__global__ void __launch_bounds__(256) k_nh_ap_l(Geom g, int nz, NhTables t,
+                                                 const uint8_t* __restrict__ m8,
+                                                 const double* __restrict__ p,
+                                                 double* __restrict__ Ap,
+                                                 const SpScal* __restrict__ S,
+                                                 double* __restrict__ part) {
+#pragma clang fp contract(off)
+    if (S->done) return;
+    const size_t plane = (size_t)g.pitch * (g.ny + 4);
+    const size_t P = g.pitch;
+    const int k0 = blockIdx.y * LPT;
+    const size_t q_in = 2 * P + 2;
+    double s_dot[LPT];
+#pragma unroll
+    for (int l = 0; l < LPT; ++l) s_dot[l] = 0.0;
+    const size_t chunk = (size_t)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    for (size_t q = chunk * blockDim.x + threadIdx.x; q < plane;
+         q += (size_t)gridDim.x * blockDim.x) {
+        int row, col;
+        sp_rc(g, q, row, col);
+        const bool inner = sp_inner(g, row, col);
+        const size_t qc = inner ? q : q_in;
+        const int rc = inner ? row : 2, cc = inner ? col : 2;
+        // nh_coef's level-independent parts
+        const double c2 = t.c2[rc];
+        const double e0 = t.ex_e[cc] / c2, w0 = t.ex_w[cc] / c2, n0 = t.tn[rc], s0 = t.ts[rc];
+        // the level below the first (k_nh_ap reads the point itself at k = 0)
+        size_t oc = (size_t)k0 * plane + qc;
+        double pd = p[k0 > 0 ? oc - plane : oc];
+        uint8_t m_d = m8[k0 > 0 ? oc - plane : oc];
+        double pc = p[oc];
+#pragma unroll
+        for (int l = 0; l < LPT; ++l) {
+            const int k = k0 + l;
+            const uint8_t m = m8[oc], m_w = m8[oc - 1], m_s = m8[oc - P];
+            const double pu = p[k < nz - 1 ? oc + plane : oc];
+            NhCoef f;
+            f.e = (m & NH_MU) ? e0 : 0.0;
+            f.w = (m_w & NH_MU) ? w0 : 0.0;
+            f.n = (m & NH_MV) ? n0 : 0.0;
+            f.s = (m_s & NH_MV) ? s0 : 0.0;
+            f.u = (k < nz - 1 && (m & NH_MW)) ? t.tu[k] : 0.0;
+            f.d = (k > 0 && (m_d & NH_MW)) ? t.td[k] : 0.0;
+            double c0 = (((0.0 - f.e) - f.w) - f.n) - f.s;
+            if (k < nz - 1) c0 = c0 - f.u;
+            if (k > 0) c0 = c0 - f.d;
+            f.c = c0;
+            const double a = nh_apply(f, pd, p[oc - P], p[oc - 1], pc, p[oc + 1], p[oc + P], pu);
+            const double mt = inner ? (double)((m >> 3) & 1) : 0.0;
+            Ap[(size_t)k * plane + q] = inner ? a : 0.0;
+            s_dot[l] = s_dot[l] + pc * (inner ? a : 0.0) * mt;
+            pd = pc;
+            pc = pu;
+            m_d = m;
+            oc += plane;
+        }
+    }
+    __shared__ double sh[LPT][4];
+#pragma unroll
+    for (int l = 0; l < LPT; ++l) {
+        double v = s_dot[l];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0) sh[l][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < LPT) {
+        const int l = threadIdx.x;
+        part[(size_t)(k0 + l) * gridDim.x + blockIdx.x] =
+            ((sh[l][0] + sh[l][1]) + sh[l][2]) + sh[l][3];
+    }
+}
+
 // p_non_hydro += alpha p, res -= alpha Ap over the whole array (:188-189), and
 // the partials of dot_3D(res, res) (:190).
 __global__ void __launch_bounds__(256) k_nh_update(Geom g, int nz,
@@ -432,6 +514,29 @@ void nh_release_graphs(psi_nh* c) {
 
 bool valid_nh(const psi_nh* c) { return c && c->m8 && c->S; }
 
+// A p with its dot partials: several levels per thread where nz allows
+// (k_nh_ap_l, the same results), else one (k_nh_ap); PSI_NH_LEVELS=1 keeps
+// one level per thread (tests compare the two).
+void nh_launch_ap(psi_nh* c, hipStream_t s, const double* pin, const SpScal* S) {
+    using namespace psi;
+    const int nz = c->nz;
+    const uint8_t* m8 = c->m8;
+    const char* ev = getenv("PSI_NH_LEVELS");
+    const bool one = ev && atoi(ev) == 1;
+    if (one)
+        hipLaunchKernelGGL(k_nh_ap, dim3((unsigned)c->gx, (unsigned)nz), dim3(256), 0, s, c->g,
+                           nz, c->T, m8, pin, c->Ap, S, c->part);
+    else if (nz % 4 == 0)
+        hipLaunchKernelGGL(k_nh_ap_l<4>, dim3((unsigned)c->gx, (unsigned)(nz / 4)), dim3(256), 0,
+                           s, c->g, nz, c->T, m8, pin, c->Ap, S, c->part);
+    else if (nz % 2 == 0)
+        hipLaunchKernelGGL(k_nh_ap_l<2>, dim3((unsigned)c->gx, (unsigned)(nz / 2)), dim3(256), 0,
+                           s, c->g, nz, c->T, m8, pin, c->Ap, S, c->part);
+    else
+        hipLaunchKernelGGL(k_nh_ap, dim3((unsigned)c->gx, (unsigned)nz), dim3(256), 0, s, c->g,
+                           nz, c->T, m8, pin, c->Ap, S, c->part);
+}
+
 void nh_iterations(psi_nh* c, hipStream_t s, double* x) {
     using namespace psi;
     const dim3 grid((unsigned)c->gx, (unsigned)c->nz);
@@ -440,9 +545,7 @@ void nh_iterations(psi_nh* c, hipStream_t s, double* x) {
         hipLaunchKernelGGL(k_nh_p, grid, dim3(256), 0, s, c->g, (const uint8_t*)c->m8,
                            (const double*)c->r, (const double*)c->pv[h], c->pv[1 - h],
                            (const SpScal*)c->S, c->part);
-        hipLaunchKernelGGL(k_nh_ap, grid, dim3(256), 0, s, c->g, c->nz, c->T,
-                           (const uint8_t*)c->m8, (const double*)c->pv[1 - h], c->Ap,
-                           (const SpScal*)c->S, c->part);
+        nh_launch_ap(c, s, (const double*)c->pv[1 - h], (const SpScal*)c->S);
         hipLaunchKernelGGL(k_sp_fold_a, dim3(1), dim3(1024), 0, s, (const double*)c->part, nb,
                            c->S);
         hipLaunchKernelGGL(k_nh_update, grid, dim3(256), 0, s, c->g, c->nz,
@@ -823,8 +926,7 @@ int psi_nh_kernel_times(psi_nh* c, int32_t reps, double* us, int32_t n) {
     us[0] = timed([&] {
         hipLaunchKernelGGL(k_nh_p, grid, dim3(256), 0, c->gs, c->g, (const uint8_t*)c->m8,
                            (const double*)c->r, (const double*)c->pv[0], c->pv[1], St, c->part);
-        hipLaunchKernelGGL(k_nh_ap, grid, dim3(256), 0, c->gs, c->g, c->nz, c->T,
-                           (const uint8_t*)c->m8, (const double*)c->pv[1], c->Ap, St, c->part);
+        nh_launch_ap(c, c->gs, (const double*)c->pv[1], St);
     });
     if (n >= 6) {
         us[4] = timed([&] {
@@ -833,9 +935,7 @@ int psi_nh_kernel_times(psi_nh* c, int32_t reps, double* us, int32_t n) {
                                c->part);
         });
         us[5] = timed([&] {
-            hipLaunchKernelGGL(k_nh_ap, grid, dim3(256), 0, c->gs, c->g, c->nz, c->T,
-                               (const uint8_t*)c->m8, (const double*)c->pv[1], c->Ap, St,
-                               c->part);
+            nh_launch_ap(c, c->gs, (const double*)c->pv[1], St);
         });
     }
     us[1] = timed([&] {

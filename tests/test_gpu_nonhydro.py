@@ -184,3 +184,32 @@ def test_continuity_at_256():
     assert np.abs(div_new - res)[m].max() <= 1e-10 * scale
     assert np.abs(res[m]).max() <= 1e-5 * scale
     s.close()
+
+
+@pytest.mark.parametrize("nz,cyclic", [(16, False), (10, True), (7, False)])
+def test_levels_per_thread_match_one_level_per_thread(nz, cyclic, monkeypatch):
+    """A p on several levels per thread (k_nh_ap_l: four when nz is a
+    multiple of 4, two when even) against one level per thread (k_nh_ap,
+    PSI_NH_LEVELS=1): the same coefficients, stencil and partial sums, so the
+    whole solve -- iterations, its end and p -- is bit for bit the same (also
+    where the reference's CG ends on its own divergence test)."""
+    print(f"\n[nh levels per thread nz={nz} cyclic={cyclic}]", flush=True)
+    b, col = basins.nonhydro_box(11, 130, 96, nz, cyclic, 50.0, 800.0, 1.0, n_islands=2)
+    mW = O.mask_w(col.maskT)
+    nh = basins.nonhydro_fields(col, 11)
+    out = {}
+    for levels in ("1", "auto"):
+        if levels == "1":
+            monkeypatch.setenv("PSI_NH_LEVELS", "1")
+        else:
+            monkeypatch.delenv("PSI_NH_LEVELS", raising=False)
+        s = NonHydrostaticSolver.from_column(b, col, mW, device=DEV)
+        u, v, w, dw, dw_mix, p = _state(nh)
+        its, conv, info = s.solve_non_hydrostatic(u, v, w, dw, dw_mix, p, 60.0,
+                                                  congr_epsilon=1e-12, max_iterations=20000)
+        print(f"  levels={levels}: its={its} conv={conv}", flush=True)
+        out[levels] = (its, N(p[2]), N(u[2]), N(w[2]), conv)
+        s.close()
+    assert out["1"][0] == out["auto"][0] and out["1"][4] == out["auto"][4]
+    for a, b_ in zip(out["1"][1:4], out["auto"][1:4]):
+        np.testing.assert_array_equal(a, b_)
