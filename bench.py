@@ -224,7 +224,7 @@ def main():
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
             ts = time.perf_counter()
-            dpsi[taup1] = 2 * dpsi[tau] - dpsi[taum1]
+            torch.lerp(dpsi[taum1], dpsi[tau], 2.0, out=dpsi[taup1])   # 2 dpsi(tau) - dpsi(taum1)
             if ls is not None:
                 its, conv, linfo = ls.solve(forcings[k], dpsi[taup1], relres_target=args.relres,
                                             max_iterations=2000, policy="learned", poll_every=1)

@@ -103,8 +103,9 @@ typedef struct psi_opts {
                                   [2 poly_lower, 2]; <= 0 picks 0.005 */
     int32_t leg_max_iterations;/* mixed/poly: inner iterations per FP32 leg; <= 0: no cap */
     int32_t time_kernels;      /* PSI_MODE_MG: nonzero records HIP events around the
-                                  four heaviest kernels of every outer iteration and
-                                  reports their mean durations in psi_info.kernel_us */
+                                  four heaviest kernels of one outer iteration per solve
+                                  (the last of the first poll batch) and reports their
+                                  mean durations in psi_info.kernel_us */
     int32_t mg_coarse_degree;  /* PSI_MODE_MG: Chebyshev smoothing steps on the coarse
                                   levels (at most 256 tiles of 64 x 16 cells, whose
                                   kernels are bound by launch latency); 1..10, <= 0

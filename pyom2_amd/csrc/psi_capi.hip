@@ -716,14 +716,19 @@ int cg_iterate(psi_ctx* c, const CgPath<T>& P, T* x, Launches& Ln, int batch, lo
 }
 
 // FP64 residual of (b, x): returns |r|^2, |b|^2 (active points), max|r|, count.
-int residual(psi_ctx* c, const double* b, const double* x, double* r_out, double res[4]) {
+// The FP64 residual's norms into c->result on stream s (r_out: r as well).
+void residual_launch(psi_ctx* c, hipStream_t s, const double* b, const double* x, double* r_out) {
     const Geom& g = c->g;
     // 4 workgroups per CU, a multiple of 8; part holds NRED x (ntiles + 1024)
     const int Gr = std::max(8, std::min(4 * c->ncu, g.ntiles + 1024) & ~7);
-    hipLaunchKernelGGL(k_residual, dim3(Gr), dim3(NT), 0, c->stream, g, c->E, c->N, c->rx, c->ry,
-                       c->act, b, x, r_out, c->part);
-    hipLaunchKernelGGL(k_residual_fold, dim3(1), dim3(NT), 0, c->stream, (const double*)c->part,
-                       Gr, c->result);
+    hipLaunchKernelGGL(k_residual, dim3(Gr), dim3(NT), 0, s, g, c->E, c->N, c->rx, c->ry, c->act,
+                       b, x, r_out, c->part);
+    hipLaunchKernelGGL(k_residual_fold, dim3(1), dim3(NT), 0, s, (const double*)c->part, Gr,
+                       c->result);
+}
+
+int residual(psi_ctx* c, const double* b, const double* x, double* r_out, double res[4]) {
+    residual_launch(c, c->stream, b, x, r_out);
     int rc = check_launch(c);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->h_res, c->result, 4 * sizeof(double), hipMemcpyDeviceToHost,
@@ -989,6 +994,7 @@ void mg_release(psi_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     for (void* p : m->allocs) (void)hipFree(p);
     if (m->h_S) (void)hipHostFree(m->h_S);
+    if (m->h_res) (void)hipHostFree(m->h_res);
     if (m->e_in) (void)hipEventDestroy(m->e_in);
     if (m->e_out) (void)hipEventDestroy(m->e_out);
     if (m->t0) (void)hipEventDestroy(m->t0);
@@ -1181,6 +1187,7 @@ int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
     }
 #undef MGTRY
     HIPCHK(c, hipHostMalloc((void**)&m->h_S, sizeof(MgScal), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc((void**)&m->h_res, 4 * sizeof(double), hipHostMallocDefault));
     HIPCHK(c, hipStreamCreateWithFlags(&m->gs, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&m->e_in, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&m->e_out, hipEventDisableTiming));
@@ -1417,22 +1424,46 @@ int mg_graph(psi_ctx* c, int K, double ratio, bool timed = false, int Kc = 0) {
         Kc);
 }
 
-// Replay the outer-iteration graph on m->gs from iteration n_start (|r|^2 =
-// rr0) until the device flags the end, polling the scalars after every batch.
-// Replays after the end still run the smoothing kernels (they test the flag
-// beside their loads), so a batch is cut to the iterations the observed
-// contraction of |r|^2 says remain.  timed: the last replay of each batch
-// is the timed graph, whose kernel durations are added to info.
-int mg_replay(psi_ctx* c, MgState* m, int batch, bool timed, int n_start, double rr0,
-              psi_info* info) {
+// What the legs of a multigrid solve need from its operator: where its
+// FP64 residual kernels leave the norms {|r|^2, |b|^2, max|r|, count}, and
+// the psi-cell areas that weight the V-cycle input.
+struct MgOperator {
+    const double* res_dev;
+    const double *dxu, *cosu, *dyu;
+};
+
+// Replay the outer-iteration graph on m->gs from iteration n_start until the
+// device flags the end, polling the scalars and the leg residual's norms
+// after every batch (the leg's starting |r|^2 is read at the first poll).
+// Replays after the end still launch their kernels (they test the flag
+// beside their first loads), so a batch is cut to the iterations the
+// observed contraction of |r|^2 says remain, and such a batch, expected to be
+// the leg's last, is followed by resid(m->gs, nullptr) (the FP64 true
+// residual of x) and k_mg_confirm, so the poll that sees the end also sees
+// it confirmed.  timed: the last replay of the first batch is the timed
+// graph, whose kernel durations are added to info (its event nodes cost a
+// few us each, so later batches replay the plain graph).
+template <class Resid>
+int mg_replay(psi_ctx* c, MgState* m, const MgOperator& op, Resid resid, int batch, bool timed,
+              int n_start, psi_info* info) {
     int n_poll = n_start, next = batch;
-    double rr_poll = rr0;
-    for (;;) {
+    double rr_poll = -1.0;
+    for (;; timed = false) {
         for (int k = 0; k < next; ++k)
             HIPCHK(c, hipGraphLaunch(timed && k == next - 1 ? m->exec_t : m->exec, m->gs));
+        const bool last = next < batch;
+        if (last) {
+            resid(m->gs, nullptr);
+            hipLaunchKernelGGL(k_mg_confirm, dim3(1), dim3(64), 0, m->gs, op.res_dev, m->S);
+            HIPCHK(c, hipGetLastError());
+            info->setup_launches += 1;
+        }
         HIPCHK(c, hipMemcpyAsync(m->h_S, m->S, sizeof(MgScal), hipMemcpyDeviceToHost, m->gs));
+        HIPCHK(c, hipMemcpyAsync(m->h_res, op.res_dev, 4 * sizeof(double),
+                                 hipMemcpyDeviceToHost, m->gs));
         HIPCHK(c, hipStreamSynchronize(m->gs));
         const MgScal& h = *m->h_S;
+        if (rr_poll < 0.0) rr_poll = m->h_res[0];   // first poll: never after a confirmation
         if (timed && h.n - n_poll == next) {
             // every replay of the batch iterated, so the events hold the last
             // one's kernels
@@ -1453,6 +1484,78 @@ int mg_replay(psi_ctx* c, MgState* m, int batch, bool timed, int n_start, double
         }
         n_poll = h.n;
         rr_poll = h.rr;
+    }
+}
+
+// The legs of a multigrid solve from m->x (which holds the first guess).  A
+// leg starts on the device: resid(stream, m->r) (the FP64 residual into r and
+// op.res_dev), k_mg_init (state and end tests) and k_mg_start (V-cycle
+// input), so its first batch of replays queues behind them without a host
+// round trip.  A leg that ends on the recursive residual is confirmed by the
+// FP64 true residual, after its last batch or here when the batch that ended
+// it was not expected to; else a new leg starts from the true residual.
+// Out: res (the true residual's norms), bb, the iterations n, the status and
+// the legs that iterated.
+template <class Resid>
+int mg_run_legs(psi_ctx* c, MgState* m, const MgOperator& op, Resid resid, int batch,
+                bool timed, double target, int max_it, psi_info* info, double res[4], double& bb,
+                int& n, int& status, int& legs) {
+    const Geom& g = c->g;
+    n = 0;
+    legs = 0;
+    status = ST_RUNNING;
+    auto sync_resid = [&]() -> int {
+        resid(c->stream, nullptr);
+        int rc = check_launch(c);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(m->h_res, op.res_dev, 4 * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < 4; ++k) res[k] = m->h_res[k];
+        info->setup_launches += 1;
+        return PSI_OK;
+    };
+    for (int leg = 0;; ++leg) {
+        resid(c->stream, m->r);
+        info->setup_launches += 1;
+        hipLaunchKernelGGL(k_mg_init, dim3(1), dim3(64), 0, c->stream, op.res_dev, m->S,
+                           m->zscale, leg == 0 ? 1 : 0, n, max_it, target * target, m->w_mean);
+        hipLaunchKernelGGL(k_mg_start, dim3(nblk((size_t)g.nx * g.ny)), dim3(256), 0, c->stream,
+                           g, m->lev[0], op.dxu, op.cosu, op.dyu, (const uint8_t*)c->act,
+                           (const double*)m->r, m->p, m->q, (const double*)m->zscale,
+                           (const MgScal*)m->S);
+        int rc = check_launch(c);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(m->e_in, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
+        const int n_start = n;
+        HIPCHK(c, hipEventRecord(m->t0, m->gs));
+        rc = mg_replay(c, m, op, resid, batch, timed && leg == 0, n_start, info);
+        if (rc) return rc;
+        HIPCHK(c, hipEventRecord(m->t1, m->gs));
+        HIPCHK(c, hipEventSynchronize(m->t1));
+        float span = 0.0f;
+        (void)hipEventElapsedTime(&span, m->t0, m->t1);
+        info->sweep_ms += span;                          // graph replays of this leg
+        HIPCHK(c, hipEventRecord(m->e_out, m->gs));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, m->e_out, 0));
+        const MgScal h = *m->h_S;
+        info->sweep_launches += h.n - n_start;           // outer iterations they ran
+        if (h.n > n_start) ++legs;
+        n = h.n;
+        status = h.status;
+        bb = h.bb;
+        if (status != ST_CONVERGED) return sync_resid();   // the final iterate's
+        if (h.confirmed == 1) {   // by the norms in op.res_dev (leg start or confirmation)
+            for (int k = 0; k < 4; ++k) res[k] = m->h_res[k];
+            return PSI_OK;
+        }
+        if (h.confirmed == 0) {
+            rc = sync_resid();
+            if (rc) return rc;
+            if (res[0] <= target * target * bb) return PSI_OK;
+        }
+        // the true residual is above the target: a new leg from it
     }
 }
 
@@ -1497,87 +1600,30 @@ int solve_mg(psi_ctx* c, const psi_opts* o, const double* forc, double* sol, psi
     // replays per poll: 8 measured fastest at 2048^2 (each poll idles the GPU
     // ~20 us; batches are cut near the end, so a long one wastes little)
     const int batch = o->check_every > 0 ? o->check_every : 8;
-    // with time_kernels the last replay of every poll batch runs the timed
-    // graph (its event nodes cost about 3 us each)
+    // with time_kernels the last replay of the first poll batch runs the
+    // timed graph (its event nodes cost about 3 us each)
     const bool timed = o->time_kernels != 0;
     int rc = mg_graph(c, K, ratio, false, Kc == K ? 0 : Kc);
     if (rc == PSI_OK && timed) rc = mg_graph(c, K, ratio, true, Kc == K ? 0 : Kc);
     if (rc) return rc;
     const Geom& g = c->g;
     HIPCHK(c, hipMemcpyAsync(m->x, sol, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
-    double res[4];
+    double res[4] = {0.0, 0.0, 0.0, 0.0};
     double bb = 0.0;
-    int n = 0, restarts = 0, status = ST_RUNNING;
-    for (;;) {
-        rc = residual(c, forc, m->x, m->r, res);
-        if (rc) return rc;
-        info->setup_launches += 1;
-        if (restarts == 0) bb = res[1] > 0 ? res[1] : res[0];
-        if (res[0] <= target * target * bb) {
-            status = ST_CONVERGED;
-            break;
-        }
-        if (res[0] != res[0]) {
-            status = ST_NAN;
-            break;
-        }
-        if (n >= o->max_iterations) {
-            status = ST_MAXIT;
-            break;
-        }
-        const double s0 = 1.0 / (m->w_mean * sqrt(res[0] / std::max(res[3], 1.0)));
-        MgScal S0{};
-        S0.s_r = s0;
-        S0.s_pending = s0;
-        S0.rr = res[0];
-        S0.bb = bb;
-        S0.rtarget2 = target * target;
-        S0.n = n;
-        S0.max_it = o->max_iterations;
-        S0.status = ST_RUNNING;
-        *m->h_S = S0;
-        HIPCHK(c, hipMemcpyAsync(m->S, m->h_S, sizeof(MgScal), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(m->zscale, &m->h_S->s_r, sizeof(double), hipMemcpyHostToDevice,
-                                 c->stream));
-        HIPCHK(c, hipMemsetAsync(m->p, 0, c->plane * 8, c->stream));
-        HIPCHK(c, hipMemsetAsync(m->q, 0, c->plane * 8, c->stream));
-        hipLaunchKernelGGL(k_mg_start, dim3(nblk((size_t)g.nx * g.ny)), dim3(256), 0, c->stream,
-                           g, m->lev[0], (const double*)c->dxu, (const double*)c->cosu,
-                           (const double*)c->dyu, (const uint8_t*)c->act, (const double*)m->r,
-                           s0);
-        rc = check_launch(c);
-        if (rc) return rc;
-        HIPCHK(c, hipEventRecord(m->e_in, c->stream));
-        HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
-        const int n_start = n;
-        HIPCHK(c, hipEventRecord(m->t0, m->gs));
-        rc = mg_replay(c, m, batch, timed, n_start, S0.rr, info);
-        if (rc) return rc;
-        HIPCHK(c, hipEventRecord(m->t1, m->gs));
-        HIPCHK(c, hipEventSynchronize(m->t1));
-        float span = 0.0f;
-        (void)hipEventElapsedTime(&span, m->t0, m->t1);
-        info->sweep_ms += span;                          // graph replays of this leg
-        info->sweep_launches += m->h_S->n - n_start;     // outer iterations they ran
-        HIPCHK(c, hipEventRecord(m->e_out, m->gs));
-        HIPCHK(c, hipStreamWaitEvent(c->stream, m->e_out, 0));
-        n = m->h_S->n;
-        status = m->h_S->status;
-        if (status != ST_CONVERGED) break;
-        ++restarts;   // confirm on the FP64 true residual (top of the loop)
-    }
-    // the FP64 true residual of the final iterate
-    if (status != ST_CONVERGED || restarts == 0) {
-        rc = residual(c, forc, m->x, nullptr, res);
-        if (rc) return rc;
-    }
+    int n = 0, status = ST_RUNNING, legs = 0;
+    const MgOperator op{c->result, c->dxu, c->cosu, c->dyu};
+    rc = mg_run_legs(
+        c, m, op,
+        [&](hipStream_t s, double* r_out) { residual_launch(c, s, forc, m->x, r_out); }, batch,
+        timed, target, o->max_iterations, info, res, bb, n, status, legs);
+    if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(sol, m->x, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
     if (info->kernel_samples > 0)
         for (int k = 0; k < 4; ++k) info->kernel_us[k] /= info->kernel_samples;
     info->iterations = status == ST_MAXIT ? o->max_iterations + 1 : n;
     info->status = status_of(status);
     info->converged = status == ST_CONVERGED;
-    info->outer_iterations = restarts;
+    info->outer_iterations = legs;
     info->relres_true = bb > 0 ? sqrt(res[0] / bb) : sqrt(res[0]);
     info->relres_recursive = bb > 0 ? sqrt(m->h_S->rr / bb) : 0.0;
     return info->status;

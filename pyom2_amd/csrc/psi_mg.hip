@@ -83,6 +83,9 @@ struct MgScal {
     double s_r, s_pending;           // scale of the FP32 residual fed to the V-cycle
     double rr, bb, rtarget2;
     int n, done, status, max_it;
+    // FP64 true residual of a leg that converged on the recursive one:
+    // 0 not checked yet, 1 confirmed, -1 above the target (a new leg)
+    int confirmed;
 };
 
 // Coarsest level: y = Ainv b over its active cells.  The workgroup gathers b
@@ -92,16 +95,34 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
                                                   const int* __restrict__ cells, int n,
                                                   const MgScal* __restrict__ S) {
     __shared__ float4 sB[MG_MAX_DENSE];   // 4 MG_MAX_DENSE floats: the largest coarsest level
-    if (S->done) return;
+    const int done = S->done;
+    const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // the row's first MG_MAX_DENSE / 256 16-byte pieces per lane (all of a
+    // row of at most MG_MAX_DENSE cells), loaded beside the gather of b; rows
+    // past n read row 0 and store nothing
+    constexpr int PL = MG_MAX_DENSE / 256;
+    const float4* a = reinterpret_cast<const float4*>(Ainv + (size_t)(row < n ? row : 0) * ld);
+    float4 av[PL];
+#pragma unroll
+    for (int t = 0; t < PL; ++t) {
+        const int k = lane + 64 * t;
+        av[t] = a[k < ld / 4 ? k : 0];
+    }
     float* sb = reinterpret_cast<float*>(sB);
     for (int k = threadIdx.x; k < ld; k += blockDim.x) sb[k] = k < n ? L.b[cells[k]] : 0.0f;
     __syncthreads();
-    const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= n) return;
-    const float4* a = reinterpret_cast<const float4*>(Ainv + (size_t)row * ld);
+    if (done || row >= n) return;
     float s = 0.0f;
-    for (int k = lane; k < ld / 4; k += 64) {
+#pragma unroll
+    for (int t = 0; t < PL; ++t) {
+        const int k = lane + 64 * t;
+        if (k < ld / 4) {
+            const float4 v = av[t], w = sB[k];
+            s += ((v.x * w.x + v.y * w.y) + v.z * w.z) + v.w * w.w;
+        }
+    }
+    for (int k = lane + 64 * PL; k < ld / 4; k += 64) {   // coarsest levels above MG_MAX_DENSE
         const float4 v = a[k], w = sB[k];
         s += ((v.x * w.x + v.y * w.y) + v.z * w.z) + v.w * w.w;
     }
@@ -501,8 +522,23 @@ template <int NR>
 __device__ __forceinline__ void mg_fold(const double* __restrict__ part, int nblocks,
                                         double (&v)[NR]) {
     __shared__ double sh[NR * 16];
-    for (int k = 0; k < NR; ++k) v[k] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    // the first four partials of each thread loaded at once (past nblocks:
+    // partial 0, dropped), then any further ones; the sum runs in block order
+    double x[4][NR];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int b = threadIdx.x + q * blockDim.x;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) x[q][k] = part[(size_t)k * nblocks + (b < nblocks ? b : 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        v[k] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (threadIdx.x + q * blockDim.x < nblocks) v[k] += x[q][k];
+    }
+    for (int b = threadIdx.x + 4 * blockDim.x; b < nblocks; b += blockDim.x)
         for (int k = 0; k < NR; ++k) v[k] += part[(size_t)k * nblocks + b];
     for (int k = 0; k < NR; ++k) {
         double x = v[k];
@@ -591,9 +627,10 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b1(
 // the scale of the next V-cycle input.
 __global__ void __launch_bounds__(1024) k_mg_fold_b(const double* __restrict__ part, int nblocks,
                                                     int nact, MgScal* S, double* zscale) {
-    if (S->done) return;
+    const int done = S->done;   // tested once the fold's loads are in flight
     double v[2];
     mg_fold<2>(part, nblocks, v);
+    if (done) return;
     if (threadIdx.x == 0) {
         *zscale = S->s_r;   // what pass A of the next iteration divides by
         S->rr = v[0];
@@ -614,6 +651,17 @@ __global__ void __launch_bounds__(1024) k_mg_fold_b(const double* __restrict__ p
     }
 }
 
+// Pins a loaded value in registers at this point of the program.  The
+// compiler moves a load that only a data-dependent branch uses below that
+// branch, so a point's operands would be requested only after its mask had
+// arrived: two memory round trips per row segment instead of one.  An empty
+// asm statement that reads and writes the value keeps its load above it,
+// issued with the others.
+__device__ __forceinline__ void mg_pin(double& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void mg_pin(double2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+__device__ __forceinline__ void mg_pin(float2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+__device__ __forceinline__ void mg_pin(float& v) { asm volatile("" : "+v"(v)); }
+
 // ---- the outer passes on pairs of points -----------------------------------
 // The same two passes with each thread taking the points (i, i+1), i even,
 // of a row segment through 16-byte loads and stores (used when nx is even,
@@ -622,35 +670,50 @@ __global__ void __launch_bounds__(1024) k_mg_fold_b(const double* __restrict__ p
 // those of k_mg_fcg_a1 / k_mg_fcg_b1.  Pass A runs on MG_PASS_A_THREADS-thread
 // workgroups, two per CU, so that it leaves few partials: pass B folds them
 // itself instead of a one-workgroup fold kernel between the passes.
-constexpr int MG_PASS_A_THREADS = 1024;
+#ifndef PSI_MG_PASS_A_THREADS
+#define PSI_MG_PASS_A_THREADS 1024
+#endif
+constexpr int MG_PASS_A_THREADS = PSI_MG_PASS_A_THREADS;
 struct MgZ2 {
     double c0, c1;   // z at (i, j), (i+1, j)
     double w0, e1;   // z west of i, east of i+1
     double s0, s1, n0, n1;
 };
 
-__device__ __forceinline__ MgZ2 mg_z_pair(const Geom& g, const MgLev& L0, int i, int j,
-                                         double inv) {
+// The V-cycle output around the pair (i, i+1) of row j, as stored (FP32).
+struct MgY2 {
+    float2 c, s, n;   // rows j, j-1, j+1
+    float w, e;       // west of i, east of i+1
+};
+
+__device__ __forceinline__ MgY2 mg_y_pair(const Geom& g, const MgLev& L0, int i, int j) {
     const int P = L0.pitch, q = (j + 1) * P + i + MG_XOFF;   // even: 8-byte aligned pairs
     int dw = -1, de = 2;
     if (g.cyclic) {
         if (i == 0) dw = g.nx - 1;
         if (i + 1 == g.nx - 1) de = 2 - g.nx;
     }
-    const float2 yc = *reinterpret_cast<const float2*>(L0.y + q);
-    const float2 ys = *reinterpret_cast<const float2*>(L0.y + q - P);
-    const float2 yn = *reinterpret_cast<const float2*>(L0.y + q + P);
-    const float yw = L0.y[q + dw], ye = L0.y[q + de];
+    MgY2 y;
+    y.c = *reinterpret_cast<const float2*>(L0.y + q);
+    y.s = *reinterpret_cast<const float2*>(L0.y + q - P);
+    y.n = *reinterpret_cast<const float2*>(L0.y + q + P);
+    y.w = L0.y[q + dw];
+    y.e = L0.y[q + de];
+    return y;
+}
+
+// z = y / s_r with the values outside the interior replaced by zero.
+__device__ __forceinline__ MgZ2 mg_z_pair(const Geom& g, const MgY2& y, int i, int j, double inv) {
     const bool hs = j > 0, hn = j < g.ny - 1;
     MgZ2 z;
-    z.c0 = (double)yc.x * inv;
-    z.c1 = (double)yc.y * inv;
-    z.s0 = hs ? (double)ys.x * inv : 0.0;
-    z.s1 = hs ? (double)ys.y * inv : 0.0;
-    z.n0 = hn ? (double)yn.x * inv : 0.0;
-    z.n1 = hn ? (double)yn.y * inv : 0.0;
-    z.w0 = (g.cyclic || i > 0) ? (double)yw * inv : 0.0;
-    z.e1 = (g.cyclic || i + 2 < g.nx) ? (double)ye * inv : 0.0;
+    z.c0 = (double)y.c.x * inv;
+    z.c1 = (double)y.c.y * inv;
+    z.s0 = hs ? (double)y.s.x * inv : 0.0;
+    z.s1 = hs ? (double)y.s.y * inv : 0.0;
+    z.n0 = hn ? (double)y.n.x * inv : 0.0;
+    z.n1 = hn ? (double)y.n.y * inv : 0.0;
+    z.w0 = (g.cyclic || i > 0) ? (double)y.w * inv : 0.0;
+    z.e1 = (g.cyclic || i + 2 < g.nx) ? (double)y.e * inv : 0.0;
     return z;
 }
 
@@ -662,7 +725,9 @@ __global__ void __launch_bounds__(MG_PASS_A_THREADS) k_mg_fcg_a(
     const double* __restrict__ q, const double* __restrict__ r, double* __restrict__ az_out,
     const double* __restrict__ zscale, const MgScal* __restrict__ S,
     double* __restrict__ part) {
-    if (S->done) return;
+    // tested once the first row segment's loads are in flight; every thread
+    // of a finished solve returns before the partials' barrier
+    const int done = S->done;
     const double inv = 1.0 / *zscale;   // the scale of the V-cycle's input
     double acc[3] = {0, 0, 0};
     // row segments of 2 MG_PASS_A_THREADS points, each XCD a contiguous range of rows
@@ -674,17 +739,26 @@ __global__ void __launch_bounds__(MG_PASS_A_THREADS) k_mg_fcg_a(
         if (i >= g.nx) continue;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
         const int c = i + 2, rr = j + 2;
-        // every load in bounds, issued before the active test
-        const MgZ2 z = mg_z_pair(g, L0, i, j, inv);
-        const double2 e2 = *reinterpret_cast<const double2*>(E + o);
-        const double2 n2 = *reinterpret_cast<const double2*>(N + o);
-        const double2 s2 = *reinterpret_cast<const double2*>(N + o - g.pitch);
-        const double2 q2 = *reinterpret_cast<const double2*>(q + o);
-        const double2 r2 = *reinterpret_cast<const double2*>(r + o);
-        const double em = E[o - 1], rx0 = rx[c], rx1 = rx[c + 1], ryv = ry[rr];
-        const double wa0 = dxu[c] * cosu[rr] * dyu[rr], wa1 = dxu[c + 1] * cosu[rr] * dyu[rr];
+        // every load in bounds, issued before the active test (pinned there)
+        MgY2 y = mg_y_pair(g, L0, i, j);
+        double2 e2 = *reinterpret_cast<const double2*>(E + o);
+        double2 n2 = *reinterpret_cast<const double2*>(N + o);
+        double2 s2 = *reinterpret_cast<const double2*>(N + o - g.pitch);
+        double2 q2 = *reinterpret_cast<const double2*>(q + o);
+        double2 r2 = *reinterpret_cast<const double2*>(r + o);
+        double em = E[o - 1], rx0 = rx[c], rx1 = rx[c + 1];
+        const double ryv = ry[rr];
+        double dx0 = dxu[c], dx1 = dxu[c + 1];
+        const double cr = cosu[rr], dr = dyu[rr];
         const uint16_t a2 = *reinterpret_cast<const uint16_t*>(act + o);
+        __builtin_amdgcn_sched_barrier(0);   // every load above, in flight together
+        if (done) return;
+        mg_pin(e2), mg_pin(n2), mg_pin(s2), mg_pin(q2), mg_pin(r2);
+        mg_pin(em), mg_pin(rx0), mg_pin(rx1), mg_pin(dx0), mg_pin(dx1);
+        mg_pin(y.c), mg_pin(y.s), mg_pin(y.n), mg_pin(y.w), mg_pin(y.e);
         if (!a2) continue;
+        const MgZ2 z = mg_z_pair(g, y, i, j, inv);
+        const double wa0 = dx0 * cr * dr, wa1 = dx1 * cr * dr;
         double2 az2 = make_double2(0.0, 0.0);
         if (a2 & 0xff) {
             const double e = e2.x, w = em * rx0, n = n2.x, so = s2.x * ryv;
@@ -706,6 +780,7 @@ __global__ void __launch_bounds__(MG_PASS_A_THREADS) k_mg_fcg_a(
         }
         *reinterpret_cast<double2*>(az_out + o) = az2;
     }
+    if (done) return;
     mg_block_partials<3, MG_PASS_A_THREADS / 64>(acc, part);
 }
 
@@ -752,7 +827,9 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
     const double* __restrict__ az_in, const double* __restrict__ zscale, MgScal* S,
     const double* __restrict__ part_a, int ga, double* __restrict__ part) {
     __shared__ double sf[3 * 4];
-    if (S->done) return;
+    // with FOLD tested after the fold (its loads need nothing from S)
+    const int done = S->done;
+    if (!FOLD && done) return;
     const double inv = 1.0 / *zscale;
     // row segments of 512 points, each XCD a contiguous range of rows
     const int segs = (g.nx + 511) / 512;
@@ -776,6 +853,7 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
         for (int k = 0; k < 3; ++k) v[k] = ((sf[k * 4] + sf[k * 4 + 1]) + sf[k * 4 + 2]) + sf[k * 4 + 3];
         // beta = -<z,q>/<p,q>_old, <p,q> = <z,Az> + 2 beta <z,q> + beta^2 <p,q>_old,
         // alpha = <r,z> / <p,q>
+        if (done) return;
         const double pq_old = S->pq_old;
         beta = pq_old != 0.0 ? -v[1] / pq_old : 0.0;
         const double pq = v[0] + 2.0 * beta * v[1] + beta * beta * pq_old;
@@ -804,6 +882,9 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
         if (i < g.nx) {
             MgB2 cur;
             mg_b2_load(g, L0, dxu, cosu, dyu, act, p, q, r, x, az_in, j, i, cur);
+            __builtin_amdgcn_sched_barrier(0);   // every load above, in flight together
+            mg_pin(cur.y), mg_pin(cur.p), mg_pin(cur.q), mg_pin(cur.x), mg_pin(cur.r);
+            mg_pin(cur.az), mg_pin(cur.w0), mg_pin(cur.w1);
             const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
             const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
             float2 b2 = make_float2(0.0f, 0.0f);
@@ -856,18 +937,68 @@ void mg_launch_pass_b(const Geom& g, unsigned nblk, hipStream_t s, const MgLev& 
                            act, p, q, r, x, az, zscale, S, (const double*)nullptr, 0, part);
 }
 
-// Start of a (re)started outer iteration from the residual r: the V-cycle
-// input w r s on level 0 with s = 1/rms(w r) (computed by the host).
+// Start of an outer leg, on the device, from the FP64 residual's norms res =
+// {|r|^2, |b|^2, max|r|, count} (k_residual_fold): the state the passes
+// test, the end tests of that residual (converged, NaN, iteration limit)
+// and the V-cycle input scale s = 1/(w_mean rms(r)) in s_r, s_pending and
+// zscale.  first: the reference norm bb is |b|^2 (|r|^2 when b = 0); later
+// legs keep the first leg's.
+__global__ void k_mg_init(const double* __restrict__ res, MgScal* S, double* zscale, int first,
+                          int n, int max_it, double rtarget2, double w_mean) {
+    if (threadIdx.x != 0) return;
+    const double rr = res[0];
+    MgScal v = {};
+    v.bb = first ? (res[1] > 0.0 ? res[1] : rr) : S->bb;
+    v.rr = rr;
+    v.rtarget2 = rtarget2;
+    v.n = n;
+    v.max_it = max_it;
+    v.status = ST_RUNNING;
+    v.s_r = v.s_pending = 1.0 / (w_mean * sqrt(rr / (res[3] > 1.0 ? res[3] : 1.0)));
+    if (rr <= rtarget2 * v.bb) {
+        v.status = ST_CONVERGED;
+        v.done = 1;
+        v.confirmed = 1;
+    } else if (rr != rr) {
+        v.status = ST_NAN;
+        v.done = 1;
+    } else if (n >= max_it) {
+        v.status = ST_MAXIT;
+        v.done = 1;
+    }
+    *S = v;
+    *zscale = v.s_r;
+}
+
+// After a leg's replays: when the recursive residual met the target, the
+// FP64 true residual of x (res, k_residual_fold) confirms it or asks for a
+// new leg; nothing while the leg runs or when it ended otherwise.
+__global__ void k_mg_confirm(const double* __restrict__ res, MgScal* S) {
+    if (threadIdx.x != 0) return;
+    if (!S->done || S->status != ST_CONVERGED || S->confirmed != 0) return;
+    S->confirmed = res[0] <= S->rtarget2 * S->bb ? 1 : -1;   // NaN: a new leg reports it
+}
+
+// Leg start from the residual r (after k_mg_init): p = q = 0 and the V-cycle
+// input w r s on level 0 with s = *zscale; nothing when k_mg_init ended the
+// solve.
 __global__ void __launch_bounds__(256) k_mg_start(Geom g, MgLev L0, const double* __restrict__ dxu,
                                                   const double* __restrict__ cosu,
                                                   const double* __restrict__ dyu,
                                                   const uint8_t* __restrict__ act,
-                                                  const double* __restrict__ r, double s) {
+                                                  const double* __restrict__ r,
+                                                  double* __restrict__ p, double* __restrict__ q,
+                                                  const double* __restrict__ zscale,
+                                                  const MgScal* __restrict__ S) {
+    if (S->done) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.nx * g.ny) return;
     const int i = t % g.nx, j = t / g.nx;
     const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
     const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
+    const double s = *zscale;
+    p[o] = 0.0;
+    q[o] = 0.0;
     L0.b[ql] = act[o] ? (float)(mg_area(dxu, cosu, dyu, i, j) * r[o] * s) : 0.0f;
 }
 
@@ -998,6 +1129,7 @@ struct MgState {
     MgScal* S_time = nullptr;        // scalars of psi_mg_kernel_times (never done)
     hipEvent_t t0 = nullptr, t1 = nullptr;
     MgScal* h_S = nullptr;           // pinned
+    double* h_res = nullptr;         // pinned: a leg residual's norms
     double* zscale = nullptr;
     double* part = nullptr;
     int nblk = 0;                    // workgroups of the outer passes

@@ -819,15 +819,22 @@ int sp_mg_setup(psi_press* c) {
     return mg_build(c, H);
 }
 
-int sp_residual(psi_press* c, const double* b, const double* x, double* r_out, double res[4]) {
+// The FP64 residual's norms into c->sp_part on stream s (r_out: r as well).
+void sp_residual_launch(psi_press* c, hipStream_t s, const double* b, const double* x,
+                        double* r_out) {
     using namespace psi;
     MgState* m = c->mg;
-    hipLaunchKernelGGL(k_spmg_resid, dim3(m->nblk), dim3(256), 0, c->stream, c->g,
+    hipLaunchKernelGGL(k_spmg_resid, dim3(m->nblk), dim3(256), 0, s, c->g,
                        (const double*)c->cE, (const double*)c->cW, (const double*)c->cN,
                        (const double*)c->cS, (const uint8_t*)c->m8, c->fs, c->kfs,
                        (const uint8_t*)c->act, b, x, r_out, m->part);
-    hipLaunchKernelGGL(k_spmg_resid_fold, dim3(1), dim3(1024), 0, c->stream,
-                       (const double*)m->part, m->nblk, c->sp_part);
+    hipLaunchKernelGGL(k_spmg_resid_fold, dim3(1), dim3(1024), 0, s, (const double*)m->part,
+                       m->nblk, c->sp_part);
+}
+
+int sp_residual(psi_press* c, const double* b, const double* x, double* r_out, double res[4]) {
+    using namespace psi;
+    sp_residual_launch(c, c->stream, b, x, r_out);
     int rc = check_launch(c);
     if (rc) return rc;
     double h[4];
@@ -897,71 +904,15 @@ int sp_solve_mg(psi_press* c, const psi_opts* o, const double* forc, double* sol
     const Geom& g = c->g;
     HIPCHK(c, hipEventRecord(c->t0, c->stream));
     HIPCHK(c, hipMemcpyAsync(m->x, sol, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
-    double res[4];
+    double res[4] = {0.0, 0.0, 0.0, 0.0};
     double bb = 0.0;
-    int n = 0, restarts = 0, status = ST_RUNNING;
-    for (;;) {
-        rc = sp_residual(c, forc, m->x, m->r, res);
-        if (rc) return rc;
-        info->setup_launches += 2;
-        if (restarts == 0) bb = res[1] > 0 ? res[1] : res[0];
-        if (res[0] <= target * target * bb) {
-            status = ST_CONVERGED;
-            break;
-        }
-        if (res[0] != res[0]) {
-            status = ST_NAN;
-            break;
-        }
-        if (n >= o->max_iterations) {
-            status = ST_MAXIT;
-            break;
-        }
-        const double s0 = 1.0 / (m->w_mean * sqrt(res[0] / std::max(res[3], 1.0)));
-        MgScal S0{};
-        S0.s_r = s0;
-        S0.s_pending = s0;
-        S0.rr = res[0];
-        S0.bb = bb;
-        S0.rtarget2 = target * target;
-        S0.n = n;
-        S0.max_it = o->max_iterations;
-        S0.status = ST_RUNNING;
-        *m->h_S = S0;
-        HIPCHK(c, hipMemcpyAsync(m->S, m->h_S, sizeof(MgScal), hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(m->zscale, &m->h_S->s_r, sizeof(double), hipMemcpyHostToDevice,
-                                 c->stream));
-        HIPCHK(c, hipMemsetAsync(m->p, 0, c->plane * 8, c->stream));
-        HIPCHK(c, hipMemsetAsync(m->q, 0, c->plane * 8, c->stream));
-        hipLaunchKernelGGL(k_mg_start, dim3(nblk((size_t)g.nx * g.ny)), dim3(256), 0, c->stream,
-                           g, m->lev[0], (const double*)c->dxt, (const double*)c->cost,
-                           (const double*)c->dyt, (const uint8_t*)c->act, (const double*)m->r,
-                           s0);
-        rc = check_launch(c);
-        if (rc) return rc;
-        HIPCHK(c, hipEventRecord(m->e_in, c->stream));
-        HIPCHK(c, hipStreamWaitEvent(m->gs, m->e_in, 0));
-        const int n_start = n;
-        HIPCHK(c, hipEventRecord(m->t0, m->gs));
-        rc = mg_replay(c, m, batch, timed, n_start, S0.rr, info);
-        if (rc) return rc;
-        HIPCHK(c, hipEventRecord(m->t1, m->gs));
-        HIPCHK(c, hipEventSynchronize(m->t1));
-        float span = 0.0f;
-        (void)hipEventElapsedTime(&span, m->t0, m->t1);
-        info->sweep_ms += span;
-        info->sweep_launches += m->h_S->n - n_start;
-        HIPCHK(c, hipEventRecord(m->e_out, m->gs));
-        HIPCHK(c, hipStreamWaitEvent(c->stream, m->e_out, 0));
-        n = m->h_S->n;
-        status = m->h_S->status;
-        if (status != ST_CONVERGED) break;
-        ++restarts;   // confirm on the FP64 true residual (top of the loop)
-    }
-    if (status != ST_CONVERGED || restarts == 0) {
-        rc = sp_residual(c, forc, m->x, nullptr, res);
-        if (rc) return rc;
-    }
+    int n = 0, status = ST_RUNNING, legs = 0;
+    const MgOperator op{c->sp_part, c->dxt, c->cost, c->dyt};
+    rc = mg_run_legs(
+        c, m, op,
+        [&](hipStream_t s, double* r_out) { sp_residual_launch(c, s, forc, m->x, r_out); },
+        batch, timed, target, o->max_iterations, info, res, bb, n, status, legs);
+    if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(sol, m->x, c->plane * 8, hipMemcpyDeviceToDevice, c->stream));
     if (g.cyclic)
         hipLaunchKernelGGL(k_setcyclic, dim3(nblk(g.ny + 4)), dim3(256), 0, c->stream, g, sol);
@@ -975,7 +926,7 @@ int sp_solve_mg(psi_press* c, const psi_opts* o, const double* forc, double* sol
     info->iterations = status == ST_MAXIT ? o->max_iterations + 1 : n;
     info->status = status_of(status);
     info->converged = status == ST_CONVERGED;
-    info->outer_iterations = restarts;
+    info->outer_iterations = legs;
     info->relres_true = bb > 0 ? sqrt(res[0] / bb) : sqrt(res[0]);
     info->relres_recursive = bb > 0 ? sqrt(m->h_S->rr / bb) : 0.0;
     if (info->status == PSI_NOT_CONVERGED)

@@ -197,8 +197,9 @@ def test_mg_spherical_1024_two_islands():
 
 def test_mg_in_solve_kernel_timing():
     """time_kernels records HIP events inside the iteration graph: the solve
-    is bit-identical to the untimed one, every full poll batch gives a sample,
-    and the four durations are positive and below the iteration's time."""
+    is bit-identical to the untimed one, the first poll batch (every replay of
+    it iterating) gives the solve's one sample, and the four durations are
+    positive and below the iteration's time."""
     print("\n[mg kernel timing 512]", flush=True)
     b = basins.spherical_basin(seed=4, n=512, n_islands=2)
     isl = islands_trace(b.kbot)
@@ -211,7 +212,7 @@ def test_mg_in_solve_kernel_timing():
                                                    check_every=2, time_kernels=timed)
         assert conv
         if timed:
-            assert info["kernel_samples"] > 0 and info["kernel_samples"] >= its // 2 - 4
+            assert info["kernel_samples"] == 1
             us_iter = 1e3 * info["sweep_ms"] / info["sweep_launches"]
             assert all(0.0 < u < us_iter for u in info["kernel_us"])
             np.testing.assert_array_equal(N(sol), out[False][1])
