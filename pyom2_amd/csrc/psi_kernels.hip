@@ -154,6 +154,18 @@ __device__ __forceinline__ void load_metrics_1d(const Geom& g, int c0, int r0, c
 // a contiguous range of tiles, so neighbouring tiles, which share halo rows,
 // are swept through the same L2.
 // ---------------------------------------------------------------------------
+// Pins a loaded value in registers at this point of the program.  The
+// compiler moves a load that only a data-dependent branch uses below that
+// branch, so the load would be issued only after the branch's own operand
+// had arrived: two memory round trips instead of one.  An empty asm
+// statement that reads and writes the value keeps its load above it, issued
+// with the others.
+__device__ __forceinline__ void pin_loaded(double& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin_loaded(double2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+__device__ __forceinline__ void pin_loaded(float& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin_loaded(float2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+__device__ __forceinline__ void pin_loaded(int& v) { asm volatile("" : "+v"(v)); }
+
 __device__ __forceinline__ void range_schedule(int n, int& first, int& stride, int& end) {
     const int b = blockIdx.x, G = gridDim.x;
     if (G < 8 || (G & 7)) {

@@ -651,16 +651,6 @@ __global__ void __launch_bounds__(1024) k_mg_fold_b(const double* __restrict__ p
     }
 }
 
-// Pins a loaded value in registers at this point of the program.  The
-// compiler moves a load that only a data-dependent branch uses below that
-// branch, so a point's operands would be requested only after its mask had
-// arrived: two memory round trips per row segment instead of one.  An empty
-// asm statement that reads and writes the value keeps its load above it,
-// issued with the others.
-__device__ __forceinline__ void mg_pin(double& v) { asm volatile("" : "+v"(v)); }
-__device__ __forceinline__ void mg_pin(double2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
-__device__ __forceinline__ void mg_pin(float2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
-__device__ __forceinline__ void mg_pin(float& v) { asm volatile("" : "+v"(v)); }
 
 // ---- the outer passes on pairs of points -----------------------------------
 // The same two passes with each thread taking the points (i, i+1), i even,
@@ -753,9 +743,9 @@ __global__ void __launch_bounds__(MG_PASS_A_THREADS) k_mg_fcg_a(
         const uint16_t a2 = *reinterpret_cast<const uint16_t*>(act + o);
         __builtin_amdgcn_sched_barrier(0);   // every load above, in flight together
         if (done) return;
-        mg_pin(e2), mg_pin(n2), mg_pin(s2), mg_pin(q2), mg_pin(r2);
-        mg_pin(em), mg_pin(rx0), mg_pin(rx1), mg_pin(dx0), mg_pin(dx1);
-        mg_pin(y.c), mg_pin(y.s), mg_pin(y.n), mg_pin(y.w), mg_pin(y.e);
+        pin_loaded(e2), pin_loaded(n2), pin_loaded(s2), pin_loaded(q2), pin_loaded(r2);
+        pin_loaded(em), pin_loaded(rx0), pin_loaded(rx1), pin_loaded(dx0), pin_loaded(dx1);
+        pin_loaded(y.c), pin_loaded(y.s), pin_loaded(y.n), pin_loaded(y.w), pin_loaded(y.e);
         if (!a2) continue;
         const MgZ2 z = mg_z_pair(g, y, i, j, inv);
         const double wa0 = dx0 * cr * dr, wa1 = dx1 * cr * dr;
@@ -827,9 +817,10 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
     const double* __restrict__ az_in, const double* __restrict__ zscale, MgScal* S,
     const double* __restrict__ part_a, int ga, double* __restrict__ part) {
     __shared__ double sf[3 * 4];
-    // with FOLD tested after the fold (its loads need nothing from S)
+    // tested after the fold (with FOLD: its loads need nothing from S) and
+    // once the first row segment's loads are in flight; every thread of a
+    // finished solve returns before the partials' barrier
     const int done = S->done;
-    if (!FOLD && done) return;
     const double inv = 1.0 / *zscale;
     // row segments of 512 points, each XCD a contiguous range of rows
     const int segs = (g.nx + 511) / 512;
@@ -883,8 +874,9 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
             MgB2 cur;
             mg_b2_load(g, L0, dxu, cosu, dyu, act, p, q, r, x, az_in, j, i, cur);
             __builtin_amdgcn_sched_barrier(0);   // every load above, in flight together
-            mg_pin(cur.y), mg_pin(cur.p), mg_pin(cur.q), mg_pin(cur.x), mg_pin(cur.r);
-            mg_pin(cur.az), mg_pin(cur.w0), mg_pin(cur.w1);
+            pin_loaded(cur.y), pin_loaded(cur.p), pin_loaded(cur.q), pin_loaded(cur.x);
+            pin_loaded(cur.r), pin_loaded(cur.az), pin_loaded(cur.w0), pin_loaded(cur.w1);
+            if (done) return;
             const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
             const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
             float2 b2 = make_float2(0.0f, 0.0f);
@@ -918,6 +910,7 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
             *reinterpret_cast<float2*>(L0.b + ql) = b2;
         }
     }
+    if (done) return;
     mg_block_partials<2>(acc, part);
 }
 

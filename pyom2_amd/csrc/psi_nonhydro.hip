@@ -190,7 +190,9 @@ __global__ void __launch_bounds__(256) k_nh_p(Geom g, const uint8_t* __restrict_
                                               const SpScal* __restrict__ S,
                                               double* __restrict__ part) {
 #pragma clang fp contract(off)
-    if (S->done) return;
+    // the flag is tested once the first point's loads are in flight; every
+    // thread of a finished solve returns before the partials' barrier
+    const int done = S->done;
     const double beta = S->beta;   // 0 in iteration 1, with the old direction zeroed
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
     const size_t nx = (size_t)g.nx;
@@ -203,11 +205,16 @@ __global__ void __launch_bounds__(256) k_nh_p(Geom g, const uint8_t* __restrict_
         sp_rc(g, q, row, col);
         const size_t o = base + q;
         const size_t src = o + ((cyc && col < 2) ? nx : 0) - ((cyc && col >= g.nx + 2) ? nx : 0);
-        const double v = r[src] + beta * p_old[src];
+        double rv = r[src], pv = p_old[src];
+        int mm = m8[o];
+        pin_loaded(rv), pin_loaded(pv), pin_loaded(mm);
+        if (done) return;
+        const double v = rv + beta * pv;
         p_new[o] = v;
-        const double mt = sp_inner(g, row, col) ? (double)((m8[o] >> 3) & 1) : 0.0;
+        const double mt = sp_inner(g, row, col) ? (double)((mm >> 3) & 1) : 0.0;
         s_max = fmax(fabs(v * mt), s_max);
     }
+    if (done) return;
     __shared__ double sh[4];
     for (int off = 32; off > 0; off >>= 1) s_max = fmax(s_max, __shfl_xor(s_max, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s_max;
@@ -355,7 +362,9 @@ __global__ void __launch_bounds__(256) k_nh_update(Geom g, int nz,
                                                    const SpScal* __restrict__ S,
                                                    double* __restrict__ part) {
 #pragma clang fp contract(off)
-    if (S->done) return;
+    // the flag is tested once the first point's loads are in flight; every
+    // thread of a finished solve returns before the partials' barrier
+    const int done = S->done;
     const double alpha = S->alpha;
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
     const size_t base = (size_t)blockIdx.y * plane;
@@ -363,13 +372,18 @@ __global__ void __launch_bounds__(256) k_nh_update(Geom g, int nz,
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
          q += (size_t)gridDim.x * blockDim.x) {
         const size_t o = base + q;
-        x[o] = x[o] + alpha * p[o];
-        const double rn = r[o] - alpha * Ap[o];
+        double xv = x[o], pv = p[o], rv = r[o], av = Ap[o];
+        int mm = m8[o];
+        pin_loaded(xv), pin_loaded(pv), pin_loaded(rv), pin_loaded(av), pin_loaded(mm);
+        if (done) return;
+        x[o] = xv + alpha * pv;
+        const double rn = rv - alpha * av;
         r[o] = rn;
         int row, col;
         sp_rc(g, q, row, col);
-        if (sp_inner(g, row, col)) acc = acc + rn * rn * (double)((m8[o] >> 3) & 1);
+        if (sp_inner(g, row, col)) acc = acc + rn * rn * (double)((mm >> 3) & 1);
     }
+    if (done) return;
     __shared__ double sh[4];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;

@@ -140,8 +140,22 @@ template <int NR, int NS>
 __device__ __forceinline__ void sp_fold(const double* __restrict__ part, int nblocks,
                                         double (&v)[NR]) {
     __shared__ double sh[NR * 16];
+    // the first four partials of each thread loaded at once (past nblocks:
+    // partial 0, dropped), then any further ones; block order throughout
+    double x4[4][NR];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int b = threadIdx.x + q * blockDim.x;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) x4[q][k] = part[(size_t)k * nblocks + (b < nblocks ? b : 0)];
+    }
     for (int k = 0; k < NR; ++k) v[k] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (threadIdx.x + q * blockDim.x < nblocks)
+#pragma unroll
+            for (int k = 0; k < NR; ++k) v[k] = k < NS ? v[k] + x4[q][k] : fmax(v[k], x4[q][k]);
+    for (int b = threadIdx.x + 4 * blockDim.x; b < nblocks; b += blockDim.x)
         for (int k = 0; k < NR; ++k) {
             const double y = part[(size_t)k * nblocks + b];
             v[k] = k < NS ? v[k] + y : fmax(v[k], y);
@@ -323,10 +337,10 @@ __global__ void __launch_bounds__(256) k_sp_update2(Geom g, const uint8_t* __res
 __global__ void __launch_bounds__(1024) k_sp_fold_a(const double* __restrict__ part,
                                                     int nblocks, SpScal* S) {
 #pragma clang fp contract(off)
-    if (S->done) return;
+    const int done = S->done;   // tested once the fold's loads are in flight
     double v[2];
     sp_fold<2, 1>(part, nblocks, v);
-    if (threadIdx.x != 0) return;
+    if (done || threadIdx.x != 0) return;
     const int n = S->n;
     if (n > 1) {
         const int m = n - 1;
@@ -356,20 +370,27 @@ __global__ void __launch_bounds__(256) k_sp_update(Geom g, const uint8_t* __rest
                                                    const SpScal* __restrict__ S,
                                                    double* __restrict__ part) {
 #pragma clang fp contract(off)
-    if (S->done) return;
+    // the flag is tested once the first point's loads are in flight; every
+    // thread of a finished solve returns before the partials' barrier
+    const int done = S->done;
     const double alpha = S->alpha;
     const size_t plane = (size_t)g.pitch * (g.ny + 4);
     const size_t P = g.pitch;
     double acc[1] = {0.0};
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane;
          q += (size_t)gridDim.x * blockDim.x) {
-        x[q] = x[q] + alpha * p[q];
-        const double rn = r[q] - alpha * Ap[q];
+        double xv = x[q], pv = p[q], rv = r[q], av = Ap[q];
+        int mm = m8[q];
+        pin_loaded(xv), pin_loaded(pv), pin_loaded(rv), pin_loaded(av), pin_loaded(mm);
+        if (done) return;
+        x[q] = xv + alpha * pv;
+        const double rn = rv - alpha * av;
         r[q] = rn;
         int row, col;
         sp_rc(g, q, row, col);
-        if (sp_inner(g, row, col)) acc[0] = acc[0] + rn * rn * (double)m8[q];
+        if (sp_inner(g, row, col)) acc[0] = acc[0] + rn * rn * (double)mm;
     }
+    if (done) return;
     sp_block_partials<1, 1>(acc, part);
 }
 
@@ -378,10 +399,10 @@ __global__ void __launch_bounds__(256) k_sp_update(Geom g, const uint8_t* __rest
 __global__ void __launch_bounds__(1024) k_sp_fold_b(const double* __restrict__ part,
                                                     int nblocks, SpScal* S) {
 #pragma clang fp contract(off)
-    if (S->done) return;
+    const int done = S->done;   // tested once the fold's loads are in flight
     double v[1];
     sp_fold<1, 1>(part, nblocks, v);
-    if (threadIdx.x != 0) return;
+    if (done || threadIdx.x != 0) return;
     const int n = S->n;
     const double rsnew = v[0];
     S->beta = rsnew / S->rsold;
@@ -515,7 +536,9 @@ __global__ void __launch_bounds__(256) k_spmg_fcg_a(
     const double* __restrict__ r, double* __restrict__ az_out,
     const double* __restrict__ zscale, const MgScal* __restrict__ S,
     double* __restrict__ part) {
-    if (S->done) return;
+    // the flag is tested once the first point's loads are in flight; every
+    // thread of a finished solve returns before the partials' barrier
+    const int done = S->done;
     const double inv = 1.0 / *zscale;
     double acc[3] = {0, 0, 0};
     // row segments of 256 points, each XCD a contiguous range of rows
@@ -526,18 +549,25 @@ __global__ void __launch_bounds__(256) k_spmg_fcg_a(
         const int j = it / segs, i = (it - j * segs) * 256 + threadIdx.x;
         if (i >= g.nx) continue;
         const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
-        // every load in bounds, issued before the active test
+        // every load in bounds, issued before the active test (pinned there)
         double z, zs, zw, ze, zn;
         mg_z5(g, L0, i, j, inv, z, zs, zw, ze, zn);
-        const double e = cE[o], w = cW[o], n = cN[o], so = cS[o], mk = (double)m8[o];
-        const double qv = q[o], rv = r[o], wa = mg_area(dxt, cost, dyt, i, j);
-        if (!act[o]) continue;
+        double e = cE[o], w = cW[o], n = cN[o], so = cS[o];
+        int mb = m8[o], a = act[o];
+        double qv = q[o], rv = r[o], wa = mg_area(dxt, cost, dyt, i, j);
+        pin_loaded(z), pin_loaded(zs), pin_loaded(zw), pin_loaded(ze), pin_loaded(zn);
+        pin_loaded(e), pin_loaded(w), pin_loaded(n), pin_loaded(so), pin_loaded(mb);
+        pin_loaded(a), pin_loaded(qv), pin_loaded(rv), pin_loaded(wa);
+        if (done) return;
+        if (!a) continue;
+        const double mk = (double)mb;
         const double az = sp_apply(e, w, n, so, mk, fs, kfs, zs, zw, z, ze, zn);
         az_out[o] = az;
         acc[0] += wa * z * az;
         acc[1] += wa * z * qv;
         acc[2] += wa * rv * z;
     }
+    if (done) return;
     mg_block_partials<3>(acc, part);
 }
 
