@@ -1161,6 +1161,21 @@ int mg_build(psi_ctx* c, std::vector<MgHostLev>& H) {
         HIPCHK(c, hipMemcpy(m->Ainv, padded.data(), padded.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(m->cells, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
     }
+    if (m->L > 0 && nc > 0) {
+        // the restriction into the coarsest level also writes b in row order
+        MgLev& cl = m->lev[m->L];
+        const size_t np = (size_t)cl.pitch * (cl.ny + 2);
+        std::vector<int> hidx(np, -1);
+        for (int k = 0; k < nc; ++k) hidx[(size_t)cells[k]] = k;
+        int* didx = nullptr;
+        float* bd = nullptr;
+        MGTRY(mg_alloc(c, m, &didx, np));
+        MGTRY(mg_alloc(c, m, &bd, (size_t)ld + 4));
+        HIPCHK(c, hipStreamSynchronize(c->stream));   // the allocation's clearing first
+        HIPCHK(c, hipMemcpy(didx, hidx.data(), np * 4, hipMemcpyHostToDevice));
+        cl.didx = didx;
+        cl.bd = bd;
+    }
     MGTRY(mg_alloc(c, m, &m->x, c->plane));
     MGTRY(mg_alloc(c, m, &m->r, c->plane));
     MGTRY(mg_alloc(c, m, &m->p, c->plane));

@@ -68,6 +68,11 @@ struct MgLev {
     float *b;                        // right-hand side (restricted residual)
     float *x;                        // pre-smoothed solution
     float *y;                        // solution after the coarse correction and post-smoothing
+    // the coarsest level below a smoothing level only: each cell's row of the
+    // dense inverse (-1: none) and b in row order, written by the restriction
+    // so that the dense solve reads it without an index lookup
+    const int* didx;
+    float* bd;
 };
 
 struct MgCheb {
@@ -98,6 +103,7 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
     const int done = S->done;
     const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    const int yq = cells[row < n ? row : 0];   // where the row's result goes, loaded up front
     // the row's first MG_MAX_DENSE / 256 16-byte pieces per lane (all of a
     // row of at most MG_MAX_DENSE cells), loaded beside the gather of b; rows
     // past n read row 0 and store nothing
@@ -110,7 +116,10 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
         av[t] = a[k < ld / 4 ? k : 0];
     }
     float* sb = reinterpret_cast<float*>(sB);
-    for (int k = threadIdx.x; k < ld; k += blockDim.x) sb[k] = k < n ? L.b[cells[k]] : 0.0f;
+    if (L.bd)
+        for (int k = threadIdx.x; k < ld; k += blockDim.x) sb[k] = k < n ? L.bd[k] : 0.0f;
+    else
+        for (int k = threadIdx.x; k < ld; k += blockDim.x) sb[k] = k < n ? L.b[cells[k]] : 0.0f;
     __syncthreads();
     if (done || row >= n) return;
     float s = 0.0f;
@@ -127,7 +136,7 @@ __global__ void __launch_bounds__(256) k_mg_dense(MgLev L, const float* __restri
         s += ((v.x * w.x + v.y * w.y) + v.z * w.z) + v.w * w.w;
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) L.y[cells[row]] = s;
+    if (lane == 0) L.y[yq] = s;
 }
 
 // ---- fused level kernels ------------------------------------------------------
@@ -282,6 +291,17 @@ __global__ void __launch_bounds__(64 * NW) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc
     const int I0 = (tile % tiles_x) * TX, J0 = (tile / tiles_x) * TY;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int I = I0 - K + lane;
+    // the tile's coarse cells, one per thread at most: this thread's cell and
+    // its mask are loaded beside the frame, so the restriction at the end
+    // waits for no load
+    constexpr int NC = (TX / 2) * (TY / 2);
+    static_assert(NC <= 64 * NW, "one coarse cell per thread");
+    const int ccy = threadIdx.x / (TX / 2), ccx = threadIdx.x - ccy * (TX / 2);
+    const int IC = I0 / 2 + ccx, JC = J0 / 2 + ccy;
+    const bool cown = threadIdx.x < NC && IC < Lc.nx && JC < Lc.ny;
+    const int QC = cown ? (JC + 1) * Lc.pitch + IC + MG_XOFF : MG_XOFF;
+    const uint8_t cact = Lc.act[QC];
+    const int cdq = Lc.didx ? Lc.didx[QC] : -1;
     float r[R], x[R], d[R], cC[R], cE[R], cN[R], cW[R], cS[R], di[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -326,19 +346,16 @@ __global__ void __launch_bounds__(64 * NW) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc
         }
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < (TX / 2) * (TY / 2); l += 64 * NW) {
-        const int cy = l / (TX / 2), cx = l - cy * (TX / 2);   // one coarse cell per thread
-        const int IC = I0 / 2 + cx, JC = J0 / 2 + cy;
-        if (IC < Lc.nx && JC < Lc.ny) {
-            float sum = 0.0f;
-            for (int dj = 0; dj < 2; ++dj)
-                for (int dx = 0; dx < 2; ++dx) {
-                    const int tx = 2 * cx + dx, ty = 2 * cy + dj;
-                    if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sR[ty + K][tx + K];
-                }
-            const int Q = (JC + 1) * Lc.pitch + IC + MG_XOFF;
-            Lc.b[Q] = Lc.act[Q] ? sum : 0.0f;
-        }
+    if (cown) {
+        float sum = 0.0f;
+        for (int dj = 0; dj < 2; ++dj)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int tx = 2 * ccx + dx, ty = 2 * ccy + dj;
+                if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sR[ty + K][tx + K];
+            }
+        const float bc = cact ? sum : 0.0f;
+        Lc.b[QC] = bc;
+        if (cdq >= 0) Lc.bd[cdq] = bc;
     }
 }
 
