@@ -216,6 +216,12 @@ __device__ __forceinline__ int mg_frame_cell(const MgLev& L, int I, int J) {
     return (J + 1) * L.pitch + I + MG_XOFF;
 }
 
+// A level cell every load may use in place of one outside the level (row
+// 0 of the interior, first column), so the frame's loads need no branch.
+__device__ __forceinline__ int mg_safe_cell(const MgLev& L, int q) {
+    return q >= 0 ? q : L.pitch + MG_XOFF;
+}
+
 // The value of the lane to the west (lane - 1) and to the east (lane + 1);
 // the frame's edge lanes get zero.
 __device__ __forceinline__ float mg_from_west(float v) {
@@ -305,10 +311,12 @@ __global__ void __launch_bounds__(64 * NW) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc
     float r[R], x[R], d[R], cC[R], cE[R], cN[R], cW[R], cS[R], di[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int q = mg_frame_cell(L, I, J0 - K + w * R + k);
-        const float b = q >= 0 ? L.b[q] : 0.0f, c = q >= 0 ? L.C[q] : 1.0f;
-        cE[k] = q >= 0 ? L.E[q] : 0.0f;
-        cN[k] = q >= 0 ? L.N[q] : 0.0f;
+        // loads at a safe cell outside the level (no branch, one round trip)
+        const int q = mg_frame_cell(L, I, J0 - K + w * R + k), qs = mg_safe_cell(L, q);
+        const float lb = L.b[qs], lc = L.C[qs], le = L.E[qs], ln = L.N[qs];
+        const float b = q >= 0 ? lb : 0.0f, c = q >= 0 ? lc : 1.0f;
+        cE[k] = q >= 0 ? le : 0.0f;
+        cN[k] = q >= 0 ? ln : 0.0f;
         cC[k] = c;
         di[k] = 1.0f / c;
         const float d0 = b * ch.c0 * di[k];
@@ -316,9 +324,9 @@ __global__ void __launch_bounds__(64 * NW) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc
         x[k] = d0;
         d[k] = d0;
     }
-    if (done) return;
     int buf = 0;
-    mg_band_couplings(X, buf, cE, cN, cW, cS);
+    mg_band_couplings(X, buf, cE, cN, cW, cS);   // (uses every loaded row: no load below the test)
+    if (done) return;
     // steps 1..K-1; step K: the residual
 #pragma unroll
     for (int s = 1; s <= K; ++s) {
@@ -379,25 +387,26 @@ __global__ void __launch_bounds__(64 * NW) k_mg_post(MgLev L, MgLev Lc, MgCheb c
     for (int k = 0; k < R; ++k) {
         int I = I0 - K + lane;
         const int J = J0 - K + w * R + k;
-        const int q = mg_frame_cell(L, I, J);
-        float yv = 0.0f;
-        if (q >= 0) {
-            // x, the coarse correction and the mask in one round of loads
-            if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
-            const float xv = L.x[q], yc = Lc.y[(J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF];
-            yv = L.act[q] ? xv + ch.cscale * yc : 0.0f;
-        }
-        const float c = q >= 0 ? L.C[q] : 1.0f;
-        cE[k] = q >= 0 ? L.E[q] : 0.0f;
-        cN[k] = q >= 0 ? L.N[q] : 0.0f;
+        const int q = mg_frame_cell(L, I, J), qs = mg_safe_cell(L, q);
+        // x, the coarse correction, the mask and the couplings in one round
+        // of loads, at a safe cell outside the level (no branch)
+        if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
+        const int qc = q >= 0 ? (J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF : mg_safe_cell(Lc, -1);
+        const float xv = L.x[qs], yc = Lc.y[qc], lc = L.C[qs], le = L.E[qs], ln = L.N[qs];
+        const float lb = L.b[qs];
+        const uint8_t la = L.act[qs];
+        const float yv = q >= 0 && la ? xv + ch.cscale * yc : 0.0f;
+        const float c = q >= 0 ? lc : 1.0f;
+        cE[k] = q >= 0 ? le : 0.0f;
+        cN[k] = q >= 0 ? ln : 0.0f;
         cC[k] = c;
         di[k] = 1.0f / c;
-        r[k] = q >= 0 ? L.b[q] : 0.0f;   // b
+        r[k] = q >= 0 ? lb : 0.0f;   // b
         y[k] = yv;
     }
-    if (done) return;
     int buf = 0;
-    mg_band_couplings(X, buf, cE, cN, cW, cS);
+    mg_band_couplings(X, buf, cE, cN, cW, cS);   // (uses every loaded row: no load below the test)
+    if (done) return;
     // step 0: r = b - S y', d0 = D^-1 r / theta, y = y' + d0
     {
         float below, above, sv[R];
