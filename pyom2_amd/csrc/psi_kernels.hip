@@ -161,9 +161,19 @@ __device__ __forceinline__ void load_metrics_1d(const Geom& g, int c0, int r0, c
 // statement that reads and writes the value keeps its load above it, issued
 // with the others.
 __device__ __forceinline__ void pin_loaded(double& v) { asm volatile("" : "+v"(v)); }
-__device__ __forceinline__ void pin_loaded(double2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+typedef double pin_d2 __attribute__((ext_vector_type(2)));
+typedef float pin_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void pin_loaded(double2& v) {   // one 128-bit operand
+    pin_d2 t = {v.x, v.y};
+    asm volatile("" : "+v"(t));
+    v.x = t.x, v.y = t.y;
+}
 __device__ __forceinline__ void pin_loaded(float& v) { asm volatile("" : "+v"(v)); }
-__device__ __forceinline__ void pin_loaded(float2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+__device__ __forceinline__ void pin_loaded(float2& v) {
+    pin_f2 t = {v.x, v.y};
+    asm volatile("" : "+v"(t));
+    v.x = t.x, v.y = t.y;
+}
 __device__ __forceinline__ void pin_loaded(int& v) { asm volatile("" : "+v"(v)); }
 
 __device__ __forceinline__ void range_schedule(int n, int& first, int& stride, int& end) {

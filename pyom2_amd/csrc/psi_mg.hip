@@ -383,26 +383,38 @@ __global__ void __launch_bounds__(64 * NW) k_mg_post(MgLev L, MgLev Lc, MgCheb c
     const int I0 = (tile % tiles_x) * TX, J0 = (tile / tiles_x) * TY;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float r[R], y[R], d[R], cC[R], cE[R], cN[R], cW[R], cS[R], di[R];
+    // x, the coarse correction, the mask and the couplings of every row in
+    // one round of loads, at a safe cell outside the level (no branch), all
+    // issued before any is used
+    float lx[R], lyc[R], lc[R], le[R], ln[R], lb[R];
+    int la[R];
+    bool in[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         int I = I0 - K + lane;
         const int J = J0 - K + w * R + k;
         const int q = mg_frame_cell(L, I, J), qs = mg_safe_cell(L, q);
-        // x, the coarse correction, the mask and the couplings in one round
-        // of loads, at a safe cell outside the level (no branch)
         if (L.cyc) I = (I % L.nx + L.nx) % L.nx;
         const int qc = q >= 0 ? (J / 2 + 1) * Lc.pitch + I / 2 + MG_XOFF : mg_safe_cell(Lc, -1);
-        const float xv = L.x[qs], yc = Lc.y[qc], lc = L.C[qs], le = L.E[qs], ln = L.N[qs];
-        const float lb = L.b[qs];
-        const uint8_t la = L.act[qs];
-        const float yv = q >= 0 && la ? xv + ch.cscale * yc : 0.0f;
-        const float c = q >= 0 ? lc : 1.0f;
-        cE[k] = q >= 0 ? le : 0.0f;
-        cN[k] = q >= 0 ? ln : 0.0f;
+        in[k] = q >= 0;
+        lx[k] = L.x[qs];
+        lyc[k] = Lc.y[qc];
+        lc[k] = L.C[qs];
+        le[k] = L.E[qs];
+        ln[k] = L.N[qs];
+        lb[k] = L.b[qs];
+        la[k] = L.act[qs];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const float c = in[k] ? lc[k] : 1.0f;
+        y[k] = in[k] && la[k] ? lx[k] + ch.cscale * lyc[k] : 0.0f;
+        cE[k] = in[k] ? le[k] : 0.0f;
+        cN[k] = in[k] ? ln[k] : 0.0f;
         cC[k] = c;
         di[k] = 1.0f / c;
-        r[k] = q >= 0 ? lb : 0.0f;   // b
-        y[k] = yv;
+        r[k] = in[k] ? lb[k] : 0.0f;   // b
     }
     int buf = 0;
     mg_band_couplings(X, buf, cE, cN, cW, cS);   // (uses every loaded row: no load below the test)
@@ -733,7 +745,11 @@ __device__ __forceinline__ MgZ2 mg_z_pair(const Geom& g, const MgY2& y, int i, i
     return z;
 }
 
-__global__ void __launch_bounds__(MG_PASS_A_THREADS) k_mg_fcg_a(
+#ifndef PSI_MG_PASS_A_WAVES
+#define PSI_MG_PASS_A_WAVES 8
+#endif
+__global__ void __launch_bounds__(MG_PASS_A_THREADS)
+__attribute__((amdgpu_waves_per_eu(1, PSI_MG_PASS_A_WAVES))) k_mg_fcg_a(
     Geom g, MgLev L0, const double* __restrict__ E, const double* __restrict__ N,
     const double* __restrict__ rx, const double* __restrict__ ry,
     const double* __restrict__ dxu, const double* __restrict__ cosu,
@@ -854,8 +870,22 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
     range_schedule(g.ny * segs, first, stride, end);
     double beta, alpha, sr;
     if (FOLD) {
+        // the first four partials per thread and slot loaded at once (past ga:
+        // partial 0, dropped), any further ones after; block order throughout
         double v[3] = {0.0, 0.0, 0.0};
-        for (int b = threadIdx.x; b < ga; b += 256)
+        double pa[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int b = threadIdx.x + 256 * q;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pa[q][k] = part_a[(size_t)k * ga + (b < ga ? b : 0)];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (threadIdx.x + 256 * q < ga)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v[k] += pa[q][k];
+        for (int b = threadIdx.x + 1024; b < ga; b += 256)
 #pragma unroll
             for (int k = 0; k < 3; ++k) v[k] += part_a[(size_t)k * ga + b];
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
