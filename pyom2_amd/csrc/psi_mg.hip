@@ -297,11 +297,10 @@ __global__ void __launch_bounds__(64 * NW) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc
     const int I0 = (tile % tiles_x) * TX, J0 = (tile / tiles_x) * TY;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int I = I0 - K + lane;
-    // the tile's coarse cells, one per thread at most: this thread's cell and
-    // its mask are loaded beside the frame, so the restriction at the end
-    // waits for no load
+    // the tile's coarse cells: this thread's first cell and its mask are
+    // loaded beside the frame, so the restriction at the end waits for no
+    // load (shapes with more coarse cells than threads load the rest there)
     constexpr int NC = (TX / 2) * (TY / 2);
-    static_assert(NC <= 64 * NW, "one coarse cell per thread");
     const int ccy = threadIdx.x / (TX / 2), ccx = threadIdx.x - ccy * (TX / 2);
     const int IC = I0 / 2 + ccx, JC = J0 / 2 + ccy;
     const bool cown = threadIdx.x < NC && IC < Lc.nx && JC < Lc.ny;
@@ -354,17 +353,27 @@ __global__ void __launch_bounds__(64 * NW) k_mg_pre(MgLev L, MgCheb ch, MgLev Lc
         }
     }
     __syncthreads();
-    if (cown) {
+    auto restrict_cell = [&](int cx, int cy, int Q, uint8_t a, int dq) {
         float sum = 0.0f;
         for (int dj = 0; dj < 2; ++dj)
             for (int dx = 0; dx < 2; ++dx) {
-                const int tx = 2 * ccx + dx, ty = 2 * ccy + dj;
+                const int tx = 2 * cx + dx, ty = 2 * cy + dj;
                 if (I0 + tx < L.nx && J0 + ty < L.ny) sum += sR[ty + K][tx + K];
             }
-        const float bc = cact ? sum : 0.0f;
-        Lc.b[QC] = bc;
-        if (cdq >= 0) Lc.bd[cdq] = bc;
-    }
+        const float bc = a ? sum : 0.0f;
+        Lc.b[Q] = bc;
+        if (dq >= 0) Lc.bd[dq] = bc;
+    };
+    if (cown) restrict_cell(ccx, ccy, QC, cact, cdq);
+    if (NC > 64 * NW)
+        for (int l = threadIdx.x + 64 * NW; l < NC; l += 64 * NW) {
+            const int cy = l / (TX / 2), cx = l - cy * (TX / 2);
+            const int ic = I0 / 2 + cx, jc = J0 / 2 + cy;
+            if (ic < Lc.nx && jc < Lc.ny) {
+                const int Q = (jc + 1) * Lc.pitch + ic + MG_XOFF;
+                restrict_cell(cx, cy, Q, Lc.act[Q], Lc.didx ? Lc.didx[Q] : -1);
+            }
+        }
 }
 
 // Prolongation of the coarse correction (weight ch.cscale, active cells) and
