@@ -166,6 +166,17 @@ __global__ void k_fcg_scalar(double* sc, int what) {
     }
 }
 
+// The network's value at solver point idx = (r, c) (network layout H x W),
+// zero off active points; both loads are issued whatever the mask (at a
+// clamped in-range position), so neither waits for the other.
+__device__ __forceinline__ float fcg_net(const float* __restrict__ znet, int H, int W,
+                                         const uint8_t* __restrict__ act, size_t idx, int r,
+                                         int c) {
+    const uint8_t a = act[idx];
+    const float z = znet[(size_t)min(r, H - 1) * W + min(c, W - 1)];
+    return (a && r < H && c < W) ? z : 0.f;
+}
+
 // z (network layout H x W) -> zs on the solver grid; (zs, W q_old)
 __global__ void k_fcg_scatter(Geom g, const float* __restrict__ znet, int H, int W,
                               const uint8_t* __restrict__ act, const float* __restrict__ q32,
@@ -176,11 +187,12 @@ __global__ void k_fcg_scatter(Geom g, const float* __restrict__ znet, int H, int
     double acc[1] = {0.0};
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
-        const int r = (int)(idx / g.pitch), c = (int)(idx % g.pitch);
-        float z = 0.f;
-        if (act[idx] && r < H && c < W) z = znet[(size_t)r * W + c];
-        zs[idx] = z;
-        acc[0] += (double)z * (double)q32[idx] * area_w(g, dxu, cosu, dyu, r, c);
+        const int r = (int)((unsigned)idx / (unsigned)g.pitch), c = (int)idx - r * g.pitch;
+        const float zv = fcg_net(znet, H, W, act, idx, r, c);
+        const float qv = q32[idx];
+        const double wa = area_w(g, dxu, cosu, dyu, r, c);
+        zs[idx] = zv;
+        acc[0] += (double)zv * (double)qv * wa;
     }
     block_reduce<1>(acc, 0u, sh);
     if (threadIdx.x == 0) atomicAdd(&sc[0], acc[0]);
@@ -210,21 +222,26 @@ __global__ void k_fcg_apply(Geom g, const double* __restrict__ E, const double* 
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     const int P = g.pitch;
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
-        const int r = (int)(idx / P), c = (int)(idx % P);
-        float v = 0.f;
-        if (r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
-            const int cl = map_col(c - 1, g), cr = map_col(c + 1, g);
-            const size_t il = (size_t)r * P + cl, ir = (size_t)r * P + cr;
-            const double e = E[idx], w = E[il] * rx[c], n = N[idx], s = N[idx - P] * ry[r];
-            const double cc = (((-e) - w) - n) - s;
-            const double vd = s * (double)p[idx - P] + w * (double)p[il] + cc * (double)p[idx] +
-                              e * (double)p[ir] + n * (double)p[idx + P];
-            v = (float)vd;
-            const double wa = area_w(g, dxu, cosu, dyu, r, c);
-            acc[0] += (double)p[idx] * vd * wa;
-            acc[1] += (double)r32[idx] * (double)p[idx] * wa;
+        int r = (int)((unsigned)idx / (unsigned)P), c = (int)idx - r * P;
+        // halo points compute at the first interior point and keep nothing,
+        // so every load is issued without a branch
+        const bool in = r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1;
+        if (!in) r = 2, c = 2;
+        const size_t ic = (size_t)r * P + c;
+        const int cl = map_col(c - 1, g), cr = map_col(c + 1, g);
+        const size_t il = (size_t)r * P + cl, ir = (size_t)r * P + cr;
+        const double e = E[ic], w = E[il] * rx[c], n = N[ic], s = N[ic - P] * ry[r];
+        const float ps = p[ic - P], pw = p[il], pc = p[ic], pe = p[ir], pn = p[ic + P];
+        const float rv = r32[ic];
+        const double wa = area_w(g, dxu, cosu, dyu, r, c);
+        const double cc = (((-e) - w) - n) - s;
+        const double vd = s * (double)ps + w * (double)pw + cc * (double)pc + e * (double)pe +
+                          n * (double)pn;
+        q[idx] = in ? (float)vd : 0.f;
+        if (in) {   // (arithmetic only: the loads are above)
+            acc[0] += (double)pc * vd * wa;
+            acc[1] += (double)rv * (double)pc * wa;
         }
-        q[idx] = v;
     }
     block_reduce<2>(acc, 0u, sh);
     if (threadIdx.x == 0) {
@@ -241,9 +258,12 @@ __global__ void k_fcg_update(Geom g, const float* __restrict__ p, const float* _
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     const float alpha = (float)(ld_dev(sc + 3) / ld_dev(sc + 1));
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
-        if (!act[idx]) continue;
-        e32[idx] += alpha * p[idx];
-        const float rv = r32[idx] - alpha * q[idx];
+        int a = act[idx];
+        float ev = e32[idx], pv = p[idx], rv0 = r32[idx], qv = q[idx];
+        pin_loaded(a), pin_loaded(ev), pin_loaded(pv), pin_loaded(rv0), pin_loaded(qv);
+        if (!a) continue;
+        e32[idx] = ev + alpha * pv;
+        const float rv = rv0 - alpha * qv;
         r32[idx] = rv;
         acc[0] += (double)rv * (double)rv;
     }
@@ -256,11 +276,6 @@ __global__ void k_fcg_update(Geom g, const float* __restrict__ p, const float* _
 // masked to active points), c = (n, W r) / (n, S n) its A-optimal scale
 // (S = W A), V the aggregation-multigrid V-cycle on what the network leaves.
 // Scalars sc[10] = (n, W r), sc[11] = (n, S n).
-__device__ __forceinline__ float fcg_net(const float* __restrict__ znet, int H, int W,
-                                         const uint8_t* __restrict__ act, size_t idx, int P) {
-    const int r = (int)(idx / P), c = (int)(idx % P);
-    return (act[idx] && r < H && c < W) ? znet[(size_t)r * W + c] : 0.f;
-}
 
 // A n (FP32 operator) and the two products of the scale.
 __global__ void k_fcgmg_scale(Geom g, const float* __restrict__ znet, int H, int W,
@@ -275,22 +290,29 @@ __global__ void k_fcgmg_scale(Geom g, const float* __restrict__ znet, int H, int
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     const int P = g.pitch;
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
-        const int r = (int)(idx / P), c = (int)(idx % P);
-        float v = 0.f;
-        if (r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1) {
-            const int cl = map_col(c - 1, g), cr = map_col(c + 1, g);
-            const size_t il = (size_t)r * P + cl, ir = (size_t)r * P + cr;
-            const float e = E[idx], w = E[il] * rx[c], n = N[idx], s = N[idx - P] * ry[r];
-            const float cc = (((-e) - w) - n) - s;
-            const float nc = fcg_net(znet, H, W, act, idx, P);
-            v = s * fcg_net(znet, H, W, act, idx - P, P) + w * fcg_net(znet, H, W, act, il, P) +
-                cc * nc + e * fcg_net(znet, H, W, act, ir, P) +
-                n * fcg_net(znet, H, W, act, idx + P, P);
-            const double wa = area_w(g, dxu, cosu, dyu, r, c);
-            acc[0] += (double)nc * (double)r32[idx] * wa;
+        int r = (int)((unsigned)idx / (unsigned)P), c = (int)idx - r * P;
+        // halo points compute at the first interior point and keep nothing,
+        // so every load is issued without a branch
+        const bool in = r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1;
+        if (!in) r = 2, c = 2;
+        const size_t ic = (size_t)r * P + c;
+        const int cl = map_col(c - 1, g), cr = map_col(c + 1, g);
+        const size_t il = (size_t)r * P + cl, ir = (size_t)r * P + cr;
+        const float e = E[ic], w = E[il] * rx[c], n = N[ic], s = N[ic - P] * ry[r];
+        const float nc = fcg_net(znet, H, W, act, ic, r, c);
+        const float ns = fcg_net(znet, H, W, act, ic - P, r - 1, c);
+        const float nw = fcg_net(znet, H, W, act, il, r, cl);
+        const float ne = fcg_net(znet, H, W, act, ir, r, cr);
+        const float nn = fcg_net(znet, H, W, act, ic + P, r + 1, c);
+        const float rv = r32[ic];
+        const double wa = area_w(g, dxu, cosu, dyu, r, c);
+        const float cc = (((-e) - w) - n) - s;
+        const float v = s * ns + w * nw + cc * nc + e * ne + n * nn;
+        an[idx] = in ? v : 0.f;
+        if (in) {   // (arithmetic only: the loads are above)
+            acc[0] += (double)nc * (double)rv * wa;
             acc[1] += (double)nc * (double)v * wa;
         }
-        an[idx] = v;
     }
     block_reduce<2>(acc, 0u, sh);
     if (threadIdx.x == 0) {
@@ -318,10 +340,13 @@ __global__ void k_fcgmg_input(Geom g, MgLev L0, const uint8_t* __restrict__ act,
     const int i = t % g.nx, j = t / g.nx;
     const size_t o = (size_t)(j + 2) * g.pitch + (i + 2);
     const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
+    // the point's loads first, the device scalars beside them
+    int a = act[o];
+    float rv = r32[o], av = an[o];
+    double wa = area_w(g, dxu, cosu, dyu, j + 2, i + 2);
+    pin_loaded(a), pin_loaded(rv), pin_loaded(av), pin_loaded(wa);
     const float cn = fcgmg_c(sc);
-    L0.b[ql] = act[o] ? (float)(area_w(g, dxu, cosu, dyu, j + 2, i + 2) * inv_wmean *
-                                (double)(r32[o] - cn * an[o]))
-                      : 0.0f;
+    L0.b[ql] = a ? (float)(wa * inv_wmean * (double)(rv - cn * av)) : 0.0f;
 }
 
 // zs = c n + w_mean y on active points; (zs, W q_old)
@@ -335,12 +360,13 @@ __global__ void k_fcgmg_scatter(Geom g, MgLev L0, const float* __restrict__ znet
     const size_t plane = (size_t)(g.nx + 4) * (g.ny + 4);
     const float cn = fcgmg_c(sc);
     for (size_t idx = (size_t)blockIdx.x * NT + threadIdx.x; idx < plane; idx += (size_t)gridDim.x * NT) {
-        const int r = (int)(idx / g.pitch), c = (int)(idx % g.pitch);
-        float z = 0.f;
-        if (act[idx]) {
-            const float y = L0.y[(r - 1) * L0.pitch + (c - 2) + MG_XOFF];
-            z = cn * fcg_net(znet, H, W, act, idx, g.pitch) + w_mean * y;
-        }
+        const int r = (int)((unsigned)idx / (unsigned)g.pitch), c = (int)idx - r * g.pitch;
+        // every load issued whatever the mask (y at an in-range cell)
+        const bool in = r >= 2 && r <= g.ny + 1 && c >= 2 && c <= g.nx + 1;
+        const uint8_t a = act[idx];
+        const float y = L0.y[in ? (r - 1) * L0.pitch + (c - 2) + MG_XOFF : L0.pitch + MG_XOFF];
+        const float nv = fcg_net(znet, H, W, act, idx, r, c);
+        const float z = a ? cn * nv + w_mean * y : 0.f;
         zs[idx] = z;
         acc[0] += (double)z * (double)q32[idx] * area_w(g, dxu, cosu, dyu, r, c);
     }

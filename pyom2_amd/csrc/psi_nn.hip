@@ -481,12 +481,10 @@ __global__ void k_net_input(const T* __restrict__ r, const float* __restrict__ Z
         c1 = hinv[o];
         c2 = z != 0.f ? 1.f : 0.f;
     }
-    uint16_t* q = out + p * 16;
-    q[0] = f2bf(c0);
-    q[1] = f2bf(c1);
-    q[2] = f2bf(c2);
-#pragma unroll
-    for (int k = 3; k < 16; ++k) q[k] = 0;
+    // the pixel's 16 channels (3 set, 13 zero) as two 16-byte stores
+    uint4* q = reinterpret_cast<uint4*>(out + p * 16);
+    q[0] = make_uint4((uint32_t)f2bf(c0) | ((uint32_t)f2bf(c1) << 16), (uint32_t)f2bf(c2), 0u, 0u);
+    q[1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 }  // namespace psinn
