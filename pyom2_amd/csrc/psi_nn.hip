@@ -72,7 +72,11 @@ __global__ void k_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
 // fragment order, k_pack_weights).  Epilogue: + bias, optional ReLU; output
 // bf16 or fp32 with cout_store channels (<= 16 NB).
 // ---------------------------------------------------------------------------
-constexpr int CV_TH = 4, CV_TW = 64;
+#ifndef PSI_CV_TH
+#define PSI_CV_TH 4
+#endif
+constexpr int CV_TH = PSI_CV_TH, CV_TW = 64;
+constexpr int CV_NT = 64 * CV_TH;               // threads: one wave per output row
 constexpr int CV_FX = CV_TW + 2, CV_FY = CV_TH + 2;
 constexpr int CV_PS = 24;                       // LDS bf16 per frame pixel (16 + 8 pad)
 constexpr int CV_IN16 = CV_FX * CV_FY * 2;      // 16-byte units of an input chunk
@@ -81,15 +85,15 @@ template <int NB>
 struct ConvShape {
     static constexpr int MR = NB <= 4 ? 4 : (NB <= 8 ? 2 : 1);   // 16-pixel segments per wave
     static constexpr int W16 = 5 * NB * 64;                       // 16-byte units of a weight chunk
-    static constexpr int PER_IN = (CV_IN16 + 255) / 256;
-    static constexpr int PER_W = (W16 + 255) / 256;
+    static constexpr int PER_IN = (CV_IN16 + CV_NT - 1) / CV_NT;
+    static constexpr int PER_W = (W16 + CV_NT - 1) / CV_NT;
 };
 
 // Workgroups of a small image may split the output blocks: blockIdx.z =
 // image * groups + group, the group computing blocks [group NB, group NB +
 // NB) of nb_total.
 template <int NB>
-__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in1, int C1, int up1,
+__global__ __launch_bounds__(CV_NT) void k_conv3x3(const uint16_t* __restrict__ in1, int C1, int up1,
                                                  const uint16_t* __restrict__ in2, int C2, int H,
                                                  int W, const uint16_t* __restrict__ wpack,
                                                  int nb_total, const float* __restrict__ bias,
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
         const int c0 = ch * 16;
 #pragma unroll
         for (int k = 0; k < PER_IN; ++k) {
-            const int u = tid + k * 256;
+            const int u = tid + k * CV_NT;
             uint4 val = make_uint4(0u, 0u, 0u, 0u);
             if (u < CV_IN16) {
                 const int px = u >> 1, half = u & 1;
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
         const uint4* src = reinterpret_cast<const uint4*>(wpack) + (size_t)ch * 5 * nb_total * 64;
 #pragma unroll
         for (int k = 0; k < PER_W; ++k) {
-            const int u = tid + k * 256;
+            const int u = tid + k * CV_NT;
             const int ks = u / (NB * 64), rem = u - ks * (NB * 64);
             v[k] = u < W16 ? src[(size_t)(ks * nb_total + nb0) * 64 + rem]
                            : make_uint4(0u, 0u, 0u, 0u);
@@ -149,12 +153,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
     auto store = [&](const uint4 (&vi)[PER_IN], const uint4 (&vw)[PER_W]) {
 #pragma unroll
         for (int k = 0; k < PER_IN; ++k) {
-            const int u = tid + k * 256;
+            const int u = tid + k * CV_NT;
             if (u < CV_IN16) sIn[(u >> 1) * (CV_PS / 8) + (u & 1)] = vi[k];
         }
 #pragma unroll
         for (int k = 0; k < PER_W; ++k) {
-            const int u = tid + k * 256;
+            const int u = tid + k * CV_NT;
             if (u < W16) sW[u] = vw[k];
         }
     };
@@ -244,7 +248,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
             const int nval = min(16, cout_store - cb);
             if (nval <= 0) continue;
             if (out16) {
-                for (int u = tid; u < TPX * 2; u += 256) {
+                for (int u = tid; u < TPX * 2; u += CV_NT) {
                     const int tp = u >> 1, half = u & 1;
                     const int wy = tp / (MR * 16), px = x0 + pass * MR * 16 + (tp - wy * MR * 16);
                     const int yy = y0 + wy;
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ in
                             reinterpret_cast<const uint4*>(sOut16)[tp * 2 + half];
                 }
             } else {
-                for (int u = tid; u < TPX * nval; u += 256) {
+                for (int u = tid; u < TPX * nval; u += CV_NT) {
                     const int tp = u / nval, c = u - tp * nval;
                     const int wy = tp / (MR * 16), px = x0 + pass * MR * 16 + (tp - wy * MR * 16);
                     const int yy = y0 + wy;
@@ -516,7 +520,7 @@ int psinn_conv3x3(const uint16_t* in1, int C1, int up1, const uint16_t* in2, int
     // a small image with many output blocks: split them over workgroups
     int sub = NB;
     if (NB > 4 && tiles < 1024) sub = (NB % 4 == 0) ? 4 : (NB % 3 == 0 ? 3 : NB);
-    const dim3 grid((W + CV_TW - 1) / CV_TW, (H + CV_TH - 1) / CV_TH, B * (NB / sub)), blk(256);
+    const dim3 grid((W + CV_TW - 1) / CV_TW, (H + CV_TH - 1) / CV_TH, B * (NB / sub)), blk(CV_NT);
     hipStream_t s = (hipStream_t)stream;
 #define CONV_CASE(K)                                                                          \
     case K:                                                                                   \
