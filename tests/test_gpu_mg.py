@@ -115,6 +115,34 @@ def test_mg_warm_start_and_iteration_cap(case):
     assert abs(info["relres_true"] - rel) <= 1e-6 * rel
 
 
+def test_mg_legs_past_the_attainable_residual():
+    """A target below what the FP64 true residual can reach: the recursive
+    residual meets it, the confirmation after the leg refuses it, and new legs
+    start from the true residual until the iteration cap -- the reported true
+    residual is the oracle's for the returned iterate (or, if the target is
+    met after all, the oracle confirms it)."""
+    b, d = fixtures.load("spherical_96")
+    isl = Islands(int(d["nisle"]), d["offsets"], d["bi"], d["bj"], d["di"], d["dj"])
+    s = StreamfunctionSolver.from_basin(b, islands=isl, device=DEV)
+    sol = s.zeros()
+    target = 1e-15
+    its, conv, info = s.congrad_streamfunction(T(b.forc), sol, mode="mg", criterion="relres",
+                                               relres_target=target, max_iterations=120)
+    rel, _ = _oracle_relres(b, isl, N(sol))
+    print(f"  its={its} conv={conv} legs={info['outer_iterations']} "
+          f"relres_true={info['relres_true']:.3e} oracle={rel:.3e}", flush=True)
+    # (at 1e-15 the residual itself is rounding: FP64 sums in another order
+    # differ by about eps |A| |x| / |b|)
+    assert abs(info["relres_true"] - rel) <= 1e-3 * rel + 5e-15
+    if conv:
+        assert rel <= 1.01 * target
+    else:
+        assert info["status"] == _lib.PSI_NOT_CONVERGED and its == 121
+        assert info["outer_iterations"] >= 2     # at least one leg ended on a refused confirmation
+        assert rel < 1e-12
+    s.close()
+
+
 @pytest.mark.parametrize("degree", [1, 2, 4])
 def test_mg_smoothing_degrees(degree):
     b, d = fixtures.load("spherical_96")
