@@ -74,6 +74,20 @@ struct BFinal {
     int done, status, iterations, legs, launches, pad;
 };
 
+// Values every lane of the workgroup holds alike (read from LDS after a
+// barrier), declared so: the member, its geometry, its phase and scalars
+// then live in scalar registers instead of one VGPR per lane each.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float wave_uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ bool needs_fold(int ph) {
     return ph == BP_RESID0 || ph == BP_SETUP || ph == BP_SWEEP || ph == BP_RESID ||
            ph == BP_REFRESH;
@@ -579,7 +593,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(
     if (threadIdx.x == 0)
         s_chunk = __hip_atomic_fetch_add(ctr + par, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const int chunk = s_chunk;
+    const int chunk = wave_uniform(s_chunk);
     if ((long long)chunk * P.grab >= nitems) break;
     const int i_end = min(nitems, (chunk + 1) * P.grab);
     for (int it = chunk * P.grab; it < i_end;) {
@@ -596,12 +610,13 @@ __global__ __launch_bounds__(NT) void k_batch_step(
                                                              : (ph_in == BP_SETUP ? 0 : n_in));
         if (may_sweep) fr.fetch(m.g, T0, sweep_ptrs(m, A, n_pred));
         member_step(m, s, seg0, ph_in, P, st_in, st_out, part_in, fin, sst, sRed);
-        const int ph = sst.phase;
+        const int ph = wave_uniform(sst.phase);
         if (ph == BP_SWEEP) {
-            const int n = sst.c.n;
+            const int n = wave_uniform(sst.c.n);
             const SweepPtrs q = sweep_ptrs(m, A, n);
             if (!may_sweep || n != n_pred) fr.fetch(m.g, T0, q);
-            work_sweep_run<BX, BY>(m, q, T0, T1, P.K, (float)sst.c.alpha, (float)sst.c.beta, A,
+            work_sweep_run<BX, BY>(m, q, T0, T1, P.K, wave_uniform((float)sst.c.alpha),
+                                   wave_uniform((float)sst.c.beta), A,
                                    fr, F, sRed, part_out);
         } else if (ph != BP_DONE) {
             for (int j = 0; j < run; ++j) {
@@ -618,7 +633,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(
                         work_start<BX, BY>(m, A, tb, te, true, sst.c.n, (float)sst.scale, acc,
                                            F);
                         break;
-                    case BP_CORR: work_corr<BX, BY>(m, A, tb, te, 1.0 / sst.scale); break;
+                    case BP_CORR: work_corr<BX, BY>(m, A, tb, te, 1.0 / wave_uniform(sst.scale)); break;
                     default: break;
                 }
                 if (ph != BP_CORR) {
@@ -652,11 +667,11 @@ __device__ __forceinline__ void poly_run(const BMember& m, const BArrays& A, con
     q.act = A.act + m.off;
     q.x = A.e32 + m.off;
     q.r64 = A.r64 + m.off;
-    q.rscale = st.scale;
-    const int n = MODE == PM_SETUP ? 0 : st.c.n;
+    q.rscale = wave_uniform(st.scale);
+    const int n = MODE == PM_SETUP ? 0 : wave_uniform(st.c.n);
     poly_buffers<MODE>(q, n, A.u0 + m.off, A.u1 + m.off, A.p0 + m.off, A.p1 + m.off,
                        A.r0 + m.off, A.r1 + m.off);
-    const float alpha = (float)st.c.alpha, beta = (float)st.c.beta;
+    const float alpha = wave_uniform((float)st.c.alpha), beta = wave_uniform((float)st.c.beta);
     double acc[NRED] = {0, 0, 0, 0, 0, 0};
     for (int t = T0; t < T1; ++t) {
         poly_tile<KP, MODE>(m.g, t, q, alpha, beta, P.ch, X, xbuf, acc);
@@ -677,10 +692,10 @@ __device__ __forceinline__ void poly_run(const BMember& m, const BArrays& A, con
 // legs (the member's tiles are those of the poly tiling, (64 - 2R) x
 // (4 RPT - 2R)); work distribution and state machine as k_batch_step.
 // Waves per SIMD the register allocation must allow.  The register-resident
-// polynomial tile (8 arrays of RPT rows per thread) and the member state
-// machine take 236 VGPRs at degree 4, so 2 waves per SIMD; forcing 3 or 4
-// spills 64 / 122 VGPRs to scratch and measured 17% / 44% fewer ensemble
-// solves per second at 512 members.
+// polynomial tile (7 arrays of RPT rows per thread) and the member state
+// machine take 223 VGPRs at degree 4, so 2 waves per SIMD; forcing 3 spills
+// 43 VGPRs to scratch and measured 4% fewer ensemble solves per second at
+// 512 members (4 waves spilled 122 and lost 44%).
 #ifndef PSI_BATCH_POLY_WAVES
 #define PSI_BATCH_POLY_WAVES 1
 #endif
@@ -706,7 +721,7 @@ void k_batch_poly(
             s_chunk =
                 __hip_atomic_fetch_add(ctr + par, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const int chunk = s_chunk;
+        const int chunk = wave_uniform(s_chunk);
         if ((long long)chunk * P.grab >= nitems) break;
         const int i_end = min(nitems, (chunk + 1) * P.grab);
         for (int it = chunk * P.grab; it < i_end;) {
@@ -716,7 +731,7 @@ void k_batch_poly(
             const int run = min(i_end - it, m.nseg - seg0);
             const int T0 = seg0 * P.K, T1 = min(m.g.ntiles, (seg0 + run) * P.K);
             member_step(m, s, seg0, st_in[s].phase, P, st_in, st_out, part_in, fin, sst, sRed);
-            const int ph = sst.phase;
+            const int ph = wave_uniform(sst.phase);
             if (ph == BP_SWEEP) {
                 poly_run<KP, PM_SWEEP>(m, A, P, sst, T0, T1, X, xbuf, sRed, part_out);
             } else if (ph == BP_SETUP) {
@@ -728,7 +743,7 @@ void k_batch_poly(
                     const int seg = seg0 + j;
                     const int tb = seg * P.K, te = min(m.g.ntiles, tb + P.K);
                     if (ph == BP_CORR) {
-                        work_corr<TXo, TYo>(m, A, tb, te, 1.0 / sst.scale);
+                        work_corr<TXo, TYo>(m, A, tb, te, 1.0 / wave_uniform(sst.scale));
                     } else {
                         double acc[NRED] = {0, 0, 0, 0, 0, 0};
                         work_resid<TXo, TYo>(m, A, tb, te, acc);

@@ -54,10 +54,18 @@ inline int PolyRowsHost(int k) {
     return PSI_POLY_RPT > 0 ? PSI_POLY_RPT : (k <= 2 ? 6 : (k <= 4 ? 7 : 8));
 }
 
-// Neighbouring columns of the frame are neighbouring lanes.  Lanes 0 and 63
-// (frame edges, never inside a ring that is computed) get their own value.
-__device__ __forceinline__ float from_left(float v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ float from_right(float v) { return __shfl_down(v, 1, 64); }
+// Neighbouring columns of the frame are neighbouring lanes, moved by DPP
+// (wave_shr:1 = 0x138: lane l gets lane l-1; wave_shl:1 = 0x130: lane l gets
+// lane l+1) rather than through the LDS crossbar.  Lanes 0 and 63 (frame
+// edges, never inside a ring that is computed) keep their own value.
+__device__ __forceinline__ float from_left(float v) {
+    const int b = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    const int b = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));
+}
 
 // Rows above and below a wave's band, exchanged through LDS:
 // xch[buf][vec][0 = first row | 1 = last row][wave][lane].
@@ -67,18 +75,21 @@ struct Exchange {
 
 // The 5-point operator on a strip held in registers, summed S, W, C, E, N
 // (the order of stencil5); up / dn are the rows above and below the strip.
+// The west coupling is the west neighbour's east one times this column's
+// metric factor rxv, formed here from cE rather than kept in registers.
 template <int RPT>
 __device__ __forceinline__ void apply_strip(const float (&v)[RPT], float up, float dn,
-                                            const float (&cS)[RPT], const float (&cW)[RPT],
+                                            const float (&cS)[RPT], float rxv,
                                             const float (&cE)[RPT], const float (&cN)[RPT],
                                             float (&out)[RPT]) {
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
         const float vu = i ? v[i - 1] : up, vd = (i < RPT - 1) ? v[i + 1] : dn;
         const float vl = from_left(v[i]), vr = from_right(v[i]);
-        const float cc = (((-cE[i]) - cW[i]) - cN[i]) - cS[i];
+        const float cW = from_left(cE[i]) * rxv;
+        const float cc = (((-cE[i]) - cW) - cN[i]) - cS[i];
         float a = cS[i] * vu;
-        a = a + cW[i] * vl;
+        a = a + cW * vl;
         a = a + cc * v[i];
         a = a + cE[i] * vr;
         a = a + cN[i] * vd;
@@ -106,6 +117,20 @@ __device__ __forceinline__ void exchange(Exchange& X, int& buf, const float (&a)
         dn[v] = w < NW - 1 ? X.x[buf][v][0][w + 1][lane] : 0.f;
     }
     buf ^= 1;
+}
+
+// Element idx of a plane whose base address is wave-uniform: the byte
+// offset is formed in 32 bits, so the access takes a scalar base plus one
+// 32-bit VGPR offset instead of a 64-bit address per lane (planes here stay
+// far below 4 GiB).
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* base, unsigned idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T* base, unsigned idx, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
 }
 
 // Vectors of one polynomial-preconditioned launch over one grid.
@@ -153,14 +178,15 @@ __device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtr
     const int fr0 = r0 + 2 - R, fc0 = c0 + 2 - R;
     const int col = map_col(fc0 + lane, g);
     const bool col_in = col >= 0 && col < g.nx + 4;
-    const float rxv = col_in ? q.rx[col] : 0.f;
+    const float rxv = col_in ? ld_at(q.rx, (unsigned)col) : 0.f;
     // ---- load the band: vectors, coefficients, own x -----------------------
-    float P[RPT], Rr[RPT], cS[RPT], cW[RPT], cE[RPT], cN[RPT], zi[RPT], xo[RPT];
+    float P[RPT], Rr[RPT], cS[RPT], cE[RPT], cN[RPT], zi[RPT], xo[RPT];
     unsigned amask = 0u;
     float nprev;
     {
         const int r = fr0 + w * RPT - 1;
-        nprev = (col_in && r >= 0 && r < g.ny + 4) ? q.N[(size_t)r * g.pitch + col] : 0.f;
+        nprev = (col_in && r >= 0 && r < g.ny + 4) ? ld_at(q.N, (unsigned)(r * g.pitch + col))
+                                                   : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
@@ -168,20 +194,20 @@ __device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtr
         float pv = 0.f, rv = 0.f, e = 0.f, nn = 0.f, ryv = 0.f, xv = 0.f;
         uint8_t a = 0;
         if (col_in && r >= 0 && r < g.ny + 4) {
-            const size_t o = (size_t)r * g.pitch + col;
+            const unsigned o = (unsigned)(r * g.pitch + col);
             if (MODE == PM_SWEEP) {
-                pv = q.z_in[o] + beta * q.p_in[o];
-                rv = q.r_in[o];
+                pv = ld_at(q.z_in, o) + beta * ld_at(q.p_in, o);
+                rv = ld_at(q.r_in, o);
             } else {
-                if (MODE == PM_REFRESH) pv = q.p_in[o];
-                rv = (float)(q.r64[o] * q.rscale);
+                if (MODE == PM_REFRESH) pv = ld_at(q.p_in, o);
+                rv = (float)(ld_at(q.r64, o) * q.rscale);
             }
-            e = q.E[o];
-            nn = q.N[o];
-            a = q.act[o];
-            ryv = q.ry[r];
+            e = ld_at(q.E, o);
+            nn = ld_at(q.N, o);
+            a = ld_at(q.act, o);
+            ryv = ld_at(q.ry, (unsigned)r);
             if (MODE == PM_SWEEP && gg >= R && gg < FH - R && lane >= R && lane < FW - R)
-                xv = q.x[o];
+                xv = ld_at((const float*)q.x, o);
         }
         P[i] = pv;
         Rr[i] = rv;
@@ -194,15 +220,15 @@ __device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtr
     }
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
-        cW[i] = from_left(cE[i]) * rxv;
-        const float cc = (((-cE[i]) - cW[i]) - cN[i]) - cS[i];
+        const float cW = from_left(cE[i]) * rxv;
+        const float cc = (((-cE[i]) - cW) - cN[i]) - cS[i];
         zi[i] = ((amask >> i) & 1u) ? 1.f / cc : 0.f;
     }
     float up[2], dn[2], tmp[RPT];
     if (MODE == PM_SWEEP) {
         // r_{k+1} = r_k - alpha A p_k (valid on radius K+1)
         exchange<RPT, 1>(X, xbuf, P, P, up, dn);
-        apply_strip<RPT>(P, up[0], dn[0], cS, cW, cE, cN, tmp);
+        apply_strip<RPT>(P, up[0], dn[0], cS, rxv, cE, cN, tmp);
 #pragma unroll
         for (int i = 0; i < RPT; ++i)
             if ((amask >> i) & 1u) Rr[i] = Rr[i] - alpha * tmp[i];
@@ -219,7 +245,7 @@ __device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtr
 #pragma unroll
     for (int j = 1; j <= K; ++j) {
         exchange<RPT, 1>(X, xbuf, D, D, up, dn);
-        apply_strip<RPT>(D, up[0], dn[0], cS, cW, cE, cN, tmp);
+        apply_strip<RPT>(D, up[0], dn[0], cS, rxv, cE, cN, tmp);
         const float c1 = ch.c1[j - 1], c2 = ch.c2[j - 1];
 #pragma unroll
         for (int i = 0; i < RPT; ++i) {
@@ -231,26 +257,26 @@ __device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtr
     // ---- own points: w = A z, s = A p, outputs and partials ---------------
     float Wv[RPT], Sv[RPT];
     exchange<RPT, 2>(X, xbuf, Y, P, up, dn);
-    apply_strip<RPT>(Y, up[0], dn[0], cS, cW, cE, cN, Wv);
-    apply_strip<RPT>(P, up[1], dn[1], cS, cW, cE, cN, Sv);
+    apply_strip<RPT>(Y, up[0], dn[0], cS, rxv, cE, cN, Wv);
+    apply_strip<RPT>(P, up[1], dn[1], cS, rxv, cE, cN, Sv);
     const int c = fc0 + lane;
     const bool own_col = lane >= R && lane < FW - R && c <= g.nx + 1;
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
         const int gg = w * RPT + i, r = fr0 + gg;
         if (own_col && gg >= R && gg < FH - R && r <= g.ny + 1) {
-            const size_t o = (size_t)r * g.pitch + c;
+            const unsigned o = (unsigned)(r * g.pitch + c);
             const float zn = Y[i], pn = P[i], rn = Rr[i];
             if (MODE == PM_SWEEP) {
-                q.x[o] = xo[i] + alpha * pn;
-                q.p_out[o] = pn;
+                st_at(q.x, o, xo[i] + alpha * pn);
+                st_at(q.p_out, o, pn);
             }
             if (MODE == PM_SETUP) {
-                q.x[o] = 0.f;
-                q.p_out[o] = 0.f;
+                st_at(q.x, o, 0.f);
+                st_at(q.p_out, o, 0.f);
             }
-            q.z_out[o] = zn;
-            q.r_out[o] = rn;
+            st_at(q.z_out, o, zn);
+            st_at(q.r_out, o, rn);
             const double zd = (double)zn, pd = (double)pn, rd = (double)rn;
             const double wd = (double)Wv[i], sd = (double)Sv[i];
             if ((amask >> i) & 1u) {
@@ -269,7 +295,12 @@ __device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtr
 // (the poly tiling of the grid); MODE as in poly_tile.  The sweep advances
 // the state as k_cg_sweep does; the refresh handles it as k_cg_refresh.
 template <int K, int MODE>
-__global__ __launch_bounds__(NT) void k_poly_pcg(
+// Waves per SIMD the register allocation must allow.  Degree 4 takes 140
+// VGPRs (3 waves); forcing 4 spills 12 and measured 15% slower at 2048^2.
+#ifndef PSI_POLY_WAVES
+#define PSI_POLY_WAVES 1
+#endif
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PSI_POLY_WAVES))) void k_poly_pcg(
     Geom gp, ChebCoef ch, const float* __restrict__ E, const float* __restrict__ N,
     const float* __restrict__ rx, const float* __restrict__ ry, const uint8_t* __restrict__ act,
     float* __restrict__ x, float* zb0, float* zb1, float* pb0, float* pb1, float* rb0,
