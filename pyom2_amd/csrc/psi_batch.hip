@@ -653,6 +653,17 @@ __global__ __launch_bounds__(NT) void k_batch_step(
     }
 }
 
+// Rows per thread of the batched polynomial tile.  The batched kernel runs
+// at 2 waves per SIMD whatever the frame height (its member state machine
+// holds registers beside the tile), so it takes the tallest frame that fits
+// 256 VGPRs without spilling: 9 rows (a 64 x 36 frame, 1.85x its own cells at
+// degree 4 against 2.15x with 7 rows; 1,811 -> 1,968 solves/s at 512
+// members, 8 rows 1,908).
+#ifndef PSI_BATCH_POLY_RPT
+#define PSI_BATCH_POLY_RPT 9
+#endif
+constexpr int batch_poly_rows(int k) { return PSI_POLY_RPT > 0 ? PSI_POLY_RPT : PSI_BATCH_POLY_RPT; }
+
 // Polynomial-preconditioned FP32 legs of one member over tiles [T0, T1) of
 // its poly tiling (partials per work item of K tiles).
 template <int KP, int MODE>
@@ -674,7 +685,7 @@ __device__ __forceinline__ void poly_run(const BMember& m, const BArrays& A, con
     const float alpha = wave_uniform((float)st.c.alpha), beta = wave_uniform((float)st.c.beta);
     double acc[NRED] = {0, 0, 0, 0, 0, 0};
     for (int t = T0; t < T1; ++t) {
-        poly_tile<KP, MODE>(m.g, t, q, alpha, beta, P.ch, X, xbuf, acc);
+        poly_tile<KP, MODE, batch_poly_rows(KP)>(m.g, t, q, alpha, beta, P.ch, X, xbuf, acc);
         if ((t + 1) % P.K == 0 || t + 1 == T1) {    // end of a work item
             block_reduce<NRED>(acc, 1u << 2, sRed);
             if (threadIdx.x == 0) {
@@ -693,9 +704,9 @@ __device__ __forceinline__ void poly_run(const BMember& m, const BArrays& A, con
 // (4 RPT - 2R)); work distribution and state machine as k_batch_step.
 // Waves per SIMD the register allocation must allow.  The register-resident
 // polynomial tile (7 arrays of RPT rows per thread) and the member state
-// machine take 223 VGPRs at degree 4, so 2 waves per SIMD; forcing 3 spills
-// 43 VGPRs to scratch and measured 4% fewer ensemble solves per second at
-// 512 members (4 waves spilled 122 and lost 44%).
+// machine took 223 VGPRs at degree 4 with 7 rows, so 2 waves per SIMD;
+// forcing 3 spilled 43 VGPRs to scratch and measured 4% fewer ensemble
+// solves per second at 512 members (4 waves spilled 122 and lost 44%).
 #ifndef PSI_BATCH_POLY_WAVES
 #define PSI_BATCH_POLY_WAVES 1
 #endif
@@ -706,7 +717,7 @@ void k_batch_poly(
     const int* __restrict__ nitems_p, BArrays A, BParams P, const BState* __restrict__ st_in,
     BState* __restrict__ st_out, const double* __restrict__ part_in,
     double* __restrict__ part_out, BFinal* __restrict__ fin, int* ctr, int par) {
-    constexpr int RPT = PolyRows<KP>::value;
+    constexpr int RPT = batch_poly_rows(KP);
     constexpr int TXo = PolyShape<KP, RPT>::TXo, TYo = PolyShape<KP, RPT>::TYo;
     __shared__ Exchange X;
     __shared__ double sRed[NRED * NW];
@@ -820,7 +831,7 @@ const void* poly_kernel_for(int k) {
     }
 }
 void poly_tile_dims(int k, int* bx, int* by) {
-    const int R = k + 2, rpt = PolyRowsHost(k);
+    const int R = k + 2, rpt = batch_poly_rows(k);
     *bx = 64 - 2 * R;
     *by = 4 * rpt - 2 * R;
 }

@@ -43,16 +43,18 @@ struct PolyShape {
 };
 
 // Rows per thread for degree K: the own tile is (64 - 2R) x (4 RPT - 2R).
+// Taller frames re-read fewer margin cells but hold more registers: at
+// degree 4 the single-grid kernel takes 164 VGPRs with 8 rows (3 waves per
+// SIMD, 2% faster than 7 rows at 2048^2) and 176 with 9 (2 waves, 15% slower).
 #ifndef PSI_POLY_RPT
 #define PSI_POLY_RPT 0            // > 0: rows per thread for every degree
 #endif
+constexpr int poly_rows(int k) { return PSI_POLY_RPT > 0 ? PSI_POLY_RPT : (k <= 2 ? 6 : 8); }
 template <int K>
 struct PolyRows {
-    static constexpr int value = PSI_POLY_RPT > 0 ? PSI_POLY_RPT : (K <= 2 ? 6 : (K <= 4 ? 7 : 8));
+    static constexpr int value = poly_rows(K);
 };
-inline int PolyRowsHost(int k) {
-    return PSI_POLY_RPT > 0 ? PSI_POLY_RPT : (k <= 2 ? 6 : (k <= 4 ? 7 : 8));
-}
+inline int PolyRowsHost(int k) { return poly_rows(k); }
 
 // Neighbouring columns of the frame are neighbouring lanes, moved by DPP
 // (wave_shr:1 = 0x138: lane l gets lane l-1; wave_shl:1 = 0x130: lane l gets
@@ -164,11 +166,10 @@ __device__ __forceinline__ void poly_buffers(PolyPtrs& q, int n, float* zb0, flo
 //  PM_SWEEP   one fused iteration with alpha, beta.
 //  PM_REFRESH r = r64 * rscale, z = M r next to the kept p; partials for
 //             cg_from_refresh.
-template <int K, int MODE>
+template <int K, int MODE, int RPT>
 __device__ __forceinline__ void poly_tile(const Geom& g, int tile, const PolyPtrs& q, float alpha,
                                           float beta, const ChebCoef& ch, Exchange& X, int& xbuf,
                                           double (&acc)[NRED]) {
-    constexpr int RPT = PolyRows<K>::value;
     using PS = PolyShape<K, RPT>;
     constexpr int R = PS::R, FW = PS::FW, FH = PS::FH, TXo = PS::TXo, TYo = PS::TYo;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PSI_POLY_WAV
     int t0, ts, te;
     tile_schedule(gp, t0, ts, te);
     for (int tile = t0; tile < te; tile += ts)
-        poly_tile<K, MODE>(gp, tile, q, alpha, beta, ch, X, xbuf, acc);
+        poly_tile<K, MODE, PolyRows<K>::value>(gp, tile, q, alpha, beta, ch, X, xbuf, acc);
     block_reduce<NRED>(acc, 1u << 2, sRed);
     write_partials<NRED>(acc, part_out);
 }
