@@ -833,6 +833,32 @@ struct MgB2 {
     uint16_t a;
 };
 
+// Pass B's x and p are read and written by pass B alone, once per
+// iteration: with non-temporal loads and stores they do not displace, in the
+// 256 MB Infinity Cache, what the V-cycle and pass A read again within the
+// iteration (the operator's FP64 couplings, q, r, the level vectors).  At
+// 2048^2 the iteration went from 233.7 to 225.1 us (pass A 45.9 -> 42.2,
+// finest post-smoothing 31.4 -> 28.5 us); x alone gave 230.0.
+// PSI_MG_B_NT: 1 = x and p non-temporal, 2 = x alone, 0 = neither.
+#ifndef PSI_MG_B_NT
+#define PSI_MG_B_NT 1
+#endif
+__device__ __forceinline__ double2 mg_ld2(const double* a, bool nt) {
+    if (nt) {
+        const pin_d2 t = __builtin_nontemporal_load(reinterpret_cast<const pin_d2*>(a));
+        return make_double2(t.x, t.y);
+    }
+    return *reinterpret_cast<const double2*>(a);
+}
+__device__ __forceinline__ void mg_st2(double* a, double2 v, bool nt) {
+    if (nt) {
+        const pin_d2 t = {v.x, v.y};
+        __builtin_nontemporal_store(t, reinterpret_cast<pin_d2*>(a));
+    } else {
+        *reinterpret_cast<double2*>(a) = v;
+    }
+}
+
 __device__ __forceinline__ void mg_b2_load(const Geom& g, const MgLev& L0,
                                            const double* __restrict__ dxu,
                                            const double* __restrict__ cosu,
@@ -845,9 +871,9 @@ __device__ __forceinline__ void mg_b2_load(const Geom& g, const MgLev& L0,
     const int ql = (j + 1) * L0.pitch + i + MG_XOFF;
     const int c = i + 2, rr = j + 2;
     v.y = *reinterpret_cast<const float2*>(L0.y + ql);
-    v.p = *reinterpret_cast<const double2*>(p + o);
+    v.p = mg_ld2(p + o, PSI_MG_B_NT == 1);
     v.q = *reinterpret_cast<const double2*>(q + o);
-    v.x = *reinterpret_cast<const double2*>(x + o);
+    v.x = mg_ld2(x + o, PSI_MG_B_NT >= 1);
     v.r = *reinterpret_cast<const double2*>(r + o);
     v.az = *reinterpret_cast<const double2*>(az_in + o);
     v.w0 = dxu[c] * cosu[rr] * dyu[rr];
@@ -967,9 +993,9 @@ __global__ void __launch_bounds__(256) k_mg_fcg_b(
                     acc[0] += rn.y * rn.y;
                     acc[1] += (cur.w1 * rn.y) * (cur.w1 * rn.y);
                 }
-                *reinterpret_cast<double2*>(p + o) = pn;
+                mg_st2(p + o, pn, PSI_MG_B_NT == 1);
                 *reinterpret_cast<double2*>(q + o) = qn;
-                *reinterpret_cast<double2*>(x + o) = xn;
+                mg_st2(x + o, xn, PSI_MG_B_NT >= 1);
                 *reinterpret_cast<double2*>(r + o) = rn;
             }
             *reinterpret_cast<float2*>(L0.b + ql) = b2;
