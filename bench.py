@@ -82,7 +82,7 @@ def sweep_kernel(n, fp64):
 
 # ... of the multigrid path's kernels (profiles/k_mg_spherical_2048.md)
 MG_TRAFFIC_2048 = {"post_level0": 114.2e6, "pre_level0": 92.9e6, "fcg_a": 193.9e6,
-                   "fcg_b": 343.1e6}
+                   "fcg_b": 347.6e6}
 # ... of the surface-pressure CG kernels (profiles/k_sp_press_2048x40.md) and of the
 # velocity update at 2048^2 x 40 levels (profiles/levels_kernels_step_2048x40.md)
 PRESS_TRAFFIC_2048 = {"dir_apply": 283.2e6, "update": 206.4e6}
